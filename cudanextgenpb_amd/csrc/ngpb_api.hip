@@ -1,0 +1,597 @@
+// C ABI of the linearized-PB library (declarations and reference mapping in
+// include/ngpb.h). Host-side argument checks, state ordering and error reporting;
+// all numerical work runs in the kernels of the other translation units.
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <sstream>
+#include <string>
+#include <vector>
+
+#include "ngpb_internal.h"
+
+namespace ngpb {
+
+void throw_hip(hipError_t e, const char* what, const char* file, int line) {
+  std::ostringstream os;
+  os << "HIP error " << hipGetErrorName(e) << " (" << hipGetErrorString(e) << ") in " << what << " at " << file << ":"
+     << line;
+  throw Error(e == hipErrorOutOfMemory ? NGPB_E_NOMEM : NGPB_E_HIP, os.str());
+}
+
+PhysConst physical_constants(const ngpb_physics& p) {
+  // include/pb_class.h:47-54; src/pb_class.cpp:1800-1806 and 2803-2814
+  constexpr double e0 = 8.85418781762e-12, kb = 1.380649e-23, e = 1.602176634e-19, Nav = 6.022e23, Angs = 1e-10,
+                   pi = 3.14159265358979323846;
+  PhysConst c;
+  c.eps_in = 4.0 * pi * e0 * p.e_in * kb * p.T * Angs / (e * e);
+  c.eps_out = 4.0 * pi * e0 * p.e_out * kb * p.T * Angs / (e * e);
+  const double C0 = 1.0e3 * Nav * p.ionic_strength;
+  c.k2 = 2.0 * C0 * Angs * Angs * e * e / (e0 * p.e_out * kb * p.T);
+  c.k = std::sqrt(c.k2);
+  const double inv_4pi = 1.0 / (4.0 * pi);
+  c.constant_pol = (1.0 / c.eps_out - 1.0 / c.eps_in) * inv_4pi;
+  c.constant_react = (1.0 / c.eps_out) * inv_4pi;
+  return c;
+}
+
+void run_markers(ngpb_ctx* c);
+
+}  // namespace ngpb
+
+int64_t ngpb_ctx::device_bytes() const {
+  int64_t b = 0;
+  b += node_xyz.bytes() + hang_xyz.bytes() + hang_np.bytes() + hang_par.bytes() + leaf_corner.bytes();
+  for (const auto& r : rays) b += r.key.bytes() + r.inters.bytes() + r.normals.bytes() + r.ptr.bytes();
+  b += eps.bytes() + reac.bytes() + leaf_marker.bytes() + border.bytes() + border_of.bytes();
+  b += frac.bytes() + frac_normal.bytes() + frac_found.bytes();
+  b += atom_xyz.bytes() + atom_q.bytes() + atom_owner.bytes();
+  b += bnd_node.bytes() + bnd_val.bytes() + is_bnd.bytes() + gval.bytes();
+  b += em_rowptr.bytes() + em_key.bytes() + em_kw.bytes() + slot_em.bytes() + diag_slot.bytes() + bc_rowptr.bytes() +
+       bc_col.bytes() + bc_em.bytes();
+  b += rowptr.bytes() + col.bytes() + val.bytes() + rhs.bytes() + rhs_charge.bytes() + c_edge.bytes() + v8.bytes() +
+       vol_patch.bytes() + rho.bytes();
+  b += slice_width.bytes() + slice_ptr.bytes() + slice_base_ptr.bytes() + slice_wide_ptr.bytes() + sell_val.bytes() +
+       sell_off.bytes() + sell_base.bytes() + sell_col32.bytes() + dscale.bytes();
+  b += x.bytes() + r.bytes() + z.bytes() + p.bytes() + q.bytes() + dinv.bytes();
+  b += xs.bytes() + rs.bytes() + ps.bytes() + qs.bytes() + ks.bytes();
+  b += red.partials.bytes() + red.tickets.bytes() + scan_tmp.bytes();
+  return b;
+}
+
+namespace {
+
+using ngpb::Error;
+
+template <class F>
+int guarded(ngpb_ctx* c, F&& f) {
+  if (!c) return NGPB_E_ARG;
+  try {
+    NGPB_HIP(hipSetDevice(c->device));
+    f();
+    c->err.clear();
+    return NGPB_OK;
+  } catch (const Error& e) {
+    c->err = e.what();
+    return e.code;
+  } catch (const std::bad_alloc&) {
+    c->err = "host allocation failed";
+    return NGPB_E_NOMEM;
+  } catch (const std::exception& e) {
+    c->err = e.what();
+    return NGPB_E_HIP;
+  }
+}
+
+void invalidate_after_mesh(ngpb_ctx* c) {
+  c->have_markers = c->have_frac = c->have_symbolic = c->have_system = c->have_compact = c->have_phi = false;
+}
+
+void need(bool cond, const char* what) {
+  if (!cond) throw Error(NGPB_E_STATE, what);
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* ngpb_version(void) { return "ngpb-gfx950 0.1"; }
+
+int ngpb_device_count(int* count) {
+  if (!count) return NGPB_E_ARG;
+  int n = 0;
+  hipError_t e = hipGetDeviceCount(&n);
+  if (e != hipSuccess) {
+    (void)hipGetLastError();
+    *count = 0;
+    return NGPB_E_NODEV;
+  }
+  *count = n;
+  return NGPB_OK;
+}
+
+int ngpb_create(ngpb_ctx** out, int device) {
+  if (!out) return NGPB_E_ARG;
+  *out = nullptr;
+  int n = 0;
+  if (hipGetDeviceCount(&n) != hipSuccess || n <= 0) {
+    (void)hipGetLastError();
+    return NGPB_E_NODEV;
+  }
+  if (device < 0 || device >= n) return NGPB_E_ARG;
+  ngpb_ctx* c = new (std::nothrow) ngpb_ctx();
+  if (!c) return NGPB_E_NOMEM;
+  c->device = device;
+  if (hipSetDevice(device) != hipSuccess || hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
+    (void)hipGetLastError();
+    delete c;
+    return NGPB_E_HIP;
+  }
+  ngpb_physics ph{2.0, 80.0, 298.15, 0.145, 2};
+  c->physics = ph;
+  c->pc = ngpb::physical_constants(ph);
+  *out = c;
+  return NGPB_OK;
+}
+
+int ngpb_destroy(ngpb_ctx* c) {
+  if (!c) return NGPB_OK;
+  (void)hipSetDevice(c->device);
+  if (c->stream) (void)hipStreamSynchronize(c->stream);
+  if (c->ks_host) (void)hipHostFree(c->ks_host);
+  c->ks_host = nullptr;
+  if (c->stream) (void)hipStreamDestroy(c->stream);
+  delete c;  // device buffers free themselves
+  return NGPB_OK;
+}
+
+const char* ngpb_last_error(const ngpb_ctx* c) { return c ? c->err.c_str() : "null context"; }
+
+int ngpb_constants(const ngpb_physics* p, double* out6) {
+  if (!p || !out6) return NGPB_E_ARG;
+  const ngpb::PhysConst c = ngpb::physical_constants(*p);
+  out6[0] = c.eps_in;
+  out6[1] = c.eps_out;
+  out6[2] = c.k2;
+  out6[3] = c.k;
+  out6[4] = c.constant_pol;
+  out6[5] = c.constant_react;
+  return NGPB_OK;
+}
+
+int ngpb_device_bytes(const ngpb_ctx* c, int64_t* bytes) {
+  if (!c || !bytes) return NGPB_E_ARG;
+  *bytes = c->device_bytes();
+  return NGPB_OK;
+}
+
+int ngpb_set_physics(ngpb_ctx* c, const ngpb_physics* p) {
+  return guarded(c, [&] {
+    NGPB_REQUIRE(p, NGPB_E_ARG, "physics is NULL");
+    NGPB_REQUIRE(p->e_in > 0 && p->e_out > 0 && p->T > 0 && p->ionic_strength >= 0, NGPB_E_ARG,
+                 "permittivities and temperature must be positive, ionic strength non-negative");
+    NGPB_REQUIRE(p->calc_energy >= 0 && p->calc_energy <= 2, NGPB_E_ARG, "calc_energy must be 0, 1 or 2");
+    c->physics = *p;
+    c->pc = ngpb::physical_constants(*p);
+    c->have_physics = true;
+    c->have_markers = c->have_system = c->have_compact = c->have_phi = false;
+  });
+}
+
+int ngpb_set_mesh(ngpb_ctx* c, int64_t n_nodes, const double* node_xyz, int64_t n_hang, const double* hang_xyz,
+                  const int32_t* hang_np, const int32_t* hang_par, int64_t n_leaves, const int32_t* leaf_corner) {
+  return guarded(c, [&] {
+    NGPB_REQUIRE(n_nodes > 0 && n_leaves > 0 && n_hang >= 0, NGPB_E_ARG, "mesh sizes must be positive");
+    NGPB_REQUIRE(n_nodes < INT32_MAX && n_hang < INT32_MAX && n_leaves < INT32_MAX, NGPB_E_ARG, "mesh too large");
+    NGPB_REQUIRE(node_xyz && leaf_corner, NGPB_E_ARG, "NULL mesh array");
+    NGPB_REQUIRE(n_hang == 0 || (hang_xyz && hang_np && hang_par), NGPB_E_ARG, "NULL hanging-node array");
+    for (int64_t k = 0; k < n_hang; ++k) {
+      NGPB_REQUIRE(hang_np[k] == 2 || hang_np[k] == 4, NGPB_E_ARG, "hanging nodes need 2 or 4 parents");
+      for (int j = 0; j < hang_np[k]; ++j)
+        NGPB_REQUIRE(hang_par[4 * k + j] >= 0 && hang_par[4 * k + j] < n_nodes, NGPB_E_ARG, "hanging parent out of range");
+    }
+    std::vector<uint8_t> seen((size_t)n_nodes, 0);
+    for (int64_t i = 0; i < 8 * n_leaves; ++i) {
+      const int32_t v = leaf_corner[i];
+      NGPB_REQUIRE(v < n_nodes && v >= -n_hang, NGPB_E_ARG, "leaf corner index out of range");
+      if (v >= 0) seen[v] = 1;
+    }
+    for (int64_t i = 0; i < n_nodes; ++i)
+      NGPB_REQUIRE(seen[i], NGPB_E_ARG, "every node must be a (non-hanging) corner of some leaf");
+    for (int64_t i = 0; i < 3 * n_nodes; ++i) NGPB_REQUIRE(std::isfinite(node_xyz[i]), NGPB_E_ARG, "non-finite node coordinate");
+    c->N = n_nodes;
+    c->H = n_hang;
+    c->L = n_leaves;
+    c->node_xyz.upload(node_xyz, (size_t)(3 * n_nodes), c->stream);
+    if (n_hang) {
+      c->hang_xyz.upload(hang_xyz, (size_t)(3 * n_hang), c->stream);
+      c->hang_np.upload(hang_np, (size_t)n_hang, c->stream);
+      c->hang_par.upload(hang_par, (size_t)(4 * n_hang), c->stream);
+    } else {
+      c->hang_xyz.resize(3);
+      c->hang_np.resize(1);
+      c->hang_par.resize(4);
+    }
+    c->leaf_corner.upload(leaf_corner, (size_t)(8 * n_leaves), c->stream);
+    NGPB_HIP(hipStreamSynchronize(c->stream));
+    c->have_mesh = true;
+    c->have_dirichlet = c->have_bnd_values = c->have_charges = false;
+    invalidate_after_mesh(c);
+  });
+}
+
+int ngpb_set_rays(ngpb_ctx* c, int axis, int64_t n_rays, const double* key, const int64_t* ptr, const double* inters,
+                  const double* normals) {
+  return guarded(c, [&] {
+    NGPB_REQUIRE(axis >= 0 && axis < 3, NGPB_E_ARG, "axis must be 0, 1 or 2");
+    NGPB_REQUIRE(n_rays >= 0, NGPB_E_ARG, "negative ray count");
+    auto& r = c->rays[axis];
+    if (n_rays == 0) {
+      r.n = r.k = 0;
+      r.key.resize(2);
+      r.ptr.resize(1);
+      r.inters.resize(1);
+      r.normals.resize(3);
+      const int64_t zero = 0;
+      r.ptr.upload(&zero, 1, c->stream);
+    } else {
+      NGPB_REQUIRE(key && ptr, NGPB_E_ARG, "NULL ray array");
+      NGPB_REQUIRE(ptr[0] == 0, NGPB_E_ARG, "ray pointers must start at 0");
+      for (int64_t i = 0; i < n_rays; ++i) {
+        NGPB_REQUIRE(ptr[i + 1] >= ptr[i], NGPB_E_ARG, "ray pointers must be non-decreasing");
+        if (i > 0)
+          NGPB_REQUIRE(key[2 * i - 2] < key[2 * i] || (key[2 * i - 2] == key[2 * i] && key[2 * i - 1] < key[2 * i + 1]),
+                       NGPB_E_ARG, "ray keys must be strictly increasing");
+        for (int64_t k = ptr[i] + 1; k < ptr[i + 1]; ++k)
+          NGPB_REQUIRE(inters[k - 1] <= inters[k], NGPB_E_ARG, "crossings must be sorted within a ray");
+      }
+      const int64_t K = ptr[n_rays];
+      NGPB_REQUIRE(K == 0 || (inters && normals), NGPB_E_ARG, "NULL crossing array");
+      r.n = n_rays;
+      r.k = K;
+      r.key.upload(key, (size_t)(2 * n_rays), c->stream);
+      r.ptr.upload(ptr, (size_t)(n_rays + 1), c->stream);
+      if (K) {
+        r.inters.upload(inters, (size_t)K, c->stream);
+        r.normals.upload(normals, (size_t)(3 * K), c->stream);
+      } else {
+        r.inters.resize(1);
+        r.normals.resize(3);
+      }
+    }
+    NGPB_HIP(hipStreamSynchronize(c->stream));
+    c->have_markers = c->have_frac = c->have_system = c->have_compact = c->have_phi = false;
+  });
+}
+
+int ngpb_create_markers(ngpb_ctx* c) {
+  return guarded(c, [&] {
+    need(c->have_mesh, "ngpb_set_mesh must come before ngpb_create_markers");
+    for (int d = 0; d < 3; ++d)
+      if (!c->rays[d].ptr.p) {
+        const int64_t zero = 0;
+        c->rays[d].ptr.upload(&zero, 1, c->stream);
+        c->rays[d].key.resize(2);
+        c->rays[d].inters.resize(1);
+        c->rays[d].normals.resize(3);
+      }
+    ngpb::run_markers(c);
+    c->have_markers = true;
+    c->have_frac = c->have_system = c->have_compact = c->have_phi = false;
+  });
+}
+
+int ngpb_set_markers(ngpb_ctx* c, const double* eps_node, const double* reac_node, const uint8_t* leaf_marker) {
+  return guarded(c, [&] {
+    need(c->have_mesh, "ngpb_set_mesh must come before ngpb_set_markers");
+    NGPB_REQUIRE(eps_node && reac_node && leaf_marker, NGPB_E_ARG, "NULL marker array");
+    for (int64_t i = 0; i < c->N; ++i)
+      NGPB_REQUIRE(eps_node[i] > 0 && reac_node[i] >= 0 && std::isfinite(eps_node[i]) && std::isfinite(reac_node[i]),
+                   NGPB_E_ARG, "eps must be positive and reaction non-negative");
+    for (int64_t l = 0; l < c->L; ++l) NGPB_REQUIRE(leaf_marker[l] <= 2, NGPB_E_ARG, "leaf marker must be 0, 1 or 2");
+    c->eps.upload(eps_node, (size_t)c->N, c->stream);
+    c->reac.upload(reac_node, (size_t)c->N, c->stream);
+    c->leaf_marker.upload(leaf_marker, (size_t)c->L, c->stream);
+    for (int d = 0; d < 3; ++d)
+      if (!c->rays[d].ptr.p) {
+        const int64_t zero = 0;
+        c->rays[d].ptr.upload(&zero, 1, c->stream);
+        c->rays[d].key.resize(2);
+        c->rays[d].inters.resize(1);
+        c->rays[d].normals.resize(3);
+      }
+    ngpb::run_border_from_markers(c);
+    c->have_markers = true;
+    c->have_frac = c->have_system = c->have_compact = c->have_phi = false;
+  });
+}
+
+int ngpb_get_markers(ngpb_ctx* c, double* eps_node, double* reac_node, uint8_t* leaf_marker, int64_t* n_border,
+                     int32_t* border) {
+  return guarded(c, [&] {
+    need(c->have_markers, "markers are not available");
+    if (eps_node) c->eps.download(eps_node, (size_t)c->N, c->stream);
+    if (reac_node) c->reac.download(reac_node, (size_t)c->N, c->stream);
+    if (leaf_marker) c->leaf_marker.download(leaf_marker, (size_t)c->L, c->stream);
+    if (n_border) *n_border = c->n_border;
+    if (border && c->n_border) c->border.download(border, (size_t)c->n_border, c->stream);
+    NGPB_HIP(hipStreamSynchronize(c->stream));
+  });
+}
+
+int ngpb_get_fractions(ngpb_ctx* c, double* frac) {
+  return guarded(c, [&] {
+    NGPB_REQUIRE(frac, NGPB_E_ARG, "NULL output");
+    need(c->have_markers, "markers are not available");
+    if (!c->have_frac) ngpb::run_fractions(c);
+    std::vector<double> fb((size_t)12 * c->n_border);
+    std::vector<int32_t> bl((size_t)c->n_border);
+    if (c->n_border) {
+      c->frac.download(fb.data(), fb.size(), c->stream);
+      c->border.download(bl.data(), bl.size(), c->stream);
+    }
+    NGPB_HIP(hipStreamSynchronize(c->stream));
+    for (int64_t i = 0; i < 12 * c->L; ++i) frac[i] = 0.5;
+    for (int64_t b = 0; b < c->n_border; ++b)
+      for (int e = 0; e < 12; ++e) frac[12 * (int64_t)bl[b] + e] = fb[12 * b + e];
+  });
+}
+
+int ngpb_set_charges(ngpb_ctx* c, int64_t n_atoms, const double* xyz, const double* q, const int32_t* owner) {
+  return guarded(c, [&] {
+    need(c->have_mesh, "ngpb_set_mesh must come before ngpb_set_charges");
+    NGPB_REQUIRE(n_atoms >= 0, NGPB_E_ARG, "negative atom count");
+    NGPB_REQUIRE(n_atoms == 0 || (xyz && q && owner), NGPB_E_ARG, "NULL atom array");
+    for (int64_t a = 0; a < n_atoms; ++a) {
+      NGPB_REQUIRE(owner[a] >= -1 && owner[a] < c->L, NGPB_E_ARG, "owner leaf out of range");
+      NGPB_REQUIRE(std::isfinite(q[a]) && std::isfinite(xyz[3 * a]) && std::isfinite(xyz[3 * a + 1]) &&
+                       std::isfinite(xyz[3 * a + 2]),
+                   NGPB_E_ARG, "non-finite atom data");
+    }
+    c->A = n_atoms;
+    if (n_atoms) {
+      c->atom_xyz.upload(xyz, (size_t)(3 * n_atoms), c->stream);
+      c->atom_q.upload(q, (size_t)n_atoms, c->stream);
+      c->atom_owner.upload(owner, (size_t)n_atoms, c->stream);
+    } else {
+      c->atom_xyz.resize(3);
+      c->atom_q.resize(1);
+      c->atom_owner.resize(1);
+    }
+    NGPB_HIP(hipStreamSynchronize(c->stream));
+    c->have_charges = true;
+    c->have_system = c->have_compact = c->have_phi = false;
+  });
+}
+
+int ngpb_set_dirichlet(ngpb_ctx* c, int64_t n_bnd, const int32_t* nodes, const double* values) {
+  return guarded(c, [&] {
+    need(c->have_mesh, "ngpb_set_mesh must come before ngpb_set_dirichlet");
+    NGPB_REQUIRE(n_bnd >= 0 && (n_bnd == 0 || nodes), NGPB_E_ARG, "bad Dirichlet list");
+    std::vector<uint8_t> isb((size_t)c->N, 0);
+    std::vector<double> g((size_t)c->N, 0.0);
+    for (int64_t k = 0; k < n_bnd; ++k) {
+      NGPB_REQUIRE(nodes[k] >= 0 && nodes[k] < c->N, NGPB_E_ARG, "Dirichlet node out of range");
+      NGPB_REQUIRE(!isb[nodes[k]], NGPB_E_ARG, "duplicate Dirichlet node");
+      isb[nodes[k]] = 1;
+      if (values) {
+        NGPB_REQUIRE(std::isfinite(values[k]), NGPB_E_ARG, "non-finite Dirichlet value");
+        g[nodes[k]] = values[k];
+      }
+    }
+    c->NB = n_bnd;
+    c->bnd_node.upload(n_bnd ? nodes : nullptr, (size_t)n_bnd, c->stream);
+    if (!n_bnd) c->bnd_node.resize(1);
+    c->is_bnd.upload(isb.data(), isb.size(), c->stream);
+    c->gval.upload(g.data(), g.size(), c->stream);
+    if (values && n_bnd) c->bnd_val.upload(values, (size_t)n_bnd, c->stream);
+    NGPB_HIP(hipStreamSynchronize(c->stream));
+    c->have_dirichlet = true;
+    c->have_bnd_values = values != nullptr || n_bnd == 0;
+    c->have_symbolic = c->have_system = c->have_compact = c->have_phi = false;
+  });
+}
+
+int ngpb_dirichlet_values(ngpb_ctx* c, int bc_type) {
+  return guarded(c, [&] {
+    need(c->have_dirichlet, "ngpb_set_dirichlet must come before ngpb_dirichlet_values");
+    ngpb::run_dirichlet_values(c, bc_type);
+    c->have_system = c->have_compact = c->have_phi = false;
+  });
+}
+
+int ngpb_assemble(ngpb_ctx* c) {
+  return guarded(c, [&] {
+    need(c->have_mesh, "no mesh");
+    need(c->have_markers, "markers must be created or set before assembly");
+    if (!c->have_dirichlet) {
+      c->NB = 0;
+      std::vector<uint8_t> isb((size_t)c->N, 0);
+      std::vector<double> g((size_t)c->N, 0.0);
+      c->is_bnd.upload(isb.data(), isb.size(), c->stream);
+      c->gval.upload(g.data(), g.size(), c->stream);
+      c->bnd_node.resize(1);
+      c->have_dirichlet = c->have_bnd_values = true;
+    }
+    need(c->have_bnd_values, "Dirichlet values missing (ngpb_dirichlet_values)");
+    if (!c->have_frac) ngpb::run_fractions(c);
+    if (!c->have_symbolic) ngpb::run_symbolic(c);
+    ngpb::run_numeric(c);
+    c->have_phi = false;
+  });
+}
+
+int ngpb_system_size(ngpb_ctx* c, int64_t* n, int64_t* nnz) {
+  return guarded(c, [&] {
+    NGPB_REQUIRE(n && nnz, NGPB_E_ARG, "NULL output");
+    need(c->have_system, "no assembled system");
+    *n = c->n;
+    *nnz = c->nnz;
+  });
+}
+
+int ngpb_export_csr(ngpb_ctx* c, int32_t* rowptr, int32_t* col, double* val, double* rhs) {
+  return guarded(c, [&] {
+    need(c->have_system, "no assembled system");
+    if (rowptr) c->rowptr.download(rowptr, (size_t)c->n + 1, c->stream);
+    if (col) c->col.download(col, (size_t)c->nnz, c->stream);
+    if (val) c->val.download(val, (size_t)c->nnz, c->stream);
+    if (rhs) c->rhs.download(rhs, (size_t)c->n, c->stream);
+    NGPB_HIP(hipStreamSynchronize(c->stream));
+  });
+}
+
+int ngpb_set_csr(ngpb_ctx* c, int64_t n, int64_t nnz, const int32_t* rowptr, const int32_t* col, const double* val,
+                 const double* rhs) {
+  return guarded(c, [&] {
+    NGPB_REQUIRE(n > 0 && nnz > 0 && n < INT32_MAX && nnz < INT32_MAX, NGPB_E_ARG, "bad CSR sizes");
+    NGPB_REQUIRE(rowptr && col && val && rhs, NGPB_E_ARG, "NULL CSR array");
+    NGPB_REQUIRE(rowptr[0] == 0 && rowptr[n] == nnz, NGPB_E_ARG, "row pointers must run from 0 to nnz");
+    for (int64_t i = 0; i < n; ++i) {
+      NGPB_REQUIRE(rowptr[i + 1] >= rowptr[i], NGPB_E_ARG, "row pointers must be non-decreasing");
+      for (int64_t k = rowptr[i]; k < rowptr[i + 1]; ++k) {
+        NGPB_REQUIRE(col[k] >= 0 && col[k] < n, NGPB_E_ARG, "column index out of range");
+        if (k > rowptr[i]) NGPB_REQUIRE(col[k] > col[k - 1], NGPB_E_ARG, "columns must be strictly ascending");
+      }
+    }
+    c->n = n;
+    c->nnz = nnz;
+    c->rowptr.upload(rowptr, (size_t)n + 1, c->stream);
+    c->col.upload(col, (size_t)nnz, c->stream);
+    c->val.upload(val, (size_t)nnz, c->stream);
+    c->rhs.upload(rhs, (size_t)n, c->stream);
+    NGPB_HIP(hipStreamSynchronize(c->stream));
+    c->have_system = true;
+    c->have_compact = c->have_phi = false;
+  });
+}
+
+int ngpb_parse_lis_options(const char* options, ngpb_solve_opts* o) {
+  if (!options || !o) return NGPB_E_ARG;
+  o->precision = NGPB_PREC_FP64;
+  o->tol = 1e-12;  // LIS default
+  o->maxiter = 1000;  // LIS default
+  o->drop = 0.05;
+  o->check_every = 32;
+  std::istringstream is(options);
+  std::vector<std::string> tok;
+  for (std::string t; is >> t;) tok.push_back(t);
+  for (size_t i = 0; i < tok.size(); ++i) {
+    const std::string& k = tok[i];
+    auto arg = [&](std::string* v) {
+      if (i + 1 >= tok.size()) return false;
+      *v = tok[++i];
+      return true;
+    };
+    std::string v;
+    if (k == "-i") {
+      if (!arg(&v) || (v != "cg" && v != "1")) return NGPB_E_ARG;
+    } else if (k == "-p") {
+      if (!arg(&v) || (v != "jacobi" && v != "1")) return NGPB_E_ARG;
+    } else if (k == "-tol") {
+      if (!arg(&v)) return NGPB_E_ARG;
+      o->tol = std::atof(v.c_str());
+    } else if (k == "-maxiter") {
+      if (!arg(&v)) return NGPB_E_ARG;
+      o->maxiter = std::atoi(v.c_str());
+    } else if (k == "-precision" || k == "-f") {
+      if (!arg(&v)) return NGPB_E_ARG;
+      if (v == "mixed" || v == "fp32") o->precision = NGPB_PREC_MIXED;
+      else if (v == "double" || v == "fp64") o->precision = NGPB_PREC_FP64;
+      else return NGPB_E_ARG;
+    } else if (k == "-print" || k == "-conv_cond" || k == "-tol_w" || k == "-initx_zeros" || k == "-scale") {
+      if (!arg(&v)) return NGPB_E_ARG;  // accepted, no effect (x0 = 0, ||r|| <= tol ||b||)
+    } else {
+      return NGPB_E_ARG;
+    }
+  }
+  return (o->tol > 0 && o->maxiter > 0) ? NGPB_OK : NGPB_E_ARG;
+}
+
+int ngpb_solve(ngpb_ctx* c, const ngpb_solve_opts* o, double* phi_out, ngpb_solve_stats* st) {
+  ngpb_solve_stats local{};
+  ngpb_solve_stats& s = st ? *st : local;
+  std::memset(&s, 0, sizeof(s));
+  const int rc = guarded(c, [&] {
+    NGPB_REQUIRE(o, NGPB_E_ARG, "NULL options");
+    NGPB_REQUIRE(o->tol > 0 && o->maxiter > 0, NGPB_E_ARG, "tol and maxiter must be positive");
+    need(c->have_system, "no assembled system");
+    if (o->precision == NGPB_PREC_FP64) ngpb::solve_fp64(c, *o, s);
+    else if (o->precision == NGPB_PREC_MIXED) ngpb::solve_mixed(c, *o, s);
+    else throw Error(NGPB_E_ARG, "unknown precision");
+    if (phi_out) {
+      c->x.download(phi_out, (size_t)c->n, c->stream);
+      NGPB_HIP(hipStreamSynchronize(c->stream));
+    }
+  });
+  if (rc != NGPB_OK) return rc;
+  if (s.status != NGPB_OK) {
+    c->err = "Krylov solver stopped at maxiter (or stagnated) before reaching the tolerance";
+    return s.status;
+  }
+  return NGPB_OK;
+}
+
+int ngpb_get_phi(ngpb_ctx* c, double* phi) {
+  return guarded(c, [&] {
+    NGPB_REQUIRE(phi, NGPB_E_ARG, "NULL output");
+    need(c->have_phi, "no solution");
+    c->x.download(phi, (size_t)c->n, c->stream);
+    NGPB_HIP(hipStreamSynchronize(c->stream));
+  });
+}
+
+int ngpb_set_phi(ngpb_ctx* c, const double* phi) {
+  return guarded(c, [&] {
+    NGPB_REQUIRE(phi, NGPB_E_ARG, "NULL input");
+    need(c->have_mesh, "no mesh");
+    c->x.upload(phi, (size_t)c->N, c->stream);
+    NGPB_HIP(hipStreamSynchronize(c->stream));
+    c->have_phi = true;
+  });
+}
+
+int ngpb_spmv(ngpb_ctx* c, int precision, const double* x, double* y) {
+  return guarded(c, [&] {
+    NGPB_REQUIRE(x && y, NGPB_E_ARG, "NULL vector");
+    need(c->have_system, "no assembled system");
+    ngpb::DBuf<double> dx, dy;
+    dx.upload(x, (size_t)c->n, c->stream);
+    dy.resize((size_t)c->n);
+    if (precision == NGPB_PREC_FP64) ngpb::spmv_fp64(c, dx.p, dy.p);
+    else if (precision == NGPB_PREC_MIXED) ngpb::spmv_compact(c, dx.p, dy.p);
+    else throw Error(NGPB_E_ARG, "unknown precision");
+    dy.download(y, (size_t)c->n, c->stream);
+    NGPB_HIP(hipStreamSynchronize(c->stream));
+  });
+}
+
+int ngpb_energy(ngpb_ctx* c, ngpb_energy_out* out) {
+  return guarded(c, [&] {
+    NGPB_REQUIRE(out, NGPB_E_ARG, "NULL output");
+    need(c->have_markers, "markers are not available");
+    need(c->have_phi, "no potential (solve or ngpb_set_phi first)");
+    if (!c->have_frac) ngpb::run_fractions(c);
+    ngpb::run_energy(c, out);
+  });
+}
+
+int ngpb_time_spmv(ngpb_ctx* c, int precision, int reps, double* ms, double* bytes) {
+  return guarded(c, [&] {
+    NGPB_REQUIRE(ms && bytes && reps > 0, NGPB_E_ARG, "bad arguments");
+    need(c->have_system, "no assembled system");
+    if (precision == NGPB_PREC_FP64) {
+      *ms = ngpb::time_spmv_fp64(c, reps);
+      *bytes = ngpb::spmv_fp64_bytes(c);
+    } else if (precision == NGPB_PREC_MIXED) {
+      *ms = ngpb::time_spmv_compact(c, reps);
+      *bytes = ngpb::spmv_compact_bytes(c);
+    } else {
+      throw Error(NGPB_E_ARG, "unknown precision");
+    }
+  });
+}
+
+}  // extern "C"

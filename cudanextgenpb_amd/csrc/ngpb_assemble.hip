@@ -1,0 +1,498 @@
+// Finite-volume assembly on the device (src/pb_class.cpp:1943-2159 and the bimpp
+// operators it calls: bim3a_rhs, bim3a_laplacian_frac, bim3a_reaction_frac,
+// bim3a_dirichlet_bc).
+//
+// Every matrix entry is a sum of "emissions": one per (leaf edge, parent pair) of the
+// Laplacian and one per (leaf corner, parent) of the kappa^2 mass. The symbolic phase
+// (once per mesh and Dirichlet set) writes the emissions of each row, sorts them by
+// (column, emission id, sub-index) and cuts them into CSR slots; the numeric phase
+// computes per-leaf conductances and sums each slot's emissions in that order. The
+// order equals the sequential leaf sweep of the reference, so the sums are
+// reproducible and equal to a sequential CPU assembly, and no floating-point atomics
+// are used anywhere.
+//
+// Emission ids: edge e of leaf l -> 12 l + e; corner i of leaf l -> 12 L + 8 l + i.
+// Weight code: bit 7 = negative, bits 0..6 = e with |w| = 2^-e (products of 1/np).
+#include <algorithm>
+
+#include "ngpb_internal.h"
+
+namespace ngpb {
+
+MeshView mesh_view_of(const ngpb_ctx* c);
+
+namespace {
+
+__constant__ int kEN[24] = {0, 1, 1, 3, 2, 3, 0, 2, 4, 5, 5, 7, 6, 7, 4, 6, 0, 4, 1, 5, 3, 7, 2, 6};
+__constant__ int kEA[12] = {0, 1, 0, 1, 0, 1, 0, 1, 2, 2, 2, 2};
+
+constexpr double kPi = 3.14159265358979323846;
+
+__device__ __forceinline__ int ilog2_small(int v) { return v == 1 ? 0 : v == 2 ? 1 : v == 4 ? 2 : v == 8 ? 3 : 4; }
+
+__device__ __forceinline__ double decode_w(uint16_t kw) {
+  const int code = kw & 0xff;
+  const double mag = ldexp(1.0, -(code & 0x7f));
+  return (code & 0x80) ? -mag : mag;
+}
+
+struct Parents {
+  int np[8];
+  int32_t par[8][4];
+};
+
+__device__ __forceinline__ void leaf_parents(const MeshView& m, int64_t l, Parents& P) {
+#pragma unroll
+  for (int i = 0; i < 8; ++i) {
+    const int32_t c = m.leaf_corner[8 * l + i];
+    if (c >= 0) {
+      P.np[i] = 1;
+      P.par[i][0] = c;
+    } else {
+      const int64_t h = -1 - (int64_t)c;
+      P.np[i] = m.hang_np[h];
+      for (int j = 0; j < 4; ++j) P.par[i][j] = m.hang_par[4 * h + j];
+    }
+  }
+}
+
+// pass 0: count per row; pass 1: write emissions at em_rowptr[row] + cursor
+template <int PASS>
+__global__ void k_emit(MeshView m, unsigned* __restrict__ cnt, const int64_t* __restrict__ em_rowptr,
+                       uint64_t* __restrict__ em_key, uint16_t* __restrict__ em_kw) {
+  const int64_t l = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (l >= m.L) return;
+  Parents P;
+  leaf_parents(m, l, P);
+  auto emit = [&](int32_t row, int32_t col, uint32_t id, int k, int code) {
+    if (PASS == 0) {
+      atomicAdd(&cnt[row], 1u);
+    } else {
+      const int64_t pos = em_rowptr[row] + atomicAdd(&cnt[row], 1u);
+      em_key[pos] = ((uint64_t)(uint32_t)col << 32) | (uint64_t)id;
+      em_kw[pos] = (uint16_t)((k << 8) | code);
+    }
+  };
+  for (int e = 0; e < 12; ++e) {
+    const int loc[2] = {kEN[2 * e], kEN[2 * e + 1]};
+    const uint32_t id = (uint32_t)(12 * l + e);
+    int k = 0;
+    for (int r = 0; r < 2; ++r)
+      for (int s = 0; s < 2; ++s) {
+        const int ir = loc[r], is = loc[s];
+        const int code = ((r == s) ? 0 : 0x80) | ilog2_small(P.np[ir] * P.np[is]);
+        for (int pr = 0; pr < P.np[ir]; ++pr)
+          for (int ps = 0; ps < P.np[is]; ++ps) emit(P.par[ir][pr], P.par[is][ps], id, k++, code);
+      }
+  }
+  for (int i = 0; i < 8; ++i) {
+    const uint32_t id = (uint32_t)(12 * m.L + 8 * l + i);
+    const int code = ilog2_small(P.np[i]);
+    for (int j = 0; j < P.np[i]; ++j) emit(P.par[i][j], P.par[i][j], id, j, code);
+  }
+}
+
+__global__ void k_u32_to_i64(int64_t n, const unsigned* __restrict__ a, int64_t* __restrict__ b) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) b[i] = a[i];
+}
+
+__device__ __forceinline__ bool em_less(uint64_t ka, uint16_t wa, uint64_t kb, uint16_t wb) {
+  return ka < kb || (ka == kb && (wa >> 8) < (wb >> 8));
+}
+
+// sort each row's emissions; count final slots and Dirichlet couplings
+__global__ void k_row_sort(int64_t N, const int64_t* __restrict__ em_rowptr, uint64_t* __restrict__ key,
+                           uint16_t* __restrict__ kw, const uint8_t* __restrict__ is_bnd,
+                           int64_t* __restrict__ fcount, int64_t* __restrict__ bcount) {
+  const int64_t row = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (row >= N) return;
+  const int64_t b = em_rowptr[row], e = em_rowptr[row + 1];
+  for (int64_t i = b + 1; i < e; ++i) {
+    const uint64_t kk = key[i];
+    const uint16_t ww = kw[i];
+    int64_t j = i - 1;
+    while (j >= b && em_less(kk, ww, key[j], kw[j])) {
+      key[j + 1] = key[j];
+      kw[j + 1] = kw[j];
+      --j;
+    }
+    key[j + 1] = kk;
+    kw[j + 1] = ww;
+  }
+  const bool rb = is_bnd[row] != 0;
+  int64_t nf = 0, nbc = 0;
+  int64_t prev = -1;
+  for (int64_t i = b; i < e; ++i) {
+    const int64_t col = (int64_t)(key[i] >> 32);
+    if (col == prev) continue;
+    prev = col;
+    if (rb) {
+      if (col == row) ++nf;
+    } else if (is_bnd[col]) {
+      ++nbc;
+    } else {
+      ++nf;
+    }
+  }
+  fcount[row] = nf;
+  bcount[row] = nbc;
+}
+
+__global__ void k_row_fill(int64_t N, const int64_t* __restrict__ em_rowptr, const uint64_t* __restrict__ key,
+                           const uint8_t* __restrict__ is_bnd, const int64_t* __restrict__ foff,
+                           const int64_t* __restrict__ boff, int32_t* __restrict__ rowptr, int32_t* __restrict__ col,
+                           int64_t* __restrict__ slot_em, int32_t* __restrict__ diag_slot,
+                           int32_t* __restrict__ bc_rowptr, int32_t* __restrict__ bc_col,
+                           int64_t* __restrict__ bc_em, int64_t nnz_total, int64_t nbc_total) {
+  const int64_t row = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (row >= N) return;
+  const int64_t b = em_rowptr[row], e = em_rowptr[row + 1];
+  const bool rb = is_bnd[row] != 0;
+  int64_t fs = foff[row], bs = boff[row];
+  rowptr[row] = (int32_t)fs;
+  bc_rowptr[row] = (int32_t)bs;
+  if (row == N - 1) {
+    rowptr[N] = (int32_t)nnz_total;
+    bc_rowptr[N] = (int32_t)nbc_total;
+  }
+  int64_t i = b;
+  while (i < e) {
+    const int64_t cl = (int64_t)(key[i] >> 32);
+    int64_t j = i + 1;
+    while (j < e && (int64_t)(key[j] >> 32) == cl) ++j;
+    if (rb) {
+      if (cl == row) {
+        col[fs] = (int32_t)cl;
+        slot_em[2 * fs] = i;
+        slot_em[2 * fs + 1] = j;
+        diag_slot[row] = (int32_t)fs;
+        ++fs;
+      }
+    } else if (is_bnd[cl]) {
+      bc_col[bs] = (int32_t)cl;
+      bc_em[2 * bs] = i;
+      bc_em[2 * bs + 1] = j;
+      ++bs;
+    } else {
+      col[fs] = (int32_t)cl;
+      slot_em[2 * fs] = i;
+      slot_em[2 * fs + 1] = j;
+      if (cl == row) diag_slot[row] = (int32_t)fs;
+      ++fs;
+    }
+    i = j;
+  }
+}
+
+// ------------------------------------------------------------------------ numeric
+__device__ __forceinline__ double wha(double eps1, double eps2, double frac) {
+  return 1.0 / (frac / eps1 + (1 - frac) / eps2);
+}
+
+__global__ void k_leaf_coeffs(MeshView m, const double* __restrict__ eps, const int32_t* __restrict__ border_of,
+                              const double* __restrict__ frac, double* __restrict__ c_edge,
+                              double* __restrict__ v8) {
+  const int64_t l = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (l >= m.L) return;
+  const int32_t c0 = m.leaf_corner[8 * l], c7 = m.leaf_corner[8 * l + 7];
+  double h[3];
+#pragma unroll
+  for (int d = 0; d < 3; ++d) h[d] = dev::corner_coord(m, c7, d) - dev::corner_coord(m, c0, d);
+  double ec[8];
+#pragma unroll
+  for (int i = 0; i < 8; ++i) {
+    const int32_t c = m.leaf_corner[8 * l + i];
+    if (c >= 0) {
+      ec[i] = eps[c];
+    } else {
+      const int64_t hh = -1 - (int64_t)c;
+      const int np = m.hang_np[hh];
+      double s = 0.0;
+      for (int j = 0; j < np; ++j) s += eps[m.hang_par[4 * hh + j]] / np;
+      ec[i] = s;
+    }
+  }
+  const int32_t bi = border_of[l];
+#pragma unroll
+  for (int e = 0; e < 12; ++e) {
+    const int a = kEA[e], b = (a + 1) % 3, cc = (a + 2) % 3;
+    const double f = bi >= 0 ? frac[12 * (int64_t)bi + e] : 0.5;
+    c_edge[12 * l + e] = wha(ec[kEN[2 * e]], ec[kEN[2 * e + 1]], f) * (h[b] * h[cc] / (4.0 * h[a]));
+  }
+  v8[l] = h[0] * h[1] * h[2] * 0.125;
+}
+
+__device__ __forceinline__ double emission_value(uint64_t key, uint16_t kw, int64_t row, int64_t L,
+                                                 const double* c_edge, const double* reac, const double* v8) {
+  const uint32_t id = (uint32_t)(key & 0xffffffffULL);
+  const double w = decode_w(kw);
+  if ((int64_t)id < 12 * L) return c_edge[id] * w;
+  const int64_t leaf = ((int64_t)id - 12 * L) >> 3;
+  return reac[row] * v8[leaf] * w;
+}
+
+__global__ void k_slot_values(int64_t N, int64_t L, const int32_t* __restrict__ rowptr,
+                              const int64_t* __restrict__ slot_em, const uint64_t* __restrict__ key,
+                              const uint16_t* __restrict__ kw, const double* __restrict__ c_edge,
+                              const double* __restrict__ reac, const double* __restrict__ v8,
+                              const int32_t* __restrict__ diag_slot, double* __restrict__ val,
+                              double* __restrict__ vol_patch) {
+  const int64_t row = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (row >= N) return;
+  for (int64_t s = rowptr[row]; s < rowptr[row + 1]; ++s) {
+    double v = 0.0;
+    for (int64_t i = slot_em[2 * s]; i < slot_em[2 * s + 1]; ++i)
+      v += emission_value(key[i], kw[i], row, L, c_edge, reac, v8);
+    val[s] = v;
+  }
+  // patch volume: mass emissions of the diagonal with unit coefficient
+  const int64_t ds = diag_slot[row];
+  double vp = 0.0;
+  for (int64_t i = slot_em[2 * ds]; i < slot_em[2 * ds + 1]; ++i) {
+    const uint32_t id = (uint32_t)(key[i] & 0xffffffffULL);
+    if ((int64_t)id < 12 * L) continue;
+    vp += v8[((int64_t)id - 12 * L) >> 3] * decode_w(kw[i]);
+  }
+  vol_patch[row] = vp;
+}
+
+// charge projection (src/pb_class.cpp:2002-2039): contribution t = 32 a + 4 ii + jj
+__global__ void k_rho_push(MeshView m, int64_t A, const double* __restrict__ axyz, const double* __restrict__ aq,
+                           const int32_t* __restrict__ owner, const double* __restrict__ vol_patch,
+                           double* __restrict__ cval, int32_t* __restrict__ cnode, int32_t* __restrict__ cnext,
+                           int32_t* __restrict__ head) {
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= 32 * A) return;
+  const int64_t a = t >> 5;
+  const int ii = (int)((t >> 2) & 7), jj = (int)(t & 3);
+  cnode[t] = -1;
+  const int64_t l = owner[a];
+  if (l < 0) return;
+  const int32_t c = m.leaf_corner[8 * l + ii];
+  int32_t node;
+  int npi = 1;
+  if (c >= 0) {
+    if (jj != 0) return;
+    node = c;
+  } else {
+    const int64_t h = -1 - (int64_t)c;
+    npi = m.hang_np[h];
+    if (jj >= npi) return;
+    node = m.hang_par[4 * h + jj];
+  }
+  const int32_t c0 = m.leaf_corner[8 * l], c7 = m.leaf_corner[8 * l + 7];
+  const double volume = (dev::corner_coord(m, c7, 0) - dev::corner_coord(m, c0, 0)) *
+                        (dev::corner_coord(m, c7, 1) - dev::corner_coord(m, c0, 1)) *
+                        (dev::corner_coord(m, c7, 2) - dev::corner_coord(m, c0, 2));
+  const int32_t co = m.leaf_corner[8 * l + 7 - ii];
+  const double weight = fabs((axyz[3 * a] - dev::corner_coord(m, co, 0)) *
+                             (axyz[3 * a + 1] - dev::corner_coord(m, co, 1)) *
+                             (axyz[3 * a + 2] - dev::corner_coord(m, co, 2))) / volume;
+  double v;
+  if (c >= 0) {
+    v = aq[a] * 4.0 * kPi * weight / vol_patch[node];
+  } else {
+    const double denom = npi * vol_patch[node];
+    v = aq[a] * 4.0 * kPi * weight / denom;
+  }
+  cval[t] = v;
+  cnode[t] = node;
+  cnext[t] = atomicExch(&head[node], (int32_t)t);
+}
+
+// one thread per list: sums the node's contributions in increasing contribution index
+__global__ void k_rho_reduce(int64_t C, const double* __restrict__ cval, const int32_t* __restrict__ cnode,
+                             const int32_t* __restrict__ cnext, const int32_t* __restrict__ head,
+                             double* __restrict__ rho) {
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= C) return;
+  const int32_t node = cnode[t];
+  if (node < 0 || head[node] != (int32_t)t) return;
+  double s = 0.0;
+  int64_t last = -1;
+  while (true) {
+    int64_t best = INT64_MAX;
+    for (int32_t u = head[node]; u >= 0; u = cnext[u])
+      if ((int64_t)u > last && (int64_t)u < best) best = u;
+    if (best == INT64_MAX) break;
+    s += cval[best];
+    last = best;
+  }
+  rho[node] = s;
+}
+
+__global__ void k_rhs_charge(int64_t N, int64_t L, const int64_t* __restrict__ slot_em,
+                             const uint64_t* __restrict__ key, const uint16_t* __restrict__ kw,
+                             const double* __restrict__ v8, const int32_t* __restrict__ diag_slot,
+                             const double* __restrict__ rho, double* __restrict__ rhs_charge) {
+  const int64_t row = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (row >= N) return;
+  const int64_t ds = diag_slot[row];
+  const double r = rho[row];
+  double s = 0.0;
+  for (int64_t i = slot_em[2 * ds]; i < slot_em[2 * ds + 1]; ++i) {
+    const uint32_t id = (uint32_t)(key[i] & 0xffffffffULL);
+    if ((int64_t)id < 12 * L) continue;
+    s += v8[((int64_t)id - 12 * L) >> 3] * decode_w(kw[i]) * r;
+  }
+  rhs_charge[row] = s;
+}
+
+__global__ void k_dirichlet_rhs(int64_t N, int64_t L, const uint8_t* __restrict__ is_bnd,
+                                const double* __restrict__ gval, const int32_t* __restrict__ diag_slot,
+                                const double* __restrict__ val, const int32_t* __restrict__ bc_rowptr,
+                                const int32_t* __restrict__ bc_col, const int64_t* __restrict__ bc_em,
+                                const uint64_t* __restrict__ key, const uint16_t* __restrict__ kw,
+                                const double* __restrict__ c_edge, const double* __restrict__ reac,
+                                const double* __restrict__ v8, const double* __restrict__ rhs_charge,
+                                double* __restrict__ rhs) {
+  const int64_t row = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (row >= N) return;
+  if (is_bnd[row]) {
+    rhs[row] = val[diag_slot[row]] * gval[row];
+    return;
+  }
+  double r = rhs_charge[row];
+  for (int64_t s = bc_rowptr[row]; s < bc_rowptr[row + 1]; ++s) {
+    double a = 0.0;
+    for (int64_t i = bc_em[2 * s]; i < bc_em[2 * s + 1]; ++i)
+      a += emission_value(key[i], kw[i], row, L, c_edge, reac, v8);
+    r -= a * gval[bc_col[s]];
+  }
+  rhs[row] = r;
+}
+
+// ------------------------------------------------------------------ Dirichlet data
+__global__ void k_bc_values(int64_t nb, const int32_t* __restrict__ nodes, const double* __restrict__ xyz, int bc_type,
+                            double eps_in, double eps_out, double k, int64_t A, const double* __restrict__ axyz,
+                            const double* __restrict__ aq, double* __restrict__ bval, double* __restrict__ gval) {
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= nb) return;
+  const int64_t nd = nodes[t];
+  const double x = xyz[3 * nd], y = xyz[3 * nd + 1], z = xyz[3 * nd + 2];
+  double v = 0.0;
+  if (bc_type == 2) {
+    // coulomb_boundary_conditions (src/pb_class.cpp:3530-3547) over the charges
+    for (int64_t a = 0; a < A; ++a) {
+      const double dx = axyz[3 * a] - x, dy = axyz[3 * a + 1] - y, dz = axyz[3 * a + 2] - z;
+      const double dist = sqrt(dx * dx + dy * dy + dz * dz);
+      v += aq[a] * exp(-k * dist) / (dist * eps_out);
+    }
+  } else if (bc_type == 3) {
+    // analytic_solution (src/pb_class.cpp:3549-3571): q = +1, rs = 2
+    const double rs = 2.0;
+    const double pout = eps_out * (1.0 + k * rs), pin = eps_in * rs;
+    const double dist = sqrt(x * x + y * y + z * z);
+    v = dist <= rs ? 1.0 / (pout * rs) + (rs - dist) / (pin * dist) : exp(k * (rs - dist)) / (dist * pout);
+  }
+  bval[t] = v;
+  gval[nd] = v;
+}
+
+}  // namespace
+
+void run_symbolic(ngpb_ctx* c) {
+  const MeshView m = mesh_view_of(c);
+  NGPB_REQUIRE(20 * m.L < (int64_t)UINT32_MAX, NGPB_E_ARG, "too many leaves for 32-bit emission ids");
+  const int64_t N = c->N;
+  DBuf<unsigned> cnt;
+  cnt.resize((size_t)N);
+  NGPB_HIP(hipMemsetAsync(cnt.p, 0, N * sizeof(unsigned), c->stream));
+  hipLaunchKernelGGL(k_emit<0>, dim3(blocks_for(m.L, 128)), dim3(128), 0, c->stream, m, cnt.p, nullptr, nullptr, nullptr);
+  c->em_rowptr.resize((size_t)N + 1);
+  hipLaunchKernelGGL(k_u32_to_i64, dim3(blocks_for(N, 256)), dim3(256), 0, c->stream, N, cnt.p, c->em_rowptr.p);
+  NGPB_HIP(hipGetLastError());
+  const int64_t total = exclusive_scan(c, c->em_rowptr.p, c->em_rowptr.p, N);
+  NGPB_HIP(hipMemcpyAsync(c->em_rowptr.p + N, &total, sizeof(int64_t), hipMemcpyHostToDevice, c->stream));
+  c->em_key.resize((size_t)total);
+  c->em_kw.resize((size_t)total);
+  NGPB_HIP(hipMemsetAsync(cnt.p, 0, N * sizeof(unsigned), c->stream));
+  hipLaunchKernelGGL(k_emit<1>, dim3(blocks_for(m.L, 128)), dim3(128), 0, c->stream, m, cnt.p, c->em_rowptr.p,
+                     c->em_key.p, c->em_kw.p);
+  NGPB_HIP(hipGetLastError());
+  DBuf<int64_t> fcount, bcount;
+  fcount.resize((size_t)N);
+  bcount.resize((size_t)N);
+  hipLaunchKernelGGL(k_row_sort, dim3(blocks_for(N, 128)), dim3(128), 0, c->stream, N, c->em_rowptr.p, c->em_key.p,
+                     c->em_kw.p, c->is_bnd.p, fcount.p, bcount.p);
+  NGPB_HIP(hipGetLastError());
+  const int64_t nnz = exclusive_scan(c, fcount.p, fcount.p, N);
+  const int64_t nbc = exclusive_scan(c, bcount.p, bcount.p, N);
+  NGPB_REQUIRE(nnz < (int64_t)INT32_MAX && nbc < (int64_t)INT32_MAX, NGPB_E_ARG, "nnz exceeds 32-bit CSR");
+  c->n = N;
+  c->nnz = nnz;
+  c->rowptr.resize((size_t)N + 1);
+  c->col.resize((size_t)std::max<int64_t>(nnz, 1));
+  c->slot_em.resize((size_t)2 * std::max<int64_t>(nnz, 1));
+  c->diag_slot.resize((size_t)N);
+  c->bc_rowptr.resize((size_t)N + 1);
+  c->bc_col.resize((size_t)std::max<int64_t>(nbc, 1));
+  c->bc_em.resize((size_t)2 * std::max<int64_t>(nbc, 1));
+  hipLaunchKernelGGL(k_row_fill, dim3(blocks_for(N, 128)), dim3(128), 0, c->stream, N, c->em_rowptr.p, c->em_key.p,
+                     c->is_bnd.p, fcount.p, bcount.p, c->rowptr.p, c->col.p, c->slot_em.p, c->diag_slot.p,
+                     c->bc_rowptr.p, c->bc_col.p, c->bc_em.p, nnz, nbc);
+  NGPB_HIP(hipGetLastError());
+  NGPB_HIP(hipStreamSynchronize(c->stream));
+  c->have_symbolic = true;
+}
+
+void run_numeric(ngpb_ctx* c) {
+  const MeshView m = mesh_view_of(c);
+  const int64_t N = c->N, L = c->L;
+  c->c_edge.resize((size_t)12 * L);
+  c->v8.resize((size_t)L);
+  c->val.resize((size_t)std::max<int64_t>(c->nnz, 1));
+  c->vol_patch.resize((size_t)N);
+  c->rho.resize((size_t)N);
+  c->rhs_charge.resize((size_t)N);
+  c->rhs.resize((size_t)N);
+  hipLaunchKernelGGL(k_leaf_coeffs, dim3(blocks_for(L, 128)), dim3(128), 0, c->stream, m, c->eps.p, c->border_of.p,
+                     c->frac.p, c->c_edge.p, c->v8.p);
+  hipLaunchKernelGGL(k_slot_values, dim3(blocks_for(N, 128)), dim3(128), 0, c->stream, N, L, c->rowptr.p,
+                     c->slot_em.p, c->em_key.p, c->em_kw.p, c->c_edge.p, c->reac.p, c->v8.p, c->diag_slot.p,
+                     c->val.p, c->vol_patch.p);
+  NGPB_HIP(hipGetLastError());
+  // charge density
+  NGPB_HIP(hipMemsetAsync(c->rho.p, 0, N * sizeof(double), c->stream));
+  if (c->have_charges && c->A > 0) {
+    const int64_t C = 32 * c->A;
+    DBuf<double> cval;
+    DBuf<int32_t> cnode, cnext, head;
+    cval.resize((size_t)C);
+    cnode.resize((size_t)C);
+    cnext.resize((size_t)C);
+    head.resize((size_t)N);
+    NGPB_HIP(hipMemsetAsync(head.p, 0xff, N * sizeof(int32_t), c->stream));
+    hipLaunchKernelGGL(k_rho_push, dim3(blocks_for(C, 256)), dim3(256), 0, c->stream, m, c->A, c->atom_xyz.p,
+                       c->atom_q.p, c->atom_owner.p, c->vol_patch.p, cval.p, cnode.p, cnext.p, head.p);
+    hipLaunchKernelGGL(k_rho_reduce, dim3(blocks_for(C, 256)), dim3(256), 0, c->stream, C, cval.p, cnode.p, cnext.p,
+                       head.p, c->rho.p);
+    NGPB_HIP(hipGetLastError());
+    NGPB_HIP(hipStreamSynchronize(c->stream));
+  }
+  hipLaunchKernelGGL(k_rhs_charge, dim3(blocks_for(N, 256)), dim3(256), 0, c->stream, N, L, c->slot_em.p,
+                     c->em_key.p, c->em_kw.p, c->v8.p, c->diag_slot.p, c->rho.p, c->rhs_charge.p);
+  hipLaunchKernelGGL(k_dirichlet_rhs, dim3(blocks_for(N, 128)), dim3(128), 0, c->stream, N, L, c->is_bnd.p,
+                     c->gval.p, c->diag_slot.p, c->val.p, c->bc_rowptr.p, c->bc_col.p, c->bc_em.p, c->em_key.p,
+                     c->em_kw.p, c->c_edge.p, c->reac.p, c->v8.p, c->rhs_charge.p, c->rhs.p);
+  NGPB_HIP(hipGetLastError());
+  NGPB_HIP(hipStreamSynchronize(c->stream));
+  c->have_system = true;
+  c->have_compact = false;
+}
+
+void run_dirichlet_values(ngpb_ctx* c, int bc_type) {
+  NGPB_REQUIRE(bc_type >= 1 && bc_type <= 3, NGPB_E_ARG, "bc_type must be 1, 2 or 3");
+  NGPB_REQUIRE(bc_type != 2 || c->have_charges, NGPB_E_STATE, "Coulomb boundary data need the charges");
+  c->bnd_val.resize((size_t)std::max<int64_t>(c->NB, 1));
+  if (c->NB > 0)
+    hipLaunchKernelGGL(k_bc_values, dim3(blocks_for(c->NB, 128)), dim3(128), 0, c->stream, c->NB, c->bnd_node.p,
+                       c->node_xyz.p, bc_type, c->pc.eps_in, c->pc.eps_out, c->pc.k, bc_type == 2 ? c->A : 0,
+                       c->atom_xyz.p, c->atom_q.p, c->bnd_val.p, c->gval.p);
+  NGPB_HIP(hipGetLastError());
+  NGPB_HIP(hipStreamSynchronize(c->stream));
+  c->have_bnd_values = true;
+}
+
+}  // namespace ngpb

@@ -1,0 +1,426 @@
+// Polarization and direct ionic energy (poisson_boltzmann::energy,
+// src/pb_class.cpp:2790-3063; energy_fast :3065-3338 is the same on meshes without
+// hanging nodes).
+//
+//   k_sources     one thread per border leaf: corner eps/phi (hanging corners: mean of
+//                 parents), flux edges (eps(i1) != eps(i2)) with their crossing point
+//                 and flux, and, for the ionic term, the marching-cubes triangles
+//                 (classifyCube + getTriangles) with per-vertex phi0, normal and area.
+//   normal carry  normal_intersection only overwrites the normal when a crossing is
+//                 found, so a vertex without a crossing reuses the last normal written
+//                 in the reference's sequential sweep (possibly in an earlier leaf); a
+//                 single-block scan supplies that carried normal.
+//   k_nbody       one thread per source, charged atoms (|q| > 1e-5) staged in LDS,
+//                 FP64 sums in atom order.
+//   k_energy_fin  fixed-order reduction of the per-source sums; energies in FP64.
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+
+#include "ngpb_internal.h"
+#include "mc_table.inc"
+
+namespace ngpb {
+
+MeshView mesh_view_of(const ngpb_ctx* c);
+
+namespace {
+
+__constant__ int kEN[24] = {0, 1, 1, 3, 2, 3, 0, 2, 4, 5, 5, 7, 6, 7, 4, 6, 0, 4, 1, 5, 3, 7, 2, 6};
+__constant__ int kEA[12] = {0, 1, 0, 1, 0, 1, 0, 1, 2, 2, 2, 2};
+__constant__ signed char kTri[256][16];
+
+constexpr double kPi = 3.14159265358979323846;
+constexpr int kFluxSlots = 12;
+constexpr int kTriSlots = 15;
+
+// flux source: x y z flux ; vertex source: x y z nx ny nz phi_sup area
+constexpr int kFS = 4;
+constexpr int kVS = 8;
+
+__device__ __forceinline__ double wha(double eps1, double eps2, double frac) {
+  return 1.0 / (frac / eps1 + (1 - frac) / eps2);
+}
+__device__ __forceinline__ double phi0(double eps1, double eps2, double phi1, double phi2, double frac) {
+  return phi1 + frac * eps2 * (phi2 - phi1) / (eps2 * frac + eps1 * (1 - frac));
+}
+
+struct EnergyArgs {
+  MeshView m;
+  int64_t nb;
+  const int32_t* border;
+  const double* eps;
+  const double* phi;
+  const double* frac;
+  const double* nrm;
+  const uint8_t* found;
+  double eps_out;
+  int ionic;
+};
+
+__global__ void k_sources(EnergyArgs a, double* __restrict__ fsrc, uint8_t* __restrict__ fvalid,
+                          double* __restrict__ vsrc, uint8_t* __restrict__ vvalid, uint8_t* __restrict__ vneed,
+                          double* __restrict__ leaf_charge, double* __restrict__ leaf_last, uint8_t* __restrict__ leaf_has,
+                          int64_t* __restrict__ counts) {
+  const int64_t b = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (b >= a.nb) return;
+  const MeshView& m = a.m;
+  const int64_t l = a.border[b];
+  const int32_t c0 = m.leaf_corner[8 * l], c7 = m.leaf_corner[8 * l + 7];
+  double h[3], p0[8][3];
+  for (int d = 0; d < 3; ++d) h[d] = dev::corner_coord(m, c7, d) - dev::corner_coord(m, c0, d);
+  const double area_h[3] = {h[1] * h[2] / h[0] * 0.25, h[0] * h[2] / h[1] * 0.25, h[0] * h[1] / h[2] * 0.25};
+  double te[8], tp[8];
+  for (int i = 0; i < 8; ++i) {
+    const int32_t c = m.leaf_corner[8 * l + i];
+    for (int d = 0; d < 3; ++d) p0[i][d] = dev::corner_coord(m, c, d);
+    if (c >= 0) {
+      te[i] = a.eps[c];
+      tp[i] = a.phi[c];
+    } else {
+      const int64_t hh = -1 - (int64_t)c;
+      const int np = m.hang_np[hh];
+      double se = 0.0, sp = 0.0;
+      for (int j = 0; j < np; ++j) {
+        const int32_t pj = m.hang_par[4 * hh + j];
+        se += a.eps[pj] / np;
+        sp += a.phi[pj] / np;
+      }
+      te[i] = se;
+      tp[i] = sp;
+    }
+  }
+  // the normal most recently written in this leaf, if any
+  bool has = false;
+  double N[3] = {0.0, 0.0, 0.0};
+  double cp = 0.0;
+  int nflux = 0;
+  for (int e = 0; e < 12; ++e) {
+    const int64_t slot = kFluxSlots * b + e;
+    const int i1 = kEN[2 * e], i2 = kEN[2 * e + 1];
+    double fl;
+    if (te[i1] < te[i2]) fl = 1.0;
+    else if (te[i1] > te[i2]) fl = -1.0;
+    else {
+      fvalid[slot] = 0;
+      continue;
+    }
+    const int ax = kEA[e];
+    const double f = a.frac[12 * b + e];
+    if (a.found[12 * b + e]) {
+      has = true;
+      for (int d = 0; d < 3; ++d) N[d] = a.nrm[36 * b + 3 * e + d];
+    }
+    double V[3] = {p0[i1][0], p0[i1][1], p0[i1][2]};
+    V[ax] += f * h[ax];
+    const double flux = -(tp[i2] - tp[i1]) * wha(te[i1], te[i2], f) * fl * area_h[ax];
+    cp += flux;
+    ++nflux;
+    fsrc[kFS * slot + 0] = V[0];
+    fsrc[kFS * slot + 1] = V[1];
+    fsrc[kFS * slot + 2] = V[2];
+    fsrc[kFS * slot + 3] = flux;
+    fvalid[slot] = 1;
+  }
+  int ntri = 0;
+  for (int s = 0; s < kTriSlots; ++s) vvalid[kTriSlots * b + s] = 0;
+  if (a.ionic) {
+    static constexpr int order[8] = {0, 1, 3, 2, 4, 5, 7, 6};
+    int ci = 0, bit = 1;
+    for (int o = 0; o < 8; ++o) {
+      if (te[order[o]] < a.eps_out - 1e-10) ci |= bit;
+      bit *= 2;
+    }
+    if (ci != 0 && ci != 255) {
+      for (int t = 0; t < 5 && kTri[ci][3 * t] >= 0; ++t) {
+        double vt[3][3];
+        int64_t vs[3];
+        for (int jj = 0; jj < 3; ++jj) {
+          const int edge = kTri[ci][3 * t + jj];
+          const int ax = kEA[edge];
+          const int i1 = kEN[2 * edge], i2 = kEN[2 * edge + 1];
+          const double f = a.frac[12 * b + edge];
+          const int64_t slot = kTriSlots * b + 3 * t + jj;
+          vs[jj] = slot;
+          uint8_t need = 0;
+          if (a.found[12 * b + edge]) {
+            has = true;
+            for (int d = 0; d < 3; ++d) N[d] = a.nrm[36 * b + 3 * edge + d];
+          } else if (!has) {
+            need = 1;
+          }
+          double V[3] = {p0[i1][0], p0[i1][1], p0[i1][2]};
+          V[ax] += f * h[ax];
+          for (int d = 0; d < 3; ++d) vt[jj][d] = V[d];
+          double* o = vsrc + kVS * slot;
+          o[0] = V[0];
+          o[1] = V[1];
+          o[2] = V[2];
+          o[3] = N[0];
+          o[4] = N[1];
+          o[5] = N[2];
+          o[6] = phi0(te[i1], te[i2], tp[i1], tp[i2], f);
+          vvalid[slot] = 1;
+          vneed[slot] = need;
+        }
+        double ab[3], ac[3];
+        for (int i = 0; i < 3; ++i) {
+          ab[i] = vt[0][i] - vt[1][i];
+          ac[i] = vt[0][i] - vt[2][i];
+        }
+        const double cx = ab[1] * ac[2] - ab[2] * ac[1], cy = ab[0] * ac[2] - ab[2] * ac[0],
+                     cz = ab[1] * ac[0] - ab[0] * ac[1];
+        const double area = 0.5 * sqrt(cx * cx + cy * cy + cz * cz);
+        for (int jj = 0; jj < 3; ++jj) vsrc[kVS * vs[jj] + 7] = area;
+        ++ntri;
+      }
+    }
+  }
+  leaf_charge[b] = cp;
+  leaf_has[b] = has ? 1 : 0;
+  leaf_last[3 * b] = N[0];
+  leaf_last[3 * b + 1] = N[1];
+  leaf_last[3 * b + 2] = N[2];
+  if (nflux) atomicAdd((unsigned long long*)&counts[0], (unsigned long long)nflux);
+  if (ntri) atomicAdd((unsigned long long*)&counts[1], (unsigned long long)ntri);
+}
+
+// carry[b] = normal last written before leaf b (zero before the first one).
+// One block: each thread owns a contiguous chunk of leaves.
+__global__ void __launch_bounds__(1024) k_normal_carry(int64_t nb, const uint8_t* __restrict__ has,
+                                                       const double* __restrict__ last, double* __restrict__ carry) {
+  __shared__ int64_t s_idx[1024];
+  const int T = 1024;
+  const int64_t chunk = (nb + T - 1) / T;
+  const int64_t b0 = threadIdx.x * chunk, b1 = min(nb, b0 + chunk);
+  int64_t li = -1;
+  for (int64_t b = b0; b < b1; ++b)
+    if (has[b]) li = b;
+  s_idx[threadIdx.x] = li;
+  __syncthreads();
+  // exclusive "last valid" prefix over the chunks (serial over 1024 entries by thread 0)
+  if (threadIdx.x == 0) {
+    int64_t run = -1;
+    for (int t = 0; t < T; ++t) {
+      const int64_t v = s_idx[t];
+      s_idx[t] = run;
+      if (v >= 0) run = v;
+    }
+  }
+  __syncthreads();
+  int64_t cur = s_idx[threadIdx.x];
+  for (int64_t b = b0; b < b1; ++b) {
+    for (int d = 0; d < 3; ++d) carry[3 * b + d] = cur >= 0 ? last[3 * cur + d] : 0.0;
+    if (has[b]) cur = b;
+  }
+}
+
+__global__ void k_apply_carry(int64_t nv, const uint8_t* __restrict__ vvalid, const uint8_t* __restrict__ vneed,
+                              const double* __restrict__ carry, double* __restrict__ vsrc) {
+  const int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= nv || !vvalid[s] || !vneed[s]) return;
+  const int64_t b = s / kTriSlots;
+  for (int d = 0; d < 3; ++d) vsrc[kVS * s + 3 + d] = carry[3 * b + d];
+}
+
+template <int W>
+__global__ void k_gather_src(int64_t n, const int32_t* __restrict__ idx, const double* __restrict__ src,
+                             double* __restrict__ dst) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const int64_t s = idx[i];
+  for (int k = 0; k < W; ++k) dst[W * i + k] = src[W * s + k];
+}
+
+__global__ void k_charged_flags(int64_t A, const double* __restrict__ q, uint8_t* __restrict__ f) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < A) f[i] = fabs(q[i]) > 1.e-5 ? 1 : 0;
+}
+
+__global__ void k_gather_atoms(int64_t n, const int32_t* __restrict__ idx, const double* __restrict__ xyz,
+                               const double* __restrict__ q, double4* __restrict__ out) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const int64_t a = idx[i];
+  out[i] = make_double4(xyz[3 * a], xyz[3 * a + 1], xyz[3 * a + 2], q[a]);
+}
+
+constexpr int kNB = 256;
+
+// MODE 0: flux, polarization-only branch  first += q * (flux / r)
+// MODE 1: flux, ionic branch             first += q * flux / r
+// MODE 2: triangle vertex                second += q phi dot / r^3 / (4 pi) * area / 3
+template <int MODE>
+__global__ void __launch_bounds__(kNB) k_nbody(int64_t ns, const double* __restrict__ src, int64_t nq,
+                                               const double4* __restrict__ atoms, double* __restrict__ acc) {
+  __shared__ double4 s_at[kNB];
+  const int64_t s = (int64_t)blockIdx.x * kNB + threadIdx.x;
+  const bool act = s < ns;
+  double v[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+  if (act)
+    for (int k = 0; k < (MODE == 2 ? kVS : kFS); ++k) v[k] = src[(MODE == 2 ? kVS : kFS) * s + k];
+  const double inv_4pi = 1.0 / (4.0 * kPi);
+  double sum = 0.0;
+  for (int64_t t0 = 0; t0 < nq; t0 += kNB) {
+    __syncthreads();
+    if (t0 + threadIdx.x < nq) s_at[threadIdx.x] = atoms[t0 + threadIdx.x];
+    __syncthreads();
+    const int cnt = (int)min((int64_t)kNB, nq - t0);
+    if (!act) continue;
+    for (int j = 0; j < cnt; ++j) {
+      const double4 at = s_at[j];
+      if (MODE < 2) {
+        const double dx = at.x - v[0], dy = at.y - v[1], dz = at.z - v[2];
+        const double r = sqrt(dx * dx + dy * dy + dz * dz);
+        if (MODE == 0) {
+          const double qflux = v[3] / r;
+          sum += at.w * qflux;
+        } else {
+          sum += at.w * v[3] / r;
+        }
+      } else {
+        const double dx = v[0] - at.x, dy = v[1] - at.y, dz = v[2] - at.z;
+        const double r2 = dx * dx + dy * dy + dz * dz;
+        const double r = sqrt(r2);
+        const double inv_r3 = 1.0 / (r2 * r);
+        const double dot = dx * v[3] + dy * v[4] + dz * v[5];
+        sum += at.w * v[6] * dot * inv_r3 * inv_4pi * v[7] / 3.0;
+      }
+    }
+  }
+  if (act) acc[s] = sum;
+}
+
+// out[0] = sum of a[0..na), out[1] = sum of b[0..nb), out[2] = sum of c[0..nc)
+__global__ void __launch_bounds__(1024) k_energy_fin(const double* __restrict__ a, int64_t na, const double* __restrict__ b,
+                                                     int64_t nbv, const double* __restrict__ c, int64_t nc,
+                                                     double* __restrict__ out) {
+  __shared__ double s_red[1024 / 64];
+  const double* arr[3] = {a, b, c};
+  const int64_t cnt[3] = {na, nbv, nc};
+  for (int k = 0; k < 3; ++k) {
+    double s = 0.0;
+    for (int64_t i = threadIdx.x; i < cnt[k]; i += 1024) s += arr[k][i];
+    s = dev::block_sum<1024>(s, s_red);
+    if (threadIdx.x == 0) out[k] = s;
+  }
+}
+
+void upload_tri_table(hipStream_t s) {
+  static signed char tab[256][16];
+  static bool ready = false;
+  if (!ready) {
+    for (int c = 0; c < 256; ++c) {
+      const char* t = kMcTriangles[c];
+      const int len = (int)std::strlen(t);
+      for (int i = 0; i < 16; ++i) tab[c][i] = -1;
+      for (int i = 0; i < len && i < 15; ++i) tab[c][i] = (signed char)(t[i] <= '9' ? t[i] - '0' : t[i] - 'a' + 10);
+    }
+    ready = true;
+  }
+  NGPB_HIP(hipMemcpyToSymbolAsync(HIP_SYMBOL(kTri), tab, sizeof(tab), 0, hipMemcpyHostToDevice, s));
+}
+
+}  // namespace
+
+void run_energy(ngpb_ctx* c, ngpb_energy_out* out) {
+  const int64_t nb = c->n_border;
+  const bool ionic = c->physics.calc_energy == 2 && c->pc.k > 1.e-5;
+  const bool polar_only = c->physics.calc_energy == 1 || (c->physics.calc_energy == 2 && c->pc.k < 1.e-5);
+  std::memset(out, 0, sizeof(*out));
+  out->n_border = nb;
+  if (nb == 0 || (!ionic && !polar_only)) return;
+  upload_tri_table(c->stream);
+  // charged atoms, in order
+  DBuf<uint8_t> aflag;
+  DBuf<int32_t> aidx;
+  DBuf<double4> atoms;
+  int64_t nq = 0;
+  if (c->A > 0) {
+    aflag.resize((size_t)c->A);
+    aidx.resize((size_t)c->A);
+    hipLaunchKernelGGL(k_charged_flags, dim3(blocks_for(c->A, 256)), dim3(256), 0, c->stream, c->A, c->atom_q.p, aflag.p);
+    nq = compact_flags(c, aflag.p, c->A, aidx.p);
+    atoms.resize((size_t)std::max<int64_t>(nq, 1));
+    if (nq)
+      hipLaunchKernelGGL(k_gather_atoms, dim3(blocks_for(nq, 256)), dim3(256), 0, c->stream, nq, aidx.p, c->atom_xyz.p,
+                         c->atom_q.p, atoms.p);
+  }
+  const int64_t nf = kFluxSlots * nb, nv = kTriSlots * nb;
+  DBuf<double> fsrc, vsrc, leaf_charge, leaf_last, carry;
+  DBuf<uint8_t> fvalid, vvalid, vneed, leaf_has;
+  DBuf<int64_t> counts;
+  fsrc.resize((size_t)kFS * nf);
+  vsrc.resize((size_t)kVS * nv);
+  fvalid.resize((size_t)nf);
+  vvalid.resize((size_t)nv);
+  vneed.resize((size_t)nv);
+  leaf_charge.resize((size_t)nb);
+  leaf_last.resize((size_t)3 * nb);
+  leaf_has.resize((size_t)nb);
+  carry.resize((size_t)3 * nb);
+  counts.resize(2);
+  NGPB_HIP(hipMemsetAsync(counts.p, 0, 2 * sizeof(int64_t), c->stream));
+  NGPB_HIP(hipMemsetAsync(vneed.p, 0, nv, c->stream));
+  EnergyArgs ea;
+  ea.m = mesh_view_of(c);
+  ea.nb = nb;
+  ea.border = c->border.p;
+  ea.eps = c->eps.p;
+  ea.phi = c->x.p;
+  ea.frac = c->frac.p;
+  ea.nrm = c->frac_normal.p;
+  ea.found = c->frac_found.p;
+  ea.eps_out = c->pc.eps_out;
+  ea.ionic = ionic ? 1 : 0;
+  hipLaunchKernelGGL(k_sources, dim3(blocks_for(nb, 64)), dim3(64), 0, c->stream, ea, fsrc.p, fvalid.p, vsrc.p, vvalid.p,
+                     vneed.p, leaf_charge.p, leaf_last.p, leaf_has.p, counts.p);
+  NGPB_HIP(hipGetLastError());
+  if (ionic) {
+    hipLaunchKernelGGL(k_normal_carry, dim3(1), dim3(1024), 0, c->stream, nb, leaf_has.p, leaf_last.p, carry.p);
+    hipLaunchKernelGGL(k_apply_carry, dim3(blocks_for(nv, 256)), dim3(256), 0, c->stream, nv, vvalid.p, vneed.p, carry.p,
+                       vsrc.p);
+    NGPB_HIP(hipGetLastError());
+  }
+  // compact valid sources (slot order is the reference's summation order)
+  DBuf<int32_t> fidx, vidx;
+  fidx.resize((size_t)nf);
+  vidx.resize((size_t)nv);
+  const int64_t nfv = compact_flags(c, fvalid.p, nf, fidx.p);
+  const int64_t nvv = ionic ? compact_flags(c, vvalid.p, nv, vidx.p) : 0;
+  DBuf<double> fd, vd, facc, vacc;
+  fd.resize((size_t)kFS * std::max<int64_t>(nfv, 1));
+  vd.resize((size_t)kVS * std::max<int64_t>(nvv, 1));
+  facc.resize((size_t)std::max<int64_t>(nfv, 1));
+  vacc.resize((size_t)std::max<int64_t>(nvv, 1));
+  if (nfv)
+    hipLaunchKernelGGL(k_gather_src<kFS>, dim3(blocks_for(nfv, 256)), dim3(256), 0, c->stream, nfv, fidx.p, fsrc.p, fd.p);
+  if (nvv)
+    hipLaunchKernelGGL(k_gather_src<kVS>, dim3(blocks_for(nvv, 256)), dim3(256), 0, c->stream, nvv, vidx.p, vsrc.p, vd.p);
+  if (nfv) {
+    if (ionic)
+      hipLaunchKernelGGL(k_nbody<1>, dim3(blocks_for(nfv, kNB)), dim3(kNB), 0, c->stream, nfv, fd.p, nq, atoms.p, facc.p);
+    else
+      hipLaunchKernelGGL(k_nbody<0>, dim3(blocks_for(nfv, kNB)), dim3(kNB), 0, c->stream, nfv, fd.p, nq, atoms.p, facc.p);
+  }
+  if (nvv)
+    hipLaunchKernelGGL(k_nbody<2>, dim3(blocks_for(nvv, kNB)), dim3(kNB), 0, c->stream, nvv, vd.p, nq, atoms.p, vacc.p);
+  NGPB_HIP(hipGetLastError());
+  c->en_out.resize(4);
+  hipLaunchKernelGGL(k_energy_fin, dim3(1), dim3(1024), 0, c->stream, facc.p, nfv, vacc.p, nvv, leaf_charge.p, nb,
+                     c->en_out.p);
+  NGPB_HIP(hipGetLastError());
+  double h[3];
+  int64_t hc[2];
+  NGPB_HIP(hipMemcpyAsync(h, c->en_out.p, 3 * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  NGPB_HIP(hipMemcpyAsync(hc, counts.p, 2 * sizeof(int64_t), hipMemcpyDeviceToHost, c->stream));
+  NGPB_HIP(hipStreamSynchronize(c->stream));
+  const double first_int = h[0], second_int = h[1];
+  out->charge_pol = h[2];
+  out->n_flux_edges = hc[0];
+  out->n_triangles = hc[1];
+  out->energy_pol = 0.5 * c->pc.constant_pol * first_int;
+  out->energy_react = ionic ? 0.5 * (second_int - first_int * c->pc.constant_react) : 0.0;
+}
+
+}  // namespace ngpb

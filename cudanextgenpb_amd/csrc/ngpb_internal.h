@@ -1,0 +1,332 @@
+// Internal declarations of the gfx950 linearized-PB library (not part of the ABI).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "ngpb.h"
+
+namespace ngpb {
+
+// --------------------------------------------------------------- errors
+struct Error : std::runtime_error {
+  int code;
+  Error(int c, const std::string& m) : std::runtime_error(m), code(c) {}
+};
+
+[[noreturn]] void throw_hip(hipError_t e, const char* what, const char* file, int line);
+
+#define NGPB_HIP(call)                                               \
+  do {                                                               \
+    hipError_t e_ = (call);                                          \
+    if (e_ != hipSuccess) ::ngpb::throw_hip(e_, #call, __FILE__, __LINE__); \
+  } while (0)
+
+#define NGPB_REQUIRE(cond, code, msg)                 \
+  do {                                                \
+    if (!(cond)) throw ::ngpb::Error((code), (msg));  \
+  } while (0)
+
+// --------------------------------------------------------------- device buffers
+template <class T>
+struct DBuf {
+  T* p = nullptr;
+  size_t n = 0;
+  DBuf() = default;
+  DBuf(const DBuf&) = delete;
+  DBuf& operator=(const DBuf&) = delete;
+  ~DBuf() { release(); }
+  void release() {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    n = 0;
+  }
+  // (re)allocates when the size differs; contents are undefined afterwards
+  void resize(size_t count) {
+    if (count == n && p) return;
+    release();
+    if (count == 0) return;
+    hipError_t e = hipMalloc((void**)&p, count * sizeof(T));
+    if (e != hipSuccess) {
+      p = nullptr;
+      (void)hipGetLastError();
+      throw Error(NGPB_E_NOMEM, "hipMalloc of " + std::to_string(count * sizeof(T)) + " bytes failed");
+    }
+    n = count;
+  }
+  // grows only
+  void reserve(size_t count) {
+    if (count > n) resize(count);
+  }
+  void upload(const T* h, size_t count, hipStream_t s) {
+    resize(count);
+    if (count) NGPB_HIP(hipMemcpyAsync(p, h, count * sizeof(T), hipMemcpyHostToDevice, s));
+  }
+  void download(T* h, size_t count, hipStream_t s) const {
+    if (count) NGPB_HIP(hipMemcpyAsync(h, p, count * sizeof(T), hipMemcpyDeviceToHost, s));
+  }
+  size_t bytes() const { return n * sizeof(T); }
+};
+
+// --------------------------------------------------------------- physics
+struct PhysConst {
+  double eps_in, eps_out, k2, k, constant_pol, constant_react;
+};
+PhysConst physical_constants(const ngpb_physics& p);
+
+// --------------------------------------------------------------- device views
+struct RayView {
+  int64_t n = 0;
+  const double* key = nullptr;     // 2 per ray
+  const int64_t* ptr = nullptr;    // n+1
+  const double* inters = nullptr;  // K
+  const double* normals = nullptr; // 3K
+};
+
+struct MeshView {
+  int64_t N = 0, H = 0, L = 0;
+  const double* node_xyz = nullptr;
+  const double* hang_xyz = nullptr;
+  const int32_t* hang_np = nullptr;
+  const int32_t* hang_par = nullptr;
+  const int32_t* leaf_corner = nullptr;
+};
+
+// deterministic block/grid reduction bookkeeping
+struct GridReduce {
+  DBuf<double> partials;    // per-block partial sums (up to 4 per block)
+  DBuf<unsigned> tickets;   // arrival counters, self-resetting
+};
+
+// scalar state of a Krylov solve on the device
+struct KrylovScalars {
+  double rho;       // (r, z) of the current iterate
+  double pq;        // (p, A p)
+  double alpha, beta;
+  double rr;        // (r, r) of the current iterate
+  double stop_rr;   // stop when rr <= stop_rr
+  double bb;        // (b, b)
+  double aux[4];
+  int iter;         // iterations done in this solve
+  int done;         // 1 = stop flag raised
+  int maxiter;      // iteration limit for the current cycle
+  int status;       // 0 converged, 4 maxiter
+};
+
+// ----------------------------------------------------------------- context
+}  // namespace ngpb
+
+struct ngpb_ctx {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  std::string err;
+
+  bool have_physics = false;
+  ngpb_physics physics{};
+  ngpb::PhysConst pc{};
+
+  // mesh
+  int64_t N = 0, H = 0, L = 0;
+  ngpb::DBuf<double> node_xyz, hang_xyz;
+  ngpb::DBuf<int32_t> hang_np, hang_par, leaf_corner;
+  bool have_mesh = false;
+
+  // rays
+  struct Rays {
+    int64_t n = 0, k = 0;
+    ngpb::DBuf<double> key, inters, normals;
+    ngpb::DBuf<int64_t> ptr;
+  } rays[3];
+
+  // markers
+  bool have_markers = false;
+  ngpb::DBuf<double> eps, reac;
+  ngpb::DBuf<uint8_t> leaf_marker;
+  ngpb::DBuf<int32_t> border;      // border leaf ids
+  ngpb::DBuf<int32_t> border_of;   // leaf -> border index or -1
+  int64_t n_border = 0;
+
+  // fractions of border leaves
+  bool have_frac = false;
+  ngpb::DBuf<double> frac, frac_normal;  // 12 / 36 per border leaf
+  ngpb::DBuf<uint8_t> frac_found;        // 12 per border leaf
+
+  // charges
+  int64_t A = 0;
+  ngpb::DBuf<double> atom_xyz, atom_q;
+  ngpb::DBuf<int32_t> atom_owner;
+  bool have_charges = false;
+
+  // Dirichlet data
+  int64_t NB = 0;
+  ngpb::DBuf<int32_t> bnd_node;
+  ngpb::DBuf<double> bnd_val;
+  ngpb::DBuf<uint8_t> is_bnd;   // N
+  ngpb::DBuf<double> gval;      // N (value on boundary nodes, 0 elsewhere)
+  bool have_dirichlet = false, have_bnd_values = false;
+
+  // symbolic assembly (depends on mesh + Dirichlet set)
+  bool have_symbolic = false;
+  ngpb::DBuf<int64_t> em_rowptr;  // N+1
+  ngpb::DBuf<uint64_t> em_key;    // (col << 32) | id
+  ngpb::DBuf<uint16_t> em_kw;     // (k << 8) | weight code
+  ngpb::DBuf<int64_t> slot_em;    // 2 per final slot: emission range
+  ngpb::DBuf<int32_t> diag_slot;  // N
+  ngpb::DBuf<int32_t> bc_rowptr;  // N+1
+  ngpb::DBuf<int32_t> bc_col;
+  ngpb::DBuf<int64_t> bc_em;      // 2 per coupling
+
+  // assembled system
+  bool have_system = false;
+  int64_t n = 0, nnz = 0;
+  ngpb::DBuf<int32_t> rowptr, col;
+  ngpb::DBuf<double> val, rhs, rhs_charge;
+  ngpb::DBuf<double> c_edge, v8;  // per leaf-edge conductance, per leaf V/8
+  ngpb::DBuf<double> vol_patch, rho;
+
+  // compact mixed-precision operator
+  bool have_compact = false;
+  int64_t n_slices = 0, sell_entries = 0, sell_wide_entries = 0, sell_bases = 0;
+  ngpb::DBuf<int32_t> slice_width;   // per slice
+  ngpb::DBuf<int64_t> slice_ptr;     // per slice: first entry (64 * sum of widths)
+  ngpb::DBuf<int64_t> slice_base_ptr;// per slice: first base
+  ngpb::DBuf<int64_t> slice_wide_ptr;// per slice: first 32-bit column (or -1)
+  ngpb::DBuf<float> sell_val;
+  ngpb::DBuf<uint16_t> sell_off;
+  ngpb::DBuf<int32_t> sell_base, sell_col32;
+  ngpb::DBuf<double> dscale;         // 1/sqrt(diag)
+
+  // solver state
+  ngpb::DBuf<double> x, r, z, p, q, dinv;
+  ngpb::DBuf<float> xs, rs, ps, qs;
+  ngpb::DBuf<ngpb::KrylovScalars> ks;
+  ngpb::KrylovScalars* ks_host = nullptr;  // pinned mirror
+  ngpb::GridReduce red;
+  bool have_phi = false;
+
+  // energy scratch
+  ngpb::DBuf<double> en_src, en_acc;
+  ngpb::DBuf<double> en_out;
+  ngpb::DBuf<int64_t> en_count;
+
+  // scan scratch
+  ngpb::DBuf<int64_t> scan_tmp;
+
+  int64_t device_bytes() const;
+};
+
+namespace ngpb {
+
+// ----------------------------------------------------------------- kernels (host wrappers)
+// scan
+int64_t exclusive_scan(ngpb_ctx* c, const int64_t* d_in, int64_t* d_out, int64_t n);
+// compaction: indices i in [0, n) with flag[i] != 0, in order; returns count
+int64_t compact_flags(ngpb_ctx* c, const uint8_t* d_flag, int64_t n, int32_t* d_out);
+
+// markers
+void run_markers(ngpb_ctx* c);
+void run_border_from_markers(ngpb_ctx* c);
+void run_fractions(ngpb_ctx* c);
+
+// assembly
+void run_symbolic(ngpb_ctx* c);
+void run_numeric(ngpb_ctx* c);
+void run_dirichlet_values(ngpb_ctx* c, int bc_type);
+
+// Krylov
+void ensure_reduce(ngpb_ctx* c, int64_t blocks);
+void spmv_fp64(ngpb_ctx* c, const double* x, double* y);
+void solve_fp64(ngpb_ctx* c, const ngpb_solve_opts& o, ngpb_solve_stats& st);
+double spmv_fp64_bytes(const ngpb_ctx* c);
+double time_spmv_fp64(ngpb_ctx* c, int reps);
+
+// mixed
+void build_compact(ngpb_ctx* c);
+void spmv_compact(ngpb_ctx* c, const double* x, double* y);
+void solve_mixed(ngpb_ctx* c, const ngpb_solve_opts& o, ngpb_solve_stats& st);
+double spmv_compact_bytes(const ngpb_ctx* c);
+double time_spmv_compact(ngpb_ctx* c, int reps);
+
+// energy
+void run_energy(ngpb_ctx* c, ngpb_energy_out* out);
+
+// launch helper
+inline unsigned blocks_for(int64_t n, int bs) { return (unsigned)((n + bs - 1) / bs); }
+
+}  // namespace ngpb
+
+// ------------------------------------------------------------------- device helpers
+namespace ngpb {
+namespace dev {
+
+__device__ __forceinline__ double corner_coord(const MeshView& m, int32_t c, int d) {
+  return c >= 0 ? m.node_xyz[3 * (int64_t)c + d] : m.hang_xyz[3 * (int64_t)(-1 - c) + d];
+}
+
+// index of the ray with key (a, b) or -1 (exact comparison, lexicographic order)
+__device__ __forceinline__ int64_t ray_find(const RayView& r, double a, double b) {
+  int64_t lo = 0, hi = r.n;
+  while (lo < hi) {
+    const int64_t mid = (lo + hi) >> 1;
+    const double ka = r.key[2 * mid], kb = r.key[2 * mid + 1];
+    if (ka < a || (ka == a && kb < b)) lo = mid + 1;
+    else hi = mid;
+  }
+  if (lo < r.n && r.key[2 * lo] == a && r.key[2 * lo + 1] == b) return lo;
+  return -1;
+}
+
+// wave64 sum (fixed butterfly order)
+__device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+  return v;
+}
+
+// block sum for blockDim.x == BS (multiple of 64), result valid in all threads.
+template <int BS>
+__device__ __forceinline__ double block_sum(double v, double* lds /* >= BS/64 */) {
+  v = wave_sum(v);
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  if (lane == 0) lds[wid] = v;
+  __syncthreads();
+  double t = 0.0;
+#pragma unroll
+  for (int w = 0; w < BS / 64; ++w) t += lds[w];
+  __syncthreads();
+  return t;
+}
+
+// Publishes this block's partial(s) and returns true in the block that arrives last.
+// The caller of the last block must then combine partials[0..gridDim.x) in a fixed order.
+__device__ __forceinline__ bool arrive_last(unsigned* ticket, int* lds_flag) {
+  __threadfence();
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    const unsigned t = atomicAdd(ticket, 1u);
+    *lds_flag = (t == gridDim.x - 1) ? 1 : 0;
+  }
+  __syncthreads();
+  const bool last = *lds_flag != 0;
+  if (last) __threadfence();
+  return last;
+}
+
+__device__ __forceinline__ double load_coherent(const double* p) {
+  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// fixed-order sum of `count` partials with stride `stride` by one block of BS threads
+template <int BS>
+__device__ __forceinline__ double sum_partials(const double* part, int count, int stride, double* lds) {
+  double s = 0.0;
+  for (int i = threadIdx.x; i < count; i += BS) s += load_coherent(part + (int64_t)i * stride);
+  return block_sum<BS>(s, lds);
+}
+
+}  // namespace dev
+}  // namespace ngpb

@@ -1,0 +1,289 @@
+// FP64 CSR SpMV and the plain Jacobi-preconditioned CG (LIS -i cg -p jacobi as
+// called from lis_compute_electric_potential, src/pb_class.cpp:2243-2329).
+//
+// SpMV: a block of 256 threads owns 256 consecutive rows. Their column indices and
+// values are read once, coalesced, and each (value, x[column]) pair is staged in
+// LDS; each thread then forms its row sum in column order as a chain of fused
+// multiply-adds starting from zero (the same arithmetic as the oracle's SpMV, so the
+// results are bit-identical). Blocks whose rows hold more than kCap entries read
+// global memory directly.
+//
+// CG: three kernels per iteration (SpMV + (p,q); x/r update + (r,z), (r,r); new
+// direction). Dot products are reduced per block in a fixed order and combined by
+// the last block to arrive, which also updates the scalars (alpha, beta, stop flag)
+// in device memory, so the host only polls the stop flag every `check_every`
+// iterations and iterations launched after convergence exit immediately.
+#include <algorithm>
+#include <cmath>
+
+#include "ngpb_internal.h"
+
+namespace ngpb {
+namespace {
+
+constexpr int kBS = 256;
+constexpr int kCap = 2048;  // staged entries per block
+
+enum { T_SPMV = 0, T_UPD = 1, T_INIT = 2, T_MISC = 3 };
+
+template <bool DOT>
+__global__ void __launch_bounds__(kBS) k_spmv64(int64_t n, const int32_t* __restrict__ rowptr,
+                                                 const int32_t* __restrict__ col, const double* __restrict__ val,
+                                                 const double* __restrict__ x, double* __restrict__ y,
+                                                 KrylovScalars* ks, double* partials, unsigned* ticket) {
+  __shared__ double s_val[kCap];
+  __shared__ double s_x[kCap];
+  __shared__ double s_red[kBS / 64];
+  __shared__ int s_flag;
+  if (DOT && ks->done) return;
+  const int64_t r0 = (int64_t)blockIdx.x * kBS;
+  const int64_t r1 = min(r0 + kBS, n);
+  const int64_t row = r0 + threadIdx.x;
+  const int64_t e0 = rowptr[r0], e1 = rowptr[r1];
+  double sum = 0.0;
+  if (e1 - e0 <= kCap) {
+    for (int64_t k = e0 + threadIdx.x; k < e1; k += kBS) {
+      s_val[k - e0] = val[k];
+      s_x[k - e0] = x[col[k]];
+    }
+    __syncthreads();
+    if (row < n) {
+      const int64_t b = rowptr[row] - e0, e = rowptr[row + 1] - e0;
+      for (int64_t k = b; k < e; ++k) sum = fma(s_val[k], s_x[k], sum);
+      y[row] = sum;
+    }
+  } else if (row < n) {
+    for (int64_t k = rowptr[row]; k < rowptr[row + 1]; ++k) sum = fma(val[k], x[col[k]], sum);
+    y[row] = sum;
+  }
+  if (DOT) {
+    const double part = dev::block_sum<kBS>(row < n ? x[row] * sum : 0.0, s_red);
+    if (threadIdx.x == 0) partials[blockIdx.x] = part;
+    if (dev::arrive_last(ticket, &s_flag)) {
+      const double pq = dev::sum_partials<kBS>(partials, gridDim.x, 1, s_red);
+      if (threadIdx.x == 0) {
+        ks->pq = pq;
+        ks->alpha = ks->rho / pq;
+        *ticket = 0;
+      }
+    }
+  }
+}
+
+// x = 0, r = b, p = D^-1 b; rho = (r, D^-1 r), bb = (b, b)
+__global__ void __launch_bounds__(kBS) k_pcg_init(int64_t n, const double* __restrict__ b, const double* __restrict__ dinv,
+                                                  double* __restrict__ x, double* __restrict__ r, double* __restrict__ p,
+                                                  KrylovScalars* ks, double tol2, int maxiter, double* partials,
+                                                  unsigned* ticket) {
+  __shared__ double s_red[kBS / 64];
+  __shared__ int s_flag;
+  double rz = 0.0, bb = 0.0;
+  for (int64_t i = (int64_t)blockIdx.x * kBS + threadIdx.x; i < n; i += (int64_t)gridDim.x * kBS) {
+    const double bi = b[i], zi = dinv[i] * bi;
+    x[i] = 0.0;
+    r[i] = bi;
+    p[i] = zi;
+    rz += bi * zi;
+    bb += bi * bi;
+  }
+  rz = dev::block_sum<kBS>(rz, s_red);
+  bb = dev::block_sum<kBS>(bb, s_red);
+  if (threadIdx.x == 0) {
+    partials[2 * blockIdx.x] = rz;
+    partials[2 * blockIdx.x + 1] = bb;
+  }
+  if (dev::arrive_last(ticket, &s_flag)) {
+    const double RZ = dev::sum_partials<kBS>(partials, gridDim.x, 2, s_red);
+    const double BB = dev::sum_partials<kBS>(partials + 1, gridDim.x, 2, s_red);
+    if (threadIdx.x == 0) {
+      ks->rho = RZ;
+      ks->bb = BB;
+      ks->rr = BB;
+      ks->stop_rr = tol2 * BB;
+      ks->iter = 0;
+      ks->maxiter = maxiter;
+      ks->status = 0;
+      ks->done = (BB == 0.0) ? 1 : 0;
+      *ticket = 0;
+    }
+  }
+}
+
+// x += alpha p; r -= alpha q; then (r, D^-1 r), (r, r); stop test; beta
+__global__ void __launch_bounds__(kBS) k_pcg_update(int64_t n, const double* __restrict__ dinv, double* __restrict__ x,
+                                                    double* __restrict__ r, const double* __restrict__ p,
+                                                    const double* __restrict__ q, KrylovScalars* ks, double* partials,
+                                                    unsigned* ticket) {
+  __shared__ double s_red[kBS / 64];
+  __shared__ int s_flag;
+  if (ks->done) return;
+  const double alpha = ks->alpha;
+  double rz = 0.0, rr = 0.0;
+  for (int64_t i = (int64_t)blockIdx.x * kBS + threadIdx.x; i < n; i += (int64_t)gridDim.x * kBS) {
+    x[i] += alpha * p[i];
+    const double ri = r[i] - alpha * q[i];
+    r[i] = ri;
+    rz += ri * (dinv[i] * ri);
+    rr += ri * ri;
+  }
+  rz = dev::block_sum<kBS>(rz, s_red);
+  rr = dev::block_sum<kBS>(rr, s_red);
+  if (threadIdx.x == 0) {
+    partials[2 * blockIdx.x] = rz;
+    partials[2 * blockIdx.x + 1] = rr;
+  }
+  if (dev::arrive_last(ticket, &s_flag)) {
+    const double RZ = dev::sum_partials<kBS>(partials, gridDim.x, 2, s_red);
+    const double RR = dev::sum_partials<kBS>(partials + 1, gridDim.x, 2, s_red);
+    if (threadIdx.x == 0) {
+      const int it = ks->iter + 1;
+      ks->iter = it;
+      ks->rr = RR;
+      if (RR <= ks->stop_rr) {
+        ks->done = 1;
+        ks->status = 0;
+      } else if (it >= ks->maxiter) {
+        ks->done = 1;
+        ks->status = NGPB_E_NOCONV;
+      } else {
+        ks->beta = RZ / ks->rho;
+        ks->rho = RZ;
+      }
+      *ticket = 0;
+    }
+  }
+}
+
+__global__ void __launch_bounds__(kBS) k_pcg_dir(int64_t n, const double* __restrict__ dinv, const double* __restrict__ r,
+                                                 double* __restrict__ p, const KrylovScalars* ks) {
+  if (ks->done) return;
+  const double beta = ks->beta;
+  for (int64_t i = (int64_t)blockIdx.x * kBS + threadIdx.x; i < n; i += (int64_t)gridDim.x * kBS)
+    p[i] = dinv[i] * r[i] + beta * p[i];
+}
+
+__global__ void k_diag_inv(int64_t n, const int32_t* __restrict__ rowptr, const int32_t* __restrict__ col,
+                           const double* __restrict__ val, double* __restrict__ dinv, int* __restrict__ bad) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  double d = 0.0;
+  for (int64_t k = rowptr[i]; k < rowptr[i + 1]; ++k) {
+    const double v = val[k];
+    if (!isfinite(v)) atomicOr(bad, 2);
+    if (col[k] == i) d = v;
+  }
+  if (!(d > 0.0)) atomicOr(bad, 1);
+  dinv[i] = 1.0 / d;
+}
+
+unsigned vec_grid(int64_t n) { return (unsigned)std::max<int64_t>(1, std::min<int64_t>((n + 4 * kBS - 1) / (4 * kBS), 2048)); }
+
+}  // namespace
+
+void ensure_reduce(ngpb_ctx* c, int64_t blocks) {
+  const size_t need = (size_t)std::max<int64_t>(blocks, 2048) * 4;
+  if (c->red.partials.n < need) c->red.partials.resize(need);
+  if (c->red.tickets.n < 16) {
+    c->red.tickets.resize(16);
+    NGPB_HIP(hipMemsetAsync(c->red.tickets.p, 0, 16 * sizeof(unsigned), c->stream));
+  }
+  if (!c->ks.p) c->ks.resize(1);
+  if (!c->ks_host) NGPB_HIP(hipHostMalloc((void**)&c->ks_host, sizeof(KrylovScalars), hipHostMallocDefault));
+}
+
+double spmv_fp64_bytes(const ngpb_ctx* c) {
+  // nnz (value + column) + row pointers + x read + y written
+  return (double)c->nnz * (8.0 + 4.0) + (double)(c->n + 1) * 4.0 + (double)c->n * 16.0;
+}
+
+void spmv_fp64(ngpb_ctx* c, const double* x, double* y) {
+  hipLaunchKernelGGL(k_spmv64<false>, dim3(blocks_for(c->n, kBS)), dim3(kBS), 0, c->stream, c->n, c->rowptr.p, c->col.p,
+                     c->val.p, x, y, nullptr, nullptr, nullptr);
+  NGPB_HIP(hipGetLastError());
+}
+
+double time_spmv_fp64(ngpb_ctx* c, int reps) {
+  c->p.reserve((size_t)c->n);
+  c->q.reserve((size_t)c->n);
+  NGPB_HIP(hipMemsetAsync(c->p.p, 0, c->n * sizeof(double), c->stream));
+  hipEvent_t e0, e1;
+  NGPB_HIP(hipEventCreate(&e0));
+  NGPB_HIP(hipEventCreate(&e1));
+  spmv_fp64(c, c->p.p, c->q.p);  // warm-up
+  NGPB_HIP(hipEventRecord(e0, c->stream));
+  for (int i = 0; i < reps; ++i) spmv_fp64(c, c->p.p, c->q.p);
+  NGPB_HIP(hipEventRecord(e1, c->stream));
+  NGPB_HIP(hipEventSynchronize(e1));
+  float ms = 0.f;
+  NGPB_HIP(hipEventElapsedTime(&ms, e0, e1));
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  return (double)ms / std::max(reps, 1);
+}
+
+void check_operator(ngpb_ctx* c) {
+  c->dinv.resize((size_t)c->n);
+  DBuf<int> bad;
+  bad.resize(1);
+  NGPB_HIP(hipMemsetAsync(bad.p, 0, sizeof(int), c->stream));
+  hipLaunchKernelGGL(k_diag_inv, dim3(blocks_for(c->n, 256)), dim3(256), 0, c->stream, c->n, c->rowptr.p, c->col.p,
+                     c->val.p, c->dinv.p, bad.p);
+  NGPB_HIP(hipGetLastError());
+  int hb = 0;
+  NGPB_HIP(hipMemcpyAsync(&hb, bad.p, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+  NGPB_HIP(hipStreamSynchronize(c->stream));
+  NGPB_REQUIRE(!(hb & 2), NGPB_E_NUMERIC, "operator holds NaN or Inf entries");
+  NGPB_REQUIRE(!(hb & 1), NGPB_E_NUMERIC, "operator has a zero or negative diagonal entry");
+}
+
+void solve_fp64(ngpb_ctx* c, const ngpb_solve_opts& o, ngpb_solve_stats& st) {
+  const int64_t n = c->n;
+  check_operator(c);
+  c->x.resize((size_t)n);
+  c->r.resize((size_t)n);
+  c->p.reserve((size_t)n);
+  c->q.reserve((size_t)n);
+  const unsigned gs = blocks_for(n, kBS), gv = vec_grid(n);
+  ensure_reduce(c, std::max<int64_t>(gs, 2 * (int64_t)gv));
+  double* part = c->red.partials.p;
+  unsigned* tk = c->red.tickets.p;
+  KrylovScalars* ks = c->ks.p;
+  hipEvent_t e0, e1;
+  NGPB_HIP(hipEventCreate(&e0));
+  NGPB_HIP(hipEventCreate(&e1));
+  NGPB_HIP(hipEventRecord(e0, c->stream));
+  hipLaunchKernelGGL(k_pcg_init, dim3(gv), dim3(kBS), 0, c->stream, n, c->rhs.p, c->dinv.p, c->x.p, c->r.p, c->p.p, ks,
+                     o.tol * o.tol, o.maxiter, part, tk + T_INIT);
+  NGPB_HIP(hipGetLastError());
+  const int chunk = std::max(1, o.check_every);
+  while (true) {
+    for (int k = 0; k < chunk; ++k) {
+      hipLaunchKernelGGL(k_spmv64<true>, dim3(gs), dim3(kBS), 0, c->stream, n, c->rowptr.p, c->col.p, c->val.p, c->p.p,
+                         c->q.p, ks, part, tk + T_SPMV);
+      hipLaunchKernelGGL(k_pcg_update, dim3(gv), dim3(kBS), 0, c->stream, n, c->dinv.p, c->x.p, c->r.p, c->p.p, c->q.p,
+                         ks, part, tk + T_UPD);
+      hipLaunchKernelGGL(k_pcg_dir, dim3(gv), dim3(kBS), 0, c->stream, n, c->dinv.p, c->r.p, c->p.p, ks);
+    }
+    NGPB_HIP(hipGetLastError());
+    NGPB_HIP(hipMemcpyAsync(c->ks_host, ks, sizeof(KrylovScalars), hipMemcpyDeviceToHost, c->stream));
+    NGPB_HIP(hipStreamSynchronize(c->stream));
+    if (c->ks_host->done) break;
+  }
+  NGPB_HIP(hipEventRecord(e1, c->stream));
+  NGPB_HIP(hipEventSynchronize(e1));
+  float ms = 0.f;
+  NGPB_HIP(hipEventElapsedTime(&ms, e0, e1));
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  const KrylovScalars& h = *c->ks_host;
+  st.status = h.status;
+  st.iterations = h.iter;
+  st.replacements = 0;
+  st.relres = h.bb > 0 ? std::sqrt(h.rr / h.bb) : 0.0;
+  st.ms_solve = ms;
+  // SpMV + update (x, p, r, q, dinv read; x, r written) + direction (dinv, r, p read; p written)
+  st.bytes_per_iter = spmv_fp64_bytes(c) + (double)n * 8.0 * (7.0 + 4.0);
+  c->have_phi = true;
+}
+
+}  // namespace ngpb

@@ -1,0 +1,141 @@
+// Device-wide exclusive prefix sum (int64) and stream compaction.
+// Three passes over tiles of 2048 items: tile sums, a single-block scan of the
+// tile sums, and the per-tile scan with its offset. Integer arithmetic, so the
+// result does not depend on scheduling.
+#include "ngpb_internal.h"
+
+namespace ngpb {
+namespace {
+
+constexpr int kBS = 256;
+constexpr int kIPT = 8;
+constexpr int kTile = kBS * kIPT;
+
+__device__ __forceinline__ int64_t wave_incl_scan(int64_t v) {
+  const int lane = threadIdx.x & 63;
+#pragma unroll
+  for (int off = 1; off < 64; off <<= 1) {
+    const int64_t t = __shfl_up(v, off, 64);
+    if (lane >= off) v += t;
+  }
+  return v;
+}
+
+// exclusive scan across the block of one value per thread; *total = block sum
+__device__ __forceinline__ int64_t block_excl_scan(int64_t v, int64_t* lds, int64_t* total) {
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  const int64_t inc = wave_incl_scan(v);
+  if (lane == 63) lds[wid] = inc;
+  __syncthreads();
+  int64_t before = 0, all = 0;
+#pragma unroll
+  for (int w = 0; w < kBS / 64; ++w) {
+    const int64_t t = lds[w];
+    if (w < wid) before += t;
+    all += t;
+  }
+  __syncthreads();
+  *total = all;
+  return before + inc - v;
+}
+
+__global__ void __launch_bounds__(kBS) k_tile_sums(const int64_t* in, int64_t n, int64_t* tile_sums) {
+  __shared__ int64_t lds[kBS / 64];
+  const int64_t base = (int64_t)blockIdx.x * kTile + (int64_t)threadIdx.x * kIPT;
+  int64_t s = 0;
+#pragma unroll
+  for (int k = 0; k < kIPT; ++k)
+    if (base + k < n) s += in[base + k];
+  int64_t tot;
+  (void)block_excl_scan(s, lds, &tot);
+  if (threadIdx.x == 0) tile_sums[blockIdx.x] = tot;
+}
+
+__global__ void __launch_bounds__(kBS) k_scan_single(int64_t* a, int64_t n, int64_t* total) {
+  __shared__ int64_t lds[kBS / 64];
+  int64_t carry = 0;
+  for (int64_t t0 = 0; t0 < n; t0 += kTile) {
+    const int64_t base = t0 + (int64_t)threadIdx.x * kIPT;
+    int64_t v[kIPT];
+    int64_t s = 0;
+#pragma unroll
+    for (int k = 0; k < kIPT; ++k) {
+      v[k] = (base + k < n) ? a[base + k] : 0;
+      s += v[k];
+    }
+    int64_t tot;
+    int64_t ex = block_excl_scan(s, lds, &tot) + carry;
+#pragma unroll
+    for (int k = 0; k < kIPT; ++k)
+      if (base + k < n) {
+        a[base + k] = ex;
+        ex += v[k];
+      }
+    carry += tot;
+  }
+  if (threadIdx.x == 0 && total) *total = carry;
+}
+
+__global__ void __launch_bounds__(kBS) k_tile_apply(const int64_t* in, int64_t* out, int64_t n,
+                                                    const int64_t* tile_off) {
+  __shared__ int64_t lds[kBS / 64];
+  const int64_t base = (int64_t)blockIdx.x * kTile + (int64_t)threadIdx.x * kIPT;
+  int64_t v[kIPT];
+  int64_t s = 0;
+#pragma unroll
+  for (int k = 0; k < kIPT; ++k) {
+    v[k] = (base + k < n) ? in[base + k] : 0;
+    s += v[k];
+  }
+  int64_t tot;
+  int64_t ex = block_excl_scan(s, lds, &tot) + tile_off[blockIdx.x];
+#pragma unroll
+  for (int k = 0; k < kIPT; ++k)
+    if (base + k < n) {
+      out[base + k] = ex;
+      ex += v[k];
+    }
+}
+
+__global__ void k_flags_to_counts(const uint8_t* f, int64_t n, int64_t* cnt) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) cnt[i] = f[i] ? 1 : 0;
+}
+
+__global__ void k_scatter_flags(const uint8_t* f, const int64_t* pos, int64_t n, int32_t* out) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n && f[i]) out[pos[i]] = (int32_t)i;
+}
+
+}  // namespace
+
+// out may alias in. Returns the total (synchronizes the stream to read it).
+int64_t exclusive_scan(ngpb_ctx* c, const int64_t* d_in, int64_t* d_out, int64_t n) {
+  if (n <= 0) return 0;
+  const int64_t ntiles = (n + kTile - 1) / kTile;
+  c->scan_tmp.reserve((size_t)ntiles + 2);
+  int64_t* tiles = c->scan_tmp.p;
+  int64_t* total = c->scan_tmp.p + ntiles;
+  hipLaunchKernelGGL(k_tile_sums, dim3((unsigned)ntiles), dim3(kBS), 0, c->stream, d_in, n, tiles);
+  hipLaunchKernelGGL(k_scan_single, dim3(1), dim3(kBS), 0, c->stream, tiles, ntiles, total);
+  hipLaunchKernelGGL(k_tile_apply, dim3((unsigned)ntiles), dim3(kBS), 0, c->stream, d_in, d_out, n, tiles);
+  NGPB_HIP(hipGetLastError());
+  int64_t h = 0;
+  NGPB_HIP(hipMemcpyAsync(&h, total, sizeof(int64_t), hipMemcpyDeviceToHost, c->stream));
+  NGPB_HIP(hipStreamSynchronize(c->stream));
+  return h;
+}
+
+int64_t compact_flags(ngpb_ctx* c, const uint8_t* d_flag, int64_t n, int32_t* d_out) {
+  if (n <= 0) return 0;
+  DBuf<int64_t> cnt;
+  cnt.resize((size_t)n);
+  hipLaunchKernelGGL(k_flags_to_counts, dim3(blocks_for(n, 256)), dim3(256), 0, c->stream, d_flag, n, cnt.p);
+  const int64_t total = exclusive_scan(c, cnt.p, cnt.p, n);
+  hipLaunchKernelGGL(k_scatter_flags, dim3(blocks_for(n, 256)), dim3(256), 0, c->stream, d_flag, cnt.p, n, d_out);
+  NGPB_HIP(hipGetLastError());
+  NGPB_HIP(hipStreamSynchronize(c->stream));
+  return total;
+}
+
+}  // namespace ngpb

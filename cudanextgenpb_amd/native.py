@@ -1,0 +1,285 @@
+"""ctypes binding of libngpb.so (the C ABI declared in include/ngpb.h).
+
+The library must be built in-tree (``python -m cudanextgenpb_amd.build``); there
+is no fallback: a missing library or a missing GPU raises.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+
+import numpy as np
+
+from . import _paths
+
+_P = ctypes.c_void_p
+_D = ctypes.c_double
+_I = ctypes.c_int
+_I64 = ctypes.c_int64
+
+NGPB_OK = 0
+PREC_FP64 = 0
+PREC_MIXED = 1
+
+STATUS_NAMES = {0: "OK", 1: "E_ARG", 2: "E_STATE", 3: "E_HIP", 4: "E_NOCONV", 5: "E_NUMERIC",
+                6: "E_NOMEM", 7: "E_NODEV"}
+
+
+class NGPBError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"{STATUS_NAMES.get(code, code)}: {msg}")
+        self.code = code
+
+
+class Physics(ctypes.Structure):
+    _fields_ = [("e_in", _D), ("e_out", _D), ("T", _D), ("ionic_strength", _D), ("calc_energy", _I)]
+
+
+class SolveOpts(ctypes.Structure):
+    _fields_ = [("precision", _I), ("tol", _D), ("maxiter", _I), ("drop", _D), ("check_every", _I)]
+
+
+class SolveStats(ctypes.Structure):
+    _fields_ = [("status", _I), ("iterations", _I), ("replacements", _I), ("relres", _D), ("ms_solve", _D),
+                ("bytes_per_iter", _D)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+class EnergyOut(ctypes.Structure):
+    _fields_ = [("energy_pol", _D), ("energy_react", _D), ("charge_pol", _D), ("n_border", _I64),
+                ("n_flux_edges", _I64), ("n_triangles", _I64)]
+
+    def as_dict(self):
+        d = {k: getattr(self, k) for k, _ in self._fields_}
+        d["flux_charge"] = self.charge_pol / (4.0 * np.pi)
+        d["polar"] = self.energy_pol + self.energy_react
+        return d
+
+
+# name -> (restype, argtypes)
+SIGNATURES = {
+    "ngpb_version": (ctypes.c_char_p, []),
+    "ngpb_device_count": (_I, [_P]),
+    "ngpb_create": (_I, [_P, _I]),
+    "ngpb_destroy": (_I, [_P]),
+    "ngpb_last_error": (ctypes.c_char_p, [_P]),
+    "ngpb_constants": (_I, [_P, _P]),
+    "ngpb_device_bytes": (_I, [_P, _P]),
+    "ngpb_set_physics": (_I, [_P, _P]),
+    "ngpb_set_mesh": (_I, [_P, _I64, _P, _I64, _P, _P, _P, _I64, _P]),
+    "ngpb_set_rays": (_I, [_P, _I, _I64, _P, _P, _P, _P]),
+    "ngpb_create_markers": (_I, [_P]),
+    "ngpb_set_markers": (_I, [_P, _P, _P, _P]),
+    "ngpb_get_markers": (_I, [_P, _P, _P, _P, _P, _P]),
+    "ngpb_get_fractions": (_I, [_P, _P]),
+    "ngpb_set_charges": (_I, [_P, _I64, _P, _P, _P]),
+    "ngpb_set_dirichlet": (_I, [_P, _I64, _P, _P]),
+    "ngpb_dirichlet_values": (_I, [_P, _I]),
+    "ngpb_assemble": (_I, [_P]),
+    "ngpb_system_size": (_I, [_P, _P, _P]),
+    "ngpb_export_csr": (_I, [_P, _P, _P, _P, _P]),
+    "ngpb_set_csr": (_I, [_P, _I64, _I64, _P, _P, _P, _P]),
+    "ngpb_parse_lis_options": (_I, [ctypes.c_char_p, _P]),
+    "ngpb_solve": (_I, [_P, _P, _P, _P]),
+    "ngpb_get_phi": (_I, [_P, _P]),
+    "ngpb_set_phi": (_I, [_P, _P]),
+    "ngpb_spmv": (_I, [_P, _I, _P, _P]),
+    "ngpb_energy": (_I, [_P, _P]),
+    "ngpb_time_spmv": (_I, [_P, _I, _I, _P, _P]),
+}
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        path = _paths.native_lib()
+        if not os.path.exists(path):
+            raise RuntimeError(f"HIP library missing: {path} (run python -m cudanextgenpb_amd.build)")
+        L = ctypes.CDLL(path)
+        for name, (res, args) in SIGNATURES.items():
+            f = getattr(L, name)
+            f.restype = res
+            f.argtypes = args
+        _lib = L
+    return _lib
+
+
+def _p(a):
+    return a.ctypes.data_as(_P) if a is not None else None
+
+
+def _c(a, dt):
+    return np.ascontiguousarray(a, dtype=dt)
+
+
+def device_count() -> int:
+    n = ctypes.c_int(0)
+    lib().ngpb_device_count(ctypes.byref(n))
+    return n.value
+
+
+def constants(physics) -> dict:
+    ph = Physics(physics.e_in, physics.e_out, physics.T, physics.ionic_strength, physics.calc_energy)
+    out = np.zeros(6)
+    lib().ngpb_constants(ctypes.byref(ph), _p(out))
+    return dict(eps_in=out[0], eps_out=out[1], k2=out[2], k=out[3], constant_pol=out[4], constant_react=out[5])
+
+
+def parse_lis_options(s: str) -> SolveOpts:
+    o = SolveOpts()
+    rc = lib().ngpb_parse_lis_options(s.encode(), ctypes.byref(o))
+    if rc != NGPB_OK:
+        raise NGPBError(rc, f"cannot parse solver options {s!r}")
+    return o
+
+
+class Context:
+    """One resident system on one GPU (wraps ngpb_ctx)."""
+
+    def __init__(self, device: int = 0):
+        self._h = ctypes.c_void_p()
+        rc = lib().ngpb_create(ctypes.byref(self._h), device)
+        if rc != NGPB_OK:
+            raise NGPBError(rc, f"ngpb_create(device={device}) failed")
+        self.n = None
+        self.nnz = None
+
+    # ------------------------------------------------------------------ plumbing
+    def _check(self, rc, what):
+        if rc != NGPB_OK:
+            msg = lib().ngpb_last_error(self._h)
+            raise NGPBError(rc, f"{what}: {msg.decode() if msg else ''}")
+
+    def close(self):
+        if self._h:
+            lib().ngpb_destroy(self._h)
+            self._h = ctypes.c_void_p()
+
+    def __del__(self):  # pragma: no cover
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def device_bytes(self) -> int:
+        b = ctypes.c_int64(0)
+        self._check(lib().ngpb_device_bytes(self._h, ctypes.byref(b)), "device_bytes")
+        return b.value
+
+    # ------------------------------------------------------------------ inputs
+    def set_physics(self, physics):
+        ph = Physics(physics.e_in, physics.e_out, physics.T, physics.ionic_strength, physics.calc_energy)
+        self._check(lib().ngpb_set_physics(self._h, ctypes.byref(ph)), "set_physics")
+
+    def set_mesh(self, mesh):
+        self._keep_mesh = (_c(mesh.node_xyz, np.float64), _c(mesh.hang_xyz, np.float64),
+                           _c(mesh.hang_np, np.int32), _c(mesh.hang_par, np.int32), _c(mesh.leaf_corner, np.int32))
+        nx, hx, hn, hp, lc = self._keep_mesh
+        self._check(lib().ngpb_set_mesh(self._h, mesh.n_nodes, _p(nx), mesh.n_hang, _p(hx), _p(hn), _p(hp),
+                                        mesh.n_leaves, _p(lc)), "set_mesh")
+
+    def set_rays(self, rays):
+        for axis, r in enumerate(rays):
+            key, ptr = _c(r.key, np.float64), _c(r.ptr, np.int64)
+            it, nm = _c(r.inters, np.float64), _c(r.normals, np.float64)
+            self._check(lib().ngpb_set_rays(self._h, axis, r.n, _p(key), _p(ptr), _p(it), _p(nm)), "set_rays")
+
+    def create_markers(self):
+        self._check(lib().ngpb_create_markers(self._h), "create_markers")
+
+    def set_markers(self, eps, reac, leaf_marker):
+        self._check(lib().ngpb_set_markers(self._h, _p(_c(eps, np.float64)), _p(_c(reac, np.float64)),
+                                           _p(_c(leaf_marker, np.uint8))), "set_markers")
+
+    def get_markers(self, n_nodes, n_leaves):
+        eps = np.zeros(n_nodes)
+        reac = np.zeros(n_nodes)
+        lm = np.zeros(n_leaves, np.uint8)
+        nb = ctypes.c_int64(0)
+        border = np.zeros(n_leaves, np.int32)
+        self._check(lib().ngpb_get_markers(self._h, _p(eps), _p(reac), _p(lm), ctypes.byref(nb), _p(border)),
+                    "get_markers")
+        return eps, reac, lm, border[: nb.value].copy()
+
+    def get_fractions(self, n_leaves):
+        f = np.zeros((n_leaves, 12))
+        self._check(lib().ngpb_get_fractions(self._h, _p(f)), "get_fractions")
+        return f
+
+    def set_charges(self, xyz, q, owner):
+        xyz, q, owner = _c(xyz, np.float64), _c(q, np.float64), _c(owner, np.int32)
+        self._check(lib().ngpb_set_charges(self._h, q.shape[0], _p(xyz), _p(q), _p(owner)), "set_charges")
+
+    def set_dirichlet(self, nodes, values=None):
+        nodes = _c(nodes, np.int32)
+        vals = None if values is None else _c(values, np.float64)
+        self._check(lib().ngpb_set_dirichlet(self._h, nodes.shape[0], _p(nodes), _p(vals)), "set_dirichlet")
+
+    def dirichlet_values(self, bc_type: int):
+        self._check(lib().ngpb_dirichlet_values(self._h, bc_type), "dirichlet_values")
+
+    # ------------------------------------------------------------------ assembly / solve
+    def assemble(self):
+        self._check(lib().ngpb_assemble(self._h), "assemble")
+        self._size()
+
+    def _size(self):
+        n, nnz = ctypes.c_int64(0), ctypes.c_int64(0)
+        self._check(lib().ngpb_system_size(self._h, ctypes.byref(n), ctypes.byref(nnz)), "system_size")
+        self.n, self.nnz = n.value, nnz.value
+
+    def export_csr(self):
+        rp = np.zeros(self.n + 1, np.int32)
+        col = np.zeros(self.nnz, np.int32)
+        val = np.zeros(self.nnz)
+        rhs = np.zeros(self.n)
+        self._check(lib().ngpb_export_csr(self._h, _p(rp), _p(col), _p(val), _p(rhs)), "export_csr")
+        return rp, col, val, rhs
+
+    def set_csr(self, rowptr, col, val, rhs):
+        rp, cl, vl, b = _c(rowptr, np.int32), _c(col, np.int32), _c(val, np.float64), _c(rhs, np.float64)
+        self._check(lib().ngpb_set_csr(self._h, b.shape[0], cl.shape[0], _p(rp), _p(cl), _p(vl), _p(b)), "set_csr")
+        self._size()
+
+    def solve(self, precision=PREC_FP64, tol=1e-10, maxiter=100000, drop=0.05, check_every=32, want_phi=True,
+              raise_on_noconv=True):
+        o = SolveOpts(precision, tol, maxiter, drop, check_every)
+        st = SolveStats()
+        phi = np.zeros(self.n) if want_phi else None
+        rc = lib().ngpb_solve(self._h, ctypes.byref(o), _p(phi), ctypes.byref(st))
+        if rc != NGPB_OK and not (rc == 4 and not raise_on_noconv):
+            self._check(rc, "solve")
+        return phi, st.as_dict()
+
+    def get_phi(self):
+        phi = np.zeros(self.n)
+        self._check(lib().ngpb_get_phi(self._h, _p(phi)), "get_phi")
+        return phi
+
+    def set_phi(self, phi):
+        self._check(lib().ngpb_set_phi(self._h, _p(_c(phi, np.float64))), "set_phi")
+
+    def spmv(self, x, precision=PREC_FP64):
+        y = np.zeros(self.n)
+        self._check(lib().ngpb_spmv(self._h, precision, _p(_c(x, np.float64)), _p(y)), "spmv")
+        return y
+
+    def energy(self):
+        e = EnergyOut()
+        self._check(lib().ngpb_energy(self._h, ctypes.byref(e)), "energy")
+        return e.as_dict()
+
+    def time_spmv(self, precision=PREC_FP64, reps=50):
+        ms, b = ctypes.c_double(0), ctypes.c_double(0)
+        self._check(lib().ngpb_time_spmv(self._h, precision, reps, ctypes.byref(ms), ctypes.byref(b)), "time_spmv")
+        return ms.value, b.value

@@ -1,0 +1,174 @@
+/*
+ * ngpb.h — C ABI of the MI355X (gfx950) linearized Poisson-Boltzmann hot path.
+ *
+ * One opaque context per system: it owns resident device state (mesh, ray table,
+ * markers, assembled operator, solver vectors) and one HIP stream. Inputs are
+ * caller-owned host arrays that are copied to the device; outputs are copied into
+ * caller-owned host arrays. A context is not thread-safe. Every function returns
+ * an NGPB_* status code; ngpb_last_error() describes the most recent failure.
+ *
+ * The entry points replace these reference interfaces (NextGenPB v1.0.0):
+ *   ngpb_set_physics      model keys parsed at src/pb_class.cpp:940-945
+ *   ngpb_set_mesh         the tmesh_3d quadrant sweep: p(d,i), gt(i), is_hanging(i),
+ *                         num_parents(i), gparent(j,i) (src/pb_class.cpp:1838-1852, 2024-2036)
+ *   ngpb_set_rays         ray_cache_t::rays, one map per axis (include/raytracer.h:40-54)
+ *   ngpb_create_markers   poisson_boltzmann::create_markers (include/pb_class.h:716-717,
+ *                         src/pb_class.cpp:1792-1941) with is_in_ns_surf (:768-805)
+ *   ngpb_set_markers      the members create_markers fills (include/pb_class.h:168-188)
+ *   ngpb_set_charges      pos_atoms/charge_atoms + lookup_table from search_points
+ *                         (src/pb_class.cpp:3633-3729)
+ *   ngpb_set_dirichlet    bcells / dirichlet_bcs3 (src/pb_class.cpp:2108-2149)
+ *   ngpb_dirichlet_values bc_type 1/2/3 data (src/pb_class.cpp:2121-2149, 3530-3571)
+ *   ngpb_assemble         create_density_map + assemple_system_matrix
+ *                         (include/pb_class.h:731-735, src/pb_class.cpp:1943-2159)
+ *   ngpb_export_csr       distributed_sparse_matrix::csr as used at src/pb_class.cpp:2255
+ *   ngpb_set_csr          lis_matrix_set_csr + lis_vector_set_value (src/pb_class.cpp:2263-2304)
+ *   ngpb_parse_lis_options lis_solver_set_option (src/pb_class.cpp:2311-2313)
+ *   ngpb_solve            lis_compute_electric_potential / lis_solve
+ *                         (include/pb_class.h:740-741, src/pb_class.cpp:2243-2329)
+ *   ngpb_energy           poisson_boltzmann::energy / energy_fast
+ *                         (include/pb_class.h:758-762, src/pb_class.cpp:2790-3338)
+ */
+#ifndef NGPB_H
+#define NGPB_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct ngpb_ctx ngpb_ctx;
+
+enum ngpb_status_code {
+  NGPB_OK = 0,
+  NGPB_E_ARG = 1,      /* invalid argument or inconsistent sizes */
+  NGPB_E_STATE = 2,    /* a required earlier step has not been done */
+  NGPB_E_HIP = 3,      /* HIP runtime error */
+  NGPB_E_NOCONV = 4,   /* Krylov solver reached maxiter */
+  NGPB_E_NUMERIC = 5,  /* NaN/Inf or non-positive diagonal in the operator */
+  NGPB_E_NOMEM = 6,    /* device or host allocation failed */
+  NGPB_E_NODEV = 7     /* no HIP device */
+};
+
+enum ngpb_precision {
+  NGPB_PREC_FP64 = 0,  /* plain method: FP64 CSR, FP64 vectors, Jacobi PCG (LIS -i cg -p jacobi) */
+  NGPB_PREC_MIXED = 1  /* FP32 compact operator + FP32 inner CG, FP64 residual/reductions/update */
+};
+
+/* Model parameters (relative permittivities, temperature in K, ionic strength in M). */
+typedef struct {
+  double e_in;
+  double e_out;
+  double T;
+  double ionic_strength;
+  int calc_energy; /* 0 none, 1 polarization only, 2 polarization + direct ionic */
+} ngpb_physics;
+
+typedef struct {
+  int precision;     /* enum ngpb_precision */
+  double tol;        /* stop when ||b - A x||_2 <= tol * ||b||_2 (x0 = 0) */
+  int maxiter;       /* limit on matrix-vector products (FP64) or inner iterations (mixed) */
+  double drop;       /* mixed: FP64 residual replacement once the inner residual has
+                        dropped by this factor since the last replacement (e.g. 0.05) */
+  int check_every;   /* iterations launched between host polls of the stop flag (>= 1) */
+} ngpb_solve_opts;
+
+typedef struct {
+  int status;          /* NGPB_OK or NGPB_E_NOCONV */
+  int iterations;      /* Krylov iterations (matrix-vector products of the iteration) */
+  int replacements;    /* mixed: FP64 residual replacements */
+  double relres;       /* ||b - A x|| / ||b|| : recursive (FP64) or true FP64 (mixed) */
+  double ms_solve;     /* wall time of the solve on the device stream */
+  double bytes_per_iter;  /* algorithmic HBM bytes of one iteration */
+} ngpb_solve_stats;
+
+typedef struct {
+  double energy_pol;    /* polarization energy [kT] */
+  double energy_react;  /* direct ionic energy [kT] (calc_energy = 2, kappa > 0) */
+  double charge_pol;    /* sum of fluxes; flux charge = charge_pol / (4 pi) [e] */
+  int64_t n_border;     /* border leaves (marker 0.5) */
+  int64_t n_flux_edges; /* edges with eps(i1) != eps(i2) on border leaves */
+  int64_t n_triangles;  /* marching-cubes triangles (ionic term) */
+} ngpb_energy_out;
+
+/* ---- library / context ------------------------------------------------------- */
+const char* ngpb_version(void);
+int ngpb_device_count(int* count);
+int ngpb_create(ngpb_ctx** ctx, int device);
+int ngpb_destroy(ngpb_ctx* ctx);
+const char* ngpb_last_error(const ngpb_ctx* ctx);
+/* out[0..5] = eps_in_hat, eps_out_hat, k2 [A^-2], k [A^-1], constant_pol, constant_react
+   (include/pb_class.h:47-54, src/pb_class.cpp:1800-1806, 2803-2814). Host only. */
+int ngpb_constants(const ngpb_physics* physics, double* out6);
+/* Bytes of device memory held by the context. */
+int ngpb_device_bytes(const ngpb_ctx* ctx, int64_t* bytes);
+
+/* ---- inputs -------------------------------------------------------------------- */
+int ngpb_set_physics(ngpb_ctx* ctx, const ngpb_physics* physics);
+
+/* Leaves with 8 corners in p4est order (bit0 = x, bit1 = y, bit2 = z). leaf_corner[8l+i]
+   >= 0 is a node id (gt), < 0 is hanging node -1-c whose value is the mean of its
+   hang_np[k] (2 or 4) parents hang_par[4k+j]. Coordinates are the p(d,i) values. */
+int ngpb_set_mesh(ngpb_ctx* ctx, int64_t n_nodes, const double* node_xyz, int64_t n_hang,
+                  const double* hang_xyz, const int32_t* hang_np, const int32_t* hang_par,
+                  int64_t n_leaves, const int32_t* leaf_corner);
+
+/* Rays of one axis: key[2r], key[2r+1] are the two other coordinates in increasing
+   axis order ((y,z) for x rays, (x,z) for y rays, (x,y) for z rays), keys strictly
+   increasing lexicographically; crossings inters[ptr[r] .. ptr[r+1]) increasing,
+   normals 3 per crossing. Lines without crossings may be omitted. */
+int ngpb_set_rays(ngpb_ctx* ctx, int axis, int64_t n_rays, const double* key, const int64_t* ptr,
+                  const double* inters, const double* normals);
+
+/* In/out markers on the device from the z rays. */
+int ngpb_create_markers(ngpb_ctx* ctx);
+/* Caller-computed markers (leaf_marker: 0 inside, 1 border, 2 outside). */
+int ngpb_set_markers(ngpb_ctx* ctx, const double* eps_node, const double* reac_node,
+                     const uint8_t* leaf_marker);
+/* Any pointer may be NULL. border receives n_border leaf ids in leaf order. */
+int ngpb_get_markers(ngpb_ctx* ctx, double* eps_node, double* reac_node, uint8_t* leaf_marker,
+                     int64_t* n_border, int32_t* border);
+/* 12 edge fractions per leaf (0.5 for non-border leaves), computed by ngpb_assemble. */
+int ngpb_get_fractions(ngpb_ctx* ctx, double* frac);
+
+/* Atoms with their owner leaf (-1: not in the domain). */
+int ngpb_set_charges(ngpb_ctx* ctx, int64_t n_atoms, const double* xyz, const double* q,
+                     const int32_t* owner_leaf);
+/* Dirichlet node list; values may be NULL (then ngpb_dirichlet_values must follow). */
+int ngpb_set_dirichlet(ngpb_ctx* ctx, int64_t n_bnd, const int32_t* nodes, const double* values);
+/* Device-computed Dirichlet data: 1 zero, 2 Coulomb sum over the charges, 3 Born sphere. */
+int ngpb_dirichlet_values(ngpb_ctx* ctx, int bc_type);
+
+/* ---- assembly / solve / energy ---------------------------------------------------- */
+/* Edge fractions, Laplacian with harmonic-mean face coefficients, kappa^2 mass,
+   hanging-node constraints, trilinear charge projection and Dirichlet elimination. */
+int ngpb_assemble(ngpb_ctx* ctx);
+int ngpb_system_size(ngpb_ctx* ctx, int64_t* n, int64_t* nnz);
+int ngpb_export_csr(ngpb_ctx* ctx, int32_t* rowptr, int32_t* col, double* val, double* rhs);
+/* Load an external CSR system (0-based, columns ascending, n+1 row pointers). */
+int ngpb_set_csr(ngpb_ctx* ctx, int64_t n, int64_t nnz, const int32_t* rowptr, const int32_t* col,
+                 const double* val, const double* rhs);
+/* LIS-style option string, e.g. "-i cg -p jacobi -tol 1e-10 -maxiter 100000". */
+int ngpb_parse_lis_options(const char* options, ngpb_solve_opts* out);
+int ngpb_solve(ngpb_ctx* ctx, const ngpb_solve_opts* opts, double* phi_out, ngpb_solve_stats* stats);
+int ngpb_get_phi(ngpb_ctx* ctx, double* phi);
+int ngpb_set_phi(ngpb_ctx* ctx, const double* phi);
+/* y = A x with the resident operator: precision FP64 (CSR) or MIXED (scaled FP32 form,
+   y = D^1/2 Ahat D^1/2 x evaluated in the compact format). */
+int ngpb_spmv(ngpb_ctx* ctx, int precision, const double* x, double* y);
+int ngpb_energy(ngpb_ctx* ctx, ngpb_energy_out* out);
+
+/* ---- measurement ----------------------------------------------------------------- */
+/* Times `reps` back-to-back launches of the SpMV kernel of the given precision on the
+   context's stream with HIP events; returns the mean kernel time in ms and the
+   algorithmic bytes of one launch. */
+int ngpb_time_spmv(ngpb_ctx* ctx, int precision, int reps, double* ms_per_launch,
+                   double* bytes_per_launch);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* NGPB_H */

@@ -1,0 +1,721 @@
+// ORACLE — test infrastructure only.
+//
+// CPU restatement of NextGenPB's linearized Poisson-Boltzmann hot path, used as the
+// checker for the HIP implementation. Only the test suite, the smoke check and the
+// CPU-baseline leg of the benchmark load this library; the product path never does.
+//
+// Every function follows the reference's sequential semantics (one MPI rank):
+//   orc_constants      include/pb_class.h:47-54, src/pb_class.cpp:1800-1806, 2803-2814
+//   orc_markers        src/pb_class.cpp:768-805 (is_in_ns_surf), 1792-1941 (create_markers)
+//   orc_fractions      src/pb_class.cpp:2422-2532 (cube_fraction_intersection)
+//   orc_locate         src/pb_class.cpp:3633-3729 (controlla_coordinate, cerca_atomo, search_points)
+//   orc_rhs            src/pb_class.cpp:1943-2050 (create_density_map) + bim3a_rhs (bimpp)
+//   orc_assemble       src/pb_class.cpp:2052-2159 + bimpp bim3a_laplacian_frac /
+//                      bim3a_reaction_frac / bim3a_dirichlet_bc (conventions below)
+//   orc_pcg            src/pb_class.cpp:2243-2329 + LIS 2.1.6 CG with Jacobi (-i cg -p jacobi)
+//   orc_energy         src/pb_class.cpp:2534-2788 (helpers), 2790-3063 (energy)
+//
+// Pinning (how each is checked without the reference's libraries, which are absent):
+//   constants  closed forms; markers: brute-force distance test on sphere unions +
+//   exhaustive rules; fractions: closed form t = c +- sqrt(r^2-d^2); locate: brute
+//   force; rhs: weights sum to one, sum rhs = 4 pi sum q; assembly: uniform 7-point
+//   closed form, symmetry, zero interior row sums at kappa=0, exact reproduction of
+//   linear fields; pcg: dense direct solves of small systems; energy: Born-sphere
+//   analytic energy and the Gauss flux-charge check.
+// The bimpp conventions (hanging-node algebra, reaction lumping, Dirichlet
+// elimination) cannot be pinned against bimpp itself, which is not available:
+// "parity unpinned" against bimpp; pinned by the invariants above.
+
+#include <algorithm>
+#include <array>
+#include <cmath>
+#include <cstdint>
+#include <cstring>
+#include <map>
+#include <memory>
+#include <vector>
+
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+#include "../cudanextgenpb_amd/csrc/mc_table.inc"
+
+namespace {
+
+constexpr double kE0 = 8.85418781762e-12;  // F/m
+constexpr double kKb = 1.380649e-23;       // J/K
+constexpr double kE = 1.602176634e-19;     // C
+constexpr double kNav = 6.022e23;          // 1/mol
+constexpr double kAngs = 1e-10;            // m
+constexpr double kPi = 3.14159265358979323846;
+
+// corner pairs of the 12 cube edges (include/pb_class.h:193-207) and their axes (:190-191)
+constexpr int kEdgeNodes[24] = {0, 1, 1, 3, 2, 3, 0, 2, 4, 5, 5, 7, 6, 7, 4, 6, 0, 4, 1, 5, 3, 7, 2, 6};
+constexpr int kEdgeAxis[12] = {0, 1, 0, 1, 0, 1, 0, 1, 2, 2, 2, 2};
+
+}  // namespace
+
+extern "C" {
+
+struct orc_mesh {
+  int64_t n_nodes, n_hang, n_leaves;
+  const double* node_xyz;
+  const double* hang_xyz;
+  const int32_t* hang_np;
+  const int32_t* hang_par;
+  const int32_t* leaf_corner;
+};
+
+struct orc_rays {
+  int64_t n_rays;
+  const double* key;
+  const int64_t* ptr;
+  const double* inters;
+  const double* normals;
+};
+
+// out[0..5] = eps_in_hat, eps_out_hat, k2, k, constant_pol, constant_react
+void orc_constants(double e_in, double e_out, double T, double I, double* out) {
+  const double eps_in = 4.0 * kPi * kE0 * e_in * kKb * T * kAngs / (kE * kE);
+  const double eps_out = 4.0 * kPi * kE0 * e_out * kKb * T * kAngs / (kE * kE);
+  const double C0 = 1.0e3 * kNav * I;
+  const double k2 = 2.0 * C0 * kAngs * kAngs * kE * kE / (kE0 * e_out * kKb * T);
+  const double inv_4pi = 1.0 / (4.0 * kPi);
+  out[0] = eps_in;
+  out[1] = eps_out;
+  out[2] = k2;
+  out[3] = std::sqrt(k2);
+  out[4] = (1.0 / eps_out - 1.0 / eps_in) * inv_4pi;
+  out[5] = (1.0 / eps_out) * inv_4pi;
+}
+
+}  // extern "C"
+
+namespace {
+
+// ray cache restated as one ordered map per axis keyed by the two transverse
+// coordinates (src/raytracer.cpp:22-63; exact key comparison)
+struct Ray {
+  const double* inters;
+  const double* normals;
+  int64_t n;
+};
+using RayMap = std::map<std::array<double, 2>, Ray>;
+
+std::array<RayMap, 3> build_maps(const orc_rays* rays) {
+  std::array<RayMap, 3> m;
+  for (int d = 0; d < 3; ++d) {
+    const orc_rays& r = rays[d];
+    for (int64_t i = 0; i < r.n_rays; ++i) {
+      const int64_t b = r.ptr[i], e = r.ptr[i + 1];
+      m[d][{r.key[2 * i], r.key[2 * i + 1]}] = Ray{r.inters + b, r.normals + 3 * b, e - b};
+    }
+  }
+  return m;
+}
+
+struct MeshView {
+  const orc_mesh* m;
+  bool hanging(int64_t l, int i) const { return m->leaf_corner[8 * l + i] < 0; }
+  int32_t gt(int64_t l, int i) const { return m->leaf_corner[8 * l + i]; }
+  int32_t hid(int64_t l, int i) const { return -1 - m->leaf_corner[8 * l + i]; }
+  double p(int64_t l, int d, int i) const {
+    const int32_t c = m->leaf_corner[8 * l + i];
+    return c >= 0 ? m->node_xyz[3 * (int64_t)c + d] : m->hang_xyz[3 * (int64_t)(-1 - c) + d];
+  }
+  int num_parents(int64_t l, int i) const { return m->hang_np[hid(l, i)]; }
+  int32_t gparent(int64_t l, int j, int i) const { return m->hang_par[4 * (int64_t)hid(l, i) + j]; }
+};
+
+// is_in_ns_surf (src/pb_class.cpp:768-805): parity of #{crossings <= x3} along the
+// axis-`dir` ray through the point; no ray or no crossing -> outside.
+int is_in_surf(const std::array<RayMap, 3>& maps, double x, double y, double z, int dir) {
+  double x1 = x, x2 = y, x3 = z;
+  if (dir == 0) { x1 = y; x2 = z; x3 = x; }
+  else if (dir == 1) { x1 = x; x2 = z; x3 = y; }
+  auto it = maps[dir].find({x1, x2});
+  if (it == maps[dir].end()) return 0;
+  const Ray& ct = it->second;
+  int64_t i = 0;
+  if (ct.n == 0 || x3 < ct.inters[0]) return 0;
+  while (i < ct.n && x3 >= ct.inters[i]) ++i;
+  return (int)(i % 2);
+}
+
+// normal_intersection (src/pb_class.cpp:2534-2568): last crossing t with x1 <= t <= x2
+// on the axis ray through corner i1 of edge `edge`; frac defaults to 0.5 and the
+// normal is only written when a crossing is found. Returns whether one was found.
+bool normal_intersection(const MeshView& mv, const std::array<RayMap, 3>& maps, int64_t l,
+                         int edge, double* norm, double* frac) {
+  const int dir = kEdgeAxis[edge];
+  const int i1 = kEdgeNodes[2 * edge], i2 = kEdgeNodes[2 * edge + 1];
+  const double x1 = mv.p(l, dir, i1), x2 = mv.p(l, dir, i2);
+  std::array<double, 2> ray;
+  int k = 0;
+  for (int d = 0; d < 3; ++d)
+    if (d != dir) ray[k++] = mv.p(l, d, i1);
+  *frac = 0.5;
+  auto it = maps[dir].find(ray);
+  if (it == maps[dir].end()) return false;
+  bool found = false;
+  for (int64_t ii = 0; ii < it->second.n; ++ii) {
+    const double t = it->second.inters[ii];
+    if (t >= x1 && t <= x2) {
+      norm[0] = it->second.normals[3 * ii + 0];
+      norm[1] = it->second.normals[3 * ii + 1];
+      norm[2] = it->second.normals[3 * ii + 2];
+      *frac = (t - x1) / (x2 - x1);
+      found = true;
+    }
+  }
+  return found;
+}
+
+double wha(double eps1, double eps2, double frac) { return 1.0 / (frac / eps1 + (1 - frac) / eps2); }
+
+double phi0(double eps1, double eps2, double phi1, double phi2, double frac) {
+  return phi1 + frac * eps2 * (phi2 - phi1) / (eps2 * frac + eps1 * (1 - frac));
+}
+
+double area_triangle(const double t[3][3]) {
+  double ab[3], ac[3];
+  for (int i = 0; i < 3; ++i) {
+    ab[i] = t[0][i] - t[1][i];
+    ac[i] = t[0][i] - t[2][i];
+  }
+  return 0.5 * std::hypot(ab[1] * ac[2] - ab[2] * ac[1], ab[0] * ac[2] - ab[2] * ac[0],
+                          ab[1] * ac[0] - ab[0] * ac[1]);
+}
+
+int mc_digit(char c) { return c <= '9' ? c - '0' : c - 'a' + 10; }
+
+}  // namespace
+
+extern "C" {
+
+// create_markers (src/pb_class.cpp:1792-1941), one rank, one cycle, no Stern layer.
+// leaf_marker: 0 = inside (0.0), 1 = border (0.5), 2 = outside (1.0).
+int orc_markers(const orc_mesh* m, const orc_rays* rays, double eps_in, double eps_out,
+                double k2, double* eps_node, double* reac_node, uint8_t* leaf_marker,
+                int32_t* border, int64_t* n_border) {
+  const auto maps = build_maps(rays);
+  MeshView mv{m};
+  for (int64_t i = 0; i < m->n_nodes; ++i) {
+    eps_node[i] = eps_out;
+    reac_node[i] = eps_out * k2;
+  }
+  int64_t nb = 0;
+  for (int64_t l = 0; l < m->n_leaves; ++l) {
+    int num_int = 0, num_hang = 0;
+    for (int ii = 0; ii < 8; ++ii) {
+      const double x = mv.p(l, 0, ii), y = mv.p(l, 1, ii), z = mv.p(l, 2, ii);
+      if (!mv.hanging(l, ii)) {
+        if (is_in_surf(maps, x, y, z, 2) > 0) {
+          ++num_int;
+          eps_node[mv.gt(l, ii)] = eps_in;
+          reac_node[mv.gt(l, ii)] = 0.0;
+        }
+      } else {
+        ++num_hang;
+      }
+    }
+    uint8_t mk = 0;
+    if (num_int == 0) mk = 2;
+    else if (num_int < 8 - num_hang) { mk = 1; border[nb++] = (int32_t)l; }
+    leaf_marker[l] = mk;
+  }
+  *n_border = nb;
+  return 0;
+}
+
+// cube_fraction_intersection (src/pb_class.cpp:2422-2532) for every leaf: 0.5 unless
+// the leaf is a border leaf and a crossing lies on the edge. found[] reports whether
+// a crossing was found (the reference does not keep this; it drives the normal).
+int orc_fractions(const orc_mesh* m, const orc_rays* rays, const uint8_t* leaf_marker,
+                  double* frac, double* normal, uint8_t* found) {
+  const auto maps = build_maps(rays);
+  MeshView mv{m};
+  for (int64_t l = 0; l < m->n_leaves; ++l) {
+    for (int e = 0; e < 12; ++e) {
+      frac[12 * l + e] = 0.5;
+      if (found) found[12 * l + e] = 0;
+      if (normal) for (int d = 0; d < 3; ++d) normal[36 * l + 3 * e + d] = 0.0;
+      if (leaf_marker[l] != 1) continue;
+      double nrm[3] = {0, 0, 0}, f = 0.5;
+      const bool fd = normal_intersection(mv, maps, l, e, nrm, &f);
+      frac[12 * l + e] = f;
+      if (found) found[12 * l + e] = fd ? 1 : 0;
+      if (normal && fd) for (int d = 0; d < 3; ++d) normal[36 * l + 3 * e + d] = nrm[d];
+    }
+  }
+  return 0;
+}
+
+// search_points / cerca_atomo / controlla_coordinate (src/pb_class.cpp:3633-3729):
+// the first leaf (in leaf order) with l - tol < x <= r - tol on every axis, where l
+// and r are the coordinates of corners 0 and 7; -1 when no leaf qualifies.
+int orc_locate(const orc_mesh* m, int64_t na, const double* xyz, double tol, int32_t* owner) {
+  MeshView mv{m};
+  for (int64_t a = 0; a < na; ++a) {
+    owner[a] = -1;
+    for (int64_t l = 0; l < m->n_leaves; ++l) {
+      bool in = true;
+      for (int d = 0; d < 3 && in; ++d) {
+        const double lo = mv.p(l, d, 0), hi = mv.p(l, d, 7), x = xyz[3 * a + d];
+        in = (x > lo - tol) && (x <= hi - tol);
+      }
+      if (in) { owner[a] = (int32_t)l; break; }
+    }
+  }
+  return 0;
+}
+
+// create_density_map (src/pb_class.cpp:1943-2050) followed by bim3a_rhs(const_ones,
+// rho) (src/pb_class.cpp:2085). bim3a_rhs assembles sum over leaves of
+// f_leaf * g(node) * V/8 per corner, a hanging corner contributing V/(8 np) to each
+// parent with the parent's value; vol_patch is the same sum with g = 1.
+int orc_rhs(const orc_mesh* m, int64_t na, const double* xyz, const double* q,
+            const int32_t* owner, double* rhs) {
+  MeshView mv{m};
+  const int64_t N = m->n_nodes;
+  std::vector<double> vol_patch(N, 0.0), rho(N, 0.0);
+  auto vol8 = [&](int64_t l) {
+    return (mv.p(l, 0, 7) - mv.p(l, 0, 0)) * (mv.p(l, 1, 7) - mv.p(l, 1, 0)) *
+           (mv.p(l, 2, 7) - mv.p(l, 2, 0)) / 8.0;
+  };
+  for (int64_t l = 0; l < m->n_leaves; ++l) {
+    const double v8 = vol8(l);
+    for (int ii = 0; ii < 8; ++ii) {
+      if (!mv.hanging(l, ii)) vol_patch[mv.gt(l, ii)] += v8;
+      else
+        for (int jj = 0; jj < mv.num_parents(l, ii); ++jj)
+          vol_patch[mv.gparent(l, jj, ii)] += v8 / mv.num_parents(l, ii);
+    }
+  }
+  // lookup_table is a std::map<int, quadrant>: atoms in increasing index order
+  for (int64_t a = 0; a < na; ++a) {
+    const int64_t l = owner[a];
+    if (l < 0) continue;
+    const double volume = (mv.p(l, 0, 7) - mv.p(l, 0, 0)) * (mv.p(l, 1, 7) - mv.p(l, 1, 0)) *
+                          (mv.p(l, 2, 7) - mv.p(l, 2, 0));
+    for (int ii = 0; ii < 8; ++ii) {
+      const double weight = std::abs((xyz[3 * a + 0] - mv.p(l, 0, 7 - ii)) *
+                                     (xyz[3 * a + 1] - mv.p(l, 1, 7 - ii)) *
+                                     (xyz[3 * a + 2] - mv.p(l, 2, 7 - ii))) / volume;
+      if (!mv.hanging(l, ii)) {
+        const int32_t g = mv.gt(l, ii);
+        rho[g] += q[a] * 4.0 * kPi * weight / vol_patch[g];
+      } else {
+        for (int jj = 0; jj < mv.num_parents(l, ii); ++jj) {
+          const int32_t g = mv.gparent(l, jj, ii);
+          const double denom = mv.num_parents(l, ii) * vol_patch[g];
+          rho[g] += q[a] * 4.0 * kPi * weight / denom;
+        }
+      }
+    }
+  }
+  for (int64_t i = 0; i < N; ++i) rhs[i] = 0.0;
+  for (int64_t l = 0; l < m->n_leaves; ++l) {
+    const double v8 = vol8(l);
+    for (int ii = 0; ii < 8; ++ii) {
+      if (!mv.hanging(l, ii)) { const int32_t g = mv.gt(l, ii); rhs[g] += v8 * rho[g]; }
+      else
+        for (int jj = 0; jj < mv.num_parents(l, ii); ++jj) {
+          const int32_t g = mv.gparent(l, jj, ii);
+          rhs[g] += v8 / mv.num_parents(l, ii) * rho[g];
+        }
+    }
+  }
+  return 0;
+}
+
+// ---------------------------------------------------------------- assembly
+struct orc_sys {
+  int64_t n = 0;
+  std::vector<int32_t> rowptr, col;
+  std::vector<double> val, rhs;
+};
+
+// assemple_system_matrix (src/pb_class.cpp:2052-2159) with map-based rows as in
+// bimpp's sparse matrix. Conventions (bimpp's source is absent; these are this
+// project's statement of them):
+//  * laplacian_frac: per leaf and per edge (i1, i2) along axis a, conductance
+//    c = wha(eps(i1), eps(i2), f) * h_b * h_c / (4 h_a), with f the edge fraction
+//    measured from i1 and eps at a hanging corner the mean of its parents
+//    (sum of eps[parent]/np in parent order, as src/pb_class.cpp:2635-2647). The
+//    leaf matrix c*[[1,-1],[-1,1]] on (i1, i2) is scattered as P^T A P, a hanging
+//    corner's row/column spread to its parents with weight 1/np. Within an edge
+//    the entries are visited (i1,i1), (i1,i2), (i2,i1), (i2,i2), each over parent
+//    pairs in parent order.
+//  * reaction_frac: lumped mass reac(node) * V/8 on the diagonal per corner; a
+//    hanging corner adds reac(parent) * V/(8 np) to each parent's diagonal.
+//  * dirichlet: symmetric elimination. A Dirichlet row keeps only its assembled
+//    diagonal d with rhs = d * g; every other row moves a_ij * g_j to its rhs and
+//    drops column j. phi is the same as with any other elimination.
+orc_sys* orc_assemble(const orc_mesh* m, const double* eps_node, const double* reac_node,
+                      const double* frac, int64_t nb, const int32_t* bnd_nodes,
+                      const double* bnd_vals, const double* rhs_charge) {
+  MeshView mv{m};
+  const int64_t N = m->n_nodes;
+  std::vector<std::map<int32_t, double>> A(N);
+  for (int64_t l = 0; l < m->n_leaves; ++l) {
+    double h[3];
+    for (int d = 0; d < 3; ++d) h[d] = mv.p(l, d, 7) - mv.p(l, d, 0);
+    double eps[8];
+    int np[8];
+    int32_t par[8][4];
+    for (int ii = 0; ii < 8; ++ii) {
+      if (!mv.hanging(l, ii)) {
+        eps[ii] = eps_node[mv.gt(l, ii)];
+        np[ii] = 1;
+        par[ii][0] = mv.gt(l, ii);
+      } else {
+        np[ii] = mv.num_parents(l, ii);
+        eps[ii] = 0.0;
+        for (int jj = 0; jj < np[ii]; ++jj) {
+          par[ii][jj] = mv.gparent(l, jj, ii);
+          eps[ii] += eps_node[par[ii][jj]] / np[ii];
+        }
+      }
+    }
+    for (int e = 0; e < 12; ++e) {
+      const int a = kEdgeAxis[e];
+      const int b = (a + 1) % 3, c = (a + 2) % 3;
+      const int i1 = kEdgeNodes[2 * e], i2 = kEdgeNodes[2 * e + 1];
+      const double cond = wha(eps[i1], eps[i2], frac[12 * l + e]) * (h[b] * h[c] / (4.0 * h[a]));
+      const int loc[2] = {i1, i2};
+      for (int r = 0; r < 2; ++r)
+        for (int s = 0; s < 2; ++s) {
+          const double sgn = (r == s) ? 1.0 : -1.0;
+          const int ir = loc[r], is = loc[s];
+          for (int pr = 0; pr < np[ir]; ++pr)
+            for (int ps = 0; ps < np[is]; ++ps) {
+              const double w = sgn / (double)(np[ir] * np[is]);
+              A[par[ir][pr]][par[is][ps]] += cond * w;
+            }
+        }
+    }
+  }
+  for (int64_t l = 0; l < m->n_leaves; ++l) {
+    const double v8 = (mv.p(l, 0, 7) - mv.p(l, 0, 0)) * (mv.p(l, 1, 7) - mv.p(l, 1, 0)) *
+                      (mv.p(l, 2, 7) - mv.p(l, 2, 0)) * 0.125;
+    for (int ii = 0; ii < 8; ++ii) {
+      if (!mv.hanging(l, ii)) {
+        const int32_t g = mv.gt(l, ii);
+        A[g][g] += reac_node[g] * v8;
+      } else {
+        const int npi = mv.num_parents(l, ii);
+        for (int jj = 0; jj < npi; ++jj) {
+          const int32_t g = mv.gparent(l, jj, ii);
+          A[g][g] += reac_node[g] * v8 * (1.0 / npi);
+        }
+      }
+    }
+  }
+  std::vector<char> isb(N, 0);
+  std::vector<double> g(N, 0.0);
+  for (int64_t k = 0; k < nb; ++k) { isb[bnd_nodes[k]] = 1; g[bnd_nodes[k]] = bnd_vals[k]; }
+  auto S = std::make_unique<orc_sys>();
+  S->n = N;
+  S->rowptr.assign(N + 1, 0);
+  S->rhs.assign(rhs_charge, rhs_charge + N);
+  for (int64_t i = 0; i < N; ++i) {
+    if (isb[i]) {
+      const double d = A[i][(int32_t)i];
+      S->col.push_back((int32_t)i);
+      S->val.push_back(d);
+      S->rhs[i] = d * g[i];
+    } else {
+      for (const auto& kv : A[i]) {
+        if (isb[kv.first]) { S->rhs[i] -= kv.second * g[kv.first]; continue; }
+        S->col.push_back(kv.first);
+        S->val.push_back(kv.second);
+      }
+    }
+    S->rowptr[i + 1] = (int32_t)S->col.size();
+  }
+  return S.release();
+}
+
+void orc_sys_sizes(const orc_sys* s, int64_t* n, int64_t* nnz) {
+  *n = s->n;
+  *nnz = (int64_t)s->col.size();
+}
+void orc_sys_copy(const orc_sys* s, int32_t* rowptr, int32_t* col, double* val, double* rhs) {
+  std::memcpy(rowptr, s->rowptr.data(), s->rowptr.size() * sizeof(int32_t));
+  std::memcpy(col, s->col.data(), s->col.size() * sizeof(int32_t));
+  std::memcpy(val, s->val.data(), s->val.size() * sizeof(double));
+  std::memcpy(rhs, s->rhs.data(), s->rhs.size() * sizeof(double));
+}
+void orc_sys_free(orc_sys* s) { delete s; }
+
+// ---------------------------------------------------------------- Krylov
+// y = A x with each row summed left to right as a fused multiply-add chain.
+void orc_spmv(int64_t n, const int32_t* rowptr, const int32_t* col, const double* val,
+              const double* x, double* y) {
+#pragma omp parallel for schedule(static)
+  for (int64_t i = 0; i < n; ++i) {
+    double s = 0.0;
+    for (int32_t k = rowptr[i]; k < rowptr[i + 1]; ++k) s = std::fma(val[k], x[col[k]], s);
+    y[i] = s;
+  }
+}
+
+namespace {
+double dot(int64_t n, const double* a, const double* b, int nth) {
+  double s = 0.0;
+  if (nth <= 1) {
+    for (int64_t i = 0; i < n; ++i) s += a[i] * b[i];
+    return s;
+  }
+#pragma omp parallel for schedule(static) reduction(+ : s) num_threads(nth)
+  for (int64_t i = 0; i < n; ++i) s += a[i] * b[i];
+  return s;
+}
+}  // namespace
+
+// LIS 2.1.6 CG with the Jacobi preconditioner as called from
+// lis_compute_electric_potential (src/pb_class.cpp:2243-2329) with x0 = 0:
+//   r = b, z = D^-1 r, p = z, rho = (r, z)
+//   repeat: q = A p; alpha = rho / (p, q); x += alpha p; r -= alpha q;
+//           stop when ||r||_2 <= tol ||b||_2;  z = D^-1 r; rho' = (r, z);
+//           p = z + (rho'/rho) p
+// iters = number of matrix-vector products. nthreads > 1 runs the vector work
+// with OpenMP (the CPU baseline); nthreads <= 1 is sequential.
+// Returns 0 on convergence, 4 if maxiter was reached, 5 on a non-positive diagonal.
+int orc_pcg(int64_t n, const int32_t* rowptr, const int32_t* col, const double* val,
+            const double* b, double tol, int maxiter, int nthreads, double* x, int* iters,
+            double* relres) {
+  const int nth = nthreads < 1 ? 1 : nthreads;
+#ifdef _OPENMP
+  const int saved = omp_get_max_threads();
+  omp_set_num_threads(nth);
+#endif
+  std::vector<double> dinv(n), r(n), z(n), p(n), q(n);
+  int status = 0;
+  for (int64_t i = 0; i < n && status == 0; ++i) {
+    double d = 0.0;
+    for (int32_t k = rowptr[i]; k < rowptr[i + 1]; ++k)
+      if (col[k] == i) d = val[k];
+    if (!(d > 0.0)) status = 5;
+    dinv[i] = 1.0 / d;
+  }
+  *iters = 0;
+  *relres = 0.0;
+  if (status) goto done;
+  {
+#pragma omp parallel for schedule(static) num_threads(nth)
+    for (int64_t i = 0; i < n; ++i) {
+      x[i] = 0.0;
+      r[i] = b[i];
+      z[i] = dinv[i] * r[i];
+      p[i] = z[i];
+    }
+    const double bnrm = std::sqrt(dot(n, b, b, nth));
+    if (bnrm == 0.0) goto done;
+    double rho = dot(n, r.data(), z.data(), nth);
+    status = 4;
+    for (int it = 1; it <= maxiter; ++it) {
+      orc_spmv(n, rowptr, col, val, p.data(), q.data());
+      const double pq = dot(n, p.data(), q.data(), nth);
+      const double alpha = rho / pq;
+#pragma omp parallel for schedule(static) num_threads(nth)
+      for (int64_t i = 0; i < n; ++i) {
+        x[i] += alpha * p[i];
+        r[i] -= alpha * q[i];
+      }
+      const double rn = std::sqrt(dot(n, r.data(), r.data(), nth));
+      *iters = it;
+      *relres = rn / bnrm;
+      if (rn <= tol * bnrm) { status = 0; break; }
+#pragma omp parallel for schedule(static) num_threads(nth)
+      for (int64_t i = 0; i < n; ++i) z[i] = dinv[i] * r[i];
+      const double rho_new = dot(n, r.data(), z.data(), nth);
+      const double beta = rho_new / rho;
+      rho = rho_new;
+#pragma omp parallel for schedule(static) num_threads(nth)
+      for (int64_t i = 0; i < n; ++i) p[i] = z[i] + beta * p[i];
+    }
+  }
+done:
+#ifdef _OPENMP
+  omp_set_num_threads(saved);
+#endif
+  return status;
+}
+
+// Timing helper for the CPU baseline: runs `iters` PCG iterations (no stopping
+// test) on n unknowns with nthreads OpenMP threads and returns seconds elapsed.
+double orc_pcg_time_iters(int64_t n, const int32_t* rowptr, const int32_t* col,
+                          const double* val, const double* b, int iters, int nthreads) {
+  std::vector<double> x(n);
+  int it = 0;
+  double rr = 0;
+#ifdef _OPENMP
+  const double t0 = omp_get_wtime();
+#endif
+  orc_pcg(n, rowptr, col, val, b, 0.0, iters, nthreads, x.data(), &it, &rr);
+#ifdef _OPENMP
+  return omp_get_wtime() - t0;
+#else
+  return -1.0;
+#endif
+}
+
+// ---------------------------------------------------------------- energy
+// energy (src/pb_class.cpp:2790-3063) for one rank, with classifyCube
+// (:2570-2598), classifyCube_flux (:2627-2661), getTriangles (:2771-2788), phi0
+// (:2700-2704), areaTriangle (:2706-2722). The normal used at a triangle vertex is
+// the one most recently written by normal_intersection; it starts at zero.
+// out[0] = energy_pol, out[1] = energy_react, out[2] = charge_pol (flux charge * 4 pi)
+int orc_energy(const orc_mesh* m, const orc_rays* rays, const double* eps_node,
+               const double* phi, int64_t n_border, const int32_t* border, int64_t na,
+               const double* xyz, const double* q, double e_in, double e_out, double T,
+               double I, int calc_energy, double* out) {
+  const auto maps = build_maps(rays);
+  MeshView mv{m};
+  double cst[6];
+  orc_constants(e_in, e_out, T, I, cst);
+  const double eps_out = cst[1];
+  const double k = cst[3];
+  const double inv_4pi = 1.0 / (4.0 * kPi);
+  const double constant_pol = cst[4], constant_react = cst[5];
+  std::vector<double> qa;
+  std::vector<std::array<double, 3>> pa;
+  for (int64_t a = 0; a < na; ++a)
+    if (std::fabs(q[a]) > 1.e-5) {
+      qa.push_back(q[a]);
+      pa.push_back({xyz[3 * a], xyz[3 * a + 1], xyz[3 * a + 2]});
+    }
+  const size_t nq = qa.size();
+  double first_int = 0.0, second_int = 0.0, charge_pol = 0.0;
+  double N[3] = {0.0, 0.0, 0.0};
+  double energy_pol = 0.0, energy_react = 0.0;
+  const bool ionic = (calc_energy == 2 && k > 1.e-5);
+  const bool polar_only = (calc_energy == 1 || (calc_energy == 2 && k < 1.e-5));
+  for (int64_t bi = 0; bi < n_border && (ionic || polar_only); ++bi) {
+    const int64_t l = border[bi];
+    double h[3], area_h[3];
+    for (int d = 0; d < 3; ++d) h[d] = mv.p(l, d, 7) - mv.p(l, d, 0);
+    area_h[0] = h[1] * h[2] / h[0] * 0.25;
+    area_h[1] = h[0] * h[2] / h[1] * 0.25;
+    area_h[2] = h[0] * h[1] / h[2] * 0.25;
+    // classifyCube_flux
+    double te[8], tp[8];
+    for (int ii = 0; ii < 8; ++ii) {
+      if (!mv.hanging(l, ii)) {
+        te[ii] = eps_node[mv.gt(l, ii)];
+        tp[ii] = phi[mv.gt(l, ii)];
+      } else {
+        te[ii] = 0.0;
+        tp[ii] = 0.0;
+        const int npi = mv.num_parents(l, ii);
+        for (int jj = 0; jj < npi; ++jj) {
+          te[ii] += eps_node[mv.gparent(l, jj, ii)] / npi;
+          tp[ii] += phi[mv.gparent(l, jj, ii)] / npi;
+        }
+      }
+    }
+    int cubeindex = -1;
+    if (ionic) {
+      // classifyCube (hanging corners: mean of parents)
+      constexpr double EPS = 1e-10;
+      static const int order[8] = {0, 1, 3, 2, 4, 5, 7, 6};
+      int ci = 0, bit = 1;
+      for (int o = 0; o < 8; ++o) {
+        const int ii = order[o];
+        double v;
+        if (!mv.hanging(l, ii)) v = eps_node[mv.gt(l, ii)];
+        else {
+          v = 0.0;
+          const int npi = mv.num_parents(l, ii);
+          for (int jj = 0; jj < npi; ++jj) v += eps_node[mv.gparent(l, jj, ii)] / npi;
+        }
+        if (v < eps_out - EPS) ci |= bit;
+        bit *= 2;
+      }
+      cubeindex = (ci == 0 || ci == 255) ? -1 : ci;
+    }
+    for (int e = 0; e < 12; ++e) {
+      const int i1 = kEdgeNodes[2 * e], i2 = kEdgeNodes[2 * e + 1];
+      double fl;
+      if (te[i1] < te[i2]) fl = 1;
+      else if (te[i1] > te[i2]) fl = -1;
+      else continue;
+      const int axis = kEdgeAxis[e];
+      double fract = 0.0;
+      normal_intersection(mv, maps, l, e, N, &fract);
+      double V[3] = {mv.p(l, 0, i1), mv.p(l, 1, i1), mv.p(l, 2, i1)};
+      V[axis] += fract * h[axis];
+      const double tmp_flux = -(tp[i2] - tp[i1]) * wha(te[i1], te[i2], fract) * fl * area_h[axis];
+      charge_pol += tmp_flux;
+      for (size_t ia = 0; ia < nq; ++ia) {
+        const double dx = pa[ia][0] - V[0], dy = pa[ia][1] - V[1], dz = pa[ia][2] - V[2];
+        const double r = std::sqrt(dx * dx + dy * dy + dz * dz);
+        if (polar_only) {
+          const double qflux = tmp_flux / r;
+          first_int += qa[ia] * qflux;
+        } else {
+          first_int += qa[ia] * tmp_flux / r;
+        }
+      }
+    }
+    if (ionic && cubeindex >= 0) {
+      const char* tri = kMcTriangles[cubeindex];
+      const int ntri = (int)std::strlen(tri) / 3;
+      for (int t = 0; t < ntri; ++t) {
+        double vt[3][3], nv[3][3], ps[3];
+        for (int jj = 0; jj < 3; ++jj) {
+          const int edge = mc_digit(tri[3 * t + jj]);
+          const int axis = kEdgeAxis[edge];
+          const int i1 = kEdgeNodes[2 * edge], i2 = kEdgeNodes[2 * edge + 1];
+          double fract = 0.0;
+          normal_intersection(mv, maps, l, edge, N, &fract);
+          double V[3] = {mv.p(l, 0, i1), mv.p(l, 1, i1), mv.p(l, 2, i1)};
+          V[axis] += fract * h[axis];
+          for (int d = 0; d < 3; ++d) { vt[jj][d] = V[d]; nv[jj][d] = N[d]; }
+          ps[jj] = phi0(te[i1], te[i2], tp[i1], tp[i2], fract);
+        }
+        const double area = area_triangle(vt);
+        for (size_t ia = 0; ia < nq; ++ia) {
+          const double qi = qa[ia];
+          for (int kk = 0; kk < 3; ++kk) {
+            const double dv[3] = {vt[kk][0] - pa[ia][0], vt[kk][1] - pa[ia][1], vt[kk][2] - pa[ia][2]};
+            const double r2 = dv[0] * dv[0] + dv[1] * dv[1] + dv[2] * dv[2];
+            const double r = std::sqrt(r2);
+            const double inv_r3 = 1.0 / (r2 * r);
+            const double dt = dv[0] * nv[kk][0] + dv[1] * nv[kk][1] + dv[2] * nv[kk][2];
+            second_int += qi * ps[kk] * dt * inv_r3 * inv_4pi * area / 3.0;
+          }
+        }
+      }
+    }
+  }
+  if (polar_only) energy_pol = 0.5 * constant_pol * first_int;
+  if (ionic) {
+    energy_pol = 0.5 * constant_pol * first_int;
+    energy_react = 0.5 * (second_int - first_int * constant_react);
+  }
+  out[0] = energy_pol;
+  out[1] = energy_react;
+  out[2] = charge_pol;
+  return 0;
+}
+
+// Born-sphere potential used for bc_type 3 (src/pb_class.cpp:3549-3571): q = +1 and
+// rs = 2 are fixed there.
+double orc_analytic_solution(double x, double y, double z, double e_in, double e_out,
+                             double T, double I) {
+  double c[6];
+  orc_constants(e_in, e_out, T, I, c);
+  const double eps_in = c[0], eps_out = c[1], k = c[3];
+  const double rs = 2.0;
+  const double pippo_out = eps_out * (1.0 + k * rs);
+  const double pippo_in = eps_in * rs;
+  const double dist = std::hypot(x, y, z);
+  if (dist <= rs) return 1.0 / (pippo_out * rs) + (rs - dist) / (pippo_in * dist);
+  return std::exp(k * (rs - dist)) / (dist * pippo_out);
+}
+
+}  // extern "C"
