@@ -1,0 +1,224 @@
+#!/usr/bin/env python3
+"""Benchmark of the linearized-PB hot path on one MI355X.
+
+Workload (default): the synthetic globular protein — 2,500 atoms seeded (20502) in a
+20 A ball, radii U[1.2, 2.0] A, charges N(0, 0.35) e shifted to a net -3 e,
+eps_in 2 / eps_out 80, 0.145 M, T 298.15 K, domain edge 3x the molecular extent,
+adaptive 2:1 octree with 0.5 A leaves near the molecule coarsening to 2 A at the box
+boundary (about 8.0M unknowns with hanging nodes), Coulomb Dirichlet data.
+
+A step is one PB solve on resident inputs: in/out markers -> edge fractions ->
+numeric assembly (Laplacian, kappa^2 mass, hanging-node constraints, charge
+projection, Dirichlet elimination) -> Krylov solve to ||r|| <= 1e-10 ||b|| ->
+polarization + ionic energy. The mesh topology (symbolic assembly) is built once
+before timing. `value` is mixed-precision solves/s; the plain FP64 solve of the
+same system is timed beside it, with the relative energy error between the two.
+
+Prints one JSON line on stdout; progress goes to stderr.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+REPO = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, REPO)
+
+HBM_PEAK_GBS = 8000.0
+
+
+def log(msg):
+    print(f"[bench {time.strftime('%H:%M:%S')}] {msg}", file=sys.stderr, flush=True)
+
+
+def pct(a, q):
+    return float(np.percentile(np.asarray(a), q))
+
+
+def run_steps(pb, precision, steps, tol, check_every, profile):
+    from cudanextgenpb_amd import native
+
+    ctx = pb.ctx
+    ctx.set_profiling(profile)
+    lat, iters, st, E = [], [], None, None
+    t0 = time.perf_counter()
+    for k in range(steps):
+        s0 = time.perf_counter()
+        pb.create_markers()
+        ctx.assemble()
+        _, st = ctx.solve(precision, tol=tol, maxiter=200000, drop=0.05, check_every=check_every, want_phi=False)
+        E = ctx.energy()
+        lat.append((time.perf_counter() - s0) * 1e3)
+        iters.append(st["iterations"])
+        log(f"  {'mixed' if precision == native.PREC_MIXED else 'fp64 '} step {k + 1}/{steps}: {lat[-1]:.1f} ms, "
+            f"{st['iterations']} it, solve {st['ms_solve']:.1f} ms, relres {st['relres']:.2e}, "
+            f"E_polar {E['polar']:.10f}")
+    total = time.perf_counter() - t0
+    kst = ctx.kernel_stats(precision) if profile else None
+    ctx.set_profiling(False)
+    return dict(total_s=total, lat_ms=lat, iters=iters, stats=st, energy=E, kernel=kst)
+
+
+def cpu_baseline(pb, fp64_iters, seconds):
+    sys.path.insert(0, os.path.join(REPO, "oracle"))
+    import oracle as O
+
+    rp, col, val, rhs = pb.ctx.export_csr()
+    S = O.System(len(rhs), rp, col, val, rhs)
+    threads = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or (os.cpu_count() or 1)
+    t = O.pcg_time_iters(S, 3, threads)
+    per_it = max(t / 3, 1e-6)
+    n_it = int(max(5, min(2000, seconds / per_it)))
+    t = O.pcg_time_iters(S, n_it, threads)
+    per_it = t / n_it
+    solve_s = per_it * fp64_iters
+    return {
+        "value": 1.0 / solve_s,
+        "unit": "solves/s",
+        "cores": threads,
+        "kind": "port",
+        "sample": (f"{n_it} Jacobi-PCG iterations (restated LIS -i cg -p jacobi, OpenMP, {threads} threads) on the "
+                   f"same {S.n}-row FP64 CSR in {t:.1f} s = {per_it * 1e3:.2f} ms/iteration, scaled to the "
+                   f"{fp64_iters} iterations of the FP64 solve; markers, assembly and energy not included"),
+        "dof_iter_per_s": S.n / per_it,
+    }
+
+
+def main():
+    ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=8)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--workload", choices=["globular", "born"], default="globular")
+    ap.add_argument("--tol", type=float, default=1e-10)
+    ap.add_argument("--check-every", type=int, default=32)
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--no-fp64", action="store_true", help="skip the FP64 comparison leg")
+    args = ap.parse_args()
+
+    world = int(os.environ.get("WORLD_SIZE", "1"))
+    if args.gpus != 1 or world != 1:
+        print("bench.py measures one GPU: this build has no multi-GPU path (each system is solved on one "
+              "device; see the sharding notes in DESIGN.md). Run with --gpus 1.", file=sys.stderr)
+        sys.exit(2)
+
+    from cudanextgenpb_amd import native, workload as W
+    from cudanextgenpb_amd.pb import PoissonBoltzmann
+
+    if native.device_count() < 1:
+        raise SystemExit("no HIP device visible")
+
+    t = time.perf_counter()
+    if args.workload == "globular":
+        P = W.globular()
+        wname = ("globular protein: 2500 atoms (seed 20502) in a 20 A ball, radii U[1.2,2.0], q N(0,0.35) net -3 e; "
+                 "eps 2/80, 0.145 M; box 3x extent; 2:1 octree 0.5 A near the molecule -> 2 A at the boundary; "
+                 "Coulomb Dirichlet; tol 1e-10")
+    else:
+        P = W.born(h=0.25)
+        wname = "Born sphere: r 2 A, q +1, 32 A box, uniform 0.25 A (128^3 leaves), analytic Dirichlet; tol 1e-10"
+    m = P.mesh
+    log(f"workload {P.name}: N={m.n_nodes} leaves={m.n_leaves} hanging={m.n_hang} atoms={P.atoms.n} "
+        f"(host generation {time.perf_counter() - t:.1f} s)")
+
+    t = time.perf_counter()
+    pb = PoissonBoltzmann(P)
+    pb.create_markers()
+    pb.create_density_map()
+    pb.assemple_system_matrix()
+    ctx = pb.ctx
+    setup_ms = (time.perf_counter() - t) * 1e3
+    log(f"resident on device: n={ctx.n} nnz={ctx.nnz} device bytes={ctx.device_bytes() / 2**30:.2f} GiB "
+        f"(upload + symbolic + first assembly {setup_ms:.0f} ms)")
+
+    for _ in range(args.warmup):
+        run_steps(pb, native.PREC_MIXED, 1, args.tol, args.check_every, False)
+        if not args.no_fp64:
+            run_steps(pb, native.PREC_FP64, 1, args.tol, args.check_every, False)
+
+    mixed = run_steps(pb, native.PREC_MIXED, args.steps, args.tol, args.check_every, True)
+    fp64 = None if args.no_fp64 else run_steps(pb, native.PREC_FP64, args.steps, args.tol, args.check_every, True)
+
+    n = ctx.n
+    value = args.steps / mixed["total_s"]
+    mixed_solve_ms = mixed["stats"]["ms_solve"]
+    mk = mixed["kernel"]
+    avg_ms = mk["total_ms"] / max(mk["launches"], 1)
+    achieved = mk["bytes_per_launch"] / (avg_ms * 1e-3) / 1e9 if avg_ms > 0 else 0.0
+    out = {
+        "metric": "solves/s & DOF-iter/s at 1e-6 rel energy err, mixed vs FP64 GPU; p99 latency",
+        "value": value,
+        "unit": "solves/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": mixed["total_s"] * 1e3 / args.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "fp32+fp64",
+        "data": "synthetic",
+        "config": {
+            "workload": wname,
+            "unknowns": n,
+            "nnz": ctx.nnz,
+            "leaves": m.n_leaves,
+            "hanging_nodes": m.n_hang,
+            "atoms": P.atoms.n,
+            "parallelism": "single GPU",
+        },
+        "p50_latency_ms": pct(mixed["lat_ms"], 50),
+        "p99_latency_ms": pct(mixed["lat_ms"], 99),
+        "mixed": {
+            "iterations": mixed["iters"][-1],
+            "replacements": mixed["stats"]["replacements"],
+            "solve_ms": mixed_solve_ms,
+            "dof_iter_per_s": n * mixed["iters"][-1] / (mixed_solve_ms * 1e-3),
+            "relres_fp64": mixed["stats"]["relres"],
+            "energy_polar_kT": mixed["energy"]["polar"],
+        },
+        "roofline": {
+            "kernel": "compact FP32 SpMV (SELL-64, 16-bit column offsets) of the mixed inner CG",
+            "bound": "hbm",
+            "achieved": achieved,
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": achieved / HBM_PEAK_GBS,
+            "traffic": None,
+            "bytes_per_launch": mk["bytes_per_launch"],
+            "avg_launch_ms": avg_ms,
+            "launches": mk["launches"],
+        },
+    }
+    if fp64 is not None:
+        fk = fp64["kernel"]
+        f_avg = fk["total_ms"] / max(fk["launches"], 1)
+        e64 = fp64["energy"]["polar"]
+        out["fp64"] = {
+            "solves_per_s": args.steps / fp64["total_s"],
+            "iterations": fp64["iters"][-1],
+            "solve_ms": fp64["stats"]["ms_solve"],
+            "dof_iter_per_s": n * fp64["iters"][-1] / (fp64["stats"]["ms_solve"] * 1e-3),
+            "p50_latency_ms": pct(fp64["lat_ms"], 50),
+            "p99_latency_ms": pct(fp64["lat_ms"], 99),
+            "energy_polar_kT": e64,
+            "spmv_avg_launch_ms": f_avg,
+            "spmv_gbs": fk["bytes_per_launch"] / (f_avg * 1e-3) / 1e9 if f_avg > 0 else 0.0,
+        }
+        out["mixed_over_fp64_solves"] = value / out["fp64"]["solves_per_s"]
+        out["mixed_over_fp64_solve_time"] = fp64["stats"]["ms_solve"] / mixed_solve_ms
+        out["rel_energy_err_vs_fp64"] = abs(mixed["energy"]["polar"] - e64) / abs(e64)
+    if not args.no_cpu:
+        log("CPU baseline (restated LIS PCG-Jacobi, OpenMP)")
+        fp64_iters = fp64["iters"][-1] if fp64 is not None else mixed["iters"][-1]
+        out["cpu_baseline"] = cpu_baseline(pb, fp64_iters, args.cpu_seconds)
+    pb.close()
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

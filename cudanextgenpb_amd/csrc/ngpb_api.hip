@@ -53,11 +53,12 @@ int64_t ngpb_ctx::device_bytes() const {
        bc_col.bytes() + bc_em.bytes();
   b += rowptr.bytes() + col.bytes() + val.bytes() + rhs.bytes() + rhs_charge.bytes() + c_edge.bytes() + v8.bytes() +
        vol_patch.bytes() + rho.bytes();
-  b += slice_width.bytes() + slice_ptr.bytes() + slice_base_ptr.bytes() + slice_wide_ptr.bytes() + sell_val.bytes() +
-       sell_off.bytes() + sell_base.bytes() + sell_col32.bytes() + dscale.bytes();
+  b += slice_width.bytes() + slice_base_ptr.bytes() + slice_wide_ptr.bytes() + slice_ov_ptr.bytes() +
+       sell_val.bytes() + sell_off.bytes() + sell_base.bytes() + sell_col32.bytes() + sell_ov_cnt.bytes() +
+       sell_ov_val.bytes() + sell_ov_col.bytes() + dscale.bytes();
   b += x.bytes() + r.bytes() + z.bytes() + p.bytes() + q.bytes() + dinv.bytes();
   b += xs.bytes() + rs.bytes() + ps.bytes() + qs.bytes() + ks.bytes();
-  b += red.partials.bytes() + red.tickets.bytes() + scan_tmp.bytes();
+  b += red.partials.bytes() + scan_tmp.bytes();
   return b;
 }
 
@@ -141,6 +142,8 @@ int ngpb_destroy(ngpb_ctx* c) {
   (void)hipSetDevice(c->device);
   if (c->stream) (void)hipStreamSynchronize(c->stream);
   if (c->ks_host) (void)hipHostFree(c->ks_host);
+  for (auto& p : c->kprof)
+    for (hipEvent_t e : p.ev) (void)hipEventDestroy(e);
   c->ks_host = nullptr;
   if (c->stream) (void)hipStreamDestroy(c->stream);
   delete c;  // device buffers free themselves
@@ -591,6 +594,38 @@ int ngpb_time_spmv(ngpb_ctx* c, int precision, int reps, double* ms, double* byt
     } else {
       throw Error(NGPB_E_ARG, "unknown precision");
     }
+  });
+}
+
+int ngpb_set_profiling(ngpb_ctx* c, int enable) {
+  return guarded(c, [&] {
+    c->profile = enable != 0;
+    for (auto& p : c->kprof) {
+      p.used = 0;
+      p.launches = 0;
+      p.ms = 0.0;
+    }
+  });
+}
+
+int ngpb_kernel_stats(ngpb_ctx* c, int precision, int64_t* launches, double* total_ms, double* bytes_per_launch) {
+  return guarded(c, [&] {
+    NGPB_REQUIRE(launches && total_ms && bytes_per_launch, NGPB_E_ARG, "NULL output");
+    NGPB_REQUIRE(precision == NGPB_PREC_FP64 || precision == NGPB_PREC_MIXED, NGPB_E_ARG, "unknown precision");
+    const auto& p = c->kprof[precision];
+    *launches = p.launches;
+    *total_ms = p.ms;
+    *bytes_per_launch = !c->have_system ? 0.0
+                        : precision == NGPB_PREC_FP64 ? ngpb::spmv_fp64_bytes(c)
+                        : (c->have_compact ? ngpb::spmv_compact_bytes(c) : 0.0);
+  });
+}
+
+int ngpb_time_copy(ngpb_ctx* c, int64_t bytes, int reps, double* ms_per_launch) {
+  return guarded(c, [&] {
+    NGPB_REQUIRE(ms_per_launch && reps > 0 && bytes >= 16 && bytes % 16 == 0, NGPB_E_ARG,
+                 "bytes must be a positive multiple of 16");
+    *ms_per_launch = ngpb::time_copy(c, bytes, reps);
   });
 }
 

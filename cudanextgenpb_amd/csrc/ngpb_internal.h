@@ -96,10 +96,9 @@ struct MeshView {
   const int32_t* leaf_corner = nullptr;
 };
 
-// deterministic block/grid reduction bookkeeping
+// per-block partial sums of the Krylov reductions (up to 4 per block)
 struct GridReduce {
-  DBuf<double> partials;    // per-block partial sums (up to 4 per block)
-  DBuf<unsigned> tickets;   // arrival counters, self-resetting
+  DBuf<double> partials;
 };
 
 // scalar state of a Krylov solve on the device
@@ -190,14 +189,17 @@ struct ngpb_ctx {
 
   // compact mixed-precision operator
   bool have_compact = false;
-  int64_t n_slices = 0, sell_entries = 0, sell_wide_entries = 0, sell_bases = 0;
-  ngpb::DBuf<int32_t> slice_width;   // per slice
-  ngpb::DBuf<int64_t> slice_ptr;     // per slice: first entry (64 * sum of widths)
-  ngpb::DBuf<int64_t> slice_base_ptr;// per slice: first base
-  ngpb::DBuf<int64_t> slice_wide_ptr;// per slice: first 32-bit column (or -1)
+  int64_t n_slices = 0, sell_entries = 0, sell_wide_entries = 0, sell_bases = 0, sell_ov_entries = 0;
+  ngpb::DBuf<int32_t> slice_width;    // per slice: main slots
+  ngpb::DBuf<int64_t> slice_base_ptr; // per slice: first slot (entries start at 64x this)
+  ngpb::DBuf<int64_t> slice_wide_ptr; // per slice: first 32-bit column (or -1)
+  ngpb::DBuf<int64_t> slice_ov_ptr;   // per slice: first overflow entry (or -1)
   ngpb::DBuf<float> sell_val;
   ngpb::DBuf<uint16_t> sell_off;
   ngpb::DBuf<int32_t> sell_base, sell_col32;
+  ngpb::DBuf<uint8_t> sell_ov_cnt;    // per row: overflow entries
+  ngpb::DBuf<float> sell_ov_val;
+  ngpb::DBuf<int32_t> sell_ov_col;
   ngpb::DBuf<double> dscale;         // 1/sqrt(diag)
 
   // solver state
@@ -215,6 +217,15 @@ struct ngpb_ctx {
 
   // scan scratch
   ngpb::DBuf<int64_t> scan_tmp;
+
+  // optional HIP-event timing of SpMV launches (kind 0: FP64 CSR, 1: compact FP32)
+  bool profile = false;
+  struct KProf {
+    std::vector<hipEvent_t> ev;
+    size_t used = 0;
+    int64_t launches = 0;
+    double ms = 0.0;
+  } kprof[2];
 
   int64_t device_bytes() const;
 };
@@ -238,6 +249,8 @@ void run_numeric(ngpb_ctx* c);
 void run_dirichlet_values(ngpb_ctx* c, int bc_type);
 
 // Krylov
+unsigned vec_grid(int64_t n);
+void check_operator(ngpb_ctx* c);
 void ensure_reduce(ngpb_ctx* c, int64_t blocks);
 void spmv_fp64(ngpb_ctx* c, const double* x, double* y);
 void solve_fp64(ngpb_ctx* c, const ngpb_solve_opts& o, ngpb_solve_stats& st);
@@ -253,6 +266,11 @@ double time_spmv_compact(ngpb_ctx* c, int reps);
 
 // energy
 void run_energy(ngpb_ctx* c, ngpb_energy_out* out);
+
+// SpMV launch timing
+void prof_mark(ngpb_ctx* c, int kind);
+void prof_collect(ngpb_ctx* c);
+double time_copy(ngpb_ctx* c, int64_t bytes, int reps);
 
 // launch helper
 inline unsigned blocks_for(int64_t n, int bs) { return (unsigned)((n + bs - 1) / bs); }
@@ -301,30 +319,12 @@ __device__ __forceinline__ double block_sum(double v, double* lds /* >= BS/64 */
   return t;
 }
 
-// Publishes this block's partial(s) and returns true in the block that arrives last.
-// The caller of the last block must then combine partials[0..gridDim.x) in a fixed order.
-__device__ __forceinline__ bool arrive_last(unsigned* ticket, int* lds_flag) {
-  __threadfence();
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    const unsigned t = atomicAdd(ticket, 1u);
-    *lds_flag = (t == gridDim.x - 1) ? 1 : 0;
-  }
-  __syncthreads();
-  const bool last = *lds_flag != 0;
-  if (last) __threadfence();
-  return last;
-}
-
-__device__ __forceinline__ double load_coherent(const double* p) {
-  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
-
-// fixed-order sum of `count` partials with stride `stride` by one block of BS threads
+// Fixed-order sum of `count` partials (stride `stride`) by one block of BS threads.
+// The partials were written by an earlier kernel on the same stream.
 template <int BS>
 __device__ __forceinline__ double sum_partials(const double* part, int count, int stride, double* lds) {
   double s = 0.0;
-  for (int i = threadIdx.x; i < count; i += BS) s += load_coherent(part + (int64_t)i * stride);
+  for (int i = threadIdx.x; i < count; i += BS) s += part[(int64_t)i * stride];
   return block_sum<BS>(s, lds);
 }
 

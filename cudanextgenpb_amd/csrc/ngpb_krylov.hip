@@ -8,11 +8,14 @@
 // results are bit-identical). Blocks whose rows hold more than kCap entries read
 // global memory directly.
 //
-// CG: three kernels per iteration (SpMV + (p,q); x/r update + (r,z), (r,r); new
-// direction). Dot products are reduced per block in a fixed order and combined by
-// the last block to arrive, which also updates the scalars (alpha, beta, stop flag)
-// in device memory, so the host only polls the stop flag every `check_every`
-// iterations and iterations launched after convergence exit immediately.
+// CG iteration: SpMV + per-block (p, q) -> finalize (alpha) -> x/r update +
+// per-block (r, D^-1 r), (r, r) -> finalize (stop test, beta) -> new direction.
+// Per-block partial sums are combined by a one-block finalize kernel in a fixed
+// order, so reductions are deterministic and no kernel needs a device-wide memory
+// fence (on a multi-XCD GPU an agent-scope fence writes back the XCD's L2, which
+// would serialize every block). Scalars live in device memory; the host polls the
+// stop flag every `check_every` iterations and kernels launched after the stop exit
+// at once.
 #include <algorithm>
 #include <cmath>
 
@@ -23,18 +26,16 @@ namespace {
 
 constexpr int kBS = 256;
 constexpr int kCap = 2048;  // staged entries per block
-
-enum { T_SPMV = 0, T_UPD = 1, T_INIT = 2, T_MISC = 3 };
+constexpr int kFin = 1024;  // finalize block size
 
 template <bool DOT>
 __global__ void __launch_bounds__(kBS) k_spmv64(int64_t n, const int32_t* __restrict__ rowptr,
                                                  const int32_t* __restrict__ col, const double* __restrict__ val,
                                                  const double* __restrict__ x, double* __restrict__ y,
-                                                 KrylovScalars* ks, double* partials, unsigned* ticket) {
+                                                 const KrylovScalars* ks, double* __restrict__ partials) {
   __shared__ double s_val[kCap];
   __shared__ double s_x[kCap];
   __shared__ double s_red[kBS / 64];
-  __shared__ int s_flag;
   if (DOT && ks->done) return;
   const int64_t r0 = (int64_t)blockIdx.x * kBS;
   const int64_t r1 = min(r0 + kBS, n);
@@ -59,24 +60,25 @@ __global__ void __launch_bounds__(kBS) k_spmv64(int64_t n, const int32_t* __rest
   if (DOT) {
     const double part = dev::block_sum<kBS>(row < n ? x[row] * sum : 0.0, s_red);
     if (threadIdx.x == 0) partials[blockIdx.x] = part;
-    if (dev::arrive_last(ticket, &s_flag)) {
-      const double pq = dev::sum_partials<kBS>(partials, gridDim.x, 1, s_red);
-      if (threadIdx.x == 0) {
-        ks->pq = pq;
-        ks->alpha = ks->rho / pq;
-        *ticket = 0;
-      }
-    }
   }
 }
 
-// x = 0, r = b, p = D^-1 b; rho = (r, D^-1 r), bb = (b, b)
+// alpha = rho / (p, q)
+__global__ void __launch_bounds__(kFin) k_fin_alpha(KrylovScalars* ks, const double* __restrict__ partials, int np) {
+  __shared__ double s_red[kFin / 64];
+  if (ks->done) return;
+  const double pq = dev::sum_partials<kFin>(partials, np, 1, s_red);
+  if (threadIdx.x == 0) {
+    ks->pq = pq;
+    ks->alpha = ks->rho / pq;
+  }
+}
+
+// x = 0, r = b, p = D^-1 b; per block (r, D^-1 r), (b, b)
 __global__ void __launch_bounds__(kBS) k_pcg_init(int64_t n, const double* __restrict__ b, const double* __restrict__ dinv,
                                                   double* __restrict__ x, double* __restrict__ r, double* __restrict__ p,
-                                                  KrylovScalars* ks, double tol2, int maxiter, double* partials,
-                                                  unsigned* ticket) {
+                                                  double* __restrict__ partials) {
   __shared__ double s_red[kBS / 64];
-  __shared__ int s_flag;
   double rz = 0.0, bb = 0.0;
   for (int64_t i = (int64_t)blockIdx.x * kBS + threadIdx.x; i < n; i += (int64_t)gridDim.x * kBS) {
     const double bi = b[i], zi = dinv[i] * bi;
@@ -92,30 +94,31 @@ __global__ void __launch_bounds__(kBS) k_pcg_init(int64_t n, const double* __res
     partials[2 * blockIdx.x] = rz;
     partials[2 * blockIdx.x + 1] = bb;
   }
-  if (dev::arrive_last(ticket, &s_flag)) {
-    const double RZ = dev::sum_partials<kBS>(partials, gridDim.x, 2, s_red);
-    const double BB = dev::sum_partials<kBS>(partials + 1, gridDim.x, 2, s_red);
-    if (threadIdx.x == 0) {
-      ks->rho = RZ;
-      ks->bb = BB;
-      ks->rr = BB;
-      ks->stop_rr = tol2 * BB;
-      ks->iter = 0;
-      ks->maxiter = maxiter;
-      ks->status = 0;
-      ks->done = (BB == 0.0) ? 1 : 0;
-      *ticket = 0;
-    }
+}
+
+__global__ void __launch_bounds__(kFin) k_fin_init(KrylovScalars* ks, const double* __restrict__ partials, int np,
+                                                   double tol2, int maxiter) {
+  __shared__ double s_red[kFin / 64];
+  const double RZ = dev::sum_partials<kFin>(partials, np, 2, s_red);
+  const double BB = dev::sum_partials<kFin>(partials + 1, np, 2, s_red);
+  if (threadIdx.x == 0) {
+    ks->rho = RZ;
+    ks->bb = BB;
+    ks->rr = BB;
+    ks->stop_rr = tol2 * BB;
+    ks->iter = 0;
+    ks->maxiter = maxiter;
+    ks->status = 0;
+    ks->done = (BB == 0.0) ? 1 : 0;
   }
 }
 
-// x += alpha p; r -= alpha q; then (r, D^-1 r), (r, r); stop test; beta
+// x += alpha p; r -= alpha q; per block (r, D^-1 r), (r, r)
 __global__ void __launch_bounds__(kBS) k_pcg_update(int64_t n, const double* __restrict__ dinv, double* __restrict__ x,
                                                     double* __restrict__ r, const double* __restrict__ p,
-                                                    const double* __restrict__ q, KrylovScalars* ks, double* partials,
-                                                    unsigned* ticket) {
+                                                    const double* __restrict__ q, const KrylovScalars* ks,
+                                                    double* __restrict__ partials) {
   __shared__ double s_red[kBS / 64];
-  __shared__ int s_flag;
   if (ks->done) return;
   const double alpha = ks->alpha;
   double rz = 0.0, rr = 0.0;
@@ -132,24 +135,27 @@ __global__ void __launch_bounds__(kBS) k_pcg_update(int64_t n, const double* __r
     partials[2 * blockIdx.x] = rz;
     partials[2 * blockIdx.x + 1] = rr;
   }
-  if (dev::arrive_last(ticket, &s_flag)) {
-    const double RZ = dev::sum_partials<kBS>(partials, gridDim.x, 2, s_red);
-    const double RR = dev::sum_partials<kBS>(partials + 1, gridDim.x, 2, s_red);
-    if (threadIdx.x == 0) {
-      const int it = ks->iter + 1;
-      ks->iter = it;
-      ks->rr = RR;
-      if (RR <= ks->stop_rr) {
-        ks->done = 1;
-        ks->status = 0;
-      } else if (it >= ks->maxiter) {
-        ks->done = 1;
-        ks->status = NGPB_E_NOCONV;
-      } else {
-        ks->beta = RZ / ks->rho;
-        ks->rho = RZ;
-      }
-      *ticket = 0;
+}
+
+// stop test on ||r||^2, then beta = (r', z') / (r, z)
+__global__ void __launch_bounds__(kFin) k_fin_update(KrylovScalars* ks, const double* __restrict__ partials, int np) {
+  __shared__ double s_red[kFin / 64];
+  if (ks->done) return;
+  const double RZ = dev::sum_partials<kFin>(partials, np, 2, s_red);
+  const double RR = dev::sum_partials<kFin>(partials + 1, np, 2, s_red);
+  if (threadIdx.x == 0) {
+    const int it = ks->iter + 1;
+    ks->iter = it;
+    ks->rr = RR;
+    if (RR <= ks->stop_rr) {
+      ks->done = 1;
+      ks->status = 0;
+    } else if (it >= ks->maxiter) {
+      ks->done = 1;
+      ks->status = NGPB_E_NOCONV;
+    } else {
+      ks->beta = RZ / ks->rho;
+      ks->rho = RZ;
     }
   }
 }
@@ -176,17 +182,67 @@ __global__ void k_diag_inv(int64_t n, const int32_t* __restrict__ rowptr, const 
   dinv[i] = 1.0 / d;
 }
 
-unsigned vec_grid(int64_t n) { return (unsigned)std::max<int64_t>(1, std::min<int64_t>((n + 4 * kBS - 1) / (4 * kBS), 2048)); }
+__global__ void __launch_bounds__(kBS) k_copy(int64_t n16, const uint4* __restrict__ a, uint4* __restrict__ b) {
+  for (int64_t i = (int64_t)blockIdx.x * kBS + threadIdx.x; i < n16; i += (int64_t)gridDim.x * kBS) b[i] = a[i];
+}
 
 }  // namespace
+
+unsigned vec_grid(int64_t n) {
+  return (unsigned)std::max<int64_t>(1, std::min<int64_t>((n + 4 * kBS - 1) / (4 * kBS), 2048));
+}
+
+void prof_mark(ngpb_ctx* c, int kind) {
+  if (!c->profile) return;
+  auto& p = c->kprof[kind];
+  if (p.used == p.ev.size()) {
+    hipEvent_t e;
+    NGPB_HIP(hipEventCreate(&e));
+    p.ev.push_back(e);
+  }
+  NGPB_HIP(hipEventRecord(p.ev[p.used++], c->stream));
+}
+
+// call after the stream has been synchronized
+void prof_collect(ngpb_ctx* c) {
+  for (auto& p : c->kprof) {
+    for (size_t i = 0; i + 1 < p.used; i += 2) {
+      float ms = 0.f;
+      NGPB_HIP(hipEventElapsedTime(&ms, p.ev[i], p.ev[i + 1]));
+      p.ms += ms;
+      ++p.launches;
+    }
+    p.used = 0;
+  }
+}
+
+// plain device copy of `bytes` (multiple of 16): reads and writes `bytes` each launch
+double time_copy(ngpb_ctx* c, int64_t bytes, int reps) {
+  const int64_t n16 = bytes / 16;
+  DBuf<uint4> a, b;
+  a.resize((size_t)n16);
+  b.resize((size_t)n16);
+  NGPB_HIP(hipMemsetAsync(a.p, 0, n16 * 16, c->stream));
+  const unsigned g = (unsigned)std::min<int64_t>((n16 + kBS - 1) / kBS, 8192);
+  hipEvent_t e0, e1;
+  NGPB_HIP(hipEventCreate(&e0));
+  NGPB_HIP(hipEventCreate(&e1));
+  hipLaunchKernelGGL(k_copy, dim3(g), dim3(kBS), 0, c->stream, n16, a.p, b.p);
+  NGPB_HIP(hipEventRecord(e0, c->stream));
+  for (int i = 0; i < reps; ++i) hipLaunchKernelGGL(k_copy, dim3(g), dim3(kBS), 0, c->stream, n16, a.p, b.p);
+  NGPB_HIP(hipEventRecord(e1, c->stream));
+  NGPB_HIP(hipEventSynchronize(e1));
+  NGPB_HIP(hipGetLastError());
+  float ms = 0.f;
+  NGPB_HIP(hipEventElapsedTime(&ms, e0, e1));
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  return (double)ms / std::max(reps, 1);
+}
 
 void ensure_reduce(ngpb_ctx* c, int64_t blocks) {
   const size_t need = (size_t)std::max<int64_t>(blocks, 2048) * 4;
   if (c->red.partials.n < need) c->red.partials.resize(need);
-  if (c->red.tickets.n < 16) {
-    c->red.tickets.resize(16);
-    NGPB_HIP(hipMemsetAsync(c->red.tickets.p, 0, 16 * sizeof(unsigned), c->stream));
-  }
   if (!c->ks.p) c->ks.resize(1);
   if (!c->ks_host) NGPB_HIP(hipHostMalloc((void**)&c->ks_host, sizeof(KrylovScalars), hipHostMallocDefault));
 }
@@ -198,7 +254,7 @@ double spmv_fp64_bytes(const ngpb_ctx* c) {
 
 void spmv_fp64(ngpb_ctx* c, const double* x, double* y) {
   hipLaunchKernelGGL(k_spmv64<false>, dim3(blocks_for(c->n, kBS)), dim3(kBS), 0, c->stream, c->n, c->rowptr.p, c->col.p,
-                     c->val.p, x, y, nullptr, nullptr, nullptr);
+                     c->val.p, x, y, nullptr, nullptr);
   NGPB_HIP(hipGetLastError());
 }
 
@@ -246,27 +302,31 @@ void solve_fp64(ngpb_ctx* c, const ngpb_solve_opts& o, ngpb_solve_stats& st) {
   const unsigned gs = blocks_for(n, kBS), gv = vec_grid(n);
   ensure_reduce(c, std::max<int64_t>(gs, 2 * (int64_t)gv));
   double* part = c->red.partials.p;
-  unsigned* tk = c->red.tickets.p;
   KrylovScalars* ks = c->ks.p;
   hipEvent_t e0, e1;
   NGPB_HIP(hipEventCreate(&e0));
   NGPB_HIP(hipEventCreate(&e1));
   NGPB_HIP(hipEventRecord(e0, c->stream));
-  hipLaunchKernelGGL(k_pcg_init, dim3(gv), dim3(kBS), 0, c->stream, n, c->rhs.p, c->dinv.p, c->x.p, c->r.p, c->p.p, ks,
-                     o.tol * o.tol, o.maxiter, part, tk + T_INIT);
+  hipLaunchKernelGGL(k_pcg_init, dim3(gv), dim3(kBS), 0, c->stream, n, c->rhs.p, c->dinv.p, c->x.p, c->r.p, c->p.p, part);
+  hipLaunchKernelGGL(k_fin_init, dim3(1), dim3(kFin), 0, c->stream, ks, part, (int)gv, o.tol * o.tol, o.maxiter);
   NGPB_HIP(hipGetLastError());
   const int chunk = std::max(1, o.check_every);
   while (true) {
     for (int k = 0; k < chunk; ++k) {
+      prof_mark(c, 0);
       hipLaunchKernelGGL(k_spmv64<true>, dim3(gs), dim3(kBS), 0, c->stream, n, c->rowptr.p, c->col.p, c->val.p, c->p.p,
-                         c->q.p, ks, part, tk + T_SPMV);
+                         c->q.p, ks, part);
+      prof_mark(c, 0);
+      hipLaunchKernelGGL(k_fin_alpha, dim3(1), dim3(kFin), 0, c->stream, ks, part, (int)gs);
       hipLaunchKernelGGL(k_pcg_update, dim3(gv), dim3(kBS), 0, c->stream, n, c->dinv.p, c->x.p, c->r.p, c->p.p, c->q.p,
-                         ks, part, tk + T_UPD);
+                         ks, part);
+      hipLaunchKernelGGL(k_fin_update, dim3(1), dim3(kFin), 0, c->stream, ks, part, (int)gv);
       hipLaunchKernelGGL(k_pcg_dir, dim3(gv), dim3(kBS), 0, c->stream, n, c->dinv.p, c->r.p, c->p.p, ks);
     }
     NGPB_HIP(hipGetLastError());
     NGPB_HIP(hipMemcpyAsync(c->ks_host, ks, sizeof(KrylovScalars), hipMemcpyDeviceToHost, c->stream));
     NGPB_HIP(hipStreamSynchronize(c->stream));
+    prof_collect(c);
     if (c->ks_host->done) break;
   }
   NGPB_HIP(hipEventRecord(e1, c->stream));

@@ -2,12 +2,17 @@
 // replacement, FP64 reductions.
 //
 // Operator. With D = diag(A) and S = D^-1/2, CG on Ahat = S A S (unit diagonal) is
-// Jacobi-preconditioned CG on A. Ahat's off-diagonal part is stored as SELL-64: a
-// slice is 64 consecutive rows (one wavefront, one row per lane) padded to the
-// slice's longest row; slot k of a slice holds the k-th off-diagonal (ascending
-// column) of each row as an FP32 value and a 16-bit offset from the slot's 32-bit
-// base column. Slices whose columns in some slot span 2^16 or more keep 32-bit
-// columns instead. The diagonal is not stored. Row sums accumulate in FP64.
+// Jacobi-preconditioned CG on A. Ahat's off-diagonal part is stored as SELL-64 plus
+// an overflow list:
+//   * a slice is 64 consecutive rows (one wavefront, one row per lane);
+//   * the slice keeps w "main" slots per row, w chosen per slice to minimise bytes
+//     (padding of short rows against overflow of long rows); slot k holds the k-th
+//     off-diagonal (ascending column) of each row as an FP32 value and a 16-bit
+//     offset from the slot's 32-bit base column (slices whose columns in some slot
+//     span 2^16 or more keep 32-bit columns instead);
+//   * the off-diagonals of a row beyond its w main slots go to the slice's overflow
+//     list (FP32 value + 32-bit column, rows in lane order, an 8-bit count per row).
+// The diagonal is not stored. Row sums accumulate in FP64 and are rounded once.
 //
 // Solve (reliable updates in the sense of Sleijpen & van der Vorst, 1996):
 //   x, b, the true residual r = b - A x and all dot products are FP64; the inner
@@ -16,7 +21,7 @@
 //   replacement (or below the convergence target); then x += S xhat, the FP64
 //   residual is recomputed with the FP64 CSR operator, rhat = S r, and CG continues
 //   with the same direction p. The solve stops when ||b - A x||_2 <= tol ||b||_2 in
-//   FP64.
+//   FP64. Reductions use per-block partials and one-block finalize kernels.
 #include <algorithm>
 #include <cmath>
 
@@ -24,14 +29,13 @@
 
 namespace ngpb {
 
-void check_operator(ngpb_ctx* c);
+void spmv_fp64(ngpb_ctx* c, const double* x, double* y);
 
 namespace {
 
 constexpr int kBS = 256;
 constexpr int kW = 64;
-
-enum { T_SPMV = 4, T_UPD = 5, T_RES = 6 };
+constexpr int kFin = 1024;
 
 __device__ __forceinline__ int lane_id() { return threadIdx.x & 63; }
 
@@ -45,6 +49,21 @@ __device__ __forceinline__ int wave_max_i(int v) {
   for (int off = 32; off > 0; off >>= 1) v = max(v, __shfl_xor(v, off, 64));
   return v;
 }
+__device__ __forceinline__ int wave_sum_i(int v) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+  return v;
+}
+__device__ __forceinline__ int wave_excl_scan_i(int v) {
+  const int lane = lane_id();
+  int inc = v;
+#pragma unroll
+  for (int off = 1; off < 64; off <<= 1) {
+    const int t = __shfl_up(inc, off, 64);
+    if (lane >= off) inc += t;
+  }
+  return inc - v;
+}
 
 __global__ void k_scale(int64_t n, const int32_t* __restrict__ rowptr, const int32_t* __restrict__ col,
                         const double* __restrict__ val, double* __restrict__ dscale) {
@@ -56,43 +75,60 @@ __global__ void k_scale(int64_t n, const int32_t* __restrict__ rowptr, const int
   dscale[i] = 1.0 / sqrt(d);
 }
 
-// k-th off-diagonal entry index of row i (CSR minus the diagonal), or -1
-__device__ __forceinline__ int64_t offdiag_entry(const int32_t* rowptr, const int32_t* col, int64_t i, int k) {
+// position of the diagonal in row i (or -1) and the number of off-diagonals
+__device__ __forceinline__ void row_info(const int32_t* rowptr, const int32_t* col, int64_t i, int64_t* dpos, int* cnt) {
   const int64_t b = rowptr[i], e = rowptr[i + 1];
-  int64_t idx = b + k;
-  // skip the diagonal if it precedes position idx
-  int64_t dpos = -1;
+  int64_t d = -1;
   for (int64_t t = b; t < e; ++t)
-    if (col[t] == i) { dpos = t; break; }
-  if (dpos >= 0 && dpos <= idx) ++idx;
-  return idx < e ? idx : -1;
+    if (col[t] == i) { d = t; break; }
+  *dpos = d;
+  *cnt = (int)(e - b) - (d >= 0 ? 1 : 0);
 }
 
-// one wave per slice: width and whether some slot needs 32-bit columns
+// CSR index of the k-th off-diagonal of a row
+__device__ __forceinline__ int64_t offdiag_at(int64_t b, int64_t dpos, int k) {
+  const int64_t idx = b + k;
+  return (dpos >= 0 && dpos <= idx) ? idx + 1 : idx;
+}
+
+// one wave per slice: main width, 32-bit flag, overflow size
 __global__ void __launch_bounds__(kBS) k_sell_shape(int64_t n, int64_t nslices, const int32_t* __restrict__ rowptr,
                                                     const int32_t* __restrict__ col, int32_t* __restrict__ width,
-                                                    int64_t* __restrict__ wcount, int64_t* __restrict__ wide_count) {
+                                                    int64_t* __restrict__ wcount, int64_t* __restrict__ wide_count,
+                                                    int64_t* __restrict__ ov_count, int* __restrict__ bad) {
   const int64_t s = (int64_t)blockIdx.x * (kBS / kW) + (threadIdx.x >> 6);
   if (s >= nslices) return;
   const int64_t row = s * kW + lane_id();
   int cnt = 0;
-  if (row < n) {
-    cnt = rowptr[row + 1] - rowptr[row];
-    for (int64_t t = rowptr[row]; t < rowptr[row + 1]; ++t)
-      if (col[t] == row) { --cnt; break; }
+  int64_t dpos = -1;
+  if (row < n) row_info(rowptr, col, row, &dpos, &cnt);
+  const int mx = wave_max_i(cnt);
+  // bytes of main slots (6 per entry) against overflow entries (8 each + 1 per row count)
+  int best_w = mx;
+  long long best = (long long)mx * kW * 6;
+  for (int w = mx - 1; w >= 0; --w) {
+    const int extra = wave_sum_i(max(0, cnt - w));
+    const long long cost = (long long)w * kW * 6 + (long long)extra * 8 + kW;
+    if (cost < best) {
+      best = cost;
+      best_w = w;
+    }
   }
-  const int w = wave_max_i(cnt);
+  if (cnt - best_w > 255) atomicOr(bad, 1);
   int wide = 0;
-  for (int k = 0; k < w; ++k) {
-    const int64_t e = (row < n) ? offdiag_entry(rowptr, col, row, k) : -1;
-    const int cmin = wave_min_i(e >= 0 ? col[e] : INT32_MAX);
-    const int cmax = wave_max_i(e >= 0 ? col[e] : INT32_MIN);
+  const int64_t b = row < n ? rowptr[row] : 0;
+  for (int k = 0; k < best_w; ++k) {
+    const int cc = (row < n && k < cnt) ? col[offdiag_at(b, dpos, k)] : INT32_MAX;
+    const int cmin = wave_min_i(cc);
+    const int cmax = wave_max_i(cc == INT32_MAX ? INT32_MIN : cc);
     if (cmin != INT32_MAX && (int64_t)cmax - (int64_t)cmin > 65535) wide = 1;
   }
+  const int extra = wave_sum_i(max(0, cnt - best_w));
   if (lane_id() == 0) {
-    width[s] = w;
-    wcount[s] = w;
-    wide_count[s] = wide ? (int64_t)w * kW : 0;
+    width[s] = best_w;
+    wcount[s] = best_w;
+    wide_count[s] = wide ? (int64_t)best_w * kW : 0;
+    ov_count[s] = extra;
   }
 }
 
@@ -100,26 +136,37 @@ __global__ void __launch_bounds__(kBS) k_sell_fill(int64_t n, int64_t nslices, c
                                                    const int32_t* __restrict__ col, const double* __restrict__ val,
                                                    const double* __restrict__ dscale, const int32_t* __restrict__ width,
                                                    const int64_t* __restrict__ base_ptr,
-                                                   const int64_t* __restrict__ wide_ptr, const int64_t* __restrict__ wide_cnt,
+                                                   const int64_t* __restrict__ wide_off,
+                                                   const int64_t* __restrict__ wide_cnt,
+                                                   const int64_t* __restrict__ ov_off, const int64_t* __restrict__ ov_cnt_s,
                                                    float* __restrict__ sval, uint16_t* __restrict__ soff,
                                                    int32_t* __restrict__ sbase, int32_t* __restrict__ scol32,
-                                                   int64_t* __restrict__ slice_ptr_out, int64_t* __restrict__ wide_ptr_out) {
+                                                   int64_t* __restrict__ wide_ptr, int64_t* __restrict__ ov_ptr,
+                                                   uint8_t* __restrict__ ov_cnt, float* __restrict__ ov_val,
+                                                   int32_t* __restrict__ ov_col) {
   const int64_t s = (int64_t)blockIdx.x * (kBS / kW) + (threadIdx.x >> 6);
   if (s >= nslices) return;
   const int lane = lane_id();
   const int64_t row = s * kW + lane;
   const int w = width[s];
   const int64_t bp = base_ptr[s];
-  const int64_t sp = bp * kW;  // entries before this slice
+  const int64_t sp = bp * kW;
   const bool wide = wide_cnt[s] > 0;
-  const int64_t wp = wide ? wide_ptr[s] : -1;
+  const int64_t wp = wide ? wide_off[s] : -1;
+  const bool has_ov = ov_cnt_s[s] > 0;
   if (lane == 0) {
-    slice_ptr_out[s] = sp;
-    wide_ptr_out[s] = wp;
+    wide_ptr[s] = wp;
+    ov_ptr[s] = has_ov ? ov_off[s] : -1;
+  }
+  int cnt = 0;
+  int64_t dpos = -1, b = 0;
+  if (row < n) {
+    row_info(rowptr, col, row, &dpos, &cnt);
+    b = rowptr[row];
   }
   const double di = row < n ? dscale[row] : 0.0;
   for (int k = 0; k < w; ++k) {
-    const int64_t e = (row < n) ? offdiag_entry(rowptr, col, row, k) : -1;
+    const int64_t e = (row < n && k < cnt) ? offdiag_at(b, dpos, k) : -1;
     const int cc = e >= 0 ? col[e] : INT32_MAX;
     const int cmin = wave_min_i(cc);
     const int base = (cmin == INT32_MAX) ? 0 : cmin;
@@ -131,43 +178,68 @@ __global__ void __launch_bounds__(kBS) k_sell_fill(int64_t n, int64_t nslices, c
     if (lane == 0) sbase[bp + k] = base;
     if (wide) scol32[wp + (int64_t)k * kW + lane] = e >= 0 ? cc : base;
   }
+  const int extra = max(0, cnt - w);
+  if (row < n) ov_cnt[row] = (uint8_t)extra;
+  if (has_ov) {
+    const int64_t start = ov_off[s] + wave_excl_scan_i(extra);
+    for (int k = 0; k < extra; ++k) {
+      const int64_t e = offdiag_at(b, dpos, w + k);
+      const int cc = col[e];
+      ov_val[start + k] = (float)(val[e] * di * dscale[cc]);
+      ov_col[start + k] = cc;
+    }
+  }
 }
 
-// q = Ahat p (unit diagonal + SELL off-diagonals); optional (p, q) reduction
+struct Sell {
+  int64_t n, nslices;
+  const int32_t* width;
+  const int64_t* base_ptr;
+  const int64_t* wide_ptr;
+  const int64_t* ov_ptr;
+  const float* val;
+  const uint16_t* off;
+  const int32_t* base;
+  const int32_t* col32;
+  const uint8_t* ov_cnt;
+  const float* ov_val;
+  const int32_t* ov_col;
+};
+
+// q = Ahat p (unit diagonal + off-diagonals); optional per-block (p, q)
 template <bool DOT>
-__global__ void __launch_bounds__(kBS) k_sell_spmv(int64_t n, int64_t nslices, const int32_t* __restrict__ width,
-                                                   const int64_t* __restrict__ slice_ptr,
-                                                   const int64_t* __restrict__ base_ptr,
-                                                   const int64_t* __restrict__ wide_ptr, const float* __restrict__ sval,
-                                                   const uint16_t* __restrict__ soff, const int32_t* __restrict__ sbase,
-                                                   const int32_t* __restrict__ scol32, const float* __restrict__ x,
-                                                   float* __restrict__ y, KrylovScalars* ks, double* partials,
-                                                   unsigned* ticket) {
+__global__ void __launch_bounds__(kBS) k_sell_spmv(Sell A, const float* __restrict__ x, float* __restrict__ y,
+                                                   const KrylovScalars* ks, double* __restrict__ partials) {
   __shared__ double s_red[kBS / 64];
-  __shared__ int s_flag;
   if (DOT && ks->done) return;
   const int64_t s = __builtin_amdgcn_readfirstlane((int)(blockIdx.x * (kBS / kW) + (threadIdx.x >> 6)));
   const int lane = lane_id();
   const int64_t row = s * kW + lane;
   double part = 0.0;
-  if (s < nslices) {
-    const int w = width[s];
-    const int64_t sp = slice_ptr[s], bp = base_ptr[s], wp = wide_ptr[s];
-    const float xi = row < n ? x[row] : 0.f;
+  if (s < A.nslices) {
+    const int w = A.width[s];
+    const int64_t bp = A.base_ptr[s], wp = A.wide_ptr[s], op = A.ov_ptr[s];
+    const int64_t sp = bp * kW;
+    const float xi = row < A.n ? x[row] : 0.f;
     double acc = (double)xi;
     if (wp < 0) {
       for (int k = 0; k < w; ++k) {
         const int64_t idx = sp + (int64_t)k * kW + lane;
-        const int c = sbase[bp + k] + (int)soff[idx];
-        acc = fma((double)sval[idx], (double)x[c], acc);
+        const int c = A.base[bp + k] + (int)A.off[idx];
+        acc = fma((double)A.val[idx], (double)x[c], acc);
       }
     } else {
       for (int k = 0; k < w; ++k) {
         const int64_t idx = sp + (int64_t)k * kW + lane;
-        acc = fma((double)sval[idx], (double)x[scol32[wp + (int64_t)k * kW + lane]], acc);
+        acc = fma((double)A.val[idx], (double)x[A.col32[wp + (int64_t)k * kW + lane]], acc);
       }
     }
-    if (row < n) {
+    if (op >= 0) {
+      const int cnt = row < A.n ? (int)A.ov_cnt[row] : 0;
+      const int64_t start = op + wave_excl_scan_i(cnt);
+      for (int k = 0; k < cnt; ++k) acc = fma((double)A.ov_val[start + k], (double)x[A.ov_col[start + k]], acc);
+    }
+    if (row < A.n) {
       const float yi = (float)acc;
       y[row] = yi;
       part = (double)xi * (double)yi;
@@ -176,23 +248,24 @@ __global__ void __launch_bounds__(kBS) k_sell_spmv(int64_t n, int64_t nslices, c
   if (DOT) {
     part = dev::block_sum<kBS>(part, s_red);
     if (threadIdx.x == 0) partials[blockIdx.x] = part;
-    if (dev::arrive_last(ticket, &s_flag)) {
-      const double pq = dev::sum_partials<kBS>(partials, gridDim.x, 1, s_red);
-      if (threadIdx.x == 0) {
-        ks->pq = pq;
-        ks->alpha = ks->rho / pq;
-        *ticket = 0;
-      }
-    }
   }
 }
 
-// xhat += alpha p; rhat -= alpha q; rr = (rhat, rhat); stop test; beta
+__global__ void __launch_bounds__(kFin) k_mx_fin_alpha(KrylovScalars* ks, const double* __restrict__ partials, int np) {
+  __shared__ double s_red[kFin / 64];
+  if (ks->done) return;
+  const double pq = dev::sum_partials<kFin>(partials, np, 1, s_red);
+  if (threadIdx.x == 0) {
+    ks->pq = pq;
+    ks->alpha = ks->rho / pq;
+  }
+}
+
+// xhat += alpha p; rhat -= alpha q; per block (rhat, rhat)
 __global__ void __launch_bounds__(kBS) k_mx_update(int64_t n, float* __restrict__ xs, float* __restrict__ rs,
                                                    const float* __restrict__ ps, const float* __restrict__ qs,
-                                                   KrylovScalars* ks, double* partials, unsigned* ticket) {
+                                                   const KrylovScalars* ks, double* __restrict__ partials) {
   __shared__ double s_red[kBS / 64];
-  __shared__ int s_flag;
   if (ks->done) return;
   const float alpha = (float)ks->alpha;
   double rr = 0.0;
@@ -204,19 +277,21 @@ __global__ void __launch_bounds__(kBS) k_mx_update(int64_t n, float* __restrict_
   }
   rr = dev::block_sum<kBS>(rr, s_red);
   if (threadIdx.x == 0) partials[blockIdx.x] = rr;
-  if (dev::arrive_last(ticket, &s_flag)) {
-    const double RR = dev::sum_partials<kBS>(partials, gridDim.x, 1, s_red);
-    if (threadIdx.x == 0) {
-      const int it = ks->iter + 1;
-      ks->iter = it;
-      ks->rr = RR;
-      if (RR <= ks->stop_rr || it >= ks->maxiter) {
-        ks->done = 1;
-      } else {
-        ks->beta = RR / ks->rho;
-        ks->rho = RR;
-      }
-      *ticket = 0;
+}
+
+__global__ void __launch_bounds__(kFin) k_mx_fin_update(KrylovScalars* ks, const double* __restrict__ partials, int np) {
+  __shared__ double s_red[kFin / 64];
+  if (ks->done) return;
+  const double RR = dev::sum_partials<kFin>(partials, np, 1, s_red);
+  if (threadIdx.x == 0) {
+    const int it = ks->iter + 1;
+    ks->iter = it;
+    ks->rr = RR;
+    if (RR <= ks->stop_rr || it >= ks->maxiter) {
+      ks->done = 1;
+    } else {
+      ks->beta = RR / ks->rho;
+      ks->rho = RR;
     }
   }
 }
@@ -238,14 +313,13 @@ __global__ void __launch_bounds__(kBS) k_mx_accum(int64_t n, const double* __res
   }
 }
 
-// r = b - t (t = A x in FP64); rhat = S r; reductions (r, r) and (rhat, rhat).
-// first: also x = 0, xhat = 0, p = 0, bb = (b, b), beta = 0.
+// r = b - t (t = A x in FP64); rhat = S r; per block (r, r), (rhat, rhat).
+// first: r = b and x = 0, xhat = 0, p = 0.
 __global__ void __launch_bounds__(kBS) k_mx_resid(int64_t n, const double* __restrict__ b, const double* __restrict__ t,
                                                   const double* __restrict__ dscale, float* __restrict__ rs,
                                                   double* __restrict__ x, float* __restrict__ xs, float* __restrict__ ps,
-                                                  int first, KrylovScalars* ks, double* partials, unsigned* ticket) {
+                                                  int first, double* __restrict__ partials) {
   __shared__ double s_red[kBS / 64];
-  __shared__ int s_flag;
   double rr = 0.0, ss = 0.0;
   for (int64_t i = (int64_t)blockIdx.x * kBS + threadIdx.x; i < n; i += (int64_t)gridDim.x * kBS) {
     double ri;
@@ -268,23 +342,25 @@ __global__ void __launch_bounds__(kBS) k_mx_resid(int64_t n, const double* __res
     partials[2 * blockIdx.x] = rr;
     partials[2 * blockIdx.x + 1] = ss;
   }
-  if (dev::arrive_last(ticket, &s_flag)) {
-    const double RR = dev::sum_partials<kBS>(partials, gridDim.x, 2, s_red);
-    const double SS = dev::sum_partials<kBS>(partials + 1, gridDim.x, 2, s_red);
-    if (threadIdx.x == 0) {
-      if (first) {
-        ks->bb = RR;
-        ks->beta = 0.0;
-        ks->iter = 0;
-      } else {
-        ks->beta = ks->rho > 0.0 ? SS / ks->rho : 0.0;
-      }
-      ks->aux[0] = RR;  // true FP64 residual norm^2
-      ks->aux[1] = SS;  // scaled residual norm^2 after replacement
-      ks->rho = SS;
-      ks->rr = SS;
-      *ticket = 0;
+}
+
+__global__ void __launch_bounds__(kFin) k_mx_fin_resid(KrylovScalars* ks, const double* __restrict__ partials, int np,
+                                                       int first) {
+  __shared__ double s_red[kFin / 64];
+  const double RR = dev::sum_partials<kFin>(partials, np, 2, s_red);
+  const double SS = dev::sum_partials<kFin>(partials + 1, np, 2, s_red);
+  if (threadIdx.x == 0) {
+    if (first) {
+      ks->bb = RR;
+      ks->beta = 0.0;
+      ks->iter = 0;
+    } else {
+      ks->beta = ks->rho > 0.0 ? SS / ks->rho : 0.0;
     }
+    ks->aux[0] = RR;  // true FP64 residual norm^2
+    ks->aux[1] = SS;  // scaled residual norm^2 after replacement
+    ks->rho = SS;
+    ks->rr = SS;
   }
 }
 
@@ -299,18 +375,32 @@ __global__ void k_from_scaled(int64_t n, const float* __restrict__ ys, const dou
   if (i < n) y[i] = (double)ys[i] / dscale[i];
 }
 
-unsigned vec_grid(int64_t n) { return (unsigned)std::max<int64_t>(1, std::min<int64_t>((n + 4 * kBS - 1) / (4 * kBS), 2048)); }
+Sell sell_view(const ngpb_ctx* c) {
+  Sell A;
+  A.n = c->n;
+  A.nslices = c->n_slices;
+  A.width = c->slice_width.p;
+  A.base_ptr = c->slice_base_ptr.p;
+  A.wide_ptr = c->slice_wide_ptr.p;
+  A.ov_ptr = c->slice_ov_ptr.p;
+  A.val = c->sell_val.p;
+  A.off = c->sell_off.p;
+  A.base = c->sell_base.p;
+  A.col32 = c->sell_col32.p;
+  A.ov_cnt = c->sell_ov_cnt.p;
+  A.ov_val = c->sell_ov_val.p;
+  A.ov_col = c->sell_ov_col.p;
+  return A;
+}
 
-void launch_sell(ngpb_ctx* c, bool dot, const float* x, float* y, double* part, unsigned* tk) {
-  const unsigned g = (unsigned)((c->n_slices + (kBS / kW) - 1) / (kBS / kW));
+unsigned sell_grid(const ngpb_ctx* c) { return (unsigned)((c->n_slices + (kBS / kW) - 1) / (kBS / kW)); }
+
+void launch_sell(ngpb_ctx* c, bool dot, const float* x, float* y, double* part) {
   if (dot)
-    hipLaunchKernelGGL(k_sell_spmv<true>, dim3(g), dim3(kBS), 0, c->stream, c->n, c->n_slices, c->slice_width.p,
-                       c->slice_ptr.p, c->slice_base_ptr.p, c->slice_wide_ptr.p, c->sell_val.p, c->sell_off.p,
-                       c->sell_base.p, c->sell_col32.p, x, y, c->ks.p, part, tk);
+    hipLaunchKernelGGL(k_sell_spmv<true>, dim3(sell_grid(c)), dim3(kBS), 0, c->stream, sell_view(c), x, y, c->ks.p, part);
   else
-    hipLaunchKernelGGL(k_sell_spmv<false>, dim3(g), dim3(kBS), 0, c->stream, c->n, c->n_slices, c->slice_width.p,
-                       c->slice_ptr.p, c->slice_base_ptr.p, c->slice_wide_ptr.p, c->sell_val.p, c->sell_off.p,
-                       c->sell_base.p, c->sell_col32.p, x, y, nullptr, nullptr, nullptr);
+    hipLaunchKernelGGL(k_sell_spmv<false>, dim3(sell_grid(c)), dim3(kBS), 0, c->stream, sell_view(c), x, y, nullptr,
+                       nullptr);
 }
 
 }  // namespace
@@ -323,41 +413,57 @@ void build_compact(ngpb_ctx* c) {
   hipLaunchKernelGGL(k_scale, dim3(blocks_for(n, 256)), dim3(256), 0, c->stream, n, c->rowptr.p, c->col.p, c->val.p,
                      c->dscale.p);
   c->slice_width.resize((size_t)ns);
-  DBuf<int64_t> wcount, widec;
+  DBuf<int64_t> wcount, widec, wide_off, ovc, ov_off;
+  DBuf<int> bad;
   wcount.resize((size_t)ns);
   widec.resize((size_t)ns);
+  wide_off.resize((size_t)ns);
+  ovc.resize((size_t)ns);
+  ov_off.resize((size_t)ns);
+  bad.resize(1);
+  NGPB_HIP(hipMemsetAsync(bad.p, 0, sizeof(int), c->stream));
   const unsigned g = (unsigned)((ns + (kBS / kW) - 1) / (kBS / kW));
   hipLaunchKernelGGL(k_sell_shape, dim3(g), dim3(kBS), 0, c->stream, n, ns, c->rowptr.p, c->col.p, c->slice_width.p,
-                     wcount.p, widec.p);
+                     wcount.p, widec.p, ovc.p, bad.p);
   NGPB_HIP(hipGetLastError());
-  DBuf<int64_t> wide_off;
-  wide_off.resize((size_t)ns);
+  int hb = 0;
+  NGPB_HIP(hipMemcpyAsync(&hb, bad.p, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+  NGPB_HIP(hipStreamSynchronize(c->stream));
+  NGPB_REQUIRE(hb == 0, NGPB_E_ARG, "a row has more than 255 off-diagonal overflow entries");
   const int64_t nbases = exclusive_scan(c, wcount.p, wcount.p, ns);  // wcount -> base_ptr
   const int64_t nwide = exclusive_scan(c, widec.p, wide_off.p, ns);
+  const int64_t nov = exclusive_scan(c, ovc.p, ov_off.p, ns);
   c->sell_bases = nbases;
   c->sell_entries = nbases * kW;
   c->sell_wide_entries = nwide;
+  c->sell_ov_entries = nov;
   c->slice_base_ptr.resize((size_t)ns);
   NGPB_HIP(hipMemcpyAsync(c->slice_base_ptr.p, wcount.p, ns * sizeof(int64_t), hipMemcpyDeviceToDevice, c->stream));
-  c->slice_ptr.resize((size_t)ns);
   c->slice_wide_ptr.resize((size_t)ns);
+  c->slice_ov_ptr.resize((size_t)ns);
   c->sell_val.resize((size_t)std::max<int64_t>(c->sell_entries, 1));
   c->sell_off.resize((size_t)std::max<int64_t>(c->sell_entries, 1));
   c->sell_base.resize((size_t)std::max<int64_t>(nbases, 1));
   c->sell_col32.resize((size_t)std::max<int64_t>(nwide, 1));
+  c->sell_ov_cnt.resize((size_t)n);
+  c->sell_ov_val.resize((size_t)std::max<int64_t>(nov, 1));
+  c->sell_ov_col.resize((size_t)std::max<int64_t>(nov, 1));
   hipLaunchKernelGGL(k_sell_fill, dim3(g), dim3(kBS), 0, c->stream, n, ns, c->rowptr.p, c->col.p, c->val.p, c->dscale.p,
-                     c->slice_width.p, c->slice_base_ptr.p, wide_off.p, widec.p, c->sell_val.p, c->sell_off.p,
-                     c->sell_base.p, c->sell_col32.p, c->slice_ptr.p, c->slice_wide_ptr.p);
+                     c->slice_width.p, c->slice_base_ptr.p, wide_off.p, widec.p, ov_off.p, ovc.p, c->sell_val.p,
+                     c->sell_off.p, c->sell_base.p, c->sell_col32.p, c->slice_wide_ptr.p, c->slice_ov_ptr.p,
+                     c->sell_ov_cnt.p, c->sell_ov_val.p, c->sell_ov_col.p);
   NGPB_HIP(hipGetLastError());
   NGPB_HIP(hipStreamSynchronize(c->stream));
   c->have_compact = true;
 }
 
 double spmv_compact_bytes(const ngpb_ctx* c) {
-  // values (4) + offsets (2) per stored entry, 4 per slot base, 32-bit columns of
-  // wide slices, slice metadata (width 4 + 3 offsets x 8), x read once, y written
+  // main entries (4 value + 2 offset), slot bases (4), 32-bit columns of wide slices,
+  // overflow entries (4 + 4), slice metadata (width 4 + three 8-byte offsets), the
+  // overflow counts of slices that have overflow (64 per slice, counted for all rows
+  // as an upper bound), x read once and y written
   return (double)c->sell_entries * 6.0 + (double)c->sell_bases * 4.0 + (double)c->sell_wide_entries * 4.0 +
-         (double)c->n_slices * 28.0 + (double)c->n * 8.0;
+         (double)c->sell_ov_entries * 8.0 + (double)c->n_slices * 28.0 + (double)c->n * (4.0 + 4.0);
 }
 
 void spmv_compact(ngpb_ctx* c, const double* x, double* y) {
@@ -366,7 +472,7 @@ void spmv_compact(ngpb_ctx* c, const double* x, double* y) {
   c->ps.reserve((size_t)n);
   c->qs.reserve((size_t)n);
   hipLaunchKernelGGL(k_to_scaled, dim3(blocks_for(n, 256)), dim3(256), 0, c->stream, n, x, c->dscale.p, c->ps.p);
-  launch_sell(c, false, c->ps.p, c->qs.p, nullptr, nullptr);
+  launch_sell(c, false, c->ps.p, c->qs.p, nullptr);
   hipLaunchKernelGGL(k_from_scaled, dim3(blocks_for(n, 256)), dim3(256), 0, c->stream, n, c->qs.p, c->dscale.p, y);
   NGPB_HIP(hipGetLastError());
 }
@@ -379,9 +485,9 @@ double time_spmv_compact(ngpb_ctx* c, int reps) {
   hipEvent_t e0, e1;
   NGPB_HIP(hipEventCreate(&e0));
   NGPB_HIP(hipEventCreate(&e1));
-  launch_sell(c, false, c->ps.p, c->qs.p, nullptr, nullptr);
+  launch_sell(c, false, c->ps.p, c->qs.p, nullptr);
   NGPB_HIP(hipEventRecord(e0, c->stream));
-  for (int i = 0; i < reps; ++i) launch_sell(c, false, c->ps.p, c->qs.p, nullptr, nullptr);
+  for (int i = 0; i < reps; ++i) launch_sell(c, false, c->ps.p, c->qs.p, nullptr);
   NGPB_HIP(hipEventRecord(e1, c->stream));
   NGPB_HIP(hipEventSynchronize(e1));
   float ms = 0.f;
@@ -391,8 +497,6 @@ double time_spmv_compact(ngpb_ctx* c, int reps) {
   NGPB_HIP(hipGetLastError());
   return (double)ms / std::max(reps, 1);
 }
-
-void spmv_fp64(ngpb_ctx* c, const double* x, double* y);
 
 void solve_mixed(ngpb_ctx* c, const ngpb_solve_opts& o, ngpb_solve_stats& st) {
   const int64_t n = c->n;
@@ -405,10 +509,9 @@ void solve_mixed(ngpb_ctx* c, const ngpb_solve_opts& o, ngpb_solve_stats& st) {
   c->ps.reserve((size_t)n);
   c->qs.reserve((size_t)n);
   const unsigned gv = vec_grid(n);
-  const unsigned gs = (unsigned)((c->n_slices + (kBS / kW) - 1) / (kBS / kW));
+  const unsigned gs = sell_grid(c);
   ensure_reduce(c, std::max<int64_t>(gs, 2 * (int64_t)gv));
   double* part = c->red.partials.p;
-  unsigned* tk = c->red.tickets.p;
   KrylovScalars* ks = c->ks.p;
   KrylovScalars* h = c->ks_host;
   const double drop = (o.drop > 0.0 && o.drop < 1.0) ? o.drop : 0.05;
@@ -419,7 +522,8 @@ void solve_mixed(ngpb_ctx* c, const ngpb_solve_opts& o, ngpb_solve_stats& st) {
   NGPB_HIP(hipEventCreate(&e1));
   NGPB_HIP(hipEventRecord(e0, c->stream));
   hipLaunchKernelGGL(k_mx_resid, dim3(gv), dim3(kBS), 0, c->stream, n, c->rhs.p, (const double*)nullptr, c->dscale.p,
-                     c->rs.p, c->x.p, c->xs.p, c->ps.p, 1, ks, part, tk + T_RES);
+                     c->rs.p, c->x.p, c->xs.p, c->ps.p, 1, part);
+  hipLaunchKernelGGL(k_mx_fin_resid, dim3(1), dim3(kFin), 0, c->stream, ks, part, (int)gv, 1);
   NGPB_HIP(hipGetLastError());
   NGPB_HIP(hipMemcpyAsync(h, ks, sizeof(KrylovScalars), hipMemcpyDeviceToHost, c->stream));
   NGPB_HIP(hipStreamSynchronize(c->stream));
@@ -432,8 +536,8 @@ void solve_mixed(ngpb_ctx* c, const ngpb_solve_opts& o, ngpb_solve_stats& st) {
     double prev_true = bb;
     int stagnant = 0;
     while (iters < o.maxiter) {
-      // cycle parameters: stop at drop^2 of the current scaled residual or at the
-      // scaled image of the FP64 target (ratio taken at the last replacement)
+      // cycle: stop at drop^2 of the current scaled residual or at the scaled image
+      // of the FP64 target (ratio of the two norms taken at the last replacement)
       const double ss = h->aux[1], tt = h->aux[0];
       const double ratio = tt > 0.0 ? ss / tt : 1.0;
       h->stop_rr = std::max(drop * drop * ss, 0.25 * target * ratio);
@@ -444,32 +548,41 @@ void solve_mixed(ngpb_ctx* c, const ngpb_solve_opts& o, ngpb_solve_stats& st) {
       hipLaunchKernelGGL(k_mx_dir, dim3(gv), dim3(kBS), 0, c->stream, n, c->rs.p, c->ps.p, ks);
       while (true) {
         for (int k = 0; k < chunk; ++k) {
-          launch_sell(c, true, c->ps.p, c->qs.p, part, tk + T_SPMV);
+          prof_mark(c, 1);
+          launch_sell(c, true, c->ps.p, c->qs.p, part);
+          prof_mark(c, 1);
+          hipLaunchKernelGGL(k_mx_fin_alpha, dim3(1), dim3(kFin), 0, c->stream, ks, part, (int)gs);
           hipLaunchKernelGGL(k_mx_update, dim3(gv), dim3(kBS), 0, c->stream, n, c->xs.p, c->rs.p, c->ps.p, c->qs.p, ks,
-                             part, tk + T_UPD);
+                             part);
+          hipLaunchKernelGGL(k_mx_fin_update, dim3(1), dim3(kFin), 0, c->stream, ks, part, (int)gv);
           hipLaunchKernelGGL(k_mx_dir, dim3(gv), dim3(kBS), 0, c->stream, n, c->rs.p, c->ps.p, ks);
         }
         NGPB_HIP(hipGetLastError());
         NGPB_HIP(hipMemcpyAsync(h, ks, sizeof(KrylovScalars), hipMemcpyDeviceToHost, c->stream));
         NGPB_HIP(hipStreamSynchronize(c->stream));
+        prof_collect(c);
         if (h->done) break;
       }
       iters = h->iter;
       // FP64 residual replacement
       hipLaunchKernelGGL(k_mx_accum, dim3(gv), dim3(kBS), 0, c->stream, n, c->dscale.p, c->x.p, c->xs.p);
+      prof_mark(c, 0);
       spmv_fp64(c, c->x.p, c->q.p);
+      prof_mark(c, 0);
       hipLaunchKernelGGL(k_mx_resid, dim3(gv), dim3(kBS), 0, c->stream, n, c->rhs.p, c->q.p, c->dscale.p, c->rs.p, c->x.p,
-                         c->xs.p, c->ps.p, 0, ks, part, tk + T_RES);
+                         c->xs.p, c->ps.p, 0, part);
+      hipLaunchKernelGGL(k_mx_fin_resid, dim3(1), dim3(kFin), 0, c->stream, ks, part, (int)gv, 0);
       NGPB_HIP(hipGetLastError());
       NGPB_HIP(hipMemcpyAsync(h, ks, sizeof(KrylovScalars), hipMemcpyDeviceToHost, c->stream));
       NGPB_HIP(hipStreamSynchronize(c->stream));
+      prof_collect(c);
       ++repl;
       true_rr = h->aux[0];
       if (true_rr <= target) {
         status = 0;
         break;
       }
-      // the FP32 inner solve can no longer reduce the FP64 residual
+      // the FP32 inner solve no longer reduces the FP64 residual
       if (true_rr > 0.81 * prev_true) {
         if (++stagnant >= 3) break;
       } else {

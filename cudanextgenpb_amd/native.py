@@ -88,6 +88,9 @@ SIGNATURES = {
     "ngpb_spmv": (_I, [_P, _I, _P, _P]),
     "ngpb_energy": (_I, [_P, _P]),
     "ngpb_time_spmv": (_I, [_P, _I, _I, _P, _P]),
+    "ngpb_set_profiling": (_I, [_P, _I]),
+    "ngpb_kernel_stats": (_I, [_P, _I, _P, _P, _P]),
+    "ngpb_time_copy": (_I, [_P, _I64, _I, _P]),
 }
 
 _lib = None
@@ -278,6 +281,20 @@ class Context:
         e = EnergyOut()
         self._check(lib().ngpb_energy(self._h, ctypes.byref(e)), "energy")
         return e.as_dict()
+
+    def set_profiling(self, enable: bool):
+        self._check(lib().ngpb_set_profiling(self._h, 1 if enable else 0), "set_profiling")
+
+    def kernel_stats(self, precision):
+        n, ms, b = ctypes.c_int64(0), ctypes.c_double(0), ctypes.c_double(0)
+        self._check(lib().ngpb_kernel_stats(self._h, precision, ctypes.byref(n), ctypes.byref(ms), ctypes.byref(b)),
+                    "kernel_stats")
+        return dict(launches=n.value, total_ms=ms.value, bytes_per_launch=b.value)
+
+    def time_copy(self, nbytes, reps=20):
+        ms = ctypes.c_double(0)
+        self._check(lib().ngpb_time_copy(self._h, int(nbytes), reps, ctypes.byref(ms)), "time_copy")
+        return ms.value
 
     def time_spmv(self, precision=PREC_FP64, reps=50):
         ms, b = ctypes.c_double(0), ctypes.c_double(0)

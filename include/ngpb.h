@@ -163,9 +163,20 @@ int ngpb_energy(ngpb_ctx* ctx, ngpb_energy_out* out);
 /* ---- measurement ----------------------------------------------------------------- */
 /* Times `reps` back-to-back launches of the SpMV kernel of the given precision on the
    context's stream with HIP events; returns the mean kernel time in ms and the
-   algorithmic bytes of one launch. */
+   algorithmic bytes of one launch. No counterpart in the reference. */
 int ngpb_time_spmv(ngpb_ctx* ctx, int precision, int reps, double* ms_per_launch,
                    double* bytes_per_launch);
+
+/* With profiling on, every SpMV launched by ngpb_solve is bracketed by HIP events on
+   the context's stream; ngpb_kernel_stats returns the launch count, their summed time
+   and the algorithmic bytes of one launch for FP64 (CSR) or MIXED (compact) SpMV.
+   Enabling or disabling resets the counters. No counterpart in the reference. */
+int ngpb_set_profiling(ngpb_ctx* ctx, int enable);
+int ngpb_kernel_stats(ngpb_ctx* ctx, int precision, int64_t* launches, double* total_ms,
+                      double* bytes_per_launch);
+/* Mean time of a plain device copy kernel moving `bytes` (read + write each), used to
+   calibrate hardware traffic counters. No counterpart in the reference. */
+int ngpb_time_copy(ngpb_ctx* ctx, int64_t bytes, int reps, double* ms_per_launch);
 
 #ifdef __cplusplus
 }
