@@ -144,6 +144,9 @@ def main():
     fp64 = None if args.no_fp64 else run_steps(pb, native.PREC_FP64, args.steps, args.tol, args.check_every, True)
 
     n = ctx.n
+    op_info = ctx.operator_info()
+    copy_bytes = 1 << 30
+    copy_ms = ctx.time_copy(copy_bytes, reps=20)
     value = args.steps / mixed["total_s"]
     mixed_solve_ms = mixed["stats"]["ms_solve"]
     mk = mixed["kernel"]
@@ -192,7 +195,9 @@ def main():
             "bytes_per_launch": mk["bytes_per_launch"],
             "avg_launch_ms": avg_ms,
             "launches": mk["launches"],
+            "copy_kernel_gbs": 2.0 * copy_bytes / (copy_ms * 1e-3) / 1e9,
         },
+        "operator": op_info,
     }
     if fp64 is not None:
         fk = fp64["kernel"]

@@ -621,6 +621,27 @@ int ngpb_kernel_stats(ngpb_ctx* c, int precision, int64_t* launches, double* tot
   });
 }
 
+int ngpb_operator_info(ngpb_ctx* c, int64_t* info) {
+  return guarded(c, [&] {
+    NGPB_REQUIRE(info, NGPB_E_ARG, "NULL output");
+    need(c->have_system, "no assembled system");
+    if (!c->have_compact) ngpb::build_compact(c);
+    std::vector<int32_t> w((size_t)c->n_slices);
+    c->slice_width.download(w.data(), w.size(), c->stream);
+    NGPB_HIP(hipStreamSynchronize(c->stream));
+    int64_t nd = 0;
+    for (int32_t v : w) nd += v < 0;
+    info[0] = c->n_slices;
+    info[1] = nd;
+    info[2] = c->sell_delta_entries;
+    info[3] = c->sell_entries - c->sell_delta_entries;
+    info[4] = c->sell_wide_entries;
+    info[5] = c->sell_ov_entries;
+    info[6] = (int64_t)ngpb::spmv_compact_bytes(c);
+    info[7] = (int64_t)ngpb::spmv_fp64_bytes(c);
+  });
+}
+
 int ngpb_time_copy(ngpb_ctx* c, int64_t bytes, int reps, double* ms_per_launch) {
   return guarded(c, [&] {
     NGPB_REQUIRE(ms_per_launch && reps > 0 && bytes >= 16 && bytes % 16 == 0, NGPB_E_ARG,

@@ -101,15 +101,22 @@ struct GridReduce {
   DBuf<double> partials;
 };
 
-// scalar state of a Krylov solve on the device
+// Scalar state of a Krylov solve on the device. Inside an iteration the scalars that
+// one kernel writes while other blocks of the same kernel still read the previous
+// value are double-buffered by the parity of the host's launch index (slot [par] is
+// read, slot [par ^ 1] written).
 struct KrylovScalars {
-  double rho;       // (r, z) of the current iterate
+  double rho;       // (r, z) of the current iterate (outside the iteration loop)
   double pq;        // (p, A p)
   double alpha, beta;
   double rr;        // (r, r) of the current iterate
   double stop_rr;   // stop when rr <= stop_rr
   double bb;        // (b, b)
   double aux[4];
+  double rho2[2];   // (r, z) entering the iteration of parity par
+  double alpha2[2]; // alpha of the iteration of parity par
+  int iter2[2];     // iterations completed before the iteration of parity par
+  int stop_par;     // parity of the iteration that raised the stop flag
   int iter;         // iterations done in this solve
   int done;         // 1 = stop flag raised
   int maxiter;      // iteration limit for the current cycle
@@ -189,7 +196,8 @@ struct ngpb_ctx {
 
   // compact mixed-precision operator
   bool have_compact = false;
-  int64_t n_slices = 0, sell_entries = 0, sell_wide_entries = 0, sell_bases = 0, sell_ov_entries = 0;
+  int64_t n_slices = 0, sell_entries = 0, sell_delta_entries = 0, sell_wide_entries = 0, sell_bases = 0,
+          sell_ov_entries = 0;
   ngpb::DBuf<int32_t> slice_width;    // per slice: main slots
   ngpb::DBuf<int64_t> slice_base_ptr; // per slice: first slot (entries start at 64x this)
   ngpb::DBuf<int64_t> slice_wide_ptr; // per slice: first 32-bit column (or -1)
@@ -221,8 +229,9 @@ struct ngpb_ctx {
   // optional HIP-event timing of SpMV launches (kind 0: FP64 CSR, 1: compact FP32)
   bool profile = false;
   struct KProf {
-    std::vector<hipEvent_t> ev;
-    size_t used = 0;
+    std::vector<hipEvent_t> ev;  // begin/end pairs
+    std::vector<int64_t> tag;    // launch index of each pair
+    size_t used = 0;             // events recorded since the last collection
     int64_t launches = 0;
     double ms = 0.0;
   } kprof[2];
@@ -267,9 +276,11 @@ double time_spmv_compact(ngpb_ctx* c, int reps);
 // energy
 void run_energy(ngpb_ctx* c, ngpb_energy_out* out);
 
-// SpMV launch timing
-void prof_mark(ngpb_ctx* c, int kind);
-void prof_collect(ngpb_ctx* c);
+// SpMV launch timing: every kProfEvery-th launch is bracketed by events
+constexpr int64_t kProfEvery = 8;
+void prof_begin(ngpb_ctx* c, int kind, int64_t tag);
+void prof_end(ngpb_ctx* c, int kind);
+void prof_collect(ngpb_ctx* c, int kind, int64_t limit);
 double time_copy(ngpb_ctx* c, int64_t bytes, int reps);
 
 // launch helper

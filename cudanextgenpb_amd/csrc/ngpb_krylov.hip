@@ -8,14 +8,14 @@
 // results are bit-identical). Blocks whose rows hold more than kCap entries read
 // global memory directly.
 //
-// CG iteration: SpMV + per-block (p, q) -> finalize (alpha) -> x/r update +
-// per-block (r, D^-1 r), (r, r) -> finalize (stop test, beta) -> new direction.
-// Per-block partial sums are combined by a one-block finalize kernel in a fixed
-// order, so reductions are deterministic and no kernel needs a device-wide memory
-// fence (on a multi-XCD GPU an agent-scope fence writes back the XCD's L2, which
-// would serialize every block). Scalars live in device memory; the host polls the
-// stop flag every `check_every` iterations and kernels launched after the stop exit
-// at once.
+// CG iteration, three kernels: SpMV + per-block (p, q) -> x/r update + per-block
+// (r, D^-1 r), (r, r) -> stop test and new direction. Each kernel first combines the
+// previous kernel's per-block partial sums itself, every block in the same fixed
+// order, so all blocks hold identical scalars; reductions are deterministic and no
+// kernel needs a device-wide memory fence (on a multi-XCD GPU an agent-scope fence
+// writes back the XCD's L2, which would serialize every block). Scalars live in
+// device memory; the host polls the stop flag every `check_every` iterations and
+// kernels launched after the stop exit at once.
 #include <algorithm>
 #include <cmath>
 
@@ -37,41 +37,55 @@ __global__ void __launch_bounds__(kBS) k_spmv64(int64_t n, const int32_t* __rest
   __shared__ double s_x[kCap];
   __shared__ double s_red[kBS / 64];
   if (DOT && ks->done) return;
-  const int64_t r0 = (int64_t)blockIdx.x * kBS;
-  const int64_t r1 = min(r0 + kBS, n);
-  const int64_t row = r0 + threadIdx.x;
-  const int64_t e0 = rowptr[r0], e1 = rowptr[r1];
-  double sum = 0.0;
-  if (e1 - e0 <= kCap) {
-    for (int64_t k = e0 + threadIdx.x; k < e1; k += kBS) {
-      s_val[k - e0] = val[k];
-      s_x[k - e0] = x[col[k]];
-    }
-    __syncthreads();
-    if (row < n) {
-      const int64_t b = rowptr[row] - e0, e = rowptr[row + 1] - e0;
-      for (int64_t k = b; k < e; ++k) sum = fma(s_val[k], s_x[k], sum);
+  double part = 0.0;
+  const int64_t nchunks = (n + kBS - 1) / kBS;
+  for (int64_t chunk = blockIdx.x; chunk < nchunks; chunk += gridDim.x) {
+    const int64_t r0 = chunk * kBS;
+    const int64_t r1 = min(r0 + kBS, n);
+    const int64_t row = r0 + threadIdx.x;
+    const int64_t e0 = rowptr[r0], e1 = rowptr[r1];
+    double sum = 0.0;
+    if (e1 > e0 && e1 - e0 <= kCap) {
+      __syncthreads();  // the previous chunk's rows are done with the staging buffers
+      // all loads of the chunk are issued before any is used: indices past the
+      // chunk's last entry are clamped to it (every row holds its diagonal, so the
+      // chunk is not empty) and their LDS slots are never read
+      constexpr int kU = kCap / kBS;
+      double v[kU];
+      int c[kU];
+      const int64_t kl = e1 - 1;
+#pragma unroll
+      for (int u = 0; u < kU; ++u) {
+        const int64_t k = min(e0 + threadIdx.x + (int64_t)u * kBS, kl);
+        v[u] = val[k];
+        c[u] = col[k];
+      }
+#pragma unroll
+      for (int u = 0; u < kU; ++u) {
+        const int64_t k = e0 + threadIdx.x + (int64_t)u * kBS;
+        s_val[k - e0] = v[u];
+        s_x[k - e0] = x[c[u]];
+      }
+      __syncthreads();
+      if (row < n) {
+        const int64_t b = rowptr[row] - e0, e = rowptr[row + 1] - e0;
+        for (int64_t k = b; k < e; ++k) sum = fma(s_val[k], s_x[k], sum);
+        y[row] = sum;
+      }
+    } else if (row < n) {
+      for (int64_t k = rowptr[row]; k < rowptr[row + 1]; ++k) sum = fma(val[k], x[col[k]], sum);
       y[row] = sum;
     }
-  } else if (row < n) {
-    for (int64_t k = rowptr[row]; k < rowptr[row + 1]; ++k) sum = fma(val[k], x[col[k]], sum);
-    y[row] = sum;
+    if (DOT && row < n) part += x[row] * sum;
   }
   if (DOT) {
-    const double part = dev::block_sum<kBS>(row < n ? x[row] * sum : 0.0, s_red);
+    part = dev::block_sum<kBS>(part, s_red);
     if (threadIdx.x == 0) partials[blockIdx.x] = part;
   }
 }
 
-// alpha = rho / (p, q)
-__global__ void __launch_bounds__(kFin) k_fin_alpha(KrylovScalars* ks, const double* __restrict__ partials, int np) {
-  __shared__ double s_red[kFin / 64];
-  if (ks->done) return;
-  const double pq = dev::sum_partials<kFin>(partials, np, 1, s_red);
-  if (threadIdx.x == 0) {
-    ks->pq = pq;
-    ks->alpha = ks->rho / pq;
-  }
+unsigned spmv64_grid(int64_t n) {
+  return (unsigned)std::max<int64_t>(1, std::min<int64_t>((n + kBS - 1) / kBS, 2048));
 }
 
 // x = 0, r = b, p = D^-1 b; per block (r, D^-1 r), (b, b)
@@ -103,6 +117,8 @@ __global__ void __launch_bounds__(kFin) k_fin_init(KrylovScalars* ks, const doub
   const double BB = dev::sum_partials<kFin>(partials + 1, np, 2, s_red);
   if (threadIdx.x == 0) {
     ks->rho = RZ;
+    ks->rho2[0] = RZ;
+    ks->iter2[0] = 0;
     ks->bb = BB;
     ks->rr = BB;
     ks->stop_rr = tol2 * BB;
@@ -113,14 +129,22 @@ __global__ void __launch_bounds__(kFin) k_fin_init(KrylovScalars* ks, const doub
   }
 }
 
+// Iteration of parity par. Every block forms (p, q) from the SpMV partials in the
+// same fixed order, so all blocks use the same alpha = rho / (p, q).
 // x += alpha p; r -= alpha q; per block (r, D^-1 r), (r, r)
 __global__ void __launch_bounds__(kBS) k_pcg_update(int64_t n, const double* __restrict__ dinv, double* __restrict__ x,
                                                     double* __restrict__ r, const double* __restrict__ p,
-                                                    const double* __restrict__ q, const KrylovScalars* ks,
-                                                    double* __restrict__ partials) {
+                                                    const double* __restrict__ q, KrylovScalars* ks,
+                                                    const double* __restrict__ pq_part, int npq,
+                                                    double* __restrict__ partials, int par) {
   __shared__ double s_red[kBS / 64];
   if (ks->done) return;
-  const double alpha = ks->alpha;
+  const double pq = dev::sum_partials<kBS>(pq_part, npq, 1, s_red);
+  const double alpha = ks->rho2[par] / pq;
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    ks->pq = pq;
+    ks->alpha2[par] = alpha;
+  }
   double rz = 0.0, rr = 0.0;
   for (int64_t i = (int64_t)blockIdx.x * kBS + threadIdx.x; i < n; i += (int64_t)gridDim.x * kBS) {
     x[i] += alpha * p[i];
@@ -137,33 +161,35 @@ __global__ void __launch_bounds__(kBS) k_pcg_update(int64_t n, const double* __r
   }
 }
 
-// stop test on ||r||^2, then beta = (r', z') / (r, z)
-__global__ void __launch_bounds__(kFin) k_fin_update(KrylovScalars* ks, const double* __restrict__ partials, int np) {
-  __shared__ double s_red[kFin / 64];
+// Stop test on ||r||^2 (every block decides alike), then beta = (r', z') / (r, z) and
+// p = D^-1 r + beta p. Block 0 records the scalars for the next iteration and the host.
+__global__ void __launch_bounds__(kBS) k_pcg_dir(int64_t n, const double* __restrict__ dinv, const double* __restrict__ r,
+                                                 double* __restrict__ p, KrylovScalars* ks,
+                                                 const double* __restrict__ part, int np, int par) {
+  __shared__ double s_red[kBS / 64];
   if (ks->done) return;
-  const double RZ = dev::sum_partials<kFin>(partials, np, 2, s_red);
-  const double RR = dev::sum_partials<kFin>(partials + 1, np, 2, s_red);
-  if (threadIdx.x == 0) {
-    const int it = ks->iter + 1;
-    ks->iter = it;
+  const double RZ = dev::sum_partials<kBS>(part, np, 2, s_red);
+  const double RR = dev::sum_partials<kBS>(part + 1, np, 2, s_red);
+  const int it = ks->iter2[par] + 1;
+  const bool conv = RR <= ks->stop_rr;
+  const bool stop = conv || it >= ks->maxiter;
+  const double rho = ks->rho2[par];
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
     ks->rr = RR;
-    if (RR <= ks->stop_rr) {
+    ks->iter = it;
+    ks->iter2[par ^ 1] = it;
+    if (stop) {
       ks->done = 1;
-      ks->status = 0;
-    } else if (it >= ks->maxiter) {
-      ks->done = 1;
-      ks->status = NGPB_E_NOCONV;
+      ks->status = conv ? 0 : NGPB_E_NOCONV;
+      ks->stop_par = par;
     } else {
-      ks->beta = RZ / ks->rho;
+      ks->beta = RZ / rho;
       ks->rho = RZ;
+      ks->rho2[par ^ 1] = RZ;
     }
   }
-}
-
-__global__ void __launch_bounds__(kBS) k_pcg_dir(int64_t n, const double* __restrict__ dinv, const double* __restrict__ r,
-                                                 double* __restrict__ p, const KrylovScalars* ks) {
-  if (ks->done) return;
-  const double beta = ks->beta;
+  if (stop) return;
+  const double beta = RZ / rho;
   for (int64_t i = (int64_t)blockIdx.x * kBS + threadIdx.x; i < n; i += (int64_t)gridDim.x * kBS)
     p[i] = dinv[i] * r[i] + beta * p[i];
 }
@@ -192,8 +218,8 @@ unsigned vec_grid(int64_t n) {
   return (unsigned)std::max<int64_t>(1, std::min<int64_t>((n + 4 * kBS - 1) / (4 * kBS), 2048));
 }
 
-void prof_mark(ngpb_ctx* c, int kind) {
-  if (!c->profile) return;
+namespace {
+void prof_record(ngpb_ctx* c, int kind) {
   auto& p = c->kprof[kind];
   if (p.used == p.ev.size()) {
     hipEvent_t e;
@@ -202,18 +228,37 @@ void prof_mark(ngpb_ctx* c, int kind) {
   }
   NGPB_HIP(hipEventRecord(p.ev[p.used++], c->stream));
 }
+}  // namespace
 
-// call after the stream has been synchronized
-void prof_collect(ngpb_ctx* c) {
-  for (auto& p : c->kprof) {
-    for (size_t i = 0; i + 1 < p.used; i += 2) {
-      float ms = 0.f;
-      NGPB_HIP(hipEventElapsedTime(&ms, p.ev[i], p.ev[i + 1]));
-      p.ms += ms;
-      ++p.launches;
-    }
-    p.used = 0;
+// brackets the launch with events when profiling is on and tag is a multiple of
+// kProfEvery (tag < 0: always)
+void prof_begin(ngpb_ctx* c, int kind, int64_t tag) {
+  if (!c->profile || (tag >= 0 && tag % kProfEvery != 0)) return;
+  auto& p = c->kprof[kind];
+  if (p.tag.size() < p.used / 2 + 1) p.tag.resize(p.used / 2 + 1);
+  p.tag[p.used / 2] = tag;
+  prof_record(c, kind);
+}
+
+void prof_end(ngpb_ctx* c, int kind) {
+  if (!c->profile) return;
+  auto& p = c->kprof[kind];
+  if (p.used % 2 == 1) prof_record(c, kind);
+}
+
+// Call after the stream has been synchronized: adds the recorded launches whose tag
+// is below `limit` (launches queued after the stop flag exit at once and would bias
+// the average; limit < 0 keeps all) and clears the record.
+void prof_collect(ngpb_ctx* c, int kind, int64_t limit) {
+  auto& p = c->kprof[kind];
+  for (size_t i = 0; i + 1 < p.used; i += 2) {
+    if (limit >= 0 && p.tag[i / 2] >= limit) continue;
+    float ms = 0.f;
+    NGPB_HIP(hipEventElapsedTime(&ms, p.ev[i], p.ev[i + 1]));
+    p.ms += ms;
+    ++p.launches;
   }
+  p.used = 0;
 }
 
 // plain device copy of `bytes` (multiple of 16): reads and writes `bytes` each launch
@@ -253,7 +298,7 @@ double spmv_fp64_bytes(const ngpb_ctx* c) {
 }
 
 void spmv_fp64(ngpb_ctx* c, const double* x, double* y) {
-  hipLaunchKernelGGL(k_spmv64<false>, dim3(blocks_for(c->n, kBS)), dim3(kBS), 0, c->stream, c->n, c->rowptr.p, c->col.p,
+  hipLaunchKernelGGL(k_spmv64<false>, dim3(spmv64_grid(c->n)), dim3(kBS), 0, c->stream, c->n, c->rowptr.p, c->col.p,
                      c->val.p, x, y, nullptr, nullptr);
   NGPB_HIP(hipGetLastError());
 }
@@ -299,8 +344,8 @@ void solve_fp64(ngpb_ctx* c, const ngpb_solve_opts& o, ngpb_solve_stats& st) {
   c->r.resize((size_t)n);
   c->p.reserve((size_t)n);
   c->q.reserve((size_t)n);
-  const unsigned gs = blocks_for(n, kBS), gv = vec_grid(n);
-  ensure_reduce(c, std::max<int64_t>(gs, 2 * (int64_t)gv));
+  const unsigned gs = spmv64_grid(n), gv = vec_grid(n);
+  ensure_reduce(c, std::max<int64_t>(gs, 2048) + 2 * (int64_t)gv);
   double* part = c->red.partials.p;
   KrylovScalars* ks = c->ks.p;
   hipEvent_t e0, e1;
@@ -310,23 +355,27 @@ void solve_fp64(ngpb_ctx* c, const ngpb_solve_opts& o, ngpb_solve_stats& st) {
   hipLaunchKernelGGL(k_pcg_init, dim3(gv), dim3(kBS), 0, c->stream, n, c->rhs.p, c->dinv.p, c->x.p, c->r.p, c->p.p, part);
   hipLaunchKernelGGL(k_fin_init, dim3(1), dim3(kFin), 0, c->stream, ks, part, (int)gv, o.tol * o.tol, o.maxiter);
   NGPB_HIP(hipGetLastError());
+  // partial-sum regions: SpMV (p, q) and update (r, z), (r, r)
+  double* ppq = part;
+  double* pup = part + std::max<int64_t>(gs, 2048);
   const int chunk = std::max(1, o.check_every);
+  int64_t launch = 0;
   while (true) {
-    for (int k = 0; k < chunk; ++k) {
-      prof_mark(c, 0);
+    for (int k = 0; k < chunk; ++k, ++launch) {
+      const int par = (int)(launch & 1);
+      prof_begin(c, 0, launch);
       hipLaunchKernelGGL(k_spmv64<true>, dim3(gs), dim3(kBS), 0, c->stream, n, c->rowptr.p, c->col.p, c->val.p, c->p.p,
-                         c->q.p, ks, part);
-      prof_mark(c, 0);
-      hipLaunchKernelGGL(k_fin_alpha, dim3(1), dim3(kFin), 0, c->stream, ks, part, (int)gs);
+                         c->q.p, ks, ppq);
+      prof_end(c, 0);
       hipLaunchKernelGGL(k_pcg_update, dim3(gv), dim3(kBS), 0, c->stream, n, c->dinv.p, c->x.p, c->r.p, c->p.p, c->q.p,
-                         ks, part);
-      hipLaunchKernelGGL(k_fin_update, dim3(1), dim3(kFin), 0, c->stream, ks, part, (int)gv);
-      hipLaunchKernelGGL(k_pcg_dir, dim3(gv), dim3(kBS), 0, c->stream, n, c->dinv.p, c->r.p, c->p.p, ks);
+                         ks, ppq, (int)gs, pup, par);
+      hipLaunchKernelGGL(k_pcg_dir, dim3(gv), dim3(kBS), 0, c->stream, n, c->dinv.p, c->r.p, c->p.p, ks, pup, (int)gv,
+                         par);
     }
     NGPB_HIP(hipGetLastError());
     NGPB_HIP(hipMemcpyAsync(c->ks_host, ks, sizeof(KrylovScalars), hipMemcpyDeviceToHost, c->stream));
     NGPB_HIP(hipStreamSynchronize(c->stream));
-    prof_collect(c);
+    prof_collect(c, 0, c->ks_host->iter);
     if (c->ks_host->done) break;
   }
   NGPB_HIP(hipEventRecord(e1, c->stream));

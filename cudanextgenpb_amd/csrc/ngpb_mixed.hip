@@ -2,17 +2,20 @@
 // replacement, FP64 reductions.
 //
 // Operator. With D = diag(A) and S = D^-1/2, CG on Ahat = S A S (unit diagonal) is
-// Jacobi-preconditioned CG on A. Ahat's off-diagonal part is stored as SELL-64 plus
-// an overflow list:
-//   * a slice is 64 consecutive rows (one wavefront, one row per lane);
-//   * the slice keeps w "main" slots per row, w chosen per slice to minimise bytes
-//     (padding of short rows against overflow of long rows); slot k holds the k-th
-//     off-diagonal (ascending column) of each row as an FP32 value and a 16-bit
-//     offset from the slot's 32-bit base column (slices whose columns in some slot
-//     span 2^16 or more keep 32-bit columns instead);
-//   * the off-diagonals of a row beyond its w main slots go to the slice's overflow
-//     list (FP32 value + 32-bit column, rows in lane order, an 8-bit count per row).
-// The diagonal is not stored. Row sums accumulate in FP64 and are rounded once.
+// Jacobi-preconditioned CG on A. Ahat's off-diagonal part is stored in slices of 64
+// consecutive rows (one wavefront, one row per lane), each slice in one of two forms:
+//   * delta slice: the slice's off-diagonals take at most 16 distinct values of
+//     (column - row) -- the stencil of a uniform region in lexicographic node order;
+//     slot k holds, per row, the FP32 value at column row + delta_k (0 if absent) and
+//     only the values and the slot's delta are stored;
+//   * offset slice: w "main" slots per row, w chosen to minimise bytes (padding of
+//     short rows against overflow of long rows); slot k holds the k-th off-diagonal
+//     (ascending column) as an FP32 value and a 16-bit offset from the slot's 32-bit
+//     base column (32-bit columns when a slot spans 2^16 or more); off-diagonals
+//     beyond w go to the slice's overflow list (FP32 value + 32-bit column, rows in
+//     lane order, an 8-bit count per row).
+// The form with fewer bytes is chosen per slice. The diagonal is not stored. Row
+// sums accumulate in FP64 and are rounded once. Each wavefront walks many slices.
 //
 // Solve (reliable updates in the sense of Sleijpen & van der Vorst, 1996):
 //   x, b, the true residual r = b - A x and all dot products are FP64; the inner
@@ -21,7 +24,9 @@
 //   replacement (or below the convergence target); then x += S xhat, the FP64
 //   residual is recomputed with the FP64 CSR operator, rhat = S r, and CG continues
 //   with the same direction p. The solve stops when ||b - A x||_2 <= tol ||b||_2 in
-//   FP64. Reductions use per-block partials and one-block finalize kernels.
+//   FP64. An iteration is three kernels (SpMV, update, direction); each combines
+//   the previous kernel's per-block partial sums itself, in the same fixed order in
+//   every block.
 #include <algorithm>
 #include <cmath>
 
@@ -91,44 +96,77 @@ __device__ __forceinline__ int64_t offdiag_at(int64_t b, int64_t dpos, int k) {
   return (dpos >= 0 && dpos <= idx) ? idx + 1 : idx;
 }
 
-// one wave per slice: main width, 32-bit flag, overflow size
+constexpr int kDMax = 16;  // most distinct column deltas of a delta slice
+
+// Walks the distinct values of (column - row) over the off-diagonals of the slice's
+// rows in increasing order (each lane's off-diagonals are sorted by column). For the
+// k-th distinct delta `m`, visit(k, m, e) is called with e = this lane's CSR index of
+// the entry with that delta, or -1. Stops after kDMax + 1 deltas; returns the count.
+template <class F>
+__device__ __forceinline__ int for_each_delta(const int32_t* col, int64_t row, bool valid, int64_t b, int64_t dpos,
+                                              int cnt, F&& visit) {
+  int pos = 0, nd = 0;
+  while (true) {
+    const int64_t e = (valid && pos < cnt) ? offdiag_at(b, dpos, pos) : -1;
+    const int d = e >= 0 ? (int)((int64_t)col[e] - row) : INT32_MAX;
+    const int m = wave_min_i(d);
+    if (m == INT32_MAX || nd > kDMax) break;
+    visit(nd, m, d == m ? e : (int64_t)-1);
+    ++nd;
+    if (d == m) ++pos;
+  }
+  return nd;
+}
+
+// One wave per slice: chooses the delta form (values only, column = row + delta of
+// the slot) when the slice has at most kDMax distinct deltas and it is not larger
+// than the offset form; otherwise the offset form with a per-slice main width that
+// minimises padding + overflow bytes. width[s] < 0 marks a delta slice.
 __global__ void __launch_bounds__(kBS) k_sell_shape(int64_t n, int64_t nslices, const int32_t* __restrict__ rowptr,
                                                     const int32_t* __restrict__ col, int32_t* __restrict__ width,
                                                     int64_t* __restrict__ wcount, int64_t* __restrict__ wide_count,
-                                                    int64_t* __restrict__ ov_count, int* __restrict__ bad) {
+                                                    int64_t* __restrict__ ov_count, int64_t* __restrict__ dslots,
+                                                    int* __restrict__ bad) {
   const int64_t s = (int64_t)blockIdx.x * (kBS / kW) + (threadIdx.x >> 6);
   if (s >= nslices) return;
   const int64_t row = s * kW + lane_id();
   int cnt = 0;
   int64_t dpos = -1;
   if (row < n) row_info(rowptr, col, row, &dpos, &cnt);
+  const int64_t b = row < n ? rowptr[row] : 0;
   const int mx = wave_max_i(cnt);
-  // bytes of main slots (6 per entry) against overflow entries (8 each + 1 per row count)
+  // offset form: main slots (6 bytes per entry) against overflow entries, which cost
+  // 8 bytes but are walked serially per row (two dependent loads each), so they are
+  // charged 32 bytes apiece
   int best_w = mx;
   long long best = (long long)mx * kW * 6;
   for (int w = mx - 1; w >= 0; --w) {
     const int extra = wave_sum_i(max(0, cnt - w));
-    const long long cost = (long long)w * kW * 6 + (long long)extra * 8 + kW;
+    const long long cost = (long long)w * kW * 6 + (long long)extra * 32 + kW;
     if (cost < best) {
       best = cost;
       best_w = w;
     }
   }
-  if (cnt - best_w > 255) atomicOr(bad, 1);
   int wide = 0;
-  const int64_t b = row < n ? rowptr[row] : 0;
   for (int k = 0; k < best_w; ++k) {
     const int cc = (row < n && k < cnt) ? col[offdiag_at(b, dpos, k)] : INT32_MAX;
     const int cmin = wave_min_i(cc);
     const int cmax = wave_max_i(cc == INT32_MAX ? INT32_MIN : cc);
     if (cmin != INT32_MAX && (int64_t)cmax - (int64_t)cmin > 65535) wide = 1;
   }
-  const int extra = wave_sum_i(max(0, cnt - best_w));
+  if (wide) best += (long long)best_w * kW * 4;
+  // delta form: 4 bytes per slot entry
+  const int nd = for_each_delta(col, row, row < n, b, dpos, cnt, [](int, int, int64_t) {});
+  const bool use_delta = nd <= kDMax && (long long)nd * kW * 4 <= best;
+  if (!use_delta && cnt - best_w > 255) atomicOr(bad, 1);
+  const int extra = use_delta ? 0 : wave_sum_i(max(0, cnt - best_w));
   if (lane_id() == 0) {
-    width[s] = best_w;
-    wcount[s] = best_w;
-    wide_count[s] = wide ? (int64_t)best_w * kW : 0;
+    width[s] = use_delta ? -nd : best_w;
+    wcount[s] = use_delta ? nd : best_w;
+    wide_count[s] = (!use_delta && wide) ? (int64_t)best_w * kW : 0;
     ov_count[s] = extra;
+    dslots[s] = use_delta ? nd : 0;
   }
 }
 
@@ -165,6 +203,17 @@ __global__ void __launch_bounds__(kBS) k_sell_fill(int64_t n, int64_t nslices, c
     b = rowptr[row];
   }
   const double di = row < n ? dscale[row] : 0.0;
+  if (w < 0) {
+    // delta slice: slot k holds the entry of column row + delta_k (0 if absent)
+    for_each_delta(col, row, row < n, b, dpos, cnt, [&](int k, int m, int64_t e) {
+      const int64_t idx = sp + (int64_t)k * kW + lane;
+      sval[idx] = e >= 0 ? (float)(val[e] * di * dscale[col[e]]) : 0.f;
+      soff[idx] = 0;
+      if (lane == 0) sbase[bp + k] = m;
+    });
+    if (row < n) ov_cnt[row] = 0;
+    return;
+  }
   for (int k = 0; k < w; ++k) {
     const int64_t e = (row < n && k < cnt) ? offdiag_at(b, dpos, k) : -1;
     const int cc = e >= 0 ? col[e] : INT32_MAX;
@@ -206,44 +255,267 @@ struct Sell {
   const int32_t* ov_col;
 };
 
-// q = Ahat p (unit diagonal + off-diagonals); optional per-block (p, q)
+// Delta slice with a compile-time slot count: every value and delta is requested
+// before any gather, and every gather before the first multiply-add, so one slice
+// costs two memory round trips.
+template <int ND>
+__device__ __forceinline__ double delta_rows(const Sell& A, int64_t bp, int64_t sp, int64_t row, int64_t last,
+                                             const float* __restrict__ x, double acc) {
+  float v[ND], xv[ND];
+  int d[ND];
+#pragma unroll
+  for (int j = 0; j < ND; ++j) {
+    v[j] = A.val[sp + (int64_t)j * kW];
+    d[j] = A.base[bp + j];
+  }
+#pragma unroll
+  for (int j = 0; j < ND; ++j) xv[j] = x[min(max(row + (int64_t)d[j], (int64_t)0), last)];
+#pragma unroll
+  for (int j = 0; j < ND; ++j) acc = fma((double)v[j], (double)xv[j], acc);
+  return acc;
+}
+
+// offset slice with 32-bit columns and a compile-time width
+template <int W>
+__device__ __forceinline__ double wide_rows(const Sell& A, int64_t sp, int64_t wp, const float* __restrict__ x,
+                                            double acc) {
+  float v[W], xv[W];
+  int c[W];
+#pragma unroll
+  for (int j = 0; j < W; ++j) {
+    v[j] = A.val[sp + (int64_t)j * kW];
+    c[j] = A.col32[wp + (int64_t)j * kW];
+  }
+#pragma unroll
+  for (int j = 0; j < W; ++j) xv[j] = x[c[j]];
+#pragma unroll
+  for (int j = 0; j < W; ++j) acc = fma((double)v[j], (double)xv[j], acc);
+  return acc;
+}
+
+// offset slice (16-bit offsets) with a compile-time width
+template <int W>
+__device__ __forceinline__ double offset_rows(const Sell& A, int64_t bp, int64_t sp, const float* __restrict__ x,
+                                              double acc) {
+  float v[W], xv[W];
+  int c[W];
+#pragma unroll
+  for (int j = 0; j < W; ++j) {
+    v[j] = A.val[sp + (int64_t)j * kW];
+    c[j] = A.base[bp + j] + (int)A.off[sp + (int64_t)j * kW];
+  }
+#pragma unroll
+  for (int j = 0; j < W; ++j) xv[j] = x[c[j]];
+#pragma unroll
+  for (int j = 0; j < W; ++j) acc = fma((double)v[j], (double)xv[j], acc);
+  return acc;
+}
+
+__device__ __forceinline__ double offset_rows_any(const Sell& A, int64_t s, int w, int64_t bp, int64_t sp, int64_t row,
+                                               bool in, const float* __restrict__ x, double acc) {
+  const int lane = lane_id();
+  const int64_t wp = A.wide_ptr[s], op = A.ov_ptr[s];
+  if (wp < 0) {
+    switch (w) {
+      case 0: break;
+      case 1: acc = offset_rows<1>(A, bp, sp, x, acc); break;
+      case 2: acc = offset_rows<2>(A, bp, sp, x, acc); break;
+      case 3: acc = offset_rows<3>(A, bp, sp, x, acc); break;
+      case 4: acc = offset_rows<4>(A, bp, sp, x, acc); break;
+      case 5: acc = offset_rows<5>(A, bp, sp, x, acc); break;
+      case 6: acc = offset_rows<6>(A, bp, sp, x, acc); break;
+      case 7: acc = offset_rows<7>(A, bp, sp, x, acc); break;
+      case 8: acc = offset_rows<8>(A, bp, sp, x, acc); break;
+      default:
+        for (int k0 = 0; k0 < w; k0 += 8) {
+          // eight slots per round: indices past the last slot repeat it, with weight 0
+          float v[8], xv[8];
+          int c[8];
+#pragma unroll
+          for (int j = 0; j < 8; ++j) {
+            const int k = min(k0 + j, w - 1);
+            v[j] = k0 + j < w ? A.val[sp + (int64_t)k * kW] : 0.f;
+            c[j] = A.base[bp + k] + (int)A.off[sp + (int64_t)k * kW];
+          }
+#pragma unroll
+          for (int j = 0; j < 8; ++j) xv[j] = x[c[j]];
+#pragma unroll
+          for (int j = 0; j < 8; ++j) acc = fma((double)v[j], (double)xv[j], acc);
+        }
+    }
+  } else {
+    const int64_t wl = wp + lane;
+    switch (w) {
+      case 0: break;
+      case 1: acc = wide_rows<1>(A, sp, wl, x, acc); break;
+      case 2: acc = wide_rows<2>(A, sp, wl, x, acc); break;
+      case 3: acc = wide_rows<3>(A, sp, wl, x, acc); break;
+      case 4: acc = wide_rows<4>(A, sp, wl, x, acc); break;
+      case 5: acc = wide_rows<5>(A, sp, wl, x, acc); break;
+      case 6: acc = wide_rows<6>(A, sp, wl, x, acc); break;
+      case 7: acc = wide_rows<7>(A, sp, wl, x, acc); break;
+      case 8: acc = wide_rows<8>(A, sp, wl, x, acc); break;
+      default:
+        for (int k0 = 0; k0 < w; k0 += 8) {
+          float v[8], xv[8];
+          int c[8];
+#pragma unroll
+          for (int j = 0; j < 8; ++j) {
+            const int k = min(k0 + j, w - 1);
+            v[j] = k0 + j < w ? A.val[sp + (int64_t)k * kW] : 0.f;
+            c[j] = A.col32[wl + (int64_t)k * kW];
+          }
+#pragma unroll
+          for (int j = 0; j < 8; ++j) xv[j] = x[c[j]];
+#pragma unroll
+          for (int j = 0; j < 8; ++j) acc = fma((double)v[j], (double)xv[j], acc);
+        }
+    }
+  }
+  if (op >= 0) {
+    const int cnt = in ? (int)A.ov_cnt[row] : 0;
+    const int64_t start = op + wave_excl_scan_i(cnt);
+    for (int k = 0; k < cnt; ++k) acc = fma((double)A.ov_val[start + k], (double)x[A.ov_col[start + k]], acc);
+  }
+  return acc;
+}
+
+// one slice of any form: q = x_i + off-diagonal part (the unit diagonal is added
+// last so that the x_i load overlaps the gathers)
+__device__ __forceinline__ void one_slice(const Sell& A, int64_t s, int w, int64_t bp, const float* __restrict__ x,
+                                          float* __restrict__ y, double* part) {
+  const int lane = lane_id();
+  const int64_t last = A.n - 1;
+  const int64_t row = s * kW + lane;
+  const int64_t sp = bp * kW + lane;
+  const bool in = row < A.n;
+  const float xi = x[min(row, last)];
+  double acc = 0.0;
+  switch (w) {
+    case -1: acc = delta_rows<1>(A, bp, sp, row, last, x, acc); break;
+    case -2: acc = delta_rows<2>(A, bp, sp, row, last, x, acc); break;
+    case -3: acc = delta_rows<3>(A, bp, sp, row, last, x, acc); break;
+    case -4: acc = delta_rows<4>(A, bp, sp, row, last, x, acc); break;
+    case -5: acc = delta_rows<5>(A, bp, sp, row, last, x, acc); break;
+    case -6: acc = delta_rows<6>(A, bp, sp, row, last, x, acc); break;
+    case -7: acc = delta_rows<7>(A, bp, sp, row, last, x, acc); break;
+    case -8: acc = delta_rows<8>(A, bp, sp, row, last, x, acc); break;
+    case -9: acc = delta_rows<9>(A, bp, sp, row, last, x, acc); break;
+    case -10: acc = delta_rows<10>(A, bp, sp, row, last, x, acc); break;
+    case -11: acc = delta_rows<11>(A, bp, sp, row, last, x, acc); break;
+    case -12: acc = delta_rows<12>(A, bp, sp, row, last, x, acc); break;
+    case -13: acc = delta_rows<13>(A, bp, sp, row, last, x, acc); break;
+    case -14: acc = delta_rows<14>(A, bp, sp, row, last, x, acc); break;
+    case -15: acc = delta_rows<15>(A, bp, sp, row, last, x, acc); break;
+    case -16: acc = delta_rows<16>(A, bp, sp, row, last, x, acc); break;
+    default: acc = offset_rows_any(A, s, w, bp, sp, row, in, x, acc);
+  }
+  acc += (double)xi;
+  if (in) {
+    const float yi = (float)acc;
+    y[row] = yi;
+    *part += (double)xi * (double)yi;
+  }
+}
+
+// two adjacent delta slices (s0, s0 + 1) with ND slots each
+template <int ND>
+__device__ __forceinline__ void delta_pair(const Sell& A, int64_t s0, int64_t b0, int64_t b1,
+                                           const float* __restrict__ x, float* __restrict__ y, double* part) {
+  const int lane = lane_id();
+  const int64_t last = A.n - 1;
+  const int64_t r0 = s0 * kW + lane, r1 = r0 + kW;
+  const int64_t p0 = b0 * kW + lane, p1 = b1 * kW + lane;
+  float v0[ND], v1[ND], x0[ND], x1[ND];
+  int d0[ND], d1[ND];
+#pragma unroll
+  for (int j = 0; j < ND; ++j) {
+    v0[j] = A.val[p0 + (int64_t)j * kW];
+    v1[j] = A.val[p1 + (int64_t)j * kW];
+    d0[j] = A.base[b0 + j];
+    d1[j] = A.base[b1 + j];
+  }
+  const float xi0 = x[min(r0, last)], xi1 = x[min(r1, last)];
+#pragma unroll
+  for (int j = 0; j < ND; ++j) {
+    x0[j] = x[min(max(r0 + (int64_t)d0[j], (int64_t)0), last)];
+    x1[j] = x[min(max(r1 + (int64_t)d1[j], (int64_t)0), last)];
+  }
+  double a0 = 0.0, a1 = 0.0;
+#pragma unroll
+  for (int j = 0; j < ND; ++j) {
+    a0 = fma((double)v0[j], (double)x0[j], a0);
+    a1 = fma((double)v1[j], (double)x1[j], a1);
+  }
+  a0 += (double)xi0;
+  a1 += (double)xi1;
+  if (r0 < A.n) {
+    const float yi = (float)a0;
+    y[r0] = yi;
+    *part += (double)xi0 * (double)yi;
+  }
+  if (r1 < A.n) {
+    const float yi = (float)a1;
+    y[r1] = yi;
+    *part += (double)xi1 * (double)yi;
+  }
+}
+
+// q = Ahat p with per-block (p, q) when DOT
 template <bool DOT>
 __global__ void __launch_bounds__(kBS) k_sell_spmv(Sell A, const float* __restrict__ x, float* __restrict__ y,
                                                    const KrylovScalars* ks, double* __restrict__ partials) {
   __shared__ double s_red[kBS / 64];
   if (DOT && ks->done) return;
-  const int64_t s = __builtin_amdgcn_readfirstlane((int)(blockIdx.x * (kBS / kW) + (threadIdx.x >> 6)));
-  const int lane = lane_id();
-  const int64_t row = s * kW + lane;
+  const int wave = __builtin_amdgcn_readfirstlane((int)(blockIdx.x * (kBS / kW) + (threadIdx.x >> 6)));
+  const int64_t nwaves = (int64_t)gridDim.x * (kBS / kW);
+  const int64_t npairs = (A.nslices + 1) / 2;
   double part = 0.0;
-  if (s < A.nslices) {
-    const int w = A.width[s];
-    const int64_t bp = A.base_ptr[s], wp = A.wide_ptr[s], op = A.ov_ptr[s];
-    const int64_t sp = bp * kW;
-    const float xi = row < A.n ? x[row] : 0.f;
-    double acc = (double)xi;
-    if (wp < 0) {
-      for (int k = 0; k < w; ++k) {
-        const int64_t idx = sp + (int64_t)k * kW + lane;
-        const int c = A.base[bp + k] + (int)A.off[idx];
-        acc = fma((double)A.val[idx], (double)x[c], acc);
+  // a wave takes adjacent slices (2t, 2t+1) for t = wave, wave + nwaves, ...; two
+  // delta slices with the same slot count are processed together so that twice as
+  // many loads are in flight; the next pair's widths and slot pointers are requested
+  // before the current pair is processed
+  int64_t t = wave;
+  int w0n = 0, w1n = 0;
+  int64_t b0n = 0, b1n = 0;
+  if (t < npairs) {
+    w0n = A.width[2 * t];
+    b0n = A.base_ptr[2 * t];
+    if (2 * t + 1 < A.nslices) {
+      w1n = A.width[2 * t + 1];
+      b1n = A.base_ptr[2 * t + 1];
+    }
+  }
+  while (t < npairs) {
+    const int w0 = w0n, w1 = w1n;
+    const int64_t b0 = b0n, b1 = b1n;
+    const int64_t s0 = 2 * t;
+    const bool has1 = s0 + 1 < A.nslices;
+    const int64_t t2 = t + nwaves;
+    if (t2 < npairs) {
+      w0n = A.width[2 * t2];
+      b0n = A.base_ptr[2 * t2];
+      if (2 * t2 + 1 < A.nslices) {
+        w1n = A.width[2 * t2 + 1];
+        b1n = A.base_ptr[2 * t2 + 1];
       }
-    } else {
-      for (int k = 0; k < w; ++k) {
-        const int64_t idx = sp + (int64_t)k * kW + lane;
-        acc = fma((double)A.val[idx], (double)x[A.col32[wp + (int64_t)k * kW + lane]], acc);
+    }
+    bool done = false;
+    if (has1 && w0 == w1) {
+      switch (w0) {
+        case -4: delta_pair<4>(A, s0, b0, b1, x, y, &part); done = true; break;
+        case -5: delta_pair<5>(A, s0, b0, b1, x, y, &part); done = true; break;
+        case -6: delta_pair<6>(A, s0, b0, b1, x, y, &part); done = true; break;
+        case -7: delta_pair<7>(A, s0, b0, b1, x, y, &part); done = true; break;
+        case -8: delta_pair<8>(A, s0, b0, b1, x, y, &part); done = true; break;
+        default: break;
       }
     }
-    if (op >= 0) {
-      const int cnt = row < A.n ? (int)A.ov_cnt[row] : 0;
-      const int64_t start = op + wave_excl_scan_i(cnt);
-      for (int k = 0; k < cnt; ++k) acc = fma((double)A.ov_val[start + k], (double)x[A.ov_col[start + k]], acc);
+    if (!done) {
+      one_slice(A, s0, w0, b0, x, y, &part);
+      if (has1) one_slice(A, s0 + 1, w1, b1, x, y, &part);
     }
-    if (row < A.n) {
-      const float yi = (float)acc;
-      y[row] = yi;
-      part = (double)xi * (double)yi;
-    }
+    t = t2;
   }
   if (DOT) {
     part = dev::block_sum<kBS>(part, s_red);
@@ -251,26 +523,24 @@ __global__ void __launch_bounds__(kBS) k_sell_spmv(Sell A, const float* __restri
   }
 }
 
-__global__ void __launch_bounds__(kFin) k_mx_fin_alpha(KrylovScalars* ks, const double* __restrict__ partials, int np) {
-  __shared__ double s_red[kFin / 64];
-  if (ks->done) return;
-  const double pq = dev::sum_partials<kFin>(partials, np, 1, s_red);
-  if (threadIdx.x == 0) {
-    ks->pq = pq;
-    ks->alpha = ks->rho / pq;
-  }
-}
-
-// xhat += alpha p; rhat -= alpha q; per block (rhat, rhat)
-__global__ void __launch_bounds__(kBS) k_mx_update(int64_t n, float* __restrict__ xs, float* __restrict__ rs,
-                                                   const float* __restrict__ ps, const float* __restrict__ qs,
-                                                   const KrylovScalars* ks, double* __restrict__ partials) {
+// Iteration of parity par: every block forms (p, q) from the SpMV partials in the
+// same order, alpha = rho / (p, q); rhat -= alpha q; per block (rhat, rhat).
+// (xhat += alpha p is done by the direction kernel of the same iteration, or by the
+// accumulation after the iteration that raised the stop flag.)
+__global__ void __launch_bounds__(kBS) k_mx_update(int64_t n, float* __restrict__ rs, const float* __restrict__ qs,
+                                                   KrylovScalars* ks, const double* __restrict__ pq_part, int npq,
+                                                   double* __restrict__ partials, int par) {
   __shared__ double s_red[kBS / 64];
   if (ks->done) return;
-  const float alpha = (float)ks->alpha;
+  const double pq = dev::sum_partials<kBS>(pq_part, npq, 1, s_red);
+  const double alpha_d = ks->rho2[par] / pq;
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    ks->pq = pq;
+    ks->alpha2[par] = alpha_d;
+  }
+  const float alpha = (float)alpha_d;
   double rr = 0.0;
   for (int64_t i = (int64_t)blockIdx.x * kBS + threadIdx.x; i < n; i += (int64_t)gridDim.x * kBS) {
-    xs[i] = fmaf(alpha, ps[i], xs[i]);
     const float ri = fmaf(-alpha, qs[i], rs[i]);
     rs[i] = ri;
     rr += (double)ri * (double)ri;
@@ -279,23 +549,7 @@ __global__ void __launch_bounds__(kBS) k_mx_update(int64_t n, float* __restrict_
   if (threadIdx.x == 0) partials[blockIdx.x] = rr;
 }
 
-__global__ void __launch_bounds__(kFin) k_mx_fin_update(KrylovScalars* ks, const double* __restrict__ partials, int np) {
-  __shared__ double s_red[kFin / 64];
-  if (ks->done) return;
-  const double RR = dev::sum_partials<kFin>(partials, np, 1, s_red);
-  if (threadIdx.x == 0) {
-    const int it = ks->iter + 1;
-    ks->iter = it;
-    ks->rr = RR;
-    if (RR <= ks->stop_rr || it >= ks->maxiter) {
-      ks->done = 1;
-    } else {
-      ks->beta = RR / ks->rho;
-      ks->rho = RR;
-    }
-  }
-}
-
+// p = rhat + beta p (first direction of a cycle)
 __global__ void __launch_bounds__(kBS) k_mx_dir(int64_t n, const float* __restrict__ rs, float* __restrict__ ps,
                                                 const KrylovScalars* ks) {
   if (ks->done) return;
@@ -304,11 +558,47 @@ __global__ void __launch_bounds__(kBS) k_mx_dir(int64_t n, const float* __restri
     ps[i] = fmaf(beta, ps[i], rs[i]);
 }
 
-// x += S xhat; xhat = 0
-__global__ void __launch_bounds__(kBS) k_mx_accum(int64_t n, const double* __restrict__ dscale, double* __restrict__ x,
-                                                  float* __restrict__ xs) {
+// Stop test on (rhat, rhat) (every block decides alike); otherwise xhat += alpha p and
+// p = rhat + beta p. Block 0 records the scalars for the next iteration and the host.
+__global__ void __launch_bounds__(kBS) k_mx_dir_x(int64_t n, const float* __restrict__ rs, float* __restrict__ ps,
+                                                  float* __restrict__ xs, KrylovScalars* ks,
+                                                  const double* __restrict__ rr_part, int npr, int par) {
+  __shared__ double s_red[kBS / 64];
+  if (ks->done) return;
+  const double RR = dev::sum_partials<kBS>(rr_part, npr, 1, s_red);
+  const int it = ks->iter2[par] + 1;
+  const bool stop = RR <= ks->stop_rr || it >= ks->maxiter;
+  const double rho = ks->rho2[par];
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    ks->rr = RR;
+    ks->iter = it;
+    ks->iter2[par ^ 1] = it;
+    if (stop) {
+      ks->done = 1;
+      ks->stop_par = par;
+    } else {
+      ks->beta = RR / rho;
+      ks->rho = RR;
+      ks->rho2[par ^ 1] = RR;
+    }
+  }
+  if (stop) return;
+  const float alpha = (float)ks->alpha2[par];
+  const float beta = (float)(RR / rho);
   for (int64_t i = (int64_t)blockIdx.x * kBS + threadIdx.x; i < n; i += (int64_t)gridDim.x * kBS) {
-    x[i] += dscale[i] * (double)xs[i];
+    const float pi = ps[i];
+    xs[i] = fmaf(alpha, pi, xs[i]);
+    ps[i] = fmaf(beta, pi, rs[i]);
+  }
+}
+
+// x += S (xhat + alpha p) for the iteration that raised the stop flag; xhat = 0
+__global__ void __launch_bounds__(kBS) k_mx_accum(int64_t n, const double* __restrict__ dscale, double* __restrict__ x,
+                                                  float* __restrict__ xs, const float* __restrict__ ps,
+                                                  const KrylovScalars* ks) {
+  const float alpha = (float)ks->alpha2[ks->stop_par];
+  for (int64_t i = (int64_t)blockIdx.x * kBS + threadIdx.x; i < n; i += (int64_t)gridDim.x * kBS) {
+    x[i] += dscale[i] * (double)fmaf(alpha, ps[i], xs[i]);
     xs[i] = 0.f;
   }
 }
@@ -393,7 +683,11 @@ Sell sell_view(const ngpb_ctx* c) {
   return A;
 }
 
-unsigned sell_grid(const ngpb_ctx* c) { return (unsigned)((c->n_slices + (kBS / kW) - 1) / (kBS / kW)); }
+// at most 2048 blocks of 4 waves (8 blocks per CU); each wave walks several slices
+unsigned sell_grid(const ngpb_ctx* c) {
+  const int64_t pairs = (c->n_slices + 1) / 2;
+  return (unsigned)std::max<int64_t>(1, std::min<int64_t>((pairs + (kBS / kW) - 1) / (kBS / kW), 2048));
+}
 
 void launch_sell(ngpb_ctx* c, bool dot, const float* x, float* y, double* part) {
   if (dot)
@@ -413,18 +707,19 @@ void build_compact(ngpb_ctx* c) {
   hipLaunchKernelGGL(k_scale, dim3(blocks_for(n, 256)), dim3(256), 0, c->stream, n, c->rowptr.p, c->col.p, c->val.p,
                      c->dscale.p);
   c->slice_width.resize((size_t)ns);
-  DBuf<int64_t> wcount, widec, wide_off, ovc, ov_off;
+  DBuf<int64_t> wcount, widec, wide_off, ovc, ov_off, dslots;
   DBuf<int> bad;
   wcount.resize((size_t)ns);
   widec.resize((size_t)ns);
   wide_off.resize((size_t)ns);
   ovc.resize((size_t)ns);
   ov_off.resize((size_t)ns);
+  dslots.resize((size_t)ns);
   bad.resize(1);
   NGPB_HIP(hipMemsetAsync(bad.p, 0, sizeof(int), c->stream));
   const unsigned g = (unsigned)((ns + (kBS / kW) - 1) / (kBS / kW));
   hipLaunchKernelGGL(k_sell_shape, dim3(g), dim3(kBS), 0, c->stream, n, ns, c->rowptr.p, c->col.p, c->slice_width.p,
-                     wcount.p, widec.p, ovc.p, bad.p);
+                     wcount.p, widec.p, ovc.p, dslots.p, bad.p);
   NGPB_HIP(hipGetLastError());
   int hb = 0;
   NGPB_HIP(hipMemcpyAsync(&hb, bad.p, sizeof(int), hipMemcpyDeviceToHost, c->stream));
@@ -433,8 +728,10 @@ void build_compact(ngpb_ctx* c) {
   const int64_t nbases = exclusive_scan(c, wcount.p, wcount.p, ns);  // wcount -> base_ptr
   const int64_t nwide = exclusive_scan(c, widec.p, wide_off.p, ns);
   const int64_t nov = exclusive_scan(c, ovc.p, ov_off.p, ns);
+  const int64_t ndslots = exclusive_scan(c, dslots.p, dslots.p, ns);
   c->sell_bases = nbases;
   c->sell_entries = nbases * kW;
+  c->sell_delta_entries = ndslots * kW;
   c->sell_wide_entries = nwide;
   c->sell_ov_entries = nov;
   c->slice_base_ptr.resize((size_t)ns);
@@ -458,12 +755,14 @@ void build_compact(ngpb_ctx* c) {
 }
 
 double spmv_compact_bytes(const ngpb_ctx* c) {
-  // main entries (4 value + 2 offset), slot bases (4), 32-bit columns of wide slices,
-  // overflow entries (4 + 4), slice metadata (width 4 + three 8-byte offsets), the
-  // overflow counts of slices that have overflow (64 per slice, counted for all rows
-  // as an upper bound), x read once and y written
-  return (double)c->sell_entries * 6.0 + (double)c->sell_bases * 4.0 + (double)c->sell_wide_entries * 4.0 +
-         (double)c->sell_ov_entries * 8.0 + (double)c->n_slices * 28.0 + (double)c->n * (4.0 + 4.0);
+  // delta-slice entries (4: value), offset-slice entries (4 value + 2 offset), slot
+  // bases/deltas (4), 32-bit columns of wide slices, overflow entries (4 + 4) and
+  // their per-row counts, slice metadata (width 4 + three 8-byte offsets), x read
+  // once and y written
+  const double off_entries = (double)(c->sell_entries - c->sell_delta_entries);
+  return (double)c->sell_delta_entries * 4.0 + off_entries * 6.0 + (double)c->sell_bases * 4.0 +
+         (double)c->sell_wide_entries * 4.0 + (double)c->sell_ov_entries * 8.0 + (double)c->n_slices * 28.0 +
+         (double)c->n * (4.0 + 4.0);
 }
 
 void spmv_compact(ngpb_ctx* c, const double* x, double* y) {
@@ -510,8 +809,11 @@ void solve_mixed(ngpb_ctx* c, const ngpb_solve_opts& o, ngpb_solve_stats& st) {
   c->qs.reserve((size_t)n);
   const unsigned gv = vec_grid(n);
   const unsigned gs = sell_grid(c);
-  ensure_reduce(c, std::max<int64_t>(gs, 2 * (int64_t)gv));
+  ensure_reduce(c, std::max<int64_t>(gs, 2048) + 2 * (int64_t)gv);
   double* part = c->red.partials.p;
+  double* ppq = part;                                  // SpMV (p, q) partials
+  double* pup = part + std::max<int64_t>(gs, 2048);    // update (rhat, rhat) partials
+  int64_t launch = 0;                                  // iteration launches (parity)
   KrylovScalars* ks = c->ks.p;
   KrylovScalars* h = c->ks_host;
   const double drop = (o.drop > 0.0 && o.drop < 1.0) ? o.drop : 0.05;
@@ -544,38 +846,41 @@ void solve_mixed(ngpb_ctx* c, const ngpb_solve_opts& o, ngpb_solve_stats& st) {
       h->maxiter = o.maxiter;
       h->iter = iters;
       h->done = 0;
+      h->rho2[launch & 1] = h->rho;
+      h->iter2[launch & 1] = iters;
       NGPB_HIP(hipMemcpyAsync(ks, h, sizeof(KrylovScalars), hipMemcpyHostToDevice, c->stream));
+      const int64_t cycle_start = launch;
       hipLaunchKernelGGL(k_mx_dir, dim3(gv), dim3(kBS), 0, c->stream, n, c->rs.p, c->ps.p, ks);
       while (true) {
-        for (int k = 0; k < chunk; ++k) {
-          prof_mark(c, 1);
-          launch_sell(c, true, c->ps.p, c->qs.p, part);
-          prof_mark(c, 1);
-          hipLaunchKernelGGL(k_mx_fin_alpha, dim3(1), dim3(kFin), 0, c->stream, ks, part, (int)gs);
-          hipLaunchKernelGGL(k_mx_update, dim3(gv), dim3(kBS), 0, c->stream, n, c->xs.p, c->rs.p, c->ps.p, c->qs.p, ks,
-                             part);
-          hipLaunchKernelGGL(k_mx_fin_update, dim3(1), dim3(kFin), 0, c->stream, ks, part, (int)gv);
-          hipLaunchKernelGGL(k_mx_dir, dim3(gv), dim3(kBS), 0, c->stream, n, c->rs.p, c->ps.p, ks);
+        for (int k = 0; k < chunk; ++k, ++launch) {
+          const int par = (int)(launch & 1);
+          prof_begin(c, 1, launch);
+          launch_sell(c, true, c->ps.p, c->qs.p, ppq);
+          prof_end(c, 1);
+          hipLaunchKernelGGL(k_mx_update, dim3(gv), dim3(kBS), 0, c->stream, n, c->rs.p, c->qs.p, ks, ppq, (int)gs, pup,
+                             par);
+          hipLaunchKernelGGL(k_mx_dir_x, dim3(gv), dim3(kBS), 0, c->stream, n, c->rs.p, c->ps.p, c->xs.p, ks, pup,
+                             (int)gv, par);
         }
         NGPB_HIP(hipGetLastError());
         NGPB_HIP(hipMemcpyAsync(h, ks, sizeof(KrylovScalars), hipMemcpyDeviceToHost, c->stream));
         NGPB_HIP(hipStreamSynchronize(c->stream));
-        prof_collect(c);
+        prof_collect(c, 1, cycle_start + (h->iter - iters));
         if (h->done) break;
       }
       iters = h->iter;
       // FP64 residual replacement
-      hipLaunchKernelGGL(k_mx_accum, dim3(gv), dim3(kBS), 0, c->stream, n, c->dscale.p, c->x.p, c->xs.p);
-      prof_mark(c, 0);
+      hipLaunchKernelGGL(k_mx_accum, dim3(gv), dim3(kBS), 0, c->stream, n, c->dscale.p, c->x.p, c->xs.p, c->ps.p, ks);
+      prof_begin(c, 0, -1);
       spmv_fp64(c, c->x.p, c->q.p);
-      prof_mark(c, 0);
+      prof_end(c, 0);
       hipLaunchKernelGGL(k_mx_resid, dim3(gv), dim3(kBS), 0, c->stream, n, c->rhs.p, c->q.p, c->dscale.p, c->rs.p, c->x.p,
                          c->xs.p, c->ps.p, 0, part);
       hipLaunchKernelGGL(k_mx_fin_resid, dim3(1), dim3(kFin), 0, c->stream, ks, part, (int)gv, 0);
       NGPB_HIP(hipGetLastError());
       NGPB_HIP(hipMemcpyAsync(h, ks, sizeof(KrylovScalars), hipMemcpyDeviceToHost, c->stream));
       NGPB_HIP(hipStreamSynchronize(c->stream));
-      prof_collect(c);
+      prof_collect(c, 0, -1);
       ++repl;
       true_rr = h->aux[0];
       if (true_rr <= target) {
@@ -602,8 +907,8 @@ void solve_mixed(ngpb_ctx* c, const ngpb_solve_opts& o, ngpb_solve_stats& st) {
   st.replacements = repl;
   st.relres = bb > 0 ? std::sqrt(true_rr / bb) : 0.0;
   st.ms_solve = ms;
-  // SpMV + update (xhat, p, rhat, q read; xhat, rhat written) + direction (rhat, p read; p written)
-  st.bytes_per_iter = spmv_compact_bytes(c) + (double)n * 4.0 * (6.0 + 3.0);
+  // SpMV + update (rhat, q read; rhat written) + direction (rhat, p, xhat read; p, xhat written)
+  st.bytes_per_iter = spmv_compact_bytes(c) + (double)n * 4.0 * (3.0 + 5.0);
   c->have_phi = true;
 }
 

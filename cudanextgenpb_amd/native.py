@@ -91,6 +91,7 @@ SIGNATURES = {
     "ngpb_set_profiling": (_I, [_P, _I]),
     "ngpb_kernel_stats": (_I, [_P, _I, _P, _P, _P]),
     "ngpb_time_copy": (_I, [_P, _I64, _I, _P]),
+    "ngpb_operator_info": (_I, [_P, _P]),
 }
 
 _lib = None
@@ -295,6 +296,13 @@ class Context:
         ms = ctypes.c_double(0)
         self._check(lib().ngpb_time_copy(self._h, int(nbytes), reps, ctypes.byref(ms)), "time_copy")
         return ms.value
+
+    def operator_info(self):
+        info = np.zeros(8, np.int64)
+        self._check(lib().ngpb_operator_info(self._h, _p(info)), "operator_info")
+        keys = ["slices", "delta_slices", "delta_entries", "offset_entries", "wide_entries", "overflow_entries",
+                "compact_spmv_bytes", "fp64_spmv_bytes"]
+        return dict(zip(keys, (int(v) for v in info)))
 
     def time_spmv(self, precision=PREC_FP64, reps=50):
         ms, b = ctypes.c_double(0), ctypes.c_double(0)

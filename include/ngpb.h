@@ -177,6 +177,12 @@ int ngpb_kernel_stats(ngpb_ctx* ctx, int precision, int64_t* launches, double* t
 /* Mean time of a plain device copy kernel moving `bytes` (read + write each), used to
    calibrate hardware traffic counters. No counterpart in the reference. */
 int ngpb_time_copy(ngpb_ctx* ctx, int64_t bytes, int reps, double* ms_per_launch);
+/* Layout of the compact mixed-precision operator (built on demand): info[0] slices,
+   [1] delta slices, [2] stored slot entries in delta slices, [3] in offset slices,
+   [4] 32-bit column entries, [5] overflow entries, [6] algorithmic bytes of one
+   compact SpMV, [7] algorithmic bytes of one FP64 CSR SpMV. No counterpart in the
+   reference. */
+int ngpb_operator_info(ngpb_ctx* ctx, int64_t* info8);
 
 #ifdef __cplusplus
 }

@@ -1,0 +1,312 @@
+"""Pins the CPU oracle without the reference's libraries (absent here).
+
+Each check is chosen so that a plausible mistake fails it: closed forms (physical
+constants, Born energy, uniform stencil values, sphere crossings), invariants
+(symmetry, positive definiteness, zero row sums, exact reproduction of linear fields
+through hanging-node constraints, charge conservation, Gauss's law), brute force
+(distance test for in/out, exhaustive leaf search for atom location) and an
+independent direct solver (committed golden vectors).
+"""
+import os
+
+import numpy as np
+import pytest
+
+import oracle as O
+from cudanextgenpb_amd import workload as W
+
+GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+BORN_EXACT = -68.654008445
+BORN_EXACT_K0 = -68.305979850
+
+
+# ---------------------------------------------------------------- constants (a1)
+def test_constants_closed_form():
+    c = O.constants(W.Physics())
+    assert c["eps_in"] == pytest.approx(3.568501623678e-3, rel=1e-12)
+    assert c["eps_out"] == pytest.approx(1.427400649471e-1, rel=1e-12)
+    assert c["k2"] == pytest.approx(1.537456096972e-2, rel=1e-12)
+    assert c["eps_out"] * c["k2"] == pytest.approx(2.194565831350e-3, rel=1e-12)
+    assert c["constant_pol"] == pytest.approx((1 / c["eps_out"] - 1 / c["eps_in"]) / (4 * np.pi), rel=1e-15)
+    # Born: (q^2 / 2a) (1 / (eps_out (1 + k a)) - 1 / eps_in), a = 2, q = 1
+    a = 2.0
+    born = 0.5 / a * (1 / (c["eps_out"] * (1 + c["k"] * a)) - 1 / c["eps_in"])
+    assert born == pytest.approx(BORN_EXACT, rel=1e-9)
+    born0 = 0.5 / a * (1 / c["eps_out"] - 1 / c["eps_in"])
+    assert born0 == pytest.approx(BORN_EXACT_K0, rel=1e-9)
+
+
+def test_analytic_solution_matches_born_potential():
+    ph = W.Physics()
+    c = O.constants(ph)
+    # outside: exp(k (a - r)) / (eps_out (1 + k a) r); continuous at r = a
+    r = 5.0
+    expect = np.exp(c["k"] * (2 - r)) / (c["eps_out"] * (1 + 2 * c["k"]) * r)
+    assert O.analytic_solution(r, 0, 0, ph) == pytest.approx(expect, rel=1e-14)
+    assert O.analytic_solution(2.0, 0, 0, ph) == pytest.approx(O.analytic_solution(0, 2.0 + 1e-12, 0, ph), rel=1e-9)
+
+
+# ---------------------------------------------------------------- markers (a2-a4)
+def _inside_distance(P, xyz):
+    a = P.atoms
+    d2 = ((xyz[:, None, :] - a.xyz[None, :, :]) ** 2).sum(-1) - a.r[None, :] ** 2
+    return d2.min(axis=1)
+
+
+@pytest.mark.parametrize("maker", [lambda: W.born(h=1.0), lambda: W.three_spheres()])
+def test_markers_match_distance_test(maker):
+    P = maker()
+    eps, reac, lm, border = O.markers(P)
+    c = O.constants(P.physics)
+    g = _inside_distance(P, P.mesh.node_xyz)
+    inside = eps == c["eps_in"]
+    clear = np.abs(g) > 1e-9
+    assert np.array_equal(inside[clear], (g < 0)[clear])
+    assert np.all(reac[inside] == 0.0) and np.all(reac[~inside] == c["eps_out"] * c["k2"])
+    # leaf markers from the corner flags
+    lc = P.mesh.leaf_corner
+    reg = lc >= 0
+    nin = np.where(reg, inside[np.where(reg, lc, 0)], False).sum(axis=1)
+    nh = (~reg).sum(axis=1)
+    expect = np.where(nin == 0, 2, np.where(nin < 8 - nh, 1, 0))
+    assert np.array_equal(lm, expect)
+    assert np.array_equal(border, np.nonzero(expect == 1)[0])
+
+
+def test_marker_tie_break_on_sphere_poles():
+    # src/pb_class.cpp:798-804: a node on the upper crossing is outside, on the lower inside
+    P = W.born(h=0.25)
+    eps, _, _, _ = O.markers(P)
+    c = O.constants(P.physics)
+    xyz = P.mesh.node_xyz
+    top = np.nonzero(np.all(np.isclose(xyz, [0, 0, 2.0], atol=0), axis=1))[0]
+    bot = np.nonzero(np.all(np.isclose(xyz, [0, 0, -2.0], atol=0), axis=1))[0]
+    assert eps[top[0]] == c["eps_out"]
+    assert eps[bot[0]] == c["eps_in"]
+
+
+# ---------------------------------------------------------------- fractions (a5)
+def test_fractions_closed_form_sphere():
+    P = W.born(h=1.0)
+    _, _, lm, border = O.markers(P)
+    frac, nrm, found = O.fractions(P, lm)
+    EN = [(0, 1), (1, 3), (2, 3), (0, 2), (4, 5), (5, 7), (6, 7), (4, 6), (0, 4), (1, 5), (3, 7), (2, 6)]
+    AX = [0, 1, 0, 1, 0, 1, 0, 1, 2, 2, 2, 2]
+    m = P.mesh
+    checked = 0
+    for l in border:
+        for e in range(12):
+            if not found[l, e]:
+                assert frac[l, e] == 0.5
+                continue
+            p1 = m.node_xyz[m.leaf_corner[l, EN[e][0]]]
+            p2 = m.node_xyz[m.leaf_corner[l, EN[e][1]]]
+            ax = AX[e]
+            d2 = 4.0 - (p1 ** 2).sum() + p1[ax] ** 2
+            roots = [-np.sqrt(d2), np.sqrt(d2)]
+            hits = [t for t in roots if p1[ax] <= t <= p2[ax]]
+            t = hits[-1]  # the last crossing in the closed interval
+            assert frac[l, e] == pytest.approx((t - p1[ax]) / (p2[ax] - p1[ax]), abs=1e-12)
+            pt = p1.copy()
+            pt[ax] = t
+            np.testing.assert_allclose(nrm[l, e], pt / 2.0, atol=1e-12)
+            checked += 1
+    assert checked > 100
+
+
+# ---------------------------------------------------------------- atom location (a8)
+def test_locate_matches_generator_and_half_open_rule():
+    P = W.globular(n_atoms=40, ball_radius=5.0, min_sep=2.0, fine_band=1.5, mid_band=3.0)
+    assert np.array_equal(O.locate(P), P.owner)
+    # l - tol < x <= r - tol: an atom exactly on a shared face belongs to the leaf
+    # whose lower face it is (the upper neighbour)
+    m = P.mesh
+    l = int(P.owner[0])
+    hi = m.node_xyz[m.leaf_corner[l, 7]]
+    lo = m.node_xyz[m.leaf_corner[l, 0]]
+    P.atoms.xyz[0] = 0.5 * (lo + hi)
+    P.atoms.xyz[0, 0] = hi[0]
+    own = O.locate(P)
+    assert own[0] != l
+    nlo = m.node_xyz[m.leaf_corner[own[0], 0]]
+    nhi = m.node_xyz[m.leaf_corner[own[0], 7]]
+    assert nlo[0] == hi[0]
+    assert np.all(nlo - P.atom_tol < P.atoms.xyz[0]) and np.all(P.atoms.xyz[0] <= nhi - P.atom_tol)
+
+
+# ---------------------------------------------------------------- charge projection (a8)
+def test_rhs_conserves_charge_and_weights():
+    P = W.three_spheres()
+    r = O.rhs(P)
+    assert r.sum() == pytest.approx(4 * np.pi * P.atoms.q.sum(), rel=1e-13)
+    # a single unit charge at a regular node puts 4 pi on that node only
+    node = P.mesh.leaf_corner[P.owner[0], 0]
+    assert node >= 0
+    P.atoms.xyz[0] = P.mesh.node_xyz[node] + 1e-300
+    P.atoms.q[:] = [1.0, 0.0, 0.0]
+    P.owner[0] = O.locate(P)[0]
+    r = O.rhs(P)
+    assert r.sum() == pytest.approx(4 * np.pi, rel=1e-13)
+    assert r.max() == pytest.approx(4 * np.pi, rel=1e-12)
+
+
+# ---------------------------------------------------------------- assembly (a6, a7, a9)
+def _pipeline_system(P, bnd_vals=None):
+    eps, reac, lm, border = O.markers(P)
+    frac, _, _ = O.fractions(P, lm)
+    return O.assemble(P, eps, reac, frac, O.rhs(P), bnd_vals), eps, reac
+
+
+def test_uniform_stencil_closed_form():
+    P = W.born(h=1.0)
+    S, eps, reac = _pipeline_system(P)
+    c = O.constants(P.physics)
+    A = S  # CSR
+    m = P.mesh
+    xyz = m.node_xyz
+    far = np.nonzero((np.abs(xyz).max(axis=1) < 14) & (np.sqrt((xyz ** 2).sum(axis=1)) > 6))[0]
+    h = 1.0
+    for i in far[:200]:
+        cols = A.col[A.rowptr[i]:A.rowptr[i + 1]]
+        vals = A.val[A.rowptr[i]:A.rowptr[i + 1]]
+        assert len(cols) == 7
+        off = vals[cols != i]
+        assert np.allclose(off, -c["eps_out"] * h, rtol=1e-14)
+        d = vals[cols == i][0]
+        assert d == pytest.approx(6 * c["eps_out"] * h + c["eps_out"] * c["k2"] * h ** 3, rel=1e-14)
+
+
+@pytest.mark.parametrize("maker", [lambda: W.three_spheres(), lambda: W.globular(
+    n_atoms=40, ball_radius=5.0, min_sep=2.0, fine_band=1.5, mid_band=3.0)])
+def test_assembly_symmetric_positive_definite(maker):
+    P = maker()
+    S, _, _ = _pipeline_system(P)
+    if S.n > 6000:
+        pytest.skip("dense check limited to small meshes")
+    A = S.to_dense()
+    assert np.array_equal(A, A.T)
+    assert np.linalg.eigvalsh(A).min() > 0
+
+
+def test_linear_field_reproduced_through_hanging_nodes():
+    # eps uniform, kappa = 0: every row not touching the Dirichlet set annihilates a
+    # linear potential, also rows coupled through hanging-node constraints
+    ph = W.Physics(e_in=80.0, e_out=80.0, ionic_strength=0.0)
+    P = W.three_spheres(physics=ph)
+    assert P.mesh.n_hang > 0
+    geometric_bnd = P.mesh.bnd_nodes.copy()
+    P.mesh.bnd_nodes = np.zeros(0, np.int32)  # no Dirichlet rows: the plain operator
+    S, _, _ = _pipeline_system(P, bnd_vals=np.zeros(0))
+    xyz = P.mesh.node_xyz
+    phi = 0.3 * xyz[:, 0] - 1.1 * xyz[:, 1] + 0.7 * xyz[:, 2] + 2.0
+    interior = np.ones(S.n, bool)
+    interior[geometric_bnd] = False
+    Aphi = O.spmv(S, phi)
+    hang_rows = np.unique(P.mesh.hang_par[P.mesh.hang_par >= 0])
+    assert np.intersect1d(np.nonzero(interior)[0], hang_rows).size > 50
+    scale = np.abs(S.val).max() * np.abs(phi).max()
+    assert np.abs(Aphi[interior]).max() <= 1e-13 * scale
+    # and the domain-boundary rows do not (natural boundary flux), as expected
+    assert np.abs(Aphi[geometric_bnd]).max() > 1e-3 * scale
+
+
+def test_zero_row_sums_without_salt():
+    # variable eps (2 / 80 across the surface), kappa = 0, no Dirichlet rows: every
+    # row sums to zero (constants are in the kernel of the flux operator)
+    ph = W.Physics(ionic_strength=0.0)
+    P = W.three_spheres(physics=ph)
+    P.mesh.bnd_nodes = np.zeros(0, np.int32)
+    S, _, _ = _pipeline_system(P, bnd_vals=np.zeros(0))
+    row = O.spmv(S, np.ones(S.n))
+    assert np.abs(row).max() <= 1e-14 * np.abs(S.val).max()
+
+
+def test_dirichlet_rows_and_elimination():
+    P = W.three_spheres(bc_type=3)
+    bv = O.dirichlet_values(P)
+    S, _, _ = _pipeline_system(P, bnd_vals=bv)
+    for k, i in enumerate(P.mesh.bnd_nodes[:200]):
+        cols = S.col[S.rowptr[i]:S.rowptr[i + 1]]
+        vals = S.val[S.rowptr[i]:S.rowptr[i + 1]]
+        assert list(cols) == [i]
+        assert S.rhs[i] == vals[0] * bv[k]
+    isb = np.zeros(S.n, bool)
+    isb[P.mesh.bnd_nodes] = True
+    nonb = np.nonzero(~isb)[0]
+    assert not np.any(isb[np.concatenate([S.col[S.rowptr[i]:S.rowptr[i + 1]] for i in nonb])])
+
+
+# ---------------------------------------------------------------- Krylov (a10)
+@pytest.mark.parametrize("name", ["born_h2", "three_spheres_h1"])
+def test_pcg_matches_direct_solver_golden(name):
+    g = np.load(os.path.join(GOLDEN, name + ".npz"))
+    S = O.System(len(g["rhs"]), g["rowptr"], g["col"], g["val"], g["rhs"])
+    x, it, rr, st = O.pcg(S, tol=1e-13)
+    assert st == 0
+    assert np.linalg.norm(x - g["phi_direct"]) / np.linalg.norm(g["phi_direct"]) <= 1e-10
+
+
+def test_pcg_reports_maxiter():
+    g = np.load(os.path.join(GOLDEN, "born_h2.npz"))
+    S = O.System(len(g["rhs"]), g["rowptr"], g["col"], g["val"], g["rhs"])
+    x, it, rr, st = O.pcg(S, tol=1e-13, maxiter=3)
+    assert st == 4 and it == 3 and rr > 1e-13
+
+
+def test_golden_systems_regenerate_identically():
+    P = W.born(h=2.0)
+    S, _, _ = _pipeline_system(P)
+    g = np.load(os.path.join(GOLDEN, "born_h2.npz"))
+    assert np.array_equal(S.rowptr, g["rowptr"]) and np.array_equal(S.col, g["col"])
+    assert np.array_equal(S.val, g["val"]) and np.array_equal(S.rhs, g["rhs"])
+
+
+# ---------------------------------------------------------------- energy (a11)
+def test_born_energy_converges_to_closed_form():
+    errs = []
+    for h in (1.0, 0.5):
+        R = O.full_pipeline(W.born(h=h))
+        E = R["energy"]
+        assert E["flux_charge"] == pytest.approx(1.0, abs=1e-9)
+        errs.append(abs(E["polar"] - BORN_EXACT) / abs(BORN_EXACT))
+    assert errs[1] < errs[0] < 2e-3
+    assert errs[1] < 5e-4
+
+
+def test_born_energy_without_salt_matches_kappa0_form():
+    R = O.full_pipeline(W.born(h=0.5, physics=W.Physics(ionic_strength=0.0)))
+    E = R["energy"]
+    assert E["energy_react"] == 0.0
+    assert abs(E["polar"] - BORN_EXACT_K0) / abs(BORN_EXACT_K0) < 5e-4
+
+
+def test_gauss_flux_charge_equals_net_charge():
+    P = W.globular(n_atoms=40, ball_radius=5.0, min_sep=2.0, fine_band=1.5, mid_band=3.0, bc_type=2)
+    R = O.full_pipeline(P)
+    assert R["energy"]["flux_charge"] == pytest.approx(P.atoms.q.sum(), abs=1e-6)
+
+
+def test_energy_from_golden_direct_solution():
+    g = np.load(os.path.join(GOLDEN, "born_h2.npz"))
+    P = W.born(h=2.0)
+    eps, _, _, border = O.markers(P)
+    E = O.energy(P, eps, g["phi_direct"], border)
+    assert E["energy_pol"] == pytest.approx(float(g["energy_pol"]), rel=1e-12)
+    assert E["energy_react"] == pytest.approx(float(g["energy_react"]), rel=1e-12)
+
+
+def test_mc_table_structure():
+    src = open(os.path.join(os.path.dirname(O.__file__), "..", "cudanextgenpb_amd", "csrc", "mc_table.inc")).read()
+    import re
+    strings = re.findall(r'"([0-9a-f]*)"', src.split("{", 1)[1])
+    assert len(strings) == 256
+    order = [0, 1, 3, 2, 4, 5, 7, 6]  # MC vertex -> p4est corner
+    EN = [(0, 1), (1, 3), (2, 3), (0, 2), (4, 5), (5, 7), (6, 7), (4, 6), (0, 4), (1, 5), (3, 7), (2, 6)]
+    for ci, s in enumerate(strings):
+        assert len(s) % 3 == 0 and len(s) <= 15
+        inside = {order[v] for v in range(8) if ci >> v & 1}
+        crossing = {e for e, (a, b) in enumerate(EN) if (a in inside) != (b in inside)}
+        used = {int(ch, 16) for ch in s}
+        assert used == crossing, ci
+        # complementary classification crosses the same edges
+        assert {int(ch, 16) for ch in strings[255 - ci]} == crossing

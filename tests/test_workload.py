@@ -1,0 +1,93 @@
+"""Synthetic workload generator: octree validity, rays, atoms."""
+import numpy as np
+import pytest
+
+from cudanextgenpb_amd import workload as W
+
+
+def _corner_xyz(m, i):
+    c = m.leaf_corner[:, i]
+    return np.where((c >= 0)[:, None], m.node_xyz[np.maximum(c, 0)], m.hang_xyz[np.maximum(-1 - c, 0)])
+
+
+def _leaf_boxes(m):
+    lo = _corner_xyz(m, 0)
+    h = (2.0 ** m.leaf_level) * m.hf
+    return lo, h
+
+
+@pytest.mark.parametrize("maker", [lambda: W.three_spheres(), lambda: W.globular(
+    n_atoms=40, ball_radius=5.0, min_sep=2.0, fine_band=1.5, mid_band=3.0)])
+def test_octree_is_valid_and_balanced(maker):
+    P = maker()
+    m = P.mesh
+    lo, h = _leaf_boxes(m)
+    # leaves tile the box
+    assert np.isclose((h ** 3).sum(), np.prod(m.n * m.hf), rtol=1e-12)
+    # corners are at anchor + bit * h
+    for i in range(8):
+        xyz = _corner_xyz(m, i)
+        off = np.array([(i >> 0) & 1, (i >> 1) & 1, (i >> 2) & 1], float)
+        np.testing.assert_allclose(xyz, lo + off[None, :] * h[:, None], atol=1e-12)
+    # hanging nodes: midpoint of 2 parents or centre of 4; parents are regular nodes
+    assert np.all(np.isin(m.hang_np, [2, 4]))
+    for k in range(m.n_hang):
+        par = m.hang_par[k, : m.hang_np[k]]
+        assert np.all(par >= 0)
+        np.testing.assert_allclose(m.node_xyz[par].mean(axis=0), m.hang_xyz[k], atol=1e-12)
+    # 2:1 balance over faces, edges and corners: neighbouring leaves differ by <= 1 level
+    idx = np.rint((lo - m.origin) / m.hf).astype(np.int64)
+    lev = np.zeros(tuple(m.n), np.int8)
+    for l in range(m.n_leaves):
+        s = 1 << m.leaf_level[l]
+        x, y, z = idx[l]
+        lev[x:x + s, y:y + s, z:z + s] = m.leaf_level[l]
+    for d in range(3):
+        for shift in (1, -1):
+            a = np.roll(lev, shift, axis=d)
+            sl = [slice(None)] * 3
+            sl[d] = slice(1, None) if shift == 1 else slice(None, -1)
+            assert np.abs(a[tuple(sl)].astype(int) - lev[tuple(sl)]).max() <= 1
+    # nodes are numbered lexicographically (x fastest, then y, then z)
+    key = np.lexsort((m.node_xyz[:, 0], m.node_xyz[:, 1], m.node_xyz[:, 2]))
+    assert np.array_equal(key, np.arange(m.n_nodes))
+
+
+def test_rays_are_sorted_and_on_spheres():
+    P = W.three_spheres()
+    for axis, r in enumerate(P.rays):
+        k = r.key
+        assert np.all((k[1:, 0] > k[:-1, 0]) | ((k[1:, 0] == k[:-1, 0]) & (k[1:, 1] > k[:-1, 1])))
+        for i in range(r.n):
+            t = r.inters[r.ptr[i]:r.ptr[i + 1]]
+            assert len(t) % 2 == 0 and np.all(np.diff(t) >= 0)
+            others = [d for d in range(3) if d != axis]
+            for j, tj in enumerate(t):
+                p = np.zeros(3)
+                p[axis] = tj
+                p[others] = k[i]
+                g = np.sqrt(((p - P.atoms.xyz) ** 2).sum(axis=1)) - P.atoms.r
+                assert np.min(np.abs(g)) < 1e-9          # on some sphere
+                assert np.all(g > -1e-9)                 # and on the union's surface
+                nm = r.normals[r.ptr[i] + j]
+                assert abs(np.linalg.norm(nm) - 1.0) < 1e-12
+
+
+def test_atoms_distribution_and_separation():
+    a = W.make_atoms(20502, 500, [0, 0, 0], [20.0] * 3, 2.0, 1.2, 2.0, 0.35, -3.0)
+    assert a.q.sum() == pytest.approx(-3.0, abs=1e-12)
+    assert np.all((a.r >= 1.2) & (a.r <= 2.0))
+    assert np.all((a.xyz ** 2).sum(axis=1) <= 400.0)
+    d = np.sqrt(((a.xyz[:, None] - a.xyz[None]) ** 2).sum(-1)) + np.eye(500) * 1e9
+    assert d.min() >= 2.0
+    b = W.make_atoms(20502, 500, [0, 0, 0], [20.0] * 3, 2.0, 1.2, 2.0, 0.35, -3.0)
+    assert np.array_equal(a.xyz, b.xyz) and np.array_equal(a.q, b.q)  # seeded
+
+
+def test_globular_default_size():
+    # the bench configuration: about 8M unknowns with hanging nodes (generation only)
+    P = W.globular()
+    assert 7.5e6 < P.mesh.n_nodes < 8.5e6
+    assert P.mesh.n_hang > 100000
+    assert P.atoms.n == 2500 and P.atoms.q.sum() == pytest.approx(-3.0, abs=1e-9)
+    assert np.all(P.owner >= 0)
