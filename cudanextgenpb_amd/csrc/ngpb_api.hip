@@ -94,6 +94,11 @@ void need(bool cond, const char* what) {
   if (!cond) throw Error(NGPB_E_STATE, what);
 }
 
+// a context in a batch lends its operator and vectors to the batch's kernels
+void not_in_batch(const ngpb_ctx* c) {
+  if (c->in_batch) throw Error(NGPB_E_STATE, "context is in a batch (ngpb_batch_remove it first)");
+}
+
 }  // namespace
 
 extern "C" {
@@ -139,6 +144,10 @@ int ngpb_create(ngpb_ctx** out, int device) {
 
 int ngpb_destroy(ngpb_ctx* c) {
   if (!c) return NGPB_OK;
+  if (c->in_batch) {
+    c->err = "context is in a batch (ngpb_batch_remove it first)";
+    return NGPB_E_STATE;
+  }
   (void)hipSetDevice(c->device);
   if (c->stream) (void)hipStreamSynchronize(c->stream);
   if (c->ks_host) (void)hipHostFree(c->ks_host);
@@ -172,6 +181,7 @@ int ngpb_device_bytes(const ngpb_ctx* c, int64_t* bytes) {
 
 int ngpb_set_physics(ngpb_ctx* c, const ngpb_physics* p) {
   return guarded(c, [&] {
+    not_in_batch(c);
     NGPB_REQUIRE(p, NGPB_E_ARG, "physics is NULL");
     NGPB_REQUIRE(p->e_in > 0 && p->e_out > 0 && p->T > 0 && p->ionic_strength >= 0, NGPB_E_ARG,
                  "permittivities and temperature must be positive, ionic strength non-negative");
@@ -186,6 +196,7 @@ int ngpb_set_physics(ngpb_ctx* c, const ngpb_physics* p) {
 int ngpb_set_mesh(ngpb_ctx* c, int64_t n_nodes, const double* node_xyz, int64_t n_hang, const double* hang_xyz,
                   const int32_t* hang_np, const int32_t* hang_par, int64_t n_leaves, const int32_t* leaf_corner) {
   return guarded(c, [&] {
+    not_in_batch(c);
     NGPB_REQUIRE(n_nodes > 0 && n_leaves > 0 && n_hang >= 0, NGPB_E_ARG, "mesh sizes must be positive");
     NGPB_REQUIRE(n_nodes < INT32_MAX && n_hang < INT32_MAX && n_leaves < INT32_MAX, NGPB_E_ARG, "mesh too large");
     NGPB_REQUIRE(node_xyz && leaf_corner, NGPB_E_ARG, "NULL mesh array");
@@ -228,6 +239,7 @@ int ngpb_set_mesh(ngpb_ctx* c, int64_t n_nodes, const double* node_xyz, int64_t 
 int ngpb_set_rays(ngpb_ctx* c, int axis, int64_t n_rays, const double* key, const int64_t* ptr, const double* inters,
                   const double* normals) {
   return guarded(c, [&] {
+    not_in_batch(c);
     NGPB_REQUIRE(axis >= 0 && axis < 3, NGPB_E_ARG, "axis must be 0, 1 or 2");
     NGPB_REQUIRE(n_rays >= 0, NGPB_E_ARG, "negative ray count");
     auto& r = c->rays[axis];
@@ -271,6 +283,7 @@ int ngpb_set_rays(ngpb_ctx* c, int axis, int64_t n_rays, const double* key, cons
 
 int ngpb_create_markers(ngpb_ctx* c) {
   return guarded(c, [&] {
+    not_in_batch(c);
     need(c->have_mesh, "ngpb_set_mesh must come before ngpb_create_markers");
     for (int d = 0; d < 3; ++d)
       if (!c->rays[d].ptr.p) {
@@ -288,6 +301,7 @@ int ngpb_create_markers(ngpb_ctx* c) {
 
 int ngpb_set_markers(ngpb_ctx* c, const double* eps_node, const double* reac_node, const uint8_t* leaf_marker) {
   return guarded(c, [&] {
+    not_in_batch(c);
     need(c->have_mesh, "ngpb_set_mesh must come before ngpb_set_markers");
     NGPB_REQUIRE(eps_node && reac_node && leaf_marker, NGPB_E_ARG, "NULL marker array");
     for (int64_t i = 0; i < c->N; ++i)
@@ -344,6 +358,7 @@ int ngpb_get_fractions(ngpb_ctx* c, double* frac) {
 
 int ngpb_set_charges(ngpb_ctx* c, int64_t n_atoms, const double* xyz, const double* q, const int32_t* owner) {
   return guarded(c, [&] {
+    not_in_batch(c);
     need(c->have_mesh, "ngpb_set_mesh must come before ngpb_set_charges");
     NGPB_REQUIRE(n_atoms >= 0, NGPB_E_ARG, "negative atom count");
     NGPB_REQUIRE(n_atoms == 0 || (xyz && q && owner), NGPB_E_ARG, "NULL atom array");
@@ -371,6 +386,7 @@ int ngpb_set_charges(ngpb_ctx* c, int64_t n_atoms, const double* xyz, const doub
 
 int ngpb_set_dirichlet(ngpb_ctx* c, int64_t n_bnd, const int32_t* nodes, const double* values) {
   return guarded(c, [&] {
+    not_in_batch(c);
     need(c->have_mesh, "ngpb_set_mesh must come before ngpb_set_dirichlet");
     NGPB_REQUIRE(n_bnd >= 0 && (n_bnd == 0 || nodes), NGPB_E_ARG, "bad Dirichlet list");
     std::vector<uint8_t> isb((size_t)c->N, 0);
@@ -399,6 +415,7 @@ int ngpb_set_dirichlet(ngpb_ctx* c, int64_t n_bnd, const int32_t* nodes, const d
 
 int ngpb_dirichlet_values(ngpb_ctx* c, int bc_type) {
   return guarded(c, [&] {
+    not_in_batch(c);
     need(c->have_dirichlet, "ngpb_set_dirichlet must come before ngpb_dirichlet_values");
     ngpb::run_dirichlet_values(c, bc_type);
     c->have_system = c->have_compact = c->have_phi = false;
@@ -407,6 +424,7 @@ int ngpb_dirichlet_values(ngpb_ctx* c, int bc_type) {
 
 int ngpb_assemble(ngpb_ctx* c) {
   return guarded(c, [&] {
+    not_in_batch(c);
     need(c->have_mesh, "no mesh");
     need(c->have_markers, "markers must be created or set before assembly");
     if (!c->have_dirichlet) {
@@ -449,6 +467,7 @@ int ngpb_export_csr(ngpb_ctx* c, int32_t* rowptr, int32_t* col, double* val, dou
 int ngpb_set_csr(ngpb_ctx* c, int64_t n, int64_t nnz, const int32_t* rowptr, const int32_t* col, const double* val,
                  const double* rhs) {
   return guarded(c, [&] {
+    not_in_batch(c);
     NGPB_REQUIRE(n > 0 && nnz > 0 && n < INT32_MAX && nnz < INT32_MAX, NGPB_E_ARG, "bad CSR sizes");
     NGPB_REQUIRE(rowptr && col && val && rhs, NGPB_E_ARG, "NULL CSR array");
     NGPB_REQUIRE(rowptr[0] == 0 && rowptr[n] == nnz, NGPB_E_ARG, "row pointers must run from 0 to nnz");
@@ -518,6 +537,7 @@ int ngpb_solve(ngpb_ctx* c, const ngpb_solve_opts* o, double* phi_out, ngpb_solv
   ngpb_solve_stats& s = st ? *st : local;
   std::memset(&s, 0, sizeof(s));
   const int rc = guarded(c, [&] {
+    not_in_batch(c);
     NGPB_REQUIRE(o, NGPB_E_ARG, "NULL options");
     NGPB_REQUIRE(o->tol > 0 && o->maxiter > 0, NGPB_E_ARG, "tol and maxiter must be positive");
     need(c->have_system, "no assembled system");
@@ -548,6 +568,7 @@ int ngpb_get_phi(ngpb_ctx* c, double* phi) {
 
 int ngpb_set_phi(ngpb_ctx* c, const double* phi) {
   return guarded(c, [&] {
+    not_in_batch(c);
     NGPB_REQUIRE(phi, NGPB_E_ARG, "NULL input");
     need(c->have_mesh, "no mesh");
     c->x.upload(phi, (size_t)c->N, c->stream);
@@ -558,6 +579,7 @@ int ngpb_set_phi(ngpb_ctx* c, const double* phi) {
 
 int ngpb_spmv(ngpb_ctx* c, int precision, const double* x, double* y) {
   return guarded(c, [&] {
+    not_in_batch(c);
     NGPB_REQUIRE(x && y, NGPB_E_ARG, "NULL vector");
     need(c->have_system, "no assembled system");
     ngpb::DBuf<double> dx, dy;
@@ -583,6 +605,7 @@ int ngpb_energy(ngpb_ctx* c, ngpb_energy_out* out) {
 
 int ngpb_time_spmv(ngpb_ctx* c, int precision, int reps, double* ms, double* bytes) {
   return guarded(c, [&] {
+    not_in_batch(c);
     NGPB_REQUIRE(ms && bytes && reps > 0, NGPB_E_ARG, "bad arguments");
     need(c->have_system, "no assembled system");
     if (precision == NGPB_PREC_FP64) {
@@ -623,6 +646,7 @@ int ngpb_kernel_stats(ngpb_ctx* c, int precision, int64_t* launches, double* tot
 
 int ngpb_operator_info(ngpb_ctx* c, int64_t* info) {
   return guarded(c, [&] {
+    not_in_batch(c);
     NGPB_REQUIRE(info, NGPB_E_ARG, "NULL output");
     need(c->have_system, "no assembled system");
     if (!c->have_compact) ngpb::build_compact(c);
@@ -644,6 +668,7 @@ int ngpb_operator_info(ngpb_ctx* c, int64_t* info) {
 
 int ngpb_time_copy(ngpb_ctx* c, int64_t bytes, int reps, double* ms_per_launch) {
   return guarded(c, [&] {
+    not_in_batch(c);
     NGPB_REQUIRE(ms_per_launch && reps > 0 && bytes >= 16 && bytes % 16 == 0, NGPB_E_ARG,
                  "bytes must be a positive multiple of 16");
     *ms_per_launch = ngpb::time_copy(c, bytes, reps);

@@ -232,36 +232,69 @@ __device__ __forceinline__ double emission_value(uint64_t key, uint16_t kw, int6
   return reac[row] * v8[leaf] * w;
 }
 
-__global__ void k_slot_values(int64_t N, int64_t L, const int32_t* __restrict__ rowptr,
-                              const int64_t* __restrict__ slot_em, const uint64_t* __restrict__ key,
-                              const uint16_t* __restrict__ kw, const double* __restrict__ c_edge,
-                              const double* __restrict__ reac, const double* __restrict__ v8,
-                              const int32_t* __restrict__ diag_slot, double* __restrict__ val,
-                              double* __restrict__ vol_patch) {
-  const int64_t row = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (row >= N) return;
-  for (int64_t s = rowptr[row]; s < rowptr[row + 1]; ++s) {
+// Matrix values. A block owns kSvRows consecutive rows: their emissions (contiguous
+// after the symbolic sort) are loaded coalesced and evaluated in parallel into LDS,
+// then one thread per CSR slot adds its run of values in order, so every sum equals
+// the sequential sum of the reference's leaf sweep. Blocks whose rows hold more than
+// kSvCap emissions evaluate straight from global memory.
+constexpr int kSvRows = 32;
+constexpr int kSvCap = 4096;
+constexpr int kSvBS = 256;
+
+__global__ void __launch_bounds__(kSvBS) k_slot_values(int64_t N, int64_t L, const int32_t* __restrict__ rowptr,
+                                                       const int64_t* __restrict__ em_rowptr,
+                                                       const int64_t* __restrict__ slot_em,
+                                                       const uint64_t* __restrict__ key,
+                                                       const uint16_t* __restrict__ kw,
+                                                       const double* __restrict__ c_edge,
+                                                       const double* __restrict__ reac,
+                                                       const double* __restrict__ v8, double* __restrict__ val) {
+  __shared__ double s_v[kSvCap];
+  __shared__ int64_t s_rp[kSvRows + 1];
+  const int64_t r0 = (int64_t)blockIdx.x * kSvRows;
+  const int64_t r1 = min(r0 + kSvRows, N);
+  const int nr = (int)(r1 - r0);
+  for (int i = threadIdx.x; i <= nr; i += kSvBS) s_rp[i] = em_rowptr[r0 + i];
+  __syncthreads();
+  const int64_t e0 = s_rp[0], e1 = s_rp[nr];
+  const bool staged = e1 - e0 <= kSvCap;
+  if (staged) {
+    for (int64_t i = e0 + threadIdx.x; i < e1; i += kSvBS) {
+      // row of emission i: last r with s_rp[r] <= i
+      int lo = 0, hi = nr - 1;
+      while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (s_rp[mid] <= i) lo = mid;
+        else hi = mid - 1;
+      }
+      s_v[i - e0] = emission_value(key[i], kw[i], r0 + lo, L, c_edge, reac, v8);
+    }
+  }
+  __syncthreads();
+  const int64_t sl0 = rowptr[r0], sl1 = rowptr[r1];
+  for (int64_t sl = sl0 + threadIdx.x; sl < sl1; sl += kSvBS) {
+    const int64_t b = slot_em[2 * sl], e = slot_em[2 * sl + 1];
     double v = 0.0;
-    for (int64_t i = slot_em[2 * s]; i < slot_em[2 * s + 1]; ++i)
-      v += emission_value(key[i], kw[i], row, L, c_edge, reac, v8);
-    val[s] = v;
+    if (staged) {
+      for (int64_t i = b; i < e; ++i) v += s_v[i - e0];
+    } else {
+      int lo = 0, hi = nr - 1;
+      while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (s_rp[mid] <= b) lo = mid;
+        else hi = mid - 1;
+      }
+      for (int64_t i = b; i < e; ++i) v += emission_value(key[i], kw[i], r0 + lo, L, c_edge, reac, v8);
+    }
+    val[sl] = v;
   }
-  // patch volume: mass emissions of the diagonal with unit coefficient
-  const int64_t ds = diag_slot[row];
-  double vp = 0.0;
-  for (int64_t i = slot_em[2 * ds]; i < slot_em[2 * ds + 1]; ++i) {
-    const uint32_t id = (uint32_t)(key[i] & 0xffffffffULL);
-    if ((int64_t)id < 12 * L) continue;
-    vp += v8[((int64_t)id - 12 * L) >> 3] * decode_w(kw[i]);
-  }
-  vol_patch[row] = vp;
 }
 
-// charge projection (src/pb_class.cpp:2002-2039): contribution t = 32 a + 4 ii + jj
-__global__ void k_rho_push(MeshView m, int64_t A, const double* __restrict__ axyz, const double* __restrict__ aq,
-                           const int32_t* __restrict__ owner, const double* __restrict__ vol_patch,
-                           double* __restrict__ cval, int32_t* __restrict__ cnode, int32_t* __restrict__ cnext,
-                           int32_t* __restrict__ head) {
+// Charge projection (src/pb_class.cpp:2002-2039). Contribution t = 32 a + 4 ii + jj
+// (atom a, leaf corner ii, parent jj); each valid one is pushed onto the list of its
+// node.
+__global__ void k_rho_push(MeshView m, int64_t A, const int32_t* __restrict__ owner, int32_t* __restrict__ cnode,
+                           int32_t* __restrict__ cnext, int32_t* __restrict__ head) {
   const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= 32 * A) return;
   const int64_t a = t >> 5;
@@ -271,44 +304,42 @@ __global__ void k_rho_push(MeshView m, int64_t A, const double* __restrict__ axy
   if (l < 0) return;
   const int32_t c = m.leaf_corner[8 * l + ii];
   int32_t node;
-  int npi = 1;
   if (c >= 0) {
     if (jj != 0) return;
     node = c;
   } else {
     const int64_t h = -1 - (int64_t)c;
-    npi = m.hang_np[h];
-    if (jj >= npi) return;
+    if (jj >= m.hang_np[h]) return;
     node = m.hang_par[4 * h + jj];
   }
-  const int32_t c0 = m.leaf_corner[8 * l], c7 = m.leaf_corner[8 * l + 7];
-  const double volume = (dev::corner_coord(m, c7, 0) - dev::corner_coord(m, c0, 0)) *
-                        (dev::corner_coord(m, c7, 1) - dev::corner_coord(m, c0, 1)) *
-                        (dev::corner_coord(m, c7, 2) - dev::corner_coord(m, c0, 2));
-  const int32_t co = m.leaf_corner[8 * l + 7 - ii];
-  const double weight = fabs((axyz[3 * a] - dev::corner_coord(m, co, 0)) *
-                             (axyz[3 * a + 1] - dev::corner_coord(m, co, 1)) *
-                             (axyz[3 * a + 2] - dev::corner_coord(m, co, 2))) / volume;
-  double v;
-  if (c >= 0) {
-    v = aq[a] * 4.0 * kPi * weight / vol_patch[node];
-  } else {
-    const double denom = npi * vol_patch[node];
-    v = aq[a] * 4.0 * kPi * weight / denom;
-  }
-  cval[t] = v;
   cnode[t] = node;
   cnext[t] = atomicExch(&head[node], (int32_t)t);
 }
 
-// one thread per list: sums the node's contributions in increasing contribution index
-__global__ void k_rho_reduce(int64_t C, const double* __restrict__ cval, const int32_t* __restrict__ cnode,
-                             const int32_t* __restrict__ cnext, const int32_t* __restrict__ head,
-                             double* __restrict__ rho) {
+// One thread per touched node (the head of its list): the patch volume (the node's
+// lumped-mass emissions with unit coefficient, in order), the density
+// rho = sum over its contributions in increasing t of q 4 pi w / vol_patch
+// (/ (np vol_patch) through a hanging corner), and the right-hand side
+// sum over the same emissions of (V/8 w) rho. Untouched nodes keep rhs 0.
+__global__ void k_rho_node(MeshView m, int64_t C, int64_t L, const double* __restrict__ axyz,
+                           const double* __restrict__ aq, const int32_t* __restrict__ owner,
+                           const int32_t* __restrict__ cnode, const int32_t* __restrict__ cnext,
+                           const int32_t* __restrict__ head, const int32_t* __restrict__ diag_slot,
+                           const int64_t* __restrict__ slot_em, const uint64_t* __restrict__ key,
+                           const uint16_t* __restrict__ kw, const double* __restrict__ v8,
+                           double* __restrict__ rho, double* __restrict__ rhs_charge) {
   const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= C) return;
   const int32_t node = cnode[t];
   if (node < 0 || head[node] != (int32_t)t) return;
+  const int64_t ds = diag_slot[node];
+  const int64_t b = slot_em[2 * ds], e = slot_em[2 * ds + 1];
+  double vp = 0.0;
+  for (int64_t i = b; i < e; ++i) {
+    const uint32_t id = (uint32_t)(key[i] & 0xffffffffULL);
+    if ((int64_t)id < 12 * L) continue;
+    vp += v8[((int64_t)id - 12 * L) >> 3] * decode_w(kw[i]);
+  }
   double s = 0.0;
   int64_t last = -1;
   while (true) {
@@ -316,27 +347,35 @@ __global__ void k_rho_reduce(int64_t C, const double* __restrict__ cval, const i
     for (int32_t u = head[node]; u >= 0; u = cnext[u])
       if ((int64_t)u > last && (int64_t)u < best) best = u;
     if (best == INT64_MAX) break;
-    s += cval[best];
     last = best;
+    const int64_t a = best >> 5;
+    const int ii = (int)((best >> 2) & 7);
+    const int64_t l = owner[a];
+    const int32_t c = m.leaf_corner[8 * l + ii];
+    const int32_t c0 = m.leaf_corner[8 * l], c7 = m.leaf_corner[8 * l + 7];
+    const double volume = (dev::corner_coord(m, c7, 0) - dev::corner_coord(m, c0, 0)) *
+                          (dev::corner_coord(m, c7, 1) - dev::corner_coord(m, c0, 1)) *
+                          (dev::corner_coord(m, c7, 2) - dev::corner_coord(m, c0, 2));
+    const int32_t co = m.leaf_corner[8 * l + 7 - ii];
+    const double weight = fabs((axyz[3 * a] - dev::corner_coord(m, co, 0)) *
+                               (axyz[3 * a + 1] - dev::corner_coord(m, co, 1)) *
+                               (axyz[3 * a + 2] - dev::corner_coord(m, co, 2))) / volume;
+    if (c >= 0) {
+      s += aq[a] * 4.0 * kPi * weight / vp;
+    } else {
+      const int npi = m.hang_np[-1 - (int64_t)c];
+      const double denom = npi * vp;
+      s += aq[a] * 4.0 * kPi * weight / denom;
+    }
   }
   rho[node] = s;
-}
-
-__global__ void k_rhs_charge(int64_t N, int64_t L, const int64_t* __restrict__ slot_em,
-                             const uint64_t* __restrict__ key, const uint16_t* __restrict__ kw,
-                             const double* __restrict__ v8, const int32_t* __restrict__ diag_slot,
-                             const double* __restrict__ rho, double* __restrict__ rhs_charge) {
-  const int64_t row = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (row >= N) return;
-  const int64_t ds = diag_slot[row];
-  const double r = rho[row];
-  double s = 0.0;
-  for (int64_t i = slot_em[2 * ds]; i < slot_em[2 * ds + 1]; ++i) {
+  double r = 0.0;
+  for (int64_t i = b; i < e; ++i) {
     const uint32_t id = (uint32_t)(key[i] & 0xffffffffULL);
     if ((int64_t)id < 12 * L) continue;
-    s += v8[((int64_t)id - 12 * L) >> 3] * decode_w(kw[i]) * r;
+    r += v8[((int64_t)id - 12 * L) >> 3] * decode_w(kw[i]) * s;
   }
-  rhs_charge[row] = s;
+  rhs_charge[node] = r;
 }
 
 __global__ void k_dirichlet_rhs(int64_t N, int64_t L, const uint8_t* __restrict__ is_bnd,
@@ -443,36 +482,31 @@ void run_numeric(ngpb_ctx* c) {
   c->c_edge.resize((size_t)12 * L);
   c->v8.resize((size_t)L);
   c->val.resize((size_t)std::max<int64_t>(c->nnz, 1));
-  c->vol_patch.resize((size_t)N);
   c->rho.resize((size_t)N);
   c->rhs_charge.resize((size_t)N);
   c->rhs.resize((size_t)N);
   hipLaunchKernelGGL(k_leaf_coeffs, dim3(blocks_for(L, 128)), dim3(128), 0, c->stream, m, c->eps.p, c->border_of.p,
                      c->frac.p, c->c_edge.p, c->v8.p);
-  hipLaunchKernelGGL(k_slot_values, dim3(blocks_for(N, 128)), dim3(128), 0, c->stream, N, L, c->rowptr.p,
-                     c->slot_em.p, c->em_key.p, c->em_kw.p, c->c_edge.p, c->reac.p, c->v8.p, c->diag_slot.p,
-                     c->val.p, c->vol_patch.p);
+  hipLaunchKernelGGL(k_slot_values, dim3(blocks_for(N, kSvRows)), dim3(kSvBS), 0, c->stream, N, L, c->rowptr.p,
+                     c->em_rowptr.p, c->slot_em.p, c->em_key.p, c->em_kw.p, c->c_edge.p, c->reac.p, c->v8.p, c->val.p);
   NGPB_HIP(hipGetLastError());
-  // charge density
-  NGPB_HIP(hipMemsetAsync(c->rho.p, 0, N * sizeof(double), c->stream));
+  // charge density and its right-hand side, on the nodes the atoms touch
+  NGPB_HIP(hipMemsetAsync(c->rhs_charge.p, 0, N * sizeof(double), c->stream));
   if (c->have_charges && c->A > 0) {
     const int64_t C = 32 * c->A;
-    DBuf<double> cval;
     DBuf<int32_t> cnode, cnext, head;
-    cval.resize((size_t)C);
     cnode.resize((size_t)C);
     cnext.resize((size_t)C);
     head.resize((size_t)N);
     NGPB_HIP(hipMemsetAsync(head.p, 0xff, N * sizeof(int32_t), c->stream));
-    hipLaunchKernelGGL(k_rho_push, dim3(blocks_for(C, 256)), dim3(256), 0, c->stream, m, c->A, c->atom_xyz.p,
-                       c->atom_q.p, c->atom_owner.p, c->vol_patch.p, cval.p, cnode.p, cnext.p, head.p);
-    hipLaunchKernelGGL(k_rho_reduce, dim3(blocks_for(C, 256)), dim3(256), 0, c->stream, C, cval.p, cnode.p, cnext.p,
-                       head.p, c->rho.p);
+    hipLaunchKernelGGL(k_rho_push, dim3(blocks_for(C, 256)), dim3(256), 0, c->stream, m, c->A, c->atom_owner.p, cnode.p,
+                       cnext.p, head.p);
+    hipLaunchKernelGGL(k_rho_node, dim3(blocks_for(C, 256)), dim3(256), 0, c->stream, m, C, L, c->atom_xyz.p,
+                       c->atom_q.p, c->atom_owner.p, cnode.p, cnext.p, head.p, c->diag_slot.p, c->slot_em.p,
+                       c->em_key.p, c->em_kw.p, c->v8.p, c->rho.p, c->rhs_charge.p);
     NGPB_HIP(hipGetLastError());
     NGPB_HIP(hipStreamSynchronize(c->stream));
   }
-  hipLaunchKernelGGL(k_rhs_charge, dim3(blocks_for(N, 256)), dim3(256), 0, c->stream, N, L, c->slot_em.p,
-                     c->em_key.p, c->em_kw.p, c->v8.p, c->diag_slot.p, c->rho.p, c->rhs_charge.p);
   hipLaunchKernelGGL(k_dirichlet_rhs, dim3(blocks_for(N, 128)), dim3(128), 0, c->stream, N, L, c->is_bnd.p,
                      c->gval.p, c->diag_slot.p, c->val.p, c->bc_rowptr.p, c->bc_col.p, c->bc_em.p, c->em_key.p,
                      c->em_kw.p, c->c_edge.p, c->reac.p, c->v8.p, c->rhs_charge.p, c->rhs.p);

@@ -247,9 +247,10 @@ __global__ void k_gather_atoms(int64_t n, const int32_t* __restrict__ idx, const
 
 constexpr int kNB = 256;
 
-// MODE 0: flux, polarization-only branch  first += q * (flux / r)
-// MODE 1: flux, ionic branch             first += q * flux / r
-// MODE 2: triangle vertex                second += q phi dot / r^3 / (4 pi) * area / 3
+// MODE 0/1: flux sources, first += q flux / r
+// MODE 2:   triangle-vertex sources, second += q (phi area / (3 4 pi)) (d . n) / r^3
+// 1/r comes from rsqrt (about one ulp), so the terms are not rounded exactly like
+// the reference's sqrt and divisions; the sums agree to ~1e-13 relative.
 template <int MODE>
 __global__ void __launch_bounds__(kNB) k_nbody(int64_t ns, const double* __restrict__ src, int64_t nq,
                                                const double4* __restrict__ atoms, double* __restrict__ acc) {
@@ -260,6 +261,8 @@ __global__ void __launch_bounds__(kNB) k_nbody(int64_t ns, const double* __restr
   if (act)
     for (int k = 0; k < (MODE == 2 ? kVS : kFS); ++k) v[k] = src[(MODE == 2 ? kVS : kFS) * s + k];
   const double inv_4pi = 1.0 / (4.0 * kPi);
+  // per-source weight: flux, or phi_sup * area / 3 / (4 pi)
+  const double wsrc = MODE == 2 ? v[6] * inv_4pi * v[7] / 3.0 : v[3];
   double sum = 0.0;
   for (int64_t t0 = 0; t0 < nq; t0 += kNB) {
     __syncthreads();
@@ -267,28 +270,21 @@ __global__ void __launch_bounds__(kNB) k_nbody(int64_t ns, const double* __restr
     __syncthreads();
     const int cnt = (int)min((int64_t)kNB, nq - t0);
     if (!act) continue;
+#pragma unroll 4
     for (int j = 0; j < cnt; ++j) {
       const double4 at = s_at[j];
+      const double dx = v[0] - at.x, dy = v[1] - at.y, dz = v[2] - at.z;
+      const double r2 = dx * dx + dy * dy + dz * dz;
+      const double ir = rsqrt(r2);
       if (MODE < 2) {
-        const double dx = at.x - v[0], dy = at.y - v[1], dz = at.z - v[2];
-        const double r = sqrt(dx * dx + dy * dy + dz * dz);
-        if (MODE == 0) {
-          const double qflux = v[3] / r;
-          sum += at.w * qflux;
-        } else {
-          sum += at.w * v[3] / r;
-        }
+        sum += at.w * ir;
       } else {
-        const double dx = v[0] - at.x, dy = v[1] - at.y, dz = v[2] - at.z;
-        const double r2 = dx * dx + dy * dy + dz * dz;
-        const double r = sqrt(r2);
-        const double inv_r3 = 1.0 / (r2 * r);
         const double dot = dx * v[3] + dy * v[4] + dz * v[5];
-        sum += at.w * v[6] * dot * inv_r3 * inv_4pi * v[7] / 3.0;
+        sum += at.w * dot * (ir * ir * ir);
       }
     }
   }
-  if (act) acc[s] = sum;
+  if (act) acc[s] = sum * wsrc;
 }
 
 // out[0] = sum of a[0..na), out[1] = sum of b[0..nb), out[2] = sum of c[0..nc)

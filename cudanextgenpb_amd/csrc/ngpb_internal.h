@@ -217,6 +217,7 @@ struct ngpb_ctx {
   ngpb::KrylovScalars* ks_host = nullptr;  // pinned mirror
   ngpb::GridReduce red;
   bool have_phi = false;
+  bool in_batch = false;  // solver vectors are referenced by a batch: no reallocation
 
   // energy scratch
   ngpb::DBuf<double> en_src, en_acc;

@@ -42,33 +42,8 @@ constexpr int kBS = 256;
 constexpr int kW = 64;
 constexpr int kFin = 1024;
 
-__device__ __forceinline__ int lane_id() { return threadIdx.x & 63; }
+#include "ngpb_sell.inc"
 
-__device__ __forceinline__ int wave_min_i(int v) {
-#pragma unroll
-  for (int off = 32; off > 0; off >>= 1) v = min(v, __shfl_xor(v, off, 64));
-  return v;
-}
-__device__ __forceinline__ int wave_max_i(int v) {
-#pragma unroll
-  for (int off = 32; off > 0; off >>= 1) v = max(v, __shfl_xor(v, off, 64));
-  return v;
-}
-__device__ __forceinline__ int wave_sum_i(int v) {
-#pragma unroll
-  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
-  return v;
-}
-__device__ __forceinline__ int wave_excl_scan_i(int v) {
-  const int lane = lane_id();
-  int inc = v;
-#pragma unroll
-  for (int off = 1; off < 64; off <<= 1) {
-    const int t = __shfl_up(inc, off, 64);
-    if (lane >= off) inc += t;
-  }
-  return inc - v;
-}
 
 __global__ void k_scale(int64_t n, const int32_t* __restrict__ rowptr, const int32_t* __restrict__ col,
                         const double* __restrict__ val, double* __restrict__ dscale) {
@@ -237,227 +212,6 @@ __global__ void __launch_bounds__(kBS) k_sell_fill(int64_t n, int64_t nslices, c
       ov_val[start + k] = (float)(val[e] * di * dscale[cc]);
       ov_col[start + k] = cc;
     }
-  }
-}
-
-struct Sell {
-  int64_t n, nslices;
-  const int32_t* width;
-  const int64_t* base_ptr;
-  const int64_t* wide_ptr;
-  const int64_t* ov_ptr;
-  const float* val;
-  const uint16_t* off;
-  const int32_t* base;
-  const int32_t* col32;
-  const uint8_t* ov_cnt;
-  const float* ov_val;
-  const int32_t* ov_col;
-};
-
-// Delta slice with a compile-time slot count: every value and delta is requested
-// before any gather, and every gather before the first multiply-add, so one slice
-// costs two memory round trips.
-template <int ND>
-__device__ __forceinline__ double delta_rows(const Sell& A, int64_t bp, int64_t sp, int64_t row, int64_t last,
-                                             const float* __restrict__ x, double acc) {
-  float v[ND], xv[ND];
-  int d[ND];
-#pragma unroll
-  for (int j = 0; j < ND; ++j) {
-    v[j] = A.val[sp + (int64_t)j * kW];
-    d[j] = A.base[bp + j];
-  }
-#pragma unroll
-  for (int j = 0; j < ND; ++j) xv[j] = x[min(max(row + (int64_t)d[j], (int64_t)0), last)];
-#pragma unroll
-  for (int j = 0; j < ND; ++j) acc = fma((double)v[j], (double)xv[j], acc);
-  return acc;
-}
-
-// offset slice with 32-bit columns and a compile-time width
-template <int W>
-__device__ __forceinline__ double wide_rows(const Sell& A, int64_t sp, int64_t wp, const float* __restrict__ x,
-                                            double acc) {
-  float v[W], xv[W];
-  int c[W];
-#pragma unroll
-  for (int j = 0; j < W; ++j) {
-    v[j] = A.val[sp + (int64_t)j * kW];
-    c[j] = A.col32[wp + (int64_t)j * kW];
-  }
-#pragma unroll
-  for (int j = 0; j < W; ++j) xv[j] = x[c[j]];
-#pragma unroll
-  for (int j = 0; j < W; ++j) acc = fma((double)v[j], (double)xv[j], acc);
-  return acc;
-}
-
-// offset slice (16-bit offsets) with a compile-time width
-template <int W>
-__device__ __forceinline__ double offset_rows(const Sell& A, int64_t bp, int64_t sp, const float* __restrict__ x,
-                                              double acc) {
-  float v[W], xv[W];
-  int c[W];
-#pragma unroll
-  for (int j = 0; j < W; ++j) {
-    v[j] = A.val[sp + (int64_t)j * kW];
-    c[j] = A.base[bp + j] + (int)A.off[sp + (int64_t)j * kW];
-  }
-#pragma unroll
-  for (int j = 0; j < W; ++j) xv[j] = x[c[j]];
-#pragma unroll
-  for (int j = 0; j < W; ++j) acc = fma((double)v[j], (double)xv[j], acc);
-  return acc;
-}
-
-__device__ __forceinline__ double offset_rows_any(const Sell& A, int64_t s, int w, int64_t bp, int64_t sp, int64_t row,
-                                               bool in, const float* __restrict__ x, double acc) {
-  const int lane = lane_id();
-  const int64_t wp = A.wide_ptr[s], op = A.ov_ptr[s];
-  if (wp < 0) {
-    switch (w) {
-      case 0: break;
-      case 1: acc = offset_rows<1>(A, bp, sp, x, acc); break;
-      case 2: acc = offset_rows<2>(A, bp, sp, x, acc); break;
-      case 3: acc = offset_rows<3>(A, bp, sp, x, acc); break;
-      case 4: acc = offset_rows<4>(A, bp, sp, x, acc); break;
-      case 5: acc = offset_rows<5>(A, bp, sp, x, acc); break;
-      case 6: acc = offset_rows<6>(A, bp, sp, x, acc); break;
-      case 7: acc = offset_rows<7>(A, bp, sp, x, acc); break;
-      case 8: acc = offset_rows<8>(A, bp, sp, x, acc); break;
-      default:
-        for (int k0 = 0; k0 < w; k0 += 8) {
-          // eight slots per round: indices past the last slot repeat it, with weight 0
-          float v[8], xv[8];
-          int c[8];
-#pragma unroll
-          for (int j = 0; j < 8; ++j) {
-            const int k = min(k0 + j, w - 1);
-            v[j] = k0 + j < w ? A.val[sp + (int64_t)k * kW] : 0.f;
-            c[j] = A.base[bp + k] + (int)A.off[sp + (int64_t)k * kW];
-          }
-#pragma unroll
-          for (int j = 0; j < 8; ++j) xv[j] = x[c[j]];
-#pragma unroll
-          for (int j = 0; j < 8; ++j) acc = fma((double)v[j], (double)xv[j], acc);
-        }
-    }
-  } else {
-    const int64_t wl = wp + lane;
-    switch (w) {
-      case 0: break;
-      case 1: acc = wide_rows<1>(A, sp, wl, x, acc); break;
-      case 2: acc = wide_rows<2>(A, sp, wl, x, acc); break;
-      case 3: acc = wide_rows<3>(A, sp, wl, x, acc); break;
-      case 4: acc = wide_rows<4>(A, sp, wl, x, acc); break;
-      case 5: acc = wide_rows<5>(A, sp, wl, x, acc); break;
-      case 6: acc = wide_rows<6>(A, sp, wl, x, acc); break;
-      case 7: acc = wide_rows<7>(A, sp, wl, x, acc); break;
-      case 8: acc = wide_rows<8>(A, sp, wl, x, acc); break;
-      default:
-        for (int k0 = 0; k0 < w; k0 += 8) {
-          float v[8], xv[8];
-          int c[8];
-#pragma unroll
-          for (int j = 0; j < 8; ++j) {
-            const int k = min(k0 + j, w - 1);
-            v[j] = k0 + j < w ? A.val[sp + (int64_t)k * kW] : 0.f;
-            c[j] = A.col32[wl + (int64_t)k * kW];
-          }
-#pragma unroll
-          for (int j = 0; j < 8; ++j) xv[j] = x[c[j]];
-#pragma unroll
-          for (int j = 0; j < 8; ++j) acc = fma((double)v[j], (double)xv[j], acc);
-        }
-    }
-  }
-  if (op >= 0) {
-    const int cnt = in ? (int)A.ov_cnt[row] : 0;
-    const int64_t start = op + wave_excl_scan_i(cnt);
-    for (int k = 0; k < cnt; ++k) acc = fma((double)A.ov_val[start + k], (double)x[A.ov_col[start + k]], acc);
-  }
-  return acc;
-}
-
-// one slice of any form: q = x_i + off-diagonal part (the unit diagonal is added
-// last so that the x_i load overlaps the gathers)
-__device__ __forceinline__ void one_slice(const Sell& A, int64_t s, int w, int64_t bp, const float* __restrict__ x,
-                                          float* __restrict__ y, double* part) {
-  const int lane = lane_id();
-  const int64_t last = A.n - 1;
-  const int64_t row = s * kW + lane;
-  const int64_t sp = bp * kW + lane;
-  const bool in = row < A.n;
-  const float xi = x[min(row, last)];
-  double acc = 0.0;
-  switch (w) {
-    case -1: acc = delta_rows<1>(A, bp, sp, row, last, x, acc); break;
-    case -2: acc = delta_rows<2>(A, bp, sp, row, last, x, acc); break;
-    case -3: acc = delta_rows<3>(A, bp, sp, row, last, x, acc); break;
-    case -4: acc = delta_rows<4>(A, bp, sp, row, last, x, acc); break;
-    case -5: acc = delta_rows<5>(A, bp, sp, row, last, x, acc); break;
-    case -6: acc = delta_rows<6>(A, bp, sp, row, last, x, acc); break;
-    case -7: acc = delta_rows<7>(A, bp, sp, row, last, x, acc); break;
-    case -8: acc = delta_rows<8>(A, bp, sp, row, last, x, acc); break;
-    case -9: acc = delta_rows<9>(A, bp, sp, row, last, x, acc); break;
-    case -10: acc = delta_rows<10>(A, bp, sp, row, last, x, acc); break;
-    case -11: acc = delta_rows<11>(A, bp, sp, row, last, x, acc); break;
-    case -12: acc = delta_rows<12>(A, bp, sp, row, last, x, acc); break;
-    case -13: acc = delta_rows<13>(A, bp, sp, row, last, x, acc); break;
-    case -14: acc = delta_rows<14>(A, bp, sp, row, last, x, acc); break;
-    case -15: acc = delta_rows<15>(A, bp, sp, row, last, x, acc); break;
-    case -16: acc = delta_rows<16>(A, bp, sp, row, last, x, acc); break;
-    default: acc = offset_rows_any(A, s, w, bp, sp, row, in, x, acc);
-  }
-  acc += (double)xi;
-  if (in) {
-    const float yi = (float)acc;
-    y[row] = yi;
-    *part += (double)xi * (double)yi;
-  }
-}
-
-// two adjacent delta slices (s0, s0 + 1) with ND slots each
-template <int ND>
-__device__ __forceinline__ void delta_pair(const Sell& A, int64_t s0, int64_t b0, int64_t b1,
-                                           const float* __restrict__ x, float* __restrict__ y, double* part) {
-  const int lane = lane_id();
-  const int64_t last = A.n - 1;
-  const int64_t r0 = s0 * kW + lane, r1 = r0 + kW;
-  const int64_t p0 = b0 * kW + lane, p1 = b1 * kW + lane;
-  float v0[ND], v1[ND], x0[ND], x1[ND];
-  int d0[ND], d1[ND];
-#pragma unroll
-  for (int j = 0; j < ND; ++j) {
-    v0[j] = A.val[p0 + (int64_t)j * kW];
-    v1[j] = A.val[p1 + (int64_t)j * kW];
-    d0[j] = A.base[b0 + j];
-    d1[j] = A.base[b1 + j];
-  }
-  const float xi0 = x[min(r0, last)], xi1 = x[min(r1, last)];
-#pragma unroll
-  for (int j = 0; j < ND; ++j) {
-    x0[j] = x[min(max(r0 + (int64_t)d0[j], (int64_t)0), last)];
-    x1[j] = x[min(max(r1 + (int64_t)d1[j], (int64_t)0), last)];
-  }
-  double a0 = 0.0, a1 = 0.0;
-#pragma unroll
-  for (int j = 0; j < ND; ++j) {
-    a0 = fma((double)v0[j], (double)x0[j], a0);
-    a1 = fma((double)v1[j], (double)x1[j], a1);
-  }
-  a0 += (double)xi0;
-  a1 += (double)xi1;
-  if (r0 < A.n) {
-    const float yi = (float)a0;
-    y[r0] = yi;
-    *part += (double)xi0 * (double)yi;
-  }
-  if (r1 < A.n) {
-    const float yi = (float)a1;
-    y[r1] = yi;
-    *part += (double)xi1 * (double)yi;
   }
 }
 

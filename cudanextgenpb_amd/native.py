@@ -92,6 +92,17 @@ SIGNATURES = {
     "ngpb_kernel_stats": (_I, [_P, _I, _P, _P, _P]),
     "ngpb_time_copy": (_I, [_P, _I64, _I, _P]),
     "ngpb_operator_info": (_I, [_P, _P]),
+    "ngpb_batch_create": (_I, [_P, _I, _I]),
+    "ngpb_batch_destroy": (_I, [_P]),
+    "ngpb_batch_last_error": (ctypes.c_char_p, [_P]),
+    "ngpb_batch_set_options": (_I, [_P, _P, _I]),
+    "ngpb_batch_add": (_I, [_P, _P, _P]),
+    "ngpb_batch_remove": (_I, [_P, _I]),
+    "ngpb_batch_autotune": (_I, [_P, _P, _P, _P]),
+    "ngpb_batch_set_tiling": (_I, [_P, _I, _I]),
+    "ngpb_batch_run": (_I, [_P, _I, _P]),
+    "ngpb_batch_status": (_I, [_P, _I, _P, _P]),
+    "ngpb_batch_timing": (_I, [_P, _P, _P]),
 }
 
 _lib = None
@@ -308,3 +319,92 @@ class Context:
         ms, b = ctypes.c_double(0), ctypes.c_double(0)
         self._check(lib().ngpb_time_spmv(self._h, precision, reps, ctypes.byref(ms), ctypes.byref(b)), "time_spmv")
         return ms.value, b.value
+
+
+class Batch:
+    """Many independent systems solved together in mixed precision (wraps ngpb_batch).
+
+    Each system is an assembled Context; while it is in the batch the Context accepts
+    only read calls. Systems join at replay boundaries; one replay is an FP64 residual
+    replacement step plus `inner_per_replay` FP32 CG iterations of every system.
+    """
+
+    def __init__(self, device: int = 0, max_systems: int = 1024, tol: float = 1e-10, maxiter: int = 100000,
+                 drop: float = 0.05, inner_per_replay: int = 8):
+        self._h = ctypes.c_void_p()
+        rc = lib().ngpb_batch_create(ctypes.byref(self._h), device, max_systems)
+        if rc != NGPB_OK:
+            raise NGPBError(rc, f"ngpb_batch_create(device={device}, max_systems={max_systems}) failed")
+        self.members = {}
+        self.set_options(tol, maxiter, drop, inner_per_replay)
+
+    def _check(self, rc, what):
+        if rc != NGPB_OK:
+            msg = lib().ngpb_batch_last_error(self._h)
+            raise NGPBError(rc, f"{what}: {msg.decode() if msg else ''}")
+
+    def close(self):
+        if self._h:
+            lib().ngpb_batch_destroy(self._h)
+            self._h = ctypes.c_void_p()
+            self.members = {}
+
+    def __del__(self):  # pragma: no cover
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def set_options(self, tol=1e-10, maxiter=100000, drop=0.05, inner_per_replay=8):
+        o = SolveOpts(PREC_MIXED, tol, maxiter, drop, 1)
+        self._check(lib().ngpb_batch_set_options(self._h, ctypes.byref(o), int(inner_per_replay)), "batch_set_options")
+
+    def add(self, ctx: "Context") -> int:
+        slot = ctypes.c_int(-1)
+        self._check(lib().ngpb_batch_add(self._h, ctx._h, ctypes.byref(slot)), "batch_add")
+        self.members[slot.value] = ctx
+        return slot.value
+
+    def remove(self, slot: int):
+        self._check(lib().ngpb_batch_remove(self._h, int(slot)), "batch_remove")
+        self.members.pop(slot, None)
+
+    def autotune(self):
+        ts, w, ms = ctypes.c_int(0), ctypes.c_int(0), ctypes.c_double(0)
+        self._check(lib().ngpb_batch_autotune(self._h, ctypes.byref(ts), ctypes.byref(w), ctypes.byref(ms)),
+                    "batch_autotune")
+        return dict(tile_slices=ts.value, waves_per_block=w.value, ms_per_spmv=ms.value)
+
+    def set_tiling(self, tile_slices: int, waves_per_block: int):
+        self._check(lib().ngpb_batch_set_tiling(self._h, int(tile_slices), int(waves_per_block)), "batch_set_tiling")
+
+    def run(self, max_replays: int = 1) -> int:
+        left = ctypes.c_int(0)
+        self._check(lib().ngpb_batch_run(self._h, int(max_replays), ctypes.byref(left)), "batch_run")
+        return left.value
+
+    def status(self, slot: int):
+        done, st = ctypes.c_int(0), SolveStats()
+        self._check(lib().ngpb_batch_status(self._h, int(slot), ctypes.byref(done), ctypes.byref(st)), "batch_status")
+        return bool(done.value), st.as_dict()
+
+    def timing(self):
+        r, ms = ctypes.c_int64(0), ctypes.c_double(0)
+        self._check(lib().ngpb_batch_timing(self._h, ctypes.byref(r), ctypes.byref(ms)), "batch_timing")
+        return dict(replays=r.value, ms_total=ms.value)
+
+    def solve_all(self, max_replays: int = 1 << 20) -> int:
+        """Runs replays until every system is done; returns the replays run."""
+        t0 = self.timing()["replays"]
+        left = 1
+        while left > 0:
+            left = self.run(64)
+            if self.timing()["replays"] - t0 >= max_replays:
+                break
+        return self.timing()["replays"] - t0

@@ -268,14 +268,22 @@ def _molecule_problem(name, atoms, h_fine, levels, bands, domain_factor, bc_type
 def ligand(index: int, base_seed: int = 512, h: float = 0.4, physics: Optional[Physics] = None,
            bc_type: int = 2) -> Problem:
     """Ligand-like molecule `index` of the batched configuration: U{20..80} atoms in a
-    6-10 A ball, radii U[1.2, 1.9], charges N(0, 0.3), uniform leaves of edge h over a
-    cube 3x the molecule extent (rounded to a whole number of leaves)."""
+    6-10 A ball (seed base_seed*1000 + index), radii U[1.2, 1.9], charges N(0, 0.3),
+    uniform leaves of edge h on a cube of n^3 leaves centred on the molecule, with
+    n = 1.6x the molecule extent in leaves rounded up to a multiple of 4 and clamped to
+    [48, 96] (48^3 to 96^3 leaves at h = 0.4 A)."""
     seed = base_seed * 1000 + index
     rng = np.random.Generator(np.random.PCG64(seed))
     n_atoms = int(rng.integers(20, 81))
     ball = float(rng.uniform(6.0, 10.0))
     atoms = make_atoms(seed, n_atoms, [0.0, 0.0, 0.0], [ball] * 3, 1.2, 1.2, 1.9, 0.3, float("nan"))
-    return _molecule_problem(f"ligand_{index}", atoms, h, 0, [], 3.0, bc_type, physics)
+    lo = (atoms.xyz - atoms.r[:, None]).min(axis=0)
+    hi = (atoms.xyz + atoms.r[:, None]).max(axis=0)
+    extent = float((hi - lo).max())
+    n = int(min(96, max(48, 4 * math.ceil(1.6 * extent / (4 * h)))))
+    origin = 0.5 * (lo + hi) - n * h / 2
+    mesh = make_mesh(origin, h, [n, n, n], 0, [], atoms)
+    return _assemble_problem(f"ligand_{index}", physics or Physics(), mesh, atoms, bc_type)
 
 
 def cluster(atoms_xyz, radii, charges, h_fine: float = 0.5, levels: int = 2, bands=(1.5, 4.0),

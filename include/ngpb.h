@@ -184,6 +184,35 @@ int ngpb_time_copy(ngpb_ctx* ctx, int64_t bytes, int reps, double* ms_per_launch
    reference. */
 int ngpb_operator_info(ngpb_ctx* ctx, int64_t* info8);
 
+/* ---- batched solve of many independent systems ------------------------------------
+ * No single counterpart in the reference, which solves one system per run
+ * (lis_compute_electric_potential, src/pb_class.cpp:2243-2329); a batch runs that
+ * solve (mixed precision) for many contexts at once. Each system keeps its own
+ * context (assembled with ngpb_assemble); the batch references the context's operator
+ * and vectors, so a context in a batch accepts only read calls (ngpb_get_phi,
+ * ngpb_energy, ngpb_export_csr ...) until ngpb_batch_remove. Systems join at replay
+ * boundaries; one replay of the batch's captured hipGraph is one FP64 residual
+ * replacement step plus inner_per_replay FP32 CG iterations over every system. A
+ * system's results do not depend on the other systems in the batch or on the tiling. */
+typedef struct ngpb_batch ngpb_batch;
+int ngpb_batch_create(ngpb_batch** batch, int device, int max_systems);
+int ngpb_batch_destroy(ngpb_batch* batch);
+const char* ngpb_batch_last_error(const ngpb_batch* batch);
+/* tol (||b - A x|| <= tol ||b||, FP64), maxiter (inner iterations), drop; applies to
+   systems added afterwards. */
+int ngpb_batch_set_options(ngpb_batch* batch, const ngpb_solve_opts* opts, int inner_per_replay);
+int ngpb_batch_add(ngpb_batch* batch, ngpb_ctx* ctx, int* slot);
+int ngpb_batch_remove(ngpb_batch* batch, int slot);
+/* Autotuner: times the batched compact SpMV for tile sizes (slices of 64 rows per
+   block) and waves per block over the current systems and keeps the fastest. */
+int ngpb_batch_autotune(ngpb_batch* batch, int* tile_slices, int* waves_per_block, double* ms_per_spmv);
+int ngpb_batch_set_tiling(ngpb_batch* batch, int tile_slices, int waves_per_block);
+/* Up to max_replays replays, returning early once every system is done;
+   *running = systems not yet done. */
+int ngpb_batch_run(ngpb_batch* batch, int max_replays, int* running);
+int ngpb_batch_status(ngpb_batch* batch, int slot, int* done, ngpb_solve_stats* stats);
+int ngpb_batch_timing(ngpb_batch* batch, int64_t* replays, double* ms_total);
+
 #ifdef __cplusplus
 }
 #endif
