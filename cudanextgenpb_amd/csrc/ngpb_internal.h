@@ -209,6 +209,9 @@ struct ngpb_ctx {
   ngpb::DBuf<float> sell_ov_val;
   ngpb::DBuf<int32_t> sell_ov_col;
   ngpb::DBuf<double> dscale;         // 1/sqrt(diag)
+  int spmv_sched = 0;                // SpMV work schedule (0, 1, 2)
+  int sell_resident = 0;             // resident blocks of the compact SpMV (0: not yet queried)
+  int csr_resident = 0;              // resident blocks of the FP64 CSR SpMV (0: not yet queried)
 
   // solver state
   ngpb::DBuf<double> x, r, z, p, q, dinv;

@@ -28,7 +28,11 @@ constexpr int kBS = 256;
 constexpr int kCap = 2048;  // staged entries per block
 constexpr int kFin = 1024;  // finalize block size
 
-template <bool DOT>
+// SCHED 0: block b takes chunks b, b + G, ...; SCHED 1: block b takes one contiguous
+// range of chunks, with blocks renumbered so that the blocks on one XCD (block id
+// mod 8) cover one contiguous eighth of the rows (L2 reuse of the x gathers); the
+// grid is then the number of blocks resident at once.
+template <bool DOT, int SCHED>
 __global__ void __launch_bounds__(kBS) k_spmv64(int64_t n, const int32_t* __restrict__ rowptr,
                                                  const int32_t* __restrict__ col, const double* __restrict__ val,
                                                  const double* __restrict__ x, double* __restrict__ y,
@@ -39,7 +43,15 @@ __global__ void __launch_bounds__(kBS) k_spmv64(int64_t n, const int32_t* __rest
   if (DOT && ks->done) return;
   double part = 0.0;
   const int64_t nchunks = (n + kBS - 1) / kBS;
-  for (int64_t chunk = blockIdx.x; chunk < nchunks; chunk += gridDim.x) {
+  int64_t c_begin = blockIdx.x, c_end = nchunks, c_step = gridDim.x;
+  if (SCHED == 1) {
+    const unsigned nb = gridDim.x, b = blockIdx.x;
+    const int64_t lb = (nb % 8u == 0u) ? (int64_t)((b % 8u) * (nb / 8u) + b / 8u) : (int64_t)b;
+    c_begin = lb * nchunks / nb;
+    c_end = (lb + 1) * nchunks / nb;
+    c_step = 1;
+  }
+  for (int64_t chunk = c_begin; chunk < c_end; chunk += c_step) {
     const int64_t r0 = chunk * kBS;
     const int64_t r1 = min(r0 + kBS, n);
     const int64_t row = r0 + threadIdx.x;
@@ -84,8 +96,28 @@ __global__ void __launch_bounds__(kBS) k_spmv64(int64_t n, const int32_t* __rest
   }
 }
 
-unsigned spmv64_grid(int64_t n) {
-  return (unsigned)std::max<int64_t>(1, std::min<int64_t>((n + kBS - 1) / kBS, 2048));
+unsigned spmv64_grid(ngpb_ctx* c) {
+  const int64_t need = std::max<int64_t>(1, (c->n + kBS - 1) / kBS);
+  if (c->spmv_sched == 0) return (unsigned)std::min<int64_t>(need, 2048);
+  if (c->csr_resident <= 0) {
+    int cus = 0, per_cu = 0;
+    NGPB_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, c->device));
+    NGPB_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_spmv64<true, 1>, kBS, 0));
+    c->csr_resident = std::max(8, (std::max(cus, 1) * std::max(per_cu, 1)) / 8 * 8);
+  }
+  if (need >= c->csr_resident) return (unsigned)c->csr_resident;
+  return (unsigned)(c->spmv_sched == 1 ? (need + 7) / 8 * 8 : need);
+}
+
+template <bool DOT>
+void launch_spmv64(ngpb_ctx* c, unsigned grid, const double* x, double* y, double* part) {
+  const KrylovScalars* ks = DOT ? c->ks.p : nullptr;
+  if (c->spmv_sched == 1)
+    hipLaunchKernelGGL((k_spmv64<DOT, 1>), dim3(grid), dim3(kBS), 0, c->stream, c->n, c->rowptr.p, c->col.p, c->val.p,
+                       x, y, ks, part);
+  else
+    hipLaunchKernelGGL((k_spmv64<DOT, 0>), dim3(grid), dim3(kBS), 0, c->stream, c->n, c->rowptr.p, c->col.p, c->val.p,
+                       x, y, ks, part);
 }
 
 // x = 0, r = b, p = D^-1 b; per block (r, D^-1 r), (b, b)
@@ -298,8 +330,7 @@ double spmv_fp64_bytes(const ngpb_ctx* c) {
 }
 
 void spmv_fp64(ngpb_ctx* c, const double* x, double* y) {
-  hipLaunchKernelGGL(k_spmv64<false>, dim3(spmv64_grid(c->n)), dim3(kBS), 0, c->stream, c->n, c->rowptr.p, c->col.p,
-                     c->val.p, x, y, nullptr, nullptr);
+  launch_spmv64<false>(c, spmv64_grid(c), x, y, nullptr);
   NGPB_HIP(hipGetLastError());
 }
 
@@ -344,7 +375,7 @@ void solve_fp64(ngpb_ctx* c, const ngpb_solve_opts& o, ngpb_solve_stats& st) {
   c->r.resize((size_t)n);
   c->p.reserve((size_t)n);
   c->q.reserve((size_t)n);
-  const unsigned gs = spmv64_grid(n), gv = vec_grid(n);
+  const unsigned gs = spmv64_grid(c), gv = vec_grid(n);
   ensure_reduce(c, std::max<int64_t>(gs, 2048) + 2 * (int64_t)gv);
   double* part = c->red.partials.p;
   KrylovScalars* ks = c->ks.p;
@@ -364,8 +395,7 @@ void solve_fp64(ngpb_ctx* c, const ngpb_solve_opts& o, ngpb_solve_stats& st) {
     for (int k = 0; k < chunk; ++k, ++launch) {
       const int par = (int)(launch & 1);
       prof_begin(c, 0, launch);
-      hipLaunchKernelGGL(k_spmv64<true>, dim3(gs), dim3(kBS), 0, c->stream, n, c->rowptr.p, c->col.p, c->val.p, c->p.p,
-                         c->q.p, ks, ppq);
+      launch_spmv64<true>(c, gs, c->p.p, c->q.p, ppq);
       prof_end(c, 0);
       hipLaunchKernelGGL(k_pcg_update, dim3(gv), dim3(kBS), 0, c->stream, n, c->dinv.p, c->x.p, c->r.p, c->p.p, c->q.p,
                          ks, ppq, (int)gs, pup, par);

@@ -215,24 +215,42 @@ __global__ void __launch_bounds__(kBS) k_sell_fill(int64_t n, int64_t nslices, c
   }
 }
 
-// q = Ahat p with per-block (p, q) when DOT
-template <bool DOT>
+// q = Ahat p with per-block (p, q) when DOT.
+// Work is counted in pairs of adjacent slices. SCHED selects who takes which pairs:
+//   0  wave w takes pairs w, w + W, w + 2W, ... (W waves in the grid)
+//   1  each wave takes one contiguous range of pairs; blocks are renumbered so that
+//      the blocks the hardware places on one XCD (block id mod 8) cover one
+//      contiguous eighth of the rows, which keeps the x gathers of neighbouring
+//      slices (and of the +-plane neighbours) in that XCD's L2
+//   2  as 0; the grid is sized to the resident waves (no second round of blocks)
+// The grid for 1 and 2 is the number of blocks that fit on the device at once.
+template <bool DOT, int SCHED>
 __global__ void __launch_bounds__(kBS) k_sell_spmv(Sell A, const float* __restrict__ x, float* __restrict__ y,
                                                    const KrylovScalars* ks, double* __restrict__ partials) {
   __shared__ double s_red[kBS / 64];
   if (DOT && ks->done) return;
-  const int wave = __builtin_amdgcn_readfirstlane((int)(blockIdx.x * (kBS / kW) + (threadIdx.x >> 6)));
-  const int64_t nwaves = (int64_t)gridDim.x * (kBS / kW);
   const int64_t npairs = (A.nslices + 1) / 2;
+  const int64_t nwaves = (int64_t)gridDim.x * (kBS / kW);
+  int64_t t, t_end, step;
+  if (SCHED == 1) {
+    const unsigned nb = gridDim.x, b = blockIdx.x;
+    const unsigned lb = (nb % 8u == 0u) ? (b % 8u) * (nb / 8u) + b / 8u : b;
+    const int64_t lw = __builtin_amdgcn_readfirstlane((int)(lb * (kBS / kW) + (threadIdx.x >> 6)));
+    t = lw * npairs / nwaves;
+    t_end = (lw + 1) * npairs / nwaves;
+    step = 1;
+  } else {
+    t = __builtin_amdgcn_readfirstlane((int)(blockIdx.x * (kBS / kW) + (threadIdx.x >> 6)));
+    t_end = npairs;
+    step = nwaves;
+  }
   double part = 0.0;
-  // a wave takes adjacent slices (2t, 2t+1) for t = wave, wave + nwaves, ...; two
-  // delta slices with the same slot count are processed together so that twice as
-  // many loads are in flight; the next pair's widths and slot pointers are requested
-  // before the current pair is processed
-  int64_t t = wave;
+  // two delta slices with the same slot count are processed together so that twice
+  // as many loads are in flight; the next pair's widths and slot pointers are
+  // requested before the current pair is processed
   int w0n = 0, w1n = 0;
   int64_t b0n = 0, b1n = 0;
-  if (t < npairs) {
+  if (t < t_end) {
     w0n = A.width[2 * t];
     b0n = A.base_ptr[2 * t];
     if (2 * t + 1 < A.nslices) {
@@ -240,13 +258,13 @@ __global__ void __launch_bounds__(kBS) k_sell_spmv(Sell A, const float* __restri
       b1n = A.base_ptr[2 * t + 1];
     }
   }
-  while (t < npairs) {
+  while (t < t_end) {
     const int w0 = w0n, w1 = w1n;
     const int64_t b0 = b0n, b1 = b1n;
     const int64_t s0 = 2 * t;
     const bool has1 = s0 + 1 < A.nslices;
-    const int64_t t2 = t + nwaves;
-    if (t2 < npairs) {
+    const int64_t t2 = t + step;
+    if (t2 < t_end) {
       w0n = A.width[2 * t2];
       b0n = A.base_ptr[2 * t2];
       if (2 * t2 + 1 < A.nslices) {
@@ -437,18 +455,42 @@ Sell sell_view(const ngpb_ctx* c) {
   return A;
 }
 
-// at most 2048 blocks of 4 waves (8 blocks per CU); each wave walks several slices
-unsigned sell_grid(const ngpb_ctx* c) {
+// blocks of k_sell_spmv<.., SCHED> resident on the device at once (multiple of 8)
+unsigned sell_resident(ngpb_ctx* c) {
+  if (c->sell_resident > 0) return (unsigned)c->sell_resident;
+  int cus = 0, per_cu = 0;
+  NGPB_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, c->device));
+  NGPB_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_sell_spmv<true, 1>, kBS, 0));
+  const int r = std::max(8, (std::max(cus, 1) * std::max(per_cu, 1)) / 8 * 8);
+  c->sell_resident = r;
+  return (unsigned)r;
+}
+
+// schedule 0: at most 2048 blocks of 4 waves, each wave walks several pairs;
+// schedules 1 and 2: the resident block count (fewer if there are fewer pairs)
+unsigned sell_grid(ngpb_ctx* c) {
   const int64_t pairs = (c->n_slices + 1) / 2;
-  return (unsigned)std::max<int64_t>(1, std::min<int64_t>((pairs + (kBS / kW) - 1) / (kBS / kW), 2048));
+  const int64_t need = std::max<int64_t>(1, (pairs + (kBS / kW) - 1) / (kBS / kW));
+  if (c->spmv_sched == 0) return (unsigned)std::min<int64_t>(need, 2048);
+  const int64_t r = sell_resident(c);
+  if (need >= r) return (unsigned)r;
+  return (unsigned)(c->spmv_sched == 1 ? (need + 7) / 8 * 8 : need);
+}
+
+template <bool DOT>
+void launch_sell_t(ngpb_ctx* c, const float* x, float* y, double* part) {
+  const dim3 g(sell_grid(c)), b(kBS);
+  const KrylovScalars* ks = DOT ? c->ks.p : nullptr;
+  switch (c->spmv_sched) {
+    case 0: hipLaunchKernelGGL((k_sell_spmv<DOT, 0>), g, b, 0, c->stream, sell_view(c), x, y, ks, part); break;
+    case 1: hipLaunchKernelGGL((k_sell_spmv<DOT, 1>), g, b, 0, c->stream, sell_view(c), x, y, ks, part); break;
+    default: hipLaunchKernelGGL((k_sell_spmv<DOT, 2>), g, b, 0, c->stream, sell_view(c), x, y, ks, part); break;
+  }
 }
 
 void launch_sell(ngpb_ctx* c, bool dot, const float* x, float* y, double* part) {
-  if (dot)
-    hipLaunchKernelGGL(k_sell_spmv<true>, dim3(sell_grid(c)), dim3(kBS), 0, c->stream, sell_view(c), x, y, c->ks.p, part);
-  else
-    hipLaunchKernelGGL(k_sell_spmv<false>, dim3(sell_grid(c)), dim3(kBS), 0, c->stream, sell_view(c), x, y, nullptr,
-                       nullptr);
+  if (dot) launch_sell_t<true>(c, x, y, part);
+  else launch_sell_t<false>(c, x, y, nullptr);
 }
 
 }  // namespace

@@ -92,6 +92,7 @@ SIGNATURES = {
     "ngpb_kernel_stats": (_I, [_P, _I, _P, _P, _P]),
     "ngpb_time_copy": (_I, [_P, _I64, _I, _P]),
     "ngpb_operator_info": (_I, [_P, _P]),
+    "ngpb_set_spmv_schedule": (_I, [_P, _I]),
     "ngpb_batch_create": (_I, [_P, _I, _I]),
     "ngpb_batch_destroy": (_I, [_P]),
     "ngpb_batch_last_error": (ctypes.c_char_p, [_P]),
@@ -314,6 +315,9 @@ class Context:
         keys = ["slices", "delta_slices", "delta_entries", "offset_entries", "wide_entries", "overflow_entries",
                 "compact_spmv_bytes", "fp64_spmv_bytes"]
         return dict(zip(keys, (int(v) for v in info)))
+
+    def set_spmv_schedule(self, schedule: int):
+        self._check(lib().ngpb_set_spmv_schedule(self._h, int(schedule)), "set_spmv_schedule")
 
     def time_spmv(self, precision=PREC_FP64, reps=50):
         ms, b = ctypes.c_double(0), ctypes.c_double(0)

@@ -47,16 +47,22 @@ class Request:
     predicted_bytes: int
 
 
-def predict_bytes(mesh: W.Mesh) -> int:
+def predict_bytes(problem: W.Problem) -> int:
     """Upper estimate of the device bytes one system holds through markers, assembly,
-    mixed solve and energy, from its mesh sizes alone (before anything is uploaded).
-    Per node: coordinates 24, markers 16, Dirichlet 9, symbolic emission lists ~8 entries
-    x 26 B, CSR ~7 x 12, compact operator ~7 x 6, FP64/FP32 vectors 72; per leaf: corners
-    32, marker 1, fractions/normals of border leaves (bounded by 49 B x leaves)."""
+    mixed solve and energy, from the mesh sizes and the atoms alone (before anything
+    is uploaded). Per node: coordinates 24, markers 16, Dirichlet 9, symbolic emission
+    lists ~8 entries x 26 B, CSR ~7 x 12, compact operator ~7 x 6, solver vectors 72 and
+    reduction/scan scratch 64; per leaf: corners 32, marker and border map 5, assembly
+    coefficients 104; per border leaf (fractions, normals and the energy's source and
+    accumulator arrays) 3.6 KB, with the border leaves bounded by the atoms' summed
+    sphere areas over the finest leaf face (the surface of their union is smaller)."""
+    mesh = problem.mesh
     N, L, H = mesh.n_nodes, mesh.n_leaves, mesh.n_hang
     per_node = 24 + 16 + 9 + 8 * 26 + 7 * 12 + 8 + 7 * 6 + 72 + 64
-    per_leaf = 32 + 1 + 4 + 49 + 12 * 8
-    return int(1.25 * (per_node * N + per_leaf * L + 64 * H) + (8 << 20))
+    per_leaf = 32 + 5 + 104
+    area = float(np.sum(4.0 * np.pi * problem.atoms.r ** 2))
+    border = min(L, int(area / (mesh.hf * mesh.hf)) + 1)
+    return int(1.25 * (per_node * N + per_leaf * L + 64 * H + 3600 * border) + (8 << 20))
 
 
 def make_request_stream(n_requests: int = 48, rate_hz: float = 4.0, seed: int = 5, atoms_lo: int = 50,
@@ -77,7 +83,7 @@ def make_request_stream(n_requests: int = 48, rate_hz: float = 4.0, seed: int = 
         qnet = -3.0 * na / 2500.0
         P = W.globular(seed=seed * 100000 + k, n_atoms=na, ball_radius=radius, min_sep=2.0, h_fine=h_fine,
                        levels=2, fine_band=fine_band, mid_band=mid_band, qnet=round(qnet))
-        out.append(Request(k, t, P, P.mesh.n_nodes, P.mesh.n_leaves, predict_bytes(P.mesh)))
+        out.append(Request(k, t, P, P.mesh.n_nodes, P.mesh.n_leaves, predict_bytes(P)))
     return out
 
 

@@ -183,6 +183,14 @@ int ngpb_time_copy(ngpb_ctx* ctx, int64_t bytes, int reps, double* ms_per_launch
    compact SpMV, [7] algorithmic bytes of one FP64 CSR SpMV. No counterpart in the
    reference. */
 int ngpb_operator_info(ngpb_ctx* ctx, int64_t* info8);
+/* Work schedule of the SpMV kernels (compact FP32 and FP64 CSR): 0 (default)
+   grid-stride over slices / row chunks on up to 2048 blocks, 1 one contiguous range
+   per wave (block) on the resident grid with XCD-aware block numbering, 2 grid-stride
+   on the resident grid. Results differ only in the grouping of the partial sums of
+   (p, q). On MI355X, 1 measured 1.4-1.5x slower than 0 and 2 (the grid-stride window
+   keeps the gathers of all waves within a small moving range of x). No counterpart in
+   the reference. */
+int ngpb_set_spmv_schedule(ngpb_ctx* ctx, int schedule);
 
 /* ---- batched solve of many independent systems ------------------------------------
  * No single counterpart in the reference, which solves one system per run

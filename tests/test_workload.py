@@ -91,3 +91,28 @@ def test_globular_default_size():
     assert P.mesh.n_hang > 100000
     assert P.atoms.n == 2500 and P.atoms.q.sum() == pytest.approx(-3.0, abs=1e-9)
     assert np.all(P.owner >= 0)
+
+
+def test_ligand_grids_in_batched_range():
+    for i in (0, 1, 6, 7):
+        P = W.ligand(i)
+        n = int(P.mesh.n[0])
+        assert 48 <= n <= 96 and n % 4 == 0
+        assert P.mesh.n_leaves == n ** 3 and P.mesh.n_nodes == (n + 1) ** 3
+        assert 20 <= P.atoms.n <= 80
+        # every atom inside the box
+        lo = P.mesh.origin
+        hi = lo + n * P.mesh.hf
+        assert np.all(P.atoms.xyz > lo) and np.all(P.atoms.xyz < hi)
+
+
+def test_request_stream_deterministic_and_footprint_bound():
+    from cudanextgenpb_amd import serve
+    kw = dict(n_requests=4, rate_hz=10.0, seed=5, atoms_lo=30, atoms_hi=200, h_fine=1.0, fine_band=2.0, mid_band=4.0)
+    a = serve.make_request_stream(**kw)
+    b = serve.make_request_stream(**kw)
+    assert [r.arrival for r in a] == [r.arrival for r in b]
+    assert [r.problem.atoms.n for r in a] == [r.problem.atoms.n for r in b]
+    assert all(x.arrival < y.arrival for x, y in zip(a, a[1:]))
+    assert all(30 <= r.problem.atoms.n <= 200 for r in a)
+    assert all(r.predicted_bytes > 400 * r.n + 3600 * r.problem.atoms.n for r in a)
