@@ -38,6 +38,22 @@ def pct(a, q):
     return float(np.percentile(np.asarray(a), q))
 
 
+def measured_traffic(bytes_per_launch):
+    """HBM bytes per compact-SpMV launch from the committed rocprofv3 FETCH_SIZE /
+    WRITE_SIZE summary (copy-kernel calibrated), if it was measured on an operator of
+    exactly this many algorithmic bytes; otherwise None."""
+    path = os.path.join(REPO, "profiles", "compact_spmv_globular_8M_traffic.txt")
+    try:
+        for line in open(path):
+            if line.startswith("compact_spmv_bytes_per_launch="):
+                kv = dict(t.split("=") for t in line.split())
+                if int(kv["compact_spmv_bytes_per_launch"]) == int(bytes_per_launch):
+                    return float(kv["traffic_bytes_per_launch"])
+    except OSError:
+        pass
+    return None
+
+
 def run_steps(pb, precision, steps, tol, check_every, profile):
     from cudanextgenpb_amd import native
 
@@ -239,7 +255,7 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=8)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--workload", choices=["globular", "born", "batch", "serve"], default="globular")
+    ap.add_argument("--workload", choices=["globular", "born", "large", "batch", "serve"], default="globular")
     ap.add_argument("--requests", type=int, default=48, help="serve workload: number of requests")
     ap.add_argument("--rate", type=float, default=4.0, help="serve workload: Poisson arrival rate [1/s]")
     ap.add_argument("--systems", type=int, default=512, help="batch workload: number of ligand systems")
@@ -274,6 +290,11 @@ def main():
         wname = ("globular protein: 2500 atoms (seed 20502) in a 20 A ball, radii U[1.2,2.0], q N(0,0.35) net -3 e; "
                  "eps 2/80, 0.145 M; box 3x extent; 2:1 octree 0.5 A near the molecule -> 2 A at the boundary; "
                  "Coulomb Dirichlet; tol 1e-10")
+    elif args.workload == "large":
+        P = W.large()
+        wname = ("large complex: 40000 atoms (seed 40003) in a 60x45x45 A ellipsoid, radii U[1.2,2.0], q N(0,0.35) "
+                 "net -20 e; eps 2/80, 0.145 M; box 3x extent; 2:1 octree 0.35 A near the molecule -> 1.4 A at the "
+                 "boundary; Coulomb Dirichlet; tol 1e-10")
     else:
         P = W.born(h=0.25)
         wname = "Born sphere: r 2 A, q +1, 32 A box, uniform 0.25 A (128^3 leaves), analytic Dirichlet; tol 1e-10"
@@ -347,7 +368,8 @@ def main():
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS,
-            "traffic": None,
+            "traffic": measured_traffic(mk["bytes_per_launch"]),
+            "traffic_source": "profiles/compact_spmv_globular_8M_traffic.txt (rocprofv3 FETCH_SIZE x2.000 + WRITE_SIZE x1.000)",
             "bytes_per_launch": mk["bytes_per_launch"],
             "avg_launch_ms": avg_ms,
             "launches": mk["launches"],

@@ -116,7 +116,9 @@ int ngpbw_atoms_ellipsoid(uint64_t seed, int n, const double* c, double ax, doub
                           double qnet, long long max_tries, double* xyz, double* r,
                           double* q) {
   Rng rng(seed);
-  const double cell = std::max(min_sep, 1e-3);
+  // any cell >= min_sep finds every conflicting neighbour in the 3x3x3 block; the
+  // floor keeps the grid small when min_sep is tiny or zero
+  const double cell = std::max(min_sep, 1.0);
   const int gn[3] = {std::max(1, (int)std::ceil(2 * ax / cell)) + 1,
                      std::max(1, (int)std::ceil(2 * ay / cell)) + 1,
                      std::max(1, (int)std::ceil(2 * az / cell)) + 1};

@@ -265,6 +265,21 @@ def _molecule_problem(name, atoms, h_fine, levels, bands, domain_factor, bc_type
     return _assemble_problem(name, physics or Physics(), mesh, atoms, bc_type)
 
 
+def large(seed: int = 40003, n_atoms: int = 40000, semi_axes=(30.0, 22.5, 22.5), min_sep: float = 0.0,
+          rmin: float = 1.2, rmax: float = 2.0, qsigma: float = 0.35, qnet: float = -20.0, h_fine: float = 0.35,
+          levels: int = 2, fine_band: float = 58.0, mid_band: float = 64.0, domain_factor: float = 3.0,
+          bc_type: int = 2, physics: Optional[Physics] = None) -> Problem:
+    """Large synthetic complex: 40,000 atoms in a 60 x 45 x 45 A ellipsoid, radii
+    U[1.2, 2.0], charges N(0, 0.35) shifted to -20 e. At that density (1.6 A^3 per
+    atom) no minimum separation is feasible, so atoms may overlap (the surface is the
+    envelope of the union). Leaves of h_fine (0.35 A) inside and within fine_band of
+    the surface, 2 h_fine within mid_band, 4 h_fine (1.4 A) beyond, box 3x the extent;
+    the default bands give about 60M unknowns."""
+    atoms = make_atoms(seed, n_atoms, [0.0, 0.0, 0.0], list(semi_axes), min_sep, rmin, rmax, qsigma, qnet)
+    return _molecule_problem(f"large_s{seed}", atoms, h_fine, levels, [fine_band, mid_band], domain_factor, bc_type,
+                             physics)
+
+
 def ligand(index: int, base_seed: int = 512, h: float = 0.4, physics: Optional[Physics] = None,
            bc_type: int = 2) -> Problem:
     """Ligand-like molecule `index` of the batched configuration: U{20..80} atoms in a
