@@ -55,7 +55,8 @@ int64_t ngpb_ctx::device_bytes() const {
        vol_patch.bytes() + rho.bytes();
   b += slice_width.bytes() + slice_base_ptr.bytes() + slice_wide_ptr.bytes() + slice_ov_ptr.bytes() +
        sell_val.bytes() + sell_off.bytes() + sell_base.bytes() + sell_col32.bytes() + sell_ov_cnt.bytes() +
-       sell_ov_val.bytes() + sell_ov_col.bytes() + dscale.bytes();
+       sell_ov_val.bytes() + sell_ov_col.bytes() + dscale.bytes() + sell_pk_val.bytes() + sell_pk_idx.bytes() +
+       sell_pk_perm.bytes() + sell_pk_len.bytes() + sell_pk_dict.bytes();
   b += x.bytes() + r.bytes() + z.bytes() + p.bytes() + q.bytes() + dinv.bytes();
   b += xs.bytes() + rs.bytes() + ps.bytes() + qs.bytes() + ks.bytes();
   b += red.partials.bytes() + scan_tmp.bytes();
@@ -654,7 +655,7 @@ int ngpb_operator_info(ngpb_ctx* c, int64_t* info) {
     c->slice_width.download(w.data(), w.size(), c->stream);
     NGPB_HIP(hipStreamSynchronize(c->stream));
     int64_t nd = 0;
-    for (int32_t v : w) nd += v < 0;
+    for (int32_t v : w) nd += (v < 0 && v > -(64 + 1)) ? 1 : 0;  // packed slices have width <= -65
     info[0] = c->n_slices;
     info[1] = nd;
     info[2] = c->sell_delta_entries;
@@ -663,6 +664,8 @@ int ngpb_operator_info(ngpb_ctx* c, int64_t* info) {
     info[5] = c->sell_ov_entries;
     info[6] = (int64_t)ngpb::spmv_compact_bytes(c);
     info[7] = (int64_t)ngpb::spmv_fp64_bytes(c);
+    info[8] = c->sell_pk_slices;
+    info[9] = c->sell_pk_entries;
   });
 }
 

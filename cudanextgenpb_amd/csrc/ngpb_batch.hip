@@ -606,6 +606,11 @@ int ngpb_batch_add(ngpb_batch* b, ngpb_ctx* c, int* slot) {
     d.A.ov_cnt = c->sell_ov_cnt.p;
     d.A.ov_val = c->sell_ov_val.p;
     d.A.ov_col = c->sell_ov_col.p;
+    d.A.pk_val = c->sell_pk_val.p;
+    d.A.pk_idx = c->sell_pk_idx.p;
+    d.A.pk_perm = c->sell_pk_perm.p;
+    d.A.pk_len = c->sell_pk_len.p;
+    d.A.pk_dict = c->sell_pk_dict.p;
     d.rowptr = c->rowptr.p;
     d.col = c->col.p;
     d.val = c->val.p;

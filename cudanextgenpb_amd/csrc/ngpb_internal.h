@@ -208,6 +208,12 @@ struct ngpb_ctx {
   ngpb::DBuf<uint8_t> sell_ov_cnt;    // per row: overflow entries
   ngpb::DBuf<float> sell_ov_val;
   ngpb::DBuf<int32_t> sell_ov_col;
+  int64_t sell_pk_slices = 0, sell_pk_entries = 0, sell_pk_dict_entries = 0;
+  ngpb::DBuf<float> sell_pk_val;      // packed slices: values
+  ngpb::DBuf<uint8_t> sell_pk_idx;    //                dictionary index per value
+  ngpb::DBuf<uint8_t> sell_pk_perm;   //                per sorted position: lane of the row
+  ngpb::DBuf<uint8_t> sell_pk_len;    //                per sorted position: off-diagonal count
+  ngpb::DBuf<int32_t> sell_pk_dict;   //                column deltas
   ngpb::DBuf<double> dscale;         // 1/sqrt(diag)
   int spmv_sched = 0;                // SpMV work schedule (0, 1, 2)
   int sell_resident = 0;             // resident blocks of the compact SpMV (0: not yet queried)

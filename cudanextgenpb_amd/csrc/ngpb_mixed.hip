@@ -29,6 +29,7 @@
 //   every block.
 #include <algorithm>
 #include <cmath>
+#include <vector>
 
 #include "ngpb_internal.h"
 
@@ -75,9 +76,10 @@ constexpr int kDMax = 16;  // most distinct column deltas of a delta slice
 
 // Walks the distinct values of (column - row) over the off-diagonals of the slice's
 // rows in increasing order (each lane's off-diagonals are sorted by column). For the
-// k-th distinct delta `m`, visit(k, m, e) is called with e = this lane's CSR index of
-// the entry with that delta, or -1. Stops after kDMax + 1 deltas; returns the count.
-template <class F>
+// k-th distinct delta `m`, visit(k, m, e, pos) is called with e = this lane's CSR
+// index of the entry with that delta (or -1) and pos = that entry's rank among the
+// lane's off-diagonals. Stops after LIMIT + 1 deltas; returns the count.
+template <int LIMIT, class F>
 __device__ __forceinline__ int for_each_delta(const int32_t* col, int64_t row, bool valid, int64_t b, int64_t dpos,
                                               int cnt, F&& visit) {
   int pos = 0, nd = 0;
@@ -85,22 +87,26 @@ __device__ __forceinline__ int for_each_delta(const int32_t* col, int64_t row, b
     const int64_t e = (valid && pos < cnt) ? offdiag_at(b, dpos, pos) : -1;
     const int d = e >= 0 ? (int)((int64_t)col[e] - row) : INT32_MAX;
     const int m = wave_min_i(d);
-    if (m == INT32_MAX || nd > kDMax) break;
-    visit(nd, m, d == m ? e : (int64_t)-1);
+    if (m == INT32_MAX || nd > LIMIT) break;
+    visit(nd, m, d == m ? e : (int64_t)-1, pos);
     ++nd;
     if (d == m) ++pos;
   }
   return nd;
 }
 
-// One wave per slice: chooses the delta form (values only, column = row + delta of
-// the slot) when the slice has at most kDMax distinct deltas and it is not larger
-// than the offset form; otherwise the offset form with a per-slice main width that
-// minimises padding + overflow bytes. width[s] < 0 marks a delta slice.
+// One wave per slice: chooses the slice form.
+//   delta   at most kDMax distinct deltas: 4 bytes per slot entry, gathers contiguous
+//   packed  at most 256 distinct deltas and at most 64 off-diagonals per row: 5 bytes
+//           per entry (no padding) + 2 bytes per row + 4 per dictionary entry
+//   offset  otherwise: main width minimising padding + overflow bytes
+// The delta form is taken when it moves no more than 1.25x the packed form's bytes
+// (its loads are fewer and wider); the offset form only when neither fits.
 __global__ void __launch_bounds__(kBS) k_sell_shape(int64_t n, int64_t nslices, const int32_t* __restrict__ rowptr,
                                                     const int32_t* __restrict__ col, int32_t* __restrict__ width,
                                                     int64_t* __restrict__ wcount, int64_t* __restrict__ wide_count,
                                                     int64_t* __restrict__ ov_count, int64_t* __restrict__ dslots,
+                                                    int64_t* __restrict__ pk_count, int64_t* __restrict__ pk_dcount,
                                                     int* __restrict__ bad) {
   const int64_t s = (int64_t)blockIdx.x * (kBS / kW) + (threadIdx.x >> 6);
   if (s >= nslices) return;
@@ -110,38 +116,45 @@ __global__ void __launch_bounds__(kBS) k_sell_shape(int64_t n, int64_t nslices, 
   if (row < n) row_info(rowptr, col, row, &dpos, &cnt);
   const int64_t b = row < n ? rowptr[row] : 0;
   const int mx = wave_max_i(cnt);
+  const int tot = wave_sum_i(cnt);
+  const int nd = for_each_delta<kPackedMaxDeltas>(col, row, row < n, b, dpos, cnt, [](int, int, int64_t, int) {});
+  const bool packable = nd <= kPackedMaxDeltas && mx <= kW;
+  const long long pk_cost = (long long)tot * 5 + 2 * kW + (long long)nd * 4;
+  const bool use_delta = nd <= kDMax && (!packable || (long long)nd * kW * 4 * 4 <= pk_cost * 5);
+  const bool use_packed = !use_delta && packable;
   // offset form: main slots (6 bytes per entry) against overflow entries, which cost
   // 8 bytes but are walked serially per row (two dependent loads each), so they are
   // charged 32 bytes apiece
   int best_w = mx;
-  long long best = (long long)mx * kW * 6;
-  for (int w = mx - 1; w >= 0; --w) {
-    const int extra = wave_sum_i(max(0, cnt - w));
-    const long long cost = (long long)w * kW * 6 + (long long)extra * 32 + kW;
-    if (cost < best) {
-      best = cost;
-      best_w = w;
+  bool wide = false;
+  if (!use_delta && !use_packed) {
+    long long best = (long long)mx * kW * 6;
+    for (int w = mx - 1; w >= 0; --w) {
+      const int extra = wave_sum_i(max(0, cnt - w));
+      const long long cost = (long long)w * kW * 6 + (long long)extra * 32 + kW;
+      if (cost < best) {
+        best = cost;
+        best_w = w;
+      }
     }
+    for (int k = 0; k < best_w; ++k) {
+      const int cc = (row < n && k < cnt) ? col[offdiag_at(b, dpos, k)] : INT32_MAX;
+      const int cmin = wave_min_i(cc);
+      const int cmax = wave_max_i(cc == INT32_MAX ? INT32_MIN : cc);
+      if (cmin != INT32_MAX && (int64_t)cmax - (int64_t)cmin > 65535) wide = true;
+    }
+    if (cnt - best_w > 255) atomicOr(bad, 1);
   }
-  int wide = 0;
-  for (int k = 0; k < best_w; ++k) {
-    const int cc = (row < n && k < cnt) ? col[offdiag_at(b, dpos, k)] : INT32_MAX;
-    const int cmin = wave_min_i(cc);
-    const int cmax = wave_max_i(cc == INT32_MAX ? INT32_MIN : cc);
-    if (cmin != INT32_MAX && (int64_t)cmax - (int64_t)cmin > 65535) wide = 1;
-  }
-  if (wide) best += (long long)best_w * kW * 4;
-  // delta form: 4 bytes per slot entry
-  const int nd = for_each_delta(col, row, row < n, b, dpos, cnt, [](int, int, int64_t) {});
-  const bool use_delta = nd <= kDMax && (long long)nd * kW * 4 <= best;
-  if (!use_delta && cnt - best_w > 255) atomicOr(bad, 1);
-  const int extra = use_delta ? 0 : wave_sum_i(max(0, cnt - best_w));
+  const bool offset = !use_delta && !use_packed;
+  const int extra = offset ? wave_sum_i(max(0, cnt - best_w)) : 0;
   if (lane_id() == 0) {
-    width[s] = use_delta ? -nd : best_w;
-    wcount[s] = use_delta ? nd : best_w;
-    wide_count[s] = (!use_delta && wide) ? (int64_t)best_w * kW : 0;
+    width[s] = use_delta ? -nd : (use_packed ? -(kPackedBase + nd) : best_w);
+    wcount[s] = use_delta ? nd : (use_packed ? 0 : best_w);
+    wide_count[s] = (offset && wide) ? (int64_t)best_w * kW : 0;
     ov_count[s] = extra;
     dslots[s] = use_delta ? nd : 0;
+    pk_count[s] = use_packed ? tot : 0;
+    pk_dcount[s] = use_packed ? nd : 0;
   }
 }
 
@@ -156,7 +169,10 @@ __global__ void __launch_bounds__(kBS) k_sell_fill(int64_t n, int64_t nslices, c
                                                    int32_t* __restrict__ sbase, int32_t* __restrict__ scol32,
                                                    int64_t* __restrict__ wide_ptr, int64_t* __restrict__ ov_ptr,
                                                    uint8_t* __restrict__ ov_cnt, float* __restrict__ ov_val,
-                                                   int32_t* __restrict__ ov_col) {
+                                                   int32_t* __restrict__ ov_col, const int64_t* __restrict__ pk_off,
+                                                   const int64_t* __restrict__ pk_doff, float* __restrict__ pk_val,
+                                                   uint8_t* __restrict__ pk_idx, uint8_t* __restrict__ pk_perm,
+                                                   uint8_t* __restrict__ pk_len, int32_t* __restrict__ pk_dict) {
   const int64_t s = (int64_t)blockIdx.x * (kBS / kW) + (threadIdx.x >> 6);
   if (s >= nslices) return;
   const int lane = lane_id();
@@ -164,12 +180,13 @@ __global__ void __launch_bounds__(kBS) k_sell_fill(int64_t n, int64_t nslices, c
   const int w = width[s];
   const int64_t bp = base_ptr[s];
   const int64_t sp = bp * kW;
+  const bool packed = w <= -(kPackedBase + 1);
   const bool wide = wide_cnt[s] > 0;
   const int64_t wp = wide ? wide_off[s] : -1;
   const bool has_ov = ov_cnt_s[s] > 0;
   if (lane == 0) {
-    wide_ptr[s] = wp;
-    ov_ptr[s] = has_ov ? ov_off[s] : -1;
+    wide_ptr[s] = packed ? pk_off[s] : wp;
+    ov_ptr[s] = packed ? pk_doff[s] : (has_ov ? ov_off[s] : -1);
   }
   int cnt = 0;
   int64_t dpos = -1, b = 0;
@@ -178,9 +195,39 @@ __global__ void __launch_bounds__(kBS) k_sell_fill(int64_t n, int64_t nslices, c
     b = rowptr[row];
   }
   const double di = row < n ? dscale[row] : 0.0;
+  if (packed) {
+    // sorted position of this row: off-diagonal count descending, ties by lane
+    int rank = 0;
+    for (int j = 0; j < kW; ++j) {
+      const int cj = __shfl(cnt, j, 64);
+      rank += (cj > cnt || (cj == cnt && j < lane)) ? 1 : 0;
+    }
+    pk_perm[s * kW + rank] = (uint8_t)lane;
+    pk_len[s * kW + rank] = (uint8_t)cnt;
+    // lane k holds the start of slot k: m_0 + ... + m_{k-1}, m_j = rows with > j entries
+    const int mx = wave_max_i(cnt);
+    int my_m = 0;
+    for (int k = 0; k < mx; ++k) {
+      const int mk = __popcll(__ballot(cnt > k));
+      if (lane == k) my_m = mk;
+    }
+    const int slot_start = wave_excl_scan_i(my_m);
+    const int64_t E = pk_off[s], D = pk_doff[s];
+    for_each_delta<kPackedMaxDeltas>(col, row, row < n, b, dpos, cnt, [&](int k, int m, int64_t e, int pos) {
+      const int st = __shfl(slot_start, pos < kW ? pos : 0, 64);
+      if (lane == 0) pk_dict[D + k] = m;
+      if (e >= 0) {
+        const int64_t idx = E + st + rank;
+        pk_val[idx] = (float)(val[e] * di * dscale[col[e]]);
+        pk_idx[idx] = (uint8_t)k;
+      }
+    });
+    if (row < n) ov_cnt[row] = 0;
+    return;
+  }
   if (w < 0) {
     // delta slice: slot k holds the entry of column row + delta_k (0 if absent)
-    for_each_delta(col, row, row < n, b, dpos, cnt, [&](int k, int m, int64_t e) {
+    for_each_delta<kDMax>(col, row, row < n, b, dpos, cnt, [&](int k, int m, int64_t e, int) {
       const int64_t idx = sp + (int64_t)k * kW + lane;
       sval[idx] = e >= 0 ? (float)(val[e] * di * dscale[col[e]]) : 0.f;
       soff[idx] = 0;
@@ -452,6 +499,11 @@ Sell sell_view(const ngpb_ctx* c) {
   A.ov_cnt = c->sell_ov_cnt.p;
   A.ov_val = c->sell_ov_val.p;
   A.ov_col = c->sell_ov_col.p;
+  A.pk_val = c->sell_pk_val.p;
+  A.pk_idx = c->sell_pk_idx.p;
+  A.pk_perm = c->sell_pk_perm.p;
+  A.pk_len = c->sell_pk_len.p;
+  A.pk_dict = c->sell_pk_dict.p;
   return A;
 }
 
@@ -503,8 +555,12 @@ void build_compact(ngpb_ctx* c) {
   hipLaunchKernelGGL(k_scale, dim3(blocks_for(n, 256)), dim3(256), 0, c->stream, n, c->rowptr.p, c->col.p, c->val.p,
                      c->dscale.p);
   c->slice_width.resize((size_t)ns);
-  DBuf<int64_t> wcount, widec, wide_off, ovc, ov_off, dslots;
+  DBuf<int64_t> wcount, widec, wide_off, ovc, ov_off, dslots, pkc, pk_off, pkd, pk_doff;
   DBuf<int> bad;
+  pkc.resize((size_t)ns);
+  pk_off.resize((size_t)ns);
+  pkd.resize((size_t)ns);
+  pk_doff.resize((size_t)ns);
   wcount.resize((size_t)ns);
   widec.resize((size_t)ns);
   wide_off.resize((size_t)ns);
@@ -515,7 +571,7 @@ void build_compact(ngpb_ctx* c) {
   NGPB_HIP(hipMemsetAsync(bad.p, 0, sizeof(int), c->stream));
   const unsigned g = (unsigned)((ns + (kBS / kW) - 1) / (kBS / kW));
   hipLaunchKernelGGL(k_sell_shape, dim3(g), dim3(kBS), 0, c->stream, n, ns, c->rowptr.p, c->col.p, c->slice_width.p,
-                     wcount.p, widec.p, ovc.p, dslots.p, bad.p);
+                     wcount.p, widec.p, ovc.p, dslots.p, pkc.p, pkd.p, bad.p);
   NGPB_HIP(hipGetLastError());
   int hb = 0;
   NGPB_HIP(hipMemcpyAsync(&hb, bad.p, sizeof(int), hipMemcpyDeviceToHost, c->stream));
@@ -525,6 +581,25 @@ void build_compact(ngpb_ctx* c) {
   const int64_t nwide = exclusive_scan(c, widec.p, wide_off.p, ns);
   const int64_t nov = exclusive_scan(c, ovc.p, ov_off.p, ns);
   const int64_t ndslots = exclusive_scan(c, dslots.p, dslots.p, ns);
+  const int64_t npk = exclusive_scan(c, pkc.p, pk_off.p, ns);
+  const int64_t npkd = exclusive_scan(c, pkd.p, pk_doff.p, ns);
+  {
+    // packed slices = slices with a dictionary (every packed slice has >= 1 delta or
+    // rows without off-diagonals only, which are counted through their width below)
+    std::vector<int32_t> wh((size_t)ns);
+    NGPB_HIP(hipMemcpyAsync(wh.data(), c->slice_width.p, ns * sizeof(int32_t), hipMemcpyDeviceToHost, c->stream));
+    NGPB_HIP(hipStreamSynchronize(c->stream));
+    int64_t k = 0;
+    for (int32_t w : wh) k += w <= -(kPackedBase + 1) ? 1 : 0;
+    c->sell_pk_slices = k;
+  }
+  c->sell_pk_entries = npk;
+  c->sell_pk_dict_entries = npkd;
+  c->sell_pk_val.resize((size_t)std::max<int64_t>(npk, 1));
+  c->sell_pk_idx.resize((size_t)std::max<int64_t>(npk, 1));
+  c->sell_pk_perm.resize((size_t)ns * kW);
+  c->sell_pk_len.resize((size_t)ns * kW);
+  c->sell_pk_dict.resize((size_t)std::max<int64_t>(npkd, 1));
   c->sell_bases = nbases;
   c->sell_entries = nbases * kW;
   c->sell_delta_entries = ndslots * kW;
@@ -544,7 +619,8 @@ void build_compact(ngpb_ctx* c) {
   hipLaunchKernelGGL(k_sell_fill, dim3(g), dim3(kBS), 0, c->stream, n, ns, c->rowptr.p, c->col.p, c->val.p, c->dscale.p,
                      c->slice_width.p, c->slice_base_ptr.p, wide_off.p, widec.p, ov_off.p, ovc.p, c->sell_val.p,
                      c->sell_off.p, c->sell_base.p, c->sell_col32.p, c->slice_wide_ptr.p, c->slice_ov_ptr.p,
-                     c->sell_ov_cnt.p, c->sell_ov_val.p, c->sell_ov_col.p);
+                     c->sell_ov_cnt.p, c->sell_ov_val.p, c->sell_ov_col.p, pk_off.p, pk_doff.p, c->sell_pk_val.p,
+                     c->sell_pk_idx.p, c->sell_pk_perm.p, c->sell_pk_len.p, c->sell_pk_dict.p);
   NGPB_HIP(hipGetLastError());
   NGPB_HIP(hipStreamSynchronize(c->stream));
   c->have_compact = true;
@@ -552,13 +628,15 @@ void build_compact(ngpb_ctx* c) {
 
 double spmv_compact_bytes(const ngpb_ctx* c) {
   // delta-slice entries (4: value), offset-slice entries (4 value + 2 offset), slot
-  // bases/deltas (4), 32-bit columns of wide slices, overflow entries (4 + 4) and
-  // their per-row counts, slice metadata (width 4 + three 8-byte offsets), x read
+  // bases/deltas (4), 32-bit columns of wide slices, overflow entries (4 + 4),
+  // packed entries (4 value + 1 index), packed rows (permutation + count, 1 + 1),
+  // dictionary entries (4), slice metadata (width 4 + three 8-byte offsets), x read
   // once and y written
   const double off_entries = (double)(c->sell_entries - c->sell_delta_entries);
   return (double)c->sell_delta_entries * 4.0 + off_entries * 6.0 + (double)c->sell_bases * 4.0 +
-         (double)c->sell_wide_entries * 4.0 + (double)c->sell_ov_entries * 8.0 + (double)c->n_slices * 28.0 +
-         (double)c->n * (4.0 + 4.0);
+         (double)c->sell_wide_entries * 4.0 + (double)c->sell_ov_entries * 8.0 + (double)c->sell_pk_entries * 5.0 +
+         (double)c->sell_pk_slices * kW * 2.0 + (double)c->sell_pk_dict_entries * 4.0 +
+         (double)c->n_slices * 28.0 + (double)c->n * (4.0 + 4.0);
 }
 
 void spmv_compact(ngpb_ctx* c, const double* x, double* y) {

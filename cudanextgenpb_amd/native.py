@@ -310,10 +310,10 @@ class Context:
         return ms.value
 
     def operator_info(self):
-        info = np.zeros(8, np.int64)
+        info = np.zeros(10, np.int64)
         self._check(lib().ngpb_operator_info(self._h, _p(info)), "operator_info")
         keys = ["slices", "delta_slices", "delta_entries", "offset_entries", "wide_entries", "overflow_entries",
-                "compact_spmv_bytes", "fp64_spmv_bytes"]
+                "compact_spmv_bytes", "fp64_spmv_bytes", "packed_slices", "packed_entries"]
         return dict(zip(keys, (int(v) for v in info)))
 
     def set_spmv_schedule(self, schedule: int):

@@ -180,9 +180,9 @@ int ngpb_time_copy(ngpb_ctx* ctx, int64_t bytes, int reps, double* ms_per_launch
 /* Layout of the compact mixed-precision operator (built on demand): info[0] slices,
    [1] delta slices, [2] stored slot entries in delta slices, [3] in offset slices,
    [4] 32-bit column entries, [5] overflow entries, [6] algorithmic bytes of one
-   compact SpMV, [7] algorithmic bytes of one FP64 CSR SpMV. No counterpart in the
-   reference. */
-int ngpb_operator_info(ngpb_ctx* ctx, int64_t* info8);
+   compact SpMV, [7] algorithmic bytes of one FP64 CSR SpMV, [8] packed slices,
+   [9] packed entries. No counterpart in the reference. */
+int ngpb_operator_info(ngpb_ctx* ctx, int64_t* info10);
 /* Work schedule of the SpMV kernels (compact FP32 and FP64 CSR): 0 (default)
    grid-stride over slices / row chunks on up to 2048 blocks, 1 one contiguous range
    per wave (block) on the resident grid with XCD-aware block numbering, 2 grid-stride
