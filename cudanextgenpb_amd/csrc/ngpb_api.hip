@@ -56,7 +56,8 @@ int64_t ngpb_ctx::device_bytes() const {
   b += slice_width.bytes() + slice_base_ptr.bytes() + slice_wide_ptr.bytes() + slice_ov_ptr.bytes() +
        sell_val.bytes() + sell_off.bytes() + sell_base.bytes() + sell_col32.bytes() + sell_ov_cnt.bytes() +
        sell_ov_val.bytes() + sell_ov_col.bytes() + dscale.bytes() + sell_pk_val.bytes() + sell_pk_idx.bytes() +
-       sell_pk_perm.bytes() + sell_pk_len.bytes() + sell_pk_dict.bytes();
+       sell_pk_lp.bytes() + sell_pk_dict.bytes() + sell_pk_hdr.bytes() + sell_list_delta.bytes() +
+       sell_list_packed.bytes() + sell_list_offset.bytes();
   b += x.bytes() + r.bytes() + z.bytes() + p.bytes() + q.bytes() + dinv.bytes();
   b += xs.bytes() + rs.bytes() + ps.bytes() + qs.bytes() + ks.bytes();
   b += red.partials.bytes() + scan_tmp.bytes();

@@ -608,9 +608,9 @@ int ngpb_batch_add(ngpb_batch* b, ngpb_ctx* c, int* slot) {
     d.A.ov_col = c->sell_ov_col.p;
     d.A.pk_val = c->sell_pk_val.p;
     d.A.pk_idx = c->sell_pk_idx.p;
-    d.A.pk_perm = c->sell_pk_perm.p;
-    d.A.pk_len = c->sell_pk_len.p;
+    d.A.pk_lp = c->sell_pk_lp.p;
     d.A.pk_dict = c->sell_pk_dict.p;
+    d.A.pk_hdr = reinterpret_cast<const ngpb::PkHdr*>(c->sell_pk_hdr.p);
     d.rowptr = c->rowptr.p;
     d.col = c->col.p;
     d.val = c->val.p;

@@ -208,14 +208,17 @@ struct ngpb_ctx {
   ngpb::DBuf<uint8_t> sell_ov_cnt;    // per row: overflow entries
   ngpb::DBuf<float> sell_ov_val;
   ngpb::DBuf<int32_t> sell_ov_col;
-  int64_t sell_pk_slices = 0, sell_pk_entries = 0, sell_pk_dict_entries = 0;
+  int64_t sell_pk_slices = 0, sell_pk_entries = 0, sell_pk_dict_entries = 0, sell_pk_words = 0;
   ngpb::DBuf<float> sell_pk_val;      // packed slices: values
-  ngpb::DBuf<uint8_t> sell_pk_idx;    //                dictionary index per value
-  ngpb::DBuf<uint8_t> sell_pk_perm;   //                per sorted position: lane of the row
-  ngpb::DBuf<uint8_t> sell_pk_len;    //                per sorted position: off-diagonal count
+  ngpb::DBuf<uint32_t> sell_pk_idx;   //                dictionary indices, 4 slots per word
+  ngpb::DBuf<uint16_t> sell_pk_lp;    //                per sorted position: lane | count << 8
   ngpb::DBuf<int32_t> sell_pk_dict;   //                column deltas
+  ngpb::DBuf<int64_t> sell_pk_hdr;    //                32-byte header per slice (4 x int64)
+  // slices of each form, for the form-specialised SpMV kernels
+  int64_t n_list_delta = 0, n_list_packed = 0, n_list_offset = 0;
+  ngpb::DBuf<int32_t> sell_list_delta, sell_list_packed, sell_list_offset;
   ngpb::DBuf<double> dscale;         // 1/sqrt(diag)
-  int spmv_sched = 0;                // SpMV work schedule (0, 1, 2)
+  int spmv_sched = 2;                // SpMV work schedule (0, 1, 2)
   int sell_resident = 0;             // resident blocks of the compact SpMV (0: not yet queried)
   int csr_resident = 0;              // resident blocks of the FP64 CSR SpMV (0: not yet queried)
 

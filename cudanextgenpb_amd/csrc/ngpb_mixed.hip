@@ -107,7 +107,7 @@ __global__ void __launch_bounds__(kBS) k_sell_shape(int64_t n, int64_t nslices, 
                                                     int64_t* __restrict__ wcount, int64_t* __restrict__ wide_count,
                                                     int64_t* __restrict__ ov_count, int64_t* __restrict__ dslots,
                                                     int64_t* __restrict__ pk_count, int64_t* __restrict__ pk_dcount,
-                                                    int* __restrict__ bad) {
+                                                    int64_t* __restrict__ pk_gcount, int* __restrict__ bad) {
   const int64_t s = (int64_t)blockIdx.x * (kBS / kW) + (threadIdx.x >> 6);
   if (s >= nslices) return;
   const int64_t row = s * kW + lane_id();
@@ -122,6 +122,10 @@ __global__ void __launch_bounds__(kBS) k_sell_shape(int64_t n, int64_t nslices, 
   const long long pk_cost = (long long)tot * 5 + 2 * kW + (long long)nd * 4;
   const bool use_delta = nd <= kDMax && (!packable || (long long)nd * kW * 4 * 4 <= pk_cost * 5);
   const bool use_packed = !use_delta && packable;
+  // index words of a packed slice: one per row taking part in each group of 4 slots
+  int words = 0;
+  if (use_packed)
+    for (int k = 0; k < mx; k += 4) words += __popcll(__ballot(cnt > k));
   // offset form: main slots (6 bytes per entry) against overflow entries, which cost
   // 8 bytes but are walked serially per row (two dependent loads each), so they are
   // charged 32 bytes apiece
@@ -155,6 +159,7 @@ __global__ void __launch_bounds__(kBS) k_sell_shape(int64_t n, int64_t nslices, 
     dslots[s] = use_delta ? nd : 0;
     pk_count[s] = use_packed ? tot : 0;
     pk_dcount[s] = use_packed ? nd : 0;
+    pk_gcount[s] = use_packed ? words : 0;
   }
 }
 
@@ -170,9 +175,10 @@ __global__ void __launch_bounds__(kBS) k_sell_fill(int64_t n, int64_t nslices, c
                                                    int64_t* __restrict__ wide_ptr, int64_t* __restrict__ ov_ptr,
                                                    uint8_t* __restrict__ ov_cnt, float* __restrict__ ov_val,
                                                    int32_t* __restrict__ ov_col, const int64_t* __restrict__ pk_off,
-                                                   const int64_t* __restrict__ pk_doff, float* __restrict__ pk_val,
-                                                   uint8_t* __restrict__ pk_idx, uint8_t* __restrict__ pk_perm,
-                                                   uint8_t* __restrict__ pk_len, int32_t* __restrict__ pk_dict) {
+                                                   const int64_t* __restrict__ pk_doff,
+                                                   const int64_t* __restrict__ pk_goff, float* __restrict__ pk_val,
+                                                   uint8_t* __restrict__ pk_idx_bytes, uint16_t* __restrict__ pk_lp,
+                                                   int32_t* __restrict__ pk_dict, PkHdr* __restrict__ pk_hdr) {
   const int64_t s = (int64_t)blockIdx.x * (kBS / kW) + (threadIdx.x >> 6);
   if (s >= nslices) return;
   const int lane = lane_id();
@@ -202,8 +208,7 @@ __global__ void __launch_bounds__(kBS) k_sell_fill(int64_t n, int64_t nslices, c
       const int cj = __shfl(cnt, j, 64);
       rank += (cj > cnt || (cj == cnt && j < lane)) ? 1 : 0;
     }
-    pk_perm[s * kW + rank] = (uint8_t)lane;
-    pk_len[s * kW + rank] = (uint8_t)cnt;
+    pk_lp[s * kW + rank] = (uint16_t)(lane | (cnt << 8));
     // lane k holds the start of slot k: m_0 + ... + m_{k-1}, m_j = rows with > j entries
     const int mx = wave_max_i(cnt);
     int my_m = 0;
@@ -212,14 +217,29 @@ __global__ void __launch_bounds__(kBS) k_sell_fill(int64_t n, int64_t nslices, c
       if (lane == k) my_m = mk;
     }
     const int slot_start = wave_excl_scan_i(my_m);
-    const int64_t E = pk_off[s], D = pk_doff[s];
+    // lane g < 16 holds the first index word of slot group g. The permute runs in
+    // every lane: a lane whose source lane is inactive would read 0.
+    const int m_of_group = __shfl(my_m, (4 * lane) & (kW - 1), 64);
+    const int my_mg = lane < 16 ? m_of_group : 0;
+    const int group_start = wave_excl_scan_i(my_mg);
+    const int64_t E = pk_off[s], D = pk_doff[s], G = pk_goff[s];
+    if (lane < 16) pk_hdr[s].m[lane] = (uint8_t)my_m;
+    if (lane == 0) {
+      pk_hdr[s].E = (int32_t)E;
+      pk_hdr[s].G = (int32_t)G;
+      pk_hdr[s].D = (int32_t)D;
+      pk_hdr[s].nd = (uint16_t)(-w - kPackedBase);
+      pk_hdr[s].maxlen = (uint8_t)mx;
+      pk_hdr[s].pad = 0;
+    }
     for_each_delta<kPackedMaxDeltas>(col, row, row < n, b, dpos, cnt, [&](int k, int m, int64_t e, int pos) {
-      const int st = __shfl(slot_start, pos < kW ? pos : 0, 64);
+      const int pk = pos < kW ? pos : 0;
+      const int st = __shfl(slot_start, pk, 64);
+      const int gst = __shfl(group_start, pk >> 2, 64);
       if (lane == 0) pk_dict[D + k] = m;
       if (e >= 0) {
-        const int64_t idx = E + st + rank;
-        pk_val[idx] = (float)(val[e] * di * dscale[col[e]]);
-        pk_idx[idx] = (uint8_t)k;
+        pk_val[E + st + rank] = (float)(val[e] * di * dscale[col[e]]);
+        pk_idx_bytes[(G + gst + rank) * 4 + (pos & 3)] = (uint8_t)k;
       }
     });
     if (row < n) ov_cnt[row] = 0;
@@ -262,42 +282,23 @@ __global__ void __launch_bounds__(kBS) k_sell_fill(int64_t n, int64_t nslices, c
   }
 }
 
-// q = Ahat p with per-block (p, q) when DOT.
-// Work is counted in pairs of adjacent slices. SCHED selects who takes which pairs:
-//   0  wave w takes pairs w, w + W, w + 2W, ... (W waves in the grid)
-//   1  each wave takes one contiguous range of pairs; blocks are renumbered so that
-//      the blocks the hardware places on one XCD (block id mod 8) cover one
-//      contiguous eighth of the rows, which keeps the x gathers of neighbouring
-//      slices (and of the +-plane neighbours) in that XCD's L2
-//   2  as 0; the grid is sized to the resident waves (no second round of blocks)
-// The grid for 1 and 2 is the number of blocks that fit on the device at once.
-template <bool DOT, int SCHED>
+// q = Ahat p with per-block (p, q) when DOT: one kernel for every slice form. Wave w
+// takes the pairs of adjacent slices w, w + W, w + 2W, ... (W waves in the grid).
+template <bool DOT>
 __global__ void __launch_bounds__(kBS) k_sell_spmv(Sell A, const float* __restrict__ x, float* __restrict__ y,
                                                    const KrylovScalars* ks, double* __restrict__ partials) {
   __shared__ double s_red[kBS / 64];
   if (DOT && ks->done) return;
   const int64_t npairs = (A.nslices + 1) / 2;
-  const int64_t nwaves = (int64_t)gridDim.x * (kBS / kW);
-  int64_t t, t_end, step;
-  if (SCHED == 1) {
-    const unsigned nb = gridDim.x, b = blockIdx.x;
-    const unsigned lb = (nb % 8u == 0u) ? (b % 8u) * (nb / 8u) + b / 8u : b;
-    const int64_t lw = __builtin_amdgcn_readfirstlane((int)(lb * (kBS / kW) + (threadIdx.x >> 6)));
-    t = lw * npairs / nwaves;
-    t_end = (lw + 1) * npairs / nwaves;
-    step = 1;
-  } else {
-    t = __builtin_amdgcn_readfirstlane((int)(blockIdx.x * (kBS / kW) + (threadIdx.x >> 6)));
-    t_end = npairs;
-    step = nwaves;
-  }
+  const int64_t step = (int64_t)gridDim.x * (kBS / kW);
+  int64_t t = __builtin_amdgcn_readfirstlane((int)(blockIdx.x * (kBS / kW) + (threadIdx.x >> 6)));
   double part = 0.0;
   // two delta slices with the same slot count are processed together so that twice
   // as many loads are in flight; the next pair's widths and slot pointers are
   // requested before the current pair is processed
   int w0n = 0, w1n = 0;
   int64_t b0n = 0, b1n = 0;
-  if (t < t_end) {
+  if (t < npairs) {
     w0n = A.width[2 * t];
     b0n = A.base_ptr[2 * t];
     if (2 * t + 1 < A.nslices) {
@@ -305,13 +306,13 @@ __global__ void __launch_bounds__(kBS) k_sell_spmv(Sell A, const float* __restri
       b1n = A.base_ptr[2 * t + 1];
     }
   }
-  while (t < t_end) {
+  while (t < npairs) {
     const int w0 = w0n, w1 = w1n;
     const int64_t b0 = b0n, b1 = b1n;
     const int64_t s0 = 2 * t;
     const bool has1 = s0 + 1 < A.nslices;
     const int64_t t2 = t + step;
-    if (t2 < t_end) {
+    if (t2 < npairs) {
       w0n = A.width[2 * t2];
       b0n = A.base_ptr[2 * t2];
       if (2 * t2 + 1 < A.nslices) {
@@ -340,6 +341,137 @@ __global__ void __launch_bounds__(kBS) k_sell_spmv(Sell A, const float* __restri
     part = dev::block_sum<kBS>(part, s_red);
     if (threadIdx.x == 0) partials[blockIdx.x] = part;
   }
+}
+
+// Form-specialised SpMV (the default): the delta slices, the packed slices and the
+// offset slices each have a kernel of their own, walking a list of their slices, so
+// each kernel holds only the registers its form needs. Each writes its per-block
+// (p, q) partials into its own range of the partial array.
+
+// delta slices: list entries taken two at a time; two adjacent slices with the same
+// slot count go through the paired path; the next pair's slice ids, widths and slot
+// pointers are requested before the current pair is processed
+template <bool DOT>
+__global__ void __launch_bounds__(kBS) k_sell_delta(Sell A, const int32_t* __restrict__ list, int64_t cnt,
+                                                    const float* __restrict__ x, float* __restrict__ y,
+                                                    const KrylovScalars* ks, double* __restrict__ partials) {
+  __shared__ double s_red[kBS / 64];
+  if (DOT && ks->done) return;
+  const int64_t npairs = (cnt + 1) / 2;
+  const int64_t step = (int64_t)gridDim.x * (kBS / kW);
+  int64_t t = __builtin_amdgcn_readfirstlane((int)(blockIdx.x * (kBS / kW) + (threadIdx.x >> 6)));
+  double part = 0.0;
+  int64_t s0n = 0, s1n = -1, b0n = 0, b1n = 0;
+  int w0n = 0, w1n = 0;
+  if (t < npairs) {
+    s0n = list[2 * t];
+    s1n = 2 * t + 1 < cnt ? list[2 * t + 1] : -1;
+    w0n = A.width[s0n];
+    b0n = A.base_ptr[s0n];
+    if (s1n >= 0) {
+      w1n = A.width[s1n];
+      b1n = A.base_ptr[s1n];
+    }
+  }
+  while (t < npairs) {
+    const int64_t s0 = s0n, s1 = s1n, b0 = b0n, b1 = b1n;
+    const int w0 = w0n, w1 = w1n;
+    const int64_t t2 = t + step;
+    if (t2 < npairs) {
+      s0n = list[2 * t2];
+      s1n = 2 * t2 + 1 < cnt ? list[2 * t2 + 1] : -1;
+      w0n = A.width[s0n];
+      b0n = A.base_ptr[s0n];
+      if (s1n >= 0) {
+        w1n = A.width[s1n];
+        b1n = A.base_ptr[s1n];
+      }
+    }
+    bool done = false;
+    if (s1 == s0 + 1 && w0 == w1) {
+      switch (w0) {
+        case -4: delta_pair<4>(A, s0, b0, b1, x, y, &part); done = true; break;
+        case -5: delta_pair<5>(A, s0, b0, b1, x, y, &part); done = true; break;
+        case -6: delta_pair<6>(A, s0, b0, b1, x, y, &part); done = true; break;
+        case -7: delta_pair<7>(A, s0, b0, b1, x, y, &part); done = true; break;
+        case -8: delta_pair<8>(A, s0, b0, b1, x, y, &part); done = true; break;
+        default: break;
+      }
+    }
+    if (!done) {
+      delta_slice(A, s0, w0, b0, x, y, &part);
+      if (s1 >= 0) delta_slice(A, s1, w1, b1, x, y, &part);
+    }
+    t = t2;
+  }
+  if (DOT) {
+    part = dev::block_sum<kBS>(part, s_red);
+    if (threadIdx.x == 0) partials[blockIdx.x] = part;
+  }
+}
+
+// packed slices: one per wave visit; the next slice's 32-byte header is requested
+// before the current slice is processed
+template <bool DOT>
+__global__ void __launch_bounds__(kBS) k_sell_packed(Sell A, const int32_t* __restrict__ list, int64_t cnt,
+                                                     const float* __restrict__ x, float* __restrict__ y,
+                                                     const KrylovScalars* ks, double* __restrict__ partials) {
+  __shared__ double s_red[kBS / 64];
+  if (DOT && ks->done) return;
+  const int64_t step = (int64_t)gridDim.x * (kBS / kW);
+  int64_t i = __builtin_amdgcn_readfirstlane((int)(blockIdx.x * (kBS / kW) + (threadIdx.x >> 6)));
+  double part = 0.0;
+  int64_t sn = i < cnt ? list[i] : 0;
+  PkHdr hn;
+  if (i < cnt) hn = A.pk_hdr[sn];
+  while (i < cnt) {
+    const int64_t s = sn;
+    const PkHdr h = hn;
+    const int64_t i2 = i + step;
+    if (i2 < cnt) {
+      sn = list[i2];
+      hn = A.pk_hdr[sn];
+    }
+    if (h.nd <= kW) packed_slice_h<1>(A, s, h, x, y, &part);
+    else if (h.nd <= 2 * kW) packed_slice_h<2>(A, s, h, x, y, &part);
+    else packed_slice_h<4>(A, s, h, x, y, &part);
+    i = i2;
+  }
+  if (DOT) {
+    part = dev::block_sum<kBS>(part, s_red);
+    if (threadIdx.x == 0) partials[blockIdx.x] = part;
+  }
+}
+
+// offset slices (rows too irregular for the other forms; rare)
+template <bool DOT>
+__global__ void __launch_bounds__(kBS) k_sell_offset(Sell A, const int32_t* __restrict__ list, int64_t cnt,
+                                                     const float* __restrict__ x, float* __restrict__ y,
+                                                     const KrylovScalars* ks, double* __restrict__ partials) {
+  __shared__ double s_red[kBS / 64];
+  if (DOT && ks->done) return;
+  const int64_t step = (int64_t)gridDim.x * (kBS / kW);
+  double part = 0.0;
+  for (int64_t i = __builtin_amdgcn_readfirstlane((int)(blockIdx.x * (kBS / kW) + (threadIdx.x >> 6))); i < cnt;
+       i += step) {
+    const int64_t s = list[i];
+    offset_slice(A, s, A.width[s], A.base_ptr[s], x, y, &part);
+  }
+  if (DOT) {
+    part = dev::block_sum<kBS>(part, s_red);
+    if (threadIdx.x == 0) partials[blockIdx.x] = part;
+  }
+}
+
+// slice form flags from the widths (0 delta, 1 packed, 2 offset)
+__global__ void k_sell_forms(int64_t ns, const int32_t* __restrict__ width, uint8_t* __restrict__ fd,
+                             uint8_t* __restrict__ fp, uint8_t* __restrict__ fo) {
+  const int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= ns) return;
+  const int w = width[s];
+  fp[s] = w <= -(kPackedBase + 1) ? 1 : 0;
+  fd[s] = (w < 0 && w > -(kPackedBase + 1)) ? 1 : 0;
+  fo[s] = w >= 0 ? 1 : 0;
 }
 
 // Iteration of parity par: every block forms (p, q) from the SpMV partials in the
@@ -501,43 +633,61 @@ Sell sell_view(const ngpb_ctx* c) {
   A.ov_col = c->sell_ov_col.p;
   A.pk_val = c->sell_pk_val.p;
   A.pk_idx = c->sell_pk_idx.p;
-  A.pk_perm = c->sell_pk_perm.p;
-  A.pk_len = c->sell_pk_len.p;
+  A.pk_lp = c->sell_pk_lp.p;
+  A.pk_hdr = reinterpret_cast<const PkHdr*>(c->sell_pk_hdr.p);
   A.pk_dict = c->sell_pk_dict.p;
   return A;
 }
 
-// blocks of k_sell_spmv<.., SCHED> resident on the device at once (multiple of 8)
+// blocks of the form-specialised SpMV kernels resident on the device at once
 unsigned sell_resident(ngpb_ctx* c) {
   if (c->sell_resident > 0) return (unsigned)c->sell_resident;
   int cus = 0, per_cu = 0;
   NGPB_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, c->device));
-  NGPB_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_sell_spmv<true, 1>, kBS, 0));
+  NGPB_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_sell_packed<true>, kBS, 0));
   const int r = std::max(8, (std::max(cus, 1) * std::max(per_cu, 1)) / 8 * 8);
   c->sell_resident = r;
   return (unsigned)r;
 }
 
-// schedule 0: at most 2048 blocks of 4 waves, each wave walks several pairs;
-// schedules 1 and 2: the resident block count (fewer if there are fewer pairs)
-unsigned sell_grid(ngpb_ctx* c) {
-  const int64_t pairs = (c->n_slices + 1) / 2;
-  const int64_t need = std::max<int64_t>(1, (pairs + (kBS / kW) - 1) / (kBS / kW));
-  if (c->spmv_sched == 0) return (unsigned)std::min<int64_t>(need, 2048);
-  const int64_t r = sell_resident(c);
-  if (need >= r) return (unsigned)r;
-  return (unsigned)(c->spmv_sched == 1 ? (need + 7) / 8 * 8 : need);
+// blocks for `units` wave work units (4 waves per block): at most 2048 (schedule 0)
+// or the resident block count (schedule 2); 0 when there is no work
+unsigned form_grid(ngpb_ctx* c, int64_t units) {
+  if (units <= 0) return 0;
+  const int64_t need = (units + (kBS / kW) - 1) / (kBS / kW);
+  const int64_t cap = c->spmv_sched == 2 ? (int64_t)sell_resident(c) : 2048;
+  return (unsigned)std::min<int64_t>(need, cap);
+}
+
+// number of (p, q) partials the SpMV of the current schedule writes
+int sell_partials(ngpb_ctx* c) {
+  if (c->spmv_sched == 1)
+    return (int)std::max<int64_t>(1, std::min<int64_t>(((c->n_slices + 1) / 2 + (kBS / kW) - 1) / (kBS / kW), 2048));
+  return (int)(form_grid(c, (c->n_list_delta + 1) / 2) + form_grid(c, c->n_list_packed) +
+               form_grid(c, c->n_list_offset));
 }
 
 template <bool DOT>
 void launch_sell_t(ngpb_ctx* c, const float* x, float* y, double* part) {
-  const dim3 g(sell_grid(c)), b(kBS);
   const KrylovScalars* ks = DOT ? c->ks.p : nullptr;
-  switch (c->spmv_sched) {
-    case 0: hipLaunchKernelGGL((k_sell_spmv<DOT, 0>), g, b, 0, c->stream, sell_view(c), x, y, ks, part); break;
-    case 1: hipLaunchKernelGGL((k_sell_spmv<DOT, 1>), g, b, 0, c->stream, sell_view(c), x, y, ks, part); break;
-    default: hipLaunchKernelGGL((k_sell_spmv<DOT, 2>), g, b, 0, c->stream, sell_view(c), x, y, ks, part); break;
+  const Sell A = sell_view(c);
+  if (c->spmv_sched == 1) {
+    const unsigned g = (unsigned)sell_partials(c);
+    hipLaunchKernelGGL(k_sell_spmv<DOT>, dim3(g), dim3(kBS), 0, c->stream, A, x, y, ks, part);
+    return;
   }
+  const unsigned gd = form_grid(c, (c->n_list_delta + 1) / 2);
+  const unsigned gp = form_grid(c, c->n_list_packed);
+  const unsigned go = form_grid(c, c->n_list_offset);
+  if (gd)
+    hipLaunchKernelGGL(k_sell_delta<DOT>, dim3(gd), dim3(kBS), 0, c->stream, A, c->sell_list_delta.p, c->n_list_delta,
+                       x, y, ks, part);
+  if (gp)
+    hipLaunchKernelGGL(k_sell_packed<DOT>, dim3(gp), dim3(kBS), 0, c->stream, A, c->sell_list_packed.p,
+                       c->n_list_packed, x, y, ks, DOT ? part + gd : nullptr);
+  if (go)
+    hipLaunchKernelGGL(k_sell_offset<DOT>, dim3(go), dim3(kBS), 0, c->stream, A, c->sell_list_offset.p,
+                       c->n_list_offset, x, y, ks, DOT ? part + gd + gp : nullptr);
 }
 
 void launch_sell(ngpb_ctx* c, bool dot, const float* x, float* y, double* part) {
@@ -555,8 +705,10 @@ void build_compact(ngpb_ctx* c) {
   hipLaunchKernelGGL(k_scale, dim3(blocks_for(n, 256)), dim3(256), 0, c->stream, n, c->rowptr.p, c->col.p, c->val.p,
                      c->dscale.p);
   c->slice_width.resize((size_t)ns);
-  DBuf<int64_t> wcount, widec, wide_off, ovc, ov_off, dslots, pkc, pk_off, pkd, pk_doff;
+  DBuf<int64_t> wcount, widec, wide_off, ovc, ov_off, dslots, pkc, pk_off, pkd, pk_doff, pkg, pk_goff;
   DBuf<int> bad;
+  pkg.resize((size_t)ns);
+  pk_goff.resize((size_t)ns);
   pkc.resize((size_t)ns);
   pk_off.resize((size_t)ns);
   pkd.resize((size_t)ns);
@@ -571,7 +723,7 @@ void build_compact(ngpb_ctx* c) {
   NGPB_HIP(hipMemsetAsync(bad.p, 0, sizeof(int), c->stream));
   const unsigned g = (unsigned)((ns + (kBS / kW) - 1) / (kBS / kW));
   hipLaunchKernelGGL(k_sell_shape, dim3(g), dim3(kBS), 0, c->stream, n, ns, c->rowptr.p, c->col.p, c->slice_width.p,
-                     wcount.p, widec.p, ovc.p, dslots.p, pkc.p, pkd.p, bad.p);
+                     wcount.p, widec.p, ovc.p, dslots.p, pkc.p, pkd.p, pkg.p, bad.p);
   NGPB_HIP(hipGetLastError());
   int hb = 0;
   NGPB_HIP(hipMemcpyAsync(&hb, bad.p, sizeof(int), hipMemcpyDeviceToHost, c->stream));
@@ -583,6 +735,9 @@ void build_compact(ngpb_ctx* c) {
   const int64_t ndslots = exclusive_scan(c, dslots.p, dslots.p, ns);
   const int64_t npk = exclusive_scan(c, pkc.p, pk_off.p, ns);
   const int64_t npkd = exclusive_scan(c, pkd.p, pk_doff.p, ns);
+  const int64_t npkg = exclusive_scan(c, pkg.p, pk_goff.p, ns);
+  NGPB_REQUIRE(npk < INT32_MAX && npkd < INT32_MAX && npkg < INT32_MAX, NGPB_E_ARG,
+               "operator too large for 32-bit packed-slice offsets");
   {
     // packed slices = slices with a dictionary (every packed slice has >= 1 delta or
     // rows without off-diagonals only, which are counted through their width below)
@@ -596,9 +751,12 @@ void build_compact(ngpb_ctx* c) {
   c->sell_pk_entries = npk;
   c->sell_pk_dict_entries = npkd;
   c->sell_pk_val.resize((size_t)std::max<int64_t>(npk, 1));
-  c->sell_pk_idx.resize((size_t)std::max<int64_t>(npk, 1));
-  c->sell_pk_perm.resize((size_t)ns * kW);
-  c->sell_pk_len.resize((size_t)ns * kW);
+  c->sell_pk_words = npkg;
+  c->sell_pk_idx.resize((size_t)std::max<int64_t>(npkg, 1));
+  // bytes of rows that end inside a group stay 0, a valid dictionary index
+  NGPB_HIP(hipMemsetAsync(c->sell_pk_idx.p, 0, c->sell_pk_idx.bytes(), c->stream));
+  c->sell_pk_lp.resize((size_t)ns * kW);
+  c->sell_pk_hdr.resize((size_t)ns * (sizeof(PkHdr) / sizeof(int64_t)));
   c->sell_pk_dict.resize((size_t)std::max<int64_t>(npkd, 1));
   c->sell_bases = nbases;
   c->sell_entries = nbases * kW;
@@ -619,9 +777,24 @@ void build_compact(ngpb_ctx* c) {
   hipLaunchKernelGGL(k_sell_fill, dim3(g), dim3(kBS), 0, c->stream, n, ns, c->rowptr.p, c->col.p, c->val.p, c->dscale.p,
                      c->slice_width.p, c->slice_base_ptr.p, wide_off.p, widec.p, ov_off.p, ovc.p, c->sell_val.p,
                      c->sell_off.p, c->sell_base.p, c->sell_col32.p, c->slice_wide_ptr.p, c->slice_ov_ptr.p,
-                     c->sell_ov_cnt.p, c->sell_ov_val.p, c->sell_ov_col.p, pk_off.p, pk_doff.p, c->sell_pk_val.p,
-                     c->sell_pk_idx.p, c->sell_pk_perm.p, c->sell_pk_len.p, c->sell_pk_dict.p);
+                     c->sell_ov_cnt.p, c->sell_ov_val.p, c->sell_ov_col.p, pk_off.p, pk_doff.p, pk_goff.p,
+                     c->sell_pk_val.p, reinterpret_cast<uint8_t*>(c->sell_pk_idx.p), c->sell_pk_lp.p, c->sell_pk_dict.p,
+                     reinterpret_cast<PkHdr*>(c->sell_pk_hdr.p));
   NGPB_HIP(hipGetLastError());
+  {
+    DBuf<uint8_t> fd, fp, fo;
+    fd.resize((size_t)ns);
+    fp.resize((size_t)ns);
+    fo.resize((size_t)ns);
+    hipLaunchKernelGGL(k_sell_forms, dim3(blocks_for(ns, 256)), dim3(256), 0, c->stream, ns, c->slice_width.p, fd.p,
+                       fp.p, fo.p);
+    c->sell_list_delta.resize((size_t)ns);
+    c->sell_list_packed.resize((size_t)ns);
+    c->sell_list_offset.resize((size_t)ns);
+    c->n_list_delta = compact_flags(c, fd.p, ns, c->sell_list_delta.p);
+    c->n_list_packed = compact_flags(c, fp.p, ns, c->sell_list_packed.p);
+    c->n_list_offset = compact_flags(c, fo.p, ns, c->sell_list_offset.p);
+  }
   NGPB_HIP(hipStreamSynchronize(c->stream));
   c->have_compact = true;
 }
@@ -629,13 +802,14 @@ void build_compact(ngpb_ctx* c) {
 double spmv_compact_bytes(const ngpb_ctx* c) {
   // delta-slice entries (4: value), offset-slice entries (4 value + 2 offset), slot
   // bases/deltas (4), 32-bit columns of wide slices, overflow entries (4 + 4),
-  // packed entries (4 value + 1 index), packed rows (permutation + count, 1 + 1),
-  // dictionary entries (4), slice metadata (width 4 + three 8-byte offsets), x read
-  // once and y written
+  // packed values (4), packed index words (4, four slots of a row each), packed rows
+  // (permutation + count, 1 + 1), dictionary entries (4), packed-slice headers (32),
+  // slice metadata (width 4 + three 8-byte offsets), x read once and y written
   const double off_entries = (double)(c->sell_entries - c->sell_delta_entries);
   return (double)c->sell_delta_entries * 4.0 + off_entries * 6.0 + (double)c->sell_bases * 4.0 +
-         (double)c->sell_wide_entries * 4.0 + (double)c->sell_ov_entries * 8.0 + (double)c->sell_pk_entries * 5.0 +
-         (double)c->sell_pk_slices * kW * 2.0 + (double)c->sell_pk_dict_entries * 4.0 +
+         (double)c->sell_wide_entries * 4.0 + (double)c->sell_ov_entries * 8.0 + (double)c->sell_pk_entries * 4.0 +
+         (double)c->sell_pk_words * 4.0 + (double)c->sell_pk_slices * (kW * 2.0 + 32.0) +
+         (double)c->sell_pk_dict_entries * 4.0 +
          (double)c->n_slices * 28.0 + (double)c->n * (4.0 + 4.0);
 }
 
@@ -682,7 +856,7 @@ void solve_mixed(ngpb_ctx* c, const ngpb_solve_opts& o, ngpb_solve_stats& st) {
   c->ps.reserve((size_t)n);
   c->qs.reserve((size_t)n);
   const unsigned gv = vec_grid(n);
-  const unsigned gs = sell_grid(c);
+  const unsigned gs = (unsigned)sell_partials(c);
   ensure_reduce(c, std::max<int64_t>(gs, 2048) + 2 * (int64_t)gv);
   double* part = c->red.partials.p;
   double* ppq = part;                                  // SpMV (p, q) partials

@@ -183,13 +183,13 @@ int ngpb_time_copy(ngpb_ctx* ctx, int64_t bytes, int reps, double* ms_per_launch
    compact SpMV, [7] algorithmic bytes of one FP64 CSR SpMV, [8] packed slices,
    [9] packed entries. No counterpart in the reference. */
 int ngpb_operator_info(ngpb_ctx* ctx, int64_t* info10);
-/* Work schedule of the SpMV kernels (compact FP32 and FP64 CSR): 0 (default)
-   grid-stride over slices / row chunks on up to 2048 blocks, 1 one contiguous range
-   per wave (block) on the resident grid with XCD-aware block numbering, 2 grid-stride
-   on the resident grid. Results differ only in the grouping of the partial sums of
-   (p, q). On MI355X, 1 measured 1.4-1.5x slower than 0 and 2 (the grid-stride window
-   keeps the gathers of all waves within a small moving range of x). No counterpart in
-   the reference. */
+/* Work schedule of the SpMV kernels. Compact FP32: 0 one kernel per slice form
+   (delta / packed / offset, each walking a list of its slices) on up to 2048 blocks,
+   1 a single kernel for all forms, 2 (default) as 0 on the resident block count.
+   FP64 CSR: 0 grid-stride over row chunks on up to 2048 blocks, 1 one contiguous
+   chunk range per block with XCD-aware block numbering on the resident grid
+   (measured 1.36x slower), 2 grid-stride on the resident grid. Results differ only in
+   the grouping of the partial sums of (p, q). No counterpart in the reference. */
 int ngpb_set_spmv_schedule(ngpb_ctx* ctx, int schedule);
 
 /* ---- batched solve of many independent systems ------------------------------------
