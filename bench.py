@@ -131,12 +131,12 @@ def bench_batch(args):
     fp64_s = time.perf_counter() - t
     log(f"one-at-a-time FP64: {args.systems} solves in {fp64_s:.2f} s")
 
+    spmv_bytes = sum(p.ctx.operator_info()["compact_spmv_bytes"] for p in preps)
     bs = BatchedSolver(0, max_systems=max(args.systems, 1), tol=args.tol)
     for p in preps:
         bs.add(p)
     tune = bs.autotune()
     log(f"autotune: {tune}")
-    spmv_bytes = sum(p.ctx.operator_info()["compact_spmv_bytes"] for p in preps)
     t = time.perf_counter()
     res = bs.solve(energies=True)
     batch_s = time.perf_counter() - t

@@ -648,7 +648,7 @@ int ngpb_kernel_stats(ngpb_ctx* c, int precision, int64_t* launches, double* tot
 
 int ngpb_operator_info(ngpb_ctx* c, int64_t* info) {
   return guarded(c, [&] {
-    not_in_batch(c);
+    if (c->in_batch) need(c->have_compact, "context is in a batch");  // read-only then
     NGPB_REQUIRE(info, NGPB_E_ARG, "NULL output");
     need(c->have_system, "no assembled system");
     if (!c->have_compact) ngpb::build_compact(c);

@@ -152,7 +152,9 @@ __global__ void __launch_bounds__(kBS) k_sell_shape(int64_t n, int64_t nslices, 
   const bool offset = !use_delta && !use_packed;
   const int extra = offset ? wave_sum_i(max(0, cnt - best_w)) : 0;
   if (lane_id() == 0) {
-    width[s] = use_delta ? -nd : (use_packed ? -(kPackedBase + nd) : best_w);
+    // a slice without off-diagonals (rows of Dirichlet nodes) is a delta slice of
+    // zero slots, marked -kPackedBase since -0 would read as an offset slice
+    width[s] = use_delta ? (nd == 0 ? -kPackedBase : -nd) : (use_packed ? -(kPackedBase + nd) : best_w);
     wcount[s] = use_delta ? nd : (use_packed ? 0 : best_w);
     wide_count[s] = (offset && wide) ? (int64_t)best_w * kW : 0;
     ov_count[s] = extra;
