@@ -257,7 +257,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--workload", choices=["globular", "born", "large", "batch", "serve"], default="globular")
     ap.add_argument("--requests", type=int, default=48, help="serve workload: number of requests")
-    ap.add_argument("--rate", type=float, default=4.0, help="serve workload: Poisson arrival rate [1/s]")
+    ap.add_argument("--rate", type=float, default=8.0, help="serve workload: Poisson arrival rate [1/s]")
     ap.add_argument("--systems", type=int, default=512, help="batch workload: number of ligand systems")
     ap.add_argument("--tol", type=float, default=1e-10)
     ap.add_argument("--check-every", type=int, default=32)

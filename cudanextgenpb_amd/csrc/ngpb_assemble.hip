@@ -97,6 +97,12 @@ __global__ void k_u32_to_i64(int64_t n, const unsigned* __restrict__ a, int64_t*
   if (i < n) b[i] = a[i];
 }
 
+__device__ __forceinline__ int64_t wave_sum_i64(int64_t v) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+  return v;
+}
+
 __device__ __forceinline__ bool em_less(uint64_t ka, uint16_t wa, uint64_t kb, uint16_t wb) {
   return ka < kb || (ka == kb && (wa >> 8) < (wb >> 8));
 }
@@ -137,6 +143,174 @@ __global__ void k_row_sort(int64_t N, const int64_t* __restrict__ em_rowptr, uin
   }
   fcount[row] = nf;
   bcount[row] = nbc;
+}
+
+// The same sort and counts, in parallel: every element's final position is its rank
+// under (key, k, original position) -- the order the stable insertion sort above
+// produces. Rows of at most kSortCap emissions: one wave per row, staged in LDS.
+// Longer rows (parents of many hanging nodes) are listed and sorted by one block
+// per row with kSortBigCap entries staged; beyond that, the serial sort (lane 0).
+constexpr int kSortCap = 256;
+constexpr int kSortBS = 256;
+constexpr int kSortBigCap = 2048;  // 40 KB of LDS per block
+
+// distinct columns of a sorted run, counted as final slots or Dirichlet couplings
+__device__ __forceinline__ void count_slot(int64_t row, bool rb, int64_t col, const uint8_t* __restrict__ is_bnd,
+                                           int64_t& nf, int64_t& nbc) {
+  if (rb) {
+    if (col == row) ++nf;
+  } else if (is_bnd[col]) {
+    ++nbc;
+  } else {
+    ++nf;
+  }
+}
+
+__global__ void __launch_bounds__(kSortBS) k_row_sort_wave(int64_t N, const int64_t* __restrict__ em_rowptr,
+                                                           uint64_t* __restrict__ key, uint16_t* __restrict__ kw,
+                                                           const uint8_t* __restrict__ is_bnd,
+                                                           int64_t* __restrict__ fcount, int64_t* __restrict__ bcount,
+                                                           int32_t* __restrict__ big_rows, int* __restrict__ n_big) {
+  __shared__ uint64_t s_key[kSortBS / 64][kSortCap];
+  __shared__ uint64_t s_out[kSortBS / 64][kSortCap];
+  __shared__ uint16_t s_kw[kSortBS / 64][kSortCap];
+  __shared__ uint16_t s_kwo[kSortBS / 64][kSortCap];
+  const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int64_t row = (int64_t)blockIdx.x * (kSortBS / 64) + wv;
+  const bool valid = row < N;
+  const int64_t b = valid ? em_rowptr[row] : 0, e = valid ? em_rowptr[row + 1] : 0;
+  const int E = (int)(e - b);
+  const bool staged = E <= kSortCap;
+  uint64_t* sk = s_key[wv];
+  uint64_t* so = s_out[wv];
+  uint16_t* sw = s_kw[wv];
+  uint16_t* swo = s_kwo[wv];
+  if (staged)
+    for (int i = lane; i < E; i += 64) {
+      sk[i] = key[b + i];
+      sw[i] = kw[b + i];
+    }
+  __syncthreads();
+  if (staged)
+    for (int i = lane; i < E; i += 64) {
+      const uint64_t kk = sk[i];
+      const int wk = sw[i] >> 8;
+      int rank = 0;
+      for (int j = 0; j < E; ++j) {
+        const uint64_t kj = sk[j];
+        const int wj = sw[j] >> 8;
+        rank += (kj < kk || (kj == kk && (wj < wk || (wj == wk && j < i)))) ? 1 : 0;
+      }
+      so[rank] = kk;
+      swo[rank] = sw[i];
+    }
+  __syncthreads();
+  if (!valid) return;
+  if (!staged) {
+    if (lane == 0) big_rows[atomicAdd(n_big, 1)] = (int32_t)row;
+    return;
+  }
+  const bool rb = is_bnd[row] != 0;
+  int64_t nf = 0, nbc = 0;
+  for (int i = lane; i < E; i += 64) {
+    const uint64_t kk = so[i];
+    key[b + i] = kk;
+    kw[b + i] = swo[i];
+    const int64_t col = (int64_t)(kk >> 32);
+    if (i > 0 && (int64_t)(so[i - 1] >> 32) == col) continue;
+    count_slot(row, rb, col, is_bnd, nf, nbc);
+  }
+  nf = wave_sum_i64(nf);
+  nbc = wave_sum_i64(nbc);
+  if (lane == 0) {
+    fcount[row] = nf;
+    bcount[row] = nbc;
+  }
+}
+
+// one block per listed long row
+__global__ void __launch_bounds__(kSortBS) k_row_sort_block(const int32_t* __restrict__ rows,
+                                                            const int64_t* __restrict__ em_rowptr,
+                                                            uint64_t* __restrict__ key, uint16_t* __restrict__ kw,
+                                                            const uint8_t* __restrict__ is_bnd,
+                                                            int64_t* __restrict__ fcount,
+                                                            int64_t* __restrict__ bcount) {
+  extern __shared__ uint64_t s_dyn[];  // key[cap], out[cap], then kw[cap], kwo[cap]
+  __shared__ int64_t s_red[2][kSortBS / 64];
+  const int64_t row = rows[blockIdx.x];
+  const int64_t b = em_rowptr[row], e = em_rowptr[row + 1];
+  const int E = (int)(e - b);
+  const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+  const bool rb = is_bnd[row] != 0;
+  int64_t nf = 0, nbc = 0;
+  if (E <= kSortBigCap) {
+    uint64_t* sk = s_dyn;
+    uint64_t* so = s_dyn + kSortBigCap;
+    uint16_t* sw = reinterpret_cast<uint16_t*>(s_dyn + 2 * kSortBigCap);
+    uint16_t* swo = sw + kSortBigCap;
+    for (int i = tid; i < E; i += kSortBS) {
+      sk[i] = key[b + i];
+      sw[i] = kw[b + i];
+    }
+    __syncthreads();
+    for (int i = tid; i < E; i += kSortBS) {
+      const uint64_t kk = sk[i];
+      const int wk = sw[i] >> 8;
+      int rank = 0;
+      for (int j = 0; j < E; ++j) {
+        const uint64_t kj = sk[j];
+        const int wj = sw[j] >> 8;
+        rank += (kj < kk || (kj == kk && (wj < wk || (wj == wk && j < i)))) ? 1 : 0;
+      }
+      so[rank] = kk;
+      swo[rank] = sw[i];
+    }
+    __syncthreads();
+    for (int i = tid; i < E; i += kSortBS) {
+      const uint64_t kk = so[i];
+      key[b + i] = kk;
+      kw[b + i] = swo[i];
+      const int64_t col = (int64_t)(kk >> 32);
+      if (i > 0 && (int64_t)(so[i - 1] >> 32) == col) continue;
+      count_slot(row, rb, col, is_bnd, nf, nbc);
+    }
+  } else if (tid == 0) {
+    for (int64_t i = b + 1; i < e; ++i) {
+      const uint64_t kk = key[i];
+      const uint16_t ww = kw[i];
+      int64_t j = i - 1;
+      while (j >= b && em_less(kk, ww, key[j], kw[j])) {
+        key[j + 1] = key[j];
+        kw[j + 1] = kw[j];
+        --j;
+      }
+      key[j + 1] = kk;
+      kw[j + 1] = ww;
+    }
+    int64_t prev = -1;
+    for (int64_t i = b; i < e; ++i) {
+      const int64_t col = (int64_t)(key[i] >> 32);
+      if (col == prev) continue;
+      prev = col;
+      count_slot(row, rb, col, is_bnd, nf, nbc);
+    }
+  }
+  nf = wave_sum_i64(nf);
+  nbc = wave_sum_i64(nbc);
+  if (lane == 0) {
+    s_red[0][wv] = nf;
+    s_red[1][wv] = nbc;
+  }
+  __syncthreads();
+  if (tid == 0) {
+    int64_t a = 0, c2 = 0;
+    for (int w = 0; w < kSortBS / 64; ++w) {
+      a += s_red[0][w];
+      c2 += s_red[1][w];
+    }
+    fcount[row] = a;
+    bcount[row] = c2;
+  }
 }
 
 __global__ void k_row_fill(int64_t N, const int64_t* __restrict__ em_rowptr, const uint64_t* __restrict__ key,
@@ -453,9 +627,25 @@ void run_symbolic(ngpb_ctx* c) {
   DBuf<int64_t> fcount, bcount;
   fcount.resize((size_t)N);
   bcount.resize((size_t)N);
-  hipLaunchKernelGGL(k_row_sort, dim3(blocks_for(N, 128)), dim3(128), 0, c->stream, N, c->em_rowptr.p, c->em_key.p,
-                     c->em_kw.p, c->is_bnd.p, fcount.p, bcount.p);
-  NGPB_HIP(hipGetLastError());
+  {
+    DBuf<int32_t> big;
+    DBuf<int> nbig;
+    big.resize((size_t)N);
+    nbig.resize(1);
+    NGPB_HIP(hipMemsetAsync(nbig.p, 0, sizeof(int), c->stream));
+    hipLaunchKernelGGL(k_row_sort_wave, dim3(blocks_for(N, kSortBS / 64)), dim3(kSortBS), 0, c->stream, N,
+                       c->em_rowptr.p, c->em_key.p, c->em_kw.p, c->is_bnd.p, fcount.p, bcount.p, big.p, nbig.p);
+    NGPB_HIP(hipGetLastError());
+    int hbig = 0;
+    NGPB_HIP(hipMemcpyAsync(&hbig, nbig.p, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+    NGPB_HIP(hipStreamSynchronize(c->stream));
+    if (hbig > 0) {
+      const size_t lds = (size_t)kSortBigCap * (8 + 8 + 2 + 2);
+      hipLaunchKernelGGL(k_row_sort_block, dim3((unsigned)hbig), dim3(kSortBS), lds, c->stream, big.p, c->em_rowptr.p,
+                         c->em_key.p, c->em_kw.p, c->is_bnd.p, fcount.p, bcount.p);
+      NGPB_HIP(hipGetLastError());
+    }
+  }
   const int64_t nnz = exclusive_scan(c, fcount.p, fcount.p, N);
   const int64_t nbc = exclusive_scan(c, bcount.p, bcount.p, N);
   NGPB_REQUIRE(nnz < (int64_t)INT32_MAX && nbc < (int64_t)INT32_MAX, NGPB_E_ARG, "nnz exceeds 32-bit CSR");

@@ -3,11 +3,12 @@
 Requests (a molecule with its octree mesh and ray table) arrive over time. The
 server admits them while their predicted HBM footprint fits the budget:
 
-* small systems (up to `small_max_unknowns`) join the running mixed-precision batch
+* small systems (up to `small_max_unknowns`) are uploaded and assembled by worker
+  threads on their own context streams, join the running mixed-precision batch
   (``native.Batch``) at its next replay boundary and leave it as soon as they are done,
-  so new requests do not wait for the batch to drain;
-* large systems are streamed: each is solved by the single-system mixed solver on its
-  own context stream in a worker thread, overlapping the batch on the device.
+  so new requests neither wait for the batch to drain nor stall it;
+* large systems are streamed: each is prepared and solved by the single-system mixed
+  solver on its own context stream in a worker thread, overlapping the batch.
 
 Each system keeps its mesh, operator and solver state resident on the device until
 its energy has been read back. A result does not depend on what else was in flight:
@@ -24,9 +25,8 @@ the reference has no server, it solves one system per process.
 from __future__ import annotations
 
 import math
-import threading
 import time
-from concurrent.futures import Future, ThreadPoolExecutor
+from concurrent.futures import ThreadPoolExecutor
 from dataclasses import dataclass
 from typing import Dict, List, Optional
 
@@ -105,7 +105,7 @@ class Result:
 class Server:
     def __init__(self, device: int = 0, budget_bytes: float = 250e9, small_max_unknowns: int = 1_500_000,
                  max_batch: int = 256, tol: float = 1e-10, inner_per_replay: int = 8, stream_workers: int = 2,
-                 keep_phi: bool = False):
+                 prep_workers: int = 2, keep_phi: bool = False):
         self.device = device
         self.budget = float(budget_bytes)
         self.small_max = small_max_unknowns
@@ -114,27 +114,37 @@ class Server:
         self.batch = native.Batch(device, max_batch, tol=tol, inner_per_replay=inner_per_replay)
         self.max_batch = max_batch
         self.pool = ThreadPoolExecutor(max_workers=stream_workers)
-        self.lock = threading.Lock()
+        self.prep_pool = ThreadPoolExecutor(max_workers=prep_workers)
         self.peak_bytes = 0.0
 
     def close(self):
         self.pool.shutdown(wait=True)
+        self.prep_pool.shutdown(wait=True)
         self.batch.close()
 
-    def _stream_solve(self, req: Request, ctx: native.Context):
-        _, st = ctx.solve(native.PREC_MIXED, tol=self.tol, want_phi=False)
-        e = ctx.energy()
-        phi = ctx.get_phi() if self.keep_phi else None
-        return st, e, phi
+    def _stream_solve(self, req: Request):
+        """Large request: prepared and solved on its own context (worker thread)."""
+        p = prepare(req.problem, self.device)
+        _, st = p.ctx.solve(native.PREC_MIXED, tol=self.tol, want_phi=False)
+        e = p.ctx.energy()
+        phi = p.ctx.get_phi() if self.keep_phi else None
+        nbytes = p.ctx.device_bytes()
+        p.ctx.close()
+        return st, e, phi, nbytes
 
     def run(self, requests: List[Request], realtime: bool = True) -> Dict[int, Result]:
+        """Serves `requests` (arrival times relative to the start). Small requests are
+        prepared by worker threads and join the batch at a replay boundary; the main
+        loop keeps replaying the batch meanwhile. Large requests are prepared and
+        solved by stream workers. realtime=False admits every request at once."""
         reqs = sorted(requests, key=lambda r: r.arrival)
         t0 = time.perf_counter()
-        clock = (lambda: time.perf_counter() - t0) if realtime else None
+        clock = (lambda: time.perf_counter() - t0) if realtime else (lambda: 0.0)
         nxt = 0
         pending: List[Request] = []
+        preparing: List[tuple] = []       # (req, admitted, future -> Prepared)
         in_batch: Dict[int, tuple] = {}   # slot -> (req, ctx, admitted)
-        streaming: List[tuple] = []       # (req, ctx, admitted, future)
+        streaming: List[tuple] = []       # (req, admitted, future)
         used = 0.0
         results: Dict[int, Result] = {}
         while len(results) < len(reqs):
@@ -147,20 +157,27 @@ class Server:
                 small = req.n <= self.small_max
                 if used + req.predicted_bytes > self.budget:
                     continue
-                if small and len(in_batch) >= self.max_batch:
+                if small and len(in_batch) + len(preparing) >= self.max_batch:
                     continue
                 pending.remove(req)
-                adm = clock() if realtime else 0.0
-                p = prepare(req.problem, self.device)
                 used += req.predicted_bytes
                 self.peak_bytes = max(self.peak_bytes, used)
                 if small:
-                    slot = self.batch.add(p.ctx)
-                    p.ctx._keep_mesh = None
-                    in_batch[slot] = (req, p.ctx, adm)
+                    preparing.append((req, clock(), self.prep_pool.submit(prepare, req.problem, self.device)))
                 else:
-                    streaming.append((req, p.ctx, adm, self.pool.submit(self._stream_solve, req, p.ctx)))
+                    streaming.append((req, clock(), self.pool.submit(self._stream_solve, req)))
             progressed = False
+            # prepared small systems join at this replay boundary
+            for item in list(preparing):
+                req, adm, fut = item
+                if not fut.done():
+                    continue
+                p = fut.result()
+                slot = self.batch.add(p.ctx)
+                p.ctx._keep_mesh = None
+                in_batch[slot] = (req, p.ctx, adm)
+                preparing.remove(item)
+                progressed = True
             if in_batch:
                 self.batch.run(1)
                 progressed = True
@@ -171,7 +188,7 @@ class Server:
                     self.batch.remove(slot)
                     e = ctx.energy()
                     phi = ctx.get_phi() if self.keep_phi else None
-                    t = clock() if realtime else 0.0
+                    t = clock()
                     nbytes = ctx.device_bytes()
                     ctx.close()
                     used -= req.predicted_bytes
@@ -179,22 +196,20 @@ class Server:
                     results[req.rid] = Result(req.rid, "batch", req.arrival, adm, t, (t - req.arrival) * 1e3,
                                               st["iterations"], st["status"], e, nbytes, phi)
             for item in list(streaming):
-                req, ctx, adm, fut = item
+                req, adm, fut = item
                 if not fut.done():
                     continue
-                st, e, phi = fut.result()
-                t = clock() if realtime else 0.0
-                nbytes = ctx.device_bytes()
-                ctx.close()
+                st, e, phi, nbytes = fut.result()
+                t = clock()
                 used -= req.predicted_bytes
                 streaming.remove(item)
                 progressed = True
                 results[req.rid] = Result(req.rid, "stream", req.arrival, adm, t, (t - req.arrival) * 1e3,
                                           st["iterations"], st["status"], e, nbytes, phi)
             if not progressed:
-                if pending and not in_batch and not streaming:
+                if pending and not in_batch and not streaming and not preparing:
                     raise RuntimeError("a request does not fit the HBM budget on its own")
-                if realtime and nxt < len(reqs) and not pending and not streaming:
+                if realtime and nxt < len(reqs) and not pending and not streaming and not preparing:
                     time.sleep(max(0.0, min(0.005, reqs[nxt].arrival - clock())))
                 else:
                     time.sleep(0.0005)
