@@ -249,42 +249,62 @@ constexpr int kNB = 256;
 
 // MODE 0/1: flux sources, first += q flux / r
 // MODE 2:   triangle-vertex sources, second += q (phi area / (3 4 pi)) (d . n) / r^3
-// 1/r comes from rsqrt (about one ulp), so the terms are not rounded exactly like
-// the reference's sqrt and divisions; the sums agree to ~1e-13 relative.
+// Each thread takes kNS sources, so every atom read from LDS serves kNS pairs.
+// 1/r: FP32 reciprocal square root refined by one Newton step in FP64 (relative
+// error ~1e-14), not rounded exactly like the reference's sqrt and divisions; the
+// sums agree to ~1e-13 relative.
+constexpr int kNS = 2;
+
+__device__ __forceinline__ double inv_sqrt(double r2) {
+  const double y = (double)rsqrtf((float)r2);
+  return y * (1.5 - 0.5 * r2 * y * y);
+}
+
 template <int MODE>
 __global__ void __launch_bounds__(kNB) k_nbody(int64_t ns, const double* __restrict__ src, int64_t nq,
                                                const double4* __restrict__ atoms, double* __restrict__ acc) {
+  constexpr int VS = MODE == 2 ? kVS : kFS;
   __shared__ double4 s_at[kNB];
-  const int64_t s = (int64_t)blockIdx.x * kNB + threadIdx.x;
-  const bool act = s < ns;
-  double v[8] = {0, 0, 0, 0, 0, 0, 0, 0};
-  if (act)
-    for (int k = 0; k < (MODE == 2 ? kVS : kFS); ++k) v[k] = src[(MODE == 2 ? kVS : kFS) * s + k];
   const double inv_4pi = 1.0 / (4.0 * kPi);
-  // per-source weight: flux, or phi_sup * area / 3 / (4 pi)
-  const double wsrc = MODE == 2 ? v[6] * inv_4pi * v[7] / 3.0 : v[3];
-  double sum = 0.0;
+  double v[kNS][8], wsrc[kNS], sum[kNS];
+  bool act[kNS];
+#pragma unroll
+  for (int u = 0; u < kNS; ++u) {
+    const int64_t s = ((int64_t)blockIdx.x * kNS + u) * kNB + threadIdx.x;
+    act[u] = s < ns;
+#pragma unroll
+    for (int k = 0; k < 8; ++k) v[u][k] = (act[u] && k < VS) ? src[VS * s + k] : 0.0;
+    // per-source weight: flux, or phi_sup * area / 3 / (4 pi)
+    wsrc[u] = MODE == 2 ? v[u][6] * inv_4pi * v[u][7] / 3.0 : v[u][3];
+    sum[u] = 0.0;
+  }
   for (int64_t t0 = 0; t0 < nq; t0 += kNB) {
     __syncthreads();
     if (t0 + threadIdx.x < nq) s_at[threadIdx.x] = atoms[t0 + threadIdx.x];
     __syncthreads();
     const int cnt = (int)min((int64_t)kNB, nq - t0);
-    if (!act) continue;
-#pragma unroll 4
+#pragma unroll 2
     for (int j = 0; j < cnt; ++j) {
       const double4 at = s_at[j];
-      const double dx = v[0] - at.x, dy = v[1] - at.y, dz = v[2] - at.z;
-      const double r2 = dx * dx + dy * dy + dz * dz;
-      const double ir = rsqrt(r2);
-      if (MODE < 2) {
-        sum += at.w * ir;
-      } else {
-        const double dot = dx * v[3] + dy * v[4] + dz * v[5];
-        sum += at.w * dot * (ir * ir * ir);
+#pragma unroll
+      for (int u = 0; u < kNS; ++u) {
+        const double dx = v[u][0] - at.x, dy = v[u][1] - at.y, dz = v[u][2] - at.z;
+        const double r2 = dx * dx + dy * dy + dz * dz;
+        const double ir = inv_sqrt(r2);
+        if (MODE < 2) {
+          sum[u] += at.w * ir;
+        } else {
+          const double dot = dx * v[u][3] + dy * v[u][4] + dz * v[u][5];
+          sum[u] += at.w * dot * (ir * ir * ir);
+        }
       }
     }
   }
-  if (act) acc[s] = sum * wsrc;
+#pragma unroll
+  for (int u = 0; u < kNS; ++u) {
+    const int64_t s = ((int64_t)blockIdx.x * kNS + u) * kNB + threadIdx.x;
+    if (act[u]) acc[s] = sum[u] * wsrc[u];
+  }
 }
 
 // out[0] = sum of a[0..na), out[1] = sum of b[0..nb), out[2] = sum of c[0..nc)
@@ -395,12 +415,12 @@ void run_energy(ngpb_ctx* c, ngpb_energy_out* out) {
     hipLaunchKernelGGL(k_gather_src<kVS>, dim3(blocks_for(nvv, 256)), dim3(256), 0, c->stream, nvv, vidx.p, vsrc.p, vd.p);
   if (nfv) {
     if (ionic)
-      hipLaunchKernelGGL(k_nbody<1>, dim3(blocks_for(nfv, kNB)), dim3(kNB), 0, c->stream, nfv, fd.p, nq, atoms.p, facc.p);
+      hipLaunchKernelGGL(k_nbody<1>, dim3(blocks_for(nfv, kNB * kNS)), dim3(kNB), 0, c->stream, nfv, fd.p, nq, atoms.p, facc.p);
     else
-      hipLaunchKernelGGL(k_nbody<0>, dim3(blocks_for(nfv, kNB)), dim3(kNB), 0, c->stream, nfv, fd.p, nq, atoms.p, facc.p);
+      hipLaunchKernelGGL(k_nbody<0>, dim3(blocks_for(nfv, kNB * kNS)), dim3(kNB), 0, c->stream, nfv, fd.p, nq, atoms.p, facc.p);
   }
   if (nvv)
-    hipLaunchKernelGGL(k_nbody<2>, dim3(blocks_for(nvv, kNB)), dim3(kNB), 0, c->stream, nvv, vd.p, nq, atoms.p, vacc.p);
+    hipLaunchKernelGGL(k_nbody<2>, dim3(blocks_for(nvv, kNB * kNS)), dim3(kNB), 0, c->stream, nvv, vd.p, nq, atoms.p, vacc.p);
   NGPB_HIP(hipGetLastError());
   c->en_out.resize(4);
   hipLaunchKernelGGL(k_energy_fin, dim3(1), dim3(1024), 0, c->stream, facc.p, nfv, vacc.p, nvv, leaf_charge.p, nb,
