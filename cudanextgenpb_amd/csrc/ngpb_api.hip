@@ -673,7 +673,7 @@ int ngpb_operator_info(ngpb_ctx* c, int64_t* info) {
 int ngpb_set_spmv_schedule(ngpb_ctx* c, int schedule) {
   return guarded(c, [&] {
     not_in_batch(c);
-    if (schedule < 0 || schedule > 2) throw Error(NGPB_E_ARG, "schedule must be 0, 1 or 2");
+    if (schedule < 0 || schedule > 3) throw Error(NGPB_E_ARG, "schedule must be 0, 1, 2 or 3");
     c->spmv_sched = schedule;
   });
 }

@@ -353,15 +353,13 @@ __global__ void __launch_bounds__(kBS) k_sell_spmv(Sell A, const float* __restri
 // delta slices: list entries taken two at a time; two adjacent slices with the same
 // slot count go through the paired path; the next pair's slice ids, widths and slot
 // pointers are requested before the current pair is processed
-template <bool DOT>
-__global__ void __launch_bounds__(kBS) k_sell_delta(Sell A, const int32_t* __restrict__ list, int64_t cnt,
-                                                    const float* __restrict__ x, float* __restrict__ y,
-                                                    const KrylovScalars* ks, double* __restrict__ partials) {
-  __shared__ double s_red[kBS / 64];
-  if (DOT && ks->done) return;
+// delta work of wave `wave` among `nwaves`
+__device__ __forceinline__ void delta_work(const Sell& A, const int32_t* __restrict__ list, int64_t cnt, int64_t wave,
+                                           int64_t nwaves, const float* __restrict__ x, float* __restrict__ y,
+                                           double* part_out) {
   const int64_t npairs = (cnt + 1) / 2;
-  const int64_t step = (int64_t)gridDim.x * (kBS / kW);
-  int64_t t = __builtin_amdgcn_readfirstlane((int)(blockIdx.x * (kBS / kW) + (threadIdx.x >> 6)));
+  const int64_t step = nwaves;
+  int64_t t = wave;
   double part = 0.0;
   int64_t s0n = 0, s1n = -1, b0n = 0, b1n = 0;
   int w0n = 0, w1n = 0;
@@ -406,22 +404,30 @@ __global__ void __launch_bounds__(kBS) k_sell_delta(Sell A, const int32_t* __res
     }
     t = t2;
   }
+  *part_out += part;
+}
+
+template <bool DOT>
+__global__ void __launch_bounds__(kBS) k_sell_delta(Sell A, const int32_t* __restrict__ list, int64_t cnt,
+                                                    const float* __restrict__ x, float* __restrict__ y,
+                                                    const KrylovScalars* ks, double* __restrict__ partials) {
+  __shared__ double s_red[kBS / 64];
+  if (DOT && ks->done) return;
+  double part = 0.0;
+  delta_work(A, list, cnt, __builtin_amdgcn_readfirstlane((int)(blockIdx.x * (kBS / kW) + (threadIdx.x >> 6))),
+             (int64_t)gridDim.x * (kBS / kW), x, y, &part);
   if (DOT) {
     part = dev::block_sum<kBS>(part, s_red);
     if (threadIdx.x == 0) partials[blockIdx.x] = part;
   }
 }
 
-// packed slices: one per wave visit; the next slice's 32-byte header is requested
-// before the current slice is processed
-template <bool DOT>
-__global__ void __launch_bounds__(kBS) k_sell_packed(Sell A, const int32_t* __restrict__ list, int64_t cnt,
-                                                     const float* __restrict__ x, float* __restrict__ y,
-                                                     const KrylovScalars* ks, double* __restrict__ partials) {
-  __shared__ double s_red[kBS / 64];
-  if (DOT && ks->done) return;
-  const int64_t step = (int64_t)gridDim.x * (kBS / kW);
-  int64_t i = __builtin_amdgcn_readfirstlane((int)(blockIdx.x * (kBS / kW) + (threadIdx.x >> 6)));
+// packed work of wave `wave` among `nwaves`: one slice per visit; the next slice's
+// 32-byte header is requested before the current slice is processed
+__device__ __forceinline__ void packed_work(const Sell& A, const int32_t* __restrict__ list, int64_t cnt,
+                                            int64_t wave, int64_t nwaves, const float* __restrict__ x,
+                                            float* __restrict__ y, double* part_out) {
+  int64_t i = wave;
   double part = 0.0;
   int64_t sn = i < cnt ? list[i] : 0;
   PkHdr hn;
@@ -429,7 +435,7 @@ __global__ void __launch_bounds__(kBS) k_sell_packed(Sell A, const int32_t* __re
   while (i < cnt) {
     const int64_t s = sn;
     const PkHdr h = hn;
-    const int64_t i2 = i + step;
+    const int64_t i2 = i + nwaves;
     if (i2 < cnt) {
       sn = list[i2];
       hn = A.pk_hdr[sn];
@@ -438,6 +444,75 @@ __global__ void __launch_bounds__(kBS) k_sell_packed(Sell A, const int32_t* __re
     else if (h.nd <= 2 * kW) packed_slice_h<2>(A, s, h, x, y, &part);
     else packed_slice_h<4>(A, s, h, x, y, &part);
     i = i2;
+  }
+  *part_out += part;
+}
+
+template <bool DOT>
+__global__ void __launch_bounds__(kBS) k_sell_packed(Sell A, const int32_t* __restrict__ list, int64_t cnt,
+                                                     const float* __restrict__ x, float* __restrict__ y,
+                                                     const KrylovScalars* ks, double* __restrict__ partials) {
+  __shared__ double s_red[kBS / 64];
+  if (DOT && ks->done) return;
+  double part = 0.0;
+  packed_work(A, list, cnt, __builtin_amdgcn_readfirstlane((int)(blockIdx.x * (kBS / kW) + (threadIdx.x >> 6))),
+              (int64_t)gridDim.x * (kBS / kW), x, y, &part);
+  if (DOT) {
+    part = dev::block_sum<kBS>(part, s_red);
+    if (threadIdx.x == 0) partials[blockIdx.x] = part;
+  }
+}
+
+// Delta and packed slices in one launch, interleaved within every wave: each wave
+// owns an equal share of both lists (items w, w + W, ...) and alternates one delta
+// pair with one packed slice, so that at any time the round-trip-bound packed work
+// and the streaming delta work of the CU's waves overlap, and no wave idles while
+// the other list still has work.
+template <bool DOT>
+__global__ void __launch_bounds__(kBS) k_sell_both(Sell A, const int32_t* __restrict__ dlist, int64_t dcnt,
+                                                   const int32_t* __restrict__ plist, int64_t pcnt,
+                                                   const float* __restrict__ x, float* __restrict__ y,
+                                                   const KrylovScalars* ks, double* __restrict__ partials) {
+  __shared__ double s_red[kBS / 64];
+  if (DOT && ks->done) return;
+  const int64_t W = (int64_t)gridDim.x * (kBS / kW);
+  const int64_t wave = __builtin_amdgcn_readfirstlane((int)(blockIdx.x * (kBS / kW) + (threadIdx.x >> 6)));
+  const int64_t npairs = (dcnt + 1) / 2;
+  double part = 0.0;
+  int64_t td = wave, tp = wave;
+  while (td < npairs || tp < pcnt) {
+    if (td < npairs) {
+      const int64_t s0 = dlist[2 * td];
+      const int64_t s1 = 2 * td + 1 < dcnt ? dlist[2 * td + 1] : -1;
+      const int w0 = A.width[s0];
+      const int64_t b0 = A.base_ptr[s0];
+      const int w1 = s1 >= 0 ? A.width[s1] : 0;
+      const int64_t b1 = s1 >= 0 ? A.base_ptr[s1] : 0;
+      bool done = false;
+      if (s1 == s0 + 1 && w0 == w1) {
+        switch (w0) {
+          case -4: delta_pair<4>(A, s0, b0, b1, x, y, &part); done = true; break;
+          case -5: delta_pair<5>(A, s0, b0, b1, x, y, &part); done = true; break;
+          case -6: delta_pair<6>(A, s0, b0, b1, x, y, &part); done = true; break;
+          case -7: delta_pair<7>(A, s0, b0, b1, x, y, &part); done = true; break;
+          case -8: delta_pair<8>(A, s0, b0, b1, x, y, &part); done = true; break;
+          default: break;
+        }
+      }
+      if (!done) {
+        delta_slice(A, s0, w0, b0, x, y, &part);
+        if (s1 >= 0) delta_slice(A, s1, w1, b1, x, y, &part);
+      }
+      td += W;
+    }
+    if (tp < pcnt) {
+      const int64_t s = plist[tp];
+      const PkHdr h = A.pk_hdr[s];
+      if (h.nd <= kW) packed_slice_h<1>(A, s, h, x, y, &part);
+      else if (h.nd <= 2 * kW) packed_slice_h<2>(A, s, h, x, y, &part);
+      else packed_slice_h<4>(A, s, h, x, y, &part);
+      tp += W;
+    }
   }
   if (DOT) {
     part = dev::block_sum<kBS>(part, s_red);
@@ -646,7 +721,7 @@ unsigned sell_resident(ngpb_ctx* c) {
   if (c->sell_resident > 0) return (unsigned)c->sell_resident;
   int cus = 0, per_cu = 0;
   NGPB_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, c->device));
-  NGPB_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_sell_packed<true>, kBS, 0));
+  NGPB_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_sell_both<true>, kBS, 0));
   const int r = std::max(8, (std::max(cus, 1) * std::max(per_cu, 1)) / 8 * 8);
   c->sell_resident = r;
   return (unsigned)r;
@@ -663,6 +738,8 @@ unsigned form_grid(ngpb_ctx* c, int64_t units) {
 
 // number of (p, q) partials the SpMV of the current schedule writes
 int sell_partials(ngpb_ctx* c) {
+  if (c->spmv_sched == 3)  // combined delta/packed grid, then the offset kernel's blocks
+    return (int)(std::max<int64_t>(3, sell_resident(c)) + form_grid(c, c->n_list_offset));
   if (c->spmv_sched == 1)
     return (int)std::max<int64_t>(1, std::min<int64_t>(((c->n_slices + 1) / 2 + (kBS / kW) - 1) / (kBS / kW), 2048));
   return (int)(form_grid(c, (c->n_list_delta + 1) / 2) + form_grid(c, c->n_list_packed) +
@@ -676,6 +753,16 @@ void launch_sell_t(ngpb_ctx* c, const float* x, float* y, double* part) {
   if (c->spmv_sched == 1) {
     const unsigned g = (unsigned)sell_partials(c);
     hipLaunchKernelGGL(k_sell_spmv<DOT>, dim3(g), dim3(kBS), 0, c->stream, A, x, y, ks, part);
+    return;
+  }
+  if (c->spmv_sched == 3) {
+    const unsigned g = (unsigned)std::max<int64_t>(3, sell_resident(c));
+    hipLaunchKernelGGL(k_sell_both<DOT>, dim3(g), dim3(kBS), 0, c->stream, A, c->sell_list_delta.p, c->n_list_delta,
+                       c->sell_list_packed.p, c->n_list_packed, x, y, ks, part);
+    const unsigned go = form_grid(c, c->n_list_offset);
+    if (go)
+      hipLaunchKernelGGL(k_sell_offset<DOT>, dim3(go), dim3(kBS), 0, c->stream, A, c->sell_list_offset.p,
+                         c->n_list_offset, x, y, ks, DOT ? part + g : nullptr);
     return;
   }
   const unsigned gd = form_grid(c, (c->n_list_delta + 1) / 2);

@@ -185,11 +185,15 @@ int ngpb_time_copy(ngpb_ctx* ctx, int64_t bytes, int reps, double* ms_per_launch
 int ngpb_operator_info(ngpb_ctx* ctx, int64_t* info10);
 /* Work schedule of the SpMV kernels. Compact FP32: 0 one kernel per slice form
    (delta / packed / offset, each walking a list of its slices) on up to 2048 blocks,
-   1 a single kernel for all forms, 2 (default) as 0 on the resident block count.
+   1 a single kernel for all forms, 2 as 0 on the resident block count, 3 (default) the
+   delta and packed slices in one launch on the resident grid, every wave alternating
+   between its share of the two lists so that the two forms run side by side (then
+   the offset slices, if any).
    FP64 CSR: 0 grid-stride over row chunks on up to 2048 blocks, 1 one contiguous
    chunk range per block with XCD-aware block numbering on the resident grid
-   (measured 1.36x slower), 2 grid-stride on the resident grid. Results differ only in
-   the grouping of the partial sums of (p, q). No counterpart in the reference. */
+   (measured 1.36x slower), 2 and 3 grid-stride on the resident grid. Results differ
+   only in the grouping of the partial sums of (p, q). No counterpart in the
+   reference. */
 int ngpb_set_spmv_schedule(ngpb_ctx* ctx, int schedule);
 
 /* ---- batched solve of many independent systems ------------------------------------
