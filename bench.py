@@ -362,7 +362,7 @@ def main():
             "energy_polar_kT": mixed["energy"]["polar"],
         },
         "roofline": {
-            "kernel": "compact FP32 SpMV (SELL-64, 16-bit column offsets) of the mixed inner CG",
+            "kernel": "compact FP32 SpMV of the mixed inner CG (k_sell_both: SELL-64 delta + packed slices)",
             "bound": "hbm",
             "achieved": achieved,
             "peak": HBM_PEAK_GBS,
