@@ -13,6 +13,7 @@ batched mixed-precision solve.
 from __future__ import annotations
 
 import time
+from concurrent.futures import ThreadPoolExecutor
 from dataclasses import dataclass, field
 from typing import Dict, List, Optional
 
@@ -65,19 +66,23 @@ class BatchedSolver:
     def autotune(self) -> dict:
         return self.batch.autotune()
 
-    def solve(self, energies: bool = True) -> Dict[str, dict]:
+    def solve(self, energies: bool = True, energy_threads: int = 8) -> Dict[str, dict]:
         """Runs the batch until every system is done; returns per-system stats (and
-        energies) and removes the systems from the batch."""
+        energies) and removes the systems from the batch. The energies run on the
+        systems' own streams from several host threads, so they overlap on the device."""
         self.batch.solve_all()
         out = {}
+        done_ctx = []
         for slot, p in list(self.slots.items()):
             done, st = self.batch.status(slot)
             self.batch.remove(slot)
-            res = dict(stats=st, done=done, n=p.n)
-            if energies:
-                res["energy"] = p.ctx.energy()
-            out[p.name] = res
+            out[p.name] = dict(stats=st, done=done, n=p.n)
+            done_ctx.append(p)
             del self.slots[slot]
+        if energies and done_ctx:
+            with ThreadPoolExecutor(max_workers=max(1, energy_threads)) as pool:
+                for p, e in zip(done_ctx, pool.map(lambda q: q.ctx.energy(), done_ctx)):
+                    out[p.name]["energy"] = e
         return out
 
     def close(self):

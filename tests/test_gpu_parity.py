@@ -184,3 +184,29 @@ def test_repeat_runs_bit_identical():
         pb.close()
     assert np.array_equal(out[0][0], out[1][0])
     assert out[0][1]["polar"] == out[1][1]["polar"]
+
+
+@pytest.mark.parametrize("name", ["three_spheres", "globular_small"])
+def test_spmv_schedules_agree(name):
+    """Every work schedule evaluates each row with the same instructions in the same
+    order: the compact and FP64 SpMV results are bit-identical across schedules, and
+    mixed solves (whose dot-product partials are grouped differently) agree to 1e-12."""
+    P, pb = gpu_pipeline(name)
+    info = pb.ctx.operator_info()
+    assert info["packed_slices"] > 0 and info["delta_slices"] > 0
+    x = np.random.default_rng(5).standard_normal(P.mesh.n_nodes)
+    ref32 = ref64 = None
+    energies = []
+    for sched in (0, 1, 2, 3):
+        pb.ctx.set_spmv_schedule(sched)
+        y32 = pb.ctx.spmv(x, native.PREC_MIXED)
+        y64 = pb.ctx.spmv(x, native.PREC_FP64)
+        if ref32 is None:
+            ref32, ref64 = y32, y64
+        assert np.array_equal(y32, ref32), sched
+        assert np.array_equal(y64, ref64), sched
+        pb.hip_compute_electric_potential(precision=native.PREC_MIXED)
+        assert pb.stats["status"] == 0
+        energies.append(pb.energy()["polar"])
+    assert max(abs(e - energies[0]) for e in energies) <= 1e-12 * abs(energies[0])
+    pb.close()
