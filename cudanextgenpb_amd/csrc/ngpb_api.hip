@@ -595,6 +595,20 @@ int ngpb_spmv(ngpb_ctx* c, int precision, const double* x, double* y) {
   });
 }
 
+int ngpb_phi_on_atoms(ngpb_ctx* c, double* phi_atoms) {
+  return guarded(c, [&] {
+    NGPB_REQUIRE(phi_atoms, NGPB_E_ARG, "NULL output");
+    need(c->have_phi, "no potential (solve or ngpb_set_phi first)");
+    need(c->have_charges, "no atoms (ngpb_set_charges)");
+    if (c->A == 0) return;
+    ngpb::DBuf<double> d;
+    d.resize((size_t)c->A);
+    ngpb::run_phi_on_atoms(c, d.p);
+    d.download(phi_atoms, (size_t)c->A, c->stream);
+    NGPB_HIP(hipStreamSynchronize(c->stream));
+  });
+}
+
 int ngpb_energy(ngpb_ctx* c, ngpb_energy_out* out) {
   return guarded(c, [&] {
     NGPB_REQUIRE(out, NGPB_E_ARG, "NULL output");

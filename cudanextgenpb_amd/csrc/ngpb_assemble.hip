@@ -605,6 +605,42 @@ __global__ void k_bc_values(int64_t nb, const int32_t* __restrict__ nodes, const
 
 }  // namespace
 
+// Potential at the atoms (write_potential_on_atoms_fast, src/pb_class.cpp:2333-2368):
+// trilinear interpolation inside the owner leaf with the charge-projection weights,
+// hanging corners taking the mean of their parents; NaN for atoms outside the domain.
+__global__ void k_phi_atoms(MeshView m, int64_t A, const double* __restrict__ axyz, const int32_t* __restrict__ owner,
+                            const double* __restrict__ phi, double* __restrict__ out) {
+  const int64_t a = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (a >= A) return;
+  const int64_t l = owner[a];
+  if (l < 0) {
+    out[a] = __builtin_nan("");
+    return;
+  }
+  const int32_t c0 = m.leaf_corner[8 * l], c7 = m.leaf_corner[8 * l + 7];
+  const double volume = (dev::corner_coord(m, c7, 0) - dev::corner_coord(m, c0, 0)) *
+                        (dev::corner_coord(m, c7, 1) - dev::corner_coord(m, c0, 1)) *
+                        (dev::corner_coord(m, c7, 2) - dev::corner_coord(m, c0, 2));
+  double acc = 0.0;
+  for (int ii = 0; ii < 8; ++ii) {
+    const int32_t co = m.leaf_corner[8 * l + 7 - ii];
+    const double weight = fabs((axyz[3 * a + 0] - dev::corner_coord(m, co, 0)) *
+                               (axyz[3 * a + 1] - dev::corner_coord(m, co, 1)) *
+                               (axyz[3 * a + 2] - dev::corner_coord(m, co, 2))) / volume;
+    const int32_t c = m.leaf_corner[8 * l + ii];
+    if (c >= 0) {
+      acc += phi[c] * weight;
+    } else {
+      const int64_t h = -1 - (int64_t)c;
+      const int np = m.hang_np[h];
+      double ph = 0.0;
+      for (int j = 0; j < np; ++j) ph += phi[m.hang_par[4 * h + j]] / np;
+      acc += ph * weight;
+    }
+  }
+  out[a] = acc;
+}
+
 void run_symbolic(ngpb_ctx* c) {
   const MeshView m = mesh_view_of(c);
   NGPB_REQUIRE(20 * m.L < (int64_t)UINT32_MAX, NGPB_E_ARG, "too many leaves for 32-bit emission ids");
@@ -704,6 +740,14 @@ void run_numeric(ngpb_ctx* c) {
   NGPB_HIP(hipStreamSynchronize(c->stream));
   c->have_system = true;
   c->have_compact = false;
+}
+
+void run_phi_on_atoms(ngpb_ctx* c, double* d_out) {
+  const MeshView m = mesh_view_of(c);
+  if (c->A > 0)
+    hipLaunchKernelGGL(k_phi_atoms, dim3(blocks_for(c->A, 256)), dim3(256), 0, c->stream, m, c->A, c->atom_xyz.p,
+                       c->atom_owner.p, c->x.p, d_out);
+  NGPB_HIP(hipGetLastError());
 }
 
 void run_dirichlet_values(ngpb_ctx* c, int bc_type) {

@@ -269,6 +269,7 @@ void run_fractions(ngpb_ctx* c);
 void run_symbolic(ngpb_ctx* c);
 void run_numeric(ngpb_ctx* c);
 void run_dirichlet_values(ngpb_ctx* c, int bc_type);
+void run_phi_on_atoms(ngpb_ctx* c, double* d_out);
 
 // Krylov
 unsigned vec_grid(int64_t n);

@@ -87,6 +87,7 @@ SIGNATURES = {
     "ngpb_set_phi": (_I, [_P, _P]),
     "ngpb_spmv": (_I, [_P, _I, _P, _P]),
     "ngpb_energy": (_I, [_P, _P]),
+    "ngpb_phi_on_atoms": (_I, [_P, _P]),
     "ngpb_time_spmv": (_I, [_P, _I, _I, _P, _P]),
     "ngpb_set_profiling": (_I, [_P, _I]),
     "ngpb_kernel_stats": (_I, [_P, _I, _P, _P, _P]),
@@ -289,6 +290,11 @@ class Context:
         y = np.zeros(self.n)
         self._check(lib().ngpb_spmv(self._h, precision, _p(_c(x, np.float64)), _p(y)), "spmv")
         return y
+
+    def phi_on_atoms(self, n_atoms: int):
+        out = np.zeros(n_atoms)
+        self._check(lib().ngpb_phi_on_atoms(self._h, _p(out)), "phi_on_atoms")
+        return out
 
     def energy(self):
         e = EnergyOut()

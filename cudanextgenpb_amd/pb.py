@@ -65,6 +65,22 @@ class PoissonBoltzmann:
         self.phi, self.stats = self.ctx.solve(prec, o.tol, o.maxiter, drop, check_every)
         return self.phi
 
+    # write_potential_on_atoms_fast (src/pb_class.cpp:2333-2417): phi interpolated at
+    # the atoms, written as phi_on_atoms.txt lines "index x y z phi" with the
+    # reference's field widths (8, 8.3f x3, two spaces, .4f); atoms outside the
+    # domain (no owner leaf) are skipped, as the reference's lookup table skips them.
+    def write_potential_on_atoms(self, path: Optional[str] = "phi_on_atoms.txt") -> np.ndarray:
+        a = self.problem.atoms
+        phi = self.ctx.phi_on_atoms(a.n)
+        if path is not None:
+            with open(path, "w") as f:
+                for i in range(a.n):
+                    if self.problem.owner[i] < 0:
+                        continue
+                    x, y, z = a.xyz[i]
+                    f.write(f"{i + 1:8d}{x:8.3f}{y:8.3f}{z:8.3f}  {phi[i]:.4f}\n")
+        return phi
+
     # energy / energy_fast (src/pb_class.cpp:2790-3338)
     def energy(self):
         self.energies = self.ctx.energy()

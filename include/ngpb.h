@@ -28,6 +28,7 @@
  *                         (include/pb_class.h:740-741, src/pb_class.cpp:2243-2329)
  *   ngpb_energy           poisson_boltzmann::energy / energy_fast
  *                         (include/pb_class.h:758-762, src/pb_class.cpp:2790-3338)
+ *   ngpb_phi_on_atoms     write_potential_on_atoms_fast (src/pb_class.cpp:2333-2368)
  */
 #ifndef NGPB_H
 #define NGPB_H
@@ -159,6 +160,11 @@ int ngpb_set_phi(ngpb_ctx* ctx, const double* phi);
    y = D^1/2 Ahat D^1/2 x evaluated in the compact format). */
 int ngpb_spmv(ngpb_ctx* ctx, int precision, const double* x, double* y);
 int ngpb_energy(ngpb_ctx* ctx, ngpb_energy_out* out);
+/* Potential at each atom (n_atoms of ngpb_set_charges), interpolated trilinearly in
+   its owner leaf; NaN for atoms outside the domain. Replaces the computation of
+   write_potential_on_atoms_fast (src/pb_class.cpp:2333-2368); the phi_on_atoms.txt
+   writer is cudanextgenpb_amd.pb.PoissonBoltzmann.write_potential_on_atoms. */
+int ngpb_phi_on_atoms(ngpb_ctx* ctx, double* phi_atoms);
 
 /* ---- measurement ----------------------------------------------------------------- */
 /* Times `reps` back-to-back launches of the SpMV kernel of the given precision on the

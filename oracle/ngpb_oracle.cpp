@@ -14,6 +14,7 @@
 //                      bim3a_reaction_frac / bim3a_dirichlet_bc (conventions below)
 //   orc_pcg            src/pb_class.cpp:2243-2329 + LIS 2.1.6 CG with Jacobi (-i cg -p jacobi)
 //   orc_energy         src/pb_class.cpp:2534-2788 (helpers), 2790-3063 (energy)
+//   orc_phi_on_atoms   src/pb_class.cpp:2333-2368 (write_potential_on_atoms_fast)
 //
 // Pinning (how each is checked without the reference's libraries, which are absent):
 //   constants  closed forms; markers: brute-force distance test on sphere unions +
@@ -21,7 +22,8 @@
 //   force; rhs: weights sum to one, sum rhs = 4 pi sum q; assembly: uniform 7-point
 //   closed form, symmetry, zero interior row sums at kappa=0, exact reproduction of
 //   linear fields; pcg: dense direct solves of small systems; energy: Born-sphere
-//   analytic energy and the Gauss flux-charge check.
+//   analytic energy and the Gauss flux-charge check; phi on atoms: exact
+//   reproduction of linear fields (trilinear interpolation).
 // The bimpp conventions (hanging-node algebra, reaction lumping, Dirichlet
 // elimination) cannot be pinned against bimpp itself, which is not available:
 // "parity unpinned" against bimpp; pinned by the invariants above.
@@ -275,6 +277,39 @@ int orc_locate(const orc_mesh* m, int64_t na, const double* xyz, double tol, int
 // rho) (src/pb_class.cpp:2085). bim3a_rhs assembles sum over leaves of
 // f_leaf * g(node) * V/8 per corner, a hanging corner contributing V/(8 np) to each
 // parent with the parent's value; vol_patch is the same sum with g = 1.
+// write_potential_on_atoms_fast (src/pb_class.cpp:2333-2368): trilinear interpolation
+// of phi inside each atom's owner leaf with the charge-projection weights; a hanging
+// corner takes the mean of its parents (sum of phi/np). Atoms outside the domain
+// (owner -1) get NaN.
+int orc_phi_on_atoms(const orc_mesh* m, int64_t na, const double* xyz, const int32_t* owner,
+                     const double* phi, double* out) {
+  MeshView mv{m};
+  for (int64_t a = 0; a < na; ++a) {
+    const int64_t l = owner[a];
+    if (l < 0) {
+      out[a] = std::nan("");
+      continue;
+    }
+    const double volume = (mv.p(l, 0, 7) - mv.p(l, 0, 0)) * (mv.p(l, 1, 7) - mv.p(l, 1, 0)) *
+                          (mv.p(l, 2, 7) - mv.p(l, 2, 0));
+    double phi_on_atom = 0.0;
+    for (int ii = 0; ii < 8; ++ii) {
+      const double weight = std::abs((xyz[3 * a + 0] - mv.p(l, 0, 7 - ii)) *
+                                     (xyz[3 * a + 1] - mv.p(l, 1, 7 - ii)) *
+                                     (xyz[3 * a + 2] - mv.p(l, 2, 7 - ii))) / volume;
+      if (!mv.hanging(l, ii)) {
+        phi_on_atom += phi[mv.gt(l, ii)] * weight;
+      } else {
+        double ph = 0.0;
+        for (int jj = 0; jj < mv.num_parents(l, ii); ++jj) ph += phi[mv.gparent(l, jj, ii)] / mv.num_parents(l, ii);
+        phi_on_atom += ph * weight;
+      }
+    }
+    out[a] = phi_on_atom;
+  }
+  return 0;
+}
+
 int orc_rhs(const orc_mesh* m, int64_t na, const double* xyz, const double* q,
             const int32_t* owner, double* rhs) {
   MeshView mv{m};

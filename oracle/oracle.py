@@ -42,6 +42,7 @@ def lib():
         L.orc_fractions.argtypes = [_P, _P, _P, _P, _P, _P]
         L.orc_locate.argtypes = [_P, _I64, _P, _D, _P]
         L.orc_rhs.argtypes = [_P, _I64, _P, _P, _P, _P]
+        L.orc_phi_on_atoms.argtypes = [_P, _I64, _P, _P, _P, _P]
         L.orc_assemble.argtypes = [_P, _P, _P, _P, _I64, _P, _P, _P]
         L.orc_assemble.restype = _P
         L.orc_sys_sizes.argtypes = [_P, _P, _P]
@@ -147,6 +148,16 @@ def rhs(problem, owner=None):
     own = _c(problem.owner if owner is None else owner, np.int32)
     out = np.zeros(problem.mesh.n_nodes)
     lib().orc_rhs(m.ptr, problem.atoms.n, _p(xyz), _p(q), _p(own), _p(out))
+    return out
+
+
+def phi_on_atoms(problem, phi, owner=None):
+    """Potential interpolated at the atoms (write_potential_on_atoms_fast)."""
+    m = _MeshArgs(problem.mesh)
+    xyz = _c(problem.atoms.xyz, np.float64)
+    own = _c(problem.owner if owner is None else owner, np.int32)
+    out = np.zeros(problem.atoms.n)
+    lib().orc_phi_on_atoms(m.ptr, problem.atoms.n, _p(xyz), _p(own), _p(_c(phi, np.float64)), _p(out))
     return out
 
 

@@ -140,6 +140,24 @@ def test_energy_matches_oracle(name):
     pb.close()
 
 
+@pytest.mark.parametrize("name", ["three_spheres", "globular_small"])
+def test_phi_on_atoms_bitexact(name, tmp_path):
+    P = problem(name)
+    R = reference(name)
+    bv = O.dirichlet_values(P)
+    _, pb = gpu_pipeline(name, bnd_values=bv)
+    pb.ctx.set_phi(R["phi"])
+    path = tmp_path / "phi_on_atoms.txt"
+    got = pb.write_potential_on_atoms(str(path))
+    want = O.phi_on_atoms(P, R["phi"])
+    assert np.array_equal(got, want, equal_nan=True)
+    lines = path.read_text().splitlines()
+    assert len(lines) == int((P.owner >= 0).sum())
+    idx, x, y, z, v = lines[0].split()
+    assert int(idx) == 1 and float(v) == pytest.approx(want[0], abs=6e-5)
+    pb.close()
+
+
 def test_born_energy_closed_form():
     P, pb = gpu_pipeline("born_h05")
     pb.hip_compute_electric_potential()

@@ -7,6 +7,7 @@ through hanging-node constraints, charge conservation, Gauss's law), brute force
 (distance test for in/out, exhaustive leaf search for atom location) and an
 independent direct solver (committed golden vectors).
 """
+import dataclasses
 import os
 
 import numpy as np
@@ -148,6 +149,47 @@ def test_rhs_conserves_charge_and_weights():
     r = O.rhs(P)
     assert r.sum() == pytest.approx(4 * np.pi, rel=1e-13)
     assert r.max() == pytest.approx(4 * np.pi, rel=1e-12)
+
+
+def test_phi_on_atoms_reproduces_linear_fields():
+    # trilinear interpolation in the owner leaf is exact for a linear potential, also
+    # through hanging corners (the mean of the parents is the midpoint value)
+    P = W.globular(n_atoms=60, ball_radius=6.0, min_sep=2.0, h_fine=0.5, levels=2, fine_band=1.5, mid_band=4.0)
+    m = P.mesh
+    xyz = m.node_xyz
+    phi = 0.3 * xyz[:, 0] - 1.1 * xyz[:, 1] + 0.7 * xyz[:, 2] + 2.0
+    got = O.phi_on_atoms(P, phi)
+    a = P.atoms.xyz
+    want = 0.3 * a[:, 0] - 1.1 * a[:, 1] + 0.7 * a[:, 2] + 2.0
+    assert np.abs(got - want).max() <= 1e-12 * np.abs(want).max()
+    # points inside leaves with hanging corners (the atoms themselves sit in the
+    # finest, regular leaves)
+    hl = np.nonzero((m.leaf_corner < 0).any(axis=1))[0][:200]
+    assert hl.size > 50
+    def corner_xyz(c):
+        return np.where((c >= 0)[:, None], xyz[np.maximum(c, 0)], m.hang_xyz[np.maximum(-1 - c, 0)])
+
+    lo, hi = corner_xyz(m.leaf_corner[hl, 0]), corner_xyz(m.leaf_corner[hl, 7])
+    pts = lo + np.random.default_rng(1).uniform(0.05, 0.95, lo.shape) * (hi - lo)
+    Q = dataclasses.replace(P, atoms=W.Atoms(pts, np.ones(hl.size), np.zeros(hl.size)), owner=hl.astype(np.int32))
+    gotq = O.phi_on_atoms(Q, phi)
+    wantq = 0.3 * pts[:, 0] - 1.1 * pts[:, 1] + 0.7 * pts[:, 2] + 2.0
+    assert np.abs(gotq - wantq).max() <= 1e-12 * np.abs(wantq).max()
+    # a bilinear term xy is reproduced too in leaves without hanging corners
+    phi2 = xyz[:, 0] * xyz[:, 1]
+    got2 = O.phi_on_atoms(P, phi2)
+    regular = (m.leaf_corner[P.owner] >= 0).all(axis=1)
+    assert np.abs(got2 - a[:, 0] * a[:, 1])[regular].max() <= 1e-12 * np.abs(a[:, 0] * a[:, 1]).max()
+    # an atom on a regular corner takes that node's value; no owner gives NaN
+    own = P.owner.copy()
+    node = m.leaf_corner[own[0], 0]
+    assert node >= 0
+    P.atoms.xyz[0] = xyz[node]
+    rnd = np.random.default_rng(3).standard_normal(m.n_nodes)
+    own[1] = -1
+    got3 = O.phi_on_atoms(P, rnd, owner=own)
+    assert got3[0] == pytest.approx(rnd[node], abs=1e-14)
+    assert np.isnan(got3[1]) and not np.isnan(got3[2:]).any()
 
 
 # ---------------------------------------------------------------- assembly (a6, a7, a9)
