@@ -619,6 +619,31 @@ int ngpb_energy(ngpb_ctx* c, ngpb_energy_out* out) {
   });
 }
 
+int ngpb_pot_field(ngpb_ctx* c, int calc_potential_term, int calc_field_term, double* table,
+                   ngpb_pot_field_out* out) {
+  return guarded(c, [&] {
+    NGPB_REQUIRE(out, NGPB_E_ARG, "NULL output");
+    NGPB_REQUIRE(calc_potential_term >= 0 && calc_potential_term <= 2 && calc_field_term >= 0 &&
+                     calc_field_term <= 2,
+                 NGPB_E_ARG, "calc_potential_term and calc_field_term must be 0, 1 or 2");
+    need(c->have_charges, "no atoms (ngpb_set_charges)");
+    std::memset(out, 0, sizeof(*out));
+    out->branch = ngpb::pot_field_branch(calc_potential_term, calc_field_term, c->pc.k);
+    if (out->branch != 0) {
+      need(c->have_markers, "markers are not available");
+      need(c->have_phi, "no potential (solve or ngpb_set_phi first)");
+      if (!c->have_frac) ngpb::run_fractions(c);
+    }
+    double e[4];
+    int br = 0;
+    ngpb::run_pot_field(c, calc_potential_term, calc_field_term, table, e, &br);
+    out->energy_pol = e[0];
+    out->energy_react = e[1];
+    out->coul_energy = e[2];
+    out->charge_pol = e[3];
+  });
+}
+
 int ngpb_time_spmv(ngpb_ctx* c, int precision, int reps, double* ms, double* bytes) {
   return guarded(c, [&] {
     not_in_batch(c);
@@ -687,7 +712,7 @@ int ngpb_operator_info(ngpb_ctx* c, int64_t* info) {
 int ngpb_set_spmv_schedule(ngpb_ctx* c, int schedule) {
   return guarded(c, [&] {
     not_in_batch(c);
-    if (schedule < 0 || schedule > 3) throw Error(NGPB_E_ARG, "schedule must be 0, 1, 2 or 3");
+    if (schedule < 0 || schedule > 4) throw Error(NGPB_E_ARG, "schedule must be 0, 1, 2, 3 or 4");
     c->spmv_sched = schedule;
   });
 }

@@ -1,6 +1,7 @@
 // Polarization and direct ionic energy (poisson_boltzmann::energy,
 // src/pb_class.cpp:2790-3063; energy_fast :3065-3338 is the same on meshes without
-// hanging nodes).
+// hanging nodes), and the per-atom potentials and fields of pot_field / pot_field_fast
+// (:3731-5060), which sum the same sources per atom.
 //
 //   k_sources     one thread per border leaf: corner eps/phi (hanging corners: mean of
 //                 parents), flux edges (eps(i1) != eps(i2)) with their crossing point
@@ -339,6 +340,332 @@ void upload_tri_table(hipStream_t s) {
 
 }  // namespace
 
+namespace {
+
+// Flux and triangle-vertex sources of the border leaves in the reference's sweep
+// order, compacted: fd[4 i] = x y z flux, vd[8 i] = x y z nx ny nz phi_sup area.
+struct Sources {
+  DBuf<double> fd, vd, leaf_charge;
+  DBuf<int64_t> counts;  // [0] flux edges, [1] triangles
+  int64_t nf = 0, nv = 0, nb = 0;
+};
+
+void build_sources(ngpb_ctx* c, bool ionic, Sources& S) {
+  const int64_t nb = c->n_border;
+  S.nb = nb;
+  S.counts.resize(2);
+  NGPB_HIP(hipMemsetAsync(S.counts.p, 0, 2 * sizeof(int64_t), c->stream));
+  S.leaf_charge.resize((size_t)std::max<int64_t>(nb, 1));
+  if (nb == 0) return;
+  upload_tri_table(c->stream);
+  const int64_t nf = kFluxSlots * nb, nv = kTriSlots * nb;
+  DBuf<double> fsrc, vsrc, leaf_last, carry;
+  DBuf<uint8_t> fvalid, vvalid, vneed, leaf_has;
+  fsrc.resize((size_t)kFS * nf);
+  vsrc.resize((size_t)kVS * nv);
+  fvalid.resize((size_t)nf);
+  vvalid.resize((size_t)nv);
+  vneed.resize((size_t)nv);
+  leaf_last.resize((size_t)3 * nb);
+  leaf_has.resize((size_t)nb);
+  carry.resize((size_t)3 * nb);
+  NGPB_HIP(hipMemsetAsync(vneed.p, 0, nv, c->stream));
+  EnergyArgs ea;
+  ea.m = mesh_view_of(c);
+  ea.nb = nb;
+  ea.border = c->border.p;
+  ea.eps = c->eps.p;
+  ea.phi = c->x.p;
+  ea.frac = c->frac.p;
+  ea.nrm = c->frac_normal.p;
+  ea.found = c->frac_found.p;
+  ea.eps_out = c->pc.eps_out;
+  ea.ionic = ionic ? 1 : 0;
+  hipLaunchKernelGGL(k_sources, dim3(blocks_for(nb, 64)), dim3(64), 0, c->stream, ea, fsrc.p, fvalid.p, vsrc.p, vvalid.p,
+                     vneed.p, S.leaf_charge.p, leaf_last.p, leaf_has.p, S.counts.p);
+  NGPB_HIP(hipGetLastError());
+  if (ionic) {
+    hipLaunchKernelGGL(k_normal_carry, dim3(1), dim3(1024), 0, c->stream, nb, leaf_has.p, leaf_last.p, carry.p);
+    hipLaunchKernelGGL(k_apply_carry, dim3(blocks_for(nv, 256)), dim3(256), 0, c->stream, nv, vvalid.p, vneed.p, carry.p,
+                       vsrc.p);
+    NGPB_HIP(hipGetLastError());
+  }
+  // compact valid sources (slot order is the reference's summation order)
+  DBuf<int32_t> fidx, vidx;
+  fidx.resize((size_t)nf);
+  vidx.resize((size_t)nv);
+  S.nf = compact_flags(c, fvalid.p, nf, fidx.p);
+  S.nv = ionic ? compact_flags(c, vvalid.p, nv, vidx.p) : 0;
+  S.fd.resize((size_t)kFS * std::max<int64_t>(S.nf, 1));
+  S.vd.resize((size_t)kVS * std::max<int64_t>(S.nv, 1));
+  if (S.nf)
+    hipLaunchKernelGGL(k_gather_src<kFS>, dim3(blocks_for(S.nf, 256)), dim3(256), 0, c->stream, S.nf, fidx.p, fsrc.p,
+                       S.fd.p);
+  if (S.nv)
+    hipLaunchKernelGGL(k_gather_src<kVS>, dim3(blocks_for(S.nv, 256)), dim3(256), 0, c->stream, S.nv, vidx.p, vsrc.p,
+                       S.vd.p);
+  NGPB_HIP(hipGetLastError());
+}
+
+// ---- potentials and fields at the atoms (pot_field / pot_field_fast) ----------------
+// Each block takes kNB atoms (one per thread) and one chunk of sources (blockIdx.y),
+// staged through LDS in source order; per-atom partial sums of the chunk go to
+// part[(chunk * A + atom) * 4 + k] and are added over chunks in chunk order, so the
+// result does not depend on scheduling.
+
+// flux sources: S = sum flux / r, G = sum (a - V) flux / r^3
+template <bool FIELD>
+__global__ void __launch_bounds__(kNB) k_pf_flux(int64_t ns, const double* __restrict__ src, int64_t chunk, int64_t A,
+                                                 const double* __restrict__ axyz, double* __restrict__ part) {
+  __shared__ double4 s_src[kNB];
+  const int64_t a = (int64_t)blockIdx.x * kNB + threadIdx.x;
+  const bool act = a < A;
+  const double ax = act ? axyz[3 * a] : 0.0, ay = act ? axyz[3 * a + 1] : 0.0, az = act ? axyz[3 * a + 2] : 0.0;
+  const int64_t s0 = (int64_t)blockIdx.y * chunk, s1 = min(ns, s0 + chunk);
+  double S = 0.0, gx = 0.0, gy = 0.0, gz = 0.0;
+  for (int64_t t0 = s0; t0 < s1; t0 += kNB) {
+    __syncthreads();
+    if (t0 + threadIdx.x < s1) {
+      const double* v = src + kFS * (t0 + threadIdx.x);
+      s_src[threadIdx.x] = make_double4(v[0], v[1], v[2], v[3]);
+    }
+    __syncthreads();
+    const int cnt = (int)min((int64_t)kNB, s1 - t0);
+    for (int j = 0; j < cnt; ++j) {
+      const double4 v = s_src[j];
+      const double dx = ax - v.x, dy = ay - v.y, dz = az - v.z;
+      const double ir = inv_sqrt(dx * dx + dy * dy + dz * dz);
+      S += v.w * ir;
+      if (FIELD) {
+        const double w = v.w * (ir * ir * ir);
+        gx += dx * w;
+        gy += dy * w;
+        gz += dz * w;
+      }
+    }
+  }
+  if (act) {
+    double* o = part + ((int64_t)blockIdx.y * A + a) * 4;
+    o[0] = S;
+    o[1] = gx;
+    o[2] = gy;
+    o[3] = gz;
+  }
+}
+
+// triangle-vertex sources (weight w = phi_sup area / 3 / 4 pi, d = V - a):
+// T = sum w (d . n) / r^3, H = sum w (n / r^3 - 3 d (d . n) / r^5)
+template <bool FIELD>
+__global__ void __launch_bounds__(kNB) k_pf_vert(int64_t ns, const double* __restrict__ src, int64_t chunk, int64_t A,
+                                                 const double* __restrict__ axyz, double* __restrict__ part) {
+  __shared__ double4 s_p[kNB];
+  __shared__ double4 s_n[kNB];
+  const double inv_4pi = 1.0 / (4.0 * kPi);
+  const int64_t a = (int64_t)blockIdx.x * kNB + threadIdx.x;
+  const bool act = a < A;
+  const double ax = act ? axyz[3 * a] : 0.0, ay = act ? axyz[3 * a + 1] : 0.0, az = act ? axyz[3 * a + 2] : 0.0;
+  const int64_t s0 = (int64_t)blockIdx.y * chunk, s1 = min(ns, s0 + chunk);
+  double T = 0.0, hx = 0.0, hy = 0.0, hz = 0.0;
+  for (int64_t t0 = s0; t0 < s1; t0 += kNB) {
+    __syncthreads();
+    if (t0 + threadIdx.x < s1) {
+      const double* v = src + kVS * (t0 + threadIdx.x);
+      s_p[threadIdx.x] = make_double4(v[0], v[1], v[2], v[6] * inv_4pi * v[7] / 3.0);
+      s_n[threadIdx.x] = make_double4(v[3], v[4], v[5], 0.0);
+    }
+    __syncthreads();
+    const int cnt = (int)min((int64_t)kNB, s1 - t0);
+    for (int j = 0; j < cnt; ++j) {
+      const double4 v = s_p[j];
+      const double4 nv = s_n[j];
+      const double dx = v.x - ax, dy = v.y - ay, dz = v.z - az;
+      const double r2 = dx * dx + dy * dy + dz * dz;
+      const double ir = inv_sqrt(r2);
+      const double ir3 = ir * ir * ir;
+      const double dot = dx * nv.x + dy * nv.y + dz * nv.z;
+      T += v.w * dot * ir3;
+      if (FIELD) {
+        const double t5 = 3.0 * dot * ir3 * ir * ir;
+        hx += v.w * (nv.x * ir3 - dx * t5);
+        hy += v.w * (nv.y * ir3 - dy * t5);
+        hz += v.w * (nv.z * ir3 - dz * t5);
+      }
+    }
+  }
+  if (act) {
+    double* o = part + ((int64_t)blockIdx.y * A + a) * 4;
+    o[0] = T;
+    o[1] = hx;
+    o[2] = hy;
+    o[3] = hz;
+  }
+}
+
+// Coulomb terms of atom i over every other atom j in index order (src/pb_class.cpp:
+// 3776-3813 accumulates phi_c[i] over j < i first, then j > i, which is the same
+// order), with the reference's sqrt and divisions; ce[i] = its share of the Coulomb
+// energy (pairs j > i).
+__global__ void __launch_bounds__(kNB) k_pf_coulomb(int64_t A, const double* __restrict__ axyz,
+                                                    const double* __restrict__ q, double den_in,
+                                                    double* __restrict__ table, double* __restrict__ ce) {
+  __shared__ double4 s_at[kNB];
+  const int64_t i = (int64_t)blockIdx.x * kNB + threadIdx.x;
+  const bool act = i < A;
+  const double xi = act ? axyz[3 * i] : 0.0, yi = act ? axyz[3 * i + 1] : 0.0, zi = act ? axyz[3 * i + 2] : 0.0;
+  const double qi = act ? q[i] : 0.0;
+  double ph = 0.0, fx = 0.0, fy = 0.0, fz = 0.0, e = 0.0;
+  for (int64_t t0 = 0; t0 < A; t0 += kNB) {
+    __syncthreads();
+    if (t0 + threadIdx.x < A) {
+      const int64_t j = t0 + threadIdx.x;
+      s_at[threadIdx.x] = make_double4(axyz[3 * j], axyz[3 * j + 1], axyz[3 * j + 2], q[j]);
+    }
+    __syncthreads();
+    const int cnt = (int)min((int64_t)kNB, A - t0);
+    for (int jj = 0; jj < cnt; ++jj) {
+      const int64_t j = t0 + jj;
+      if (j == i) continue;
+      const double4 at = s_at[jj];
+      const double dx = xi - at.x, dy = yi - at.y, dz = zi - at.z;
+      const double r2 = dx * dx + dy * dy + dz * dz;
+      const double r = sqrt(r2);
+      const double inv_r3 = 1.0 / (r2 * r);
+      ph += at.w / r * den_in;
+      fx += dx * inv_r3 * at.w * den_in;
+      fy += dy * inv_r3 * at.w * den_in;
+      fz += dz * inv_r3 * at.w * den_in;
+      if (j > i) e += (qi * at.w) / r * den_in;
+    }
+  }
+  if (act) {
+    double* row = table + 12 * i;
+    row[0] = ph;
+    row[3] = fx;
+    row[4] = fy;
+    row[5] = fz;
+    ce[i] = e;
+  }
+}
+
+// per atom: chunk sums, constants, and the write-out corrections of the ionic columns
+__global__ void k_pf_finish(int64_t A, int64_t nfc, const double* __restrict__ fpart, int64_t nvc,
+                            const double* __restrict__ vpart, int fieldp, int ionic, int fieldi, double cp, double cr,
+                            const double* __restrict__ q, double* __restrict__ table, double* __restrict__ e1,
+                            double* __restrict__ e2) {
+  const int64_t a = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (a >= A) return;
+  double S = 0.0, G[3] = {0.0, 0.0, 0.0}, T = 0.0, H[3] = {0.0, 0.0, 0.0};
+  for (int64_t k = 0; k < nfc; ++k) {
+    const double* o = fpart + (k * A + a) * 4;
+    S += o[0];
+    for (int d = 0; d < 3; ++d) G[d] += o[1 + d];
+  }
+  for (int64_t k = 0; k < nvc; ++k) {
+    const double* o = vpart + (k * A + a) * 4;
+    T += o[0];
+    for (int d = 0; d < 3; ++d) H[d] += o[1 + d];
+  }
+  double* row = table + 12 * a;
+  const double phi_p = S * cp;
+  row[1] = phi_p;
+  row[2] = ionic ? T - phi_p / cp * cr : 0.0;
+  for (int d = 0; d < 3; ++d) {
+    const double fp = G[d] * cp;
+    row[6 + d] = fieldp ? fp : 0.0;
+    row[9 + d] = fieldi ? H[d] - fp / cp * cr : 0.0;
+  }
+  e1[a] = q[a] * S;
+  e2[a] = q[a] * T;
+}
+
+// source chunks so that atom tiles x chunks is about 2048 blocks
+int64_t pf_chunks(int64_t A, int64_t ns) {
+  if (ns <= 0) return 0;
+  const int64_t tiles = (A + kNB - 1) / kNB;
+  const int64_t want = std::max<int64_t>(1, 2048 / std::max<int64_t>(tiles, 1));
+  return std::min<int64_t>(want, (ns + kNB - 1) / kNB);
+}
+
+}  // namespace
+
+int pot_field_branch(int P, int F, double k) {
+  if ((F == 1 || (F == 2 && k < 1.e-5)) && P < 2) return 1;
+  if ((P == 1 || (P == 2 && k < 1.e-5)) && F == 0) return 2;
+  if (F == 2 && k > 1.e-5) return 3;
+  if (P == 2 && k > 1.e-5 && F == 0) return 4;
+  if (P == 2 && k > 1.e-5 && F == 1) return 5;
+  return 0;
+}
+
+void run_pot_field(ngpb_ctx* c, int P, int F, double* table_host, double out4[4], int* branch_out) {
+  const int64_t A = c->A;
+  const int branch = pot_field_branch(P, F, c->pc.k);
+  *branch_out = branch;
+  const bool fieldp = branch == 1 || branch == 3 || branch == 5;
+  const bool ionic = branch >= 3;
+  const bool fieldi = branch == 3;
+  out4[0] = out4[1] = out4[2] = out4[3] = 0.0;
+  if (A == 0) return;
+  DBuf<double> table, ce, e1, e2, fin;
+  table.resize((size_t)12 * A);
+  ce.resize((size_t)A);
+  e1.resize((size_t)A);
+  e2.resize((size_t)A);
+  fin.resize(8);
+  NGPB_HIP(hipMemsetAsync(table.p, 0, sizeof(double) * 12 * A, c->stream));
+  NGPB_HIP(hipMemsetAsync(e1.p, 0, sizeof(double) * A, c->stream));
+  NGPB_HIP(hipMemsetAsync(e2.p, 0, sizeof(double) * A, c->stream));
+  const unsigned tiles = (unsigned)((A + kNB - 1) / kNB);
+  hipLaunchKernelGGL(k_pf_coulomb, dim3(tiles), dim3(kNB), 0, c->stream, A, c->atom_xyz.p, c->atom_q.p,
+                     1.0 / c->pc.eps_in, table.p, ce.p);
+  NGPB_HIP(hipGetLastError());
+  Sources S;
+  if (branch != 0) {
+    build_sources(c, ionic, S);
+    const int64_t nfc = pf_chunks(A, S.nf), nvc = pf_chunks(A, S.nv);
+    DBuf<double> fpart, vpart;
+    fpart.resize((size_t)4 * A * std::max<int64_t>(nfc, 1));
+    vpart.resize((size_t)4 * A * std::max<int64_t>(nvc, 1));
+    if (nfc) {
+      const int64_t chunk = (S.nf + nfc - 1) / nfc;
+      if (fieldp)
+        hipLaunchKernelGGL(k_pf_flux<true>, dim3(tiles, (unsigned)nfc), dim3(kNB), 0, c->stream, S.nf, S.fd.p, chunk, A,
+                           c->atom_xyz.p, fpart.p);
+      else
+        hipLaunchKernelGGL(k_pf_flux<false>, dim3(tiles, (unsigned)nfc), dim3(kNB), 0, c->stream, S.nf, S.fd.p, chunk, A,
+                           c->atom_xyz.p, fpart.p);
+    }
+    if (nvc) {
+      const int64_t chunk = (S.nv + nvc - 1) / nvc;
+      if (fieldi)
+        hipLaunchKernelGGL(k_pf_vert<true>, dim3(tiles, (unsigned)nvc), dim3(kNB), 0, c->stream, S.nv, S.vd.p, chunk, A,
+                           c->atom_xyz.p, vpart.p);
+      else
+        hipLaunchKernelGGL(k_pf_vert<false>, dim3(tiles, (unsigned)nvc), dim3(kNB), 0, c->stream, S.nv, S.vd.p, chunk, A,
+                           c->atom_xyz.p, vpart.p);
+    }
+    NGPB_HIP(hipGetLastError());
+    hipLaunchKernelGGL(k_pf_finish, dim3(blocks_for(A, 256)), dim3(256), 0, c->stream, A, nfc, fpart.p, nvc, vpart.p,
+                       fieldp ? 1 : 0, ionic ? 1 : 0, fieldi ? 1 : 0, c->pc.constant_pol, c->pc.constant_react,
+                       c->atom_q.p, table.p, e1.p, e2.p);
+    NGPB_HIP(hipGetLastError());
+    hipLaunchKernelGGL(k_energy_fin, dim3(1), dim3(1024), 0, c->stream, e1.p, A, e2.p, A, S.leaf_charge.p, S.nb, fin.p);
+  }
+  hipLaunchKernelGGL(k_energy_fin, dim3(1), dim3(1024), 0, c->stream, ce.p, A, ce.p, (int64_t)0, ce.p, (int64_t)0,
+                     fin.p + 4);
+  NGPB_HIP(hipGetLastError());
+  double h[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+  if (branch != 0) NGPB_HIP(hipMemcpyAsync(h, fin.p, 3 * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  NGPB_HIP(hipMemcpyAsync(h + 4, fin.p + 4, sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  if (table_host) table.download(table_host, (size_t)12 * A, c->stream);
+  NGPB_HIP(hipStreamSynchronize(c->stream));
+  const double first_int = h[0], second_int = h[1];
+  if (branch != 0) out4[0] = 0.5 * c->pc.constant_pol * first_int;
+  if (ionic) out4[1] = 0.5 * (second_int - first_int * c->pc.constant_react);
+  out4[2] = h[4];
+  out4[3] = h[2];
+}
+
 void run_energy(ngpb_ctx* c, ngpb_energy_out* out) {
   const int64_t nb = c->n_border;
   const bool ionic = c->physics.calc_energy == 2 && c->pc.k > 1.e-5;
@@ -346,7 +673,6 @@ void run_energy(ngpb_ctx* c, ngpb_energy_out* out) {
   std::memset(out, 0, sizeof(*out));
   out->n_border = nb;
   if (nb == 0 || (!ionic && !polar_only)) return;
-  upload_tri_table(c->stream);
   // charged atoms, in order
   DBuf<uint8_t> aflag;
   DBuf<int32_t> aidx;
@@ -362,74 +688,32 @@ void run_energy(ngpb_ctx* c, ngpb_energy_out* out) {
       hipLaunchKernelGGL(k_gather_atoms, dim3(blocks_for(nq, 256)), dim3(256), 0, c->stream, nq, aidx.p, c->atom_xyz.p,
                          c->atom_q.p, atoms.p);
   }
-  const int64_t nf = kFluxSlots * nb, nv = kTriSlots * nb;
-  DBuf<double> fsrc, vsrc, leaf_charge, leaf_last, carry;
-  DBuf<uint8_t> fvalid, vvalid, vneed, leaf_has;
-  DBuf<int64_t> counts;
-  fsrc.resize((size_t)kFS * nf);
-  vsrc.resize((size_t)kVS * nv);
-  fvalid.resize((size_t)nf);
-  vvalid.resize((size_t)nv);
-  vneed.resize((size_t)nv);
-  leaf_charge.resize((size_t)nb);
-  leaf_last.resize((size_t)3 * nb);
-  leaf_has.resize((size_t)nb);
-  carry.resize((size_t)3 * nb);
-  counts.resize(2);
-  NGPB_HIP(hipMemsetAsync(counts.p, 0, 2 * sizeof(int64_t), c->stream));
-  NGPB_HIP(hipMemsetAsync(vneed.p, 0, nv, c->stream));
-  EnergyArgs ea;
-  ea.m = mesh_view_of(c);
-  ea.nb = nb;
-  ea.border = c->border.p;
-  ea.eps = c->eps.p;
-  ea.phi = c->x.p;
-  ea.frac = c->frac.p;
-  ea.nrm = c->frac_normal.p;
-  ea.found = c->frac_found.p;
-  ea.eps_out = c->pc.eps_out;
-  ea.ionic = ionic ? 1 : 0;
-  hipLaunchKernelGGL(k_sources, dim3(blocks_for(nb, 64)), dim3(64), 0, c->stream, ea, fsrc.p, fvalid.p, vsrc.p, vvalid.p,
-                     vneed.p, leaf_charge.p, leaf_last.p, leaf_has.p, counts.p);
-  NGPB_HIP(hipGetLastError());
-  if (ionic) {
-    hipLaunchKernelGGL(k_normal_carry, dim3(1), dim3(1024), 0, c->stream, nb, leaf_has.p, leaf_last.p, carry.p);
-    hipLaunchKernelGGL(k_apply_carry, dim3(blocks_for(nv, 256)), dim3(256), 0, c->stream, nv, vvalid.p, vneed.p, carry.p,
-                       vsrc.p);
-    NGPB_HIP(hipGetLastError());
-  }
-  // compact valid sources (slot order is the reference's summation order)
-  DBuf<int32_t> fidx, vidx;
-  fidx.resize((size_t)nf);
-  vidx.resize((size_t)nv);
-  const int64_t nfv = compact_flags(c, fvalid.p, nf, fidx.p);
-  const int64_t nvv = ionic ? compact_flags(c, vvalid.p, nv, vidx.p) : 0;
-  DBuf<double> fd, vd, facc, vacc;
-  fd.resize((size_t)kFS * std::max<int64_t>(nfv, 1));
-  vd.resize((size_t)kVS * std::max<int64_t>(nvv, 1));
+  Sources S;
+  build_sources(c, ionic, S);
+  const int64_t nfv = S.nf, nvv = S.nv;
+  DBuf<double> facc, vacc;
   facc.resize((size_t)std::max<int64_t>(nfv, 1));
   vacc.resize((size_t)std::max<int64_t>(nvv, 1));
-  if (nfv)
-    hipLaunchKernelGGL(k_gather_src<kFS>, dim3(blocks_for(nfv, 256)), dim3(256), 0, c->stream, nfv, fidx.p, fsrc.p, fd.p);
-  if (nvv)
-    hipLaunchKernelGGL(k_gather_src<kVS>, dim3(blocks_for(nvv, 256)), dim3(256), 0, c->stream, nvv, vidx.p, vsrc.p, vd.p);
   if (nfv) {
     if (ionic)
-      hipLaunchKernelGGL(k_nbody<1>, dim3(blocks_for(nfv, kNB * kNS)), dim3(kNB), 0, c->stream, nfv, fd.p, nq, atoms.p, facc.p);
+      hipLaunchKernelGGL(k_nbody<1>, dim3(blocks_for(nfv, kNB * kNS)), dim3(kNB), 0, c->stream, nfv, S.fd.p, nq, atoms.p,
+                         facc.p);
     else
-      hipLaunchKernelGGL(k_nbody<0>, dim3(blocks_for(nfv, kNB * kNS)), dim3(kNB), 0, c->stream, nfv, fd.p, nq, atoms.p, facc.p);
+      hipLaunchKernelGGL(k_nbody<0>, dim3(blocks_for(nfv, kNB * kNS)), dim3(kNB), 0, c->stream, nfv, S.fd.p, nq, atoms.p,
+                         facc.p);
   }
   if (nvv)
-    hipLaunchKernelGGL(k_nbody<2>, dim3(blocks_for(nvv, kNB * kNS)), dim3(kNB), 0, c->stream, nvv, vd.p, nq, atoms.p, vacc.p);
+    hipLaunchKernelGGL(k_nbody<2>, dim3(blocks_for(nvv, kNB * kNS)), dim3(kNB), 0, c->stream, nvv, S.vd.p, nq, atoms.p,
+                       vacc.p);
   NGPB_HIP(hipGetLastError());
   c->en_out.resize(4);
-  hipLaunchKernelGGL(k_energy_fin, dim3(1), dim3(1024), 0, c->stream, facc.p, nfv, vacc.p, nvv, leaf_charge.p, nb,
+  hipLaunchKernelGGL(k_energy_fin, dim3(1), dim3(1024), 0, c->stream, facc.p, nfv, vacc.p, nvv, S.leaf_charge.p, nb,
                      c->en_out.p);
   NGPB_HIP(hipGetLastError());
   double h[3];
   int64_t hc[2];
   NGPB_HIP(hipMemcpyAsync(h, c->en_out.p, 3 * sizeof(double), hipMemcpyDeviceToHost, c->stream));
-  NGPB_HIP(hipMemcpyAsync(hc, counts.p, 2 * sizeof(int64_t), hipMemcpyDeviceToHost, c->stream));
+  NGPB_HIP(hipMemcpyAsync(hc, S.counts.p, 2 * sizeof(int64_t), hipMemcpyDeviceToHost, c->stream));
   NGPB_HIP(hipStreamSynchronize(c->stream));
   const double first_int = h[0], second_int = h[1];
   out->charge_pol = h[2];

@@ -218,7 +218,7 @@ struct ngpb_ctx {
   int64_t n_list_delta = 0, n_list_packed = 0, n_list_offset = 0;
   ngpb::DBuf<int32_t> sell_list_delta, sell_list_packed, sell_list_offset;
   ngpb::DBuf<double> dscale;         // 1/sqrt(diag)
-  int spmv_sched = 3;                // SpMV work schedule (0 .. 3)
+  int spmv_sched = 3;                // SpMV work schedule (0 .. 4)
   int sell_resident = 0;             // resident blocks of the compact SpMV (0: not yet queried)
   int csr_resident = 0;              // resident blocks of the FP64 CSR SpMV (0: not yet queried)
 
@@ -289,6 +289,8 @@ double time_spmv_compact(ngpb_ctx* c, int reps);
 
 // energy
 void run_energy(ngpb_ctx* c, ngpb_energy_out* out);
+int pot_field_branch(int P, int F, double k);
+void run_pot_field(ngpb_ctx* c, int P, int F, double* table_host, double out4[4], int* branch);
 
 // SpMV launch timing: every kProfEvery-th launch is bracketed by events
 constexpr int64_t kProfEvery = 8;

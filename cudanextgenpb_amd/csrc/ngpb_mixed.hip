@@ -468,20 +468,37 @@ __global__ void __launch_bounds__(kBS) k_sell_packed(Sell A, const int32_t* __re
 // pair with one packed slice, so that at any time the round-trip-bound packed work
 // and the streaming delta work of the CU's waves overlap, and no wave idles while
 // the other list still has work.
-template <bool DOT>
+//
+// XCD = true: the hardware deals blocks to the 8 XCDs round-robin (block b on XCD
+// b % 8), so the blocks of one XCD take one contiguous eighth of each list and
+// interleave within it; every XCD's L2 then sees the input vector entries of its own
+// row range instead of entries from all over the vector. Needs a grid that is a
+// multiple of 8.
+template <bool DOT, bool XCD>
 __global__ void __launch_bounds__(kBS) k_sell_both(Sell A, const int32_t* __restrict__ dlist, int64_t dcnt,
                                                    const int32_t* __restrict__ plist, int64_t pcnt,
                                                    const float* __restrict__ x, float* __restrict__ y,
                                                    const KrylovScalars* ks, double* __restrict__ partials) {
   __shared__ double s_red[kBS / 64];
   if (DOT && ks->done) return;
-  const int64_t W = (int64_t)gridDim.x * (kBS / kW);
-  const int64_t wave = __builtin_amdgcn_readfirstlane((int)(blockIdx.x * (kBS / kW) + (threadIdx.x >> 6)));
   const int64_t npairs = (dcnt + 1) / 2;
+  int64_t W, wave, dlo = 0, dhi = npairs, plo = 0, phi = pcnt;
+  if (XCD) {
+    const int64_t xcd = blockIdx.x % 8u, local = blockIdx.x / 8u;
+    W = (int64_t)(gridDim.x / 8u) * (kBS / kW);
+    wave = __builtin_amdgcn_readfirstlane((int)(local * (kBS / kW) + (threadIdx.x >> 6)));
+    dlo = xcd * npairs / 8;
+    dhi = (xcd + 1) * npairs / 8;
+    plo = xcd * pcnt / 8;
+    phi = (xcd + 1) * pcnt / 8;
+  } else {
+    W = (int64_t)gridDim.x * (kBS / kW);
+    wave = __builtin_amdgcn_readfirstlane((int)(blockIdx.x * (kBS / kW) + (threadIdx.x >> 6)));
+  }
   double part = 0.0;
-  int64_t td = wave, tp = wave;
-  while (td < npairs || tp < pcnt) {
-    if (td < npairs) {
+  int64_t td = dlo + wave, tp = plo + wave;
+  while (td < dhi || tp < phi) {
+    if (td < dhi) {
       const int64_t s0 = dlist[2 * td];
       const int64_t s1 = 2 * td + 1 < dcnt ? dlist[2 * td + 1] : -1;
       const int w0 = A.width[s0];
@@ -505,7 +522,7 @@ __global__ void __launch_bounds__(kBS) k_sell_both(Sell A, const int32_t* __rest
       }
       td += W;
     }
-    if (tp < pcnt) {
+    if (tp < phi) {
       const int64_t s = plist[tp];
       const PkHdr h = A.pk_hdr[s];
       if (h.nd <= kW) packed_slice_h<1>(A, s, h, x, y, &part);
@@ -721,7 +738,7 @@ unsigned sell_resident(ngpb_ctx* c) {
   if (c->sell_resident > 0) return (unsigned)c->sell_resident;
   int cus = 0, per_cu = 0;
   NGPB_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, c->device));
-  NGPB_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_sell_both<true>, kBS, 0));
+  NGPB_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_sell_both<true, false>, kBS, 0));
   const int r = std::max(8, (std::max(cus, 1) * std::max(per_cu, 1)) / 8 * 8);
   c->sell_resident = r;
   return (unsigned)r;
@@ -738,7 +755,7 @@ unsigned form_grid(ngpb_ctx* c, int64_t units) {
 
 // number of (p, q) partials the SpMV of the current schedule writes
 int sell_partials(ngpb_ctx* c) {
-  if (c->spmv_sched == 3)  // combined delta/packed grid, then the offset kernel's blocks
+  if (c->spmv_sched >= 3)  // combined delta/packed grid, then the offset kernel's blocks
     return (int)(std::max<int64_t>(3, sell_resident(c)) + form_grid(c, c->n_list_offset));
   if (c->spmv_sched == 1)
     return (int)std::max<int64_t>(1, std::min<int64_t>(((c->n_slices + 1) / 2 + (kBS / kW) - 1) / (kBS / kW), 2048));
@@ -755,10 +772,16 @@ void launch_sell_t(ngpb_ctx* c, const float* x, float* y, double* part) {
     hipLaunchKernelGGL(k_sell_spmv<DOT>, dim3(g), dim3(kBS), 0, c->stream, A, x, y, ks, part);
     return;
   }
-  if (c->spmv_sched == 3) {
+  if (c->spmv_sched >= 3) {
     const unsigned g = (unsigned)std::max<int64_t>(3, sell_resident(c));
-    hipLaunchKernelGGL(k_sell_both<DOT>, dim3(g), dim3(kBS), 0, c->stream, A, c->sell_list_delta.p, c->n_list_delta,
-                       c->sell_list_packed.p, c->n_list_packed, x, y, ks, part);
+    if (c->spmv_sched == 4) {
+      NGPB_REQUIRE(g % 8 == 0, NGPB_E_STATE, "XCD-segmented SpMV needs a grid that is a multiple of 8");
+      hipLaunchKernelGGL((k_sell_both<DOT, true>), dim3(g), dim3(kBS), 0, c->stream, A, c->sell_list_delta.p,
+                         c->n_list_delta, c->sell_list_packed.p, c->n_list_packed, x, y, ks, part);
+    } else {
+      hipLaunchKernelGGL((k_sell_both<DOT, false>), dim3(g), dim3(kBS), 0, c->stream, A, c->sell_list_delta.p,
+                         c->n_list_delta, c->sell_list_packed.p, c->n_list_packed, x, y, ks, part);
+    }
     const unsigned go = form_grid(c, c->n_list_offset);
     if (go)
       hipLaunchKernelGGL(k_sell_offset<DOT>, dim3(go), dim3(kBS), 0, c->stream, A, c->sell_list_offset.p,

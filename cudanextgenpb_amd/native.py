@@ -58,6 +58,15 @@ class EnergyOut(ctypes.Structure):
         return d
 
 
+class PotFieldOut(ctypes.Structure):
+    _fields_ = [("energy_pol", _D), ("energy_react", _D), ("coul_energy", _D), ("charge_pol", _D),
+                ("branch", ctypes.c_int32), ("reserved", ctypes.c_int32)]
+
+
+POT_FIELD_COLUMNS = ("phi_c", "phi_p", "phi_i", "Ex_c", "Ey_c", "Ez_c", "Ex_p", "Ey_p", "Ez_p", "Ex_i", "Ey_i",
+                     "Ez_i")
+
+
 # name -> (restype, argtypes)
 SIGNATURES = {
     "ngpb_version": (ctypes.c_char_p, []),
@@ -88,6 +97,7 @@ SIGNATURES = {
     "ngpb_spmv": (_I, [_P, _I, _P, _P]),
     "ngpb_energy": (_I, [_P, _P]),
     "ngpb_phi_on_atoms": (_I, [_P, _P]),
+    "ngpb_pot_field": (_I, [_P, _I, _I, _P, _P]),
     "ngpb_time_spmv": (_I, [_P, _I, _I, _P, _P]),
     "ngpb_set_profiling": (_I, [_P, _I]),
     "ngpb_kernel_stats": (_I, [_P, _I, _P, _P, _P]),
@@ -295,6 +305,14 @@ class Context:
         out = np.zeros(n_atoms)
         self._check(lib().ngpb_phi_on_atoms(self._h, _p(out)), "phi_on_atoms")
         return out
+
+    def pot_field(self, n_atoms: int, calc_potential_term: int, calc_field_term: int):
+        """(table n_atoms x 12 in POT_FIELD_COLUMNS order, energies and branch)."""
+        table = np.zeros((n_atoms, 12))
+        o = PotFieldOut()
+        self._check(lib().ngpb_pot_field(self._h, int(calc_potential_term), int(calc_field_term), _p(table),
+                                         ctypes.byref(o)), "pot_field")
+        return table, {k: getattr(o, k) for k, _ in o._fields_ if k != "reserved"}
 
     def energy(self):
         e = EnergyOut()

@@ -81,6 +81,37 @@ class PoissonBoltzmann:
                     f.write(f"{i + 1:8d}{x:8.3f}{y:8.3f}{z:8.3f}  {phi[i]:.4f}\n")
         return phi
 
+    # pot_field / pot_field_fast (src/pb_class.cpp:3731-5060): per-atom potentials and
+    # fields; pot_field.dat is written as the reference writes it (:4276-4385): a
+    # header naming the columns of the branch, then per atom the index (width 5), the
+    # coordinates and the branch's columns (width 8, six significant digits), each
+    # followed by two spaces.
+    _PF_COLS = {1: [0, 1, 3, 4, 5, 6, 7, 8], 2: [0, 1, 3, 4, 5], 3: list(range(12)), 4: [0, 1, 2, 3, 4, 5],
+                5: list(range(9))}
+    _PF_HEAD = {1: "phi_c    phi_p    Ex_c    Ey_c    Ez_c   Ex_p    Ey_p    Ez_p\n",
+                2: "phi_c    phi_p    Ex_c    Ey_c    Ez_c\n",
+                3: "phi_c    phi_p    phi_i    Ex_c    Ey_c    Ez_c   Ex_p    Ey_p    Ez_p   Ex_i    Ey_i    Ez_i\n",
+                4: "phi_c    phi_p    phi_i    Ex_c    Ey_c    Ez_c\n",
+                5: "phi_c    phi_p    phi_i    Ex_c    Ey_c    Ez_c   Ex_p    Ey_p    Ez_p\n"}
+
+    def pot_field(self, calc_potential_term: int = 2, calc_field_term: int = 2,
+                  path: Optional[str] = "pot_field.dat"):
+        a = self.problem.atoms
+        table, out = self.ctx.pot_field(a.n, calc_potential_term, calc_field_term)
+        if path is not None:
+            br = out["branch"]
+            with open(path, "w") as f:
+                f.write("# index    x    y    z    ")
+                if br in self._PF_HEAD:
+                    f.write(self._PF_HEAD[br])
+                    cols = self._PF_COLS[br]
+                    for i in range(a.n):
+                        vals = list(a.xyz[i]) + [table[i, c] for c in cols]
+                        f.write(f"{i + 1:>5}  " + "".join(f"{v:>8g}  " for v in vals) + "\n")
+        out["polar"] = out["energy_pol"] + out["energy_react"]
+        self.pot_field_energies = out
+        return table, out
+
     # energy / energy_fast (src/pb_class.cpp:2790-3338)
     def energy(self):
         self.energies = self.ctx.energy()

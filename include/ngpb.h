@@ -29,6 +29,7 @@
  *   ngpb_energy           poisson_boltzmann::energy / energy_fast
  *                         (include/pb_class.h:758-762, src/pb_class.cpp:2790-3338)
  *   ngpb_phi_on_atoms     write_potential_on_atoms_fast (src/pb_class.cpp:2333-2368)
+ *   ngpb_pot_field        pot_field / pot_field_fast (src/pb_class.cpp:3731-5060)
  */
 #ifndef NGPB_H
 #define NGPB_H
@@ -93,6 +94,16 @@ typedef struct {
   int64_t n_flux_edges; /* edges with eps(i1) != eps(i2) on border leaves */
   int64_t n_triangles;  /* marching-cubes triangles (ionic term) */
 } ngpb_energy_out;
+
+/* Energies printed by pot_field / pot_field_fast (src/pb_class.cpp:4253-4274). */
+typedef struct {
+  double energy_pol;    /* 0.5 constant_pol sum_a q_a sum_e flux_e / r_ae (0 in branch 0) */
+  double energy_react;  /* branches 3-5: 0.5 (second_int - first_int constant_react) */
+  double coul_energy;   /* sum over atom pairs q_i q_j / (r_ij eps_in_hat) */
+  double charge_pol;    /* sum of the edge fluxes (flux charge times 4 pi) */
+  int32_t branch;       /* which columns were computed, see ngpb_pot_field */
+  int32_t reserved;
+} ngpb_pot_field_out;
 
 /* ---- library / context ------------------------------------------------------- */
 const char* ngpb_version(void);
@@ -165,6 +176,24 @@ int ngpb_energy(ngpb_ctx* ctx, ngpb_energy_out* out);
    write_potential_on_atoms_fast (src/pb_class.cpp:2333-2368); the phi_on_atoms.txt
    writer is cudanextgenpb_amd.pb.PoissonBoltzmann.write_potential_on_atoms. */
 int ngpb_phi_on_atoms(ngpb_ctx* ctx, double* phi_atoms);
+/* Per-atom potentials and fields, replacing the computation of pot_field (meshes with
+   hanging nodes, src/pb_class.cpp:4389-5060) and pot_field_fast (:3731-4387).
+   table (may be NULL): n_atoms rows of 12 doubles, the columns of pot_field.dat in
+   order phi_c phi_p phi_i Ex_c Ey_c Ez_c Ex_p Ey_p Ez_p Ex_i Ey_i Ez_i, with the
+   reference's write-out corrections of phi_i and E_i applied (:4286-4290). The
+   columns computed follow the reference's branches on calc_potential_term P and
+   calc_field_term F (out->branch):
+     1  (F == 1 || (F == 2 && k < 1e-5)) && P < 2    phi_p, E_p
+     2  (P == 1 || (P == 2 && k < 1e-5)) && F == 0    phi_p
+     3  F == 2 && k > 1e-5                             phi_p, phi_i, E_p, E_i
+     4  P == 2 && k > 1e-5 && F == 0                   phi_p, phi_i
+     5  P == 2 && k > 1e-5 && F == 1                   phi_p, phi_i, E_p
+     0  otherwise                                      Coulomb columns only
+   The Coulomb columns phi_c, E_c are always computed; other columns are 0. Every atom
+   of ngpb_set_charges takes part (no charge filter, as in the reference). The
+   pot_field.dat writer is cudanextgenpb_amd.pb.PoissonBoltzmann.pot_field. */
+int ngpb_pot_field(ngpb_ctx* ctx, int calc_potential_term, int calc_field_term, double* table,
+                   ngpb_pot_field_out* out);
 
 /* ---- measurement ----------------------------------------------------------------- */
 /* Times `reps` back-to-back launches of the SpMV kernel of the given precision on the
@@ -194,10 +223,13 @@ int ngpb_operator_info(ngpb_ctx* ctx, int64_t* info10);
    1 a single kernel for all forms, 2 as 0 on the resident block count, 3 (default) the
    delta and packed slices in one launch on the resident grid, every wave alternating
    between its share of the two lists so that the two forms run side by side (then
-   the offset slices, if any).
+   the offset slices, if any), 4 as 3 with the lists cut into 8 contiguous parts,
+   one per XCD (block b runs on XCD b % 8), the blocks of an XCD interleaving
+   within its part.
    FP64 CSR: 0 grid-stride over row chunks on up to 2048 blocks, 1 one contiguous
    chunk range per block with XCD-aware block numbering on the resident grid
-   (measured 1.36x slower), 2 and 3 grid-stride on the resident grid. Results differ
+   (measured 1.36x slower), 2 and 3 grid-stride on the resident grid, 4 grid-stride
+   within one contiguous eighth of the chunks per XCD. Results differ
    only in the grouping of the partial sums of (p, q). No counterpart in the
    reference. */
 int ngpb_set_spmv_schedule(ngpb_ctx* ctx, int schedule);

@@ -15,6 +15,7 @@
 //   orc_pcg            src/pb_class.cpp:2243-2329 + LIS 2.1.6 CG with Jacobi (-i cg -p jacobi)
 //   orc_energy         src/pb_class.cpp:2534-2788 (helpers), 2790-3063 (energy)
 //   orc_phi_on_atoms   src/pb_class.cpp:2333-2368 (write_potential_on_atoms_fast)
+//   orc_pot_field      src/pb_class.cpp:3731-5060 (pot_field_fast / pot_field)
 //
 // Pinning (how each is checked without the reference's libraries, which are absent):
 //   constants  closed forms; markers: brute-force distance test on sphere unions +
@@ -23,7 +24,10 @@
 //   closed form, symmetry, zero interior row sums at kappa=0, exact reproduction of
 //   linear fields; pcg: dense direct solves of small systems; energy: Born-sphere
 //   analytic energy and the Gauss flux-charge check; phi on atoms: exact
-//   reproduction of linear fields (trilinear interpolation).
+//   reproduction of linear fields (trilinear interpolation); pot_field: every field
+//   column is minus the gradient of its potential column with respect to the atom
+//   position (central differences), half the charge-weighted sum of phi_p + phi_i
+//   is the energy above, and the Coulomb columns match direct pair sums.
 // The bimpp conventions (hanging-node algebra, reaction lumping, Dirichlet
 // elimination) cannot be pinned against bimpp itself, which is not available:
 // "parity unpinned" against bimpp; pinned by the invariants above.
@@ -736,6 +740,197 @@ int orc_energy(const orc_mesh* m, const orc_rays* rays, const double* eps_node,
   out[1] = energy_react;
   out[2] = charge_pol;
   return 0;
+}
+
+// ---------------------------------------------------------------- potentials and fields at atoms
+// pot_field (src/pb_class.cpp:4389-5060; pot_field_fast :3731-4387 is the same on
+// meshes without hanging nodes, where every leaf has the first border leaf's h).
+// Branch, from calc_potential_term P and calc_field_term F (:3857-4156):
+//   1  (F == 1 || (F == 2 && k < 1e-5)) && P < 2   phi_p, field_p
+//   2  (P == 1 || (P == 2 && k < 1e-5)) && F == 0   phi_p
+//   3  F == 2 && k > 1e-5                            phi_p, phi_i, field_p, field_i
+//   4  P == 2 && k > 1e-5 && F == 0                  phi_p, phi_i
+//   5  P == 2 && k > 1e-5 && F == 1                  phi_p, phi_i, field_p
+//   0  none of these                                 Coulomb terms only
+// table: n_atoms rows of 12 columns phi_c phi_p phi_i Ex_c Ey_c Ez_c Ex_p Ey_p Ez_p
+// Ex_i Ey_i Ez_i, with the write-out corrections applied (:4286-4290: phi_i and
+// field_i minus phi_p and field_p times constant_react / constant_pol); columns a
+// branch does not compute are 0. out[0..3] = energy_pol, energy_react, coul_energy,
+// charge_pol.
+int orc_pot_field_branch(int P, int F, double k) {
+  if ((F == 1 || (F == 2 && k < 1.e-5)) && P < 2) return 1;
+  if ((P == 1 || (P == 2 && k < 1.e-5)) && F == 0) return 2;
+  if (F == 2 && k > 1.e-5) return 3;
+  if (P == 2 && k > 1.e-5 && F == 0) return 4;
+  if (P == 2 && k > 1.e-5 && F == 1) return 5;
+  return 0;
+}
+
+int orc_pot_field(const orc_mesh* m, const orc_rays* rays, const double* eps_node, const double* phi,
+                  int64_t n_border, const int32_t* border, int64_t na, const double* xyz, const double* q,
+                  double e_in, double e_out, double T, double I, int P, int F, double* table, double* out) {
+  const auto maps = build_maps(rays);
+  MeshView mv{m};
+  double cst[6];
+  orc_constants(e_in, e_out, T, I, cst);
+  const double eps_in = cst[0], eps_out = cst[1], k = cst[3];
+  const double inv_4pi = 1.0 / (4.0 * kPi);
+  const double den_in = 1.0 / eps_in;
+  const double constant_pol = cst[4], constant_react = cst[5];
+  const int branch = orc_pot_field_branch(P, F, k);
+  std::vector<double> phi_c(na, 0.0), fcx(na, 0.0), fcy(na, 0.0), fcz(na, 0.0);
+  std::vector<double> phi_p(na, 0.0), fpx(na, 0.0), fpy(na, 0.0), fpz(na, 0.0);
+  std::vector<double> phi_i(na, 0.0), fix(na, 0.0), fiy(na, 0.0), fiz(na, 0.0);
+  auto pos = [&](int64_t a, int d) { return xyz[3 * a + d]; };
+  double coul_energy = 0.0;
+  // Coulomb pairs (:3776-3813)
+  for (int64_t i = 0; i < na; ++i) {
+    const double qi = q[i];
+    for (int64_t j = i + 1; j < na; ++j) {
+      const double qj = q[j];
+      const double dx = pos(i, 0) - pos(j, 0), dy = pos(i, 1) - pos(j, 1), dz = pos(i, 2) - pos(j, 2);
+      const double r2 = dx * dx + dy * dy + dz * dz;
+      const double r = std::sqrt(r2);
+      const double inv_r3 = 1.0 / (r2 * r);
+      coul_energy += (qi * qj) / r * den_in;
+      phi_c[i] += qj / r * den_in;
+      phi_c[j] += qi / r * den_in;
+      fcx[i] += dx * inv_r3 * qj * den_in;
+      fcy[i] += dy * inv_r3 * qj * den_in;
+      fcz[i] += dz * inv_r3 * qj * den_in;
+      fcx[j] += -dx * inv_r3 * qi * den_in;
+      fcy[j] += -dy * inv_r3 * qi * den_in;
+      fcz[j] += -dz * inv_r3 * qi * den_in;
+    }
+  }
+  const bool fieldp = branch == 1 || branch == 3 || branch == 5;
+  const bool ionic = branch >= 3;
+  const bool fieldi = branch == 3;
+  double first_int = 0.0, second_int = 0.0, charge_pol = 0.0;
+  double N[3] = {0.0, 0.0, 0.0};
+  for (int64_t bi = 0; bi < n_border && branch != 0; ++bi) {
+    const int64_t l = border[bi];
+    double h[3], area_h[3];
+    for (int d = 0; d < 3; ++d) h[d] = mv.p(l, d, 7) - mv.p(l, d, 0);
+    area_h[0] = h[1] * h[2] / h[0] * 0.25;
+    area_h[1] = h[0] * h[2] / h[1] * 0.25;
+    area_h[2] = h[0] * h[1] / h[2] * 0.25;
+    double te[8], tp[8];
+    for (int ii = 0; ii < 8; ++ii) {
+      if (!mv.hanging(l, ii)) {
+        te[ii] = eps_node[mv.gt(l, ii)];
+        tp[ii] = phi[mv.gt(l, ii)];
+      } else {
+        te[ii] = 0.0;
+        tp[ii] = 0.0;
+        const int npi = mv.num_parents(l, ii);
+        for (int jj = 0; jj < npi; ++jj) {
+          te[ii] += eps_node[mv.gparent(l, jj, ii)] / npi;
+          tp[ii] += phi[mv.gparent(l, jj, ii)] / npi;
+        }
+      }
+    }
+    int cubeindex = -1;
+    if (ionic) {
+      static const int order[8] = {0, 1, 3, 2, 4, 5, 7, 6};
+      int ci = 0, bit = 1;
+      for (int o = 0; o < 8; ++o) {
+        if (te[order[o]] < eps_out - 1e-10) ci |= bit;
+        bit *= 2;
+      }
+      cubeindex = (ci == 0 || ci == 255) ? -1 : ci;
+    }
+    for (int e = 0; e < 12; ++e) {
+      const int i1 = kEdgeNodes[2 * e], i2 = kEdgeNodes[2 * e + 1];
+      double fl;
+      if (te[i1] < te[i2]) fl = 1;
+      else if (te[i1] > te[i2]) fl = -1;
+      else continue;
+      const int axis = kEdgeAxis[e];
+      double fract = 0.0;
+      normal_intersection(mv, maps, l, e, N, &fract);
+      double V[3] = {mv.p(l, 0, i1), mv.p(l, 1, i1), mv.p(l, 2, i1)};
+      V[axis] += fract * h[axis];
+      const double tmp_flux = -(tp[i2] - tp[i1]) * wha(te[i1], te[i2], fract) * fl * area_h[axis];
+      charge_pol += tmp_flux;
+      for (int64_t ia = 0; ia < na; ++ia) {
+        const double dx = pos(ia, 0) - V[0], dy = pos(ia, 1) - V[1], dz = pos(ia, 2) - V[2];
+        const double r = std::sqrt(dx * dx + dy * dy + dz * dz);
+        const double inv_r3 = 1.0 / (r * r * r);
+        if (branch <= 2) {
+          const double qflux = tmp_flux / r;
+          first_int += q[ia] * qflux;
+          phi_p[ia] += qflux * constant_pol;
+        } else {
+          first_int += q[ia] * tmp_flux / r;
+          phi_p[ia] += tmp_flux / r * constant_pol;
+        }
+        if (fieldp) {
+          fpx[ia] += dx * inv_r3 * tmp_flux * constant_pol;
+          fpy[ia] += dy * inv_r3 * tmp_flux * constant_pol;
+          fpz[ia] += dz * inv_r3 * tmp_flux * constant_pol;
+        }
+      }
+    }
+    if (ionic && cubeindex >= 0) {
+      const char* tri = kMcTriangles[cubeindex];
+      const int ntri = (int)std::strlen(tri) / 3;
+      for (int t = 0; t < ntri; ++t) {
+        double vt[3][3], nv[3][3], ps[3];
+        for (int jj = 0; jj < 3; ++jj) {
+          const int edge = mc_digit(tri[3 * t + jj]);
+          const int axis = kEdgeAxis[edge];
+          const int i1 = kEdgeNodes[2 * edge], i2 = kEdgeNodes[2 * edge + 1];
+          double fract = 0.0;
+          normal_intersection(mv, maps, l, edge, N, &fract);
+          double V[3] = {mv.p(l, 0, i1), mv.p(l, 1, i1), mv.p(l, 2, i1)};
+          V[axis] += fract * h[axis];
+          for (int d = 0; d < 3; ++d) { vt[jj][d] = V[d]; nv[jj][d] = N[d]; }
+          ps[jj] = phi0(te[i1], te[i2], tp[i1], tp[i2], fract);
+        }
+        const double area = area_triangle(vt);
+        for (int64_t ia = 0; ia < na; ++ia) {
+          const double qi = q[ia];
+          for (int kk = 0; kk < 3; ++kk) {
+            const double dv[3] = {vt[kk][0] - pos(ia, 0), vt[kk][1] - pos(ia, 1), vt[kk][2] - pos(ia, 2)};
+            const double r2 = dv[0] * dv[0] + dv[1] * dv[1] + dv[2] * dv[2];
+            const double r = std::sqrt(r2);
+            const double inv_r3 = 1.0 / (r2 * r);
+            const double inv_r5 = inv_r3 / r2;
+            const double dt = dv[0] * nv[kk][0] + dv[1] * nv[kk][1] + dv[2] * nv[kk][2];
+            const double factor = ps[kk] * inv_4pi * area / 3.0;
+            second_int += qi * ps[kk] * dt * inv_r3 * inv_4pi * area / 3.0;
+            phi_i[ia] += ps[kk] * dt * inv_r3 * inv_4pi * area / 3.0;
+            if (fieldi) {
+              fix[ia] += factor * (-3 * dv[0] * inv_r5 * dt + nv[kk][0] * inv_r3);
+              fiy[ia] += factor * (-3 * dv[1] * inv_r5 * dt + nv[kk][1] * inv_r3);
+              fiz[ia] += factor * (-3 * dv[2] * inv_r5 * dt + nv[kk][2] * inv_r3);
+            }
+          }
+        }
+      }
+    }
+  }
+  double energy_pol = 0.0, energy_react = 0.0;
+  if (branch != 0) energy_pol = 0.5 * constant_pol * first_int;
+  if (ionic) energy_react = 0.5 * (second_int - first_int * constant_react);
+  for (int64_t i = 0; i < na; ++i) {
+    if (ionic) phi_i[i] -= phi_p[i] / constant_pol * constant_react;
+    if (fieldi) {
+      fix[i] -= fpx[i] / constant_pol * constant_react;
+      fiy[i] -= fpy[i] / constant_pol * constant_react;
+      fiz[i] -= fpz[i] / constant_pol * constant_react;
+    }
+    double* row = table + 12 * i;
+    const double vals[12] = {phi_c[i], phi_p[i], phi_i[i], fcx[i], fcy[i], fcz[i],
+                             fpx[i],   fpy[i],   fpz[i],   fix[i], fiy[i], fiz[i]};
+    for (int c = 0; c < 12; ++c) row[c] = vals[c];
+  }
+  out[0] = energy_pol;
+  out[1] = energy_react;
+  out[2] = coul_energy;
+  out[3] = charge_pol;
+  return branch;
 }
 
 // Born-sphere potential used for bc_type 3 (src/pb_class.cpp:3549-3571): q = +1 and

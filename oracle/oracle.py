@@ -43,6 +43,9 @@ def lib():
         L.orc_locate.argtypes = [_P, _I64, _P, _D, _P]
         L.orc_rhs.argtypes = [_P, _I64, _P, _P, _P, _P]
         L.orc_phi_on_atoms.argtypes = [_P, _I64, _P, _P, _P, _P]
+        L.orc_pot_field.argtypes = [_P, _P, _P, _P, _I64, _P, _I64, _P, _P, _D, _D, _D, _D, ctypes.c_int,
+                                    ctypes.c_int, _P, _P]
+        L.orc_pot_field_branch.argtypes = [ctypes.c_int, ctypes.c_int, _D]
         L.orc_assemble.argtypes = [_P, _P, _P, _P, _I64, _P, _P, _P]
         L.orc_assemble.restype = _P
         L.orc_sys_sizes.argtypes = [_P, _P, _P]
@@ -254,6 +257,32 @@ def energy(problem, eps, phi, border):
                      ph.calc_energy, _p(out))
     return dict(energy_pol=out[0], energy_react=out[1], charge_pol=out[2],
                 flux_charge=out[2] / (4.0 * np.pi), polar=out[0] + out[1])
+
+
+POT_FIELD_COLUMNS = ("phi_c", "phi_p", "phi_i", "Ex_c", "Ey_c", "Ez_c", "Ex_p", "Ey_p", "Ez_p", "Ex_i", "Ey_i",
+                     "Ez_i")
+
+
+def pot_field(problem, eps, phi, border, calc_potential_term, calc_field_term, xyz=None):
+    """Per-atom potentials and fields (pot_field / pot_field_fast): (branch, table
+    n_atoms x 12 in POT_FIELD_COLUMNS order, energies). `xyz` overrides the atom
+    positions where the potentials are evaluated (the sources stay the same)."""
+    m, r = _MeshArgs(problem.mesh), _RayArgs(problem.rays)
+    na = problem.atoms.n
+    table = np.zeros((na, 12))
+    out = np.zeros(4)
+    pos = _c(problem.atoms.xyz if xyz is None else xyz, np.float64)
+    q = _c(problem.atoms.q, np.float64)
+    b = _c(border, np.int32)
+    ph = problem.physics
+    br = lib().orc_pot_field(m.ptr, r.ptr, _p(_c(eps, np.float64)), _p(_c(phi, np.float64)), b.shape[0], _p(b), na,
+                             _p(pos), _p(q), ph.e_in, ph.e_out, ph.T, ph.ionic_strength, int(calc_potential_term),
+                             int(calc_field_term), _p(table), _p(out))
+    return br, table, dict(energy_pol=out[0], energy_react=out[1], coul_energy=out[2], charge_pol=out[3])
+
+
+def pot_field_branch(calc_potential_term, calc_field_term, k):
+    return lib().orc_pot_field_branch(int(calc_potential_term), int(calc_field_term), float(k))
 
 
 def full_pipeline(problem, tol=1e-10, maxiter=100000, nthreads=1):

@@ -158,6 +158,36 @@ def test_phi_on_atoms_bitexact(name, tmp_path):
     pb.close()
 
 
+@pytest.mark.parametrize("name", ["three_spheres", "globular_small"])
+@pytest.mark.parametrize("pf", [(2, 2), (1, 1), (2, 0), (2, 1), (1, 0)])
+def test_pot_field_matches_oracle(name, pf, tmp_path):
+    """Coulomb columns bit-exact (same pair order, sqrt and divisions); the surface
+    columns to 1e-11 of each column group's largest value (approximate 1/r, chunked
+    sums); energies to 1e-11 relative."""
+    P = problem(name)
+    R = reference(name)
+    bv = O.dirichlet_values(P)
+    _, pb = gpu_pipeline(name, bnd_values=bv)
+    pb.ctx.set_phi(R["phi"])
+    path = tmp_path / "pot_field.dat"
+    T, E = pb.pot_field(*pf, path=str(path))
+    br, Tr, Er = O.pot_field(P, R["eps"], R["phi"], R["border"], *pf)
+    print(f"[pot_field case] {name} P,F={pf} branch={br} atoms={P.atoms.n}", flush=True)
+    assert E["branch"] == br
+    assert np.array_equal(T[:, [0, 3, 4, 5]], Tr[:, [0, 3, 4, 5]])
+    for cols in ([1], [2], [6, 7, 8], [9, 10, 11]):
+        scale = np.abs(Tr[:, cols]).max()
+        assert np.abs(T[:, cols] - Tr[:, cols]).max() <= 1e-11 * scale, cols
+        if scale == 0:
+            assert not T[:, cols].any()
+    for k in ("energy_pol", "energy_react", "coul_energy", "charge_pol"):
+        assert abs(E[k] - Er[k]) <= 1e-11 * max(abs(Er[k]), 1e-300), (k, E[k], Er[k])
+    lines = path.read_text().splitlines()
+    assert len(lines) == P.atoms.n + 1
+    assert len(lines[1].split()) == 4 + len(pb._PF_COLS[br])
+    pb.close()
+
+
 def test_born_energy_closed_form():
     P, pb = gpu_pipeline("born_h05")
     pb.hip_compute_electric_potential()
@@ -215,7 +245,7 @@ def test_spmv_schedules_agree(name):
     x = np.random.default_rng(5).standard_normal(P.mesh.n_nodes)
     ref32 = ref64 = None
     energies = []
-    for sched in (0, 1, 2, 3):
+    for sched in (0, 1, 2, 3, 4):
         pb.ctx.set_spmv_schedule(sched)
         y32 = pb.ctx.spmv(x, native.PREC_MIXED)
         y64 = pb.ctx.spmv(x, native.PREC_FP64)

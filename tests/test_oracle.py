@@ -8,6 +8,7 @@ through hanging-node constraints, charge conservation, Gauss's law), brute force
 independent direct solver (committed golden vectors).
 """
 import dataclasses
+import functools
 import os
 
 import numpy as np
@@ -335,6 +336,75 @@ def test_energy_from_golden_direct_solution():
     E = O.energy(P, eps, g["phi_direct"], border)
     assert E["energy_pol"] == pytest.approx(float(g["energy_pol"]), rel=1e-12)
     assert E["energy_react"] == pytest.approx(float(g["energy_react"]), rel=1e-12)
+
+
+# ---------------------------------------------------------------- potentials and fields at atoms
+@functools.lru_cache(maxsize=None)
+def _pf_case():
+    # hanging nodes, several charged atoms, salt (kappa > 0)
+    P = W.globular(n_atoms=12, ball_radius=3.5, min_sep=2.0, fine_band=1.5, mid_band=3.0, bc_type=2)
+    R = O.full_pipeline(P)
+    return P, R
+
+
+def test_pot_field_branches():
+    k = 0.124
+    expect = {(0, 0, k): 0, (1, 0, k): 2, (0, 1, k): 1, (1, 1, k): 1, (2, 0, k): 4, (2, 1, k): 5, (2, 2, k): 3,
+              (1, 2, k): 3, (0, 2, k): 3, (2, 0, 0.0): 2, (0, 2, 0.0): 1, (1, 2, 0.0): 1, (2, 2, 0.0): 0,
+              (2, 1, 0.0): 0}
+    for (p, f, kk), b in expect.items():
+        assert O.pot_field_branch(p, f, kk) == b, (p, f, kk)
+
+
+def test_pot_field_energies_and_coulomb():
+    P, R = _pf_case()
+    assert P.mesh.n_hang > 0
+    br, T, E = O.pot_field(P, R["eps"], R["phi"], R["border"], 2, 2)
+    assert br == 3
+    Er = R["energy"]
+    assert E["energy_pol"] == pytest.approx(Er["energy_pol"], rel=1e-12)
+    assert E["energy_react"] == pytest.approx(Er["energy_react"], rel=1e-12)
+    assert E["charge_pol"] == pytest.approx(Er["charge_pol"], rel=1e-12)
+    q = P.atoms.q
+    # half the charge-weighted potentials are the energies
+    assert 0.5 * np.dot(q, T[:, 1]) == pytest.approx(E["energy_pol"], rel=1e-10)
+    assert 0.5 * np.dot(q, T[:, 2]) == pytest.approx(E["energy_react"], rel=1e-10)
+    # Coulomb columns against direct pair sums
+    eps_in = O.constants(P.physics)["eps_in"]
+    x = P.atoms.xyz
+    d = x[:, None, :] - x[None, :, :]
+    r = np.sqrt((d ** 2).sum(-1))
+    np.fill_diagonal(r, np.inf)
+    phic = (q[None, :] / r).sum(1) / eps_in
+    Ec = (d * (q[None, :] / r ** 3)[:, :, None]).sum(1) / eps_in
+    assert np.allclose(T[:, 0], phic, rtol=1e-12, atol=0)
+    assert np.allclose(T[:, 3:6], Ec, rtol=1e-11, atol=1e-12 * np.abs(Ec).max())
+    assert E["coul_energy"] == pytest.approx(0.5 * np.dot(q, phic), rel=1e-12)
+    # the reduced branches give the same columns where they compute them, zeros elsewhere
+    for p, f, cols in ((1, 0, [0, 1, 3, 4, 5]), (2, 0, [0, 1, 2, 3, 4, 5]), (2, 1, list(range(9))),
+                       (0, 1, [0, 1, 3, 4, 5, 6, 7, 8])):
+        _, T2, _ = O.pot_field(P, R["eps"], R["phi"], R["border"], p, f)
+        other = [c for c in range(12) if c not in cols]
+        assert np.array_equal(T2[:, cols], T[:, cols]), (p, f)
+        assert not T2[:, other].any(), (p, f)
+
+
+def test_pot_field_fields_are_minus_gradients():
+    P, R = _pf_case()
+    _, T, _ = O.pot_field(P, R["eps"], R["phi"], R["border"], 2, 2)
+    h = 1e-4
+    x0 = P.atoms.xyz.copy()
+    for a in (0, 5, 11):
+        for d in range(3):
+            xp, xm = x0.copy(), x0.copy()
+            xp[a, d] += h
+            xm[a, d] -= h
+            _, Tp, _ = O.pot_field(P, R["eps"], R["phi"], R["border"], 2, 2, xyz=xp)
+            _, Tm, _ = O.pot_field(P, R["eps"], R["phi"], R["border"], 2, 2, xyz=xm)
+            for pot, fld in ((0, 3), (1, 6), (2, 9)):
+                g = -(Tp[a, pot] - Tm[a, pot]) / (2 * h)
+                scale = np.abs(T[:, fld:fld + 3]).max()
+                assert abs(g - T[a, fld + d]) <= 1e-5 * scale, (a, d, pot, g, T[a, fld + d])
 
 
 def test_mc_table_structure():
