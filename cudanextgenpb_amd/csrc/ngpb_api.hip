@@ -56,7 +56,7 @@ int64_t ngpb_ctx::device_bytes() const {
   b += slice_width.bytes() + slice_base_ptr.bytes() + slice_wide_ptr.bytes() + slice_ov_ptr.bytes() +
        sell_val.bytes() + sell_off.bytes() + sell_base.bytes() + sell_col32.bytes() + sell_ov_cnt.bytes() +
        sell_ov_val.bytes() + sell_ov_col.bytes() + dscale.bytes() + sell_pk_val.bytes() + sell_pk_idx.bytes() +
-       sell_pk_lp.bytes() + sell_pk_dict.bytes() + sell_pk_hdr.bytes() + sell_list_delta.bytes() +
+       sell_pk_lp.bytes() + sell_pk_dict.bytes() + sell_pk_hdr.bytes() + sell_cval.bytes() + sell_list_delta.bytes() +
        sell_list_packed.bytes() + sell_list_offset.bytes();
   b += x.bytes() + r.bytes() + z.bytes() + p.bytes() + q.bytes() + dinv.bytes();
   b += xs.bytes() + rs.bytes() + ps.bytes() + qs.bytes() + ks.bytes();
@@ -699,13 +699,26 @@ int ngpb_operator_info(ngpb_ctx* c, int64_t* info) {
     info[0] = c->n_slices;
     info[1] = nd;
     info[2] = c->sell_delta_entries;
-    info[3] = c->sell_entries - c->sell_delta_entries;
+    info[3] = c->sell_entries - c->sell_delta_entries - c->sell_const_slots * 64;
     info[4] = c->sell_wide_entries;
     info[5] = c->sell_ov_entries;
     info[6] = (int64_t)ngpb::spmv_compact_bytes(c);
     info[7] = (int64_t)ngpb::spmv_fp64_bytes(c);
     info[8] = c->sell_pk_slices;
     info[9] = c->sell_pk_entries;
+    info[10] = c->sell_const_slices;
+    info[11] = c->sell_const_slots;
+  });
+}
+
+int ngpb_set_compact_options(ngpb_ctx* c, int constant_slices) {
+  return guarded(c, [&] {
+    not_in_batch(c);
+    const bool v = constant_slices != 0;
+    if (v != c->sell_allow_const) {
+      c->sell_allow_const = v;
+      c->have_compact = false;  // rebuilt with the new forms on next use
+    }
   });
 }
 

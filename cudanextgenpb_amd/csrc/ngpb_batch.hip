@@ -611,6 +611,7 @@ int ngpb_batch_add(ngpb_batch* b, ngpb_ctx* c, int* slot) {
     d.A.pk_lp = c->sell_pk_lp.p;
     d.A.pk_dict = c->sell_pk_dict.p;
     d.A.pk_hdr = reinterpret_cast<const ngpb::PkHdr*>(c->sell_pk_hdr.p);
+    d.A.cval = c->sell_cval.p;
     d.rowptr = c->rowptr.p;
     d.col = c->col.p;
     d.val = c->val.p;

@@ -214,6 +214,9 @@ struct ngpb_ctx {
   ngpb::DBuf<uint16_t> sell_pk_lp;    //                per sorted position: lane | count << 8
   ngpb::DBuf<int32_t> sell_pk_dict;   //                column deltas
   ngpb::DBuf<int64_t> sell_pk_hdr;    //                32-byte header per slice (4 x int64)
+  ngpb::DBuf<float> sell_cval;        // constant delta slices: one value per slot
+  int64_t sell_const_slices = 0, sell_const_slots = 0;
+  bool sell_allow_const = true;       // build constant delta slices where they apply
   // slices of each form, for the form-specialised SpMV kernels
   int64_t n_list_delta = 0, n_list_packed = 0, n_list_offset = 0;
   ngpb::DBuf<int32_t> sell_list_delta, sell_list_packed, sell_list_offset;

@@ -103,7 +103,9 @@ __device__ __forceinline__ int for_each_delta(const int32_t* col, int64_t row, b
 // The delta form is taken when it moves no more than 1.25x the packed form's bytes
 // (its loads are fewer and wider); the offset form only when neither fits.
 __global__ void __launch_bounds__(kBS) k_sell_shape(int64_t n, int64_t nslices, const int32_t* __restrict__ rowptr,
-                                                    const int32_t* __restrict__ col, int32_t* __restrict__ width,
+                                                    const int32_t* __restrict__ col, const double* __restrict__ val,
+                                                    const double* __restrict__ dscale, int allow_const,
+                                                    int64_t* __restrict__ cslots, int32_t* __restrict__ width,
                                                     int64_t* __restrict__ wcount, int64_t* __restrict__ wide_count,
                                                     int64_t* __restrict__ ov_count, int64_t* __restrict__ dslots,
                                                     int64_t* __restrict__ pk_count, int64_t* __restrict__ pk_dcount,
@@ -122,6 +124,17 @@ __global__ void __launch_bounds__(kBS) k_sell_shape(int64_t n, int64_t nslices, 
   const long long pk_cost = (long long)tot * 5 + 2 * kW + (long long)nd * 4;
   const bool use_delta = nd <= kDMax && (!packable || (long long)nd * kW * 4 * 4 <= pk_cost * 5);
   const bool use_packed = !use_delta && packable;
+  // constant delta slice: every row of the slice has every delta, with the same FP32
+  // value (bit for bit) in each slot (lane 0's row always exists)
+  bool cst = allow_const && use_delta && nd > 0;
+  if (cst) {
+    const double di = row < n ? dscale[row] : 0.0;
+    for_each_delta<kDMax>(col, row, row < n, b, dpos, cnt, [&](int, int, int64_t e, int) {
+      const uint32_t bits = e >= 0 ? __float_as_uint((float)(val[e] * di * dscale[col[e]])) : 0u;
+      const uint32_t b0 = (uint32_t)__shfl((int)bits, 0, 64);
+      if (__ballot(row < n && (e < 0 || bits != b0))) cst = false;
+    });
+  }
   // index words of a packed slice: one per row taking part in each group of 4 slots
   int words = 0;
   if (use_packed)
@@ -154,11 +167,13 @@ __global__ void __launch_bounds__(kBS) k_sell_shape(int64_t n, int64_t nslices, 
   if (lane_id() == 0) {
     // a slice without off-diagonals (rows of Dirichlet nodes) is a delta slice of
     // zero slots, marked -kPackedBase since -0 would read as an offset slice
-    width[s] = use_delta ? (nd == 0 ? -kPackedBase : -nd) : (use_packed ? -(kPackedBase + nd) : best_w);
+    width[s] = use_delta ? (nd == 0 ? -kPackedBase : (cst ? -(kConstBase + nd) : -nd))
+                         : (use_packed ? -(kPackedBase + nd) : best_w);
     wcount[s] = use_delta ? nd : (use_packed ? 0 : best_w);
     wide_count[s] = (offset && wide) ? (int64_t)best_w * kW : 0;
     ov_count[s] = extra;
-    dslots[s] = use_delta ? nd : 0;
+    dslots[s] = (use_delta && !cst) ? nd : 0;
+    cslots[s] = cst ? nd : 0;
     pk_count[s] = use_packed ? tot : 0;
     pk_dcount[s] = use_packed ? nd : 0;
     pk_gcount[s] = use_packed ? words : 0;
@@ -180,7 +195,8 @@ __global__ void __launch_bounds__(kBS) k_sell_fill(int64_t n, int64_t nslices, c
                                                    const int64_t* __restrict__ pk_doff,
                                                    const int64_t* __restrict__ pk_goff, float* __restrict__ pk_val,
                                                    uint8_t* __restrict__ pk_idx_bytes, uint16_t* __restrict__ pk_lp,
-                                                   int32_t* __restrict__ pk_dict, PkHdr* __restrict__ pk_hdr) {
+                                                   int32_t* __restrict__ pk_dict, PkHdr* __restrict__ pk_hdr,
+                                                   float* __restrict__ cval) {
   const int64_t s = (int64_t)blockIdx.x * (kBS / kW) + (threadIdx.x >> 6);
   if (s >= nslices) return;
   const int lane = lane_id();
@@ -242,6 +258,17 @@ __global__ void __launch_bounds__(kBS) k_sell_fill(int64_t n, int64_t nslices, c
       if (e >= 0) {
         pk_val[E + st + rank] = (float)(val[e] * di * dscale[col[e]]);
         pk_idx_bytes[(G + gst + rank) * 4 + (pos & 3)] = (uint8_t)k;
+      }
+    });
+    if (row < n) ov_cnt[row] = 0;
+    return;
+  }
+  if (w <= -(kConstBase + 1) && w >= -(kConstBase + kDMax)) {
+    // constant delta slice: the slot value once (every row holds the same)
+    for_each_delta<kDMax>(col, row, row < n, b, dpos, cnt, [&](int k, int m, int64_t e, int) {
+      if (lane == 0) {
+        sbase[bp + k] = m;
+        cval[bp + k] = (float)(val[e] * di * dscale[col[e]]);
       }
     });
     if (row < n) ov_cnt[row] = 0;
@@ -330,6 +357,11 @@ __global__ void __launch_bounds__(kBS) k_sell_spmv(Sell A, const float* __restri
         case -6: delta_pair<6>(A, s0, b0, b1, x, y, &part); done = true; break;
         case -7: delta_pair<7>(A, s0, b0, b1, x, y, &part); done = true; break;
         case -8: delta_pair<8>(A, s0, b0, b1, x, y, &part); done = true; break;
+        case -36: delta_pair<4, true>(A, s0, b0, b1, x, y, &part); done = true; break;
+        case -37: delta_pair<5, true>(A, s0, b0, b1, x, y, &part); done = true; break;
+        case -38: delta_pair<6, true>(A, s0, b0, b1, x, y, &part); done = true; break;
+        case -39: delta_pair<7, true>(A, s0, b0, b1, x, y, &part); done = true; break;
+        case -40: delta_pair<8, true>(A, s0, b0, b1, x, y, &part); done = true; break;
         default: break;
       }
     }
@@ -395,6 +427,11 @@ __device__ __forceinline__ void delta_work(const Sell& A, const int32_t* __restr
         case -6: delta_pair<6>(A, s0, b0, b1, x, y, &part); done = true; break;
         case -7: delta_pair<7>(A, s0, b0, b1, x, y, &part); done = true; break;
         case -8: delta_pair<8>(A, s0, b0, b1, x, y, &part); done = true; break;
+        case -36: delta_pair<4, true>(A, s0, b0, b1, x, y, &part); done = true; break;
+        case -37: delta_pair<5, true>(A, s0, b0, b1, x, y, &part); done = true; break;
+        case -38: delta_pair<6, true>(A, s0, b0, b1, x, y, &part); done = true; break;
+        case -39: delta_pair<7, true>(A, s0, b0, b1, x, y, &part); done = true; break;
+        case -40: delta_pair<8, true>(A, s0, b0, b1, x, y, &part); done = true; break;
         default: break;
       }
     }
@@ -513,6 +550,11 @@ __global__ void __launch_bounds__(kBS) k_sell_both(Sell A, const int32_t* __rest
           case -6: delta_pair<6>(A, s0, b0, b1, x, y, &part); done = true; break;
           case -7: delta_pair<7>(A, s0, b0, b1, x, y, &part); done = true; break;
           case -8: delta_pair<8>(A, s0, b0, b1, x, y, &part); done = true; break;
+          case -36: delta_pair<4, true>(A, s0, b0, b1, x, y, &part); done = true; break;
+          case -37: delta_pair<5, true>(A, s0, b0, b1, x, y, &part); done = true; break;
+          case -38: delta_pair<6, true>(A, s0, b0, b1, x, y, &part); done = true; break;
+          case -39: delta_pair<7, true>(A, s0, b0, b1, x, y, &part); done = true; break;
+          case -40: delta_pair<8, true>(A, s0, b0, b1, x, y, &part); done = true; break;
           default: break;
         }
       }
@@ -730,6 +772,7 @@ Sell sell_view(const ngpb_ctx* c) {
   A.pk_lp = c->sell_pk_lp.p;
   A.pk_hdr = reinterpret_cast<const PkHdr*>(c->sell_pk_hdr.p);
   A.pk_dict = c->sell_pk_dict.p;
+  A.cval = c->sell_cval.p;
   return A;
 }
 
@@ -817,8 +860,9 @@ void build_compact(ngpb_ctx* c) {
   hipLaunchKernelGGL(k_scale, dim3(blocks_for(n, 256)), dim3(256), 0, c->stream, n, c->rowptr.p, c->col.p, c->val.p,
                      c->dscale.p);
   c->slice_width.resize((size_t)ns);
-  DBuf<int64_t> wcount, widec, wide_off, ovc, ov_off, dslots, pkc, pk_off, pkd, pk_doff, pkg, pk_goff;
+  DBuf<int64_t> wcount, widec, wide_off, ovc, ov_off, dslots, cslots, pkc, pk_off, pkd, pk_doff, pkg, pk_goff;
   DBuf<int> bad;
+  cslots.resize((size_t)ns);
   pkg.resize((size_t)ns);
   pk_goff.resize((size_t)ns);
   pkc.resize((size_t)ns);
@@ -834,8 +878,9 @@ void build_compact(ngpb_ctx* c) {
   bad.resize(1);
   NGPB_HIP(hipMemsetAsync(bad.p, 0, sizeof(int), c->stream));
   const unsigned g = (unsigned)((ns + (kBS / kW) - 1) / (kBS / kW));
-  hipLaunchKernelGGL(k_sell_shape, dim3(g), dim3(kBS), 0, c->stream, n, ns, c->rowptr.p, c->col.p, c->slice_width.p,
-                     wcount.p, widec.p, ovc.p, dslots.p, pkc.p, pkd.p, pkg.p, bad.p);
+  hipLaunchKernelGGL(k_sell_shape, dim3(g), dim3(kBS), 0, c->stream, n, ns, c->rowptr.p, c->col.p, c->val.p,
+                     c->dscale.p, c->sell_allow_const ? 1 : 0, cslots.p, c->slice_width.p, wcount.p, widec.p, ovc.p,
+                     dslots.p, pkc.p, pkd.p, pkg.p, bad.p);
   NGPB_HIP(hipGetLastError());
   int hb = 0;
   NGPB_HIP(hipMemcpyAsync(&hb, bad.p, sizeof(int), hipMemcpyDeviceToHost, c->stream));
@@ -845,6 +890,7 @@ void build_compact(ngpb_ctx* c) {
   const int64_t nwide = exclusive_scan(c, widec.p, wide_off.p, ns);
   const int64_t nov = exclusive_scan(c, ovc.p, ov_off.p, ns);
   const int64_t ndslots = exclusive_scan(c, dslots.p, dslots.p, ns);
+  const int64_t ncslots = exclusive_scan(c, cslots.p, cslots.p, ns);
   const int64_t npk = exclusive_scan(c, pkc.p, pk_off.p, ns);
   const int64_t npkd = exclusive_scan(c, pkd.p, pk_doff.p, ns);
   const int64_t npkg = exclusive_scan(c, pkg.p, pk_goff.p, ns);
@@ -857,8 +903,13 @@ void build_compact(ngpb_ctx* c) {
     NGPB_HIP(hipMemcpyAsync(wh.data(), c->slice_width.p, ns * sizeof(int32_t), hipMemcpyDeviceToHost, c->stream));
     NGPB_HIP(hipStreamSynchronize(c->stream));
     int64_t k = 0;
-    for (int32_t w : wh) k += w <= -(kPackedBase + 1) ? 1 : 0;
+    int64_t kc = 0;
+    for (int32_t w : wh) {
+      k += w <= -(kPackedBase + 1) ? 1 : 0;
+      kc += (w <= -(kConstBase + 1) && w >= -(kConstBase + kDMax)) ? 1 : 0;
+    }
     c->sell_pk_slices = k;
+    c->sell_const_slices = kc;
   }
   c->sell_pk_entries = npk;
   c->sell_pk_dict_entries = npkd;
@@ -873,6 +924,8 @@ void build_compact(ngpb_ctx* c) {
   c->sell_bases = nbases;
   c->sell_entries = nbases * kW;
   c->sell_delta_entries = ndslots * kW;
+  c->sell_const_slots = ncslots;
+  c->sell_cval.resize((size_t)std::max<int64_t>(nbases, 1));
   c->sell_wide_entries = nwide;
   c->sell_ov_entries = nov;
   c->slice_base_ptr.resize((size_t)ns);
@@ -891,7 +944,7 @@ void build_compact(ngpb_ctx* c) {
                      c->sell_off.p, c->sell_base.p, c->sell_col32.p, c->slice_wide_ptr.p, c->slice_ov_ptr.p,
                      c->sell_ov_cnt.p, c->sell_ov_val.p, c->sell_ov_col.p, pk_off.p, pk_doff.p, pk_goff.p,
                      c->sell_pk_val.p, reinterpret_cast<uint8_t*>(c->sell_pk_idx.p), c->sell_pk_lp.p, c->sell_pk_dict.p,
-                     reinterpret_cast<PkHdr*>(c->sell_pk_hdr.p));
+                     reinterpret_cast<PkHdr*>(c->sell_pk_hdr.p), c->sell_cval.p);
   NGPB_HIP(hipGetLastError());
   {
     DBuf<uint8_t> fd, fp, fo;
@@ -913,12 +966,13 @@ void build_compact(ngpb_ctx* c) {
 
 double spmv_compact_bytes(const ngpb_ctx* c) {
   // delta-slice entries (4: value), offset-slice entries (4 value + 2 offset), slot
-  // bases/deltas (4), 32-bit columns of wide slices, overflow entries (4 + 4),
+  // bases/deltas (4), constant-slice slot values (4), 32-bit columns of wide slices, overflow entries (4 + 4),
   // packed values (4), packed index words (4, four slots of a row each), packed rows
   // (permutation + count, 1 + 1), dictionary entries (4), packed-slice headers (32),
   // slice metadata (width 4 + three 8-byte offsets), x read once and y written
-  const double off_entries = (double)(c->sell_entries - c->sell_delta_entries);
+  const double off_entries = (double)(c->sell_entries - c->sell_delta_entries - c->sell_const_slots * kW);
   return (double)c->sell_delta_entries * 4.0 + off_entries * 6.0 + (double)c->sell_bases * 4.0 +
+         (double)c->sell_const_slots * 4.0 +
          (double)c->sell_wide_entries * 4.0 + (double)c->sell_ov_entries * 8.0 + (double)c->sell_pk_entries * 4.0 +
          (double)c->sell_pk_words * 4.0 + (double)c->sell_pk_slices * (kW * 2.0 + 32.0) +
          (double)c->sell_pk_dict_entries * 4.0 +

@@ -103,6 +103,7 @@ SIGNATURES = {
     "ngpb_kernel_stats": (_I, [_P, _I, _P, _P, _P]),
     "ngpb_time_copy": (_I, [_P, _I64, _I, _P]),
     "ngpb_operator_info": (_I, [_P, _P]),
+    "ngpb_set_compact_options": (_I, [_P, _I]),
     "ngpb_set_spmv_schedule": (_I, [_P, _I]),
     "ngpb_batch_create": (_I, [_P, _I, _I]),
     "ngpb_batch_destroy": (_I, [_P]),
@@ -334,11 +335,15 @@ class Context:
         return ms.value
 
     def operator_info(self):
-        info = np.zeros(10, np.int64)
+        info = np.zeros(12, np.int64)
         self._check(lib().ngpb_operator_info(self._h, _p(info)), "operator_info")
         keys = ["slices", "delta_slices", "delta_entries", "offset_entries", "wide_entries", "overflow_entries",
-                "compact_spmv_bytes", "fp64_spmv_bytes", "packed_slices", "packed_entries"]
+                "compact_spmv_bytes", "fp64_spmv_bytes", "packed_slices", "packed_entries", "constant_slices",
+                "constant_slots"]
         return dict(zip(keys, (int(v) for v in info)))
+
+    def set_compact_options(self, constant_slices: bool = True):
+        self._check(lib().ngpb_set_compact_options(self._h, 1 if constant_slices else 0), "set_compact_options")
 
     def set_spmv_schedule(self, schedule: int):
         self._check(lib().ngpb_set_spmv_schedule(self._h, int(schedule)), "set_spmv_schedule")

@@ -213,11 +213,19 @@ int ngpb_kernel_stats(ngpb_ctx* ctx, int precision, int64_t* launches, double* t
    calibrate hardware traffic counters. No counterpart in the reference. */
 int ngpb_time_copy(ngpb_ctx* ctx, int64_t bytes, int reps, double* ms_per_launch);
 /* Layout of the compact mixed-precision operator (built on demand): info[0] slices,
-   [1] delta slices, [2] stored slot entries in delta slices, [3] in offset slices,
-   [4] 32-bit column entries, [5] overflow entries, [6] algorithmic bytes of one
-   compact SpMV, [7] algorithmic bytes of one FP64 CSR SpMV, [8] packed slices,
-   [9] packed entries. No counterpart in the reference. */
-int ngpb_operator_info(ngpb_ctx* ctx, int64_t* info10);
+   [1] delta slices (constant ones included), [2] stored slot entries in delta slices
+   (constant ones excluded), [3] in offset slices, [4] 32-bit column entries,
+   [5] overflow entries, [6] algorithmic bytes of one compact SpMV, [7] algorithmic
+   bytes of one FP64 CSR SpMV, [8] packed slices, [9] packed entries, [10] constant
+   delta slices (64 rows with the same value in each slot, stored once per slot),
+   [11] their slots. No counterpart in the reference. */
+int ngpb_operator_info(ngpb_ctx* ctx, int64_t* info12);
+/* Slice forms of the compact operator: constant_slices != 0 (the default) stores a
+   delta slice whose 64 rows hold the same value in every slot as one value per slot.
+   SpMV results are the same bit for bit either way (same values, same order); the
+   operator is rebuilt on next use when the setting changes. No counterpart in the
+   reference. */
+int ngpb_set_compact_options(ngpb_ctx* ctx, int constant_slices);
 /* Work schedule of the SpMV kernels. Compact FP32: 0 one kernel per slice form
    (delta / packed / offset, each walking a list of its slices) on up to 2048 blocks,
    1 a single kernel for all forms, 2 as 0 on the resident block count, 3 (default) the

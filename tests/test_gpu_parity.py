@@ -25,6 +25,8 @@ def problem(name):
         return W.born(h=1.0)
     if name == "born_h05":
         return W.born(h=0.5)
+    if name == "born_h025":
+        return W.born(h=0.25)
     if name == "three_spheres":
         return W.three_spheres()
     if name == "three_spheres_bc1":
@@ -257,4 +259,29 @@ def test_spmv_schedules_agree(name):
         assert pb.stats["status"] == 0
         energies.append(pb.energy()["polar"])
     assert max(abs(e - energies[0]) for e in energies) <= 1e-12 * abs(energies[0])
+    pb.close()
+
+
+@pytest.mark.parametrize("name", ["born_h025"])
+def test_constant_slices_bit_identical(name):
+    """Constant delta slices (one value per slot) hold the same values as the plain
+    delta form and are evaluated in the same order: SpMV results and whole mixed solves
+    are bit-identical with and without them, and they cut the SpMV's bytes. (The
+    Born grid at h = 0.25 has 129-node lines, long enough for slices of 64 interior
+    rows; the smaller cases have none.)"""
+    P, pb = gpu_pipeline(name)
+    on = pb.ctx.operator_info()
+    assert on["constant_slices"] > 0
+    x = np.random.default_rng(7).standard_normal(P.mesh.n_nodes)
+    y_on = pb.ctx.spmv(x, native.PREC_MIXED)
+    phi_on = pb.hip_compute_electric_potential(precision=native.PREC_MIXED)
+    it_on = pb.stats["iterations"]
+    pb.ctx.set_compact_options(constant_slices=False)
+    off = pb.ctx.operator_info()
+    assert off["constant_slices"] == 0
+    assert off["compact_spmv_bytes"] - on["compact_spmv_bytes"] == 4 * 63 * on["constant_slots"]
+    y_off = pb.ctx.spmv(x, native.PREC_MIXED)
+    phi_off = pb.hip_compute_electric_potential(precision=native.PREC_MIXED)
+    assert np.array_equal(y_on, y_off)
+    assert np.array_equal(phi_on, phi_off) and pb.stats["iterations"] == it_on
     pb.close()
