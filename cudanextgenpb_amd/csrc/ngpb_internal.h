@@ -221,7 +221,7 @@ struct ngpb_ctx {
   int64_t n_list_delta = 0, n_list_packed = 0, n_list_offset = 0;
   ngpb::DBuf<int32_t> sell_list_delta, sell_list_packed, sell_list_offset;
   ngpb::DBuf<double> dscale;         // 1/sqrt(diag)
-  int spmv_sched = 3;                // SpMV work schedule (0 .. 4)
+  int spmv_sched = 4;                // SpMV work schedule (0 .. 4)
   int sell_resident = 0;             // resident blocks of the compact SpMV (0: not yet queried)
   int csr_resident = 0;              // resident blocks of the FP64 CSR SpMV (0: not yet queried)
 

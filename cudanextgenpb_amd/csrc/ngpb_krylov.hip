@@ -44,12 +44,6 @@ __global__ void __launch_bounds__(kBS) k_spmv64(int64_t n, const int32_t* __rest
   double part = 0.0;
   const int64_t nchunks = (n + kBS - 1) / kBS;
   int64_t c_begin = blockIdx.x, c_end = nchunks, c_step = gridDim.x;
-  if (SCHED == 4) {  // XCD b % 8 strides over its contiguous eighth of the chunks
-    const int64_t xcd = blockIdx.x % 8u;
-    c_begin = xcd * nchunks / 8 + blockIdx.x / 8u;
-    c_end = (xcd + 1) * nchunks / 8;
-    c_step = gridDim.x / 8u;
-  }
   if (SCHED == 1) {
     const unsigned nb = gridDim.x, b = blockIdx.x;
     const int64_t lb = (nb % 8u == 0u) ? (int64_t)((b % 8u) * (nb / 8u) + b / 8u) : (int64_t)b;
@@ -112,7 +106,7 @@ unsigned spmv64_grid(ngpb_ctx* c) {
     c->csr_resident = std::max(8, (std::max(cus, 1) * std::max(per_cu, 1)) / 8 * 8);
   }
   if (need >= c->csr_resident) return (unsigned)c->csr_resident;
-  return (unsigned)(c->spmv_sched == 1 || c->spmv_sched == 4 ? (need + 7) / 8 * 8 : need);
+  return (unsigned)(c->spmv_sched == 1 ? (need + 7) / 8 * 8 : need);
 }
 
 template <bool DOT>
@@ -120,9 +114,6 @@ void launch_spmv64(ngpb_ctx* c, unsigned grid, const double* x, double* y, doubl
   const KrylovScalars* ks = DOT ? c->ks.p : nullptr;
   if (c->spmv_sched == 1)
     hipLaunchKernelGGL((k_spmv64<DOT, 1>), dim3(grid), dim3(kBS), 0, c->stream, c->n, c->rowptr.p, c->col.p, c->val.p,
-                       x, y, ks, part);
-  else if (c->spmv_sched == 4)
-    hipLaunchKernelGGL((k_spmv64<DOT, 4>), dim3(grid), dim3(kBS), 0, c->stream, c->n, c->rowptr.p, c->col.p, c->val.p,
                        x, y, ks, part);
   else
     hipLaunchKernelGGL((k_spmv64<DOT, 0>), dim3(grid), dim3(kBS), 0, c->stream, c->n, c->rowptr.p, c->col.p, c->val.p,

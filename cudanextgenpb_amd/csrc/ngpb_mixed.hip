@@ -672,9 +672,13 @@ __global__ void __launch_bounds__(kBS) k_mx_dir_x(int64_t n, const float* __rest
   if (stop) return;
   const float alpha = (float)ks->alpha2[par];
   const float beta = (float)(RR / rho);
+  // xhat is next touched a whole iteration later, after every other stream of the
+  // iteration has passed through the caches, so it is loaded and stored non-temporally
+  // and leaves the last-level cache to the operator and the other vectors (measured:
+  // mixed solve of the 8M-unknown globular system 78.2 -> 74.7 ms)
   for (int64_t i = (int64_t)blockIdx.x * kBS + threadIdx.x; i < n; i += (int64_t)gridDim.x * kBS) {
     const float pi = ps[i];
-    xs[i] = fmaf(alpha, pi, xs[i]);
+    __builtin_nontemporal_store(fmaf(alpha, pi, __builtin_nontemporal_load(&xs[i])), &xs[i]);
     ps[i] = fmaf(beta, pi, rs[i]);
   }
 }

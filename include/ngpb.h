@@ -228,16 +228,16 @@ int ngpb_operator_info(ngpb_ctx* ctx, int64_t* info12);
 int ngpb_set_compact_options(ngpb_ctx* ctx, int constant_slices);
 /* Work schedule of the SpMV kernels. Compact FP32: 0 one kernel per slice form
    (delta / packed / offset, each walking a list of its slices) on up to 2048 blocks,
-   1 a single kernel for all forms, 2 as 0 on the resident block count, 3 (default) the
-   delta and packed slices in one launch on the resident grid, every wave alternating
+   1 a single kernel for all forms, 2 as 0 on the resident block count, 3 the delta
+   and packed slices in one launch on the resident grid, every wave alternating
    between its share of the two lists so that the two forms run side by side (then
-   the offset slices, if any), 4 as 3 with the lists cut into 8 contiguous parts,
-   one per XCD (block b runs on XCD b % 8), the blocks of an XCD interleaving
-   within its part.
+   the offset slices, if any), 4 (default) as 3 with the lists cut into 8 contiguous
+   parts, one per XCD (block b runs on XCD b % 8), the blocks of an XCD interleaving
+   within their part, so that each XCD's L2 holds the input entries of one row range.
    FP64 CSR: 0 grid-stride over row chunks on up to 2048 blocks, 1 one contiguous
    chunk range per block with XCD-aware block numbering on the resident grid
-   (measured 1.36x slower), 2 and 3 grid-stride on the resident grid, 4 grid-stride
-   within one contiguous eighth of the chunks per XCD. Results differ
+   (measured 1.36x slower), 2, 3 and 4 grid-stride on the resident grid (an
+   XCD-segmented FP64 variant measured 7% slower and was dropped). Results differ
    only in the grouping of the partial sums of (p, q). No counterpart in the
    reference. */
 int ngpb_set_spmv_schedule(ngpb_ctx* ctx, int schedule);
