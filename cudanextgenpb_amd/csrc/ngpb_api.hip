@@ -61,6 +61,7 @@ int64_t ngpb_ctx::device_bytes() const {
   b += x.bytes() + r.bytes() + z.bytes() + p.bytes() + q.bytes() + dinv.bytes();
   b += xs.bytes() + rs.bytes() + ps.bytes() + qs.bytes() + ks.bytes();
   b += red.partials.bytes() + scan_tmp.bytes();
+  for (const auto& t : tmp) b += t.bytes();
   return b;
 }
 

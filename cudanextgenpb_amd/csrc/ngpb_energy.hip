@@ -344,31 +344,35 @@ namespace {
 
 // Flux and triangle-vertex sources of the border leaves in the reference's sweep
 // order, compacted: fd[4 i] = x y z flux, vd[8 i] = x y z nx ny nz phi_sup area.
+// The arrays live in the context's energy scratch slots.
 struct Sources {
-  DBuf<double> fd, vd, leaf_charge;
-  DBuf<int64_t> counts;  // [0] flux edges, [1] triangles
+  double* fd = nullptr;
+  double* vd = nullptr;
+  double* leaf_charge = nullptr;
+  int64_t* counts = nullptr;  // [0] flux edges, [1] triangles
   int64_t nf = 0, nv = 0, nb = 0;
 };
 
 void build_sources(ngpb_ctx* c, bool ionic, Sources& S) {
   const int64_t nb = c->n_border;
   S.nb = nb;
-  S.counts.resize(2);
-  NGPB_HIP(hipMemsetAsync(S.counts.p, 0, 2 * sizeof(int64_t), c->stream));
-  S.leaf_charge.resize((size_t)std::max<int64_t>(nb, 1));
+  S.counts = tmp_buf<int64_t>(c, kTmpEnergy + 13, 2);
+  NGPB_HIP(hipMemsetAsync(S.counts, 0, 2 * sizeof(int64_t), c->stream));
+  S.leaf_charge = tmp_buf<double>(c, kTmpEnergy + 12, (size_t)std::max<int64_t>(nb, 1));
   if (nb == 0) return;
   upload_tri_table(c->stream);
   const int64_t nf = kFluxSlots * nb, nv = kTriSlots * nb;
-  DBuf<double> fsrc, vsrc, leaf_last, carry;
-  DBuf<uint8_t> fvalid, vvalid, vneed, leaf_has;
-  fsrc.resize((size_t)kFS * nf);
-  vsrc.resize((size_t)kVS * nv);
-  fvalid.resize((size_t)nf);
-  vvalid.resize((size_t)nv);
-  vneed.resize((size_t)nv);
-  leaf_last.resize((size_t)3 * nb);
-  leaf_has.resize((size_t)nb);
-  carry.resize((size_t)3 * nb);
+  struct D {
+    double* p;
+  };
+  struct U {
+    uint8_t* p;
+  };
+  D fsrc{tmp_buf<double>(c, kTmpEnergy + 0, (size_t)kFS * nf)}, vsrc{tmp_buf<double>(c, kTmpEnergy + 1, (size_t)kVS * nv)},
+      leaf_last{tmp_buf<double>(c, kTmpEnergy + 2, (size_t)3 * nb)},
+      carry{tmp_buf<double>(c, kTmpEnergy + 3, (size_t)3 * nb)};
+  U fvalid{tmp_buf<uint8_t>(c, kTmpEnergy + 4, (size_t)nf)}, vvalid{tmp_buf<uint8_t>(c, kTmpEnergy + 5, (size_t)nv)},
+      vneed{tmp_buf<uint8_t>(c, kTmpEnergy + 6, (size_t)nv)}, leaf_has{tmp_buf<uint8_t>(c, kTmpEnergy + 7, (size_t)nb)};
   NGPB_HIP(hipMemsetAsync(vneed.p, 0, nv, c->stream));
   EnergyArgs ea;
   ea.m = mesh_view_of(c);
@@ -382,7 +386,7 @@ void build_sources(ngpb_ctx* c, bool ionic, Sources& S) {
   ea.eps_out = c->pc.eps_out;
   ea.ionic = ionic ? 1 : 0;
   hipLaunchKernelGGL(k_sources, dim3(blocks_for(nb, 64)), dim3(64), 0, c->stream, ea, fsrc.p, fvalid.p, vsrc.p, vvalid.p,
-                     vneed.p, S.leaf_charge.p, leaf_last.p, leaf_has.p, S.counts.p);
+                     vneed.p, S.leaf_charge, leaf_last.p, leaf_has.p, S.counts);
   NGPB_HIP(hipGetLastError());
   if (ionic) {
     hipLaunchKernelGGL(k_normal_carry, dim3(1), dim3(1024), 0, c->stream, nb, leaf_has.p, leaf_last.p, carry.p);
@@ -391,19 +395,18 @@ void build_sources(ngpb_ctx* c, bool ionic, Sources& S) {
     NGPB_HIP(hipGetLastError());
   }
   // compact valid sources (slot order is the reference's summation order)
-  DBuf<int32_t> fidx, vidx;
-  fidx.resize((size_t)nf);
-  vidx.resize((size_t)nv);
-  S.nf = compact_flags(c, fvalid.p, nf, fidx.p);
-  S.nv = ionic ? compact_flags(c, vvalid.p, nv, vidx.p) : 0;
-  S.fd.resize((size_t)kFS * std::max<int64_t>(S.nf, 1));
-  S.vd.resize((size_t)kVS * std::max<int64_t>(S.nv, 1));
+  int32_t* fidx = tmp_buf<int32_t>(c, kTmpEnergy + 8, (size_t)nf);
+  int32_t* vidx = tmp_buf<int32_t>(c, kTmpEnergy + 9, (size_t)nv);
+  S.nf = compact_flags(c, fvalid.p, nf, fidx);
+  S.nv = ionic ? compact_flags(c, vvalid.p, nv, vidx) : 0;
+  S.fd = tmp_buf<double>(c, kTmpEnergy + 10, (size_t)kFS * std::max<int64_t>(S.nf, 1));
+  S.vd = tmp_buf<double>(c, kTmpEnergy + 11, (size_t)kVS * std::max<int64_t>(S.nv, 1));
   if (S.nf)
-    hipLaunchKernelGGL(k_gather_src<kFS>, dim3(blocks_for(S.nf, 256)), dim3(256), 0, c->stream, S.nf, fidx.p, fsrc.p,
-                       S.fd.p);
+    hipLaunchKernelGGL(k_gather_src<kFS>, dim3(blocks_for(S.nf, 256)), dim3(256), 0, c->stream, S.nf, fidx, fsrc.p,
+                       S.fd);
   if (S.nv)
-    hipLaunchKernelGGL(k_gather_src<kVS>, dim3(blocks_for(S.nv, 256)), dim3(256), 0, c->stream, S.nv, vidx.p, vsrc.p,
-                       S.vd.p);
+    hipLaunchKernelGGL(k_gather_src<kVS>, dim3(blocks_for(S.nv, 256)), dim3(256), 0, c->stream, S.nv, vidx, vsrc.p,
+                       S.vd);
   NGPB_HIP(hipGetLastError());
 }
 
@@ -606,12 +609,12 @@ void run_pot_field(ngpb_ctx* c, int P, int F, double* table_host, double out4[4]
   const bool fieldi = branch == 3;
   out4[0] = out4[1] = out4[2] = out4[3] = 0.0;
   if (A == 0) return;
-  DBuf<double> table, ce, e1, e2, fin;
-  table.resize((size_t)12 * A);
-  ce.resize((size_t)A);
-  e1.resize((size_t)A);
-  e2.resize((size_t)A);
-  fin.resize(8);
+  struct D {
+    double* p;
+  };
+  D table{tmp_buf<double>(c, kTmpPotField + 0, (size_t)12 * A)}, ce{tmp_buf<double>(c, kTmpPotField + 1, (size_t)A)},
+      e1{tmp_buf<double>(c, kTmpPotField + 2, (size_t)A)}, e2{tmp_buf<double>(c, kTmpPotField + 3, (size_t)A)},
+      fin{tmp_buf<double>(c, kTmpPotField + 4, 8)};
   NGPB_HIP(hipMemsetAsync(table.p, 0, sizeof(double) * 12 * A, c->stream));
   NGPB_HIP(hipMemsetAsync(e1.p, 0, sizeof(double) * A, c->stream));
   NGPB_HIP(hipMemsetAsync(e2.p, 0, sizeof(double) * A, c->stream));
@@ -623,25 +626,24 @@ void run_pot_field(ngpb_ctx* c, int P, int F, double* table_host, double out4[4]
   if (branch != 0) {
     build_sources(c, ionic, S);
     const int64_t nfc = pf_chunks(A, S.nf), nvc = pf_chunks(A, S.nv);
-    DBuf<double> fpart, vpart;
-    fpart.resize((size_t)4 * A * std::max<int64_t>(nfc, 1));
-    vpart.resize((size_t)4 * A * std::max<int64_t>(nvc, 1));
+    D fpart{tmp_buf<double>(c, kTmpPotField + 5, (size_t)4 * A * std::max<int64_t>(nfc, 1))},
+        vpart{tmp_buf<double>(c, kTmpPotField + 6, (size_t)4 * A * std::max<int64_t>(nvc, 1))};
     if (nfc) {
       const int64_t chunk = (S.nf + nfc - 1) / nfc;
       if (fieldp)
-        hipLaunchKernelGGL(k_pf_flux<true>, dim3(tiles, (unsigned)nfc), dim3(kNB), 0, c->stream, S.nf, S.fd.p, chunk, A,
+        hipLaunchKernelGGL(k_pf_flux<true>, dim3(tiles, (unsigned)nfc), dim3(kNB), 0, c->stream, S.nf, S.fd, chunk, A,
                            c->atom_xyz.p, fpart.p);
       else
-        hipLaunchKernelGGL(k_pf_flux<false>, dim3(tiles, (unsigned)nfc), dim3(kNB), 0, c->stream, S.nf, S.fd.p, chunk, A,
+        hipLaunchKernelGGL(k_pf_flux<false>, dim3(tiles, (unsigned)nfc), dim3(kNB), 0, c->stream, S.nf, S.fd, chunk, A,
                            c->atom_xyz.p, fpart.p);
     }
     if (nvc) {
       const int64_t chunk = (S.nv + nvc - 1) / nvc;
       if (fieldi)
-        hipLaunchKernelGGL(k_pf_vert<true>, dim3(tiles, (unsigned)nvc), dim3(kNB), 0, c->stream, S.nv, S.vd.p, chunk, A,
+        hipLaunchKernelGGL(k_pf_vert<true>, dim3(tiles, (unsigned)nvc), dim3(kNB), 0, c->stream, S.nv, S.vd, chunk, A,
                            c->atom_xyz.p, vpart.p);
       else
-        hipLaunchKernelGGL(k_pf_vert<false>, dim3(tiles, (unsigned)nvc), dim3(kNB), 0, c->stream, S.nv, S.vd.p, chunk, A,
+        hipLaunchKernelGGL(k_pf_vert<false>, dim3(tiles, (unsigned)nvc), dim3(kNB), 0, c->stream, S.nv, S.vd, chunk, A,
                            c->atom_xyz.p, vpart.p);
     }
     NGPB_HIP(hipGetLastError());
@@ -649,7 +651,7 @@ void run_pot_field(ngpb_ctx* c, int P, int F, double* table_host, double out4[4]
                        fieldp ? 1 : 0, ionic ? 1 : 0, fieldi ? 1 : 0, c->pc.constant_pol, c->pc.constant_react,
                        c->atom_q.p, table.p, e1.p, e2.p);
     NGPB_HIP(hipGetLastError());
-    hipLaunchKernelGGL(k_energy_fin, dim3(1), dim3(1024), 0, c->stream, e1.p, A, e2.p, A, S.leaf_charge.p, S.nb, fin.p);
+    hipLaunchKernelGGL(k_energy_fin, dim3(1), dim3(1024), 0, c->stream, e1.p, A, e2.p, A, S.leaf_charge, S.nb, fin.p);
   }
   hipLaunchKernelGGL(k_energy_fin, dim3(1), dim3(1024), 0, c->stream, ce.p, A, ce.p, (int64_t)0, ce.p, (int64_t)0,
                      fin.p + 4);
@@ -657,7 +659,8 @@ void run_pot_field(ngpb_ctx* c, int P, int F, double* table_host, double out4[4]
   double h[8] = {0, 0, 0, 0, 0, 0, 0, 0};
   if (branch != 0) NGPB_HIP(hipMemcpyAsync(h, fin.p, 3 * sizeof(double), hipMemcpyDeviceToHost, c->stream));
   NGPB_HIP(hipMemcpyAsync(h + 4, fin.p + 4, sizeof(double), hipMemcpyDeviceToHost, c->stream));
-  if (table_host) table.download(table_host, (size_t)12 * A, c->stream);
+  if (table_host)
+    NGPB_HIP(hipMemcpyAsync(table_host, table.p, sizeof(double) * 12 * A, hipMemcpyDeviceToHost, c->stream));
   NGPB_HIP(hipStreamSynchronize(c->stream));
   const double first_int = h[0], second_int = h[1];
   if (branch != 0) out4[0] = 0.5 * c->pc.constant_pol * first_int;
@@ -674,46 +677,42 @@ void run_energy(ngpb_ctx* c, ngpb_energy_out* out) {
   out->n_border = nb;
   if (nb == 0 || (!ionic && !polar_only)) return;
   // charged atoms, in order
-  DBuf<uint8_t> aflag;
-  DBuf<int32_t> aidx;
-  DBuf<double4> atoms;
+  double4* atoms = tmp_buf<double4>(c, kTmpEnergy + 16, (size_t)std::max<int64_t>(c->A, 1));
   int64_t nq = 0;
   if (c->A > 0) {
-    aflag.resize((size_t)c->A);
-    aidx.resize((size_t)c->A);
-    hipLaunchKernelGGL(k_charged_flags, dim3(blocks_for(c->A, 256)), dim3(256), 0, c->stream, c->A, c->atom_q.p, aflag.p);
-    nq = compact_flags(c, aflag.p, c->A, aidx.p);
-    atoms.resize((size_t)std::max<int64_t>(nq, 1));
+    uint8_t* aflag = tmp_buf<uint8_t>(c, kTmpEnergy + 14, (size_t)c->A);
+    int32_t* aidx = tmp_buf<int32_t>(c, kTmpEnergy + 15, (size_t)c->A);
+    hipLaunchKernelGGL(k_charged_flags, dim3(blocks_for(c->A, 256)), dim3(256), 0, c->stream, c->A, c->atom_q.p, aflag);
+    nq = compact_flags(c, aflag, c->A, aidx);
     if (nq)
-      hipLaunchKernelGGL(k_gather_atoms, dim3(blocks_for(nq, 256)), dim3(256), 0, c->stream, nq, aidx.p, c->atom_xyz.p,
-                         c->atom_q.p, atoms.p);
+      hipLaunchKernelGGL(k_gather_atoms, dim3(blocks_for(nq, 256)), dim3(256), 0, c->stream, nq, aidx, c->atom_xyz.p,
+                         c->atom_q.p, atoms);
   }
   Sources S;
   build_sources(c, ionic, S);
   const int64_t nfv = S.nf, nvv = S.nv;
-  DBuf<double> facc, vacc;
-  facc.resize((size_t)std::max<int64_t>(nfv, 1));
-  vacc.resize((size_t)std::max<int64_t>(nvv, 1));
+  double* facc = tmp_buf<double>(c, kTmpEnergy + 17, (size_t)std::max<int64_t>(nfv, 1));
+  double* vacc = tmp_buf<double>(c, kTmpEnergy + 18, (size_t)std::max<int64_t>(nvv, 1));
   if (nfv) {
     if (ionic)
-      hipLaunchKernelGGL(k_nbody<1>, dim3(blocks_for(nfv, kNB * kNS)), dim3(kNB), 0, c->stream, nfv, S.fd.p, nq, atoms.p,
-                         facc.p);
+      hipLaunchKernelGGL(k_nbody<1>, dim3(blocks_for(nfv, kNB * kNS)), dim3(kNB), 0, c->stream, nfv, S.fd, nq, atoms,
+                         facc);
     else
-      hipLaunchKernelGGL(k_nbody<0>, dim3(blocks_for(nfv, kNB * kNS)), dim3(kNB), 0, c->stream, nfv, S.fd.p, nq, atoms.p,
-                         facc.p);
+      hipLaunchKernelGGL(k_nbody<0>, dim3(blocks_for(nfv, kNB * kNS)), dim3(kNB), 0, c->stream, nfv, S.fd, nq, atoms,
+                         facc);
   }
   if (nvv)
-    hipLaunchKernelGGL(k_nbody<2>, dim3(blocks_for(nvv, kNB * kNS)), dim3(kNB), 0, c->stream, nvv, S.vd.p, nq, atoms.p,
-                       vacc.p);
+    hipLaunchKernelGGL(k_nbody<2>, dim3(blocks_for(nvv, kNB * kNS)), dim3(kNB), 0, c->stream, nvv, S.vd, nq, atoms,
+                       vacc);
   NGPB_HIP(hipGetLastError());
   c->en_out.resize(4);
-  hipLaunchKernelGGL(k_energy_fin, dim3(1), dim3(1024), 0, c->stream, facc.p, nfv, vacc.p, nvv, S.leaf_charge.p, nb,
+  hipLaunchKernelGGL(k_energy_fin, dim3(1), dim3(1024), 0, c->stream, facc, nfv, vacc, nvv, S.leaf_charge, nb,
                      c->en_out.p);
   NGPB_HIP(hipGetLastError());
   double h[3];
   int64_t hc[2];
   NGPB_HIP(hipMemcpyAsync(h, c->en_out.p, 3 * sizeof(double), hipMemcpyDeviceToHost, c->stream));
-  NGPB_HIP(hipMemcpyAsync(hc, S.counts.p, 2 * sizeof(int64_t), hipMemcpyDeviceToHost, c->stream));
+  NGPB_HIP(hipMemcpyAsync(hc, S.counts, 2 * sizeof(int64_t), hipMemcpyDeviceToHost, c->stream));
   NGPB_HIP(hipStreamSynchronize(c->stream));
   const double first_int = h[0], second_int = h[1];
   out->charge_pol = h[2];

@@ -3,6 +3,7 @@
 
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cstdint>
 #include <stdexcept>
 #include <string>
@@ -123,6 +124,21 @@ struct KrylovScalars {
   int status;       // 0 converged, 4 maxiter
 };
 
+// Grow-only device scratch slots held by the context and reused across calls, so that
+// the per-step path (markers, compact-operator build, energy) allocates and frees
+// nothing once its sizes have been reached (hipFree also waits for the device). Each
+// slot has one user; no slot is held across a call that uses it itself.
+enum TmpSlot : int {
+  kTmpFlagCount = 0,              // compact_flags: per-element counts
+  kTmpInside,                     // markers: node inside flags
+  kTmpBorderFlag,                 // markers: border leaf flags
+  kTmpCompact,                    // build_compact: 14 per-slice arrays
+  kTmpForms = kTmpCompact + 14,   // build_compact: 3 form flag arrays
+  kTmpEnergy = kTmpForms + 3,     // energy sources and sums: 19 arrays
+  kTmpPotField = kTmpEnergy + 19, // pot_field: 7 arrays
+  kTmpCount = kTmpPotField + 7
+};
+
 // ----------------------------------------------------------------- context
 }  // namespace ngpb
 
@@ -241,6 +257,7 @@ struct ngpb_ctx {
 
   // scan scratch
   ngpb::DBuf<int64_t> scan_tmp;
+  ngpb::DBuf<uint8_t> tmp[ngpb::kTmpCount];  // grow-only scratch slots (TmpSlot)
 
   // optional HIP-event timing of SpMV launches (kind 0: FP64 CSR, 1: compact FP32)
   bool profile = false;
@@ -304,6 +321,14 @@ double time_copy(ngpb_ctx* c, int64_t bytes, int reps);
 
 // launch helper
 inline unsigned blocks_for(int64_t n, int bs) { return (unsigned)((n + bs - 1) / bs); }
+
+// typed view of scratch slot `slot` with room for `count` elements (grows only)
+template <class T>
+T* tmp_buf(ngpb_ctx* c, int slot, size_t count) {
+  DBuf<uint8_t>& b = c->tmp[slot];
+  b.reserve(std::max<size_t>(count, 1) * sizeof(T));
+  return reinterpret_cast<T*>(b.p);
+}
 
 }  // namespace ngpb
 

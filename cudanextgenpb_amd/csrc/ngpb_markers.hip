@@ -135,9 +135,9 @@ RayView ray_view(const ngpb_ctx* c, int d) {
   return r;
 }
 
-void finish_border(ngpb_ctx* c, DBuf<uint8_t>& bflag) {
+void finish_border(ngpb_ctx* c, const uint8_t* bflag) {
   c->border.resize((size_t)std::max<int64_t>(c->L, 1));
-  c->n_border = compact_flags(c, bflag.p, c->L, c->border.p);
+  c->n_border = compact_flags(c, bflag, c->L, c->border.p);
   c->border_of.resize((size_t)c->L);
   hipLaunchKernelGGL(k_border_of, dim3(blocks_for(c->L, 256)), dim3(256), 0, c->stream, c->L, c->border_of.p);
   if (c->n_border > 0)
@@ -155,23 +155,21 @@ void run_markers(ngpb_ctx* c) {
   c->eps.resize((size_t)c->N);
   c->reac.resize((size_t)c->N);
   c->leaf_marker.resize((size_t)c->L);
-  DBuf<uint8_t> inside, bflag;
-  inside.resize((size_t)c->N);
-  bflag.resize((size_t)c->L);
+  uint8_t* inside = tmp_buf<uint8_t>(c, kTmpInside, (size_t)c->N);
+  uint8_t* bflag = tmp_buf<uint8_t>(c, kTmpBorderFlag, (size_t)c->L);
   hipLaunchKernelGGL(k_node_inside, dim3(blocks_for(c->N, 256)), dim3(256), 0, c->stream, m, ray_view(c, 2),
-                     c->pc.eps_in, c->pc.eps_out, c->pc.eps_out * c->pc.k2, c->eps.p, c->reac.p, inside.p);
-  hipLaunchKernelGGL(k_leaf_marker, dim3(blocks_for(c->L, 256)), dim3(256), 0, c->stream, m, inside.p,
-                     c->leaf_marker.p, bflag.p);
+                     c->pc.eps_in, c->pc.eps_out, c->pc.eps_out * c->pc.k2, c->eps.p, c->reac.p, inside);
+  hipLaunchKernelGGL(k_leaf_marker, dim3(blocks_for(c->L, 256)), dim3(256), 0, c->stream, m, inside,
+                     c->leaf_marker.p, bflag);
   NGPB_HIP(hipGetLastError());
   finish_border(c, bflag);
   NGPB_HIP(hipStreamSynchronize(c->stream));
 }
 
 void run_border_from_markers(ngpb_ctx* c) {
-  DBuf<uint8_t> bflag;
-  bflag.resize((size_t)c->L);
+  uint8_t* bflag = tmp_buf<uint8_t>(c, kTmpBorderFlag, (size_t)c->L);
   hipLaunchKernelGGL(k_border_flag, dim3(blocks_for(c->L, 256)), dim3(256), 0, c->stream, c->L,
-                     c->leaf_marker.p, bflag.p);
+                     c->leaf_marker.p, bflag);
   NGPB_HIP(hipGetLastError());
   finish_border(c, bflag);
   NGPB_HIP(hipStreamSynchronize(c->stream));

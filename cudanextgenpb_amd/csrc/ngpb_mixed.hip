@@ -864,22 +864,18 @@ void build_compact(ngpb_ctx* c) {
   hipLaunchKernelGGL(k_scale, dim3(blocks_for(n, 256)), dim3(256), 0, c->stream, n, c->rowptr.p, c->col.p, c->val.p,
                      c->dscale.p);
   c->slice_width.resize((size_t)ns);
-  DBuf<int64_t> wcount, widec, wide_off, ovc, ov_off, dslots, cslots, pkc, pk_off, pkd, pk_doff, pkg, pk_goff;
-  DBuf<int> bad;
-  cslots.resize((size_t)ns);
-  pkg.resize((size_t)ns);
-  pk_goff.resize((size_t)ns);
-  pkc.resize((size_t)ns);
-  pk_off.resize((size_t)ns);
-  pkd.resize((size_t)ns);
-  pk_doff.resize((size_t)ns);
-  wcount.resize((size_t)ns);
-  widec.resize((size_t)ns);
-  wide_off.resize((size_t)ns);
-  ovc.resize((size_t)ns);
-  ov_off.resize((size_t)ns);
-  dslots.resize((size_t)ns);
-  bad.resize(1);
+  // per-slice counts and offsets in the context's scratch slots
+  struct Arr {
+    int64_t* p;
+  };
+  int slot = kTmpCompact;
+  auto arr = [&]() { return Arr{tmp_buf<int64_t>(c, slot++, (size_t)ns)}; };
+  Arr wcount = arr(), widec = arr(), wide_off = arr(), ovc = arr(), ov_off = arr(), dslots = arr(), cslots = arr(),
+      pkc = arr(), pk_off = arr(), pkd = arr(), pk_doff = arr(), pkg = arr(), pk_goff = arr();
+  struct IArr {
+    int* p;
+  };
+  IArr bad{tmp_buf<int>(c, slot++, 1)};
   NGPB_HIP(hipMemsetAsync(bad.p, 0, sizeof(int), c->stream));
   const unsigned g = (unsigned)((ns + (kBS / kW) - 1) / (kBS / kW));
   hipLaunchKernelGGL(k_sell_shape, dim3(g), dim3(kBS), 0, c->stream, n, ns, c->rowptr.p, c->col.p, c->val.p,
@@ -951,10 +947,11 @@ void build_compact(ngpb_ctx* c) {
                      reinterpret_cast<PkHdr*>(c->sell_pk_hdr.p), c->sell_cval.p);
   NGPB_HIP(hipGetLastError());
   {
-    DBuf<uint8_t> fd, fp, fo;
-    fd.resize((size_t)ns);
-    fp.resize((size_t)ns);
-    fo.resize((size_t)ns);
+    struct BArr {
+      uint8_t* p;
+    };
+    BArr fd{tmp_buf<uint8_t>(c, kTmpForms, (size_t)ns)}, fp{tmp_buf<uint8_t>(c, kTmpForms + 1, (size_t)ns)},
+        fo{tmp_buf<uint8_t>(c, kTmpForms + 2, (size_t)ns)};
     hipLaunchKernelGGL(k_sell_forms, dim3(blocks_for(ns, 256)), dim3(256), 0, c->stream, ns, c->slice_width.p, fd.p,
                        fp.p, fo.p);
     c->sell_list_delta.resize((size_t)ns);

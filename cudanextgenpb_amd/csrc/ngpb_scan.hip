@@ -128,11 +128,10 @@ int64_t exclusive_scan(ngpb_ctx* c, const int64_t* d_in, int64_t* d_out, int64_t
 
 int64_t compact_flags(ngpb_ctx* c, const uint8_t* d_flag, int64_t n, int32_t* d_out) {
   if (n <= 0) return 0;
-  DBuf<int64_t> cnt;
-  cnt.resize((size_t)n);
-  hipLaunchKernelGGL(k_flags_to_counts, dim3(blocks_for(n, 256)), dim3(256), 0, c->stream, d_flag, n, cnt.p);
-  const int64_t total = exclusive_scan(c, cnt.p, cnt.p, n);
-  hipLaunchKernelGGL(k_scatter_flags, dim3(blocks_for(n, 256)), dim3(256), 0, c->stream, d_flag, cnt.p, n, d_out);
+  int64_t* cnt = tmp_buf<int64_t>(c, kTmpFlagCount, (size_t)n);
+  hipLaunchKernelGGL(k_flags_to_counts, dim3(blocks_for(n, 256)), dim3(256), 0, c->stream, d_flag, n, cnt);
+  const int64_t total = exclusive_scan(c, cnt, cnt, n);
+  hipLaunchKernelGGL(k_scatter_flags, dim3(blocks_for(n, 256)), dim3(256), 0, c->stream, d_flag, cnt, n, d_out);
   NGPB_HIP(hipGetLastError());
   NGPB_HIP(hipStreamSynchronize(c->stream));
   return total;

@@ -283,7 +283,8 @@ class Context:
               raise_on_noconv=True):
         o = SolveOpts(precision, tol, maxiter, drop, check_every)
         st = SolveStats()
-        phi = np.zeros(self.n) if want_phi else None
+        # no system yet: the library rejects the call before writing phi
+        phi = np.zeros(self.n or 0) if want_phi else None
         rc = lib().ngpb_solve(self._h, ctypes.byref(o), _p(phi), ctypes.byref(st))
         if rc != NGPB_OK and not (rc == 4 and not raise_on_noconv):
             self._check(rc, "solve")
