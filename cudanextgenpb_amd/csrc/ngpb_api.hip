@@ -49,8 +49,8 @@ int64_t ngpb_ctx::device_bytes() const {
   b += frac.bytes() + frac_normal.bytes() + frac_found.bytes();
   b += atom_xyz.bytes() + atom_q.bytes() + atom_owner.bytes();
   b += bnd_node.bytes() + bnd_val.bytes() + is_bnd.bytes() + gval.bytes();
-  b += em_rowptr.bytes() + em_key.bytes() + em_kw.bytes() + slot_em.bytes() + diag_slot.bytes() + bc_rowptr.bytes() +
-       bc_col.bytes() + bc_em.bytes();
+  b += em_id.bytes() + em_code.bytes() + slot_rel.bytes() + blk_em.bytes() + diag_slot.bytes() + bc_rowptr.bytes() +
+       bc_col.bytes() + bc_ptr.bytes() + bc_id.bytes() + bc_code.bytes();
   b += rowptr.bytes() + col.bytes() + val.bytes() + rhs.bytes() + rhs_charge.bytes() + c_edge.bytes() + v8.bytes() +
        vol_patch.bytes() + rho.bytes();
   b += slice_width.bytes() + slice_base_ptr.bytes() + slice_wide_ptr.bytes() + slice_ov_ptr.bytes() +

@@ -27,14 +27,9 @@ __constant__ int kEN[24] = {0, 1, 1, 3, 2, 3, 0, 2, 4, 5, 5, 7, 6, 7, 4, 6, 0, 4
 __constant__ int kEA[12] = {0, 1, 0, 1, 0, 1, 0, 1, 2, 2, 2, 2};
 
 constexpr double kPi = 3.14159265358979323846;
+constexpr int kSvRows = 8;  // rows per block of the repacked slot emissions
 
 __device__ __forceinline__ int ilog2_small(int v) { return v == 1 ? 0 : v == 2 ? 1 : v == 4 ? 2 : v == 8 ? 3 : 4; }
-
-__device__ __forceinline__ double decode_w(uint16_t kw) {
-  const int code = kw & 0xff;
-  const double mag = ldexp(1.0, -(code & 0x7f));
-  return (code & 0x80) ? -mag : mag;
-}
 
 struct Parents {
   int np[8];
@@ -359,6 +354,40 @@ __global__ void k_row_fill(int64_t N, const int64_t* __restrict__ em_rowptr, con
   }
 }
 
+// repacking of the slot and coupling emissions (after the symbolic sort)
+__global__ void k_range_len(int64_t n, const int64_t* __restrict__ rng, int64_t* __restrict__ len) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) len[i] = rng[2 * i + 1] - rng[2 * i];
+}
+
+__global__ void k_repack(int64_t n, const int64_t* __restrict__ rng, const int64_t* __restrict__ dst,
+                         const uint64_t* __restrict__ key, const uint16_t* __restrict__ kw, uint32_t* __restrict__ id,
+                         uint8_t* __restrict__ code) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const int64_t b = rng[2 * i], e = rng[2 * i + 1];
+  int64_t o = dst[i];
+  for (int64_t k = b; k < e; ++k, ++o) {
+    id[o] = (uint32_t)(key[k] & 0xffffffffULL);
+    code[o] = (uint8_t)(kw[k] & 0xff);
+  }
+}
+
+// blk_em[b] = first emission of the first slot of rows kSvRows b ..; blk_em[nblk] = total
+__global__ void k_blk_em(int64_t nblk, int64_t N, const int32_t* __restrict__ rowptr, const int64_t* __restrict__ sptr,
+                         int64_t* __restrict__ blk_em) {
+  const int64_t b = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (b <= nblk) blk_em[b] = sptr[rowptr[min(N, b * kSvRows)]];
+}
+
+__global__ void k_slot_rel(int64_t N, const int32_t* __restrict__ rowptr, const int64_t* __restrict__ sptr,
+                           const int64_t* __restrict__ blk_em, uint32_t* __restrict__ rel) {
+  const int64_t row = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (row >= N) return;
+  const int64_t base = blk_em[row / kSvRows];
+  for (int64_t sl = rowptr[row]; sl < rowptr[row + 1]; ++sl) rel[sl] = (uint32_t)(sptr[sl] - base);
+}
+
 // ------------------------------------------------------------------------ numeric
 __device__ __forceinline__ double wha(double eps1, double eps2, double frac) {
   return 1.0 / (frac / eps1 + (1 - frac) / eps2);
@@ -397,68 +426,94 @@ __global__ void k_leaf_coeffs(MeshView m, const double* __restrict__ eps, const 
   v8[l] = h[0] * h[1] * h[2] * 0.125;
 }
 
-__device__ __forceinline__ double emission_value(uint64_t key, uint16_t kw, int64_t row, int64_t L,
+__device__ __forceinline__ double decode_code(uint8_t code) {
+  const double mag = ldexp(1.0, -(code & 0x7f));
+  return (code & 0x80) ? -mag : mag;
+}
+
+__device__ __forceinline__ double emission_value(uint32_t id, uint8_t code, int64_t row, int64_t L,
                                                  const double* c_edge, const double* reac, const double* v8) {
-  const uint32_t id = (uint32_t)(key & 0xffffffffULL);
-  const double w = decode_w(kw);
+  const double w = decode_code(code);
   if ((int64_t)id < 12 * L) return c_edge[id] * w;
   const int64_t leaf = ((int64_t)id - 12 * L) >> 3;
   return reac[row] * v8[leaf] * w;
 }
 
-// Matrix values. A block owns kSvRows consecutive rows: their emissions (contiguous
-// after the symbolic sort) are loaded coalesced and evaluated in parallel into LDS,
-// then one thread per CSR slot adds its run of values in order, so every sum equals
-// the sequential sum of the reference's leaf sweep. Blocks whose rows hold more than
-// kSvCap emissions evaluate straight from global memory.
-constexpr int kSvRows = 32;
-constexpr int kSvCap = 4096;
-constexpr int kSvBS = 256;
+// Matrix values. A block (one wave) owns kSvRows consecutive rows (the blocks of the
+// repacked slot emissions): their emissions (contiguous, in slot order) are loaded
+// coalesced and evaluated in parallel into LDS, then one thread per CSR slot adds its
+// run of values in order, so every sum equals the sequential sum of the reference's
+// leaf sweep. Blocks whose rows hold more than kSvCap emissions evaluate straight from
+// global memory. Small blocks with 4 KB of LDS keep many waves resident; on the 8M
+// globular mesh (8 rows, 512 entries, 64 threads) the numeric assembly takes 4.6 ms
+// against 6.4 ms with 32-row blocks of 256 threads and 32 KB of LDS, with identical
+// results.
+constexpr int kSvCap = 512;
+constexpr int kSvBS = 64;
+
+// last r in [0, nr) with a[r] <= v (a[0] <= v)
+__device__ __forceinline__ int last_le(const uint32_t* a, int nr, uint32_t v) {
+  int lo = 0, hi = nr - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (a[mid] <= v) lo = mid;
+    else hi = mid - 1;
+  }
+  return lo;
+}
 
 __global__ void __launch_bounds__(kSvBS) k_slot_values(int64_t N, int64_t L, const int32_t* __restrict__ rowptr,
-                                                       const int64_t* __restrict__ em_rowptr,
-                                                       const int64_t* __restrict__ slot_em,
-                                                       const uint64_t* __restrict__ key,
-                                                       const uint16_t* __restrict__ kw,
+                                                       const int64_t* __restrict__ blk_em,
+                                                       const uint32_t* __restrict__ slot_rel,
+                                                       const uint32_t* __restrict__ em_id,
+                                                       const uint8_t* __restrict__ em_code,
                                                        const double* __restrict__ c_edge,
                                                        const double* __restrict__ reac,
                                                        const double* __restrict__ v8, double* __restrict__ val) {
   __shared__ double s_v[kSvCap];
-  __shared__ int64_t s_rp[kSvRows + 1];
-  const int64_t r0 = (int64_t)blockIdx.x * kSvRows;
+  __shared__ uint32_t s_re[kSvRows + 1];  // per row: first emission (block-relative)
+  __shared__ uint32_t s_rs[kSvRows + 1];  // per row: first slot (block-relative)
+  const int64_t blk = blockIdx.x;
+  const int64_t r0 = blk * kSvRows;
   const int64_t r1 = min(r0 + kSvRows, N);
   const int nr = (int)(r1 - r0);
-  for (int i = threadIdx.x; i <= nr; i += kSvBS) s_rp[i] = em_rowptr[r0 + i];
+  const int64_t base = blk_em[blk];
+  const uint32_t nem = (uint32_t)(blk_em[blk + 1] - base);
+  const int64_t sl0 = rowptr[r0], sl1 = rowptr[r1];
+  for (int i = threadIdx.x; i < nr; i += kSvBS) {
+    const int64_t sl = rowptr[r0 + i];
+    s_re[i] = sl < sl1 ? slot_rel[sl] : nem;
+    s_rs[i] = (uint32_t)(sl - sl0);
+  }
+  if (threadIdx.x == 0) {
+    s_re[nr] = nem;
+    s_rs[nr] = (uint32_t)(sl1 - sl0);
+  }
   __syncthreads();
-  const int64_t e0 = s_rp[0], e1 = s_rp[nr];
-  const bool staged = e1 - e0 <= kSvCap;
+  const bool staged = nem <= (uint32_t)kSvCap;
   if (staged) {
-    for (int64_t i = e0 + threadIdx.x; i < e1; i += kSvBS) {
-      // row of emission i: last r with s_rp[r] <= i
-      int lo = 0, hi = nr - 1;
-      while (lo < hi) {
-        const int mid = (lo + hi + 1) >> 1;
-        if (s_rp[mid] <= i) lo = mid;
-        else hi = mid - 1;
+    for (uint32_t i = threadIdx.x; i < nem; i += kSvBS) {
+      const uint32_t id = em_id[base + i];
+      const double w = decode_code(em_code[base + i]);
+      double v;
+      if ((int64_t)id < 12 * L) {
+        v = c_edge[id] * w;
+      } else {
+        const int r = last_le(s_re, nr, i);  // lumped mass: the row's own kappa^2
+        v = reac[r0 + r] * v8[((int64_t)id - 12 * L) >> 3] * w;
       }
-      s_v[i - e0] = emission_value(key[i], kw[i], r0 + lo, L, c_edge, reac, v8);
+      s_v[i] = v;
     }
   }
   __syncthreads();
-  const int64_t sl0 = rowptr[r0], sl1 = rowptr[r1];
   for (int64_t sl = sl0 + threadIdx.x; sl < sl1; sl += kSvBS) {
-    const int64_t b = slot_em[2 * sl], e = slot_em[2 * sl + 1];
+    const uint32_t b = slot_rel[sl], e = sl + 1 < sl1 ? slot_rel[sl + 1] : nem;
     double v = 0.0;
     if (staged) {
-      for (int64_t i = b; i < e; ++i) v += s_v[i - e0];
+      for (uint32_t i = b; i < e; ++i) v += s_v[i];
     } else {
-      int lo = 0, hi = nr - 1;
-      while (lo < hi) {
-        const int mid = (lo + hi + 1) >> 1;
-        if (s_rp[mid] <= b) lo = mid;
-        else hi = mid - 1;
-      }
-      for (int64_t i = b; i < e; ++i) v += emission_value(key[i], kw[i], r0 + lo, L, c_edge, reac, v8);
+      const int64_t row = r0 + last_le(s_rs, nr, (uint32_t)(sl - sl0));
+      for (uint32_t i = b; i < e; ++i) v += emission_value(em_id[base + i], em_code[base + i], row, L, c_edge, reac, v8);
     }
     val[sl] = v;
   }
@@ -499,20 +554,24 @@ __global__ void k_rho_node(MeshView m, int64_t C, int64_t L, const double* __res
                            const double* __restrict__ aq, const int32_t* __restrict__ owner,
                            const int32_t* __restrict__ cnode, const int32_t* __restrict__ cnext,
                            const int32_t* __restrict__ head, const int32_t* __restrict__ diag_slot,
-                           const int64_t* __restrict__ slot_em, const uint64_t* __restrict__ key,
-                           const uint16_t* __restrict__ kw, const double* __restrict__ v8,
+                           int64_t N, const int32_t* __restrict__ rowptr, const int64_t* __restrict__ blk_em,
+                           const uint32_t* __restrict__ slot_rel, const uint32_t* __restrict__ em_id,
+                           const uint8_t* __restrict__ em_code, const double* __restrict__ v8,
                            double* __restrict__ rho, double* __restrict__ rhs_charge) {
   const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= C) return;
   const int32_t node = cnode[t];
   if (node < 0 || head[node] != (int32_t)t) return;
   const int64_t ds = diag_slot[node];
-  const int64_t b = slot_em[2 * ds], e = slot_em[2 * ds + 1];
+  const int64_t blk = node / kSvRows;
+  const int64_t last_slot = rowptr[min(N, (blk + 1) * kSvRows)];
+  const int64_t b = blk_em[blk] + slot_rel[ds];
+  const int64_t e = ds + 1 < last_slot ? blk_em[blk] + slot_rel[ds + 1] : blk_em[blk + 1];
   double vp = 0.0;
   for (int64_t i = b; i < e; ++i) {
-    const uint32_t id = (uint32_t)(key[i] & 0xffffffffULL);
+    const uint32_t id = em_id[i];
     if ((int64_t)id < 12 * L) continue;
-    vp += v8[((int64_t)id - 12 * L) >> 3] * decode_w(kw[i]);
+    vp += v8[((int64_t)id - 12 * L) >> 3] * decode_code(em_code[i]);
   }
   double s = 0.0;
   int64_t last = -1;
@@ -545,9 +604,9 @@ __global__ void k_rho_node(MeshView m, int64_t C, int64_t L, const double* __res
   rho[node] = s;
   double r = 0.0;
   for (int64_t i = b; i < e; ++i) {
-    const uint32_t id = (uint32_t)(key[i] & 0xffffffffULL);
+    const uint32_t id = em_id[i];
     if ((int64_t)id < 12 * L) continue;
-    r += v8[((int64_t)id - 12 * L) >> 3] * decode_w(kw[i]) * s;
+    r += v8[((int64_t)id - 12 * L) >> 3] * decode_code(em_code[i]) * s;
   }
   rhs_charge[node] = r;
 }
@@ -555,8 +614,8 @@ __global__ void k_rho_node(MeshView m, int64_t C, int64_t L, const double* __res
 __global__ void k_dirichlet_rhs(int64_t N, int64_t L, const uint8_t* __restrict__ is_bnd,
                                 const double* __restrict__ gval, const int32_t* __restrict__ diag_slot,
                                 const double* __restrict__ val, const int32_t* __restrict__ bc_rowptr,
-                                const int32_t* __restrict__ bc_col, const int64_t* __restrict__ bc_em,
-                                const uint64_t* __restrict__ key, const uint16_t* __restrict__ kw,
+                                const int32_t* __restrict__ bc_col, const int64_t* __restrict__ bc_ptr,
+                                const uint32_t* __restrict__ bc_id, const uint8_t* __restrict__ bc_code,
                                 const double* __restrict__ c_edge, const double* __restrict__ reac,
                                 const double* __restrict__ v8, const double* __restrict__ rhs_charge,
                                 double* __restrict__ rhs) {
@@ -569,8 +628,7 @@ __global__ void k_dirichlet_rhs(int64_t N, int64_t L, const uint8_t* __restrict_
   double r = rhs_charge[row];
   for (int64_t s = bc_rowptr[row]; s < bc_rowptr[row + 1]; ++s) {
     double a = 0.0;
-    for (int64_t i = bc_em[2 * s]; i < bc_em[2 * s + 1]; ++i)
-      a += emission_value(key[i], kw[i], row, L, c_edge, reac, v8);
+    for (int64_t i = bc_ptr[s]; i < bc_ptr[s + 1]; ++i) a += emission_value(bc_id[i], bc_code[i], row, L, c_edge, reac, v8);
     r -= a * gval[bc_col[s]];
   }
   rhs[row] = r;
@@ -649,16 +707,20 @@ void run_symbolic(ngpb_ctx* c) {
   cnt.resize((size_t)N);
   NGPB_HIP(hipMemsetAsync(cnt.p, 0, N * sizeof(unsigned), c->stream));
   hipLaunchKernelGGL(k_emit<0>, dim3(blocks_for(m.L, 128)), dim3(128), 0, c->stream, m, cnt.p, nullptr, nullptr, nullptr);
-  c->em_rowptr.resize((size_t)N + 1);
-  hipLaunchKernelGGL(k_u32_to_i64, dim3(blocks_for(N, 256)), dim3(256), 0, c->stream, N, cnt.p, c->em_rowptr.p);
+  // per-row emission lists, sorted keys: needed only until the slots are cut
+  DBuf<int64_t> em_rowptr;
+  DBuf<uint64_t> em_key;  // (col << 32) | id
+  DBuf<uint16_t> em_kw;   // (k << 8) | weight code
+  em_rowptr.resize((size_t)N + 1);
+  hipLaunchKernelGGL(k_u32_to_i64, dim3(blocks_for(N, 256)), dim3(256), 0, c->stream, N, cnt.p, em_rowptr.p);
   NGPB_HIP(hipGetLastError());
-  const int64_t total = exclusive_scan(c, c->em_rowptr.p, c->em_rowptr.p, N);
-  NGPB_HIP(hipMemcpyAsync(c->em_rowptr.p + N, &total, sizeof(int64_t), hipMemcpyHostToDevice, c->stream));
-  c->em_key.resize((size_t)total);
-  c->em_kw.resize((size_t)total);
+  const int64_t total = exclusive_scan(c, em_rowptr.p, em_rowptr.p, N);
+  NGPB_HIP(hipMemcpyAsync(em_rowptr.p + N, &total, sizeof(int64_t), hipMemcpyHostToDevice, c->stream));
+  em_key.resize((size_t)total);
+  em_kw.resize((size_t)total);
   NGPB_HIP(hipMemsetAsync(cnt.p, 0, N * sizeof(unsigned), c->stream));
-  hipLaunchKernelGGL(k_emit<1>, dim3(blocks_for(m.L, 128)), dim3(128), 0, c->stream, m, cnt.p, c->em_rowptr.p,
-                     c->em_key.p, c->em_kw.p);
+  hipLaunchKernelGGL(k_emit<1>, dim3(blocks_for(m.L, 128)), dim3(128), 0, c->stream, m, cnt.p, em_rowptr.p, em_key.p,
+                     em_kw.p);
   NGPB_HIP(hipGetLastError());
   DBuf<int64_t> fcount, bcount;
   fcount.resize((size_t)N);
@@ -670,15 +732,15 @@ void run_symbolic(ngpb_ctx* c) {
     nbig.resize(1);
     NGPB_HIP(hipMemsetAsync(nbig.p, 0, sizeof(int), c->stream));
     hipLaunchKernelGGL(k_row_sort_wave, dim3(blocks_for(N, kSortBS / 64)), dim3(kSortBS), 0, c->stream, N,
-                       c->em_rowptr.p, c->em_key.p, c->em_kw.p, c->is_bnd.p, fcount.p, bcount.p, big.p, nbig.p);
+                       em_rowptr.p, em_key.p, em_kw.p, c->is_bnd.p, fcount.p, bcount.p, big.p, nbig.p);
     NGPB_HIP(hipGetLastError());
     int hbig = 0;
     NGPB_HIP(hipMemcpyAsync(&hbig, nbig.p, sizeof(int), hipMemcpyDeviceToHost, c->stream));
     NGPB_HIP(hipStreamSynchronize(c->stream));
     if (hbig > 0) {
       const size_t lds = (size_t)kSortBigCap * (8 + 8 + 2 + 2);
-      hipLaunchKernelGGL(k_row_sort_block, dim3((unsigned)hbig), dim3(kSortBS), lds, c->stream, big.p, c->em_rowptr.p,
-                         c->em_key.p, c->em_kw.p, c->is_bnd.p, fcount.p, bcount.p);
+      hipLaunchKernelGGL(k_row_sort_block, dim3((unsigned)hbig), dim3(kSortBS), lds, c->stream, big.p, em_rowptr.p,
+                         em_key.p, em_kw.p, c->is_bnd.p, fcount.p, bcount.p);
       NGPB_HIP(hipGetLastError());
     }
   }
@@ -689,16 +751,50 @@ void run_symbolic(ngpb_ctx* c) {
   c->nnz = nnz;
   c->rowptr.resize((size_t)N + 1);
   c->col.resize((size_t)std::max<int64_t>(nnz, 1));
-  c->slot_em.resize((size_t)2 * std::max<int64_t>(nnz, 1));
   c->diag_slot.resize((size_t)N);
   c->bc_rowptr.resize((size_t)N + 1);
   c->bc_col.resize((size_t)std::max<int64_t>(nbc, 1));
-  c->bc_em.resize((size_t)2 * std::max<int64_t>(nbc, 1));
-  hipLaunchKernelGGL(k_row_fill, dim3(blocks_for(N, 128)), dim3(128), 0, c->stream, N, c->em_rowptr.p, c->em_key.p,
-                     c->is_bnd.p, fcount.p, bcount.p, c->rowptr.p, c->col.p, c->slot_em.p, c->diag_slot.p,
-                     c->bc_rowptr.p, c->bc_col.p, c->bc_em.p, nnz, nbc);
+  DBuf<int64_t> slot_em, bc_em;  // 2 per slot / coupling: range in the sorted lists
+  slot_em.resize((size_t)2 * std::max<int64_t>(nnz, 1));
+  bc_em.resize((size_t)2 * std::max<int64_t>(nbc, 1));
+  hipLaunchKernelGGL(k_row_fill, dim3(blocks_for(N, 128)), dim3(128), 0, c->stream, N, em_rowptr.p, em_key.p,
+                     c->is_bnd.p, fcount.p, bcount.p, c->rowptr.p, c->col.p, slot_em.p, c->diag_slot.p,
+                     c->bc_rowptr.p, c->bc_col.p, bc_em.p, nnz, nbc);
   NGPB_HIP(hipGetLastError());
-  NGPB_HIP(hipStreamSynchronize(c->stream));
+  // repack the slot emissions in slot order, 5 bytes each (source id, weight code),
+  // with block-relative slot offsets; the couplings to Dirichlet nodes likewise
+  {
+    DBuf<int64_t> sptr;
+    sptr.resize((size_t)nnz + 1);
+    hipLaunchKernelGGL(k_range_len, dim3(blocks_for(nnz, 256)), dim3(256), 0, c->stream, nnz, slot_em.p, sptr.p);
+    const int64_t tot = exclusive_scan(c, sptr.p, sptr.p, nnz);
+    NGPB_HIP(hipMemcpyAsync(sptr.p + nnz, &tot, sizeof(int64_t), hipMemcpyHostToDevice, c->stream));
+    c->em_id.resize((size_t)std::max<int64_t>(tot, 1));
+    c->em_code.resize((size_t)std::max<int64_t>(tot, 1));
+    hipLaunchKernelGGL(k_repack, dim3(blocks_for(nnz, 256)), dim3(256), 0, c->stream, nnz, slot_em.p, sptr.p, em_key.p,
+                       em_kw.p, c->em_id.p, c->em_code.p);
+    const int64_t nblk = (N + kSvRows - 1) / kSvRows;
+    c->blk_em.resize((size_t)nblk + 1);
+    hipLaunchKernelGGL(k_blk_em, dim3(blocks_for(nblk + 1, 256)), dim3(256), 0, c->stream, nblk, N, c->rowptr.p, sptr.p,
+                       c->blk_em.p);
+    c->slot_rel.resize((size_t)std::max<int64_t>(nnz, 1));
+    hipLaunchKernelGGL(k_slot_rel, dim3(blocks_for(N, 256)), dim3(256), 0, c->stream, N, c->rowptr.p, sptr.p,
+                       c->blk_em.p, c->slot_rel.p);
+    c->bc_ptr.resize((size_t)nbc + 1);
+    int64_t btot = 0;
+    if (nbc > 0) {
+      hipLaunchKernelGGL(k_range_len, dim3(blocks_for(nbc, 256)), dim3(256), 0, c->stream, nbc, bc_em.p, c->bc_ptr.p);
+      btot = exclusive_scan(c, c->bc_ptr.p, c->bc_ptr.p, nbc);
+    }
+    NGPB_HIP(hipMemcpyAsync(c->bc_ptr.p + nbc, &btot, sizeof(int64_t), hipMemcpyHostToDevice, c->stream));
+    c->bc_id.resize((size_t)std::max<int64_t>(btot, 1));
+    c->bc_code.resize((size_t)std::max<int64_t>(btot, 1));
+    if (nbc > 0)
+      hipLaunchKernelGGL(k_repack, dim3(blocks_for(nbc, 256)), dim3(256), 0, c->stream, nbc, bc_em.p, c->bc_ptr.p,
+                         em_key.p, em_kw.p, c->bc_id.p, c->bc_code.p);
+    NGPB_HIP(hipGetLastError());
+    NGPB_HIP(hipStreamSynchronize(c->stream));
+  }
   c->have_symbolic = true;
 }
 
@@ -714,7 +810,7 @@ void run_numeric(ngpb_ctx* c) {
   hipLaunchKernelGGL(k_leaf_coeffs, dim3(blocks_for(L, 128)), dim3(128), 0, c->stream, m, c->eps.p, c->border_of.p,
                      c->frac.p, c->c_edge.p, c->v8.p);
   hipLaunchKernelGGL(k_slot_values, dim3(blocks_for(N, kSvRows)), dim3(kSvBS), 0, c->stream, N, L, c->rowptr.p,
-                     c->em_rowptr.p, c->slot_em.p, c->em_key.p, c->em_kw.p, c->c_edge.p, c->reac.p, c->v8.p, c->val.p);
+                     c->blk_em.p, c->slot_rel.p, c->em_id.p, c->em_code.p, c->c_edge.p, c->reac.p, c->v8.p, c->val.p);
   NGPB_HIP(hipGetLastError());
   // charge density and its right-hand side, on the nodes the atoms touch
   NGPB_HIP(hipMemsetAsync(c->rhs_charge.p, 0, N * sizeof(double), c->stream));
@@ -728,14 +824,14 @@ void run_numeric(ngpb_ctx* c) {
     hipLaunchKernelGGL(k_rho_push, dim3(blocks_for(C, 256)), dim3(256), 0, c->stream, m, c->A, c->atom_owner.p, cnode.p,
                        cnext.p, head.p);
     hipLaunchKernelGGL(k_rho_node, dim3(blocks_for(C, 256)), dim3(256), 0, c->stream, m, C, L, c->atom_xyz.p,
-                       c->atom_q.p, c->atom_owner.p, cnode.p, cnext.p, head.p, c->diag_slot.p, c->slot_em.p,
-                       c->em_key.p, c->em_kw.p, c->v8.p, c->rho.p, c->rhs_charge.p);
+                       c->atom_q.p, c->atom_owner.p, cnode.p, cnext.p, head.p, c->diag_slot.p, N, c->rowptr.p,
+                       c->blk_em.p, c->slot_rel.p, c->em_id.p, c->em_code.p, c->v8.p, c->rho.p, c->rhs_charge.p);
     NGPB_HIP(hipGetLastError());
     NGPB_HIP(hipStreamSynchronize(c->stream));
   }
   hipLaunchKernelGGL(k_dirichlet_rhs, dim3(blocks_for(N, 128)), dim3(128), 0, c->stream, N, L, c->is_bnd.p,
-                     c->gval.p, c->diag_slot.p, c->val.p, c->bc_rowptr.p, c->bc_col.p, c->bc_em.p, c->em_key.p,
-                     c->em_kw.p, c->c_edge.p, c->reac.p, c->v8.p, c->rhs_charge.p, c->rhs.p);
+                     c->gval.p, c->diag_slot.p, c->val.p, c->bc_rowptr.p, c->bc_col.p, c->bc_ptr.p, c->bc_id.p,
+                     c->bc_code.p, c->c_edge.p, c->reac.p, c->v8.p, c->rhs_charge.p, c->rhs.p);
   NGPB_HIP(hipGetLastError());
   NGPB_HIP(hipStreamSynchronize(c->stream));
   c->have_system = true;

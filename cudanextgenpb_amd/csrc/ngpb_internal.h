@@ -193,14 +193,21 @@ struct ngpb_ctx {
 
   // symbolic assembly (depends on mesh + Dirichlet set)
   bool have_symbolic = false;
-  ngpb::DBuf<int64_t> em_rowptr;  // N+1
-  ngpb::DBuf<uint64_t> em_key;    // (col << 32) | id
-  ngpb::DBuf<uint16_t> em_kw;     // (k << 8) | weight code
-  ngpb::DBuf<int64_t> slot_em;    // 2 per final slot: emission range
+  // The sorted emission lists are cut into slots, then repacked in slot order with
+  // only what the numeric phase reads (source id and weight code, 5 bytes); the sort
+  // keys are released. Slot s of a row in 8-row block b starts at
+  // blk_em[b] + slot_rel[s] and ends where slot s + 1 starts (blk_em[b + 1] for the
+  // block's last slot).
+  ngpb::DBuf<uint32_t> em_id;     // slot emissions: source id (edge 12 l + e, corner 12 L + 8 l + i)
+  ngpb::DBuf<uint8_t> em_code;    //                 weight code
+  ngpb::DBuf<uint32_t> slot_rel;  // nnz: first emission of the slot, relative to its block
+  ngpb::DBuf<int64_t> blk_em;     // ceil(N / 8) + 1: first emission of each 8-row block
   ngpb::DBuf<int32_t> diag_slot;  // N
   ngpb::DBuf<int32_t> bc_rowptr;  // N+1
   ngpb::DBuf<int32_t> bc_col;
-  ngpb::DBuf<int64_t> bc_em;      // 2 per coupling
+  ngpb::DBuf<int64_t> bc_ptr;     // couplings + 1: emission range of each Dirichlet coupling
+  ngpb::DBuf<uint32_t> bc_id;     // coupling emissions: source id
+  ngpb::DBuf<uint8_t> bc_code;    //                     weight code
 
   // assembled system
   bool have_system = false;

@@ -48,18 +48,24 @@ class Request:
 
 
 def predict_bytes(problem: W.Problem) -> int:
-    """Upper estimate of the device bytes one system holds through markers, assembly,
-    mixed solve and energy, from the mesh sizes and the atoms alone (before anything
-    is uploaded). Per node: coordinates 24, markers 16, Dirichlet 9, symbolic emission
-    lists ~8 entries x 26 B, CSR ~7 x 12, compact operator ~7 x 6, solver vectors 72 and
-    reduction/scan scratch 64; per leaf: corners 32, marker and border map 5, assembly
-    coefficients 104; per border leaf (fractions, normals and the energy's source and
-    accumulator arrays) 3.6 KB, with the border leaves bounded by the atoms' summed
-    sphere areas over the finest leaf face (the surface of their union is smaller)."""
+    """Upper estimate of the device bytes one system holds at its peak (the symbolic
+    assembly, when the sorted emission lists and their repacked copy coexist) through
+    markers, assembly, mixed solve and energy, from the mesh sizes and the atoms alone
+    (before anything is uploaded).
+    Per node: coordinates 24, markers 16, Dirichlet 9, row pointers and diagonal slots
+    12, right-hand sides 24, scaling 8, FP64 and FP32 solver vectors 64, scratch 10,
+    symbolic temporaries 210; per matrix entry (at most 8 per node outside hanging
+    regions, 12 allowed): CSR 12, block-relative slot offset 4, compact operator <= 6,
+    slot ranges during the symbolic phase 24; per leaf: corners 32, marker and border
+    map 5, conductances 104, and 56 emissions of 5 bytes kept plus 10 bytes while
+    sorting; per hanging node 64; per border leaf (fractions, normals and the energy's
+    sources and accumulators) 3.6 KB, with the border leaves bounded by the atoms'
+    summed sphere areas over the finest leaf face (the surface of their union is
+    smaller)."""
     mesh = problem.mesh
     N, L, H = mesh.n_nodes, mesh.n_leaves, mesh.n_hang
-    per_node = 24 + 16 + 9 + 8 * 26 + 7 * 12 + 8 + 7 * 6 + 72 + 64
-    per_leaf = 32 + 5 + 104
+    per_node = 24 + 16 + 9 + 12 + 24 + 8 + 64 + 10 + 210 + 12 * (12 + 4 + 6 + 24)
+    per_leaf = 32 + 5 + 104 + 56 * (5 + 10)
     area = float(np.sum(4.0 * np.pi * problem.atoms.r ** 2))
     border = min(L, int(area / (mesh.hf * mesh.hf)) + 1)
     return int(1.25 * (per_node * N + per_leaf * L + 64 * H + 3600 * border) + (8 << 20))
