@@ -65,7 +65,7 @@ def run_steps(pb, precision, steps, tol, check_every, profile):
         s0 = time.perf_counter()
         pb.create_markers()
         ctx.assemble()
-        _, st = ctx.solve(precision, tol=tol, maxiter=200000, drop=0.05, check_every=check_every, want_phi=False)
+        _, st = ctx.solve(precision, tol=tol, maxiter=200000, drop=1e-3, check_every=check_every, want_phi=False)
         E = ctx.energy()
         lat.append((time.perf_counter() - s0) * 1e3)
         iters.append(st["iterations"])

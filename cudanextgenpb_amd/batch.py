@@ -53,7 +53,7 @@ class BatchedSolver:
     """Solves a set of prepared systems together; results per system name."""
 
     def __init__(self, device: int = 0, max_systems: int = 1024, tol: float = 1e-10, maxiter: int = 100000,
-                 drop: float = 0.05, inner_per_replay: int = 8):
+                 drop: float = 1e-3, inner_per_replay: int = 8):
         self.batch = native.Batch(device, max_systems, tol, maxiter, drop, inner_per_replay)
         self.slots: Dict[int, Prepared] = {}
 

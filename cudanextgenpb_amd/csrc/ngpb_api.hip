@@ -498,7 +498,7 @@ int ngpb_parse_lis_options(const char* options, ngpb_solve_opts* o) {
   o->precision = NGPB_PREC_FP64;
   o->tol = 1e-12;  // LIS default
   o->maxiter = 1000;  // LIS default
-  o->drop = 0.05;
+  o->drop = 1e-3;
   o->check_every = 32;
   std::istringstream is(options);
   std::vector<std::string> tok;

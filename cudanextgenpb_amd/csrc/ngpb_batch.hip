@@ -354,7 +354,7 @@ struct ngpb_batch {
   int cap = 0;
   std::string err;
   // options
-  double tol = 1e-10, drop = 0.05;
+  double tol = 1e-10, drop = 1e-3;
   int maxiter = 100000;
   int inner_per_replay = 8;
   // tuning
@@ -559,7 +559,7 @@ int ngpb_batch_set_options(ngpb_batch* b, const ngpb_solve_opts* o, int inner_pe
     NGPB_REQUIRE(o->tol > 0.0 && o->maxiter > 0, NGPB_E_ARG, "tol and maxiter must be positive");
     NGPB_REQUIRE(inner_per_replay >= 1 && inner_per_replay <= 256, NGPB_E_ARG, "inner_per_replay must be in [1, 256]");
     b->tol = o->tol;
-    b->drop = (o->drop > 0.0 && o->drop < 1.0) ? o->drop : 0.05;
+    b->drop = (o->drop > 0.0 && o->drop < 1.0) ? o->drop : 1e-3;
     b->maxiter = o->maxiter;
     if (b->inner_per_replay != inner_per_replay) ngpb::drop_graph(b);
     b->inner_per_replay = inner_per_replay;

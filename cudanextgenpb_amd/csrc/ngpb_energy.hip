@@ -9,11 +9,12 @@
 //                 (classifyCube + getTriangles) with per-vertex phi0, normal and area.
 //   normal carry  normal_intersection only overwrites the normal when a crossing is
 //                 found, so a vertex without a crossing reuses the last normal written
-//                 in the reference's sequential sweep (possibly in an earlier leaf); a
-//                 single-block scan supplies that carried normal.
+//                 in the reference's sequential sweep (possibly in an earlier leaf),
+//                 found by walking back over the border leaves.
 //   k_nbody       one thread per source, charged atoms (|q| > 1e-5) staged in LDS,
 //                 FP64 sums in atom order.
-//   k_energy_fin  fixed-order reduction of the per-source sums; energies in FP64.
+//   k_sum3_*      fixed-order two-level reduction of the per-source sums; energies in
+//                 FP64.
 #include <algorithm>
 #include <cmath>
 #include <cstring>
@@ -186,42 +187,18 @@ __global__ void k_sources(EnergyArgs a, double* __restrict__ fsrc, uint8_t* __re
   if (ntri) atomicAdd((unsigned long long*)&counts[1], (unsigned long long)ntri);
 }
 
-// carry[b] = normal last written before leaf b (zero before the first one).
-// One block: each thread owns a contiguous chunk of leaves.
-__global__ void __launch_bounds__(1024) k_normal_carry(int64_t nb, const uint8_t* __restrict__ has,
-                                                       const double* __restrict__ last, double* __restrict__ carry) {
-  __shared__ int64_t s_idx[1024];
-  const int T = 1024;
-  const int64_t chunk = (nb + T - 1) / T;
-  const int64_t b0 = threadIdx.x * chunk, b1 = min(nb, b0 + chunk);
-  int64_t li = -1;
-  for (int64_t b = b0; b < b1; ++b)
-    if (has[b]) li = b;
-  s_idx[threadIdx.x] = li;
-  __syncthreads();
-  // exclusive "last valid" prefix over the chunks (serial over 1024 entries by thread 0)
-  if (threadIdx.x == 0) {
-    int64_t run = -1;
-    for (int t = 0; t < T; ++t) {
-      const int64_t v = s_idx[t];
-      s_idx[t] = run;
-      if (v >= 0) run = v;
-    }
-  }
-  __syncthreads();
-  int64_t cur = s_idx[threadIdx.x];
-  for (int64_t b = b0; b < b1; ++b) {
-    for (int d = 0; d < 3; ++d) carry[3 * b + d] = cur >= 0 ? last[3 * cur + d] : 0.0;
-    if (has[b]) cur = b;
-  }
-}
-
+// A triangle vertex whose leaf has not written a normal before it takes the normal
+// last written in an earlier leaf of the sweep (normal_intersection leaves its output
+// unchanged when it finds no crossing): the search walks back to the last leaf that
+// wrote one (almost always the previous leaf); zero if there is none.
 __global__ void k_apply_carry(int64_t nv, const uint8_t* __restrict__ vvalid, const uint8_t* __restrict__ vneed,
-                              const double* __restrict__ carry, double* __restrict__ vsrc) {
+                              const uint8_t* __restrict__ has, const double* __restrict__ last,
+                              double* __restrict__ vsrc) {
   const int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (s >= nv || !vvalid[s] || !vneed[s]) return;
-  const int64_t b = s / kTriSlots;
-  for (int d = 0; d < 3; ++d) vsrc[kVS * s + 3 + d] = carry[3 * b + d];
+  int64_t b = s / kTriSlots - 1;
+  while (b >= 0 && !has[b]) --b;
+  for (int d = 0; d < 3; ++d) vsrc[kVS * s + 3 + d] = b >= 0 ? last[3 * b + d] : 0.0;
 }
 
 template <int W>
@@ -308,19 +285,41 @@ __global__ void __launch_bounds__(kNB) k_nbody(int64_t ns, const double* __restr
   }
 }
 
-// out[0] = sum of a[0..na), out[1] = sum of b[0..nb), out[2] = sum of c[0..nc)
-__global__ void __launch_bounds__(1024) k_energy_fin(const double* __restrict__ a, int64_t na, const double* __restrict__ b,
-                                                     int64_t nbv, const double* __restrict__ c, int64_t nc,
-                                                     double* __restrict__ out) {
-  __shared__ double s_red[1024 / 64];
+// Sums of three arrays in a fixed order: block g adds the g-th of kSumBlocks equal
+// contiguous ranges of each array, then one block adds the block sums in block order.
+constexpr int kSumBlocks = 256;
+
+__global__ void __launch_bounds__(256) k_sum3_part(const double* __restrict__ a, int64_t na, const double* __restrict__ b,
+                                                   int64_t nbv, const double* __restrict__ c, int64_t nc,
+                                                   double* __restrict__ part) {
+  __shared__ double s_red[256 / 64];
   const double* arr[3] = {a, b, c};
   const int64_t cnt[3] = {na, nbv, nc};
   for (int k = 0; k < 3; ++k) {
+    const int64_t chunk = (cnt[k] + kSumBlocks - 1) / kSumBlocks;
+    const int64_t i0 = (int64_t)blockIdx.x * chunk, i1 = min(cnt[k], i0 + chunk);
     double s = 0.0;
-    for (int64_t i = threadIdx.x; i < cnt[k]; i += 1024) s += arr[k][i];
-    s = dev::block_sum<1024>(s, s_red);
+    for (int64_t i = i0 + threadIdx.x; i < i1; i += 256) s += arr[k][i];
+    s = dev::block_sum<256>(s, s_red);
+    if (threadIdx.x == 0) part[k * kSumBlocks + blockIdx.x] = s;
+  }
+}
+
+__global__ void __launch_bounds__(256) k_sum3_fin(const double* __restrict__ part, double* __restrict__ out) {
+  __shared__ double s_red[256 / 64];
+  for (int k = 0; k < 3; ++k) {
+    const double s = dev::block_sum<256>(part[k * kSumBlocks + threadIdx.x], s_red);
     if (threadIdx.x == 0) out[k] = s;
   }
+}
+
+// out[0..2] = sums of a[0..na), b[0..nb), c[0..nc) (partials in scratch slot `slot`)
+void sum3(ngpb_ctx* c, int slot, const double* a, int64_t na, const double* b, int64_t nb, const double* cc,
+          int64_t nc, double* out) {
+  double* part = tmp_buf<double>(c, slot, 3 * kSumBlocks);
+  hipLaunchKernelGGL(k_sum3_part, dim3(kSumBlocks), dim3(256), 0, c->stream, a, na, b, nb, cc, nc, part);
+  hipLaunchKernelGGL(k_sum3_fin, dim3(1), dim3(256), 0, c->stream, part, out);
+  NGPB_HIP(hipGetLastError());
 }
 
 void upload_tri_table(hipStream_t s) {
@@ -369,8 +368,7 @@ void build_sources(ngpb_ctx* c, bool ionic, Sources& S) {
     uint8_t* p;
   };
   D fsrc{tmp_buf<double>(c, kTmpEnergy + 0, (size_t)kFS * nf)}, vsrc{tmp_buf<double>(c, kTmpEnergy + 1, (size_t)kVS * nv)},
-      leaf_last{tmp_buf<double>(c, kTmpEnergy + 2, (size_t)3 * nb)},
-      carry{tmp_buf<double>(c, kTmpEnergy + 3, (size_t)3 * nb)};
+      leaf_last{tmp_buf<double>(c, kTmpEnergy + 2, (size_t)3 * nb)};
   U fvalid{tmp_buf<uint8_t>(c, kTmpEnergy + 4, (size_t)nf)}, vvalid{tmp_buf<uint8_t>(c, kTmpEnergy + 5, (size_t)nv)},
       vneed{tmp_buf<uint8_t>(c, kTmpEnergy + 6, (size_t)nv)}, leaf_has{tmp_buf<uint8_t>(c, kTmpEnergy + 7, (size_t)nb)};
   NGPB_HIP(hipMemsetAsync(vneed.p, 0, nv, c->stream));
@@ -389,9 +387,8 @@ void build_sources(ngpb_ctx* c, bool ionic, Sources& S) {
                      vneed.p, S.leaf_charge, leaf_last.p, leaf_has.p, S.counts);
   NGPB_HIP(hipGetLastError());
   if (ionic) {
-    hipLaunchKernelGGL(k_normal_carry, dim3(1), dim3(1024), 0, c->stream, nb, leaf_has.p, leaf_last.p, carry.p);
-    hipLaunchKernelGGL(k_apply_carry, dim3(blocks_for(nv, 256)), dim3(256), 0, c->stream, nv, vvalid.p, vneed.p, carry.p,
-                       vsrc.p);
+    hipLaunchKernelGGL(k_apply_carry, dim3(blocks_for(nv, 256)), dim3(256), 0, c->stream, nv, vvalid.p, vneed.p,
+                       leaf_has.p, leaf_last.p, vsrc.p);
     NGPB_HIP(hipGetLastError());
   }
   // compact valid sources (slot order is the reference's summation order)
@@ -651,10 +648,9 @@ void run_pot_field(ngpb_ctx* c, int P, int F, double* table_host, double out4[4]
                        fieldp ? 1 : 0, ionic ? 1 : 0, fieldi ? 1 : 0, c->pc.constant_pol, c->pc.constant_react,
                        c->atom_q.p, table.p, e1.p, e2.p);
     NGPB_HIP(hipGetLastError());
-    hipLaunchKernelGGL(k_energy_fin, dim3(1), dim3(1024), 0, c->stream, e1.p, A, e2.p, A, S.leaf_charge, S.nb, fin.p);
+    sum3(c, kTmpEnergy + 3, e1.p, A, e2.p, A, S.leaf_charge, S.nb, fin.p);
   }
-  hipLaunchKernelGGL(k_energy_fin, dim3(1), dim3(1024), 0, c->stream, ce.p, A, ce.p, (int64_t)0, ce.p, (int64_t)0,
-                     fin.p + 4);
+  sum3(c, kTmpEnergy + 3, ce.p, A, ce.p, 0, ce.p, 0, fin.p + 4);
   NGPB_HIP(hipGetLastError());
   double h[8] = {0, 0, 0, 0, 0, 0, 0, 0};
   if (branch != 0) NGPB_HIP(hipMemcpyAsync(h, fin.p, 3 * sizeof(double), hipMemcpyDeviceToHost, c->stream));
@@ -706,8 +702,7 @@ void run_energy(ngpb_ctx* c, ngpb_energy_out* out) {
                        vacc);
   NGPB_HIP(hipGetLastError());
   c->en_out.resize(4);
-  hipLaunchKernelGGL(k_energy_fin, dim3(1), dim3(1024), 0, c->stream, facc, nfv, vacc, nvv, S.leaf_charge, nb,
-                     c->en_out.p);
+  sum3(c, kTmpEnergy + 3, facc, nfv, vacc, nvv, S.leaf_charge, nb, c->en_out.p);
   NGPB_HIP(hipGetLastError());
   double h[3];
   int64_t hc[2];

@@ -1031,7 +1031,7 @@ void solve_mixed(ngpb_ctx* c, const ngpb_solve_opts& o, ngpb_solve_stats& st) {
   int64_t launch = 0;                                  // iteration launches (parity)
   KrylovScalars* ks = c->ks.p;
   KrylovScalars* h = c->ks_host;
-  const double drop = (o.drop > 0.0 && o.drop < 1.0) ? o.drop : 0.05;
+  const double drop = (o.drop > 0.0 && o.drop < 1.0) ? o.drop : 1e-3;
   const int chunk = std::max(1, o.check_every);
 
   hipEvent_t e0, e1;

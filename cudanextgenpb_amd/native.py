@@ -279,7 +279,7 @@ class Context:
         self._check(lib().ngpb_set_csr(self._h, b.shape[0], cl.shape[0], _p(rp), _p(cl), _p(vl), _p(b)), "set_csr")
         self._size()
 
-    def solve(self, precision=PREC_FP64, tol=1e-10, maxiter=100000, drop=0.05, check_every=32, want_phi=True,
+    def solve(self, precision=PREC_FP64, tol=1e-10, maxiter=100000, drop=1e-3, check_every=32, want_phi=True,
               raise_on_noconv=True):
         o = SolveOpts(precision, tol, maxiter, drop, check_every)
         st = SolveStats()
@@ -364,7 +364,7 @@ class Batch:
     """
 
     def __init__(self, device: int = 0, max_systems: int = 1024, tol: float = 1e-10, maxiter: int = 100000,
-                 drop: float = 0.05, inner_per_replay: int = 8):
+                 drop: float = 1e-3, inner_per_replay: int = 8):
         self._h = ctypes.c_void_p()
         rc = lib().ngpb_batch_create(ctypes.byref(self._h), device, max_systems)
         if rc != NGPB_OK:
@@ -395,7 +395,7 @@ class Batch:
     def __exit__(self, *a):
         self.close()
 
-    def set_options(self, tol=1e-10, maxiter=100000, drop=0.05, inner_per_replay=8):
+    def set_options(self, tol=1e-10, maxiter=100000, drop=1e-3, inner_per_replay=8):
         o = SolveOpts(PREC_MIXED, tol, maxiter, drop, 1)
         self._check(lib().ngpb_batch_set_options(self._h, ctypes.byref(o), int(inner_per_replay)), "batch_set_options")
 

@@ -59,7 +59,7 @@ class PoissonBoltzmann:
 
     # lis_compute_electric_potential (src/pb_class.cpp:2243-2329) on the device
     def hip_compute_electric_potential(self, options: str = DEFAULT_SOLVER_OPTIONS, precision: Optional[int] = None,
-                                       drop: float = 0.05, check_every: int = 32):
+                                       drop: float = 1e-3, check_every: int = 32):
         o = native.parse_lis_options(options)
         prec = o.precision if precision is None else precision
         self.phi, self.stats = self.ctx.solve(prec, o.tol, o.maxiter, drop, check_every)

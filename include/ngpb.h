@@ -73,7 +73,7 @@ typedef struct {
   double tol;        /* stop when ||b - A x||_2 <= tol * ||b||_2 (x0 = 0) */
   int maxiter;       /* limit on matrix-vector products (FP64) or inner iterations (mixed) */
   double drop;       /* mixed: FP64 residual replacement once the inner residual has
-                        dropped by this factor since the last replacement (e.g. 0.05) */
+                        dropped by this factor since the last replacement (default 1e-3) */
   int check_every;   /* iterations launched between host polls of the stop flag (>= 1) */
 } ngpb_solve_opts;
 
