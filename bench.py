@@ -179,6 +179,8 @@ def bench_batch(args):
             "solve_s": batch_s,
             "replays": timing["replays"],
             "device_ms": timing["ms_total"],
+            "solve_all_s": bs.last_timing["solve_all_s"],
+            "energies_s": bs.last_timing["energies_s"],
             "inner_iterations_total": iters,
             "dof_iter_per_s": dof_it / batch_s,
             "autotune": tune,

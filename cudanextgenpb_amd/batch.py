@@ -70,7 +70,9 @@ class BatchedSolver:
         """Runs the batch until every system is done; returns per-system stats (and
         energies) and removes the systems from the batch. The energies run on the
         systems' own streams from several host threads, so they overlap on the device."""
+        t0 = time.perf_counter()
         self.batch.solve_all()
+        t1 = time.perf_counter()
         out = {}
         done_ctx = []
         for slot, p in list(self.slots.items()):
@@ -83,6 +85,7 @@ class BatchedSolver:
             with ThreadPoolExecutor(max_workers=max(1, energy_threads)) as pool:
                 for p, e in zip(done_ctx, pool.map(lambda q: q.ctx.energy(), done_ctx)):
                     out[p.name]["energy"] = e
+        self.last_timing = dict(solve_all_s=t1 - t0, energies_s=time.perf_counter() - t1)
         return out
 
     def close(self):
