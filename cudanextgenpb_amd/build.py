@@ -30,6 +30,7 @@ HIP_SOURCES = [
     "ngpb_mixed.hip",
     "ngpb_energy.hip",
     "ngpb_batch.hip",
+    "ngpb_rays.hip",
 ]
 
 # -ffp-contract=off: every multiply and add is rounded on its own unless the source

@@ -284,6 +284,35 @@ int ngpb_set_rays(ngpb_ctx* c, int axis, int64_t n_rays, const double* key, cons
   });
 }
 
+int ngpb_cast_rays(ngpb_ctx* c, int64_t n_balls, const double* xyz, const double* r) {
+  return guarded(c, [&] {
+    not_in_batch(c);
+    need(c->have_mesh, "ngpb_set_mesh must come before ngpb_cast_rays");
+    NGPB_REQUIRE(n_balls >= 0 && n_balls < INT32_MAX, NGPB_E_ARG, "bad ball count");
+    NGPB_REQUIRE(n_balls == 0 || (xyz && r), NGPB_E_ARG, "NULL ball array");
+    for (int64_t a = 0; a < n_balls; ++a)
+      NGPB_REQUIRE(std::isfinite(r[a]) && r[a] >= 0 && std::isfinite(xyz[3 * a]) && std::isfinite(xyz[3 * a + 1]) &&
+                       std::isfinite(xyz[3 * a + 2]),
+                   NGPB_E_ARG, "ball centres and radii must be finite, radii >= 0");
+    ngpb::run_cast_rays(c, n_balls, xyz, r);
+    c->have_markers = c->have_frac = c->have_system = c->have_compact = c->have_phi = false;
+  });
+}
+
+int ngpb_get_rays(ngpb_ctx* c, int axis, int64_t* sizes, double* key, int64_t* ptr, double* inters, double* normals) {
+  return guarded(c, [&] {
+    NGPB_REQUIRE(axis >= 0 && axis < 3 && sizes, NGPB_E_ARG, "bad arguments");
+    const auto& R = c->rays[axis];
+    sizes[0] = R.n;
+    sizes[1] = R.k;
+    if (key && R.n) R.key.download(key, (size_t)(2 * R.n), c->stream);
+    if (ptr) R.ptr.download(ptr, (size_t)(R.n + 1), c->stream);
+    if (inters && R.k) R.inters.download(inters, (size_t)R.k, c->stream);
+    if (normals && R.k) R.normals.download(normals, (size_t)(3 * R.k), c->stream);
+    NGPB_HIP(hipStreamSynchronize(c->stream));
+  });
+}
+
 int ngpb_create_markers(ngpb_ctx* c) {
   return guarded(c, [&] {
     not_in_batch(c);

@@ -297,6 +297,7 @@ void run_symbolic(ngpb_ctx* c);
 void run_numeric(ngpb_ctx* c);
 void run_dirichlet_values(ngpb_ctx* c, int bc_type);
 void run_phi_on_atoms(ngpb_ctx* c, double* d_out);
+void run_cast_rays(ngpb_ctx* c, int64_t n_balls, const double* h_xyz, const double* h_r);
 
 // Krylov
 unsigned vec_grid(int64_t n);

@@ -97,6 +97,8 @@ SIGNATURES = {
     "ngpb_spmv": (_I, [_P, _I, _P, _P]),
     "ngpb_energy": (_I, [_P, _P]),
     "ngpb_phi_on_atoms": (_I, [_P, _P]),
+    "ngpb_cast_rays": (_I, [_P, _I64, _P, _P]),
+    "ngpb_get_rays": (_I, [_P, _I, _P, _P, _P, _P, _P]),
     "ngpb_pot_field": (_I, [_P, _I, _I, _P, _P]),
     "ngpb_time_spmv": (_I, [_P, _I, _I, _P, _P]),
     "ngpb_set_profiling": (_I, [_P, _I]),
@@ -221,6 +223,23 @@ class Context:
             key, ptr = _c(r.key, np.float64), _c(r.ptr, np.int64)
             it, nm = _c(r.inters, np.float64), _c(r.normals, np.float64)
             self._check(lib().ngpb_set_rays(self._h, axis, r.n, _p(key), _p(ptr), _p(it), _p(nm)), "set_rays")
+
+    def cast_rays(self, xyz, r):
+        """Ray table of the union of balls, cast on the device (replaces set_rays)."""
+        xyz = _c(xyz, np.float64)
+        r = _c(r, np.float64)
+        self._check(lib().ngpb_cast_rays(self._h, int(r.shape[0]), _p(xyz), _p(r)), "cast_rays")
+
+    def get_rays(self, axis: int):
+        """(key (R, 2), ptr (R + 1), inters (K), normals (K, 3)) of one axis."""
+        sz = np.zeros(2, np.int64)
+        self._check(lib().ngpb_get_rays(self._h, axis, _p(sz), None, None, None, None), "get_rays")
+        R, K = int(sz[0]), int(sz[1])
+        key, ptr = np.zeros((R, 2)), np.zeros(R + 1, np.int64)
+        inters, normals = np.zeros(K), np.zeros((K, 3))
+        self._check(lib().ngpb_get_rays(self._h, axis, _p(sz), _p(key), _p(ptr), _p(inters), _p(normals)),
+                    "get_rays")
+        return key, ptr, inters, normals
 
     def create_markers(self):
         self._check(lib().ngpb_create_markers(self._h), "create_markers")

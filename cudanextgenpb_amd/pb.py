@@ -26,12 +26,18 @@ DEFAULT_SOLVER_OPTIONS = "-i cg -p jacobi -tol 1.e-10 -maxiter 100000"
 
 
 class PoissonBoltzmann:
-    def __init__(self, problem: Problem, device: int = 0, ctx: Optional[native.Context] = None):
+    def __init__(self, problem: Problem, device: int = 0, ctx: Optional[native.Context] = None,
+                 device_rays: bool = False):
+        """device_rays: cast the axis rays of the atom spheres on the GPU instead of
+        uploading the problem's host-cast ray table (the two are bit-identical)."""
         self.problem = problem
         self.ctx = ctx or native.Context(device)
         self.ctx.set_physics(problem.physics)
         self.ctx.set_mesh(problem.mesh)
-        self.ctx.set_rays(problem.rays)
+        if device_rays:
+            self.ctx.cast_rays(problem.atoms.xyz, problem.atoms.r)
+        else:
+            self.ctx.set_rays(problem.rays)
         self.phi = None
         self.stats = None
         self.energies = None

@@ -12,6 +12,7 @@
  *   ngpb_set_mesh         the tmesh_3d quadrant sweep: p(d,i), gt(i), is_hanging(i),
  *                         num_parents(i), gparent(j,i) (src/pb_class.cpp:1838-1852, 2024-2036)
  *   ngpb_set_rays         ray_cache_t::rays, one map per axis (include/raytracer.h:40-54)
+ *   ngpb_cast_rays        the casting that fills it (src/raytracer.cpp:22-63, 337-467)
  *   ngpb_create_markers   poisson_boltzmann::create_markers (include/pb_class.h:716-717,
  *                         src/pb_class.cpp:1792-1941) with is_in_ns_surf (:768-805)
  *   ngpb_set_markers      the members create_markers fills (include/pb_class.h:168-188)
@@ -133,6 +134,19 @@ int ngpb_set_mesh(ngpb_ctx* ctx, int64_t n_nodes, const double* node_xyz, int64_
    normals 3 per crossing. Lines without crossings may be omitted. */
 int ngpb_set_rays(ngpb_ctx* ctx, int axis, int64_t n_rays, const double* key, const int64_t* ptr,
                   const double* inters, const double* normals);
+/* Casts the rays on the device instead: for each axis, every grid line through a
+   vertex of the mesh (nodes and hanging nodes) against the union of the balls (centre
+   xyz[3a..3a+2], radius r[a]). Crossings are the end points of the merged covering
+   intervals along the line, with the outward normal of the ball owning each end point;
+   a line that only touches a ball does not cross it; lines without crossings are
+   omitted. Replaces the casting behind ray_cache_t (src/raytracer.cpp:22-63, the
+   castAxisOrientedRay calls at :337-467) for the analytic union of atom spheres. */
+int ngpb_cast_rays(ngpb_ctx* ctx, int64_t n_balls, const double* xyz, const double* r);
+/* Reads the context's ray table of one axis: sizes[0] rays, sizes[1] crossings; the
+   arrays (each may be NULL) receive key[2 sizes[0]], ptr[sizes[0] + 1],
+   inters[sizes[1]], normals[3 sizes[1]]. No counterpart in the reference. */
+int ngpb_get_rays(ngpb_ctx* ctx, int axis, int64_t* sizes, double* key, int64_t* ptr, double* inters,
+                  double* normals);
 
 /* In/out markers on the device from the z rays. */
 int ngpb_create_markers(ngpb_ctx* ctx);

@@ -1,0 +1,69 @@
+"""Ray casting on the device (ngpb_cast_rays) against the host caster of the synthetic
+workloads (csrc/workload.cpp), which follows the same definition: crossings are the end
+points of the merged intervals of the atom spheres along every grid line through a
+vertex, with the outward normal of the sphere owning each end point.
+
+* the ray tables of all three axes are bit-identical (keys, pointers, crossings,
+  normals);
+* markers, border list and edge fractions computed from either table are
+  bit-identical;
+* without balls every table is empty and every node is outside.
+"""
+import numpy as np
+import pytest
+
+from cudanextgenpb_amd import native, workload as W
+from cudanextgenpb_amd.pb import PoissonBoltzmann
+
+pytestmark = pytest.mark.gpu
+
+CASES = {
+    "born_h1": lambda: W.born(h=1.0),
+    "three_spheres": lambda: W.three_spheres(),
+    "globular_small": lambda: W.globular(n_atoms=60, ball_radius=6.0, min_sep=2.0, h_fine=0.5, levels=2,
+                                         fine_band=1.5, mid_band=4.0, bc_type=2),
+}
+
+
+@pytest.mark.parametrize("name", list(CASES))
+def test_device_rays_equal_host_rays(name):
+    P = CASES[name]()
+    print(f"[rays case] {name} N={P.mesh.n_nodes} H={P.mesh.n_hang} atoms={P.atoms.n}", flush=True)
+    dev = PoissonBoltzmann(P, device_rays=True)
+    for axis in range(3):
+        key, ptr, inters, normals = dev.ctx.get_rays(axis)
+        h = P.rays[axis]
+        assert key.shape[0] == h.key.shape[0] and key.shape[0] > 0, axis
+        assert np.array_equal(key, h.key), axis
+        assert np.array_equal(ptr, h.ptr), axis
+        assert np.array_equal(inters, h.inters), axis
+        assert np.array_equal(normals, h.normals), axis
+    host = PoissonBoltzmann(P)
+    for pb in (dev, host):
+        pb.create_markers()
+    md = dev.ctx.get_markers(P.mesh.n_nodes, P.mesh.n_leaves)
+    mh = host.ctx.get_markers(P.mesh.n_nodes, P.mesh.n_leaves)
+    for a, b in zip(md, mh):
+        assert np.array_equal(a, b)
+    assert np.array_equal(dev.ctx.get_fractions(P.mesh.n_leaves), host.ctx.get_fractions(P.mesh.n_leaves))
+    dev.close()
+    host.close()
+
+
+def test_no_balls_gives_empty_tables():
+    P = W.three_spheres()
+    print("[rays case] no balls", flush=True)
+    ctx = native.Context(0)
+    try:
+        ctx.set_physics(P.physics)
+        ctx.set_mesh(P.mesh)
+        ctx.cast_rays(np.zeros((0, 3)), np.zeros(0))
+        for axis in range(3):
+            key, ptr, inters, normals = ctx.get_rays(axis)
+            assert key.shape[0] == 0 and inters.shape[0] == 0 and ptr.tolist() == [0]
+        ctx.create_markers()
+        eps, reac, lm, border = ctx.get_markers(P.mesh.n_nodes, P.mesh.n_leaves)
+        c = native.constants(P.physics)
+        assert np.all(eps == c["eps_out"]) and border.size == 0
+    finally:
+        ctx.close()
