@@ -87,6 +87,25 @@ class PoissonBoltzmann:
                     f.write(f"{i + 1:8d}{x:8.3f}{y:8.3f}{z:8.3f}  {phi[i]:.4f}\n")
         return phi
 
+    # map_type = vtu output (src/poisson_boltzmann.cpp:318-351): potential_map_0000.vtu
+    # (field "phi") and eps_map_0000.vtu (field "eps") with their total_*.pvtu indexes,
+    # laid out by cudanextgenpb_amd.vtk from the device's potential and markers.
+    def export_vtu(self, potential_map: bool = True, eps_map: bool = True, directory: str = "."):
+        from . import vtk
+        m = self.problem.mesh
+        fields, bases = [], []
+        if potential_map:
+            vtk.write_vtu(f"{directory}/potential_map_0000.vtu", m, self.ctx.get_phi(), "phi")
+            fields.append("phi")
+            bases.append("potential_map")
+        if eps_map:
+            eps = self.ctx.get_markers(m.n_nodes, m.n_leaves)[0]
+            vtk.write_vtu(f"{directory}/eps_map_0000.vtu", m, eps, "eps")
+            fields.append("eps")
+            bases.append("eps_map")
+        if fields:
+            vtk.write_pvtu(fields, bases, 1, directory)
+
     # pot_field / pot_field_fast (src/pb_class.cpp:3731-5060): per-atom potentials and
     # fields; pot_field.dat is written as the reference writes it (:4276-4385): a
     # header naming the columns of the branch, then per atom the index (width 5), the

@@ -285,3 +285,19 @@ def test_constant_slices_bit_identical(name):
     assert np.array_equal(y_on, y_off)
     assert np.array_equal(phi_on, phi_off) and pb.stats["iterations"] == it_on
     pb.close()
+
+
+def test_export_vtu_from_device(tmp_path):
+    """potential_map / eps_map VTU files hold the device's potential and markers."""
+    from cudanextgenpb_amd import vtk
+    P, pb = gpu_pipeline("three_spheres")
+    pb.hip_compute_electric_potential(precision=native.PREC_MIXED)
+    pb.export_vtu(directory=str(tmp_path))
+    phi = pb.ctx.get_phi()
+    eps = pb.ctx.get_markers(P.mesh.n_nodes, P.mesh.n_leaves)[0]
+    dp = vtk.read_vtu(str(tmp_path / "potential_map_0000.vtu"))
+    de = vtk.read_vtu(str(tmp_path / "eps_map_0000.vtu"))
+    assert np.array_equal(dp["phi"][:P.mesh.n_nodes], phi)
+    assert np.array_equal(de["eps"][:P.mesh.n_nodes], eps)
+    assert (tmp_path / "total_potential_map.pvtu").exists() and (tmp_path / "total_eps_map.pvtu").exists()
+    pb.close()

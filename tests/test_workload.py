@@ -116,3 +116,36 @@ def test_request_stream_deterministic_and_footprint_bound():
     assert all(x.arrival < y.arrival for x, y in zip(a, a[1:]))
     assert all(30 <= r.problem.atoms.n <= 200 for r in a)
     assert all(r.predicted_bytes > 400 * r.n + 3600 * r.problem.atoms.n for r in a)
+
+
+def test_vtu_writer_layout(tmp_path):
+    """VTU output laid out as the reference's VTKWriter: node points first, then one
+    point per hanging corner occurrence (leaf order) carrying its parents' mean; voxel
+    cells in corner order; appended blocks with int32 byte counts; the pvtu index."""
+    from cudanextgenpb_amd import vtk
+    P = W.three_spheres()
+    m = P.mesh
+    assert m.n_hang > 0
+    f = 0.3 * m.node_xyz[:, 0] - 1.1 * m.node_xyz[:, 1] + 0.7 * m.node_xyz[:, 2] + 2.0
+    path = str(tmp_path / "potential_map_0000.vtu")
+    vtk.write_vtu(path, m, f, "phi")
+    vtk.write_pvtu(["phi"], ["potential_map"], 1, str(tmp_path))
+    d = vtk.read_vtu(path)
+    nocc = int((m.leaf_corner < 0).sum())
+    assert d["n_cells"] == m.n_leaves and d["n_points"] == m.n_nodes + nocc
+    pts = d["Array"].reshape(-1, 3)
+    assert np.array_equal(pts[:m.n_nodes], m.node_xyz)
+    conn = d["connectivity"].reshape(-1, 8)
+    reg = m.leaf_corner >= 0
+    assert np.array_equal(conn[reg], m.leaf_corner[reg])
+    # hanging occurrences get fresh points in leaf-major order at the hanging node
+    assert np.array_equal(conn[~reg], m.n_nodes + np.arange(nocc))
+    assert np.array_equal(pts[m.n_nodes:], m.hang_xyz[-1 - m.leaf_corner[~reg]])
+    # a linear field's parent mean is its value at the hanging node
+    lin = lambda x: 0.3 * x[:, 0] - 1.1 * x[:, 1] + 0.7 * x[:, 2] + 2.0
+    assert np.array_equal(d["phi"][:m.n_nodes], f)
+    assert np.abs(d["phi"][m.n_nodes:] - lin(pts[m.n_nodes:])).max() <= 1e-12 * np.abs(f).max()
+    assert np.array_equal(d["offsets"], 8 * np.arange(1, m.n_leaves + 1))
+    assert np.all(d["types"] == 11)
+    pv = (tmp_path / "total_potential_map.pvtu").read_text()
+    assert 'Name="phi"' in pv and 'Source="potential_map_0000.vtu"' in pv
