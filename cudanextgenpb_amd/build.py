@@ -31,6 +31,7 @@ HIP_SOURCES = [
     "ngpb_energy.hip",
     "ngpb_batch.hip",
     "ngpb_rays.hip",
+    "ngpb_locate.hip",
 ]
 
 # -ffp-contract=off: every multiply and add is rounded on its own unless the source

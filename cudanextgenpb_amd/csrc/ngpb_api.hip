@@ -299,6 +299,15 @@ int ngpb_cast_rays(ngpb_ctx* c, int64_t n_balls, const double* xyz, const double
   });
 }
 
+int ngpb_locate_atoms(ngpb_ctx* c, int64_t n_atoms, const double* xyz, double tol, int32_t* owner_leaf) {
+  return guarded(c, [&] {
+    need(c->have_mesh, "ngpb_set_mesh must come before ngpb_locate_atoms");
+    NGPB_REQUIRE(n_atoms >= 0 && (n_atoms == 0 || (xyz && owner_leaf)), NGPB_E_ARG, "bad atom arrays");
+    NGPB_REQUIRE(std::isfinite(tol) && tol >= 0, NGPB_E_ARG, "tolerance must be finite and >= 0");
+    ngpb::run_locate_atoms(c, n_atoms, xyz, tol, owner_leaf);
+  });
+}
+
 int ngpb_get_rays(ngpb_ctx* c, int axis, int64_t* sizes, double* key, int64_t* ptr, double* inters, double* normals) {
   return guarded(c, [&] {
     NGPB_REQUIRE(axis >= 0 && axis < 3 && sizes, NGPB_E_ARG, "bad arguments");

@@ -98,6 +98,7 @@ SIGNATURES = {
     "ngpb_energy": (_I, [_P, _P]),
     "ngpb_phi_on_atoms": (_I, [_P, _P]),
     "ngpb_cast_rays": (_I, [_P, _I64, _P, _P]),
+    "ngpb_locate_atoms": (_I, [_P, _I64, _P, ctypes.c_double, _P]),
     "ngpb_get_rays": (_I, [_P, _I, _P, _P, _P, _P, _P]),
     "ngpb_pot_field": (_I, [_P, _I, _I, _P, _P]),
     "ngpb_time_spmv": (_I, [_P, _I, _I, _P, _P]),
@@ -229,6 +230,14 @@ class Context:
         xyz = _c(xyz, np.float64)
         r = _c(r, np.float64)
         self._check(lib().ngpb_cast_rays(self._h, int(r.shape[0]), _p(xyz), _p(r)), "cast_rays")
+
+    def locate_atoms(self, xyz, tol: float):
+        """Owner leaf of each atom (-1 outside), located on the device."""
+        xyz = _c(xyz, np.float64)
+        owner = np.zeros(xyz.shape[0], np.int32)
+        self._check(lib().ngpb_locate_atoms(self._h, int(xyz.shape[0]), _p(xyz), float(tol), _p(owner)),
+                    "locate_atoms")
+        return owner
 
     def get_rays(self, axis: int):
         """(key (R, 2), ptr (R + 1), inters (K), normals (K, 3)) of one axis."""

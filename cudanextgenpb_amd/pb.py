@@ -27,10 +27,13 @@ DEFAULT_SOLVER_OPTIONS = "-i cg -p jacobi -tol 1.e-10 -maxiter 100000"
 
 class PoissonBoltzmann:
     def __init__(self, problem: Problem, device: int = 0, ctx: Optional[native.Context] = None,
-                 device_rays: bool = False):
+                 device_rays: bool = False, device_locate: bool = False):
         """device_rays: cast the axis rays of the atom spheres on the GPU instead of
-        uploading the problem's host-cast ray table (the two are bit-identical)."""
+        uploading the problem's host-cast ray table (the two are bit-identical).
+        device_locate: find the atoms' owner leaves on the GPU (search_points) instead
+        of taking the problem's host-located owners (the two are identical)."""
         self.problem = problem
+        self.device_locate = device_locate
         self.ctx = ctx or native.Context(device)
         self.ctx.set_physics(problem.physics)
         self.ctx.set_mesh(problem.mesh)
@@ -51,7 +54,8 @@ class PoissonBoltzmann:
     # assemple_system_matrix together with the patch volumes it needs.
     def create_density_map(self):
         a = self.problem.atoms
-        self.ctx.set_charges(a.xyz, a.q, self.problem.owner)
+        owner = self.ctx.locate_atoms(a.xyz, self.problem.atom_tol) if self.device_locate else self.problem.owner
+        self.ctx.set_charges(a.xyz, a.q, owner)
 
     # assemple_system_matrix (src/pb_class.cpp:2052-2159)
     def assemple_system_matrix(self, bnd_values: Optional[np.ndarray] = None):

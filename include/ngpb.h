@@ -13,6 +13,7 @@
  *                         num_parents(i), gparent(j,i) (src/pb_class.cpp:1838-1852, 2024-2036)
  *   ngpb_set_rays         ray_cache_t::rays, one map per axis (include/raytracer.h:40-54)
  *   ngpb_cast_rays        the casting that fills it (src/raytracer.cpp:22-63, 337-467)
+ *   ngpb_locate_atoms     search_points / cerca_atomo (src/pb_class.cpp:3633-3729)
  *   ngpb_create_markers   poisson_boltzmann::create_markers (include/pb_class.h:716-717,
  *                         src/pb_class.cpp:1792-1941) with is_in_ns_surf (:768-805)
  *   ngpb_set_markers      the members create_markers fills (include/pb_class.h:168-188)
@@ -142,6 +143,12 @@ int ngpb_set_rays(ngpb_ctx* ctx, int axis, int64_t n_rays, const double* key, co
    omitted. Replaces the casting behind ray_cache_t (src/raytracer.cpp:22-63, the
    castAxisOrientedRay calls at :337-467) for the analytic union of atom spheres. */
 int ngpb_cast_rays(ngpb_ctx* ctx, int64_t n_balls, const double* xyz, const double* r);
+/* Owner leaf of each atom, located on the device as search_points / cerca_atomo /
+   controlla_coordinate build lookup_table (src/pb_class.cpp:3633-3729): the first
+   leaf, in leaf order, with corner0 - tol < x <= corner7 - tol on every axis; -1 when
+   none (tol = 2^-29 of the domain edge in the reference, include/pb_class.h:30). The
+   result is written to owner_leaf[n_atoms] and can be passed to ngpb_set_charges. */
+int ngpb_locate_atoms(ngpb_ctx* ctx, int64_t n_atoms, const double* xyz, double tol, int32_t* owner_leaf);
 /* Reads the context's ray table of one axis: sizes[0] rays, sizes[1] crossings; the
    arrays (each may be NULL) receive key[2 sizes[0]], ptr[sizes[0] + 1],
    inters[sizes[1]], normals[3 sizes[1]]. No counterpart in the reference. */

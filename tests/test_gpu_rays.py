@@ -67,3 +67,34 @@ def test_no_balls_gives_empty_tables():
         assert np.all(eps == c["eps_out"]) and border.size == 0
     finally:
         ctx.close()
+
+
+@pytest.mark.parametrize("name", list(CASES))
+def test_device_locate_equals_oracle(name):
+    """Owner leaves located on the device equal the oracle's brute-force search
+    (first leaf in leaf order passing the half-open test) and the host locator, also
+    for atoms placed exactly on leaf faces, edges and corners and outside the domain."""
+    import oracle as O
+    P = CASES[name]()
+    m = P.mesh
+    rng = np.random.default_rng(11)
+    lc = m.leaf_corner[rng.integers(0, m.n_leaves, 64)]
+    hx = np.vstack([m.hang_xyz.reshape(-1, 3), np.zeros((1, 3))])  # padded: meshes without hanging nodes
+    corner = lambda c: np.where((c >= 0)[:, None], m.node_xyz[np.maximum(c, 0)], hx[np.maximum(-1 - c, 0)])
+    lo, hi = corner(lc[:, 0]), corner(lc[:, 7])
+    pts = np.vstack([P.atoms.xyz,
+                     lo + rng.uniform(0, 1, lo.shape) * (hi - lo),          # inside
+                     lo,                                                     # corner 0
+                     np.column_stack([hi[:, 0], lo[:, 1], 0.5 * (lo[:, 2] + hi[:, 2])]),  # on an edge
+                     np.column_stack([hi[:, 0], 0.5 * (lo[:, 1] + hi[:, 1]), 0.5 * (lo[:, 2] + hi[:, 2])]),  # face
+                     [[1e4, 0.0, 0.0]]])                                     # outside
+    print(f"[locate case] {name} points={pts.shape[0]}", flush=True)
+    import dataclasses
+    Q = dataclasses.replace(P, atoms=W.Atoms(pts, np.ones(len(pts)), np.zeros(len(pts))))
+    want = O.locate(Q)
+    pb = PoissonBoltzmann(P)
+    got = pb.ctx.locate_atoms(pts, P.atom_tol)
+    assert np.array_equal(got, want)
+    assert np.array_equal(got[:P.atoms.n], P.owner)
+    assert got[-1] == -1
+    pb.close()
