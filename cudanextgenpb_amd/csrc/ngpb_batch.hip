@@ -370,6 +370,7 @@ struct ngpb_batch {
   ngpb::DBuf<ngpb::Tile> tiles;
   int64_t n_tiles = 0;
   bool tiles_dirty = true;
+  int tiled_systems = 0;  // systems in the current tile list
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
   int64_t replays = 0;
@@ -406,11 +407,16 @@ void drop_graph(ngpb_batch* b) {
   b->graph = nullptr;
 }
 
-void build_tiles(ngpb_batch* b) {
+// Tiles of the systems not yet done (with all_systems, of every system: the autotuner
+// times the SpMV over the whole batch). Finished systems are skipped by every kernel
+// anyway; leaving their tiles out saves launching their blocks.
+void build_tiles(ngpb_batch* b, bool all_systems = false) {
   b->tiles_host.clear();
+  b->tiled_systems = 0;
   for (int s = 0; s < b->cap; ++s) {
     ngpb_ctx* c = b->sys[s];
-    if (!c) continue;
+    if (!c || (!all_systems && b->scal_host[s].phase == kDone)) continue;
+    ++b->tiled_systems;
     for (int64_t k = 0; k < c->n_slices; k += b->tile_slices) b->tiles_host.push_back(Tile{s, (int32_t)k});
   }
   b->n_tiles = (int64_t)b->tiles_host.size();
@@ -687,7 +693,7 @@ int ngpb_batch_autotune(ngpb_batch* b, int* tile_slices, int* waves_per_block, d
         if (ts < w) continue;
         b->tile_slices = ts;
         b->wpb = w;
-        ngpb::build_tiles(b);
+        ngpb::build_tiles(b, true);
         ngpb::launch_spmv_force_any(b);  // warm
         const int reps = 5;
         NGPB_HIP(hipEventRecord(e0, b->stream));
@@ -764,6 +770,9 @@ int ngpb_batch_run(ngpb_batch* b, int max_replays, int* running) {
           ++left;
         }
       }
+      // once a quarter of the tiled systems are done, re-tile (and re-capture the
+      // graph) without them
+      if (left > 0 && 4 * (b->tiled_systems - left) >= b->tiled_systems) b->tiles_dirty = true;
     }
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
