@@ -146,7 +146,8 @@ int ngpb_cast_rays(ngpb_ctx* ctx, int64_t n_balls, const double* xyz, const doub
 /* Owner leaf of each atom, located on the device as search_points / cerca_atomo /
    controlla_coordinate build lookup_table (src/pb_class.cpp:3633-3729): the first
    leaf, in leaf order, with corner0 - tol < x <= corner7 - tol on every axis; -1 when
-   none (tol = 2^-29 of the domain edge in the reference, include/pb_class.h:30). The
+   none (tol = 2^-P8EST_QMAXLEVEL = 2^-18 of the domain edge in the reference,
+   include/pb_class.h:30, src/pb_class.cpp:3637-3640). The
    result is written to owner_leaf[n_atoms] and can be passed to ngpb_set_charges. */
 int ngpb_locate_atoms(ngpb_ctx* ctx, int64_t n_atoms, const double* xyz, double tol, int32_t* owner_leaf);
 /* Reads the context's ray table of one axis: sizes[0] rays, sizes[1] crossings; the
