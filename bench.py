@@ -85,9 +85,12 @@ def cpu_baseline(pb, fp64_iters, seconds):
     rp, col, val, rhs = pb.ctx.export_csr()
     S = O.System(len(rhs), rp, col, val, rhs)
     threads = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or (os.cpu_count() or 1)
-    O.pcg_time_iters(S, 3, threads)  # first touch and thread start-up
-    t = O.pcg_time_iters(S, 10, threads)
-    per_it = max(t / 10, 1e-6)
+    # each call allocates and first-touches its vectors: the per-iteration time is
+    # taken from the difference of two pilot lengths, which cancels that fixed cost
+    O.pcg_time_iters(S, 3, threads)
+    t5 = O.pcg_time_iters(S, 5, threads)
+    t25 = O.pcg_time_iters(S, 25, threads)
+    per_it = max((t25 - t5) / 20, 1e-6)
     n_it = int(max(5, min(4000, seconds / per_it)))
     t = O.pcg_time_iters(S, n_it, threads)
     per_it = t / n_it
