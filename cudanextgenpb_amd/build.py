@@ -32,6 +32,7 @@ HIP_SOURCES = [
     "ngpb_batch.hip",
     "ngpb_rays.hip",
     "ngpb_locate.hip",
+    "ngpb_octree.hip",
 ]
 
 # -ffp-contract=off: every multiply and add is rounded on its own unless the source

@@ -39,6 +39,25 @@ PhysConst physical_constants(const ngpb_physics& p) {
 
 void run_markers(ngpb_ctx* c);
 
+// the mesh arrays (node_xyz, hang_*, leaf_corner) and the Dirichlet node list
+// (bnd_node, is_bnd) were written on the device; records the sizes as ngpb_set_mesh
+// and ngpb_set_dirichlet would, with the boundary values still to come
+void install_device_mesh(ngpb_ctx* c, int64_t N, int64_t H, int64_t L, int64_t NB) {
+  c->N = N;
+  c->H = H;
+  c->L = L;
+  c->NB = NB;
+  if (!NB) c->bnd_node.resize(1);
+  c->gval.resize((size_t)N);
+  NGPB_HIP(hipMemsetAsync(c->gval.p, 0, (size_t)N * sizeof(double), c->stream));
+  NGPB_HIP(hipStreamSynchronize(c->stream));
+  c->have_mesh = true;
+  c->have_dirichlet = true;
+  c->have_bnd_values = NB == 0;
+  c->have_charges = false;
+  c->have_markers = c->have_frac = c->have_symbolic = c->have_system = c->have_compact = c->have_phi = false;
+}
+
 }  // namespace ngpb
 
 int64_t ngpb_ctx::device_bytes() const {
@@ -236,6 +255,64 @@ int ngpb_set_mesh(ngpb_ctx* c, int64_t n_nodes, const double* node_xyz, int64_t 
     c->have_mesh = true;
     c->have_dirichlet = c->have_bnd_values = c->have_charges = false;
     invalidate_after_mesh(c);
+  });
+}
+
+int ngpb_build_octree(ngpb_ctx* c, const double* origin, double hf, const int64_t* n, int smax, const double* band,
+                      int64_t n_balls, const double* xyz, const double* r, int64_t* sizes) {
+  return guarded(c, [&] {
+    not_in_batch(c);
+    NGPB_REQUIRE(origin && n, NGPB_E_ARG, "NULL origin or n");
+    NGPB_REQUIRE(smax >= 0 && smax <= 10, NGPB_E_ARG, "smax must be in 0..10");
+    NGPB_REQUIRE(std::isfinite(hf) && hf > 0, NGPB_E_ARG, "hf must be finite and > 0");
+    NGPB_REQUIRE(smax == 0 || band, NGPB_E_ARG, "NULL band");
+    for (int d = 0; d < 3; ++d) {
+      NGPB_REQUIRE(std::isfinite(origin[d]), NGPB_E_ARG, "non-finite origin");
+      NGPB_REQUIRE(n[d] > 0 && n[d] <= ((int64_t)1 << 21) && n[d] % ((int64_t)1 << smax) == 0, NGPB_E_ARG,
+                   "n must be positive multiples of 2^smax, at most 2^21");
+    }
+    NGPB_REQUIRE((double)(n[0] + 1) * (double)(n[1] + 1) * (double)(n[2] + 1) < 4294967296.0, NGPB_E_ARG,
+                 "at most 2^32 - 1 grid vertices");
+    for (int s = 0; s < smax; ++s) NGPB_REQUIRE(std::isfinite(band[s]), NGPB_E_ARG, "non-finite band");
+    for (int s = 1; s < smax; ++s) NGPB_REQUIRE(band[s] >= band[s - 1], NGPB_E_ARG, "band must be non-decreasing");
+    NGPB_REQUIRE(n_balls >= 0 && n_balls < INT32_MAX, NGPB_E_ARG, "bad ball count");
+    NGPB_REQUIRE(n_balls == 0 || (xyz && r), NGPB_E_ARG, "NULL ball array");
+    for (int64_t a = 0; a < n_balls; ++a)
+      NGPB_REQUIRE(std::isfinite(r[a]) && r[a] >= 0 && std::isfinite(xyz[3 * a]) && std::isfinite(xyz[3 * a + 1]) &&
+                       std::isfinite(xyz[3 * a + 2]),
+                   NGPB_E_ARG, "ball centres and radii must be finite, radii >= 0");
+    // a failed build leaves no mesh behind
+    c->have_mesh = c->have_dirichlet = c->have_bnd_values = c->have_charges = false;
+    invalidate_after_mesh(c);
+    const ngpb::OctreeSizes z = ngpb::run_build_octree(c, origin, hf, n, smax, band, n_balls, xyz, r);
+    if (sizes) {
+      sizes[0] = z.N;
+      sizes[1] = z.H;
+      sizes[2] = z.L;
+      sizes[3] = z.NB;
+    }
+  });
+}
+
+int ngpb_get_mesh(ngpb_ctx* c, int64_t* sizes, double* node_xyz, double* hang_xyz, int32_t* hang_np,
+                  int32_t* hang_par, int32_t* leaf_corner, int32_t* bnd_nodes) {
+  return guarded(c, [&] {
+    need(c->have_mesh, "no mesh");
+    NGPB_REQUIRE(sizes, NGPB_E_ARG, "NULL sizes");
+    const int64_t nb = c->have_dirichlet ? c->NB : 0;
+    sizes[0] = c->N;
+    sizes[1] = c->H;
+    sizes[2] = c->L;
+    sizes[3] = nb;
+    if (node_xyz) c->node_xyz.download(node_xyz, (size_t)(3 * c->N), c->stream);
+    if (c->H) {
+      if (hang_xyz) c->hang_xyz.download(hang_xyz, (size_t)(3 * c->H), c->stream);
+      if (hang_np) c->hang_np.download(hang_np, (size_t)c->H, c->stream);
+      if (hang_par) c->hang_par.download(hang_par, (size_t)(4 * c->H), c->stream);
+    }
+    if (leaf_corner) c->leaf_corner.download(leaf_corner, (size_t)(8 * c->L), c->stream);
+    if (bnd_nodes && nb) c->bnd_node.download(bnd_nodes, (size_t)nb, c->stream);
+    NGPB_HIP(hipStreamSynchronize(c->stream));
   });
 }
 

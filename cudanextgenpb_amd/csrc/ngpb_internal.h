@@ -299,6 +299,11 @@ void run_dirichlet_values(ngpb_ctx* c, int bc_type);
 void run_phi_on_atoms(ngpb_ctx* c, double* d_out);
 void run_cast_rays(ngpb_ctx* c, int64_t n_balls, const double* h_xyz, const double* h_r);
 void run_locate_atoms(ngpb_ctx* c, int64_t A, const double* h_xyz, double tol, int32_t* h_owner);
+struct OctreeSizes {
+  int64_t N, H, L, NB;
+};
+OctreeSizes run_build_octree(ngpb_ctx* c, const double* origin, double hf, const int64_t* n, int smax,
+                             const double* band, int64_t na, const double* h_xyz, const double* h_r);
 
 // Krylov
 unsigned vec_grid(int64_t n);

@@ -99,6 +99,8 @@ SIGNATURES = {
     "ngpb_phi_on_atoms": (_I, [_P, _P]),
     "ngpb_cast_rays": (_I, [_P, _I64, _P, _P]),
     "ngpb_locate_atoms": (_I, [_P, _I64, _P, ctypes.c_double, _P]),
+    "ngpb_build_octree": (_I, [_P, _P, ctypes.c_double, _P, _I, _P, _I64, _P, _P, _P]),
+    "ngpb_get_mesh": (_I, [_P, _P, _P, _P, _P, _P, _P, _P]),
     "ngpb_get_rays": (_I, [_P, _I, _P, _P, _P, _P, _P]),
     "ngpb_pot_field": (_I, [_P, _I, _I, _P, _P]),
     "ngpb_time_spmv": (_I, [_P, _I, _I, _P, _P]),
@@ -218,6 +220,34 @@ class Context:
         nx, hx, hn, hp, lc = self._keep_mesh
         self._check(lib().ngpb_set_mesh(self._h, mesh.n_nodes, _p(nx), mesh.n_hang, _p(hx), _p(hn), _p(hp),
                                         mesh.n_leaves, _p(lc)), "set_mesh")
+
+    def build_octree(self, origin, hf: float, n, smax: int, bands, xyz, r):
+        """Builds the 2:1 balanced octree of the balls (xyz, r) on the device and installs
+        it as the mesh, with the box-face nodes as the Dirichlet list. Returns
+        (n_nodes, n_hang, n_leaves, n_boundary)."""
+        o = _c(origin, np.float64)
+        nn = _c(n, np.int64)
+        b = _c(list(bands)[:smax] + [0.0] * max(0, 1 - smax), np.float64)
+        xyz = _c(xyz, np.float64)
+        r = _c(r, np.float64)
+        sz = np.zeros(4, np.int64)
+        self._check(lib().ngpb_build_octree(self._h, _p(o), float(hf), _p(nn), int(smax), _p(b), int(r.shape[0]),
+                                            _p(xyz), _p(r), _p(sz)), "build_octree")
+        return tuple(int(v) for v in sz)
+
+    def get_mesh(self):
+        """The context's mesh: dict of node_xyz (N, 3), hang_xyz (H, 3), hang_np (H,),
+        hang_par (H, 4), leaf_corner (L, 8), bnd_nodes (NB,)."""
+        sz = np.zeros(4, np.int64)
+        self._check(lib().ngpb_get_mesh(self._h, _p(sz), None, None, None, None, None, None), "get_mesh")
+        N, H, L, NB = (int(v) for v in sz)
+        out = dict(node_xyz=np.zeros((N, 3)), hang_xyz=np.zeros((H, 3)), hang_np=np.zeros(H, np.int32),
+                   hang_par=np.zeros((H, 4), np.int32), leaf_corner=np.zeros((L, 8), np.int32),
+                   bnd_nodes=np.zeros(NB, np.int32))
+        self._check(lib().ngpb_get_mesh(self._h, _p(sz), _p(out["node_xyz"]), _p(out["hang_xyz"]),
+                                        _p(out["hang_np"]), _p(out["hang_par"]), _p(out["leaf_corner"]),
+                                        _p(out["bnd_nodes"])), "get_mesh")
+        return out
 
     def set_rays(self, rays):
         for axis, r in enumerate(rays):

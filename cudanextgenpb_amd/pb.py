@@ -27,23 +27,48 @@ DEFAULT_SOLVER_OPTIONS = "-i cg -p jacobi -tol 1.e-10 -maxiter 100000"
 
 class PoissonBoltzmann:
     def __init__(self, problem: Problem, device: int = 0, ctx: Optional[native.Context] = None,
-                 device_rays: bool = False, device_locate: bool = False):
+                 device_rays: bool = False, device_locate: bool = False, device_mesh: bool = False):
         """device_rays: cast the axis rays of the atom spheres on the GPU instead of
         uploading the problem's host-cast ray table (the two are bit-identical).
         device_locate: find the atoms' owner leaves on the GPU (search_points) instead
-        of taking the problem's host-located owners (the two are identical)."""
+        of taking the problem's host-located owners (the two are identical).
+        device_mesh: build the octree on the GPU from the atoms and the mesh's box,
+        levels and bands (ngpb_build_octree) instead of uploading the problem's
+        host-built mesh (the two are identical); implies device_rays and device_locate,
+        so that only the atoms cross to the device."""
         self.problem = problem
-        self.device_locate = device_locate
+        self.device_mesh = device_mesh
+        self.device_locate = device_locate or device_mesh
+        self._mesh = None if device_mesh else problem.mesh
         self.ctx = ctx or native.Context(device)
         self.ctx.set_physics(problem.physics)
-        self.ctx.set_mesh(problem.mesh)
-        if device_rays:
+        if device_mesh:
+            m = problem.mesh
+            self.ctx.build_octree(m.origin, m.hf, m.n, m.smax, m.bands, problem.atoms.xyz, problem.atoms.r)
+        else:
+            self.ctx.set_mesh(problem.mesh)
+        if device_rays or device_mesh:
             self.ctx.cast_rays(problem.atoms.xyz, problem.atoms.r)
         else:
             self.ctx.set_rays(problem.rays)
+        self.owner = problem.owner
         self.phi = None
         self.stats = None
         self.energies = None
+
+    @property
+    def mesh(self):
+        """The mesh the context holds (read back from the device when it was built there)."""
+        if self._mesh is None:
+            from .workload import Mesh
+            m, g = self.problem.mesh, self.ctx.get_mesh()
+            lc = g["leaf_corner"]
+            hx = np.vstack([g["hang_xyz"], np.zeros((1, 3))])
+            corner = lambda c: np.where((c >= 0)[:, None], g["node_xyz"][np.maximum(c, 0)], hx[np.maximum(-1 - c, 0)])
+            edge = corner(lc[:, 7])[:, 0] - corner(lc[:, 0])[:, 0]
+            level = np.rint(np.log2(edge / m.hf)).astype(np.int32)
+            self._mesh = Mesh(origin=m.origin, hf=m.hf, n=m.n, leaf_level=level, smax=m.smax, bands=m.bands, **g)
+        return self._mesh
 
     # create_markers (src/pb_class.cpp:1792-1941)
     def create_markers(self):
@@ -56,11 +81,12 @@ class PoissonBoltzmann:
         a = self.problem.atoms
         owner = self.ctx.locate_atoms(a.xyz, self.problem.atom_tol) if self.device_locate else self.problem.owner
         self.ctx.set_charges(a.xyz, a.q, owner)
+        self.owner = owner
 
     # assemple_system_matrix (src/pb_class.cpp:2052-2159)
     def assemple_system_matrix(self, bnd_values: Optional[np.ndarray] = None):
-        m = self.problem.mesh
-        self.ctx.set_dirichlet(m.bnd_nodes, bnd_values)
+        if not self.device_mesh or bnd_values is not None:  # a device-built mesh holds its boundary list
+            self.ctx.set_dirichlet(self.mesh.bnd_nodes, bnd_values)
         if bnd_values is None:
             self.ctx.dirichlet_values(self.problem.bc_type)
         self.ctx.assemble()
@@ -85,7 +111,7 @@ class PoissonBoltzmann:
         if path is not None:
             with open(path, "w") as f:
                 for i in range(a.n):
-                    if self.problem.owner[i] < 0:
+                    if self.owner[i] < 0:
                         continue
                     x, y, z = a.xyz[i]
                     f.write(f"{i + 1:8d}{x:8.3f}{y:8.3f}{z:8.3f}  {phi[i]:.4f}\n")
@@ -96,7 +122,7 @@ class PoissonBoltzmann:
     # laid out by cudanextgenpb_amd.vtk from the device's potential and markers.
     def export_vtu(self, potential_map: bool = True, eps_map: bool = True, directory: str = "."):
         from . import vtk
-        m = self.problem.mesh
+        m = self.mesh
         fields, bases = [], []
         if potential_map:
             vtk.write_vtu(f"{directory}/potential_map_0000.vtu", m, self.ctx.get_phi(), "phi")

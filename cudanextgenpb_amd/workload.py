@@ -113,6 +113,8 @@ class Mesh:
     leaf_corner: np.ndarray       # (L, 8) int32, <0 -> hanging id -1-c
     bnd_nodes: np.ndarray         # (Nb,) int32
     leaf_level: np.ndarray        # (L,) int32, log2 of leaf edge in finest cells
+    smax: int = 0                 # refinement levels below the root cells
+    bands: tuple = ()             # split distance of a cell of level s is bands[s - 1]
     _handle: Optional[int] = field(default=None, repr=False)
 
     @property
@@ -186,7 +188,7 @@ def make_mesh(origin, hf: float, n, smax: int, bands, atoms: Atoms) -> Mesh:
     m = Mesh(origin=o, hf=hf, n=nn, node_xyz=np.zeros((N, 3)), hang_xyz=np.zeros((H, 3)),
              hang_np=np.zeros(H, np.int32), hang_par=np.zeros((H, 4), np.int32),
              leaf_corner=np.zeros((L, 8), np.int32), bnd_nodes=np.zeros(NB, np.int32),
-             leaf_level=np.zeros(L, np.int32), _handle=h)
+             leaf_level=np.zeros(L, np.int32), smax=int(smax), bands=tuple(float(v) for v in b[:smax]), _handle=h)
     lib().ngpbw_mesh_copy(h, _ptr(m.node_xyz), _ptr(m.hang_xyz), _ptr(m.hang_np), _ptr(m.hang_par),
                           _ptr(m.leaf_corner), _ptr(m.bnd_nodes), _ptr(m.leaf_level))
     return m

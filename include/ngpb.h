@@ -11,6 +11,8 @@
  *   ngpb_set_physics      model keys parsed at src/pb_class.cpp:940-945
  *   ngpb_set_mesh         the tmesh_3d quadrant sweep: p(d,i), gt(i), is_hanging(i),
  *                         num_parents(i), gparent(j,i) (src/pb_class.cpp:1838-1852, 2024-2036)
+ *   ngpb_build_octree     init_tmesh / refine_surface on p4est (src/pb_class.cpp:1289-1391,
+ *                         1467-1745) producing the same views
  *   ngpb_set_rays         ray_cache_t::rays, one map per axis (include/raytracer.h:40-54)
  *   ngpb_cast_rays        the casting that fills it (src/raytracer.cpp:22-63, 337-467)
  *   ngpb_locate_atoms     search_points / cerca_atomo (src/pb_class.cpp:3633-3729)
@@ -128,6 +130,30 @@ int ngpb_set_physics(ngpb_ctx* ctx, const ngpb_physics* physics);
 int ngpb_set_mesh(ngpb_ctx* ctx, int64_t n_nodes, const double* node_xyz, int64_t n_hang,
                   const double* hang_xyz, const int32_t* hang_np, const int32_t* hang_par,
                   int64_t n_leaves, const int32_t* leaf_corner);
+/* Builds the mesh on the device instead and installs it as ngpb_set_mesh does, with
+   the nodes on the faces of the box as the Dirichlet list (values to follow from
+   ngpb_dirichlet_values, or ngpb_set_dirichlet to replace them). Replaces
+   init_tmesh / refine_surface and the tmesh_3d views above (src/pb_class.cpp:
+   1289-1391, 1467-1745, 1838-1852) with a distance-band rule, as p4est is not part
+   of this library: the box [origin, origin + n hf] is tiled by root cells of edge
+   2^smax hf (n[d] positive multiples of 2^smax, at most 2^21; smax in 0..10); a cell
+   of level s >= 1 (edge 2^s hf) is split while some ball a has
+   |centre - xyz_a| - r_a < band[s-1] + sqrt(3)/2 * edge (band[0..smax-1] finite and
+   non-decreasing); then leaves of level s >= 2 next to (faces, edges, corners) a leaf
+   below level s - 1 are split until none is (2:1 balance). Leaves are in Morton order
+   of their anchors; nodes and hanging nodes are numbered in lexicographic order (x
+   fastest) of their grid position; hanging nodes are edge midpoints (2 parents, the
+   edge ends) and face centres (4 parents, the face corners in corner order) of a
+   leaf. sizes (may be NULL) receives {n_nodes, n_hang, n_leaves, n_boundary}. */
+int ngpb_build_octree(ngpb_ctx* ctx, const double* origin, double hf, const int64_t* n, int smax,
+                      const double* band, int64_t n_balls, const double* xyz, const double* r,
+                      int64_t* sizes);
+/* Reads the context's mesh back in ngpb_set_mesh's layout: sizes receives {n_nodes,
+   n_hang, n_leaves, n_boundary}; the arrays (each may be NULL) receive
+   node_xyz[3 N], hang_xyz[3 H], hang_np[H], hang_par[4 H], leaf_corner[8 L] and the
+   Dirichlet node list bnd_nodes[n_boundary]. No counterpart in the reference. */
+int ngpb_get_mesh(ngpb_ctx* ctx, int64_t* sizes, double* node_xyz, double* hang_xyz, int32_t* hang_np,
+                  int32_t* hang_par, int32_t* leaf_corner, int32_t* bnd_nodes);
 
 /* Rays of one axis: key[2r], key[2r+1] are the two other coordinates in increasing
    axis order ((y,z) for x rays, (x,z) for y rays, (x,y) for z rays), keys strictly
