@@ -42,15 +42,72 @@ def test_octree_is_valid_and_balanced(maker):
         s = 1 << m.leaf_level[l]
         x, y, z = idx[l]
         lev[x:x + s, y:y + s, z:z + s] = m.leaf_level[l]
-    for d in range(3):
-        for shift in (1, -1):
-            a = np.roll(lev, shift, axis=d)
-            sl = [slice(None)] * 3
-            sl[d] = slice(1, None) if shift == 1 else slice(None, -1)
-            assert np.abs(a[tuple(sl)].astype(int) - lev[tuple(sl)]).max() <= 1
+    assert _max_neighbour_jump(lev) <= 1
     # nodes are numbered lexicographically (x fastest, then y, then z)
     key = np.lexsort((m.node_xyz[:, 0], m.node_xyz[:, 1], m.node_xyz[:, 2]))
     assert np.array_equal(key, np.arange(m.n_nodes))
+
+
+def _max_neighbour_jump(lev):
+    """Largest level difference between finest cells that share a face, an edge or a
+    corner (all 26 directions)."""
+    n = lev.shape
+    worst = 0
+    for dx in (-1, 0, 1):
+        for dy in (-1, 0, 1):
+            for dz in (-1, 0, 1):
+                if (dx, dy, dz) == (0, 0, 0):
+                    continue
+                a = lev[max(dx, 0):n[0] + min(dx, 0), max(dy, 0):n[1] + min(dy, 0), max(dz, 0):n[2] + min(dz, 0)]
+                b = lev[max(-dx, 0):n[0] + min(-dx, 0), max(-dy, 0):n[1] + min(-dy, 0), max(-dz, 0):n[2] + min(-dz, 0)]
+                worst = max(worst, int(np.abs(a.astype(int) - b).max()))
+    return worst
+
+
+def _split_margin(m, atoms, anchor, s):
+    """min_a (|centre - c_a| - r_a) - (band[s-1] + sqrt(3)/2 * edge) for cells of level
+    s >= 1 with the given anchors (finest-grid units); negative means split."""
+    sz = (2.0 ** s) * m.hf
+    c = m.origin + anchor * m.hf + 0.5 * sz[:, None]
+    d = np.sqrt(((c[:, None, :] - atoms.xyz[None, :, :]) ** 2).sum(axis=2)) - atoms.r[None, :]
+    return d.min(axis=1) - (np.asarray(m.bands)[s - 1] + 0.5 * np.sqrt(3.0) * sz)
+
+
+@pytest.mark.parametrize("maker", [lambda: W.three_spheres(), lambda: W.three_spheres(levels=3, bands=(0.8, 2.0, 5.0)),
+                                   lambda: W.globular(n_atoms=40, ball_radius=5.0, min_sep=2.0, fine_band=1.5,
+                                                      mid_band=3.0)])
+def test_octree_refinement_rule_brute_force(maker):
+    """The generator's leaves against the rule evaluated over all atoms: no leaf of
+    level >= 1 meets its split test; every leaf below the root level has a parent cell
+    that either meets the split test or touches (faces, edges, corners) a leaf two or
+    more levels finer, which is what makes the balance step split it; and the mesh is
+    2:1 balanced in all 26 directions."""
+    P = maker()
+    m, at = P.mesh, P.atoms
+    lo, _ = _leaf_boxes(m)
+    idx = np.rint((lo - m.origin) / m.hf).astype(np.int64)
+    s = m.leaf_level.astype(np.int64)
+    eps = 1e-9
+    big = s >= 1
+    assert np.all(_split_margin(m, at, idx[big], s[big]) >= -eps)
+    lev = np.zeros(tuple(m.n), np.int8)
+    for l in range(m.n_leaves):
+        e = 1 << s[l]
+        x, y, z = idx[l]
+        lev[x:x + e, y:y + e, z:z + e] = s[l]
+    assert _max_neighbour_jump(lev) <= 1
+    child = s < m.smax
+    ps = s[child] + 1
+    pa = (idx[child] >> ps[:, None]) << ps[:, None]
+    keyed = np.unique(np.column_stack([pa, ps]), axis=0)
+    pa, ps = keyed[:, :3], keyed[:, 3]
+    by_rule = _split_margin(m, at, pa, ps) < eps
+    for k in np.nonzero(~by_rule)[0]:
+        e = 1 << ps[k]
+        x0, y0, z0 = np.maximum(pa[k] - 1, 0)
+        x1, y1, z1 = np.minimum(pa[k] + e + 1, m.n)
+        assert lev[x0:x1, y0:y1, z0:z1].min() < ps[k] - 1, (pa[k], ps[k])
+    assert by_rule.any()
 
 
 def test_rays_are_sorted_and_on_spheres():
