@@ -212,18 +212,20 @@ def bench_serve(args):
     tot_n = sum(r.n for r in reqs)
     log(f"{len(reqs)} requests generated in {gen_s:.1f} s: N {min(r.n for r in reqs)}..{max(r.n for r in reqs)}, "
         f"total {tot_n / 1e6:.1f}M unknowns, span {reqs[-1].arrival:.1f} s")
-    srv = serve.Server(budget_bytes=250e9, small_max_unknowns=1_500_000, max_batch=256, tol=args.tol)
+    dm = args.device_mesh
+    srv = serve.Server(budget_bytes=250e9, small_max_unknowns=1_500_000, max_batch=256, tol=args.tol,
+                       device_mesh=dm)
     # warm-up: one small request through both paths
     warm = [r for r in reqs if r.n <= 200_000][:1]
     if warm:
         srv.run(warm, realtime=False)
-        serve.run_sequential_fp64(warm, realtime=False, tol=args.tol)
+        serve.run_sequential_fp64(warm, realtime=False, tol=args.tol, device_mesh=dm)
     res = srv.run(reqs, realtime=True)
     peak = srv.peak_bytes
     srv.close()
     summ = serve.summarize(res)
     log(f"continuous batching: {summ}")
-    seq = serve.run_sequential_fp64(reqs, realtime=True, tol=args.tol)
+    seq = serve.run_sequential_fp64(reqs, realtime=True, tol=args.tol, device_mesh=dm)
     seq_summ = serve.summarize(seq)
     log(f"sequential FP64: {seq_summ}")
     errs = [abs(res[k].energy["polar"] - seq[k].energy["polar"]) / abs(seq[k].energy["polar"]) for k in res]
@@ -244,7 +246,9 @@ def bench_serve(args):
             "workload": (f"serving stream: {len(reqs)} requests, Poisson arrivals at {args.rate:g}/s (seed 5), atoms "
                          f"log-uniform in [50, 20000], globular generator, 0.5 A leaves within 3 A of the atoms, 1 A "
                          f"within 6 A, 2 A beyond; mixed solve tol {args.tol:g} + energy; admission by predicted HBM "
-                         f"footprint within 250 GB; systems <= 1.5M unknowns batched, larger streamed"),
+                         f"footprint within 250 GB; systems <= 1.5M unknowns batched, larger streamed; "
+                         + ("requests arrive as atoms and box, octree, rays and owners built on the device"
+                            if dm else "requests arrive with host-built mesh, ray table and owners")),
             "requests": len(reqs),
             "unknowns_total": tot_n,
             "parallelism": "single GPU",
@@ -265,6 +269,8 @@ def main():
     ap.add_argument("--requests", type=int, default=48, help="serve workload: number of requests")
     ap.add_argument("--rate", type=float, default=8.0, help="serve workload: Poisson arrival rate [1/s]")
     ap.add_argument("--systems", type=int, default=512, help="batch workload: number of ligand systems")
+    ap.add_argument("--device-mesh", action="store_true",
+                    help="serve workload: requests carry only atoms and box; mesh, rays and owners built on the GPU")
     ap.add_argument("--tol", type=float, default=1e-10)
     ap.add_argument("--check-every", type=int, default=32)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")

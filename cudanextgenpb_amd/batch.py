@@ -33,17 +33,28 @@ class Prepared:
     meta: Dict = field(default_factory=dict)
 
 
-def prepare(problem: Problem, device: int = 0, ctx: Optional[native.Context] = None) -> Prepared:
-    """Uploads the problem and assembles it on the device (no solve)."""
+def prepare(problem: Problem, device: int = 0, ctx: Optional[native.Context] = None,
+            device_mesh: bool = False) -> Prepared:
+    """Uploads the problem and assembles it on the device (no solve). device_mesh: only
+    the atoms and the box parameters cross to the device; the octree with its boundary
+    list, the ray table and the owner leaves are built there (identical to the
+    problem's host-built ones)."""
     t0 = time.perf_counter()
     c = ctx or native.Context(device)
     c.set_physics(problem.physics)
-    c.set_mesh(problem.mesh)
-    c.set_rays(problem.rays)
-    c.create_markers()
     a = problem.atoms
-    c.set_charges(a.xyz, a.q, problem.owner)
-    c.set_dirichlet(problem.mesh.bnd_nodes, None)
+    if device_mesh:
+        m = problem.mesh
+        c.build_octree(m.origin, m.hf, m.n, m.smax, m.bands, a.xyz, a.r)
+        c.cast_rays(a.xyz, a.r)
+        c.create_markers()
+        c.set_charges(a.xyz, a.q, c.locate_atoms(a.xyz, problem.atom_tol))
+    else:
+        c.set_mesh(problem.mesh)
+        c.set_rays(problem.rays)
+        c.create_markers()
+        c.set_charges(a.xyz, a.q, problem.owner)
+        c.set_dirichlet(problem.mesh.bnd_nodes, None)
     c.dirichlet_values(problem.bc_type)
     c.assemble()
     return Prepared(problem.name, c, c.n, c.nnz, a.n, (time.perf_counter() - t0) * 1e3)

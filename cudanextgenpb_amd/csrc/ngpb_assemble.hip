@@ -703,14 +703,14 @@ void run_symbolic(ngpb_ctx* c) {
   const MeshView m = mesh_view_of(c);
   NGPB_REQUIRE(20 * m.L < (int64_t)UINT32_MAX, NGPB_E_ARG, "too many leaves for 32-bit emission ids");
   const int64_t N = c->N;
-  DBuf<unsigned> cnt;
+  DBuf<unsigned> cnt(c->stream);
   cnt.resize((size_t)N);
   NGPB_HIP(hipMemsetAsync(cnt.p, 0, N * sizeof(unsigned), c->stream));
   hipLaunchKernelGGL(k_emit<0>, dim3(blocks_for(m.L, 128)), dim3(128), 0, c->stream, m, cnt.p, nullptr, nullptr, nullptr);
   // per-row emission lists, sorted keys: needed only until the slots are cut
-  DBuf<int64_t> em_rowptr;
-  DBuf<uint64_t> em_key;  // (col << 32) | id
-  DBuf<uint16_t> em_kw;   // (k << 8) | weight code
+  DBuf<int64_t> em_rowptr(c->stream);
+  DBuf<uint64_t> em_key(c->stream);  // (col << 32) | id
+  DBuf<uint16_t> em_kw(c->stream);  // (k << 8) | weight code
   em_rowptr.resize((size_t)N + 1);
   hipLaunchKernelGGL(k_u32_to_i64, dim3(blocks_for(N, 256)), dim3(256), 0, c->stream, N, cnt.p, em_rowptr.p);
   NGPB_HIP(hipGetLastError());
@@ -722,12 +722,12 @@ void run_symbolic(ngpb_ctx* c) {
   hipLaunchKernelGGL(k_emit<1>, dim3(blocks_for(m.L, 128)), dim3(128), 0, c->stream, m, cnt.p, em_rowptr.p, em_key.p,
                      em_kw.p);
   NGPB_HIP(hipGetLastError());
-  DBuf<int64_t> fcount, bcount;
+  DBuf<int64_t> fcount(c->stream), bcount(c->stream);
   fcount.resize((size_t)N);
   bcount.resize((size_t)N);
   {
-    DBuf<int32_t> big;
-    DBuf<int> nbig;
+    DBuf<int32_t> big(c->stream);
+    DBuf<int> nbig(c->stream);
     big.resize((size_t)N);
     nbig.resize(1);
     NGPB_HIP(hipMemsetAsync(nbig.p, 0, sizeof(int), c->stream));
@@ -754,7 +754,7 @@ void run_symbolic(ngpb_ctx* c) {
   c->diag_slot.resize((size_t)N);
   c->bc_rowptr.resize((size_t)N + 1);
   c->bc_col.resize((size_t)std::max<int64_t>(nbc, 1));
-  DBuf<int64_t> slot_em, bc_em;  // 2 per slot / coupling: range in the sorted lists
+  DBuf<int64_t> slot_em(c->stream), bc_em(c->stream);  // 2 per slot / coupling: range in the sorted lists
   slot_em.resize((size_t)2 * std::max<int64_t>(nnz, 1));
   bc_em.resize((size_t)2 * std::max<int64_t>(nbc, 1));
   hipLaunchKernelGGL(k_row_fill, dim3(blocks_for(N, 128)), dim3(128), 0, c->stream, N, em_rowptr.p, em_key.p,
@@ -764,7 +764,7 @@ void run_symbolic(ngpb_ctx* c) {
   // repack the slot emissions in slot order, 5 bytes each (source id, weight code),
   // with block-relative slot offsets; the couplings to Dirichlet nodes likewise
   {
-    DBuf<int64_t> sptr;
+    DBuf<int64_t> sptr(c->stream);
     sptr.resize((size_t)nnz + 1);
     hipLaunchKernelGGL(k_range_len, dim3(blocks_for(nnz, 256)), dim3(256), 0, c->stream, nnz, slot_em.p, sptr.p);
     const int64_t tot = exclusive_scan(c, sptr.p, sptr.p, nnz);
@@ -816,7 +816,7 @@ void run_numeric(ngpb_ctx* c) {
   NGPB_HIP(hipMemsetAsync(c->rhs_charge.p, 0, N * sizeof(double), c->stream));
   if (c->have_charges && c->A > 0) {
     const int64_t C = 32 * c->A;
-    DBuf<int32_t> cnode, cnext, head;
+    DBuf<int32_t> cnode(c->stream), cnext(c->stream), head(c->stream);
     cnode.resize((size_t)C);
     cnext.resize((size_t)C);
     head.resize((size_t)N);

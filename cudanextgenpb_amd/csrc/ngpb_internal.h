@@ -33,16 +33,29 @@ struct Error : std::runtime_error {
   } while (0)
 
 // --------------------------------------------------------------- device buffers
+// A buffer constructed with a stream (or given one by order_on) is allocated and
+// freed in that stream's order (hipMallocAsync / hipFreeAsync): freeing it waits
+// neither for the device nor for other streams, which matters for the temporaries of
+// one call while other contexts' work runs on the same GPU. The others use
+// hipMalloc / hipFree (the resident arrays of a context).
 template <class T>
 struct DBuf {
   T* p = nullptr;
   size_t n = 0;
+  hipStream_t os = nullptr;
+  bool ordered = false;
   DBuf() = default;
+  explicit DBuf(hipStream_t s) : os(s), ordered(true) {}
   DBuf(const DBuf&) = delete;
   DBuf& operator=(const DBuf&) = delete;
   ~DBuf() { release(); }
+  void order_on(hipStream_t s) {
+    release();
+    os = s;
+    ordered = true;
+  }
   void release() {
-    if (p) (void)hipFree(p);
+    if (p) (void)(ordered ? hipFreeAsync(p, os) : hipFree(p));
     p = nullptr;
     n = 0;
   }
@@ -51,7 +64,7 @@ struct DBuf {
     if (count == n && p) return;
     release();
     if (count == 0) return;
-    hipError_t e = hipMalloc((void**)&p, count * sizeof(T));
+    hipError_t e = ordered ? hipMallocAsync((void**)&p, count * sizeof(T), os) : hipMalloc((void**)&p, count * sizeof(T));
     if (e != hipSuccess) {
       p = nullptr;
       (void)hipGetLastError();

@@ -155,7 +155,7 @@ void run_locate_atoms(ngpb_ctx* c, int64_t A, const double* h_xyz, double tol, i
     std::fill(h_owner, h_owner + A, -1);
     return;
   }
-  DBuf<double> mn, mx, edge, part, ext;
+  DBuf<double> mn(c->stream), mx(c->stream), edge(c->stream), part(c->stream), ext(c->stream);
   mn.resize((size_t)3 * L);
   mx.resize((size_t)3 * L);
   edge.resize((size_t)L);
@@ -178,8 +178,8 @@ void run_locate_atoms(ngpb_ctx* c, int64_t A, const double* h_xyz, double tol, i
     nb *= G.dim[d];
   }
   NGPB_REQUIRE(nb < (int64_t)INT32_MAX, NGPB_E_ARG, "leaf bucket grid too large");
-  DBuf<int32_t> bucket, cnt, cursor, list, owner;
-  DBuf<int64_t> start;
+  DBuf<int32_t> bucket(c->stream), cnt(c->stream), cursor(c->stream), list(c->stream), owner(c->stream);
+  DBuf<int64_t> start(c->stream);
   bucket.resize((size_t)L);
   cnt.resize((size_t)nb);
   cursor.resize((size_t)nb);
@@ -195,7 +195,7 @@ void run_locate_atoms(ngpb_ctx* c, int64_t A, const double* h_xyz, double tol, i
   NGPB_HIP(hipMemcpyAsync(start.p + nb, &total, sizeof(int64_t), hipMemcpyHostToDevice, c->stream));
   hipLaunchKernelGGL(k_leaf_fill, dim3(blocks_for(L, 256)), dim3(256), 0, c->stream, L, bucket.p, start.p, cursor.p,
                      list.p);
-  DBuf<double> axyz;
+  DBuf<double> axyz(c->stream);
   axyz.upload(h_xyz, (size_t)(3 * A), c->stream);
   hipLaunchKernelGGL(k_locate, dim3(blocks_for(A, 128)), dim3(128), 0, c->stream, A, axyz.p, tol, G, start.p, list.p,
                      mn.p, mx.p, owner.p);

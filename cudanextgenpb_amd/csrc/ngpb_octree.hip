@@ -383,8 +383,8 @@ OctreeSizes run_build_octree(ngpb_ctx* c, const double* origin, double hf, const
   AtomBins B{};
   B.na = (int)na;
   B.cell = 4.0;
-  DBuf<double> bxyz, br;
-  DBuf<int32_t> bptr, batom;
+  DBuf<double> bxyz(st), br(st);
+  DBuf<int32_t> bptr(st), batom(st);
   {
     double lo[3] = {0, 0, 0}, hi[3] = {0, 0, 0};
     if (na) {
@@ -446,19 +446,18 @@ OctreeSizes run_build_octree(ngpb_ctx* c, const double* origin, double hf, const
   }
 
   // ---- top-down refinement, one level per pass
-  DBuf<OLeaf> work, next, kept[kMaxLevels + 1];
+  DBuf<OLeaf> work(st), next(st), kept[kMaxLevels + 1];
+  for (auto& k : kept) k.order_on(st);
   int64_t nkept[kMaxLevels + 1] = {0};
-  DBuf<uint8_t> flag;
-  DBuf<int64_t> cnt;
+  DBuf<uint8_t> flag(st);
+  DBuf<int64_t> cnt(st);
   int64_t nw = (n[0] / R) * (n[1] / R) * (n[2] / R);
   work.resize((size_t)nw);
   hipLaunchKernelGGL(k_roots, dim3(blocks_for(nw, 256)), dim3(256), 0, st, g, smax, n[0] / R, n[1] / R, nw, work.p);
   NGPB_HIP(hipGetLastError());
   int64_t L = 0;
-  // Buffers are only resized or freed after a stream synchronization, so that no
-  // queued kernel still uses them.
+  // Temporaries are allocated and freed in the stream's order (no device-wide wait).
   for (int pass = 0; nw > 0; ++pass) {
-    NGPB_HIP(hipStreamSynchronize(st));
     flag.resize((size_t)nw);
     cnt.resize((size_t)nw);
     hipLaunchKernelGGL(k_refine_flag, dim3(blocks_for(nw, 128)), dim3(128), 0, st, work.p, nw, g, B, flag.p, cnt.p);
@@ -472,7 +471,7 @@ OctreeSizes run_build_octree(ngpb_ctx* c, const double* origin, double hf, const
     nw = 8 * ks.second;
   }
   NGPB_REQUIRE(L > 0 && L < INT32_MAX, NGPB_E_ARG, "leaf count out of range");
-  DBuf<OLeaf> leaves;
+  DBuf<OLeaf> leaves(st);
   leaves.resize((size_t)L);
   {
     int64_t o = 0;
@@ -481,19 +480,17 @@ OctreeSizes run_build_octree(ngpb_ctx* c, const double* origin, double hf, const
         NGPB_HIP(hipMemcpyAsync(leaves.p + o, kept[pass].p, nkept[pass] * sizeof(OLeaf), hipMemcpyDeviceToDevice, st));
         o += nkept[pass];
       }
-    NGPB_HIP(hipStreamSynchronize(st));
     for (auto& k : kept) k.release();
   }
   work.release();
 
   // ---- 2:1 balance on the level map of the finest cells
   {
-    DBuf<int8_t> lev;
+    DBuf<int8_t> lev(st);
     lev.resize((size_t)ncell);
     hipLaunchKernelGGL(k_paint, dim3(blocks_for(L, 64)), dim3(64), 0, st, leaves.p, L, nullptr, g, lev.p);
     NGPB_HIP(hipGetLastError());
     for (int sweep = 0; sweep < kBalanceSweeps; ++sweep) {
-      NGPB_HIP(hipStreamSynchronize(st));
       flag.resize((size_t)L);
       cnt.resize((size_t)L);
       hipLaunchKernelGGL(k_balance_flag, dim3(blocks_for(L, 128)), dim3(128), 0, st, leaves.p, L, g, lev.p, flag.p,
@@ -512,14 +509,13 @@ OctreeSizes run_build_octree(ngpb_ctx* c, const double* origin, double hf, const
       std::swap(leaves.n, next.n);
       L = nk + 8 * ns;
     }
-    NGPB_HIP(hipStreamSynchronize(st));
   }
   next.release();
 
   // ---- Morton order
   {
-    DBuf<uint64_t> key, key2;
-    DBuf<int32_t> idx, idx2;
+    DBuf<uint64_t> key(st), key2(st);
+    DBuf<int32_t> idx(st), idx2(st);
     key.resize((size_t)L);
     key2.resize((size_t)L);
     idx.resize((size_t)L);
@@ -530,7 +526,7 @@ OctreeSizes run_build_octree(ngpb_ctx* c, const double* origin, double hf, const
     while (((int64_t)1 << bits) < std::max({n[0], n[1], n[2]})) ++bits;
     size_t tb = 0;
     NGPB_HIP(hipcub::DeviceRadixSort::SortPairs(nullptr, tb, key.p, key2.p, idx.p, idx2.p, (int)L, 0, 3 * bits, st));
-    DBuf<uint8_t> tmp;
+    DBuf<uint8_t> tmp(st);
     tmp.resize(std::max<size_t>(tb, 1));
     NGPB_HIP(hipcub::DeviceRadixSort::SortPairs(tmp.p, tb, key.p, key2.p, idx.p, idx2.p, (int)L, 0, 3 * bits, st));
     next.resize((size_t)L);
@@ -538,13 +534,12 @@ OctreeSizes run_build_octree(ngpb_ctx* c, const double* origin, double hf, const
     NGPB_HIP(hipGetLastError());
     std::swap(leaves.p, next.p);
     std::swap(leaves.n, next.n);
-    NGPB_HIP(hipStreamSynchronize(st));
     next.release();
   }
 
   // ---- vertices
-  DBuf<int32_t> vmap;
-  DBuf<int64_t> off;
+  DBuf<int32_t> vmap(st);
+  DBuf<int64_t> off(st);
   vmap.resize((size_t)V);
   off.resize((size_t)V);
   NGPB_HIP(hipMemsetAsync(vmap.p, 0, (size_t)V * sizeof(int32_t), st));
@@ -563,9 +558,8 @@ OctreeSizes run_build_octree(ngpb_ctx* c, const double* origin, double hf, const
   c->leaf_corner.resize((size_t)(8 * L));
   hipLaunchKernelGGL(k_number, dim3(blocks_for(V, 256)), dim3(256), 0, st, vmap.p, off.p, V, g, c->node_xyz.p,
                      c->is_bnd.p, c->hang_xyz.p);
-  NGPB_HIP(hipStreamSynchronize(st));
   off.release();
-  DBuf<int32_t> err;
+  DBuf<int32_t> err(st);
   err.resize(1);
   NGPB_HIP(hipMemsetAsync(err.p, 0, sizeof(int32_t), st));
   hipLaunchKernelGGL(k_hang_parents, dim3(blocks_for(L, 128)), dim3(128), 0, st, leaves.p, L, g, vmap.p,

@@ -202,7 +202,7 @@ void run_cast_rays(ngpb_ctx* c, int64_t na, const double* h_xyz, const double* h
   double rmax = 0;
   for (int64_t a = 0; a < na; ++a) rmax = std::max(rmax, h_r[a]);
   const double cell = std::max(2.0 * rmax, 1.0);
-  DBuf<double> bxyz, br;
+  DBuf<double> bxyz(c->stream), br(c->stream);
   if (na > 0) {
     bxyz.upload(h_xyz, (size_t)(3 * na), c->stream);
     br.upload(h_r, (size_t)na, c->stream);
@@ -235,13 +235,13 @@ void run_cast_rays(ngpb_ctx* c, int64_t na, const double* h_xyz, const double* h
       std::vector<int32_t> cur(bptr.begin(), bptr.end() - 1);
       for (int64_t a = 0; a < na; ++a) bball[(size_t)cur[(size_t)bin_of[a]]++] = (int32_t)a;
     }
-    DBuf<int32_t> d_bptr, d_bball;
+    DBuf<int32_t> d_bptr(c->stream), d_bball(c->stream);
     d_bptr.upload(bptr.data(), bptr.size(), c->stream);
     d_bball.upload(bball.data(), bball.size(), c->stream);
     BallGrid G{bxyz.p, br.p, d_bptr.p, d_bball.p, lo[0], lo[1], cell, gn0, gn1};
 
     // distinct lines in lexicographic order
-    DBuf<uint64_t> ku, kv, su, sv;
+    DBuf<uint64_t> ku(c->stream), kv(c->stream), su(c->stream), sv(c->stream);
     ku.resize((size_t)NV);
     kv.resize((size_t)NV);
     su.resize((size_t)NV);
@@ -252,19 +252,19 @@ void run_cast_rays(ngpb_ctx* c, int64_t na, const double* h_xyz, const double* h
     size_t tb1 = 0, tb2 = 0;
     NGPB_HIP(hipcub::DeviceRadixSort::SortPairs(nullptr, tb1, kv.p, sv.p, ku.p, su.p, (int)NV, 0, 64, c->stream));
     NGPB_HIP(hipcub::DeviceRadixSort::SortPairs(nullptr, tb2, su.p, ku.p, sv.p, kv.p, (int)NV, 0, 64, c->stream));
-    DBuf<uint8_t> tmp;
+    DBuf<uint8_t> tmp(c->stream);
     tmp.resize(std::max(tb1, tb2));
     // by the second coordinate, then stably by the first
     NGPB_HIP(hipcub::DeviceRadixSort::SortPairs(tmp.p, tb1, kv.p, sv.p, ku.p, su.p, (int)NV, 0, 64, c->stream));
     NGPB_HIP(hipcub::DeviceRadixSort::SortPairs(tmp.p, tb2, su.p, ku.p, sv.p, kv.p, (int)NV, 0, 64, c->stream));
-    DBuf<uint8_t> first;
+    DBuf<uint8_t> first(c->stream);
     first.resize((size_t)NV);
     hipLaunchKernelGGL(k_line_first, dim3(blocks_for(NV, 256)), dim3(256), 0, c->stream, NV, ku.p, kv.p, first.p);
-    DBuf<int32_t> uniq;
+    DBuf<int32_t> uniq(c->stream);
     uniq.resize((size_t)NV);
     const int64_t nl = compact_flags(c, first.p, NV, uniq.p);
     // line coordinates (keys) of the distinct lines
-    DBuf<uint64_t> lu, lv;
+    DBuf<uint64_t> lu(c->stream), lv(c->stream);
     lu.resize((size_t)std::max<int64_t>(nl, 1));
     lv.resize((size_t)std::max<int64_t>(nl, 1));
     auto& R = c->rays[axis];
@@ -281,23 +281,23 @@ void run_cast_rays(ngpb_ctx* c, int64_t na, const double* h_xyz, const double* h
     hipLaunchKernelGGL(k_gather_u64, dim3(blocks_for(nl, 256)), dim3(256), 0, c->stream, nl, uniq.p, ku.p, lu.p);
     hipLaunchKernelGGL(k_gather_u64, dim3(blocks_for(nl, 256)), dim3(256), 0, c->stream, nl, uniq.p, kv.p, lv.p);
     // pierced balls per line, their intervals sorted and merged
-    DBuf<int64_t> off, mcnt;
+    DBuf<int64_t> off(c->stream), mcnt(c->stream);
     off.resize((size_t)nl);
     mcnt.resize((size_t)nl);
     hipLaunchKernelGGL(k_count_pierced, dim3(blocks_for(nl, 128)), dim3(128), 0, c->stream, nl, lu.p, lv.p, G, d1, d2,
                        off.p);
     NGPB_HIP(hipGetLastError());
     const int64_t niv = exclusive_scan(c, off.p, off.p, nl);
-    DBuf<Iv> iv;
+    DBuf<Iv> iv(c->stream);
     iv.resize((size_t)std::max<int64_t>(niv, 1));
     hipLaunchKernelGGL(k_merge_pierced, dim3(blocks_for(nl, 128)), dim3(128), 0, c->stream, nl, lu.p, lv.p, G, axis,
                        d1, d2, off.p, iv.p, mcnt.p);
     NGPB_HIP(hipGetLastError());
     // rays = lines with crossings
-    DBuf<uint8_t> keep;
+    DBuf<uint8_t> keep(c->stream);
     keep.resize((size_t)nl);
     hipLaunchKernelGGL(k_nonzero, dim3(blocks_for(nl, 256)), dim3(256), 0, c->stream, nl, mcnt.p, keep.p);
-    DBuf<int32_t> kept;
+    DBuf<int32_t> kept(c->stream);
     kept.resize((size_t)nl);
     const int64_t nr = compact_flags(c, keep.p, nl, kept.p);
     R.n = nr;

@@ -111,8 +111,9 @@ class Result:
 class Server:
     def __init__(self, device: int = 0, budget_bytes: float = 250e9, small_max_unknowns: int = 1_500_000,
                  max_batch: int = 256, tol: float = 1e-10, inner_per_replay: int = 8, stream_workers: int = 2,
-                 prep_workers: int = 2, keep_phi: bool = False):
+                 prep_workers: int = 2, keep_phi: bool = False, device_mesh: bool = False):
         self.device = device
+        self.device_mesh = device_mesh
         self.budget = float(budget_bytes)
         self.small_max = small_max_unknowns
         self.tol = tol
@@ -130,7 +131,7 @@ class Server:
 
     def _stream_solve(self, req: Request):
         """Large request: prepared and solved on its own context (worker thread)."""
-        p = prepare(req.problem, self.device)
+        p = prepare(req.problem, self.device, device_mesh=self.device_mesh)
         _, st = p.ctx.solve(native.PREC_MIXED, tol=self.tol, want_phi=False)
         e = p.ctx.energy()
         phi = p.ctx.get_phi() if self.keep_phi else None
@@ -169,7 +170,8 @@ class Server:
                 used += req.predicted_bytes
                 self.peak_bytes = max(self.peak_bytes, used)
                 if small:
-                    preparing.append((req, clock(), self.prep_pool.submit(prepare, req.problem, self.device)))
+                    fut = self.prep_pool.submit(prepare, req.problem, self.device, None, self.device_mesh)
+                    preparing.append((req, clock(), fut))
                 else:
                     streaming.append((req, clock(), self.pool.submit(self._stream_solve, req)))
             progressed = False
@@ -223,7 +225,7 @@ class Server:
 
 
 def run_sequential_fp64(requests: List[Request], device: int = 0, tol: float = 1e-10,
-                        realtime: bool = True) -> Dict[int, Result]:
+                        realtime: bool = True, device_mesh: bool = False) -> Dict[int, Result]:
     """The same arrival stream served one request at a time with the FP64 solver."""
     reqs = sorted(requests, key=lambda r: r.arrival)
     t0 = time.perf_counter()
@@ -234,7 +236,7 @@ def run_sequential_fp64(requests: List[Request], device: int = 0, tol: float = 1
             if wait > 0:
                 time.sleep(wait)
         adm = time.perf_counter() - t0
-        p = prepare(req.problem, device)
+        p = prepare(req.problem, device, device_mesh=device_mesh)
         _, st = p.ctx.solve(native.PREC_FP64, tol=tol, want_phi=False)
         e = p.ctx.energy()
         t = time.perf_counter() - t0

@@ -118,3 +118,24 @@ def test_invalid_octree_arguments_are_rejected(args):
             ctx.get_mesh()
     finally:
         ctx.close()
+
+
+def test_prepare_from_device_mesh_equals_host_mesh():
+    """The serving/batch preparation from atoms alone (device octree, rays, owners and
+    boundary list) assembles the same system and solves to the same energies."""
+    from cudanextgenpb_amd.batch import prepare
+    P = W.ligand(5)
+    print(f"[octree prepare] ligand_5 N={P.mesh.n_nodes}", flush=True)
+    a, b = prepare(P), prepare(P, device_mesh=True)
+    try:
+        assert (a.n, a.nnz) == (b.n, b.nnz)
+        for x, y in zip(a.ctx.export_csr(), b.ctx.export_csr()):
+            assert np.array_equal(x, y)
+        ea, eb = [], []
+        for p, out in ((a, ea), (b, eb)):
+            p.ctx.solve(native.PREC_MIXED, want_phi=False)
+            out.append(p.ctx.energy())
+        assert ea == eb
+    finally:
+        a.ctx.close()
+        b.ctx.close()
