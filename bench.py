@@ -260,12 +260,86 @@ def bench_serve(args):
     }
 
 
+def bench_mesh(args):
+    """Per-request mesh preparation on the device (octree, ray table, atom owners) for
+    the globular (or large) system, against the host generator of the same mesh."""
+    import numpy as np
+    from cudanextgenpb_amd import native, workload as W
+
+    big = args.mesh_size == "large"
+    t = time.perf_counter()
+    P = W.large() if big else W.globular()
+    host_problem_s = time.perf_counter() - t
+    m, a = P.mesh, P.atoms
+    t = time.perf_counter()
+    W.make_mesh(m.origin, m.hf, m.n, m.smax, m.bands, a).close()
+    host_mesh_s = time.perf_counter() - t
+    log(f"host: problem {host_problem_s:.2f} s, mesh alone {host_mesh_s:.3f} s; N={m.n_nodes} L={m.n_leaves}")
+    ctx = native.Context(0)
+    ctx.set_physics(P.physics)
+    tb, tr, tl = [], [], []
+    for it in range(args.warmup + args.steps):
+        t0 = time.perf_counter()
+        sizes = ctx.build_octree(m.origin, m.hf, m.n, m.smax, m.bands, a.xyz, a.r)
+        t1 = time.perf_counter()
+        ctx.cast_rays(a.xyz, a.r)
+        t2 = time.perf_counter()
+        owner = ctx.locate_atoms(a.xyz, P.atom_tol)
+        t3 = time.perf_counter()
+        if it >= args.warmup:
+            tb.append(t1 - t0)
+            tr.append(t2 - t1)
+            tl.append(t3 - t2)
+    g = ctx.get_mesh()
+    same = all(np.array_equal(g[k], getattr(m, k))
+               for k in ("node_xyz", "hang_xyz", "hang_np", "hang_par", "leaf_corner", "bnd_nodes"))
+    same = same and np.array_equal(owner, P.owner)
+    for axis in range(3):
+        key, ptr, inters, normals = ctx.get_rays(axis)
+        same = same and np.array_equal(key, P.rays[axis].key) and np.array_equal(inters, P.rays[axis].inters)
+    ctx.close()
+    if not same:
+        raise SystemExit("device mesh, rays or owners differ from the host generator's")
+    med = lambda v: float(np.median(v)) * 1e3
+    total = med([x + y + z for x, y, z in zip(tb, tr, tl)])
+    return {
+        "metric": "per-request mesh preparation time (octree + rays + atom owners)",
+        "value": total,
+        "unit": "ms",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": total,
+        "higher_is_better": False,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64+int32",
+        "data": "synthetic",
+        "config": {
+            "workload": ("large complex, 40000 atoms, 0.35 A finest leaves" if big else
+                         "globular protein, 2500 atoms, 0.5 A finest leaves") + ", 2 refinement levels, 2:1 balanced",
+            "nodes": sizes[0], "hanging": sizes[1], "leaves": sizes[2], "boundary_nodes": sizes[3],
+            "parallelism": "single GPU",
+        },
+        "octree_ms": med(tb),
+        "rays_ms": med(tr),
+        "locate_ms": med(tl),
+        "identical_to_host": True,
+        "cpu_baseline": {"value": host_mesh_s * 1e3, "unit": "ms", "kind": "port",
+                         "cores": int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or (os.cpu_count() or 1),
+                         "sample": "host generator of the same octree (csrc/workload.cpp, OpenMP), one build; "
+                                   f"with rays and owners: {host_problem_s * 1e3:.0f} ms"},
+    }
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=8)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--workload", choices=["globular", "born", "large", "batch", "serve"], default="globular")
+    ap.add_argument("--workload", choices=["globular", "born", "large", "batch", "serve", "mesh"], default="globular")
+    ap.add_argument("--mesh-size", choices=["globular", "large"], default="globular",
+                    help="mesh workload: which system's mesh to build")
     ap.add_argument("--requests", type=int, default=48, help="serve workload: number of requests")
     ap.add_argument("--rate", type=float, default=8.0, help="serve workload: Poisson arrival rate [1/s]")
     ap.add_argument("--systems", type=int, default=512, help="batch workload: number of ligand systems")
@@ -294,6 +368,9 @@ def main():
         return
     if args.workload == "serve":
         print(json.dumps(bench_serve(args)), flush=True)
+        return
+    if args.workload == "mesh":
+        print(json.dumps(bench_mesh(args)), flush=True)
         return
 
     t = time.perf_counter()

@@ -31,59 +31,85 @@ constexpr int kSvRows = 8;  // rows per block of the repacked slot emissions
 
 __device__ __forceinline__ int ilog2_small(int v) { return v == 1 ? 0 : v == 2 ? 1 : v == 4 ? 2 : v == 8 ? 3 : 4; }
 
-struct Parents {
-  int np[8];
-  int32_t par[8][4];
-};
-
-__device__ __forceinline__ void leaf_parents(const MeshView& m, int64_t l, Parents& P) {
-#pragma unroll
-  for (int i = 0; i < 8; ++i) {
-    const int32_t c = m.leaf_corner[8 * l + i];
-    if (c >= 0) {
-      P.np[i] = 1;
-      P.par[i][0] = c;
-    } else {
-      const int64_t h = -1 - (int64_t)c;
-      P.np[i] = m.hang_np[h];
-      for (int j = 0; j < 4; ++j) P.par[i][j] = m.hang_par[4 * h + j];
-    }
+__device__ __forceinline__ void corner_parents(const MeshView& m, int64_t l, int i, int& np, int32_t par[4]) {
+  const int32_t c = m.leaf_corner[8 * l + i];
+  if (c >= 0) {
+    np = 1;
+    par[0] = c;
+  } else {
+    const int64_t h = -1 - (int64_t)c;
+    np = m.hang_np[h];
+    for (int j = 0; j < 4; ++j) par[j] = m.hang_par[4 * h + j];
   }
 }
 
-// pass 0: count per row; pass 1: write emissions at em_rowptr[row] + cursor
+// Emissions (row, column, id, k, code), the contributions the numeric phase sums per
+// matrix slot in (column, id, k) order:
+//   edge terms: for leaf l, edge e = (loc[0], loc[1]) (kEN order) and each pair
+//     (r, s) of its end corners, each parent pr of corner loc[r] (the row) and parent
+//     ps of corner loc[s] (the column) give id = 12 l + e, k = the emission's place
+//     in the edge's order (r, s) = (0,0), (0,1), (1,0), (1,1), then pr, then ps, and
+//     code = (r != s ? 0x80 : 0) | log2(np[loc[r]] np[loc[s]]);
+//   mass terms: corner i of leaf l gives, for each parent j, row = column = par_j,
+//     id = 12 L + 8 l + i, k = j, code = log2 np[i].
+// (id, k) is unique, so sorting a row by (column, id, k) fixes its order whatever the
+// order of writing. One thread per (leaf, corner i) writes, for each parent p of
+// corner i, every emission whose row is p -- the (i, i) and (i, j) terms of the 3
+// edges through i and the mass term of i -- into one chunk reserved with a single
+// atomic per (leaf, corner, parent); pass 0 only counts.
 template <int PASS>
-__global__ void k_emit(MeshView m, unsigned* __restrict__ cnt, const int64_t* __restrict__ em_rowptr,
-                       uint64_t* __restrict__ em_key, uint16_t* __restrict__ em_kw) {
-  const int64_t l = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (l >= m.L) return;
-  Parents P;
-  leaf_parents(m, l, P);
-  auto emit = [&](int32_t row, int32_t col, uint32_t id, int k, int code) {
-    if (PASS == 0) {
-      atomicAdd(&cnt[row], 1u);
-    } else {
-      const int64_t pos = em_rowptr[row] + atomicAdd(&cnt[row], 1u);
-      em_key[pos] = ((uint64_t)(uint32_t)col << 32) | (uint64_t)id;
-      em_kw[pos] = (uint16_t)((k << 8) | code);
-    }
-  };
+__global__ void k_emit_rows(MeshView m, unsigned* __restrict__ cnt, const int64_t* __restrict__ em_rowptr,
+                            uint64_t* __restrict__ em_key, uint16_t* __restrict__ em_kw) {
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= 8 * m.L) return;
+  const int64_t l = t >> 3;
+  const int i = (int)(t & 7);
+  int npi;
+  int32_t pi[4];
+  corner_parents(m, l, i, npi, pi);
+  // the 3 edges through corner i: edge id, position r of i in it, partner parents
+  int eid[3], rpos[3], npj[3];
+  int32_t pj[3][4];
+  int ne = 0;
   for (int e = 0; e < 12; ++e) {
-    const int loc[2] = {kEN[2 * e], kEN[2 * e + 1]};
-    const uint32_t id = (uint32_t)(12 * l + e);
-    int k = 0;
-    for (int r = 0; r < 2; ++r)
-      for (int s = 0; s < 2; ++s) {
-        const int ir = loc[r], is = loc[s];
-        const int code = ((r == s) ? 0 : 0x80) | ilog2_small(P.np[ir] * P.np[is]);
-        for (int pr = 0; pr < P.np[ir]; ++pr)
-          for (int ps = 0; ps < P.np[is]; ++ps) emit(P.par[ir][pr], P.par[is][ps], id, k++, code);
-      }
+    const int a = kEN[2 * e], b = kEN[2 * e + 1];
+    if (a != i && b != i) continue;
+    eid[ne] = e;
+    rpos[ne] = a == i ? 0 : 1;
+    corner_parents(m, l, a == i ? b : a, npj[ne], pj[ne]);
+    ++ne;
   }
-  for (int i = 0; i < 8; ++i) {
-    const uint32_t id = (uint32_t)(12 * m.L + 8 * l + i);
-    const int code = ilog2_small(P.np[i]);
-    for (int j = 0; j < P.np[i]; ++j) emit(P.par[i][j], P.par[i][j], id, j, code);
+  unsigned per_row = 1;
+  for (int q = 0; q < 3; ++q) per_row += (unsigned)(npi + npj[q]);
+  for (int pr = 0; pr < npi; ++pr) {
+    const int32_t row = pi[pr];
+    if (PASS == 0) {
+      atomicAdd(&cnt[row], per_row);
+      continue;
+    }
+    int64_t pos = em_rowptr[row] + atomicAdd(&cnt[row], per_row);
+    for (int q = 0; q < 3; ++q) {
+      const int r = rpos[q];
+      const int npa = r == 0 ? npi : npj[q], npb = r == 0 ? npj[q] : npi;  // corners loc[0], loc[1]
+      const uint32_t id = (uint32_t)(12 * l + eid[q]);
+      for (int sgn = 0; sgn < 2; ++sgn) {
+        // pairs before (r, sgn) in the order (0,0), (0,1), (1,0), (1,1)
+        const int ns = sgn == 0 ? npa : npb;
+        int off = 0;
+        if (r == 1) off += npa * npa + npa * npb;
+        if (sgn == 1) off += (r == 0 ? npa : npb) * npa;
+        const bool self = sgn == r;  // column corner is i itself
+        const int code = (self ? 0 : 0x80) | ilog2_small(npi * ns);
+        for (int ps = 0; ps < ns; ++ps) {
+          const int32_t colv = self ? pi[ps] : pj[q][ps];
+          em_key[pos] = ((uint64_t)(uint32_t)colv << 32) | (uint64_t)id;
+          em_kw[pos] = (uint16_t)(((off + pr * ns + ps) << 8) | code);
+          ++pos;
+        }
+      }
+    }
+    em_key[pos] = ((uint64_t)(uint32_t)row << 32) | (uint64_t)(uint32_t)(12 * m.L + 8 * l + i);
+    em_kw[pos] = (uint16_t)((pr << 8) | ilog2_small(npi));
   }
 }
 
@@ -102,52 +128,13 @@ __device__ __forceinline__ bool em_less(uint64_t ka, uint16_t wa, uint64_t kb, u
   return ka < kb || (ka == kb && (wa >> 8) < (wb >> 8));
 }
 
-// sort each row's emissions; count final slots and Dirichlet couplings
-__global__ void k_row_sort(int64_t N, const int64_t* __restrict__ em_rowptr, uint64_t* __restrict__ key,
-                           uint16_t* __restrict__ kw, const uint8_t* __restrict__ is_bnd,
-                           int64_t* __restrict__ fcount, int64_t* __restrict__ bcount) {
-  const int64_t row = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (row >= N) return;
-  const int64_t b = em_rowptr[row], e = em_rowptr[row + 1];
-  for (int64_t i = b + 1; i < e; ++i) {
-    const uint64_t kk = key[i];
-    const uint16_t ww = kw[i];
-    int64_t j = i - 1;
-    while (j >= b && em_less(kk, ww, key[j], kw[j])) {
-      key[j + 1] = key[j];
-      kw[j + 1] = kw[j];
-      --j;
-    }
-    key[j + 1] = kk;
-    kw[j + 1] = ww;
-  }
-  const bool rb = is_bnd[row] != 0;
-  int64_t nf = 0, nbc = 0;
-  int64_t prev = -1;
-  for (int64_t i = b; i < e; ++i) {
-    const int64_t col = (int64_t)(key[i] >> 32);
-    if (col == prev) continue;
-    prev = col;
-    if (rb) {
-      if (col == row) ++nf;
-    } else if (is_bnd[col]) {
-      ++nbc;
-    } else {
-      ++nf;
-    }
-  }
-  fcount[row] = nf;
-  bcount[row] = nbc;
-}
-
-// The same sort and counts, in parallel: every element's final position is its rank
-// under (key, k, original position) -- the order the stable insertion sort above
-// produces. Rows of at most kSortCap emissions: one wave per row, staged in LDS.
-// Longer rows (parents of many hanging nodes) are listed and sorted by one block
-// per row with kSortBigCap entries staged; beyond that, the serial sort (lane 0).
-constexpr int kSortCap = 256;
+// Each row's emissions are sorted by (column, id, k) -- unique within a row -- and its
+// distinct columns counted as matrix slots or Dirichlet couplings. Rows of at most 64
+// emissions: one wave per row (k_row_sort_wave). Longer rows (parents of hanging
+// nodes) are listed and sorted by one block per row with up to kSortBigCap entries
+// in LDS; beyond that, an insertion sort by one lane.
 constexpr int kSortBS = 256;
-constexpr int kSortBigCap = 2048;  // 40 KB of LDS per block
+constexpr int kSortBigCap = 2048;  // 20 KB of LDS per block
 
 // distinct columns of a sorted run, counted as final slots or Dirichlet couplings
 __device__ __forceinline__ void count_slot(int64_t row, bool rb, int64_t col, const uint8_t* __restrict__ is_bnd,
@@ -161,59 +148,58 @@ __device__ __forceinline__ void count_slot(int64_t row, bool rb, int64_t col, co
   }
 }
 
+__device__ __forceinline__ bool em_before(uint64_t ka, int wa, uint64_t kb, int wb) {
+  return ka < kb || (ka == kb && wa < wb);
+}
+
+__device__ __forceinline__ uint64_t shfl_xor_u64(uint64_t v, int mask) {
+  const unsigned lo = __shfl_xor((unsigned)v, mask, 64), hi = __shfl_xor((unsigned)(v >> 32), mask, 64);
+  return ((uint64_t)hi << 32) | lo;
+}
+
+__device__ __forceinline__ uint64_t shfl_u64(uint64_t v, int src) {
+  const unsigned lo = __shfl((unsigned)v, src, 64), hi = __shfl((unsigned)(v >> 32), src, 64);
+  return ((uint64_t)hi << 32) | lo;
+}
+
+// Rows of at most 64 emissions: one wave per row, one emission per lane, sorted by a
+// bitonic network over the lanes (unused lanes hold keys that sort last). Longer rows
+// are listed for k_row_sort_block.
 __global__ void __launch_bounds__(kSortBS) k_row_sort_wave(int64_t N, const int64_t* __restrict__ em_rowptr,
                                                            uint64_t* __restrict__ key, uint16_t* __restrict__ kw,
                                                            const uint8_t* __restrict__ is_bnd,
                                                            int64_t* __restrict__ fcount, int64_t* __restrict__ bcount,
                                                            int32_t* __restrict__ big_rows, int* __restrict__ n_big) {
-  __shared__ uint64_t s_key[kSortBS / 64][kSortCap];
-  __shared__ uint64_t s_out[kSortBS / 64][kSortCap];
-  __shared__ uint16_t s_kw[kSortBS / 64][kSortCap];
-  __shared__ uint16_t s_kwo[kSortBS / 64][kSortCap];
   const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const int64_t row = (int64_t)blockIdx.x * (kSortBS / 64) + wv;
-  const bool valid = row < N;
-  const int64_t b = valid ? em_rowptr[row] : 0, e = valid ? em_rowptr[row + 1] : 0;
+  if (row >= N) return;  // the whole wave
+  const int64_t b = em_rowptr[row], e = em_rowptr[row + 1];
   const int E = (int)(e - b);
-  const bool staged = E <= kSortCap;
-  uint64_t* sk = s_key[wv];
-  uint64_t* so = s_out[wv];
-  uint16_t* sw = s_kw[wv];
-  uint16_t* swo = s_kwo[wv];
-  if (staged)
-    for (int i = lane; i < E; i += 64) {
-      sk[i] = key[b + i];
-      sw[i] = kw[b + i];
-    }
-  __syncthreads();
-  if (staged)
-    for (int i = lane; i < E; i += 64) {
-      const uint64_t kk = sk[i];
-      const int wk = sw[i] >> 8;
-      int rank = 0;
-      for (int j = 0; j < E; ++j) {
-        const uint64_t kj = sk[j];
-        const int wj = sw[j] >> 8;
-        rank += (kj < kk || (kj == kk && (wj < wk || (wj == wk && j < i)))) ? 1 : 0;
-      }
-      so[rank] = kk;
-      swo[rank] = sw[i];
-    }
-  __syncthreads();
-  if (!valid) return;
-  if (!staged) {
+  if (E > 64) {
     if (lane == 0) big_rows[atomicAdd(n_big, 1)] = (int32_t)row;
     return;
   }
-  const bool rb = is_bnd[row] != 0;
+  uint64_t kk = lane < E ? key[b + lane] : ~0ULL;
+  unsigned ww = lane < E ? (unsigned)kw[b + lane] : 0xffffu;
+  for (int w = 2; w <= 64; w <<= 1)
+    for (int j = w >> 1; j > 0; j >>= 1) {
+      const uint64_t ok = shfl_xor_u64(kk, j);
+      const unsigned ow = __shfl_xor(ww, j, 64);
+      const bool keep_min = ((lane & j) == 0) == ((lane & w) == 0);
+      const bool take = keep_min ? em_before(ok, (int)(ow >> 8), kk, (int)(ww >> 8))
+                                 : em_before(kk, (int)(ww >> 8), ok, (int)(ow >> 8));
+      if (take) {
+        kk = ok;
+        ww = ow;
+      }
+    }
+  const uint64_t prev = shfl_u64(kk, lane > 0 ? lane - 1 : 0);
   int64_t nf = 0, nbc = 0;
-  for (int i = lane; i < E; i += 64) {
-    const uint64_t kk = so[i];
-    key[b + i] = kk;
-    kw[b + i] = swo[i];
+  if (lane < E) {
+    key[b + lane] = kk;
+    kw[b + lane] = (uint16_t)ww;
     const int64_t col = (int64_t)(kk >> 32);
-    if (i > 0 && (int64_t)(so[i - 1] >> 32) == col) continue;
-    count_slot(row, rb, col, is_bnd, nf, nbc);
+    if (lane == 0 || (int64_t)(prev >> 32) != col) count_slot(row, is_bnd[row] != 0, col, is_bnd, nf, nbc);
   }
   nf = wave_sum_i64(nf);
   nbc = wave_sum_i64(nbc);
@@ -230,7 +216,7 @@ __global__ void __launch_bounds__(kSortBS) k_row_sort_block(const int32_t* __res
                                                             const uint8_t* __restrict__ is_bnd,
                                                             int64_t* __restrict__ fcount,
                                                             int64_t* __restrict__ bcount) {
-  extern __shared__ uint64_t s_dyn[];  // key[cap], out[cap], then kw[cap], kwo[cap]
+  extern __shared__ uint64_t s_dyn[];  // key[cap], then kw[cap]
   __shared__ int64_t s_red[2][kSortBS / 64];
   const int64_t row = rows[blockIdx.x];
   const int64_t b = em_rowptr[row], e = em_rowptr[row + 1];
@@ -239,34 +225,38 @@ __global__ void __launch_bounds__(kSortBS) k_row_sort_block(const int32_t* __res
   const bool rb = is_bnd[row] != 0;
   int64_t nf = 0, nbc = 0;
   if (E <= kSortBigCap) {
+    // bitonic network over the next power of two, padded with keys that sort last
+    int P = 64;
+    while (P < E) P <<= 1;
     uint64_t* sk = s_dyn;
-    uint64_t* so = s_dyn + kSortBigCap;
-    uint16_t* sw = reinterpret_cast<uint16_t*>(s_dyn + 2 * kSortBigCap);
-    uint16_t* swo = sw + kSortBigCap;
-    for (int i = tid; i < E; i += kSortBS) {
-      sk[i] = key[b + i];
-      sw[i] = kw[b + i];
+    uint16_t* sw = reinterpret_cast<uint16_t*>(s_dyn + kSortBigCap);
+    for (int i = tid; i < P; i += kSortBS) {
+      sk[i] = i < E ? key[b + i] : ~0ULL;
+      sw[i] = i < E ? kw[b + i] : (uint16_t)0xffff;
     }
     __syncthreads();
+    for (int w = 2; w <= P; w <<= 1)
+      for (int j = w >> 1; j > 0; j >>= 1) {
+        for (int i = tid; i < P; i += kSortBS) {
+          const int x = i ^ j;
+          if (x <= i) continue;
+          const uint64_t ka = sk[i], kb = sk[x];
+          const uint16_t wa = sw[i], wb = sw[x];
+          if (em_before(kb, wb >> 8, ka, wa >> 8) == ((i & w) == 0)) {
+            sk[i] = kb;
+            sk[x] = ka;
+            sw[i] = wb;
+            sw[x] = wa;
+          }
+        }
+        __syncthreads();
+      }
     for (int i = tid; i < E; i += kSortBS) {
       const uint64_t kk = sk[i];
-      const int wk = sw[i] >> 8;
-      int rank = 0;
-      for (int j = 0; j < E; ++j) {
-        const uint64_t kj = sk[j];
-        const int wj = sw[j] >> 8;
-        rank += (kj < kk || (kj == kk && (wj < wk || (wj == wk && j < i)))) ? 1 : 0;
-      }
-      so[rank] = kk;
-      swo[rank] = sw[i];
-    }
-    __syncthreads();
-    for (int i = tid; i < E; i += kSortBS) {
-      const uint64_t kk = so[i];
       key[b + i] = kk;
-      kw[b + i] = swo[i];
+      kw[b + i] = sw[i];
       const int64_t col = (int64_t)(kk >> 32);
-      if (i > 0 && (int64_t)(so[i - 1] >> 32) == col) continue;
+      if (i > 0 && (int64_t)(sk[i - 1] >> 32) == col) continue;
       count_slot(row, rb, col, is_bnd, nf, nbc);
     }
   } else if (tid == 0) {
@@ -706,7 +696,8 @@ void run_symbolic(ngpb_ctx* c) {
   DBuf<unsigned> cnt(c->stream);
   cnt.resize((size_t)N);
   NGPB_HIP(hipMemsetAsync(cnt.p, 0, N * sizeof(unsigned), c->stream));
-  hipLaunchKernelGGL(k_emit<0>, dim3(blocks_for(m.L, 128)), dim3(128), 0, c->stream, m, cnt.p, nullptr, nullptr, nullptr);
+  hipLaunchKernelGGL(k_emit_rows<0>, dim3(blocks_for(8 * m.L, 128)), dim3(128), 0, c->stream, m, cnt.p, nullptr,
+                     nullptr, nullptr);
   // per-row emission lists, sorted keys: needed only until the slots are cut
   DBuf<int64_t> em_rowptr(c->stream);
   DBuf<uint64_t> em_key(c->stream);  // (col << 32) | id
@@ -719,8 +710,8 @@ void run_symbolic(ngpb_ctx* c) {
   em_key.resize((size_t)total);
   em_kw.resize((size_t)total);
   NGPB_HIP(hipMemsetAsync(cnt.p, 0, N * sizeof(unsigned), c->stream));
-  hipLaunchKernelGGL(k_emit<1>, dim3(blocks_for(m.L, 128)), dim3(128), 0, c->stream, m, cnt.p, em_rowptr.p, em_key.p,
-                     em_kw.p);
+  hipLaunchKernelGGL(k_emit_rows<1>, dim3(blocks_for(8 * m.L, 128)), dim3(128), 0, c->stream, m, cnt.p, em_rowptr.p,
+                     em_key.p, em_kw.p);
   NGPB_HIP(hipGetLastError());
   DBuf<int64_t> fcount(c->stream), bcount(c->stream);
   fcount.resize((size_t)N);
@@ -738,7 +729,7 @@ void run_symbolic(ngpb_ctx* c) {
     NGPB_HIP(hipMemcpyAsync(&hbig, nbig.p, sizeof(int), hipMemcpyDeviceToHost, c->stream));
     NGPB_HIP(hipStreamSynchronize(c->stream));
     if (hbig > 0) {
-      const size_t lds = (size_t)kSortBigCap * (8 + 8 + 2 + 2);
+      const size_t lds = (size_t)kSortBigCap * (8 + 2);
       hipLaunchKernelGGL(k_row_sort_block, dim3((unsigned)hbig), dim3(kSortBS), lds, c->stream, big.p, em_rowptr.p,
                          em_key.p, em_kw.p, c->is_bnd.p, fcount.p, bcount.p);
       NGPB_HIP(hipGetLastError());

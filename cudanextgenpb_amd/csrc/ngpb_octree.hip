@@ -136,7 +136,7 @@ __global__ void k_refine_flag(const OLeaf* __restrict__ work, int64_t nw, Geom g
 
 // kept leaves to keep[keep_off + low(off)], the 8 children of split ones to
 // child[8 high(off) + k] in corner order (bit 0 x, bit 1 y, bit 2 z)
-__global__ void k_emit(const OLeaf* __restrict__ work, int64_t nw, const uint8_t* __restrict__ flag,
+__global__ void k_split_emit(const OLeaf* __restrict__ work, int64_t nw, const uint8_t* __restrict__ flag,
                        const int64_t* __restrict__ off, OLeaf* __restrict__ keep, OLeaf* __restrict__ child) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= nw) return;
@@ -357,7 +357,7 @@ std::pair<int64_t, int64_t> split_pass(ngpb_ctx* c, const OLeaf* in, int64_t n, 
   const int64_t tot = exclusive_scan(c, cnt, cnt, n);
   const int64_t nk = tot & 0xffffffffLL, ns = tot >> 32;
   out.resize((size_t)std::max<int64_t>(8 * ns, 1));
-  hipLaunchKernelGGL(k_emit, dim3(blocks_for(n, 256)), dim3(256), 0, c->stream, in, n, flag, cnt, keep, out.p);
+  hipLaunchKernelGGL(k_split_emit, dim3(blocks_for(n, 256)), dim3(256), 0, c->stream, in, n, flag, cnt, keep, out.p);
   NGPB_HIP(hipGetLastError());
   return {nk, ns};
 }
@@ -502,7 +502,7 @@ OctreeSizes run_build_octree(ngpb_ctx* c, const double* origin, double hf, const
       NGPB_REQUIRE(nk + 8 * ns < INT32_MAX, NGPB_E_ARG, "leaf count out of range");
       hipLaunchKernelGGL(k_paint, dim3(blocks_for(L, 64)), dim3(64), 0, st, leaves.p, L, flag.p, g, lev.p);
       next.resize((size_t)(nk + 8 * ns));  // kept leaves first, children after them
-      hipLaunchKernelGGL(k_emit, dim3(blocks_for(L, 256)), dim3(256), 0, st, leaves.p, L, flag.p, cnt.p, next.p,
+      hipLaunchKernelGGL(k_split_emit, dim3(blocks_for(L, 256)), dim3(256), 0, st, leaves.p, L, flag.p, cnt.p, next.p,
                          next.p + nk);
       NGPB_HIP(hipGetLastError());
       std::swap(leaves.p, next.p);
