@@ -136,17 +136,28 @@ __device__ __forceinline__ bool em_less(uint64_t ka, uint16_t wa, uint64_t kb, u
 constexpr int kSortBS = 256;
 constexpr int kSortBigCap = 2048;  // 20 KB of LDS per block
 
-// distinct columns of a sorted run, counted as final slots or Dirichlet couplings
-__device__ __forceinline__ void count_slot(int64_t row, bool rb, int64_t col, const uint8_t* __restrict__ is_bnd,
-                                           int64_t& nf, int64_t& nbc) {
-  if (rb) {
-    if (col == row) ++nf;
-  } else if (is_bnd[col]) {
-    ++nbc;
-  } else {
-    ++nf;
-  }
+// class of a row's emission with column col: 0 a matrix slot, 1 a coupling to a
+// Dirichlet column (moved to the right-hand side), 2 dropped (off-diagonal of a
+// Dirichlet row)
+__device__ __forceinline__ int em_class(int64_t row, bool rb, int64_t col, const uint8_t* __restrict__ is_bnd) {
+  if (rb) return col == row ? 0 : 2;
+  return is_bnd[col] ? 1 : 0;
 }
+
+// per-row counts of a sorted run: slots and couplings (distinct columns) and the
+// emissions that belong to each
+struct RowCounts {
+  int64_t slots = 0, couplings = 0, slot_em = 0, coupling_em = 0;
+  __device__ void add(int cls, bool new_col) {
+    if (cls == 0) {
+      ++slot_em;
+      slots += new_col ? 1 : 0;
+    } else if (cls == 1) {
+      ++coupling_em;
+      couplings += new_col ? 1 : 0;
+    }
+  }
+};
 
 __device__ __forceinline__ bool em_before(uint64_t ka, int wa, uint64_t kb, int wb) {
   return ka < kb || (ka == kb && wa < wb);
@@ -167,8 +178,7 @@ __device__ __forceinline__ uint64_t shfl_u64(uint64_t v, int src) {
 // are listed for k_row_sort_block.
 __global__ void __launch_bounds__(kSortBS) k_row_sort_wave(int64_t N, const int64_t* __restrict__ em_rowptr,
                                                            uint64_t* __restrict__ key, uint16_t* __restrict__ kw,
-                                                           const uint8_t* __restrict__ is_bnd,
-                                                           int64_t* __restrict__ fcount, int64_t* __restrict__ bcount,
+                                                           const uint8_t* __restrict__ is_bnd, int64_t* __restrict__ cnt4,
                                                            int32_t* __restrict__ big_rows, int* __restrict__ n_big) {
   const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const int64_t row = (int64_t)blockIdx.x * (kSortBS / 64) + wv;
@@ -194,36 +204,32 @@ __global__ void __launch_bounds__(kSortBS) k_row_sort_wave(int64_t N, const int6
       }
     }
   const uint64_t prev = shfl_u64(kk, lane > 0 ? lane - 1 : 0);
-  int64_t nf = 0, nbc = 0;
+  RowCounts rc;
   if (lane < E) {
     key[b + lane] = kk;
     kw[b + lane] = (uint16_t)ww;
     const int64_t col = (int64_t)(kk >> 32);
-    if (lane == 0 || (int64_t)(prev >> 32) != col) count_slot(row, is_bnd[row] != 0, col, is_bnd, nf, nbc);
+    rc.add(em_class(row, is_bnd[row] != 0, col, is_bnd), lane == 0 || (int64_t)(prev >> 32) != col);
   }
-  nf = wave_sum_i64(nf);
-  nbc = wave_sum_i64(nbc);
-  if (lane == 0) {
-    fcount[row] = nf;
-    bcount[row] = nbc;
-  }
+  const int64_t v[4] = {wave_sum_i64(rc.slots), wave_sum_i64(rc.couplings), wave_sum_i64(rc.slot_em),
+                        wave_sum_i64(rc.coupling_em)};
+  if (lane < 4) cnt4[(int64_t)lane * N + row] = v[lane];
 }
 
 // one block per listed long row
 __global__ void __launch_bounds__(kSortBS) k_row_sort_block(const int32_t* __restrict__ rows,
                                                             const int64_t* __restrict__ em_rowptr,
                                                             uint64_t* __restrict__ key, uint16_t* __restrict__ kw,
-                                                            const uint8_t* __restrict__ is_bnd,
-                                                            int64_t* __restrict__ fcount,
-                                                            int64_t* __restrict__ bcount) {
+                                                            const uint8_t* __restrict__ is_bnd, int64_t N,
+                                                            int64_t* __restrict__ cnt4) {
   extern __shared__ uint64_t s_dyn[];  // key[cap], then kw[cap]
-  __shared__ int64_t s_red[2][kSortBS / 64];
+  __shared__ int64_t s_red[4][kSortBS / 64];
   const int64_t row = rows[blockIdx.x];
   const int64_t b = em_rowptr[row], e = em_rowptr[row + 1];
   const int E = (int)(e - b);
   const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
   const bool rb = is_bnd[row] != 0;
-  int64_t nf = 0, nbc = 0;
+  RowCounts rc;
   if (E <= kSortBigCap) {
     // bitonic network over the next power of two, padded with keys that sort last
     int P = 64;
@@ -256,8 +262,7 @@ __global__ void __launch_bounds__(kSortBS) k_row_sort_block(const int32_t* __res
       key[b + i] = kk;
       kw[b + i] = sw[i];
       const int64_t col = (int64_t)(kk >> 32);
-      if (i > 0 && (int64_t)(sk[i - 1] >> 32) == col) continue;
-      count_slot(row, rb, col, is_bnd, nf, nbc);
+      rc.add(em_class(row, rb, col, is_bnd), i == 0 || (int64_t)(sk[i - 1] >> 32) != col);
     }
   } else if (tid == 0) {
     for (int64_t i = b + 1; i < e; ++i) {
@@ -275,91 +280,91 @@ __global__ void __launch_bounds__(kSortBS) k_row_sort_block(const int32_t* __res
     int64_t prev = -1;
     for (int64_t i = b; i < e; ++i) {
       const int64_t col = (int64_t)(key[i] >> 32);
-      if (col == prev) continue;
+      rc.add(em_class(row, rb, col, is_bnd), col != prev);
       prev = col;
-      count_slot(row, rb, col, is_bnd, nf, nbc);
     }
   }
-  nf = wave_sum_i64(nf);
-  nbc = wave_sum_i64(nbc);
-  if (lane == 0) {
-    s_red[0][wv] = nf;
-    s_red[1][wv] = nbc;
-  }
+  const int64_t v[4] = {wave_sum_i64(rc.slots), wave_sum_i64(rc.couplings), wave_sum_i64(rc.slot_em),
+                        wave_sum_i64(rc.coupling_em)};
+  if (lane == 0)
+    for (int q = 0; q < 4; ++q) s_red[q][wv] = v[q];
   __syncthreads();
-  if (tid == 0) {
-    int64_t a = 0, c2 = 0;
-    for (int w = 0; w < kSortBS / 64; ++w) {
-      a += s_red[0][w];
-      c2 += s_red[1][w];
-    }
-    fcount[row] = a;
-    bcount[row] = c2;
+  if (tid < 4) {
+    int64_t a = 0;
+    for (int w = 0; w < kSortBS / 64; ++w) a += s_red[tid][w];
+    cnt4[(int64_t)tid * N + row] = a;
   }
 }
 
-__global__ void k_row_fill(int64_t N, const int64_t* __restrict__ em_rowptr, const uint64_t* __restrict__ key,
-                           const uint8_t* __restrict__ is_bnd, const int64_t* __restrict__ foff,
-                           const int64_t* __restrict__ boff, int32_t* __restrict__ rowptr, int32_t* __restrict__ col,
-                           int64_t* __restrict__ slot_em, int32_t* __restrict__ diag_slot,
-                           int32_t* __restrict__ bc_rowptr, int32_t* __restrict__ bc_col,
-                           int64_t* __restrict__ bc_em, int64_t nnz_total, int64_t nbc_total) {
-  const int64_t row = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (row >= N) return;
+// One wave per row over its sorted emissions, 64 at a time: the matrix slots (col,
+// diag_slot, sptr = first emission of the slot in the repacked list) and the couplings
+// to Dirichlet columns (bc_col, bc_ptr), and the emissions of each, repacked to 5 bytes
+// (source id, weight code) in slot order: each class compacted by ballot ranks from
+// the row's offsets (prefix sums of the per-row counts).
+constexpr int kSlotBS = 256;
+
+__global__ void __launch_bounds__(kSlotBS) k_row_slots(
+    int64_t N, const int64_t* __restrict__ em_rowptr, const uint64_t* __restrict__ key,
+    const uint16_t* __restrict__ kw, const uint8_t* __restrict__ is_bnd, const int64_t* __restrict__ off4,
+    int32_t* __restrict__ rowptr, int32_t* __restrict__ col, int32_t* __restrict__ diag_slot,
+    int32_t* __restrict__ bc_rowptr, int32_t* __restrict__ bc_col, int64_t* __restrict__ sptr,
+    uint32_t* __restrict__ em_id, uint8_t* __restrict__ em_code, int64_t* __restrict__ bc_ptr,
+    uint32_t* __restrict__ bc_id, uint8_t* __restrict__ bc_code, int64_t n_slots, int64_t n_coup, int64_t n_slot_em,
+    int64_t n_coup_em) {
+  const int lane = threadIdx.x & 63;
+  const int64_t row = (int64_t)blockIdx.x * (kSlotBS / 64) + (threadIdx.x >> 6);
+  if (row >= N) return;  // the whole wave
   const int64_t b = em_rowptr[row], e = em_rowptr[row + 1];
   const bool rb = is_bnd[row] != 0;
-  int64_t fs = foff[row], bs = boff[row];
-  rowptr[row] = (int32_t)fs;
-  bc_rowptr[row] = (int32_t)bs;
-  if (row == N - 1) {
-    rowptr[N] = (int32_t)nnz_total;
-    bc_rowptr[N] = (int32_t)nbc_total;
-  }
-  int64_t i = b;
-  while (i < e) {
-    const int64_t cl = (int64_t)(key[i] >> 32);
-    int64_t j = i + 1;
-    while (j < e && (int64_t)(key[j] >> 32) == cl) ++j;
-    if (rb) {
-      if (cl == row) {
-        col[fs] = (int32_t)cl;
-        slot_em[2 * fs] = i;
-        slot_em[2 * fs + 1] = j;
-        diag_slot[row] = (int32_t)fs;
-        ++fs;
-      }
-    } else if (is_bnd[cl]) {
-      bc_col[bs] = (int32_t)cl;
-      bc_em[2 * bs] = i;
-      bc_em[2 * bs + 1] = j;
-      ++bs;
-    } else {
-      col[fs] = (int32_t)cl;
-      slot_em[2 * fs] = i;
-      slot_em[2 * fs + 1] = j;
-      if (cl == row) diag_slot[row] = (int32_t)fs;
-      ++fs;
+  int64_t fs = off4[row], bs = off4[N + row], fe = off4[2 * N + row], be = off4[3 * N + row];
+  if (lane == 0) {
+    rowptr[row] = (int32_t)fs;
+    bc_rowptr[row] = (int32_t)bs;
+    if (row == N - 1) {
+      rowptr[N] = (int32_t)n_slots;
+      bc_rowptr[N] = (int32_t)n_coup;
+      sptr[n_slots] = n_slot_em;
+      bc_ptr[n_coup] = n_coup_em;
     }
-    i = j;
   }
-}
-
-// repacking of the slot and coupling emissions (after the symbolic sort)
-__global__ void k_range_len(int64_t n, const int64_t* __restrict__ rng, int64_t* __restrict__ len) {
-  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < n) len[i] = rng[2 * i + 1] - rng[2 * i];
-}
-
-__global__ void k_repack(int64_t n, const int64_t* __restrict__ rng, const int64_t* __restrict__ dst,
-                         const uint64_t* __restrict__ key, const uint16_t* __restrict__ kw, uint32_t* __restrict__ id,
-                         uint8_t* __restrict__ code) {
-  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
-  const int64_t b = rng[2 * i], e = rng[2 * i + 1];
-  int64_t o = dst[i];
-  for (int64_t k = b; k < e; ++k, ++o) {
-    id[o] = (uint32_t)(key[k] & 0xffffffffULL);
-    code[o] = (uint8_t)(kw[k] & 0xff);
+  const uint64_t below = (1ULL << lane) - 1;
+  int64_t lastcol = -1;
+  for (int64_t c0 = b; c0 < e; c0 += 64) {
+    const int64_t i = c0 + lane;
+    const bool valid = i < e;
+    const uint64_t kk = valid ? key[i] : 0;
+    const int64_t cl = valid ? (int64_t)(kk >> 32) : -1;
+    const int64_t up = (int64_t)shfl_u64((uint64_t)cl, lane > 0 ? lane - 1 : 0);
+    const int64_t prev = lane == 0 ? lastcol : up;
+    const bool start = valid && cl != prev;
+    const int cls = valid ? em_class(row, rb, cl, is_bnd) : 3;
+    const uint64_t mf = __ballot(cls == 0), mb = __ballot(cls == 1);
+    const uint64_t sf = __ballot(start && cls == 0), sb = __ballot(start && cls == 1);
+    if (cls == 0) {
+      const int64_t o = fe + __popcll(mf & below);
+      em_id[o] = (uint32_t)(kk & 0xffffffffULL);
+      em_code[o] = (uint8_t)(kw[i] & 0xff);
+      if (start) {
+        const int64_t sl = fs + __popcll(sf & below);
+        col[sl] = (int32_t)cl;
+        sptr[sl] = o;
+        if (cl == row) diag_slot[row] = (int32_t)sl;
+      }
+    } else if (cls == 1) {
+      const int64_t o = be + __popcll(mb & below);
+      bc_id[o] = (uint32_t)(kk & 0xffffffffULL);
+      bc_code[o] = (uint8_t)(kw[i] & 0xff);
+      if (start) {
+        const int64_t sl = bs + __popcll(sb & below);
+        bc_col[sl] = (int32_t)cl;
+        bc_ptr[sl] = o;
+      }
+    }
+    fe += __popcll(mf);
+    be += __popcll(mb);
+    fs += __popcll(sf);
+    bs += __popcll(sb);
+    lastcol = (int64_t)shfl_u64((uint64_t)cl, 63);
   }
 }
 
@@ -713,9 +718,9 @@ void run_symbolic(ngpb_ctx* c) {
   hipLaunchKernelGGL(k_emit_rows<1>, dim3(blocks_for(8 * m.L, 128)), dim3(128), 0, c->stream, m, cnt.p, em_rowptr.p,
                      em_key.p, em_kw.p);
   NGPB_HIP(hipGetLastError());
-  DBuf<int64_t> fcount(c->stream), bcount(c->stream);
-  fcount.resize((size_t)N);
-  bcount.resize((size_t)N);
+  // per row: slots, couplings, slot emissions, coupling emissions (4 planes of N)
+  DBuf<int64_t> cnt4(c->stream);
+  cnt4.resize((size_t)4 * N);
   {
     DBuf<int32_t> big(c->stream);
     DBuf<int> nbig(c->stream);
@@ -723,7 +728,7 @@ void run_symbolic(ngpb_ctx* c) {
     nbig.resize(1);
     NGPB_HIP(hipMemsetAsync(nbig.p, 0, sizeof(int), c->stream));
     hipLaunchKernelGGL(k_row_sort_wave, dim3(blocks_for(N, kSortBS / 64)), dim3(kSortBS), 0, c->stream, N,
-                       em_rowptr.p, em_key.p, em_kw.p, c->is_bnd.p, fcount.p, bcount.p, big.p, nbig.p);
+                       em_rowptr.p, em_key.p, em_kw.p, c->is_bnd.p, cnt4.p, big.p, nbig.p);
     NGPB_HIP(hipGetLastError());
     int hbig = 0;
     NGPB_HIP(hipMemcpyAsync(&hbig, nbig.p, sizeof(int), hipMemcpyDeviceToHost, c->stream));
@@ -731,12 +736,13 @@ void run_symbolic(ngpb_ctx* c) {
     if (hbig > 0) {
       const size_t lds = (size_t)kSortBigCap * (8 + 2);
       hipLaunchKernelGGL(k_row_sort_block, dim3((unsigned)hbig), dim3(kSortBS), lds, c->stream, big.p, em_rowptr.p,
-                         em_key.p, em_kw.p, c->is_bnd.p, fcount.p, bcount.p);
+                         em_key.p, em_kw.p, c->is_bnd.p, N, cnt4.p);
       NGPB_HIP(hipGetLastError());
     }
   }
-  const int64_t nnz = exclusive_scan(c, fcount.p, fcount.p, N);
-  const int64_t nbc = exclusive_scan(c, bcount.p, bcount.p, N);
+  int64_t tot[4];
+  for (int q = 0; q < 4; ++q) tot[q] = exclusive_scan(c, cnt4.p + (int64_t)q * N, cnt4.p + (int64_t)q * N, N);
+  const int64_t nnz = tot[0], nbc = tot[1];
   NGPB_REQUIRE(nnz < (int64_t)INT32_MAX && nbc < (int64_t)INT32_MAX, NGPB_E_ARG, "nnz exceeds 32-bit CSR");
   c->n = N;
   c->nnz = nnz;
@@ -745,47 +751,29 @@ void run_symbolic(ngpb_ctx* c) {
   c->diag_slot.resize((size_t)N);
   c->bc_rowptr.resize((size_t)N + 1);
   c->bc_col.resize((size_t)std::max<int64_t>(nbc, 1));
-  DBuf<int64_t> slot_em(c->stream), bc_em(c->stream);  // 2 per slot / coupling: range in the sorted lists
-  slot_em.resize((size_t)2 * std::max<int64_t>(nnz, 1));
-  bc_em.resize((size_t)2 * std::max<int64_t>(nbc, 1));
-  hipLaunchKernelGGL(k_row_fill, dim3(blocks_for(N, 128)), dim3(128), 0, c->stream, N, em_rowptr.p, em_key.p,
-                     c->is_bnd.p, fcount.p, bcount.p, c->rowptr.p, c->col.p, slot_em.p, c->diag_slot.p,
-                     c->bc_rowptr.p, c->bc_col.p, bc_em.p, nnz, nbc);
+  c->bc_ptr.resize((size_t)nbc + 1);
+  c->em_id.resize((size_t)std::max<int64_t>(tot[2], 1));
+  c->em_code.resize((size_t)std::max<int64_t>(tot[2], 1));
+  c->bc_id.resize((size_t)std::max<int64_t>(tot[3], 1));
+  c->bc_code.resize((size_t)std::max<int64_t>(tot[3], 1));
+  // slots, couplings and the repacked emissions (5 bytes each: source id, weight
+  // code) in slot order, then block-relative slot offsets
+  DBuf<int64_t> sptr(c->stream);
+  sptr.resize((size_t)nnz + 1);
+  hipLaunchKernelGGL(k_row_slots, dim3(blocks_for(N, kSlotBS / 64)), dim3(kSlotBS), 0, c->stream, N, em_rowptr.p,
+                     em_key.p, em_kw.p, c->is_bnd.p, cnt4.p, c->rowptr.p, c->col.p, c->diag_slot.p, c->bc_rowptr.p,
+                     c->bc_col.p, sptr.p, c->em_id.p, c->em_code.p, c->bc_ptr.p, c->bc_id.p, c->bc_code.p, nnz, nbc,
+                     tot[2], tot[3]);
   NGPB_HIP(hipGetLastError());
-  // repack the slot emissions in slot order, 5 bytes each (source id, weight code),
-  // with block-relative slot offsets; the couplings to Dirichlet nodes likewise
-  {
-    DBuf<int64_t> sptr(c->stream);
-    sptr.resize((size_t)nnz + 1);
-    hipLaunchKernelGGL(k_range_len, dim3(blocks_for(nnz, 256)), dim3(256), 0, c->stream, nnz, slot_em.p, sptr.p);
-    const int64_t tot = exclusive_scan(c, sptr.p, sptr.p, nnz);
-    NGPB_HIP(hipMemcpyAsync(sptr.p + nnz, &tot, sizeof(int64_t), hipMemcpyHostToDevice, c->stream));
-    c->em_id.resize((size_t)std::max<int64_t>(tot, 1));
-    c->em_code.resize((size_t)std::max<int64_t>(tot, 1));
-    hipLaunchKernelGGL(k_repack, dim3(blocks_for(nnz, 256)), dim3(256), 0, c->stream, nnz, slot_em.p, sptr.p, em_key.p,
-                       em_kw.p, c->em_id.p, c->em_code.p);
-    const int64_t nblk = (N + kSvRows - 1) / kSvRows;
-    c->blk_em.resize((size_t)nblk + 1);
-    hipLaunchKernelGGL(k_blk_em, dim3(blocks_for(nblk + 1, 256)), dim3(256), 0, c->stream, nblk, N, c->rowptr.p, sptr.p,
-                       c->blk_em.p);
-    c->slot_rel.resize((size_t)std::max<int64_t>(nnz, 1));
-    hipLaunchKernelGGL(k_slot_rel, dim3(blocks_for(N, 256)), dim3(256), 0, c->stream, N, c->rowptr.p, sptr.p,
-                       c->blk_em.p, c->slot_rel.p);
-    c->bc_ptr.resize((size_t)nbc + 1);
-    int64_t btot = 0;
-    if (nbc > 0) {
-      hipLaunchKernelGGL(k_range_len, dim3(blocks_for(nbc, 256)), dim3(256), 0, c->stream, nbc, bc_em.p, c->bc_ptr.p);
-      btot = exclusive_scan(c, c->bc_ptr.p, c->bc_ptr.p, nbc);
-    }
-    NGPB_HIP(hipMemcpyAsync(c->bc_ptr.p + nbc, &btot, sizeof(int64_t), hipMemcpyHostToDevice, c->stream));
-    c->bc_id.resize((size_t)std::max<int64_t>(btot, 1));
-    c->bc_code.resize((size_t)std::max<int64_t>(btot, 1));
-    if (nbc > 0)
-      hipLaunchKernelGGL(k_repack, dim3(blocks_for(nbc, 256)), dim3(256), 0, c->stream, nbc, bc_em.p, c->bc_ptr.p,
-                         em_key.p, em_kw.p, c->bc_id.p, c->bc_code.p);
-    NGPB_HIP(hipGetLastError());
-    NGPB_HIP(hipStreamSynchronize(c->stream));
-  }
+  const int64_t nblk = (N + kSvRows - 1) / kSvRows;
+  c->blk_em.resize((size_t)nblk + 1);
+  hipLaunchKernelGGL(k_blk_em, dim3(blocks_for(nblk + 1, 256)), dim3(256), 0, c->stream, nblk, N, c->rowptr.p, sptr.p,
+                     c->blk_em.p);
+  c->slot_rel.resize((size_t)std::max<int64_t>(nnz, 1));
+  hipLaunchKernelGGL(k_slot_rel, dim3(blocks_for(N, 256)), dim3(256), 0, c->stream, N, c->rowptr.p, sptr.p,
+                     c->blk_em.p, c->slot_rel.p);
+  NGPB_HIP(hipGetLastError());
+  NGPB_HIP(hipStreamSynchronize(c->stream));
   c->have_symbolic = true;
 }
 

@@ -69,30 +69,37 @@ __global__ void k_leaf_fill(int64_t L, const int32_t* __restrict__ bucket, const
   list[start[b] + atomicAdd(&cursor[b], 1)] = (int32_t)l;
 }
 
-__global__ void k_locate(int64_t A, const double* __restrict__ axyz, double tol, LeafGrid G,
-                         const int64_t* __restrict__ start, const int32_t* __restrict__ list,
-                         const double* __restrict__ mn, const double* __restrict__ mx, int32_t* __restrict__ owner) {
-  const int64_t a = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (a >= A) return;
+// one wave per atom: the lanes split each bucket's leaves, then the wave takes the
+// smallest passing leaf index
+constexpr int kLocateBS = 256;
+
+__global__ void __launch_bounds__(kLocateBS) k_locate(int64_t A, const double* __restrict__ axyz, double tol,
+                                                      LeafGrid G, const int64_t* __restrict__ start,
+                                                      const int32_t* __restrict__ list, const double* __restrict__ mn,
+                                                      const double* __restrict__ mx, int32_t* __restrict__ owner) {
+  const int lane = threadIdx.x & 63;
+  const int64_t a = (int64_t)blockIdx.x * (kLocateBS / 64) + (threadIdx.x >> 6);
+  if (a >= A) return;  // the whole wave
   const double x[3] = {axyz[3 * a], axyz[3 * a + 1], axyz[3 * a + 2]};
   const double p[3] = {x[0] + tol, x[1] + tol, x[2] + tol};
   int64_t bc[3];
   for (int d = 0; d < 3; ++d) bc[d] = (int64_t)floor((p[d] - G.g0[d]) / G.cell);
-  int32_t best = -1;
+  int32_t best = INT32_MAX;
   for (int64_t z = bc[2] - 1; z <= bc[2] + 1; ++z)
     for (int64_t y = bc[1] - 1; y <= bc[1] + 1; ++y)
       for (int64_t w = bc[0] - 1; w <= bc[0] + 1; ++w) {
         if (w < 0 || y < 0 || z < 0 || w >= G.dim[0] || y >= G.dim[1] || z >= G.dim[2]) continue;
         const int64_t b = (z * G.dim[1] + y) * G.dim[0] + w;
-        for (int64_t k = start[b]; k < start[b + 1]; ++k) {
+        for (int64_t k = start[b] + lane; k < start[b + 1]; k += 64) {
           const int32_t l = list[k];
-          if (best >= 0 && l >= best) continue;
+          if (l >= best) continue;
           bool in = true;
           for (int d = 0; d < 3; ++d) in = in && (mn[3 * l + d] - tol < x[d]) && (x[d] <= mx[3 * l + d] - tol);
           if (in) best = l;
         }
       }
-  owner[a] = best;
+  for (int off = 32; off > 0; off >>= 1) best = min(best, __shfl_xor(best, off, 64));
+  if (lane == 0) owner[a] = best == INT32_MAX ? -1 : best;
 }
 
 // per block g of kRedBlocks: min over mn[3 l + d], max over mx[3 l + d] (d = 0..2) and
@@ -197,8 +204,8 @@ void run_locate_atoms(ngpb_ctx* c, int64_t A, const double* h_xyz, double tol, i
                      list.p);
   DBuf<double> axyz(c->stream);
   axyz.upload(h_xyz, (size_t)(3 * A), c->stream);
-  hipLaunchKernelGGL(k_locate, dim3(blocks_for(A, 128)), dim3(128), 0, c->stream, A, axyz.p, tol, G, start.p, list.p,
-                     mn.p, mx.p, owner.p);
+  hipLaunchKernelGGL(k_locate, dim3(blocks_for(A, kLocateBS / 64)), dim3(kLocateBS), 0, c->stream, A, axyz.p, tol, G,
+                     start.p, list.p, mn.p, mx.p, owner.p);
   NGPB_HIP(hipGetLastError());
   owner.download(h_owner, (size_t)A, c->stream);
   NGPB_HIP(hipStreamSynchronize(c->stream));
