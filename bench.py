@@ -332,14 +332,77 @@ def bench_mesh(args):
     }
 
 
+def bench_request(args):
+    """One request from atoms to energy on a fresh context: octree, rays, markers,
+    owners, Dirichlet data, assembly (symbolic + numeric), mixed solve, energy."""
+    import numpy as np
+    from cudanextgenpb_amd import native, workload as W
+
+    big = args.mesh_size == "large"
+    P = W.large() if big else W.globular()
+    m, a = P.mesh, P.atoms
+    stages = ("octree", "rays", "markers", "locate", "charges", "dirichlet", "assemble", "solve", "energy")
+    rows, energy = [], None
+    for it in range(args.warmup + args.steps):
+        c = native.Context(0)
+        c.set_physics(P.physics)
+        tt = [time.perf_counter()]
+        c.build_octree(m.origin, m.hf, m.n, m.smax, m.bands, a.xyz, a.r)
+        tt.append(time.perf_counter())
+        c.cast_rays(a.xyz, a.r)
+        tt.append(time.perf_counter())
+        c.create_markers()
+        tt.append(time.perf_counter())
+        owner = c.locate_atoms(a.xyz, P.atom_tol)
+        tt.append(time.perf_counter())
+        c.set_charges(a.xyz, a.q, owner)
+        tt.append(time.perf_counter())
+        c.dirichlet_values(P.bc_type)
+        tt.append(time.perf_counter())
+        c.assemble()
+        tt.append(time.perf_counter())
+        _, st = c.solve(native.PREC_MIXED, tol=args.tol, check_every=args.check_every, want_phi=False)
+        tt.append(time.perf_counter())
+        e = c.energy()
+        tt.append(time.perf_counter())
+        c.close()
+        if energy is not None and e["polar"] != energy["polar"]:
+            raise SystemExit("requests gave different energies")
+        energy = e
+        if it >= args.warmup:
+            rows.append([(tt[k + 1] - tt[k]) * 1e3 for k in range(len(stages))])
+            log(f"  request {it + 1 - args.warmup}/{args.steps}: {sum(rows[-1]):.1f} ms, {st['iterations']} it")
+    r = np.array(rows)
+    tot = r.sum(axis=1)
+    return {
+        "metric": "request latency from atoms to energy (mesh, assembly, mixed solve, energy on the GPU)",
+        "value": float(np.median(tot)),
+        "unit": "ms",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": float(np.median(tot)),
+        "higher_is_better": False,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "fp32+fp64",
+        "data": "synthetic",
+        "config": {"workload": ("large complex, 40000 atoms" if big else "globular protein, 2500 atoms")
+                   + f", {m.n_nodes} nodes, {m.n_leaves} leaves; mixed solve tol {args.tol:g} + energy",
+                   "parallelism": "single GPU"},
+        "stages_ms": {k: float(np.median(r[:, i])) for i, k in enumerate(stages)},
+        "energy_polar": energy["polar"],
+    }
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=8)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--workload", choices=["globular", "born", "large", "batch", "serve", "mesh"], default="globular")
+    ap.add_argument("--workload", choices=["globular", "born", "large", "batch", "serve", "mesh", "request"], default="globular")
     ap.add_argument("--mesh-size", choices=["globular", "large"], default="globular",
-                    help="mesh workload: which system's mesh to build")
+                    help="mesh and request workloads: which system")
     ap.add_argument("--requests", type=int, default=48, help="serve workload: number of requests")
     ap.add_argument("--rate", type=float, default=8.0, help="serve workload: Poisson arrival rate [1/s]")
     ap.add_argument("--systems", type=int, default=512, help="batch workload: number of ligand systems")
@@ -371,6 +434,9 @@ def main():
         return
     if args.workload == "mesh":
         print(json.dumps(bench_mesh(args)), flush=True)
+        return
+    if args.workload == "request":
+        print(json.dumps(bench_request(args)), flush=True)
         return
 
     t = time.perf_counter()
