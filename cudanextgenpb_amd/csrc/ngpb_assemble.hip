@@ -191,7 +191,9 @@ __global__ void __launch_bounds__(kSortBS) k_row_sort_wave(int64_t N, const int6
   }
   uint64_t kk = lane < E ? key[b + lane] : ~0ULL;
   unsigned ww = lane < E ? (unsigned)kw[b + lane] : 0xffffu;
+#pragma unroll
   for (int w = 2; w <= 64; w <<= 1)
+#pragma unroll
     for (int j = w >> 1; j > 0; j >>= 1) {
       const uint64_t ok = shfl_xor_u64(kk, j);
       const unsigned ow = __shfl_xor(ww, j, 64);

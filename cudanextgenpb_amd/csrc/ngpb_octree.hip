@@ -73,6 +73,7 @@ struct AtomBins {
   const double* r;
   const int32_t* ptr;   // bins + 1
   const int32_t* atom;  // atoms by bin
+  const double* box;    // per bin: min corner, max corner of its atom centres, largest radius
   double lo[3];
   double cell;
   int dim[3];
@@ -100,6 +101,18 @@ __device__ bool surface_below(const AtomBins& B, const double p[3], double thr) 
     for (int y = c0[1]; y <= c1[1]; ++y)
       for (int x = c0[0]; x <= c1[0]; ++x) {
         const int64_t b = ((int64_t)z * B.dim[1] + y) * B.dim[0] + x;
+        if (B.ptr[b] == B.ptr[b + 1]) continue;
+        // skip a bin whose atoms are all clearly beyond thr: |p - c| >= the distance
+        // from p to the box of the bin's centres; the margin keeps every skipped atom
+        // one whose own test is false whatever its rounding
+        const double* bx = B.box + 7 * b;
+        double q = 0.0;
+        for (int d = 0; d < 3; ++d) {
+          const double t = fmax(fmax(bx[d] - p[d], p[d] - bx[3 + d]), 0.0);
+          q += t * t;
+        }
+        const double lower = sqrt(q) - bx[6];
+        if (lower >= thr + 1e-9 * (fabs(thr) + fabs(lower) + 1.0)) continue;
         for (int32_t k = B.ptr[b]; k < B.ptr[b + 1]; ++k) {
           const int32_t a = B.atom[k];
           const double dx = p[0] - B.xyz[3 * a], dy = p[1] - B.xyz[3 * a + 1], dz = p[2] - B.xyz[3 * a + 2];
@@ -385,6 +398,7 @@ OctreeSizes run_build_octree(ngpb_ctx* c, const double* origin, double hf, const
   B.cell = 4.0;
   DBuf<double> bxyz(st), br(st);
   DBuf<int32_t> bptr(st), batom(st);
+  DBuf<double> bbox(st);
   {
     double lo[3] = {0, 0, 0}, hi[3] = {0, 0, 0};
     if (na) {
@@ -430,6 +444,23 @@ OctreeSizes run_build_octree(ngpb_ctx* c, const double* origin, double hf, const
       std::vector<int32_t> cur(ptr.begin(), ptr.end() - 1);
       for (int64_t a = 0; a < na; ++a) atom[(size_t)cur[bin[a]]++] = (int32_t)a;
     }
+    std::vector<double> box((size_t)7 * nb);
+    for (int64_t b = 0; b < nb; ++b) {
+      for (int d = 0; d < 3; ++d) {
+        box[7 * b + d] = 1e300;
+        box[7 * b + 3 + d] = -1e300;
+      }
+      box[7 * b + 6] = 0.0;
+    }
+    for (int64_t a = 0; a < na; ++a) {
+      double* bx = box.data() + 7 * (int64_t)bin[a];
+      for (int d = 0; d < 3; ++d) {
+        bx[d] = std::min(bx[d], h_xyz[3 * a + d]);
+        bx[3 + d] = std::max(bx[3 + d], h_xyz[3 * a + d]);
+      }
+      bx[6] = std::max(bx[6], h_r[a]);
+    }
+    bbox.upload(box.data(), box.size(), st);
     bxyz.upload(na ? h_xyz : nullptr, (size_t)(3 * na), st);
     br.upload(na ? h_r : nullptr, (size_t)na, st);
     if (!na) {
@@ -442,6 +473,7 @@ OctreeSizes run_build_octree(ngpb_ctx* c, const double* origin, double hf, const
     B.r = br.p;
     B.ptr = bptr.p;
     B.atom = batom.p;
+    B.box = bbox.p;
     NGPB_HIP(hipStreamSynchronize(st));  // the host arrays above go out of scope
   }
 
