@@ -56,7 +56,8 @@ def predict_bytes(problem: W.Problem) -> int:
     12, right-hand sides 24, scaling 8, FP64 and FP32 solver vectors 64, scratch 10,
     symbolic temporaries 210; per matrix entry (at most 8 per node outside hanging
     regions, 12 allowed): CSR 12, block-relative slot offset 4, compact operator <= 6,
-    slot ranges during the symbolic phase 24; per leaf: corners 32, marker and border
+    24 for the symbolic phase's per-slot temporaries (8 are used: the slot starts); per
+    leaf: corners 32, marker and border
     map 5, conductances 104, and 56 emissions of 5 bytes kept plus 10 bytes while
     sorting; per hanging node 64; per border leaf (fractions, normals and the energy's
     sources and accumulators) 3.6 KB, with the border leaves bounded by the atoms'
