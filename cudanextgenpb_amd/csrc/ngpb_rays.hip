@@ -109,41 +109,136 @@ __device__ __forceinline__ bool iv_less(const Iv& p, const Iv& q) {
   return p.a < q.a || (p.a == q.a && (p.b < q.b || (p.b == q.b && p.ia < q.ia)));
 }
 
-// intervals of each line, sorted, merged in place; mcnt = merged intervals
-__global__ void k_merge_pierced(int64_t nl, const uint64_t* __restrict__ lu, const uint64_t* __restrict__ lv,
-                                BallGrid G, int axis, int d1, int d2, const int64_t* __restrict__ off,
-                                Iv* __restrict__ iv, int64_t* __restrict__ mcnt) {
-  const int64_t li = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+// Intervals of each line, sorted under (start, end, ball) -- a total order, so the
+// result is the host caster's std::sort result -- and merged where they overlap or
+// touch; mcnt = 2 x merged intervals, written to the line's slots of iv. One wave per
+// line: the pierced intervals are gathered into LDS (lanes split each bin's balls,
+// ballot ranks give the slots), sorted by a bitonic network (padding sorts last) and
+// merged by one lane in a single pass. A line with more than kLineCap intervals is
+// handled by one lane in global memory (insertion sort, then the same merge).
+constexpr int kLineCap = 1024;
+
+__global__ void __launch_bounds__(64) k_merge_pierced_wave(int64_t nl, const uint64_t* __restrict__ lu,
+                                                           const uint64_t* __restrict__ lv, BallGrid G, int axis,
+                                                           int d1, int d2, const int64_t* __restrict__ off,
+                                                           int64_t niv, Iv* __restrict__ iv,
+                                                           int64_t* __restrict__ mcnt) {
+  __shared__ double sa[kLineCap], sb[kLineCap];
+  __shared__ int32_t sia[kLineCap];
+  const int64_t li = blockIdx.x;
   if (li >= nl) return;
-  Iv* s = iv + off[li];
-  int64_t k = 0;
-  for_each_pierced(G, key_coord(lu[li]), key_coord(lv[li]), d1, d2, [&](int32_t a, double sq) {
-    const double c = G.xyz[3 * a + axis];
-    s[k++] = Iv{c - sq, c + sq, a, a};
-  });
-  // insertion sort under (start, end, ball): a total order, so the result is the
-  // host's std::sort result
-  for (int64_t i = 1; i < k; ++i) {
-    const Iv t = s[i];
-    int64_t j = i - 1;
-    while (j >= 0 && iv_less(t, s[j])) {
-      s[j + 1] = s[j];
-      --j;
+  const int lane = threadIdx.x;
+  const int64_t o0 = off[li], k = (li + 1 < nl ? off[li + 1] : niv) - o0;
+  Iv* s = iv + o0;
+  if (k > kLineCap) {
+    if (lane != 0) return;
+    int64_t kk = 0;
+    for_each_pierced(G, key_coord(lu[li]), key_coord(lv[li]), d1, d2, [&](int32_t a, double sq) {
+      const double c = G.xyz[3 * a + axis];
+      s[kk++] = Iv{c - sq, c + sq, a, a};
+    });
+    for (int64_t i = 1; i < kk; ++i) {
+      const Iv t = s[i];
+      int64_t j = i - 1;
+      while (j >= 0 && iv_less(t, s[j])) {
+        s[j + 1] = s[j];
+        --j;
+      }
+      s[j + 1] = t;
     }
-    s[j + 1] = t;
+    int64_t m = 0;
+    for (int64_t i = 0; i < kk; ++i) {
+      const Iv t = s[i];
+      if (m > 0 && t.a <= s[m - 1].b) {
+        if (t.b > s[m - 1].b) {
+          s[m - 1].b = t.b;
+          s[m - 1].ib = t.ib;
+        }
+      } else {
+        s[m++] = t;
+      }
+    }
+    mcnt[li] = 2 * m;
+    return;
   }
+  // gather: the same candidate bins and arithmetic as for_each_pierced
+  const double u = key_coord(lu[li]), v = key_coord(lv[li]);
+  const int b0 = (int)floor((u - G.lo0) / G.cell), b1 = (int)floor((v - G.lo1) / G.cell);
+  const uint64_t below = (1ULL << lane) - 1;
+  int n = 0;
+  for (int y = b1 - 1; y <= b1 + 1; ++y)
+    for (int x = b0 - 1; x <= b0 + 1; ++x) {
+      if (x < 0 || y < 0 || x >= G.gn0 || y >= G.gn1) continue;
+      const int64_t bin = (int64_t)y * G.gn0 + x;
+      for (int32_t base = G.bin_ptr[bin]; base < G.bin_ptr[bin + 1]; base += 64) {
+        const int32_t kb = base + lane;
+        bool hit = false;
+        double lo = 0, hi = 0;
+        int32_t a = 0;
+        if (kb < G.bin_ptr[bin + 1]) {
+          a = G.bin_ball[kb];
+          const double du = u - G.xyz[3 * a + d1], dv = v - G.xyz[3 * a + d2];
+          const double q = G.r[a] * G.r[a] - (du * du + dv * dv);
+          if (q > 0) {
+            const double sq = sqrt(q), c = G.xyz[3 * a + axis];
+            lo = c - sq;
+            hi = c + sq;
+            hit = true;
+          }
+        }
+        const uint64_t mask = __ballot(hit);
+        const int pos = n + __popcll(mask & below);
+        if (hit && pos < kLineCap) {
+          sa[pos] = lo;
+          sb[pos] = hi;
+          sia[pos] = a;
+        }
+        n += __popcll(mask);
+      }
+    }
+  n = min(n, kLineCap);  // equal to the counted k (same test), so never clipped
+  int P = 1;
+  while (P < n) P <<= 1;
+  for (int i = n + lane; i < P; i += 64) {
+    sa[i] = INFINITY;
+    sb[i] = INFINITY;
+    sia[i] = INT32_MAX;
+  }
+  __syncthreads();
+  for (int w = 2; w <= P; w <<= 1)
+    for (int j = w >> 1; j > 0; j >>= 1) {
+      for (int i = lane; i < P; i += 64) {
+        const int x = i ^ j;
+        if (x <= i) continue;
+        const Iv p{sa[i], sb[i], sia[i], sia[i]}, q{sa[x], sb[x], sia[x], sia[x]};
+        if (iv_less(q, p) == ((i & w) == 0)) {
+          sa[i] = q.a;
+          sb[i] = q.b;
+          sia[i] = q.ia;
+          sa[x] = p.a;
+          sb[x] = p.b;
+          sia[x] = p.ia;
+        }
+      }
+      __syncthreads();
+    }
+  if (lane != 0) return;
   int64_t m = 0;
-  for (int64_t i = 0; i < k; ++i) {
-    const Iv t = s[i];
-    if (m > 0 && t.a <= s[m - 1].b) {
-      if (t.b > s[m - 1].b) {
-        s[m - 1].b = t.b;
-        s[m - 1].ib = t.ib;
+  Iv last{0, 0, 0, 0};
+  for (int i = 0; i < n; ++i) {
+    const Iv t{sa[i], sb[i], sia[i], sia[i]};
+    if (m > 0 && t.a <= last.b) {
+      if (t.b > last.b) {
+        last.b = t.b;
+        last.ib = t.ib;
       }
     } else {
-      s[m++] = t;
+      if (m > 0) s[m - 1] = last;
+      last = t;
+      ++m;
     }
   }
+  if (m > 0) s[m - 1] = last;
   mcnt[li] = 2 * m;
 }
 
@@ -290,8 +385,8 @@ void run_cast_rays(ngpb_ctx* c, int64_t na, const double* h_xyz, const double* h
     const int64_t niv = exclusive_scan(c, off.p, off.p, nl);
     DBuf<Iv> iv(c->stream);
     iv.resize((size_t)std::max<int64_t>(niv, 1));
-    hipLaunchKernelGGL(k_merge_pierced, dim3(blocks_for(nl, 128)), dim3(128), 0, c->stream, nl, lu.p, lv.p, G, axis,
-                       d1, d2, off.p, iv.p, mcnt.p);
+    hipLaunchKernelGGL(k_merge_pierced_wave, dim3((unsigned)nl), dim3(64), 0, c->stream, nl, lu.p, lv.p, G, axis, d1,
+                       d2, off.p, niv, iv.p, mcnt.p);
     NGPB_HIP(hipGetLastError());
     // rays = lines with crossings
     DBuf<uint8_t> keep(c->stream);
