@@ -50,6 +50,31 @@ def test_device_rays_equal_host_rays(name):
     host.close()
 
 
+def test_lines_piercing_more_than_a_thousand_balls():
+    """1100 overlapping balls on a rod along x: the vertex lines near the axis pierce
+    every ball (more intervals than the per-line LDS holds), the y and z lines a few;
+    the device tables equal the host caster's."""
+    n = 1100
+    xyz = np.zeros((n, 3))
+    xyz[:, 0] = 0.1 * np.arange(n)
+    P = W.cluster(xyz, np.full(n, 1.5), np.zeros(n), h_fine=1.0, levels=2, bands=(1.5, 4.0), domain_factor=1.5,
+                  name="rod")
+    print(f"[rays case] rod N={P.mesh.n_nodes}", flush=True)
+    ctx = native.Context(0)
+    try:
+        ctx.set_physics(P.physics)
+        ctx.set_mesh(P.mesh)
+        ctx.cast_rays(P.atoms.xyz, P.atoms.r)
+        for axis in range(3):
+            key, ptr, inters, normals = ctx.get_rays(axis)
+            h = P.rays[axis]
+            assert np.array_equal(key, h.key) and np.array_equal(ptr, h.ptr), axis
+            assert np.array_equal(inters, h.inters) and np.array_equal(normals, h.normals), axis
+        assert P.rays[0].n >= 1 and np.array_equal(np.diff(P.rays[0].ptr), np.full(P.rays[0].n, 2))
+    finally:
+        ctx.close()
+
+
 def test_no_balls_gives_empty_tables():
     P = W.three_spheres()
     print("[rays case] no balls", flush=True)
