@@ -136,7 +136,8 @@ int ngpb_set_mesh(ngpb_ctx* ctx, int64_t n_nodes, const double* node_xyz, int64_
    init_tmesh / refine_surface and the tmesh_3d views above (src/pb_class.cpp:
    1289-1391, 1467-1745, 1838-1852) with a distance-band rule, as p4est is not part
    of this library: the box [origin, origin + n hf] is tiled by root cells of edge
-   2^smax hf (n[d] positive multiples of 2^smax, at most 2^21; smax in 0..10); a cell
+   2^smax hf (n[d] positive multiples of 2^smax, at most 2^21, with fewer than 2^32 grid
+   vertices (n[0]+1)(n[1]+1)(n[2]+1); smax in 0..10); a cell
    of level s >= 1 (edge 2^s hf) is split while some ball a has
    |centre - xyz_a| - r_a < band[s-1] + sqrt(3)/2 * edge (band[0..smax-1] finite and
    non-decreasing); then leaves of level s >= 2 next to (faces, edges, corners) a leaf
