@@ -77,4 +77,9 @@ def test_null_arguments_rejected():
     out = np.zeros(6)
     assert L.ngpb_set_physics(None, None) == 1
     assert L.ngpb_solve(None, None, None, None) == 1
+    # a null context is rejected before any device work by every context call
+    assert L.ngpb_build_octree(None, None, 0.5, None, 0, None, 0, None, None, None) == 1
+    assert L.ngpb_get_mesh(None, None, None, None, None, None, None, None) == 1
+    assert L.ngpb_cast_rays(None, 0, None, None) == 1
+    assert L.ngpb_locate_atoms(None, 0, None, 0.0, None) == 1
     del out
