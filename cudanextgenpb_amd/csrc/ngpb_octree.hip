@@ -363,8 +363,9 @@ __global__ void k_leaf_corners(const OLeaf* __restrict__ lv, int64_t L, Geom g, 
   for (int k = 0; k < 8; ++k) corner[8 * i + k] = vmap[corner_vertex(g, l, k)];
 }
 
-// one split/keep pass over `in` (flags and packed counters already computed):
-// kept leaves appended to keep (at keep_off), children to `out`; returns (kept, split)
+// one split/keep pass over `in` (flags and packed counters already computed): the
+// counters are scanned, kept cells go to keep[0 ..), the children of split cells to
+// `out` (resized to 8 per split cell); returns (kept, split)
 std::pair<int64_t, int64_t> split_pass(ngpb_ctx* c, const OLeaf* in, int64_t n, const uint8_t* flag, int64_t* cnt,
                                        OLeaf* keep, DBuf<OLeaf>& out) {
   const int64_t tot = exclusive_scan(c, cnt, cnt, n);
