@@ -417,8 +417,9 @@ def main():
 
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus != 1 or world != 1:
-        print("bench.py measures one GPU: this build has no multi-GPU path (each system is solved on one "
-              "device; see the sharding notes in DESIGN.md). Run with --gpus 1.", file=sys.stderr)
+        print("bench.py measures one GPU: there is no multi-GPU path (each system is solved on one device; "
+              "independent systems would be spread over devices, a single large one split into Morton ranges "
+              "of leaves with a halo exchange per iteration). Run with --gpus 1.", file=sys.stderr)
         sys.exit(2)
 
     from cudanextgenpb_amd import native, workload as W
