@@ -2,7 +2,8 @@
 
 * Each result is the same alone and in the mix: every request served on its own and
   within a stream of overlapping requests (batched and streamed) gives bit-identical
-  potentials and energies.
+  potentials and energies, also when the requests are prepared from their atoms on the
+  device.
 * Mixed-precision results agree with the FP64 solve to 1e-6 relative in energy.
 * The footprint prediction used for admission bounds the device bytes a request holds.
 * A real-time stream completes with every solve converged and reports p50/p99.
@@ -56,6 +57,24 @@ def test_alone_equals_mix():
         assert np.array_equal(a.phi, b.phi), req.rid
         assert a.energy["polar"] == b.energy["polar"], req.rid
         assert a.iterations == b.iterations
+
+
+def test_mix_from_atoms_equals_mix_from_host_meshes():
+    """The same stream with every request prepared from its atoms on the device (octree,
+    rays and owners built by the preparing threads while the batch replays) gives
+    bit-identical potentials and energies."""
+    m = mix()
+    s = serve.Server(budget_bytes=250e9, small_max_unknowns=small_max(), max_batch=8, keep_phi=True,
+                     device_mesh=True)
+    try:
+        d = s.run(list(stream()), realtime=False)
+    finally:
+        s.close()
+    for req in stream():
+        a, b = d[req.rid], m[req.rid]
+        assert a.mode == b.mode and a.status == 0
+        assert np.array_equal(a.phi, b.phi), req.rid
+        assert a.energy["polar"] == b.energy["polar"], req.rid
 
 
 def test_mixed_energy_vs_fp64_and_footprint():
