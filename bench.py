@@ -91,7 +91,7 @@ def cpu_baseline(pb, fp64_iters, seconds):
     t5 = O.pcg_time_iters(S, 5, threads)
     t25 = O.pcg_time_iters(S, 25, threads)
     per_it = max((t25 - t5) / 20, 1e-6)
-    n_it = int(max(5, min(4000, seconds / per_it)))
+    n_it = int(max(5, min(20000, seconds / per_it)))
     t = O.pcg_time_iters(S, n_it, threads)
     per_it = t / n_it
     solve_s = per_it * fp64_iters
