@@ -44,8 +44,7 @@ def prepare(problem: Problem, device: int = 0, ctx: Optional[native.Context] = N
     c.set_physics(problem.physics)
     a = problem.atoms
     if device_mesh:
-        m = problem.mesh
-        c.build_octree(m.origin, m.hf, m.n, m.smax, m.bands, a.xyz, a.r)
+        c.build_octree(*problem.mesh.generator_params(), a.xyz, a.r)
         c.cast_rays(a.xyz, a.r)
         c.create_markers()
         c.set_charges(a.xyz, a.q, c.locate_atoms(a.xyz, problem.atom_tol))

@@ -43,8 +43,7 @@ class PoissonBoltzmann:
         self.ctx = ctx or native.Context(device)
         self.ctx.set_physics(problem.physics)
         if device_mesh:
-            m = problem.mesh
-            self.ctx.build_octree(m.origin, m.hf, m.n, m.smax, m.bands, problem.atoms.xyz, problem.atoms.r)
+            self.ctx.build_octree(*problem.mesh.generator_params(), problem.atoms.xyz, problem.atoms.r)
         else:
             self.ctx.set_mesh(problem.mesh)
         if device_rays or device_mesh:

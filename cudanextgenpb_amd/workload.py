@@ -129,6 +129,14 @@ class Mesh:
     def n_leaves(self) -> int:
         return int(self.leaf_corner.shape[0])
 
+    def generator_params(self):
+        """(origin, hf, n, smax, bands) that rebuild this mesh with ngpb_build_octree;
+        raises ValueError for a mesh the generator did not make (leaves coarser than
+        its recorded levels)."""
+        if self.leaf_level.size and int(self.leaf_level.max()) > self.smax or len(self.bands) < self.smax:
+            raise ValueError("this mesh carries no octree generator parameters; upload it with set_mesh instead")
+        return self.origin, self.hf, self.n, self.smax, self.bands
+
     def edge_x(self) -> float:
         return float(self.n[0] * self.hf)
 

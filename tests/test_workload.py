@@ -110,6 +110,20 @@ def test_octree_refinement_rule_brute_force(maker):
     assert by_rule.any()
 
 
+def test_generator_params_rebuild_or_refuse():
+    """A generated mesh records the parameters that rebuild it; a mesh whose leaves are
+    coarser than its recorded levels (not made by the generator) is refused."""
+    import dataclasses
+    P = W.three_spheres(levels=3, bands=(0.8, 2.0, 5.0))
+    o, hf, n, smax, bands = P.mesh.generator_params()
+    assert smax == 3 and bands == (0.8, 2.0, 5.0) and hf == P.mesh.hf
+    m2 = W.make_mesh(o, hf, n, smax, bands, P.atoms)
+    assert np.array_equal(m2.leaf_corner, P.mesh.leaf_corner) and np.array_equal(m2.node_xyz, P.mesh.node_xyz)
+    foreign = dataclasses.replace(P.mesh, smax=0, bands=(), _handle=None)
+    with pytest.raises(ValueError):
+        foreign.generator_params()
+
+
 def test_rays_are_sorted_and_on_spheres():
     P = W.three_spheres()
     for axis, r in enumerate(P.rays):
