@@ -50,23 +50,59 @@ __global__ void k_leaf_extent(MeshView m, double* __restrict__ mn, double* __res
   edge[l] = e;
 }
 
-__global__ void k_leaf_bucket(MeshView m, LeafGrid G, const double* __restrict__ mn, const double* __restrict__ mx,
-                              int32_t* __restrict__ bucket, int32_t* __restrict__ cnt) {
-  const int64_t l = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (l >= m.L) return;
-  double ctr[3];
-  for (int d = 0; d < 3; ++d) ctr[d] = 0.5 * (mn[3 * l + d] + mx[3 * l + d]);
-  const int64_t b = bucket_of(G, ctr);
-  bucket[l] = (int32_t)b;
-  atomicAdd(&cnt[b], 1);
+// Runs of consecutive lanes with the same bucket (leaves in Morton order mostly share
+// one) take one atomic per run: the run's first lane adds the run length, and each
+// lane's offset is its distance from that lane. Lanes past the end form no run.
+struct LaneRun {
+  int head;  // lane holding the run's first element
+  int len;   // run length (valid on the head lane)
+};
+
+__device__ __forceinline__ LaneRun lane_run(bool valid, int32_t key) {
+  const int lane = threadIdx.x & 63;
+  const int32_t prev = __shfl(key, lane > 0 ? lane - 1 : 0, 64);
+  const bool prev_valid = lane > 0;  // lanes are filled from lane 0, so a valid lane's predecessor is valid
+  const bool head = valid && (!prev_valid || prev != key);
+  const uint64_t heads = __ballot(head), vmask = __ballot(valid);
+  // head of this lane's run: the highest head at or below the lane
+  const uint64_t upto = heads & ((lane == 63) ? ~0ULL : ((2ULL << lane) - 1));
+  LaneRun r;
+  r.head = upto ? 63 - __clzll(upto) : 0;
+  // run length on a head lane: up to the next head or the last valid lane
+  const uint64_t above = heads & ~((lane == 63) ? ~0ULL : ((2ULL << lane) - 1));
+  const int end = above ? __ffsll((unsigned long long)above) - 1 : (vmask ? 64 - __clzll(vmask) : 0);
+  r.len = end - lane;
+  return r;
 }
 
-__global__ void k_leaf_fill(int64_t L, const int32_t* __restrict__ bucket, const int64_t* __restrict__ start,
-                            int32_t* __restrict__ cursor, int32_t* __restrict__ list) {
+__global__ void __launch_bounds__(256) k_leaf_bucket(MeshView m, LeafGrid G, const double* __restrict__ mn,
+                                                     const double* __restrict__ mx, int32_t* __restrict__ bucket,
+                                                     int32_t* __restrict__ cnt) {
   const int64_t l = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (l >= L) return;
-  const int32_t b = bucket[l];
-  list[start[b] + atomicAdd(&cursor[b], 1)] = (int32_t)l;
+  const bool valid = l < m.L;
+  int32_t b = -1;
+  if (valid) {
+    double ctr[3];
+    for (int d = 0; d < 3; ++d) ctr[d] = 0.5 * (mn[3 * l + d] + mx[3 * l + d]);
+    b = (int32_t)bucket_of(G, ctr);
+    bucket[l] = b;
+  }
+  const LaneRun r = lane_run(valid, b);
+  if (valid && r.head == (int)(threadIdx.x & 63)) atomicAdd(&cnt[b], r.len);
+}
+
+__global__ void __launch_bounds__(256) k_leaf_fill(int64_t L, const int32_t* __restrict__ bucket,
+                                                   const int64_t* __restrict__ start, int32_t* __restrict__ cursor,
+                                                   int32_t* __restrict__ list) {
+  const int64_t l = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const bool valid = l < L;
+  const int32_t b = valid ? bucket[l] : -1;
+  const int lane = threadIdx.x & 63;
+  const LaneRun r = lane_run(valid, b);
+  int32_t base = 0;
+  if (valid && r.head == lane) base = atomicAdd(&cursor[b], r.len);
+  base = __shfl(base, r.head, 64);
+  if (valid) list[start[b] + base + (lane - r.head)] = (int32_t)l;
 }
 
 // one wave per atom: the lanes split each bucket's leaves, then the wave takes the
