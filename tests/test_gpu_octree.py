@@ -139,3 +139,41 @@ def test_prepare_from_device_mesh_equals_host_mesh():
     finally:
         a.ctx.close()
         b.ctx.close()
+
+
+SWEEP = [  # (seed, atoms, ball radius, h_fine, levels, bands)
+    (101, 12, 4.0, 0.7, 1, (1.0,)),
+    (202, 40, 6.0, 0.5, 2, (1.0, 2.5)),
+    (303, 90, 8.0, 0.6, 3, (0.8, 1.6, 3.2)),
+    (404, 150, 9.0, 0.45, 2, (2.0, 2.0)),
+    (505, 25, 5.0, 0.8, 3, (0.5, 1.0, 6.0)),
+]
+
+
+@pytest.mark.parametrize("case", SWEEP, ids=[f"seed{c[0]}" for c in SWEEP])
+def test_random_molecules_mesh_rays_owners_equal_host(case):
+    """Seeded random molecules over levels 1-3, equal and rising bands, several leaf
+    sizes: the device octree, ray tables and atom owners equal the host generator's."""
+    seed, na, rad, h, lev, bands = case
+    atoms = W.make_atoms(seed, na, [0.0, 0.0, 0.0], [rad] * 3, 1.5, 1.2, 2.0, 0.3, float("nan"))
+    P = W.cluster(atoms.xyz, atoms.r, atoms.q, h_fine=h, levels=lev, bands=bands, domain_factor=2.0,
+                  name=f"sweep{seed}")
+    m = P.mesh
+    print(f"[octree sweep] seed={seed} n={m.n.tolist()} smax={m.smax} N={m.n_nodes} H={m.n_hang} L={m.n_leaves}",
+          flush=True)
+    ctx = native.Context(0)
+    try:
+        ctx.set_physics(P.physics)
+        sizes, g = _device_mesh(ctx, P)
+        assert sizes[:3] == (m.n_nodes, m.n_hang, m.n_leaves)
+        for k in MESH_KEYS:
+            assert np.array_equal(g[k], getattr(m, k)), k
+        ctx.cast_rays(P.atoms.xyz, P.atoms.r)
+        for axis in range(3):
+            key, ptr, inters, normals = ctx.get_rays(axis)
+            hr = P.rays[axis]
+            assert np.array_equal(key, hr.key) and np.array_equal(ptr, hr.ptr), axis
+            assert np.array_equal(inters, hr.inters) and np.array_equal(normals, hr.normals), axis
+        assert np.array_equal(ctx.locate_atoms(P.atoms.xyz, P.atom_tol), P.owner)
+    finally:
+        ctx.close()
