@@ -34,7 +34,14 @@ def problem(name):
     if name == "globular_small":
         return W.globular(n_atoms=60, ball_radius=6.0, min_sep=2.0, h_fine=0.5, levels=2,
                           fine_band=1.5, mid_band=4.0, bc_type=2)
+    if name in SWEEP:  # random molecules on 3-level octrees (long hanging-parent rows)
+        seed, na, rad, h, bands = SWEEP[name]
+        atoms = W.make_atoms(seed, na, [0.0, 0.0, 0.0], [rad] * 3, 1.5, 1.2, 2.0, 0.3, float("nan"))
+        return W.cluster(atoms.xyz, atoms.r, atoms.q, h_fine=h, levels=3, bands=bands, domain_factor=2.0, name=name)
     raise KeyError(name)
+
+
+SWEEP = {"sweep303": (303, 90, 8.0, 0.6, (0.8, 1.6, 3.2)), "sweep505": (505, 25, 5.0, 0.8, (0.5, 1.0, 6.0))}
 
 
 @functools.lru_cache(maxsize=None)
@@ -72,7 +79,7 @@ def test_markers_border_fractions_bitexact(name):
     pb.close()
 
 
-@pytest.mark.parametrize("name", CASES + ["three_spheres_bc1"])
+@pytest.mark.parametrize("name", CASES + ["three_spheres_bc1"] + list(SWEEP))
 def test_assembly_bitexact(name):
     P = problem(name)
     R = reference(name)
@@ -84,6 +91,23 @@ def test_assembly_bitexact(name):
     assert np.array_equal(col, S.col)
     assert np.array_equal(val, S.val)
     assert np.array_equal(rhs, S.rhs)
+    pb.close()
+
+
+@pytest.mark.parametrize("name", list(SWEEP))
+def test_assembly_bitexact_from_device_mesh(name):
+    """Octree, rays and owners built on the device, then assembly: the same CSR and rhs
+    as the oracle's on the host-built mesh."""
+    P = problem(name)
+    R = reference(name)
+    pb = PoissonBoltzmann(P, device_mesh=True)
+    pb.create_markers()
+    pb.create_density_map()
+    pb.assemple_system_matrix(O.dirichlet_values(P))
+    rp, col, val, rhs = pb.ctx.export_csr()
+    S = R["system"]
+    assert np.array_equal(rp, S.rowptr) and np.array_equal(col, S.col)
+    assert np.array_equal(val, S.val) and np.array_equal(rhs, S.rhs)
     pb.close()
 
 
