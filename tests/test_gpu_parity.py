@@ -137,7 +137,7 @@ def test_spmv_fp64_bitexact(name):
     ctx.close()
 
 
-@pytest.mark.parametrize("name", CASES)
+@pytest.mark.parametrize("name", CASES + list(SWEEP))
 def test_solve_fp64_matches_oracle(name):
     R = reference(name)
     S = R["system"]
@@ -152,7 +152,7 @@ def test_solve_fp64_matches_oracle(name):
     ctx.close()
 
 
-@pytest.mark.parametrize("name", CASES)
+@pytest.mark.parametrize("name", CASES + list(SWEEP))
 def test_energy_matches_oracle(name):
     P = problem(name)
     R = reference(name)
