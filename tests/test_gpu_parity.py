@@ -166,7 +166,7 @@ def test_energy_matches_oracle(name):
     pb.close()
 
 
-@pytest.mark.parametrize("name", ["three_spheres", "globular_small"])
+@pytest.mark.parametrize("name", ["three_spheres", "globular_small"] + list(SWEEP))
 def test_phi_on_atoms_bitexact(name, tmp_path):
     P = problem(name)
     R = reference(name)
@@ -223,7 +223,7 @@ def test_born_energy_closed_form():
     pb.close()
 
 
-@pytest.mark.parametrize("name", CASES)
+@pytest.mark.parametrize("name", CASES + list(SWEEP))
 def test_mixed_precision_energy(name):
     P, pb = gpu_pipeline(name)
     pb.hip_compute_electric_potential(precision=native.PREC_FP64)
@@ -239,7 +239,7 @@ def test_mixed_precision_energy(name):
     pb.close()
 
 
-@pytest.mark.parametrize("name", CASES)
+@pytest.mark.parametrize("name", CASES + list(SWEEP))
 def test_compact_spmv_close_to_fp64(name):
     P, pb = gpu_pipeline(name)
     x = np.random.default_rng(3).standard_normal(P.mesh.n_nodes)
