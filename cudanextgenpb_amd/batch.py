@@ -41,21 +41,26 @@ def prepare(problem: Problem, device: int = 0, ctx: Optional[native.Context] = N
     problem's host-built ones)."""
     t0 = time.perf_counter()
     c = ctx or native.Context(device)
-    c.set_physics(problem.physics)
-    a = problem.atoms
-    if device_mesh:
-        c.build_octree(*problem.mesh.generator_params(), a.xyz, a.r)
-        c.cast_rays(a.xyz, a.r)
-        c.create_markers()
-        c.set_charges(a.xyz, a.q, c.locate_atoms(a.xyz, problem.atom_tol))
-    else:
-        c.set_mesh(problem.mesh)
-        c.set_rays(problem.rays)
-        c.create_markers()
-        c.set_charges(a.xyz, a.q, problem.owner)
-        c.set_dirichlet(problem.mesh.bnd_nodes, None)
-    c.dirichlet_values(problem.bc_type)
-    c.assemble()
+    try:
+        c.set_physics(problem.physics)
+        a = problem.atoms
+        if device_mesh:
+            c.build_octree(*problem.mesh.generator_params(), a.xyz, a.r)
+            c.cast_rays(a.xyz, a.r)
+            c.create_markers()
+            c.set_charges(a.xyz, a.q, c.locate_atoms(a.xyz, problem.atom_tol))
+        else:
+            c.set_mesh(problem.mesh)
+            c.set_rays(problem.rays)
+            c.create_markers()
+            c.set_charges(a.xyz, a.q, problem.owner)
+            c.set_dirichlet(problem.mesh.bnd_nodes, None)
+        c.dirichlet_values(problem.bc_type)
+        c.assemble()
+    except BaseException:
+        if ctx is None:  # a context made here does not outlive a failed preparation
+            c.close()
+        raise
     return Prepared(problem.name, c, c.n, c.nnz, a.n, (time.perf_counter() - t0) * 1e3)
 
 

@@ -15,6 +15,9 @@
 namespace ngpb {
 
 void throw_hip(hipError_t e, const char* what, const char* file, int line) {
+  // clear the thread's last-error slot, so that a failed allocation (which leaves the
+  // context usable) is not reported again by the next launch check
+  (void)hipGetLastError();
   std::ostringstream os;
   os << "HIP error " << hipGetErrorName(e) << " (" << hipGetErrorString(e) << ") in " << what << " at " << file << ":"
      << line;
@@ -54,6 +57,7 @@ void install_device_mesh(ngpb_ctx* c, int64_t N, int64_t H, int64_t L, int64_t N
   c->have_mesh = true;
   c->have_dirichlet = true;
   c->have_bnd_values = NB == 0;
+  c->bnd_bc_type = 0;
   c->have_charges = false;
   c->have_markers = c->have_frac = c->have_symbolic = c->have_system = c->have_compact = c->have_phi = false;
 }
@@ -212,6 +216,7 @@ int ngpb_set_physics(ngpb_ctx* c, const ngpb_physics* p) {
     c->pc = ngpb::physical_constants(*p);
     c->have_physics = true;
     c->have_markers = c->have_system = c->have_compact = c->have_phi = false;
+    if (c->bnd_bc_type) c->have_bnd_values = false;  // recomputed at assembly
   });
 }
 
@@ -254,6 +259,7 @@ int ngpb_set_mesh(ngpb_ctx* c, int64_t n_nodes, const double* node_xyz, int64_t 
     NGPB_HIP(hipStreamSynchronize(c->stream));
     c->have_mesh = true;
     c->have_dirichlet = c->have_bnd_values = c->have_charges = false;
+    c->bnd_bc_type = 0;
     invalidate_after_mesh(c);
   });
 }
@@ -334,16 +340,17 @@ int ngpb_set_rays(ngpb_ctx* c, int axis, int64_t n_rays, const double* key, cons
     } else {
       NGPB_REQUIRE(key && ptr, NGPB_E_ARG, "NULL ray array");
       NGPB_REQUIRE(ptr[0] == 0, NGPB_E_ARG, "ray pointers must start at 0");
-      for (int64_t i = 0; i < n_rays; ++i) {
+      for (int64_t i = 0; i < n_rays; ++i)
         NGPB_REQUIRE(ptr[i + 1] >= ptr[i], NGPB_E_ARG, "ray pointers must be non-decreasing");
+      const int64_t K = ptr[n_rays];
+      NGPB_REQUIRE(K == 0 || (inters && normals), NGPB_E_ARG, "NULL crossing array");
+      for (int64_t i = 0; i < n_rays; ++i) {
         if (i > 0)
           NGPB_REQUIRE(key[2 * i - 2] < key[2 * i] || (key[2 * i - 2] == key[2 * i] && key[2 * i - 1] < key[2 * i + 1]),
                        NGPB_E_ARG, "ray keys must be strictly increasing");
         for (int64_t k = ptr[i] + 1; k < ptr[i + 1]; ++k)
           NGPB_REQUIRE(inters[k - 1] <= inters[k], NGPB_E_ARG, "crossings must be sorted within a ray");
       }
-      const int64_t K = ptr[n_rays];
-      NGPB_REQUIRE(K == 0 || (inters && normals), NGPB_E_ARG, "NULL crossing array");
       r.n = n_rays;
       r.k = K;
       r.key.upload(key, (size_t)(2 * n_rays), c->stream);
@@ -499,6 +506,7 @@ int ngpb_set_charges(ngpb_ctx* c, int64_t n_atoms, const double* xyz, const doub
     NGPB_HIP(hipStreamSynchronize(c->stream));
     c->have_charges = true;
     c->have_system = c->have_compact = c->have_phi = false;
+    if (c->bnd_bc_type) c->have_bnd_values = false;  // recomputed at assembly
   });
 }
 
@@ -527,6 +535,7 @@ int ngpb_set_dirichlet(ngpb_ctx* c, int64_t n_bnd, const int32_t* nodes, const d
     NGPB_HIP(hipStreamSynchronize(c->stream));
     c->have_dirichlet = true;
     c->have_bnd_values = values != nullptr || n_bnd == 0;
+    c->bnd_bc_type = 0;
     c->have_symbolic = c->have_system = c->have_compact = c->have_phi = false;
   });
 }
@@ -536,6 +545,7 @@ int ngpb_dirichlet_values(ngpb_ctx* c, int bc_type) {
     not_in_batch(c);
     need(c->have_dirichlet, "ngpb_set_dirichlet must come before ngpb_dirichlet_values");
     ngpb::run_dirichlet_values(c, bc_type);
+    c->bnd_bc_type = bc_type;
     c->have_system = c->have_compact = c->have_phi = false;
   });
 }
@@ -553,11 +563,16 @@ int ngpb_assemble(ngpb_ctx* c) {
       c->gval.upload(g.data(), g.size(), c->stream);
       c->bnd_node.resize(1);
       c->have_dirichlet = c->have_bnd_values = true;
+      c->bnd_bc_type = 0;
     }
+    // computed boundary values follow the current charges and physics
+    if (!c->have_bnd_values && c->bnd_bc_type) ngpb::run_dirichlet_values(c, c->bnd_bc_type);
     need(c->have_bnd_values, "Dirichlet values missing (ngpb_dirichlet_values)");
     if (!c->have_frac) ngpb::run_fractions(c);
-    if (!c->have_symbolic) ngpb::run_symbolic(c);
+    // after ngpb_set_csr the CSR arrays no longer hold the mesh's pattern
+    if (!c->have_symbolic || c->external_csr) ngpb::run_symbolic(c);
     ngpb::run_numeric(c);
+    c->external_csr = false;
     c->have_phi = false;
   });
 }
@@ -594,7 +609,9 @@ int ngpb_set_csr(ngpb_ctx* c, int64_t n, int64_t nnz, const int32_t* rowptr, con
       for (int64_t k = rowptr[i]; k < rowptr[i + 1]; ++k) {
         NGPB_REQUIRE(col[k] >= 0 && col[k] < n, NGPB_E_ARG, "column index out of range");
         if (k > rowptr[i]) NGPB_REQUIRE(col[k] > col[k - 1], NGPB_E_ARG, "columns must be strictly ascending");
+        NGPB_REQUIRE(std::isfinite(val[k]), NGPB_E_NUMERIC, "matrix holds NaN or Inf entries");
       }
+      NGPB_REQUIRE(std::isfinite(rhs[i]), NGPB_E_NUMERIC, "right-hand side holds NaN or Inf entries");
     }
     c->n = n;
     c->nnz = nnz;
@@ -604,6 +621,10 @@ int ngpb_set_csr(ngpb_ctx* c, int64_t n, int64_t nnz, const int32_t* rowptr, con
     c->rhs.upload(rhs, (size_t)n, c->stream);
     NGPB_HIP(hipStreamSynchronize(c->stream));
     c->have_system = true;
+    // the symbolic tables describe the mesh's pattern, not this one: the next
+    // ngpb_assemble rebuilds them
+    c->have_symbolic = false;
+    c->external_csr = true;
     c->have_compact = c->have_phi = false;
   });
 }
@@ -692,6 +713,7 @@ int ngpb_set_phi(ngpb_ctx* c, const double* phi) {
     c->x.upload(phi, (size_t)c->N, c->stream);
     NGPB_HIP(hipStreamSynchronize(c->stream));
     c->have_phi = true;
+    c->phi_len = c->N;
   });
 }
 
@@ -715,6 +737,7 @@ int ngpb_phi_on_atoms(ngpb_ctx* c, double* phi_atoms) {
   return guarded(c, [&] {
     NGPB_REQUIRE(phi_atoms, NGPB_E_ARG, "NULL output");
     need(c->have_phi, "no potential (solve or ngpb_set_phi first)");
+    need(c->have_mesh && c->phi_len == c->N, "the potential does not hold one value per mesh node");
     need(c->have_charges, "no atoms (ngpb_set_charges)");
     if (c->A == 0) return;
     ngpb::DBuf<double> d;
@@ -730,6 +753,7 @@ int ngpb_energy(ngpb_ctx* c, ngpb_energy_out* out) {
     NGPB_REQUIRE(out, NGPB_E_ARG, "NULL output");
     need(c->have_markers, "markers are not available");
     need(c->have_phi, "no potential (solve or ngpb_set_phi first)");
+    need(c->phi_len == c->N, "the potential does not hold one value per mesh node");
     if (!c->have_frac) ngpb::run_fractions(c);
     ngpb::run_energy(c, out);
   });
@@ -748,6 +772,7 @@ int ngpb_pot_field(ngpb_ctx* c, int calc_potential_term, int calc_field_term, do
     if (out->branch != 0) {
       need(c->have_markers, "markers are not available");
       need(c->have_phi, "no potential (solve or ngpb_set_phi first)");
+      need(c->phi_len == c->N, "the potential does not hold one value per mesh node");
       if (!c->have_frac) ngpb::run_fractions(c);
     }
     double e[4];

@@ -765,6 +765,7 @@ int ngpb_batch_run(ngpb_batch* b, int max_replays, int* running) {
           if (!b->reported[s]) {
             b->reported[s] = 1;
             c->have_phi = true;
+            c->phi_len = c->n;
           }
         } else {
           ++left;

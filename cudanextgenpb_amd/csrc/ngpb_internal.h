@@ -203,6 +203,9 @@ struct ngpb_ctx {
   ngpb::DBuf<uint8_t> is_bnd;   // N
   ngpb::DBuf<double> gval;      // N (value on boundary nodes, 0 elsewhere)
   bool have_dirichlet = false, have_bnd_values = false;
+  // boundary values computed by ngpb_dirichlet_values (not supplied): they depend on
+  // the charges and the physics, so changing either recomputes them at assembly
+  int bnd_bc_type = 0;  // 0: supplied by the caller, else the bc_type they were computed for
 
   // symbolic assembly (depends on mesh + Dirichlet set)
   bool have_symbolic = false;
@@ -224,6 +227,7 @@ struct ngpb_ctx {
 
   // assembled system
   bool have_system = false;
+  bool external_csr = false;  // the system came from ngpb_set_csr, not from the mesh
   int64_t n = 0, nnz = 0;
   ngpb::DBuf<int32_t> rowptr, col;
   ngpb::DBuf<double> val, rhs, rhs_charge;
@@ -268,6 +272,7 @@ struct ngpb_ctx {
   ngpb::KrylovScalars* ks_host = nullptr;  // pinned mirror
   ngpb::GridReduce red;
   bool have_phi = false;
+  int64_t phi_len = 0;    // entries of x that hold the potential (N after an assembled solve)
   bool in_batch = false;  // solver vectors are referenced by a batch: no reallocation
 
   // energy scratch

@@ -423,6 +423,7 @@ void solve_fp64(ngpb_ctx* c, const ngpb_solve_opts& o, ngpb_solve_stats& st) {
   // SpMV + update (x, p, r, q, dinv read; x, r written) + direction (dinv, r, p read; p written)
   st.bytes_per_iter = spmv_fp64_bytes(c) + (double)n * 8.0 * (7.0 + 4.0);
   c->have_phi = true;
+  c->phi_len = c->n;
 }
 
 }  // namespace ngpb

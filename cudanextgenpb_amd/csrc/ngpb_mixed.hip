@@ -1125,6 +1125,7 @@ void solve_mixed(ngpb_ctx* c, const ngpb_solve_opts& o, ngpb_solve_stats& st) {
   // SpMV + update (rhat, q read; rhat written) + direction (rhat, p, xhat read; p, xhat written)
   st.bytes_per_iter = spmv_compact_bytes(c) + (double)n * 4.0 * (3.0 + 5.0);
   c->have_phi = true;
+  c->phi_len = c->n;
 }
 
 }  // namespace ngpb

@@ -104,9 +104,10 @@ class Result:
     latency_ms: float
     iterations: int
     status: int
-    energy: dict
+    energy: Optional[dict]
     device_bytes: int
     phi: Optional[np.ndarray] = None
+    error: Optional[str] = None   # set when the request failed (status != 0 then)
 
 
 class Server:
@@ -131,14 +132,24 @@ class Server:
         self.batch.close()
 
     def _stream_solve(self, req: Request):
-        """Large request: prepared and solved on its own context (worker thread)."""
+        """Large request: prepared and solved on its own context (worker thread). A
+        solve that stops at maxiter or stagnates is reported through its status, as a
+        batched one is; the context is closed whatever happens."""
         p = prepare(req.problem, self.device, device_mesh=self.device_mesh)
-        _, st = p.ctx.solve(native.PREC_MIXED, tol=self.tol, want_phi=False)
-        e = p.ctx.energy()
-        phi = p.ctx.get_phi() if self.keep_phi else None
-        nbytes = p.ctx.device_bytes()
-        p.ctx.close()
+        try:
+            _, st = p.ctx.solve(native.PREC_MIXED, tol=self.tol, want_phi=False, raise_on_noconv=False)
+            e = p.ctx.energy()
+            phi = p.ctx.get_phi() if self.keep_phi else None
+            nbytes = p.ctx.device_bytes()
+        finally:
+            p.ctx.close()
         return st, e, phi, nbytes
+
+    @staticmethod
+    def _failed(req: Request, mode: str, adm: float, t: float, exc: BaseException) -> Result:
+        status = exc.code if isinstance(exc, native.NGPBError) else -1
+        return Result(req.rid, mode, req.arrival, adm, t, (t - req.arrival) * 1e3, 0, status or -1, None, 0,
+                      None, f"{type(exc).__name__}: {exc}")
 
     def run(self, requests: List[Request], realtime: bool = True) -> Dict[int, Result]:
         """Serves `requests` (arrival times relative to the start). Small requests are
@@ -181,12 +192,23 @@ class Server:
                 req, adm, fut = item
                 if not fut.done():
                     continue
-                p = fut.result()
-                slot = self.batch.add(p.ctx)
-                p.ctx._keep_mesh = None
-                in_batch[slot] = (req, p.ctx, adm)
                 preparing.remove(item)
                 progressed = True
+                try:
+                    p = fut.result()
+                except Exception as exc:  # this request fails; the others go on
+                    used -= req.predicted_bytes
+                    results[req.rid] = self._failed(req, "batch", adm, clock(), exc)
+                    continue
+                try:
+                    slot = self.batch.add(p.ctx)
+                except Exception as exc:
+                    p.ctx.close()
+                    used -= req.predicted_bytes
+                    results[req.rid] = self._failed(req, "batch", adm, clock(), exc)
+                    continue
+                p.ctx._keep_mesh = None
+                in_batch[slot] = (req, p.ctx, adm)
             if in_batch:
                 self.batch.run(1)
                 progressed = True
@@ -194,25 +216,38 @@ class Server:
                     done, st = self.batch.status(slot)
                     if not done:
                         continue
-                    self.batch.remove(slot)
-                    e = ctx.energy()
-                    phi = ctx.get_phi() if self.keep_phi else None
-                    t = clock()
-                    nbytes = ctx.device_bytes()
-                    ctx.close()
-                    used -= req.predicted_bytes
                     del in_batch[slot]
+                    used -= req.predicted_bytes
+                    try:
+                        self.batch.remove(slot)
+                    except Exception as exc:  # the context stays with the batch
+                        results[req.rid] = self._failed(req, "batch", adm, clock(), exc)
+                        continue
+                    try:
+                        e = ctx.energy()
+                        phi = ctx.get_phi() if self.keep_phi else None
+                        t = clock()
+                        nbytes = ctx.device_bytes()
+                    except Exception as exc:
+                        results[req.rid] = self._failed(req, "batch", adm, clock(), exc)
+                        continue
+                    finally:
+                        ctx.close()
                     results[req.rid] = Result(req.rid, "batch", req.arrival, adm, t, (t - req.arrival) * 1e3,
                                               st["iterations"], st["status"], e, nbytes, phi)
             for item in list(streaming):
                 req, adm, fut = item
                 if not fut.done():
                     continue
-                st, e, phi, nbytes = fut.result()
                 t = clock()
                 used -= req.predicted_bytes
                 streaming.remove(item)
                 progressed = True
+                try:
+                    st, e, phi, nbytes = fut.result()
+                except Exception as exc:
+                    results[req.rid] = self._failed(req, "stream", adm, t, exc)
+                    continue
                 results[req.rid] = Result(req.rid, "stream", req.arrival, adm, t, (t - req.arrival) * 1e3,
                                           st["iterations"], st["status"], e, nbytes, phi)
             if not progressed:
@@ -262,4 +297,5 @@ def summarize(results: Dict[int, Result]) -> dict:
         "batched": sum(r.mode == "batch" for r in results.values()),
         "streamed": sum(r.mode == "stream" for r in results.values()),
         "failed": sum(r.status != 0 for r in results.values()),
+        "errors": sum(r.error is not None for r in results.values()),
     }

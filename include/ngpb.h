@@ -199,7 +199,10 @@ int ngpb_set_charges(ngpb_ctx* ctx, int64_t n_atoms, const double* xyz, const do
                      const int32_t* owner_leaf);
 /* Dirichlet node list; values may be NULL (then ngpb_dirichlet_values must follow). */
 int ngpb_set_dirichlet(ngpb_ctx* ctx, int64_t n_bnd, const int32_t* nodes, const double* values);
-/* Device-computed Dirichlet data: 1 zero, 2 Coulomb sum over the charges, 3 Born sphere. */
+/* Device-computed Dirichlet data: 1 zero, 2 Coulomb sum over the charges, 3 Born sphere.
+   The values depend on the charges and the physics: after ngpb_set_charges or
+   ngpb_set_physics, ngpb_assemble recomputes them for the same bc_type. Values passed
+   to ngpb_set_dirichlet are used as given. */
 int ngpb_dirichlet_values(ngpb_ctx* ctx, int bc_type);
 
 /* ---- assembly / solve / energy ---------------------------------------------------- */
@@ -208,7 +211,11 @@ int ngpb_dirichlet_values(ngpb_ctx* ctx, int bc_type);
 int ngpb_assemble(ngpb_ctx* ctx);
 int ngpb_system_size(ngpb_ctx* ctx, int64_t* n, int64_t* nnz);
 int ngpb_export_csr(ngpb_ctx* ctx, int32_t* rowptr, int32_t* col, double* val, double* rhs);
-/* Load an external CSR system (0-based, columns ascending, n+1 row pointers). */
+/* Load an external CSR system (0-based, columns ascending, n+1 row pointers).
+   NGPB_E_NUMERIC for NaN/Inf entries in val or rhs; ngpb_solve also returns
+   NGPB_E_NUMERIC for a zero or negative diagonal entry. The next ngpb_assemble
+   rebuilds the mesh's system from scratch. ngpb_energy, ngpb_phi_on_atoms and
+   ngpb_pot_field need a potential with one value per mesh node (n == n_nodes). */
 int ngpb_set_csr(ngpb_ctx* ctx, int64_t n, int64_t nnz, const int32_t* rowptr, const int32_t* col,
                  const double* val, const double* rhs);
 /* LIS-style option string, e.g. "-i cg -p jacobi -tol 1e-10 -maxiter 100000". */

@@ -128,3 +128,33 @@ def test_context_in_batch_is_read_only():
     q.ctx.solve(native.PREC_FP64, tol=TOL, want_phi=False)  # usable again
     q.ctx.close()
     b.close()
+
+
+def test_remove_running_system_mid_batch():
+    """A system taken out of the batch while it is still iterating leaves every other
+    system's result bit-identical to its result alone; the removed context can be
+    solved again on its own."""
+    probs = list(systems())
+    ref = alone()
+    b = native.Batch(0, 8, tol=TOL, maxiter=20000)
+    ctxs = [prepare(p).ctx for p in probs]
+    slots = [b.add(c) for c in ctxs]
+    b.run(3)
+    victim = 1
+    done, st = b.status(slots[victim])
+    assert not done and st["iterations"] > 0, st
+    b.remove(slots[victim])
+    b.solve_all()
+    for k, s in enumerate(slots):
+        if k == victim:
+            continue
+        done, st = b.status(s)
+        assert done and st["status"] == 0
+        b.remove(s)
+        assert np.array_equal(ctxs[k].get_phi(), ref[k][0]), probs[k].name
+        assert ctxs[k].energy()["polar"] == ref[k][1]["polar"]
+    _, st = ctxs[victim].solve(native.PREC_MIXED, tol=TOL, want_phi=False)
+    assert st["status"] == 0
+    for c in ctxs:
+        c.close()
+    b.close()

@@ -139,3 +139,136 @@ def test_calls_out_of_order_fail_with_state_error():
         assert st["status"] == 0
     finally:
         ctx.close()
+
+
+E_NUMERIC = 5
+E_NOMEM = 6
+
+
+def test_numeric_errors_in_the_operator():
+    """NaN/Inf in a loaded CSR is refused at load time; a zero or negative diagonal is
+    refused by both solvers and by the batch (NGPB_E_NUMERIC); the context stays usable."""
+    P = W.three_spheres()
+    print("[edge case] numeric errors (NaN entry, non-positive diagonal)", flush=True)
+    R = O.full_pipeline(P, tol=1e-12)
+    S = R["system"]
+    ctx = native.Context(0)
+    try:
+        for bad in (np.nan, np.inf):
+            val = S.val.copy()
+            val[len(val) // 2] = bad
+            with pytest.raises(native.NGPBError) as ei:
+                ctx.set_csr(S.rowptr, S.col, val, S.rhs)
+            assert ei.value.code == E_NUMERIC
+        rhs = S.rhs.copy()
+        rhs[3] = np.nan
+        with pytest.raises(native.NGPBError) as ei:
+            ctx.set_csr(S.rowptr, S.col, S.val, rhs)
+        assert ei.value.code == E_NUMERIC
+        row = S.n // 3
+        diag = S.rowptr[row] + int(np.nonzero(S.col[S.rowptr[row]:S.rowptr[row + 1]] == row)[0][0])
+        for d in (0.0, -1.0):
+            val = S.val.copy()
+            val[diag] = d
+            ctx.set_csr(S.rowptr, S.col, val, S.rhs)
+            for prec in (native.PREC_FP64, native.PREC_MIXED):
+                with pytest.raises(native.NGPBError) as ei:
+                    ctx.solve(prec, tol=1e-10)
+                assert ei.value.code == E_NUMERIC, (d, prec)
+            b = native.Batch(0, 2)
+            with pytest.raises(native.NGPBError) as ei:
+                b.add(ctx)
+            assert ei.value.code == E_NUMERIC
+            b.close()
+        ctx.set_csr(S.rowptr, S.col, S.val, S.rhs)
+        phi, st = ctx.solve(native.PREC_FP64, tol=1e-12)
+        assert st["status"] == 0
+        assert np.linalg.norm(phi - R["phi"]) <= 1e-10 * np.linalg.norm(R["phi"])
+    finally:
+        ctx.close()
+
+
+def test_device_allocation_failure_is_reported():
+    """A request beyond the device memory (2 x 512 GiB) fails with NGPB_E_NOMEM and the
+    context goes on working."""
+    print("[edge case] allocation beyond device memory", flush=True)
+    P = W.three_spheres()
+    ctx = native.Context(0)
+    try:
+        with pytest.raises(native.NGPBError) as ei:
+            ctx.time_copy(1 << 39, reps=1)
+        assert ei.value.code == E_NOMEM
+        assert ctx.time_copy(1 << 20, reps=2) > 0
+        pb = PoissonBoltzmann(P, ctx=ctx)
+        pb.create_markers()
+        pb.create_density_map()
+        pb.assemple_system_matrix()
+        _, st = ctx.solve(native.PREC_MIXED)
+        assert st["status"] == 0
+    finally:
+        ctx.close()
+
+
+def test_set_csr_then_assemble_rebuilds_the_mesh_system():
+    """An external CSR replaces the mesh's system; the next assembly rebuilds the
+    mesh's pattern and values (bit-identical to the first assembly), and the energy
+    refuses a potential that is not one value per mesh node."""
+    P = W.three_spheres()
+    print("[edge case] set_csr between assemblies", flush=True)
+    bv = O.dirichlet_values(P)
+    pb = gpu_pipeline(P, bv)
+    first = pb.ctx.export_csr()
+    # a smaller, unrelated system: 1D Laplacian
+    n = 50
+    rows, cols, vals = [], [], []
+    for i in range(n):
+        for j, v in ((i - 1, -1.0), (i, 2.5), (i + 1, -1.0)):
+            if 0 <= j < n:
+                rows.append(i), cols.append(j), vals.append(v)
+    rp = np.searchsorted(np.array(rows), np.arange(n + 1)).astype(np.int32)
+    pb.ctx.set_csr(rp, np.array(cols, np.int32), np.array(vals), np.ones(n))
+    phi, st = pb.ctx.solve(native.PREC_FP64, tol=1e-12)
+    assert st["status"] == 0 and phi.shape == (n,)
+    with pytest.raises(native.NGPBError) as ei:
+        pb.ctx.energy()
+    assert ei.value.code == E_STATE
+    with pytest.raises(native.NGPBError) as ei:
+        pb.ctx.phi_on_atoms(P.atoms.n)
+    assert ei.value.code == E_STATE
+    pb.ctx.assemble()
+    again = pb.ctx.export_csr()
+    for a, b in zip(first, again):
+        assert np.array_equal(a, b)
+    _, st = pb.ctx.solve(native.PREC_FP64, tol=1e-10)
+    assert st["status"] == 0
+    pb.ctx.energy()
+    pb.close()
+
+
+@pytest.mark.parametrize("what", ["charges", "physics"])
+def test_computed_boundary_values_follow_charges_and_physics(what):
+    """Coulomb (bc 2) boundary values computed on the device are recomputed at the next
+    assembly when the charges or the physics change: the rhs equals a fresh pipeline's."""
+    P = W.three_spheres(bc_type=2)
+    print(f"[edge case] boundary values after a change of {what}", flush=True)
+    pb = gpu_pipeline(P, None)
+    before = pb.ctx.export_csr()[3]
+    a = P.atoms
+    if what == "charges":
+        q2 = a.q * np.array([-2.0, 0.5, 3.0])
+        pb.ctx.set_charges(a.xyz, q2, P.owner)
+        P2 = dataclasses.replace(P, atoms=W.Atoms(a.xyz, a.r, q2))
+    else:
+        ph2 = W.Physics(e_in=4.0, e_out=60.0, ionic_strength=0.05)
+        pb.ctx.set_physics(ph2)
+        pb.ctx.create_markers()
+        P2 = dataclasses.replace(P, physics=ph2)
+    pb.ctx.assemble()
+    got = pb.ctx.export_csr()[3]
+    fresh = gpu_pipeline(P2, None)
+    want = fresh.ctx.export_csr()[3]
+    b = P.mesh.bnd_nodes
+    assert np.array_equal(got, want)
+    assert not np.array_equal(got[b], before[b])
+    fresh.close()
+    pb.close()

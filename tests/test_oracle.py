@@ -16,6 +16,7 @@ import pytest
 
 import oracle as O
 from cudanextgenpb_amd import workload as W
+from cases import hanging_charge_problem
 
 GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
 BORN_EXACT = -68.654008445
@@ -152,6 +153,40 @@ def test_rhs_conserves_charge_and_weights():
     assert r.max() == pytest.approx(4 * np.pi, rel=1e-12)
 
 
+def test_rhs_moments_through_hanging_corners():
+    """Charges inside leaves with hanging corners: the projected rhs keeps the total
+    charge (sum rhs = 4 pi sum q) and its first moment (sum_i rhs_i x_i = 4 pi sum q x_a),
+    because trilinear weights reproduce linear functions and a hanging corner is the
+    mean of its parents. Dropping the 1/np split or sending it to one parent fails."""
+    P, pick = hanging_charge_problem()
+    assert (P.mesh.leaf_corner[pick] < 0).any(axis=1).all()
+    m = P.mesh
+    n0 = P.atoms.n - pick.size
+    for a in range(n0, P.atoms.n):  # one charge at a time, then all together
+        q = np.zeros(P.atoms.n)
+        q[a] = P.atoms.q[a]
+        Q = dataclasses.replace(P, atoms=W.Atoms(P.atoms.xyz, P.atoms.r, q))
+        r = O.rhs(Q)
+        assert r.sum() == pytest.approx(4 * np.pi * q[a], rel=1e-13)
+        mom = (r[:, None] * m.node_xyz).sum(0)
+        np.testing.assert_allclose(mom, 4 * np.pi * q[a] * P.atoms.xyz[a], rtol=0, atol=1e-12 * np.abs(mom).max())
+        # every node that receives charge is a regular corner or a parent of a hanging
+        # corner of the owner leaf
+        lc = m.leaf_corner[P.owner[a]]
+        allowed = set(lc[lc >= 0].tolist())
+        for c in lc[lc < 0]:
+            h = -1 - c
+            allowed |= set(m.hang_par[h, :m.hang_np[h]].tolist())
+        assert set(np.nonzero(r)[0].tolist()) <= allowed
+        assert any(m.hang_par[-1 - c, :m.hang_np[-1 - c]].tolist() for c in lc[lc < 0])
+    r = O.rhs(P)
+    q = P.atoms.q
+    assert r.sum() == pytest.approx(4 * np.pi * q.sum(), abs=1e-12 * 4 * np.pi * np.abs(q).sum())
+    mom = (r[:, None] * m.node_xyz).sum(0)
+    want = 4 * np.pi * (q[:, None] * P.atoms.xyz).sum(0)
+    np.testing.assert_allclose(mom, want, rtol=0, atol=1e-11 * 4 * np.pi * np.abs(q).sum() * np.abs(m.node_xyz).max())
+
+
 def test_phi_on_atoms_reproduces_linear_fields():
     # trilinear interpolation in the owner leaf is exact for a linear potential, also
     # through hanging corners (the mean of the parents is the midpoint value)
@@ -262,6 +297,47 @@ def test_zero_row_sums_without_salt():
     S, _, _ = _pipeline_system(P, bnd_vals=np.zeros(0))
     row = O.spmv(S, np.ones(S.n))
     assert np.abs(row).max() <= 1e-14 * np.abs(S.val).max()
+
+
+def test_lumped_mass_moments_at_hanging_corners():
+    """kappa^2 mass through hanging corners (src/pb_class.cpp:2100). With eps uniform and
+    no Dirichlet rows the flux part annihilates constants, so A.1 is the lumped mass
+    vector m. Over a mesh with hanging nodes in the salt region:
+    * reaction uniform: sum m = reac * box volume and sum m_i x_i = reac * box volume *
+      box centre (each leaf's V/8 per corner, a hanging corner's share split 1/np over
+      parents whose mean is its position);
+    * reaction linear in x: sum m = integral of reac over the box (a hanging corner's
+      share is the mean of its parents' values, exact for linear fields).
+    Dropping the 1/np, or giving the share to one parent, fails each of these."""
+    P = W.globular(n_atoms=60, ball_radius=6.0, min_sep=2.0, h_fine=0.5, levels=2, fine_band=1.5, mid_band=4.0)
+    m = P.mesh
+    assert m.n_hang > 1000
+    P.mesh.bnd_nodes = np.zeros(0, np.int32)
+    _, _, lm, _ = O.markers(P)
+    frac, _, _ = O.fractions(P, lm)
+    lo = m.origin
+    hi = m.origin + m.n * m.hf
+    vol = float(np.prod(hi - lo))
+    centre = 0.5 * (lo + hi)
+    eps = np.full(m.n_nodes, 0.1427)
+    reac0 = 2.2e-3
+    S = O.assemble(P, eps, np.full(m.n_nodes, reac0), frac, np.zeros(m.n_nodes), np.zeros(0))
+    mass = O.spmv(S, np.ones(S.n))
+    assert mass.min() > 0
+    assert mass.sum() == pytest.approx(reac0 * vol, rel=1e-12)
+    np.testing.assert_allclose((mass[:, None] * m.node_xyz).sum(0), reac0 * vol * centre, rtol=0,
+                               atol=1e-11 * reac0 * vol * np.abs(hi).max())
+    # hanging corners lie in the salt region here: their parents' rows carry mass
+    hang_rows = np.unique(m.hang_par[m.hang_par >= 0])
+    assert hang_rows.size > 500
+    # reaction linear in x
+    alpha, beta = 3e-3, np.array([2e-5, -1e-5, 3e-5])
+    reac = alpha + (m.node_xyz - lo) @ beta
+    assert reac.min() > 0
+    S2 = O.assemble(P, eps, reac, frac, np.zeros(m.n_nodes), np.zeros(0))
+    mass2 = O.spmv(S2, np.ones(S2.n))
+    integral = vol * (alpha + (centre - lo) @ beta)
+    assert mass2.sum() == pytest.approx(integral, rel=1e-12)
 
 
 def test_dirichlet_rows_and_elimination():
