@@ -27,6 +27,7 @@ HIP_SOURCES = [
     "ngpb_markers.hip",
     "ngpb_assemble.hip",
     "ngpb_krylov.hip",
+    "ngpb_lis.hip",
     "ngpb_mixed.hip",
     "ngpb_energy.hip",
     "ngpb_batch.hip",

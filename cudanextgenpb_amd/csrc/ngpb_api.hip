@@ -82,6 +82,8 @@ int64_t ngpb_ctx::device_bytes() const {
        sell_pk_lp.bytes() + sell_pk_dict.bytes() + sell_pk_hdr.bytes() + sell_cval.bytes() + sell_list_delta.bytes() +
        sell_list_packed.bytes() + sell_list_offset.bytes();
   b += x.bytes() + r.bytes() + z.bytes() + p.bytes() + q.bytes() + dinv.bytes();
+  b += lis_rt.bytes() + lis_u.bytes() + lis_v.bytes() + lis_ph.bytes() + lis_uh.bytes() + lis_t.bytes() +
+       diag_pos.bytes() + lev_rows_f.bytes() + lev_rows_b.bytes();
   b += xs.bytes() + rs.bytes() + ps.bytes() + qs.bytes() + ks.bytes();
   b += red.partials.bytes() + scan_tmp.bytes();
   for (const auto& t : tmp) b += t.bytes();
@@ -570,7 +572,10 @@ int ngpb_assemble(ngpb_ctx* c) {
     need(c->have_bnd_values, "Dirichlet values missing (ngpb_dirichlet_values)");
     if (!c->have_frac) ngpb::run_fractions(c);
     // after ngpb_set_csr the CSR arrays no longer hold the mesh's pattern
-    if (!c->have_symbolic || c->external_csr) ngpb::run_symbolic(c);
+    if (!c->have_symbolic || c->external_csr) {
+      ngpb::run_symbolic(c);
+      c->have_levels = false;
+    }
     ngpb::run_numeric(c);
     c->external_csr = false;
     c->have_phi = false;
@@ -625,6 +630,7 @@ int ngpb_set_csr(ngpb_ctx* c, int64_t n, int64_t nnz, const int32_t* rowptr, con
     // ngpb_assemble rebuilds them
     c->have_symbolic = false;
     c->external_csr = true;
+    c->have_levels = false;
     c->have_compact = c->have_phi = false;
   });
 }
@@ -636,6 +642,9 @@ int ngpb_parse_lis_options(const char* options, ngpb_solve_opts* o) {
   o->maxiter = 1000;  // LIS default
   o->drop = 1e-3;
   o->check_every = 32;
+  o->solver = NGPB_SOLVER_CG;     // LIS default -i cg
+  o->precon = NGPB_PRECON_NONE;   // LIS default -p none
+  o->omega = 1.0;                 // LIS default -ssor_omega
   std::istringstream is(options);
   std::vector<std::string> tok;
   for (std::string t; is >> t;) tok.push_back(t);
@@ -648,9 +657,21 @@ int ngpb_parse_lis_options(const char* options, ngpb_solve_opts* o) {
     };
     std::string v;
     if (k == "-i") {
-      if (!arg(&v) || (v != "cg" && v != "1")) return NGPB_E_ARG;
+      if (!arg(&v)) return NGPB_E_ARG;
+      if (v == "cg" || v == "1") o->solver = NGPB_SOLVER_CG;
+      else if (v == "cgs" || v == "3") o->solver = NGPB_SOLVER_CGS;
+      else if (v == "bicgstab" || v == "4") o->solver = NGPB_SOLVER_BICGSTAB;
+      else return NGPB_E_ARG;
     } else if (k == "-p") {
-      if (!arg(&v) || (v != "jacobi" && v != "1")) return NGPB_E_ARG;
+      if (!arg(&v)) return NGPB_E_ARG;
+      if (v == "none" || v == "0") o->precon = NGPB_PRECON_NONE;
+      else if (v == "jacobi" || v == "1") o->precon = NGPB_PRECON_JACOBI;
+      else if (v == "ssor" || v == "3") o->precon = NGPB_PRECON_SSOR;
+      else return NGPB_E_ARG;
+    } else if (k == "-ssor_omega") {
+      if (!arg(&v)) return NGPB_E_ARG;
+      o->omega = std::atof(v.c_str());
+      if (!(o->omega > 0.0 && o->omega < 2.0)) return NGPB_E_ARG;
     } else if (k == "-tol") {
       if (!arg(&v)) return NGPB_E_ARG;
       o->tol = std::atof(v.c_str());
@@ -680,9 +701,17 @@ int ngpb_solve(ngpb_ctx* c, const ngpb_solve_opts* o, double* phi_out, ngpb_solv
     NGPB_REQUIRE(o, NGPB_E_ARG, "NULL options");
     NGPB_REQUIRE(o->tol > 0 && o->maxiter > 0, NGPB_E_ARG, "tol and maxiter must be positive");
     need(c->have_system, "no assembled system");
-    if (o->precision == NGPB_PREC_FP64) ngpb::solve_fp64(c, *o, s);
-    else if (o->precision == NGPB_PREC_MIXED) ngpb::solve_mixed(c, *o, s);
-    else throw Error(NGPB_E_ARG, "unknown precision");
+    ngpb::check_lis_options(*o);
+    const bool cg_jacobi = o->solver == NGPB_SOLVER_CG && o->precon == NGPB_PRECON_JACOBI;
+    if (o->precision == NGPB_PREC_FP64) {
+      if (cg_jacobi) ngpb::solve_fp64(c, *o, s);
+      else ngpb::solve_lis(c, *o, s);
+    } else if (o->precision == NGPB_PREC_MIXED) {
+      NGPB_REQUIRE(cg_jacobi, NGPB_E_ARG, "mixed precision runs CG with Jacobi scaling only");
+      ngpb::solve_mixed(c, *o, s);
+    } else {
+      throw Error(NGPB_E_ARG, "unknown precision");
+    }
     if (phi_out) {
       c->x.download(phi_out, (size_t)c->n, c->stream);
       NGPB_HIP(hipStreamSynchronize(c->stream));
@@ -694,6 +723,24 @@ int ngpb_solve(ngpb_ctx* c, const ngpb_solve_opts* o, double* phi_out, ngpb_solv
     return s.status;
   }
   return NGPB_OK;
+}
+
+int ngpb_precond_apply(ngpb_ctx* c, int precon, double omega, const double* r, double* z) {
+  return guarded(c, [&] {
+    not_in_batch(c);
+    NGPB_REQUIRE(r && z, NGPB_E_ARG, "NULL vector");
+    NGPB_REQUIRE(precon == NGPB_PRECON_JACOBI || precon == NGPB_PRECON_SSOR, NGPB_E_ARG,
+                 "preconditioner must be Jacobi or SSOR");
+    NGPB_REQUIRE(precon != NGPB_PRECON_SSOR || (omega > 0.0 && omega < 2.0), NGPB_E_ARG,
+                 "SSOR relaxation must lie in (0, 2)");
+    need(c->have_system, "no assembled system");
+    ngpb::DBuf<double> dr, dz;
+    dr.upload(r, (size_t)c->n, c->stream);
+    dz.resize((size_t)c->n);
+    ngpb::precond_apply(c, precon, omega, dr.p, dz.p);
+    dz.download(z, (size_t)c->n, c->stream);
+    NGPB_HIP(hipStreamSynchronize(c->stream));
+  });
 }
 
 int ngpb_get_phi(ngpb_ctx* c, double* phi) {

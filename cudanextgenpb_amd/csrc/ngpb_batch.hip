@@ -563,6 +563,8 @@ int ngpb_batch_set_options(ngpb_batch* b, const ngpb_solve_opts* o, int inner_pe
   return ngpb::bguard(b, [&] {
     NGPB_REQUIRE(o, NGPB_E_ARG, "null options");
     NGPB_REQUIRE(o->tol > 0.0 && o->maxiter > 0, NGPB_E_ARG, "tol and maxiter must be positive");
+    NGPB_REQUIRE(o->solver == NGPB_SOLVER_CG && o->precon == NGPB_PRECON_JACOBI, NGPB_E_ARG,
+                 "a batch runs mixed-precision CG with Jacobi scaling only");
     NGPB_REQUIRE(inner_per_replay >= 1 && inner_per_replay <= 256, NGPB_E_ARG, "inner_per_replay must be in [1, 256]");
     b->tol = o->tol;
     b->drop = (o->drop > 0.0 && o->drop < 1.0) ? o->drop : 1e-3;

@@ -267,6 +267,13 @@ struct ngpb_ctx {
 
   // solver state
   ngpb::DBuf<double> x, r, z, p, q, dinv;
+  // other LIS solvers (ngpb_lis.hip): work vectors and the SSOR level sets, built
+  // once per sparsity pattern
+  ngpb::DBuf<double> lis_rt, lis_u, lis_v, lis_ph, lis_uh, lis_t;
+  bool have_levels = false;
+  ngpb::DBuf<int32_t> diag_pos;              // position of each row's diagonal entry
+  ngpb::DBuf<int32_t> lev_rows_f, lev_rows_b;  // rows by forward / backward level
+  std::vector<int64_t> lev_ptr_f, lev_ptr_b;   // level offsets (host)
   ngpb::DBuf<float> xs, rs, ps, qs;
   ngpb::DBuf<ngpb::KrylovScalars> ks;
   ngpb::KrylovScalars* ks_host = nullptr;  // pinned mirror
@@ -327,6 +334,15 @@ OctreeSizes run_build_octree(ngpb_ctx* c, const double* origin, double hf, const
 unsigned vec_grid(int64_t n);
 void check_operator(ngpb_ctx* c);
 void ensure_reduce(ngpb_ctx* c, int64_t blocks);
+// FP64 CSR SpMV y = A x that also writes per-block (x, y) partials and returns at once
+// once the stop flag of c->ks is raised; grid size of its launches
+void spmv_fp64_dot(ngpb_ctx* c, const double* x, double* y, double* partials);
+unsigned spmv_fp64_grid(ngpb_ctx* c);
+void check_operator(ngpb_ctx* c);
+// other LIS solvers (ngpb_lis.hip)
+void check_lis_options(const ngpb_solve_opts& o);
+void solve_lis(ngpb_ctx* c, const ngpb_solve_opts& o, ngpb_solve_stats& st);
+void precond_apply(ngpb_ctx* c, int precon, double omega, const double* r, double* z);
 void spmv_fp64(ngpb_ctx* c, const double* x, double* y);
 void solve_fp64(ngpb_ctx* c, const ngpb_solve_opts& o, ngpb_solve_stats& st);
 double spmv_fp64_bytes(const ngpb_ctx* c);

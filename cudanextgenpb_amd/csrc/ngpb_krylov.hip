@@ -324,6 +324,12 @@ void ensure_reduce(ngpb_ctx* c, int64_t blocks) {
   if (!c->ks_host) NGPB_HIP(hipHostMalloc((void**)&c->ks_host, sizeof(KrylovScalars), hipHostMallocDefault));
 }
 
+void spmv_fp64_dot(ngpb_ctx* c, const double* x, double* y, double* partials) {
+  launch_spmv64<true>(c, spmv64_grid(c), x, y, partials);
+}
+
+unsigned spmv_fp64_grid(ngpb_ctx* c) { return spmv64_grid(c); }
+
 double spmv_fp64_bytes(const ngpb_ctx* c) {
   // nnz (value + column) + row pointers + x read + y written
   return (double)c->nnz * (8.0 + 4.0) + (double)(c->n + 1) * 4.0 + (double)c->n * 16.0;

@@ -35,8 +35,13 @@ class Physics(ctypes.Structure):
     _fields_ = [("e_in", _D), ("e_out", _D), ("T", _D), ("ionic_strength", _D), ("calc_energy", _I)]
 
 
+SOLVER_CG, SOLVER_CGS, SOLVER_BICGSTAB = 0, 1, 2
+PRECON_JACOBI, PRECON_NONE, PRECON_SSOR = 0, 1, 2
+
+
 class SolveOpts(ctypes.Structure):
-    _fields_ = [("precision", _I), ("tol", _D), ("maxiter", _I), ("drop", _D), ("check_every", _I)]
+    _fields_ = [("precision", _I), ("tol", _D), ("maxiter", _I), ("drop", _D), ("check_every", _I),
+                ("solver", _I), ("precon", _I), ("omega", _D)]
 
 
 class SolveStats(ctypes.Structure):
@@ -92,6 +97,7 @@ SIGNATURES = {
     "ngpb_set_csr": (_I, [_P, _I64, _I64, _P, _P, _P, _P]),
     "ngpb_parse_lis_options": (_I, [ctypes.c_char_p, _P]),
     "ngpb_solve": (_I, [_P, _P, _P, _P]),
+    "ngpb_precond_apply": (_I, [_P, _I, _D, _P, _P]),
     "ngpb_get_phi": (_I, [_P, _P]),
     "ngpb_set_phi": (_I, [_P, _P]),
     "ngpb_spmv": (_I, [_P, _I, _P, _P]),
@@ -338,8 +344,8 @@ class Context:
         self._size()
 
     def solve(self, precision=PREC_FP64, tol=1e-10, maxiter=100000, drop=1e-3, check_every=32, want_phi=True,
-              raise_on_noconv=True):
-        o = SolveOpts(precision, tol, maxiter, drop, check_every)
+              raise_on_noconv=True, solver=SOLVER_CG, precon=PRECON_JACOBI, omega=1.0):
+        o = SolveOpts(precision, tol, maxiter, drop, check_every, solver, precon, omega)
         st = SolveStats()
         # no system yet: the library rejects the call before writing phi
         phi = np.zeros(self.n or 0) if want_phi else None
@@ -347,6 +353,13 @@ class Context:
         if rc != NGPB_OK and not (rc == 4 and not raise_on_noconv):
             self._check(rc, "solve")
         return phi, st.as_dict()
+
+    def precond_apply(self, r, precon=PRECON_SSOR, omega=1.0):
+        """z = M^-1 r with the resident operator's Jacobi or SSOR preconditioner."""
+        z = np.zeros(self.n)
+        self._check(lib().ngpb_precond_apply(self._h, int(precon), float(omega), _p(_c(r, np.float64)), _p(z)),
+                    "precond_apply")
+        return z
 
     def get_phi(self):
         phi = np.zeros(self.n)

@@ -97,7 +97,8 @@ class PoissonBoltzmann:
                                        drop: float = 1e-3, check_every: int = 32):
         o = native.parse_lis_options(options)
         prec = o.precision if precision is None else precision
-        self.phi, self.stats = self.ctx.solve(prec, o.tol, o.maxiter, drop, check_every)
+        self.phi, self.stats = self.ctx.solve(prec, o.tol, o.maxiter, drop, check_every, solver=o.solver,
+                                              precon=o.precon, omega=o.omega)
         return self.phi
 
     # write_potential_on_atoms_fast (src/pb_class.cpp:2333-2417): phi interpolated at

@@ -72,13 +72,20 @@ typedef struct {
   int calc_energy; /* 0 none, 1 polarization only, 2 polarization + direct ionic */
 } ngpb_physics;
 
+/* Krylov method and preconditioner, as LIS's "-i" and "-p" options. */
+enum ngpb_solver { NGPB_SOLVER_CG = 0, NGPB_SOLVER_CGS = 1, NGPB_SOLVER_BICGSTAB = 2 };
+enum ngpb_precon { NGPB_PRECON_JACOBI = 0, NGPB_PRECON_NONE = 1, NGPB_PRECON_SSOR = 2 };
+
 typedef struct {
   int precision;     /* enum ngpb_precision */
   double tol;        /* stop when ||b - A x||_2 <= tol * ||b||_2 (x0 = 0) */
-  int maxiter;       /* limit on matrix-vector products (FP64) or inner iterations (mixed) */
+  int maxiter;       /* limit on iterations (FP64) or inner iterations (mixed) */
   double drop;       /* mixed: FP64 residual replacement once the inner residual has
                         dropped by this factor since the last replacement (default 1e-3) */
   int check_every;   /* iterations launched between host polls of the stop flag (>= 1) */
+  int solver;        /* enum ngpb_solver (FP64 only; MIXED runs CG) */
+  int precon;        /* enum ngpb_precon (FP64 only; MIXED uses symmetric Jacobi scaling) */
+  double omega;      /* SSOR relaxation in (0, 2); <= 0 means LIS's default 1.0 */
 } ngpb_solve_opts;
 
 typedef struct {
@@ -218,9 +225,24 @@ int ngpb_export_csr(ngpb_ctx* ctx, int32_t* rowptr, int32_t* col, double* val, d
    ngpb_pot_field need a potential with one value per mesh node (n == n_nodes). */
 int ngpb_set_csr(ngpb_ctx* ctx, int64_t n, int64_t nnz, const int32_t* rowptr, const int32_t* col,
                  const double* val, const double* rhs);
-/* LIS-style option string, e.g. "-i cg -p jacobi -tol 1e-10 -maxiter 100000". */
+/* LIS-style option string, e.g. "-i cg -p jacobi -tol 1e-10 -maxiter 100000" or the
+   reference's default "-p ssor -ssor_omega 0.51 -i cgs -tol 1.e-6" (src/pb_class.cpp:970).
+   Understood: -i cg|cgs|bicgstab (or LIS's numbers 1|3|4), -p none|jacobi|ssor (0|1|3),
+   -ssor_omega, -tol, -maxiter, -f/-precision; -print -conv_cond -initx_zeros -scale
+   -tol_w take a value and have no effect. Anything else (other LIS solvers or
+   preconditioners such as -p ilu, unknown options) returns NGPB_E_ARG and leaves *out
+   holding the LIS defaults for the options read so far: a caller should then fall
+   back to LIS (see INTEGRATION.md for the reference-side branch). */
 int ngpb_parse_lis_options(const char* options, ngpb_solve_opts* out);
+/* Solves the resident system. FP64: CG with Jacobi runs the fused CG kernels; every
+   other solver/preconditioner pair runs the general Krylov kernels with
+   level-scheduled SSOR sweeps (lis_solve with those options, src/pb_class.cpp:2294-2315).
+   MIXED needs solver CG and precon Jacobi (NGPB_E_ARG otherwise). */
 int ngpb_solve(ngpb_ctx* ctx, const ngpb_solve_opts* opts, double* phi_out, ngpb_solve_stats* stats);
+/* z = M^-1 r for the resident operator's Jacobi or SSOR preconditioner (relaxation
+   omega); host vectors of n entries. The preconditioner application inside LIS's
+   solvers; no direct counterpart in the reference's own code. */
+int ngpb_precond_apply(ngpb_ctx* ctx, int precon, double omega, const double* r, double* z);
 int ngpb_get_phi(ngpb_ctx* ctx, double* phi);
 int ngpb_set_phi(ngpb_ctx* ctx, const double* phi);
 /* y = A x with the resident operator: precision FP64 (CSR) or MIXED (scaled FP32 form,

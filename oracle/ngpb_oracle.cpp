@@ -13,6 +13,9 @@
 //   orc_assemble       src/pb_class.cpp:2052-2159 + bimpp bim3a_laplacian_frac /
 //                      bim3a_reaction_frac / bim3a_dirichlet_bc (conventions below)
 //   orc_pcg            src/pb_class.cpp:2243-2329 + LIS 2.1.6 CG with Jacobi (-i cg -p jacobi)
+//   orc_krylov         the other option strings the reference passes to LIS (default
+//                      -i cgs -p ssor, src/pb_class.cpp:970): CG / CGS / BiCGSTAB with
+//                      no, Jacobi or SSOR preconditioning (published algorithms, below)
 //   orc_energy         src/pb_class.cpp:2534-2788 (helpers), 2790-3063 (energy)
 //   orc_phi_on_atoms   src/pb_class.cpp:2333-2368 (write_potential_on_atoms_fast)
 //   orc_pot_field      src/pb_class.cpp:3731-5060 (pot_field_fast / pot_field)
@@ -22,7 +25,8 @@
 //   exhaustive rules; fractions: closed form t = c +- sqrt(r^2-d^2); locate: brute
 //   force; rhs: weights sum to one, sum rhs = 4 pi sum q; assembly: uniform 7-point
 //   closed form, symmetry, zero interior row sums at kappa=0, exact reproduction of
-//   linear fields; pcg: dense direct solves of small systems; energy: Born-sphere
+//   linear fields; pcg and krylov: dense direct solves of small systems, SSOR sweeps
+//   against M z = r with M a product of sparse factors; energy: Born-sphere
 //   analytic energy and the Gauss flux-charge check; phi on atoms: exact
 //   reproduction of linear fields (trilinear interpolation); pot_field: every field
 //   column is minus the gradient of its potential column with respect to the atom
@@ -582,6 +586,192 @@ done:
   omp_set_num_threads(saved);
 #endif
   return status;
+}
+
+// ---------------------------------------------------------------- other LIS solvers
+// The reference's default solver options are "-p ssor -ssor_omega 0.51 -i cgs
+// -tol 1.e-6" (src/pb_class.cpp:970, data/options.prm:174), handed to LIS 2.1.6
+// through lis_solver_set_option (src/pb_class.cpp:2294-2315). LIS is not vendored in
+// the reference; what follows restates the published algorithms it names:
+//   SSOR preconditioner (Saad, Iterative Methods for Sparse Linear Systems, 2nd ed.,
+//   sec. 10.2): with A = L + D + U (strict lower, diagonal, strict upper),
+//   M = (D + w L) D^-1 (D + w U). Applying M^-1 is a forward sweep
+//   y_i = (r_i - w sum_{j<i} a_ij y_j) / d_i followed by a backward sweep
+//   z_i = y_i - w (sum_{j>i} a_ij z_j) / d_i. The variant (D/w + L)(D/w)^-1(D/w + U)
+//   differs from it by the scalar 1/w, which leaves every Krylov iterate unchanged.
+//   CGS (Sonneveld 1989) and BiCGSTAB (van der Vorst 1992) in the preconditioned forms
+//   of Barrett et al., Templates for the Solution of Linear Systems (1994), sec. 2.3.7
+//   and 2.3.8, with shadow residual r~ = r0 = b (x0 = 0) and the stop test
+//   ||r||_2 <= tol ||b||_2 after each iteration (BiCGSTAB also on s at mid-step).
+// Arithmetic order (the device follows it exactly in the sweeps): within a row the
+// entries are visited in ascending column order, t = t - (w * a_ij) * y_j in the
+// forward sweep, t = t + a_ij * z_j in the backward sweep, d_i^-1 = 1.0 / d_i.
+// Parity against LIS itself is unpinned (LIS is absent); the sweeps are pinned by
+// M z = r with M formed as a product of sparse factors, the solvers by direct solves.
+namespace {
+struct SsorSplit {
+  std::vector<int32_t> diag;  // position of the diagonal entry of each row
+  std::vector<double> dinv;
+};
+
+int ssor_split(int64_t n, const int32_t* rowptr, const int32_t* col, const double* val, SsorSplit& S) {
+  S.diag.assign(n, -1);
+  S.dinv.assign(n, 0.0);
+  for (int64_t i = 0; i < n; ++i) {
+    for (int32_t k = rowptr[i]; k < rowptr[i + 1]; ++k)
+      if (col[k] == i) S.diag[i] = k;
+    if (S.diag[i] < 0 || !(val[S.diag[i]] > 0.0)) return 5;
+    S.dinv[i] = 1.0 / val[S.diag[i]];
+  }
+  return 0;
+}
+
+void ssor_apply(int64_t n, const int32_t* rowptr, const int32_t* col, const double* val,
+                const SsorSplit& S, double w, const double* r, double* z) {
+  for (int64_t i = 0; i < n; ++i) {
+    double t = r[i];
+    for (int32_t k = rowptr[i]; k < S.diag[i]; ++k) t = t - (w * val[k]) * z[col[k]];
+    z[i] = t * S.dinv[i];
+  }
+  for (int64_t i = n - 1; i >= 0; --i) {
+    double t = 0.0;
+    for (int32_t k = S.diag[i] + 1; k < rowptr[i + 1]; ++k) t = t + val[k] * z[col[k]];
+    z[i] = z[i] - (w * t) * S.dinv[i];
+  }
+}
+
+double dot1(int64_t n, const double* a, const double* b) {
+  double s = 0.0;
+  for (int64_t i = 0; i < n; ++i) s += a[i] * b[i];
+  return s;
+}
+}  // namespace
+
+// z = M^-1 r for the SSOR preconditioner with relaxation w. Returns 5 if a diagonal
+// entry is missing or not positive.
+int orc_ssor_apply(int64_t n, const int32_t* rowptr, const int32_t* col, const double* val, double w,
+                   const double* r, double* z) {
+  SsorSplit S;
+  if (ssor_split(n, rowptr, col, val, S)) return 5;
+  ssor_apply(n, rowptr, col, val, S, w, r, z);
+  return 0;
+}
+
+// solver: 0 CG, 1 CGS, 2 BiCGSTAB; precon: 0 none, 1 Jacobi, 2 SSOR (relaxation w).
+// x0 = 0; iters = iterations done; relres = ||r||_2 / ||b||_2 of the last residual
+// tested. Returns 0 on convergence, 4 at maxiter or on a breakdown (a zero rho or
+// denominator), 5 on a missing or non-positive diagonal. Sequential.
+int orc_krylov(int64_t n, const int32_t* rowptr, const int32_t* col, const double* val,
+               const double* b, double tol, int maxiter, int solver, int precon, double w,
+               double* x, int* iters, double* relres) {
+  SsorSplit S;
+  *iters = 0;
+  *relres = 0.0;
+  if (ssor_split(n, rowptr, col, val, S)) return 5;
+  auto psolve = [&](const double* in, double* out) {
+    if (precon == 2) ssor_apply(n, rowptr, col, val, S, w, in, out);
+    else if (precon == 1) for (int64_t i = 0; i < n; ++i) out[i] = S.dinv[i] * in[i];
+    else for (int64_t i = 0; i < n; ++i) out[i] = in[i];
+  };
+  std::vector<double> r(b, b + n), rt(b, b + n), p(n, 0.0), q(n, 0.0), u(n, 0.0), v(n, 0.0),
+      ph(n, 0.0), uh(n, 0.0), t(n, 0.0), z(n, 0.0);
+  for (int64_t i = 0; i < n; ++i) x[i] = 0.0;
+  const double bnrm = std::sqrt(dot1(n, b, b));
+  if (bnrm == 0.0) return 0;
+  const double stop = tol * bnrm;
+  if (solver == 0) {  // preconditioned CG
+    psolve(r.data(), z.data());
+    p = z;
+    double rho = dot1(n, r.data(), z.data());
+    for (int it = 1; it <= maxiter; ++it) {
+      orc_spmv(n, rowptr, col, val, p.data(), q.data());
+      const double pq = dot1(n, p.data(), q.data());
+      if (pq == 0.0) return 4;
+      const double alpha = rho / pq;
+      for (int64_t i = 0; i < n; ++i) {
+        x[i] += alpha * p[i];
+        r[i] -= alpha * q[i];
+      }
+      const double rn = std::sqrt(dot1(n, r.data(), r.data()));
+      *iters = it;
+      *relres = rn / bnrm;
+      if (rn <= stop) return 0;
+      psolve(r.data(), z.data());
+      const double rho1 = dot1(n, r.data(), z.data());
+      const double beta = rho1 / rho;
+      rho = rho1;
+      for (int64_t i = 0; i < n; ++i) p[i] = z[i] + beta * p[i];
+    }
+    return 4;
+  }
+  if (solver == 1) {  // CGS
+    double rho_old = 1.0;
+    for (int it = 1; it <= maxiter; ++it) {
+      const double rho = dot1(n, rt.data(), r.data());
+      if (rho == 0.0) return 4;
+      const double beta = it == 1 ? 0.0 : rho / rho_old;
+      for (int64_t i = 0; i < n; ++i) {
+        u[i] = r[i] + beta * q[i];
+        p[i] = u[i] + beta * (q[i] + beta * p[i]);
+      }
+      psolve(p.data(), ph.data());
+      orc_spmv(n, rowptr, col, val, ph.data(), v.data());
+      const double rv = dot1(n, rt.data(), v.data());
+      if (rv == 0.0) return 4;
+      const double alpha = rho / rv;
+      for (int64_t i = 0; i < n; ++i) {
+        q[i] = u[i] - alpha * v[i];
+        t[i] = u[i] + q[i];
+      }
+      psolve(t.data(), uh.data());
+      for (int64_t i = 0; i < n; ++i) x[i] += alpha * uh[i];
+      orc_spmv(n, rowptr, col, val, uh.data(), t.data());
+      for (int64_t i = 0; i < n; ++i) r[i] -= alpha * t[i];
+      const double rn = std::sqrt(dot1(n, r.data(), r.data()));
+      *iters = it;
+      *relres = rn / bnrm;
+      if (rn <= stop) return 0;
+      rho_old = rho;
+    }
+    return 4;
+  }
+  // BiCGSTAB
+  double rho_old = 1.0, alpha = 1.0, omega = 1.0;
+  std::vector<double>& s = u;
+  std::vector<double>& sh = uh;
+  for (int it = 1; it <= maxiter; ++it) {
+    const double rho = dot1(n, rt.data(), r.data());
+    if (rho == 0.0) return 4;
+    const double beta = it == 1 ? 0.0 : (rho / rho_old) * (alpha / omega);
+    for (int64_t i = 0; i < n; ++i) p[i] = r[i] + beta * (p[i] - omega * v[i]);
+    psolve(p.data(), ph.data());
+    orc_spmv(n, rowptr, col, val, ph.data(), v.data());
+    const double rv = dot1(n, rt.data(), v.data());
+    if (rv == 0.0) return 4;
+    alpha = rho / rv;
+    for (int64_t i = 0; i < n; ++i) s[i] = r[i] - alpha * v[i];
+    const double sn = std::sqrt(dot1(n, s.data(), s.data()));
+    *iters = it;
+    if (sn <= stop) {
+      for (int64_t i = 0; i < n; ++i) x[i] += alpha * ph[i];
+      *relres = sn / bnrm;
+      return 0;
+    }
+    psolve(s.data(), sh.data());
+    orc_spmv(n, rowptr, col, val, sh.data(), t.data());
+    const double ts = dot1(n, t.data(), s.data()), tt = dot1(n, t.data(), t.data());
+    if (tt == 0.0 || ts == 0.0) return 4;
+    omega = ts / tt;
+    for (int64_t i = 0; i < n; ++i) {
+      x[i] += alpha * ph[i] + omega * sh[i];
+      r[i] = s[i] - omega * t[i];
+    }
+    const double rn = std::sqrt(dot1(n, r.data(), r.data()));
+    *relres = rn / bnrm;
+    if (rn <= stop) return 0;
+    rho_old = rho;
+  }
+  return 4;
 }
 
 // Timing helper for the CPU baseline: runs `iters` PCG iterations (no stopping

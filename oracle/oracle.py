@@ -54,6 +54,10 @@ def lib():
         L.orc_spmv.argtypes = [_I64, _P, _P, _P, _P, _P]
         L.orc_pcg.argtypes = [_I64, _P, _P, _P, _P, _D, ctypes.c_int, ctypes.c_int, _P, _P, _P]
         L.orc_pcg.restype = ctypes.c_int
+        L.orc_krylov.argtypes = [_I64, _P, _P, _P, _P, _D, ctypes.c_int, ctypes.c_int, ctypes.c_int, _D, _P, _P, _P]
+        L.orc_krylov.restype = ctypes.c_int
+        L.orc_ssor_apply.argtypes = [_I64, _P, _P, _P, _D, _P, _P]
+        L.orc_ssor_apply.restype = ctypes.c_int
         L.orc_pcg_time_iters.argtypes = [_I64, _P, _P, _P, _P, ctypes.c_int, ctypes.c_int]
         L.orc_pcg_time_iters.restype = _D
         L.orc_energy.argtypes = [_P, _P, _P, _P, _I64, _P, _I64, _P, _P, _D, _D, _D, _D, ctypes.c_int, _P]
@@ -239,6 +243,31 @@ def pcg(S, b=None, tol=1e-10, maxiter=100000, nthreads=1):
     st = lib().orc_pcg(S.n, _p(S.rowptr), _p(S.col), _p(S.val), _p(b), tol, maxiter, nthreads, _p(x),
                        _p(it), _p(rr))
     return x, int(it[0]), float(rr[0]), st
+
+
+SOLVERS = {"cg": 0, "cgs": 1, "bicgstab": 2}
+PRECONS = {"none": 0, "jacobi": 1, "ssor": 2}
+
+
+def krylov(S, solver="cgs", precon="ssor", omega=1.0, b=None, tol=1e-10, maxiter=100000):
+    """CG / CGS / BiCGSTAB with no, Jacobi or SSOR preconditioning (x0 = 0);
+    returns (x, iterations, relres, status)."""
+    b = _c(S.rhs if b is None else b, np.float64)
+    x = np.zeros(S.n)
+    it = np.zeros(1, np.int32)
+    rr = np.zeros(1)
+    st = lib().orc_krylov(S.n, _p(S.rowptr), _p(S.col), _p(S.val), _p(b), tol, maxiter, SOLVERS[solver],
+                          PRECONS[precon], omega, _p(x), _p(it), _p(rr))
+    return x, int(it[0]), float(rr[0]), st
+
+
+def ssor_apply(S, omega, r):
+    """z = M^-1 r for the SSOR preconditioner M = (D + w L) D^-1 (D + w U)."""
+    z = np.zeros(S.n)
+    st = lib().orc_ssor_apply(S.n, _p(S.rowptr), _p(S.col), _p(S.val), omega, _p(_c(r, np.float64)), _p(z))
+    if st:
+        raise ValueError("missing or non-positive diagonal")
+    return z
 
 
 def pcg_time_iters(S, iters, nthreads):

@@ -51,11 +51,29 @@ def test_constants_match_oracle():
 def test_parse_lis_options():
     o = native.parse_lis_options("-i cg -p jacobi -tol 1.e-10 -maxiter 100000 -print 2 -conv_cond 2 -tol_w 0")
     assert o.precision == native.PREC_FP64 and o.tol == 1e-10 and o.maxiter == 100000
+    assert o.solver == native.SOLVER_CG and o.precon == native.PRECON_JACOBI
     o = native.parse_lis_options("-i cg -p jacobi -f mixed")
     assert o.precision == native.PREC_MIXED and o.tol == 1e-12 and o.maxiter == 1000  # LIS defaults
-    for bad in ("-i cgs -p ssor", "-i cg -p ilu", "-tol", "-tol -1", "-bogus 3"):
+    o = native.parse_lis_options("-tol 1e-8")  # LIS defaults: -i cg -p none
+    assert o.solver == native.SOLVER_CG and o.precon == native.PRECON_NONE and o.omega == 1.0
+    o = native.parse_lis_options("-i 4 -p 3 -ssor_omega 1.2")  # LIS's numeric names
+    assert o.solver == native.SOLVER_BICGSTAB and o.precon == native.PRECON_SSOR and o.omega == 1.2
+    for bad in ("-i cg -p ilu", "-i gmres", "-p ssor -ssor_omega 2.5", "-tol", "-tol -1", "-bogus 3"):
         with pytest.raises(native.NGPBError):
             native.parse_lis_options(bad)
+
+
+def test_reference_default_options_string():
+    """The reference's default solver options (src/pb_class.cpp:970, data/options.prm:174)
+    parse to CGS with SSOR at w = 0.51 and tol 1e-6; the alternative it lists,
+    "-i cg -p ilu" (data/options.prm:177), is refused with NGPB_E_ARG so that the
+    caller falls back to LIS."""
+    o = native.parse_lis_options("-p ssor -ssor_omega 0.51 -i cgs -tol 1.e-6")
+    assert (o.solver, o.precon, o.omega, o.tol) == (native.SOLVER_CGS, native.PRECON_SSOR, 0.51, 1e-6)
+    assert o.precision == native.PREC_FP64 and o.maxiter == 1000
+    o2 = native.SolveOpts()
+    rc = native.lib().ngpb_parse_lis_options(b"-i cg -p ilu -tol 1e-6", ctypes.byref(o2))
+    assert rc == 1  # NGPB_E_ARG
 
 
 def test_create_without_device_reports_nodev():

@@ -1,0 +1,139 @@
+"""The LIS solver options beyond "-i cg -p jacobi" on the GPU, against the oracle.
+
+* SSOR preconditioner (M = (D + w L) D^-1 (D + w U), level-scheduled sweeps): M^-1 r is
+  bit-identical to the oracle's sequential sweeps; the Jacobi application too.
+* CG, CGS and BiCGSTAB with no, Jacobi or SSOR preconditioning: after five iterations
+  the iterate follows the oracle's trajectory (1e-10; the dot products are summed in
+  another order), and at tolerance 1e-12 the potential is within 1e-9 of the oracle's
+  FP64 solution, with iteration counts close to the oracle's.
+* The reference's default options "-p ssor -ssor_omega 0.51 -i cgs -tol 1.e-6"
+  (src/pb_class.cpp:970): the true residual meets 1e-6 and the energy matches the
+  oracle's same solve.
+"""
+import functools
+
+import numpy as np
+import pytest
+
+import oracle as O
+from cudanextgenpb_amd import native, workload as W
+from cudanextgenpb_amd.pb import PoissonBoltzmann
+
+pytestmark = pytest.mark.gpu
+
+SOLVERS = {"cg": native.SOLVER_CG, "cgs": native.SOLVER_CGS, "bicgstab": native.SOLVER_BICGSTAB}
+PRECONS = {"none": native.PRECON_NONE, "jacobi": native.PRECON_JACOBI, "ssor": native.PRECON_SSOR}
+
+
+@functools.lru_cache(maxsize=None)
+def problem(name):
+    if name == "three_spheres":
+        return W.three_spheres()
+    if name == "globular_small":
+        return W.globular(n_atoms=60, ball_radius=6.0, min_sep=2.0, h_fine=0.5, levels=2, fine_band=1.5,
+                          mid_band=4.0, bc_type=2)
+    if name == "born_h1":
+        return W.born(h=1.0)
+    raise KeyError(name)
+
+
+@functools.lru_cache(maxsize=None)
+def reference(name):
+    P = problem(name)
+    print(f"[oracle] {name} N={P.mesh.n_nodes} H={P.mesh.n_hang}", flush=True)
+    return O.full_pipeline(P, tol=1e-12)
+
+
+def context(name):
+    S = reference(name)["system"]
+    ctx = native.Context(0)
+    ctx.set_csr(S.rowptr, S.col, S.val, S.rhs)
+    return ctx, S
+
+
+@pytest.mark.parametrize("name", ["three_spheres", "globular_small", "born_h1"])
+def test_preconditioner_application_bitexact(name):
+    ctx, S = context(name)
+    r = np.random.default_rng(4).standard_normal(S.n)
+    for w in (0.51, 1.0, 1.6):
+        print(f"[ssor apply] {name} w={w}", flush=True)
+        z = ctx.precond_apply(r, native.PRECON_SSOR, w)
+        assert np.array_equal(z, O.ssor_apply(S, w, r)), w
+    A = S.to_dense() if S.n < 3000 else None
+    d = np.array([S.val[S.rowptr[i] + np.nonzero(S.col[S.rowptr[i]:S.rowptr[i + 1]] == i)[0][0]] for i in range(S.n)])
+    assert np.array_equal(ctx.precond_apply(r, native.PRECON_JACOBI), (1.0 / d) * r)
+    del A
+    ctx.close()
+
+
+COMBOS = [(s, p) for s in SOLVERS for p in PRECONS]
+
+
+@pytest.mark.parametrize("solver,precon", COMBOS)
+def test_solvers_match_oracle(solver, precon):
+    for name in ("three_spheres", "globular_small"):
+        print(f"[lis solve] {name} -i {solver} -p {precon}", flush=True)
+        ctx, S = context(name)
+        R = reference(name)
+        w = 0.51 if precon == "ssor" else 1.0
+        kw = dict(solver=SOLVERS[solver], precon=PRECONS[precon], omega=w)
+        # trajectory: five iterations
+        x5, it5, _, st5 = O.krylov(S, solver, precon, omega=w, tol=1e-15, maxiter=5)
+        assert st5 == 4 and it5 == 5
+        phi5, st = ctx.solve(native.PREC_FP64, tol=1e-15, maxiter=5, check_every=2, raise_on_noconv=False, **kw)
+        assert st["status"] == 4 and st["iterations"] == 5
+        assert np.abs(phi5 - x5).max() <= 1e-10 * np.abs(x5).max(), (name, solver, precon)
+        # converged solution
+        xo, ito, _, sto = O.krylov(S, solver, precon, omega=w, tol=1e-12)
+        assert sto == 0
+        phi, st = ctx.solve(native.PREC_FP64, tol=1e-12, maxiter=100000, check_every=8, **kw)
+        assert st["status"] == 0 and st["relres"] <= 1e-12
+        print(f"   gpu {st['iterations']} it, oracle {ito} it, {st['ms_solve']:.1f} ms", flush=True)
+        assert abs(st["iterations"] - ito) <= max(3, ito // 10)
+        res = np.linalg.norm(S.rhs - O.spmv(S, phi)) / np.linalg.norm(S.rhs)
+        assert res <= 1e-11
+        assert np.linalg.norm(phi - R["phi"]) <= 1e-9 * np.linalg.norm(R["phi"])
+        ctx.close()
+
+
+@pytest.mark.parametrize("name", ["three_spheres", "globular_small"])
+def test_reference_default_options(name):
+    P = problem(name)
+    R = reference(name)
+    S = R["system"]
+    print(f"[lis default] {name}", flush=True)
+    xo, ito, _, sto = O.krylov(S, "cgs", "ssor", omega=0.51, tol=1e-6, maxiter=1000)
+    assert sto == 0
+    pb = PoissonBoltzmann(P)
+    pb.create_markers()
+    pb.create_density_map()
+    pb.assemple_system_matrix(O.dirichlet_values(P))
+    phi = pb.hip_compute_electric_potential("-p ssor -ssor_omega 0.51 -i cgs -tol 1.e-6")
+    st = pb.stats
+    assert st["status"] == 0 and abs(st["iterations"] - ito) <= max(2, ito // 10)
+    assert np.linalg.norm(S.rhs - O.spmv(S, phi)) <= 1.01e-6 * np.linalg.norm(S.rhs)
+    assert np.linalg.norm(phi - xo) <= 1e-6 * np.linalg.norm(xo)
+    E = pb.energy()
+    eps, _, _, border = O.markers(P)
+    Eo = O.energy(P, eps, xo, border)
+    assert abs(E["energy_pol"] - Eo["energy_pol"]) <= 1e-6 * abs(Eo["energy_pol"])
+    # against the tightly solved potential the default tolerance costs accuracy
+    rel = abs(E["polar"] - R["energy"]["polar"]) / abs(R["energy"]["polar"])
+    print(f"   {st['iterations']} it (oracle {ito}); energy rel diff vs tol-1e-12 solution {rel:.2e}", flush=True)
+    pb.close()
+
+
+def test_other_options_refused_where_not_built():
+    ctx, S = context("three_spheres")
+    with pytest.raises(native.NGPBError) as ei:
+        ctx.solve(native.PREC_MIXED, solver=native.SOLVER_CGS, precon=native.PRECON_SSOR)
+    assert ei.value.code == 1
+    with pytest.raises(native.NGPBError) as ei:
+        ctx.solve(native.PREC_FP64, precon=native.PRECON_SSOR, omega=2.5)
+    assert ei.value.code == 1
+    ctx.close()
+    b = native.Batch(0, 2)
+    o = native.SolveOpts(native.PREC_MIXED, 1e-10, 1000, 1e-3, 1, native.SOLVER_CGS, native.PRECON_JACOBI, 1.0)
+    import ctypes
+    assert native.lib().ngpb_batch_set_options(b._h, ctypes.byref(o), 8) == 1
+    b.close()
