@@ -214,6 +214,7 @@ int ngpb_set_physics(ngpb_ctx* c, const ngpb_physics* p) {
     NGPB_REQUIRE(p->e_in > 0 && p->e_out > 0 && p->T > 0 && p->ionic_strength >= 0, NGPB_E_ARG,
                  "permittivities and temperature must be positive, ionic strength non-negative");
     NGPB_REQUIRE(p->calc_energy >= 0 && p->calc_energy <= 2, NGPB_E_ARG, "calc_energy must be 0, 1 or 2");
+    NGPB_REQUIRE(p->calc_coulombic == 0 || p->calc_coulombic == 1, NGPB_E_ARG, "calc_coulombic must be 0 or 1");
     c->physics = *p;
     c->pc = ngpb::physical_constants(*p);
     c->have_physics = true;

@@ -225,6 +225,33 @@ __global__ void k_gather_atoms(int64_t n, const int32_t* __restrict__ idx, const
 
 constexpr int kNB = 256;
 
+// Coulomb pair energy of the charged atoms (src/pb_class.cpp:2842-2857): thread i sums
+// (q_i q_j) / r_ij over j > i in increasing j with the reference's arithmetic
+// (r = sqrt(dx dx + dy dy + dz dz)); the per-atom sums are then added in atom order.
+__global__ void __launch_bounds__(kNB) k_coul_rows(int64_t nq, const double4* __restrict__ atoms,
+                                                   double* __restrict__ rows) {
+  __shared__ double4 s_at[kNB];
+  const int64_t i = (int64_t)blockIdx.x * kNB + threadIdx.x;
+  const bool act = i < nq;
+  const double4 a = act ? atoms[i] : make_double4(0.0, 0.0, 0.0, 0.0);
+  double e = 0.0;
+  for (int64_t t0 = (int64_t)blockIdx.x * kNB; t0 < nq; t0 += kNB) {  // tiles from this block's first atom
+    __syncthreads();
+    if (t0 + threadIdx.x < nq) s_at[threadIdx.x] = atoms[t0 + threadIdx.x];
+    __syncthreads();
+    const int cnt = (int)min((int64_t)kNB, nq - t0);
+    for (int jj = 0; jj < cnt; ++jj) {
+      const int64_t j = t0 + jj;
+      if (j <= i) continue;
+      const double4 b = s_at[jj];
+      const double dx = a.x - b.x, dy = a.y - b.y, dz = a.z - b.z;
+      const double r = sqrt(dx * dx + dy * dy + dz * dz);
+      e += a.w * b.w / r;
+    }
+  }
+  if (act) rows[i] = e;
+}
+
 // MODE 0/1: flux sources, first += q flux / r
 // MODE 2:   triangle-vertex sources, second += q (phi area / (3 4 pi)) (d . n) / r^3
 // Each thread takes kNS sources, so every atom read from LDS serves kNS pairs.
@@ -671,7 +698,8 @@ void run_energy(ngpb_ctx* c, ngpb_energy_out* out) {
   const bool polar_only = c->physics.calc_energy == 1 || (c->physics.calc_energy == 2 && c->pc.k < 1.e-5);
   std::memset(out, 0, sizeof(*out));
   out->n_border = nb;
-  if (nb == 0 || (!ionic && !polar_only)) return;
+  const bool coulomb = c->physics.calc_energy > 0 && c->physics.calc_coulombic == 1;
+  if (!coulomb && (nb == 0 || (!ionic && !polar_only))) return;
   // charged atoms, in order
   double4* atoms = tmp_buf<double4>(c, kTmpEnergy + 16, (size_t)std::max<int64_t>(c->A, 1));
   int64_t nq = 0;
@@ -684,6 +712,18 @@ void run_energy(ngpb_ctx* c, ngpb_energy_out* out) {
       hipLaunchKernelGGL(k_gather_atoms, dim3(blocks_for(nq, 256)), dim3(256), 0, c->stream, nq, aidx, c->atom_xyz.p,
                          c->atom_q.p, atoms);
   }
+  if (coulomb && nq > 1) {
+    double* rows = tmp_buf<double>(c, kTmpEnergy + 18, (size_t)nq);
+    hipLaunchKernelGGL(k_coul_rows, dim3(blocks_for(nq, kNB)), dim3(kNB), 0, c->stream, nq, atoms, rows);
+    c->en_out.resize(4);
+    sum3(c, kTmpEnergy + 3, rows, nq, rows, 0, rows, 0, c->en_out.p);
+    NGPB_HIP(hipGetLastError());
+    double h = 0.0;
+    NGPB_HIP(hipMemcpyAsync(&h, c->en_out.p, sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    NGPB_HIP(hipStreamSynchronize(c->stream));
+    out->coul_energy = h * (1.0 / c->pc.eps_in);
+  }
+  if (nb == 0 || (!ionic && !polar_only)) return;
   Sources S;
   build_sources(c, ionic, S);
   const int64_t nfv = S.nf, nvv = S.nv;

@@ -18,6 +18,7 @@
 // kernels launched after the stop exit at once.
 #include <algorithm>
 #include <cmath>
+#include <cstring>
 
 #include "ngpb_internal.h"
 

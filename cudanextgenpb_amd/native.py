@@ -32,7 +32,8 @@ class NGPBError(RuntimeError):
 
 
 class Physics(ctypes.Structure):
-    _fields_ = [("e_in", _D), ("e_out", _D), ("T", _D), ("ionic_strength", _D), ("calc_energy", _I)]
+    _fields_ = [("e_in", _D), ("e_out", _D), ("T", _D), ("ionic_strength", _D), ("calc_energy", _I),
+                ("calc_coulombic", _I)]
 
 
 SOLVER_CG, SOLVER_CGS, SOLVER_BICGSTAB = 0, 1, 2
@@ -54,12 +55,14 @@ class SolveStats(ctypes.Structure):
 
 class EnergyOut(ctypes.Structure):
     _fields_ = [("energy_pol", _D), ("energy_react", _D), ("charge_pol", _D), ("n_border", _I64),
-                ("n_flux_edges", _I64), ("n_triangles", _I64)]
+                ("n_flux_edges", _I64), ("n_triangles", _I64), ("coul_energy", _D)]
 
     def as_dict(self):
         d = {k: getattr(self, k) for k, _ in self._fields_}
         d["flux_charge"] = self.charge_pol / (4.0 * np.pi)
         d["polar"] = self.energy_pol + self.energy_react
+        # "Sum of electrostatic energy contributions" as the reference prints it
+        d["total"] = self.energy_pol + self.energy_react + self.coul_energy
         return d
 
 
@@ -162,7 +165,8 @@ def device_count() -> int:
 
 
 def constants(physics) -> dict:
-    ph = Physics(physics.e_in, physics.e_out, physics.T, physics.ionic_strength, physics.calc_energy)
+    ph = Physics(physics.e_in, physics.e_out, physics.T, physics.ionic_strength, physics.calc_energy,
+                 getattr(physics, "calc_coulombic", 0))
     out = np.zeros(6)
     lib().ngpb_constants(ctypes.byref(ph), _p(out))
     return dict(eps_in=out[0], eps_out=out[1], k2=out[2], k=out[3], constant_pol=out[4], constant_react=out[5])
@@ -217,7 +221,8 @@ class Context:
 
     # ------------------------------------------------------------------ inputs
     def set_physics(self, physics):
-        ph = Physics(physics.e_in, physics.e_out, physics.T, physics.ionic_strength, physics.calc_energy)
+        ph = Physics(physics.e_in, physics.e_out, physics.T, physics.ionic_strength, physics.calc_energy,
+                 getattr(physics, "calc_coulombic", 0))
         self._check(lib().ngpb_set_physics(self._h, ctypes.byref(ph)), "set_physics")
 
     def set_mesh(self, mesh):

@@ -76,6 +76,7 @@ class Physics:
     T: float = 298.15
     ionic_strength: float = 0.145
     calc_energy: int = 2
+    calc_coulombic: int = 0
 
 
 @dataclass

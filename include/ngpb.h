@@ -70,6 +70,8 @@ typedef struct {
   double T;
   double ionic_strength;
   int calc_energy; /* 0 none, 1 polarization only, 2 polarization + direct ionic */
+  int calc_coulombic; /* 1: ngpb_energy also sums the Coulomb pair energy (calc_coulombic,
+                         src/pb_class.cpp:946, 2842-2857); 0 (default): not computed */
 } ngpb_physics;
 
 /* Krylov method and preconditioner, as LIS's "-i" and "-p" options. */
@@ -104,6 +106,8 @@ typedef struct {
   int64_t n_border;     /* border leaves (marker 0.5) */
   int64_t n_flux_edges; /* edges with eps(i1) != eps(i2) on border leaves */
   int64_t n_triangles;  /* marching-cubes triangles (ionic term) */
+  double coul_energy;   /* calc_coulombic = 1: sum over pairs i < j of charged atoms
+                           (|q| > 1e-5) of q_i q_j / r_ij, times 1 / eps_in_hat [kT] */
 } ngpb_energy_out;
 
 /* Energies printed by pot_field / pot_field_fast (src/pb_class.cpp:4253-4274). */

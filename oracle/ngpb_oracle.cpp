@@ -792,6 +792,28 @@ double orc_pcg_time_iters(int64_t n, const int32_t* rowptr, const int32_t* col,
 #endif
 }
 
+// ---------------------------------------------------------------- Coulomb energy
+// calc_coulombic == 1 in energy / energy_fast (src/pb_class.cpp:2842-2857 and
+// 3116-3131): over the charged atoms (|q| > 1e-5, :2827-2832) in input order, one
+// running sum of (q_i q_j) / r_ij over pairs i < j, then times 1 / eps_in_hat.
+double orc_coul_energy(int64_t na, const double* xyz, const double* q, double eps_in) {
+  std::vector<int64_t> idx;
+  for (int64_t a = 0; a < na; ++a)
+    if (std::fabs(q[a]) > 1.e-5) idx.push_back(a);
+  double e = 0.0;
+  for (size_t i = 0; i < idx.size(); ++i) {
+    const double* ri = xyz + 3 * idx[i];
+    const double qi = q[idx[i]];
+    for (size_t j = i + 1; j < idx.size(); ++j) {
+      const double* rj = xyz + 3 * idx[j];
+      const double dx = ri[0] - rj[0], dy = ri[1] - rj[1], dz = ri[2] - rj[2];
+      const double r = std::sqrt(dx * dx + dy * dy + dz * dz);
+      e += qi * q[idx[j]] / r;
+    }
+  }
+  return e * (1.0 / eps_in);
+}
+
 // ---------------------------------------------------------------- energy
 // energy (src/pb_class.cpp:2790-3063) for one rank, with classifyCube
 // (:2570-2598), classifyCube_flux (:2627-2661), getTriangles (:2771-2788), phi0

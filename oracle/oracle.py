@@ -61,6 +61,8 @@ def lib():
         L.orc_pcg_time_iters.argtypes = [_I64, _P, _P, _P, _P, ctypes.c_int, ctypes.c_int]
         L.orc_pcg_time_iters.restype = _D
         L.orc_energy.argtypes = [_P, _P, _P, _P, _I64, _P, _I64, _P, _P, _D, _D, _D, _D, ctypes.c_int, _P]
+        L.orc_coul_energy.argtypes = [_I64, _P, _P, _D]
+        L.orc_coul_energy.restype = _D
         L.orc_analytic_solution.argtypes = [_D] * 7
         L.orc_analytic_solution.restype = _D
         _lib = L
@@ -284,8 +286,18 @@ def energy(problem, eps, phi, border):
     lib().orc_energy(m.ptr, r.ptr, _p(_c(eps, np.float64)), _p(_c(phi, np.float64)), b.shape[0], _p(b),
                      problem.atoms.n, _p(xyz), _p(q), ph.e_in, ph.e_out, ph.T, ph.ionic_strength,
                      ph.calc_energy, _p(out))
+    coul = 0.0
+    if ph.calc_energy > 0 and getattr(ph, "calc_coulombic", 0) == 1:
+        coul = coul_energy(problem.atoms, ph)
     return dict(energy_pol=out[0], energy_react=out[1], charge_pol=out[2],
-                flux_charge=out[2] / (4.0 * np.pi), polar=out[0] + out[1])
+                flux_charge=out[2] / (4.0 * np.pi), polar=out[0] + out[1], coul_energy=coul,
+                total=out[0] + out[1] + coul)
+
+
+def coul_energy(atoms, physics):
+    """Coulomb pair energy of the charged atoms (|q| > 1e-5) over eps_in_hat."""
+    eps_in = constants(physics)["eps_in"]
+    return lib().orc_coul_energy(atoms.n, _p(_c(atoms.xyz, np.float64)), _p(_c(atoms.q, np.float64)), eps_in)
 
 
 POT_FIELD_COLUMNS = ("phi_c", "phi_p", "phi_i", "Ex_c", "Ey_c", "Ez_c", "Ex_p", "Ey_p", "Ez_p", "Ex_i", "Ey_i",

@@ -218,6 +218,34 @@ def test_pot_field_matches_oracle(name, pf, tmp_path):
     pb.close()
 
 
+@pytest.mark.parametrize("name", ["three_spheres", "globular_small"])
+def test_coulomb_energy_matches_oracle(name):
+    """calc_coulombic = 1: the Coulomb pair energy of ngpb_energy within 1e-12 of the
+    oracle's running sum (pair terms identical, summed per atom then in atom order);
+    with calc_coulombic = 0 it is not computed."""
+    import dataclasses
+    base = problem(name)
+    R = reference(name)
+    P = dataclasses.replace(base, physics=dataclasses.replace(base.physics, calc_coulombic=1))
+    pb = PoissonBoltzmann(P)
+    pb.create_markers()
+    pb.create_density_map()
+    pb.assemple_system_matrix(O.dirichlet_values(P))
+    pb.ctx.set_phi(R["phi"])
+    E = pb.energy()
+    want = O.coul_energy(P.atoms, P.physics)
+    print(f"[coulomb] {name} {E['coul_energy']!r} oracle {want!r}", flush=True)
+    assert abs(E["coul_energy"] - want) <= 1e-12 * abs(want)
+    assert E["total"] == E["energy_pol"] + E["energy_react"] + E["coul_energy"]
+    for k in ("energy_pol", "energy_react"):
+        assert abs(E[k] - R["energy"][k]) <= 1e-11 * abs(R["energy"][k])
+    pb.close()
+    _, pb0 = gpu_pipeline(name)
+    pb0.ctx.set_phi(R["phi"])
+    assert pb0.energy()["coul_energy"] == 0.0
+    pb0.close()
+
+
 def test_born_energy_closed_form():
     P, pb = gpu_pipeline("born_h05")
     pb.hip_compute_electric_potential()

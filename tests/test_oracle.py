@@ -471,6 +471,34 @@ def test_gauss_flux_charge_equals_net_charge():
     assert R["energy"]["flux_charge"] == pytest.approx(P.atoms.q.sum(), abs=1e-6)
 
 
+def test_coulomb_energy_pair_sums():
+    """calc_coulombic (src/pb_class.cpp:2842-2857): two unit charges at distance d give
+    q1 q2 / (eps_in d); a random set equals a vectorized pair sum; atoms with |q| <= 1e-5
+    do not take part."""
+    ph = W.Physics(calc_coulombic=1)
+    eps_in = O.constants(ph)["eps_in"]
+    two = W.Atoms(np.array([[0.0, 0.0, 0.0], [0.0, 3.0, 4.0]]), np.ones(2), np.array([1.0, -2.0]))
+    assert O.coul_energy(two, ph) == pytest.approx(-2.0 / (5.0 * eps_in), rel=1e-15)
+    rng = np.random.default_rng(8)
+    xyz = rng.uniform(-10, 10, (300, 3))
+    q = rng.normal(0, 0.4, 300)
+    q[::7] = 1e-6  # below the charge filter
+    a = W.Atoms(xyz, np.ones(300), q)
+    keep = np.abs(q) > 1e-5
+    x, qq = xyz[keep], q[keep]
+    d = np.sqrt(((x[:, None, :] - x[None, :, :]) ** 2).sum(-1))
+    iu = np.triu_indices(len(qq), 1)
+    want = (qq[:, None] * qq[None, :])[iu] / d[iu]
+    assert O.coul_energy(a, ph) == pytest.approx(want.sum() / eps_in, rel=1e-12)
+    # energy() carries it only when asked for, and in the reference's total
+    P = W.three_spheres(physics=ph)
+    R = O.full_pipeline(P)
+    E = R["energy"]
+    assert E["coul_energy"] == pytest.approx(O.coul_energy(P.atoms, ph), rel=0)
+    assert E["total"] == E["energy_pol"] + E["energy_react"] + E["coul_energy"]
+    assert O.full_pipeline(W.three_spheres())["energy"]["coul_energy"] == 0.0
+
+
 def test_energy_from_golden_direct_solution():
     g = np.load(os.path.join(GOLDEN, "born_h2.npz"))
     P = W.born(h=2.0)
