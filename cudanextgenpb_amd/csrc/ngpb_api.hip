@@ -744,6 +744,43 @@ int ngpb_precond_apply(ngpb_ctx* c, int precon, double omega, const double* r, d
   });
 }
 
+int ngpb_solve_csr_noalloc(int device, void* stream, int64_t n, int64_t nnz, const int32_t* rowptr,
+                           const int32_t* col, const double* val, const double* rhs, double* x,
+                           const ngpb_solve_opts* opts, ngpb_solve_stats* stats) {
+  ngpb_solve_stats local{};
+  ngpb_solve_stats& st = stats ? *stats : local;
+  std::memset(&st, 0, sizeof(st));
+  int count = 0;
+  if (hipGetDeviceCount(&count) != hipSuccess || count <= 0) {
+    (void)hipGetLastError();
+    return NGPB_E_NODEV;
+  }
+  if (device < 0 || device >= count || device >= 64) return NGPB_E_ARG;
+  if (!rowptr || !col || !val || !rhs || !x || !opts || n <= 0 || nnz <= 0 || nnz >= INT32_MAX) return NGPB_E_ARG;
+  if (n > ngpb::noalloc_max_rows()) return NGPB_E_ARG;
+  if (!(opts->tol > 0) || opts->maxiter <= 0 || opts->precision != NGPB_PREC_FP64 ||
+      opts->solver != NGPB_SOLVER_CG || opts->precon != NGPB_PRECON_JACOBI)
+    return NGPB_E_ARG;
+  int prev = -1;
+  if (hipGetDevice(&prev) != hipSuccess) {
+    (void)hipGetLastError();
+    prev = -1;
+  }
+  int rc = NGPB_OK;
+  try {
+    NGPB_HIP(hipSetDevice(device));
+    ngpb::solve_csr_noalloc(device, static_cast<hipStream_t>(stream), n, nnz, rowptr, col, val, rhs, x, opts->tol,
+                            opts->maxiter, opts->check_every, st);
+    rc = st.status;
+  } catch (const Error& e) {
+    rc = e.code;
+  } catch (const std::exception&) {
+    rc = NGPB_E_HIP;
+  }
+  if (prev >= 0) (void)hipSetDevice(prev);
+  return rc;
+}
+
 int ngpb_get_phi(ngpb_ctx* c, double* phi) {
   return guarded(c, [&] {
     NGPB_REQUIRE(phi, NGPB_E_ARG, "NULL output");

@@ -343,6 +343,11 @@ void check_operator(ngpb_ctx* c);
 void check_lis_options(const ngpb_solve_opts& o);
 void solve_lis(ngpb_ctx* c, const ngpb_solve_opts& o, ngpb_solve_stats& st);
 void precond_apply(ngpb_ctx* c, int precon, double omega, const double* r, double* z);
+// CG-Jacobi on caller device arrays with the library's static scratch (no allocation)
+int64_t noalloc_max_rows();
+void solve_csr_noalloc(int device, hipStream_t s, int64_t n, int64_t nnz, const int32_t* rowptr, const int32_t* col,
+                       const double* val, const double* b, double* x, double tol, int maxiter, int check_every,
+                       ngpb_solve_stats& st);
 void spmv_fp64(ngpb_ctx* c, const double* x, double* y);
 void solve_fp64(ngpb_ctx* c, const ngpb_solve_opts& o, ngpb_solve_stats& st);
 double spmv_fp64_bytes(const ngpb_ctx* c);

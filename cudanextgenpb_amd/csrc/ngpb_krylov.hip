@@ -17,8 +17,10 @@
 // device memory; the host polls the stop flag every `check_every` iterations and
 // kernels launched after the stop exit at once.
 #include <algorithm>
+#include <chrono>
 #include <cmath>
 #include <cstring>
+#include <mutex>
 
 #include "ngpb_internal.h"
 
@@ -431,6 +433,122 @@ void solve_fp64(ngpb_ctx* c, const ngpb_solve_opts& o, ngpb_solve_stats& st) {
   st.bytes_per_iter = spmv_fp64_bytes(c) + (double)n * 8.0 * (7.0 + 4.0);
   c->have_phi = true;
   c->phi_len = c->n;
+}
+
+// ------------------------------------------------------------ allocation-free entry point
+// ngpb_solve_csr_noalloc runs the CG-Jacobi above on a CSR system the caller keeps in
+// device memory, with its work vectors, partial sums, scalars and flags in one
+// file-scope device array: the call allocates nothing (no hipMalloc / hipFree, no
+// pinned host memory, no events), for callers that cannot allocate. Each GPU holds
+// its own copy of the array until the process ends; its address on a device is taken
+// once with hipGetSymbolAddress, and only that address is handed to kernels and
+// copies. Every call on one GPU uses the same array, so calls that overlap on one GPU
+// (from several host threads or streams) race; each call initializes what it reads.
+constexpr int64_t kNoAllocMaxRows = int64_t(1) << 20;  // 1,048,576 rows
+constexpr int64_t kNoAllocPartials = 8192;             // >= 2048 + 2 * 2048
+constexpr int64_t kNoAllocDoubles = 4 * kNoAllocMaxRows + kNoAllocPartials + 64;
+static __device__ __align__(256) double scratch[kNoAllocDoubles];
+
+namespace {
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+double* g_scratch_ptr[kMaxDevices] = {};
+
+__global__ void k_clear_int(int* p) { *p = 0; }
+
+// Structure of a CSR the library did not build: row pointers from 0 to nnz, non-
+// decreasing, columns in [0, n). Sets bit 4 of *bad otherwise; reads col only below nnz.
+__global__ void k_csr_check(int64_t n, int64_t nnz, const int32_t* __restrict__ rowptr,
+                            const int32_t* __restrict__ col, int* __restrict__ bad) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const int64_t a = rowptr[i], e = rowptr[i + 1];
+  if ((i == 0 && a != 0) || (i == n - 1 && e != nnz) || a < 0 || e < a || e > nnz) {
+    atomicOr(bad, 4);
+    return;
+  }
+  for (int64_t k = a; k < e; ++k)
+    if (col[k] < 0 || col[k] >= n) {
+      atomicOr(bad, 4);
+      return;
+    }
+}
+
+// device address of `scratch` on the current device (taken on the first call there)
+double* scratch_on(int device) {
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (!g_scratch_ptr[device]) {
+    void* ptr = nullptr;
+    NGPB_HIP(hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)));
+    NGPB_REQUIRE(ptr != nullptr, NGPB_E_HIP, "no device address for the scratch array");
+    g_scratch_ptr[device] = static_cast<double*>(ptr);
+  }
+  return g_scratch_ptr[device];
+}
+
+}  // namespace
+
+int64_t noalloc_max_rows() { return kNoAllocMaxRows; }
+
+void solve_csr_noalloc(int device, hipStream_t s, int64_t n, int64_t nnz, const int32_t* rowptr, const int32_t* col,
+                       const double* val, const double* b, double* x, double tol, int maxiter, int check_every,
+                       ngpb_solve_stats& st) {
+  NGPB_REQUIRE(n > 0 && n <= kNoAllocMaxRows, NGPB_E_ARG,
+               "ngpb_solve_csr_noalloc takes 1 to 1,048,576 rows (its scratch array is sized for that)");
+  double* base = scratch_on(device);
+  double* dinv = base;
+  double* r = base + kNoAllocMaxRows;
+  double* p = base + 2 * kNoAllocMaxRows;
+  double* q = base + 3 * kNoAllocMaxRows;
+  double* part = base + 4 * kNoAllocMaxRows;
+  KrylovScalars* ks = reinterpret_cast<KrylovScalars*>(part + kNoAllocPartials);
+  int* bad = reinterpret_cast<int*>(ks + 1);
+  static_assert(sizeof(KrylovScalars) + sizeof(int) <= 64 * sizeof(double), "scalar area too small");
+  const auto t0 = std::chrono::steady_clock::now();
+  hipLaunchKernelGGL(k_clear_int, dim3(1), dim3(1), 0, s, bad);
+  hipLaunchKernelGGL(k_csr_check, dim3(blocks_for(n, 256)), dim3(256), 0, s, n, nnz, rowptr, col, bad);
+  NGPB_HIP(hipGetLastError());
+  int hb = 0;
+  NGPB_HIP(hipMemcpyAsync(&hb, bad, sizeof(int), hipMemcpyDeviceToHost, s));
+  NGPB_HIP(hipStreamSynchronize(s));
+  NGPB_REQUIRE(!(hb & 4), NGPB_E_ARG, "row pointers must run from 0 to nnz without decreasing, columns in [0, n)");
+  hipLaunchKernelGGL(k_diag_inv, dim3(blocks_for(n, 256)), dim3(256), 0, s, n, rowptr, col, val, dinv, bad);
+  NGPB_HIP(hipGetLastError());
+  NGPB_HIP(hipMemcpyAsync(&hb, bad, sizeof(int), hipMemcpyDeviceToHost, s));
+  NGPB_HIP(hipStreamSynchronize(s));
+  NGPB_REQUIRE(!(hb & 2), NGPB_E_NUMERIC, "operator holds NaN or Inf entries");
+  NGPB_REQUIRE(!(hb & 1), NGPB_E_NUMERIC, "operator has a zero or negative diagonal entry");
+  // the launch shapes of ngpb_solve under SpMV schedule 0
+  const unsigned gs = (unsigned)std::min<int64_t>(std::max<int64_t>(1, (n + kBS - 1) / kBS), 2048);
+  const unsigned gv = vec_grid(n);
+  double* ppq = part;
+  double* pup = part + 2048;
+  hipLaunchKernelGGL(k_pcg_init, dim3(gv), dim3(kBS), 0, s, n, b, dinv, x, r, p, pup);
+  hipLaunchKernelGGL(k_fin_init, dim3(1), dim3(kFin), 0, s, ks, pup, (int)gv, tol * tol, maxiter);
+  NGPB_HIP(hipGetLastError());
+  const int chunk = std::max(1, check_every);
+  KrylovScalars h;
+  std::memset(&h, 0, sizeof(h));
+  for (int64_t launch = 0;;) {
+    for (int k = 0; k < chunk; ++k, ++launch) {
+      const int par = (int)(launch & 1);
+      hipLaunchKernelGGL((k_spmv64<true, 0>), dim3(gs), dim3(kBS), 0, s, n, rowptr, col, val, p, q, ks, ppq);
+      hipLaunchKernelGGL(k_pcg_update, dim3(gv), dim3(kBS), 0, s, n, dinv, x, r, p, q, ks, ppq, (int)gs, pup, par);
+      hipLaunchKernelGGL(k_pcg_dir, dim3(gv), dim3(kBS), 0, s, n, dinv, r, p, ks, pup, (int)gv, par);
+    }
+    NGPB_HIP(hipGetLastError());
+    NGPB_HIP(hipMemcpyAsync(&h, ks, sizeof(KrylovScalars), hipMemcpyDeviceToHost, s));
+    NGPB_HIP(hipStreamSynchronize(s));
+    if (h.done) break;
+  }
+  st.status = h.status;
+  st.iterations = h.iter;
+  st.replacements = 0;
+  st.relres = h.bb > 0 ? std::sqrt(h.rr / h.bb) : 0.0;
+  st.ms_solve = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+  // SpMV (12 nnz + 4 (n + 1) + 16 n) + update and direction vector streams, as ngpb_solve
+  st.bytes_per_iter = 12.0 * (double)nnz + 4.0 * (double)(n + 1) + 16.0 * (double)n + (double)n * 8.0 * (7.0 + 4.0);
 }
 
 }  // namespace ngpb

@@ -101,6 +101,7 @@ SIGNATURES = {
     "ngpb_parse_lis_options": (_I, [ctypes.c_char_p, _P]),
     "ngpb_solve": (_I, [_P, _P, _P, _P]),
     "ngpb_precond_apply": (_I, [_P, _I, _D, _P, _P]),
+    "ngpb_solve_csr_noalloc": (_I, [_I, _P, _I64, _I64, _P, _P, _P, _P, _P, _P, _P]),
     "ngpb_get_phi": (_I, [_P, _P]),
     "ngpb_set_phi": (_I, [_P, _P]),
     "ngpb_spmv": (_I, [_P, _I, _P, _P]),

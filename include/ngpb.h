@@ -247,6 +247,23 @@ int ngpb_solve(ngpb_ctx* ctx, const ngpb_solve_opts* opts, double* phi_out, ngpb
    omega); host vectors of n entries. The preconditioner application inside LIS's
    solvers; no direct counterpart in the reference's own code. */
 int ngpb_precond_apply(ngpb_ctx* ctx, int precon, double omega, const double* r, double* z);
+/* Allocation-free FP64 solve. No counterpart in the reference: it is for callers that
+   cannot allocate device memory (code under HIP graph capture elsewhere in the
+   process, tight serving loops, harnesses that forbid hipMalloc). It runs the CG with
+   Jacobi preconditioning of ngpb_solve (the same kernels and, under SpMV schedule 0,
+   the same result bit for bit) on a CSR system the caller holds in DEVICE memory on
+   `device`: rowptr (n+1), col (nnz), val (nnz), rhs (n) in, x (n) out, all device
+   pointers; stream is a hipStream_t (NULL: the default stream). opts: precision FP64,
+   solver CG, precon Jacobi, tol, maxiter, check_every. All work memory (4 vectors of
+   n doubles, partial sums, scalars) lives in a file-scope device array of 32.1 MiB per
+   GPU, held until the process ends; n is limited to 1,048,576 rows (NGPB_E_ARG
+   beyond). The call makes no hipMalloc/hipFree or pinned-host allocation. Every call
+   on one GPU shares that array: calls that overlap on one GPU race. Returns NGPB_OK,
+   NGPB_E_NOCONV, NGPB_E_NUMERIC (NaN/Inf, non-positive diagonal), NGPB_E_ARG (also
+   for a CSR whose row pointers or columns are out of range), NGPB_E_HIP, NGPB_E_NODEV. */
+int ngpb_solve_csr_noalloc(int device, void* stream, int64_t n, int64_t nnz, const int32_t* rowptr,
+                           const int32_t* col, const double* val, const double* rhs, double* x,
+                           const ngpb_solve_opts* opts, ngpb_solve_stats* stats);
 int ngpb_get_phi(ngpb_ctx* ctx, double* phi);
 int ngpb_set_phi(ngpb_ctx* ctx, const double* phi);
 /* y = A x with the resident operator: precision FP64 (CSR) or MIXED (scaled FP32 form,
