@@ -419,12 +419,66 @@ def bench_request(args):
     }
 
 
+def bench_spmv(args):
+    """SpMV kernels alone on an operator assembled from atoms on the device: the
+    compact FP32 SpMV under the chosen schedule and the FP64 CSR SpMV, each timed
+    over back-to-back launches with HIP events on the context's stream, and a plain
+    copy kernel of known size for reference (also the calibration kernel of the
+    rocprofv3 FETCH_SIZE / WRITE_SIZE runs)."""
+    from cudanextgenpb_amd import native, workload as W
+    from cudanextgenpb_amd.batch import prepare
+
+    spec = W.large_spec() if args.mesh_size == "large" else W.globular_spec()
+    p = prepare(spec)
+    ctx = p.ctx
+    info = ctx.operator_info()
+    x = np.random.default_rng(1).standard_normal(p.n)
+    ctx.set_spmv_schedule(4)
+    y_ref = ctx.spmv(x, native.PREC_MIXED)
+    ctx.set_spmv_schedule(args.spmv_schedule)
+    y = ctx.spmv(x, native.PREC_MIXED)
+    if not np.array_equal(y, y_ref):
+        raise SystemExit("SpMV schedule changed the result")
+    ms32, b32 = ctx.time_spmv(native.PREC_MIXED, reps=args.spmv_reps)
+    ms64, b64 = ctx.time_spmv(native.PREC_FP64, reps=args.spmv_reps)
+    copy_bytes = 1 << 30
+    copy_ms = ctx.time_copy(copy_bytes, reps=args.spmv_reps)
+    ctx.close()
+    gbs = b32 / (ms32 * 1e-3) / 1e9
+    log(f"compact SpMV {ms32 * 1e3:.1f} us ({b32 / 1e6:.1f} MB, {gbs:.0f} GB/s); FP64 CSR {ms64 * 1e3:.1f} us "
+        f"({b64 / (ms64 * 1e-3) / 1e9:.0f} GB/s); copy {2 * copy_bytes / (copy_ms * 1e-3) / 1e9:.0f} GB/s")
+    return {
+        "metric": "compact FP32 SpMV achieved HBM bandwidth (algorithmic bytes per launch / launch time)",
+        "value": gbs,
+        "unit": "GB/s",
+        "n_gpus": 1,
+        "steps": args.spmv_reps,
+        "warmup": 1,
+        "ms_per_step": ms32,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "fp32",
+        "data": "synthetic",
+        "config": {"workload": f"{spec.name}: operator of {p.n} rows, {p.nnz} nnz assembled from atoms on the device; "
+                               f"SpMV schedule {args.spmv_schedule}", "parallelism": "single GPU"},
+        "roofline": {"bound": "hbm", "achieved": gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": gbs / HBM_PEAK_GBS,
+                     "traffic": None, "bytes_per_launch": b32, "avg_launch_ms": ms32},
+        "fp64_spmv": {"ms": ms64, "bytes": b64, "gbs": b64 / (ms64 * 1e-3) / 1e9},
+        "copy_kernel_gbs": 2 * copy_bytes / (copy_ms * 1e-3) / 1e9,
+        "operator": info,
+    }
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=8)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--workload", choices=["globular", "born", "large", "batch", "serve", "mesh", "request"], default="globular")
+    ap.add_argument("--workload", choices=["globular", "born", "large", "batch", "serve", "mesh", "request", "spmv"],
+                    default="globular")
+    ap.add_argument("--spmv-schedule", type=int, default=4, help="spmv workload: compact SpMV schedule (0-4)")
+    ap.add_argument("--spmv-reps", type=int, default=50, help="spmv workload: timed launches per kernel")
     ap.add_argument("--mesh-size", choices=["globular", "large"], default="globular",
                     help="mesh and request workloads: which system")
     ap.add_argument("--requests", type=int, default=48, help="serve workload: number of requests")
@@ -464,6 +518,9 @@ def main():
         return
     if args.workload == "request":
         print(json.dumps(bench_request(args)), flush=True)
+        return
+    if args.workload == "spmv":
+        print(json.dumps(bench_spmv(args)), flush=True)
         return
 
     t = time.perf_counter()

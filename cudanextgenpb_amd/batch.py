@@ -41,11 +41,14 @@ def prepare(problem: Problem, device: int = 0, ctx: Optional[native.Context] = N
     problem's host-built ones)."""
     t0 = time.perf_counter()
     c = ctx or native.Context(device)
+    from .workload import Spec
+    device_mesh = device_mesh or isinstance(problem, Spec)  # a Spec carries no host mesh
     try:
         c.set_physics(problem.physics)
         a = problem.atoms
         if device_mesh:
-            c.build_octree(*problem.mesh.generator_params(), a.xyz, a.r)
+            gen = problem if isinstance(problem, Spec) else problem.mesh
+            c.build_octree(*gen.generator_params(), a.xyz, a.r)
             c.cast_rays(a.xyz, a.r)
             c.create_markers()
             c.set_charges(a.xyz, a.q, c.locate_atoms(a.xyz, problem.atom_tol))

@@ -32,6 +32,7 @@ HIP_SOURCES = [
     "ngpb_energy.hip",
     "ngpb_batch.hip",
     "ngpb_rays.hip",
+    "ngpb_ses.hip",
     "ngpb_locate.hip",
     "ngpb_octree.hip",
 ]

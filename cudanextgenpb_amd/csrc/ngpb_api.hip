@@ -381,7 +381,23 @@ int ngpb_cast_rays(ngpb_ctx* c, int64_t n_balls, const double* xyz, const double
       NGPB_REQUIRE(std::isfinite(r[a]) && r[a] >= 0 && std::isfinite(xyz[3 * a]) && std::isfinite(xyz[3 * a + 1]) &&
                        std::isfinite(xyz[3 * a + 2]),
                    NGPB_E_ARG, "ball centres and radii must be finite, radii >= 0");
-    ngpb::run_cast_rays(c, n_balls, xyz, r);
+    ngpb::run_cast_rays(c, n_balls, xyz, r, 0.0);
+    c->have_markers = c->have_frac = c->have_system = c->have_compact = c->have_phi = false;
+  });
+}
+
+int ngpb_cast_rays_ses(ngpb_ctx* c, int64_t n_balls, const double* xyz, const double* r, double probe_radius) {
+  return guarded(c, [&] {
+    not_in_batch(c);
+    need(c->have_mesh, "ngpb_set_mesh must come before ngpb_cast_rays_ses");
+    NGPB_REQUIRE(n_balls >= 0 && n_balls < (int64_t)1 << 28, NGPB_E_ARG, "bad ball count");
+    NGPB_REQUIRE(std::isfinite(probe_radius) && probe_radius > 0.0, NGPB_E_ARG, "probe radius must be finite and > 0");
+    NGPB_REQUIRE(n_balls == 0 || (xyz && r), NGPB_E_ARG, "NULL ball array");
+    for (int64_t a = 0; a < n_balls; ++a)
+      NGPB_REQUIRE(std::isfinite(r[a]) && r[a] >= 0 && std::isfinite(xyz[3 * a]) && std::isfinite(xyz[3 * a + 1]) &&
+                       std::isfinite(xyz[3 * a + 2]),
+                   NGPB_E_ARG, "ball centres and radii must be finite, radii >= 0");
+    ngpb::run_cast_rays(c, n_balls, xyz, r, probe_radius);
     c->have_markers = c->have_frac = c->have_system = c->have_compact = c->have_phi = false;
   });
 }

@@ -322,7 +322,7 @@ void run_symbolic(ngpb_ctx* c);
 void run_numeric(ngpb_ctx* c);
 void run_dirichlet_values(ngpb_ctx* c, int bc_type);
 void run_phi_on_atoms(ngpb_ctx* c, double* d_out);
-void run_cast_rays(ngpb_ctx* c, int64_t n_balls, const double* h_xyz, const double* h_r);
+void run_cast_rays(ngpb_ctx* c, int64_t na, const double* xyz, const double* r, double probe);
 void run_locate_atoms(ngpb_ctx* c, int64_t A, const double* h_xyz, double tol, int32_t* h_owner);
 struct OctreeSizes {
   int64_t N, H, L, NB;

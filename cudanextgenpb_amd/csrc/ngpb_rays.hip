@@ -291,8 +291,20 @@ __global__ void k_gather_u64(int64_t n, const int32_t* __restrict__ idx, const u
 
 }  // namespace
 
-void run_cast_rays(ngpb_ctx* c, int64_t na, const double* h_xyz, const double* h_r) {
+struct SesDevice;
+SesDevice* ses_prepare(ngpb_ctx* c, int64_t na, const double* xyz, const double* r, double rp);
+void ses_release(SesDevice* D);
+void ses_cast_lines(ngpb_ctx* c, const SesDevice* D, int axis, int64_t nl, const uint64_t* lu, const uint64_t* lv,
+                    const double* h_xyz);
+
+void run_cast_rays(ngpb_ctx* c, int64_t na, const double* h_xyz, const double* h_r, double probe) {
   const int64_t N = c->N, NV = c->N + c->H;
+  // solvent-excluded surface: SAS circles and vertices once, then per axis below
+  struct SesHolder {
+    SesDevice* d = nullptr;
+    ~SesHolder() { if (d) ses_release(d); }
+  } ses;
+  if (probe > 0.0 && na > 0) ses.d = ses_prepare(c, na, h_xyz, h_r, probe);
   // balls and their bins (built on the host as the host caster does, uploaded once)
   double rmax = 0;
   for (int64_t a = 0; a < na; ++a) rmax = std::max(rmax, h_r[a]);
@@ -375,6 +387,10 @@ void run_cast_rays(ngpb_ctx* c, int64_t na, const double* h_xyz, const double* h
     }
     hipLaunchKernelGGL(k_gather_u64, dim3(blocks_for(nl, 256)), dim3(256), 0, c->stream, nl, uniq.p, ku.p, lu.p);
     hipLaunchKernelGGL(k_gather_u64, dim3(blocks_for(nl, 256)), dim3(256), 0, c->stream, nl, uniq.p, kv.p, lv.p);
+    if (ses.d) {
+      ses_cast_lines(c, ses.d, axis, nl, lu.p, lv.p, h_xyz);
+      continue;
+    }
     // pierced balls per line, their intervals sorted and merged
     DBuf<int64_t> off(c->stream), mcnt(c->stream);
     off.resize((size_t)nl);

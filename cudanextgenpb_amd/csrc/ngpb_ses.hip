@@ -1,0 +1,715 @@
+// Axis rays against the solvent-excluded surface (SES) of a union of balls, on the
+// device. The reference's default surface is the SES with probe radius 1.4 A
+// (surface_type 0, src/pb_class.cpp:956-963; data/options.prm:136-140), cast by
+// NanoShaper (src/raytracer.cpp:337-376), which is not vendored. This follows the
+// textbook definition (Richards 1977; Connolly 1983): with SAS balls B(a_i, R_i),
+// R_i = r_i + rp, and probe centres C = { q : |q - a_i| >= R_i for all i } (interior
+// cavities included), p is inside the SES iff dist(p, C) > rp. Inside an atom's own
+// ball p is inside; outside every SAS ball p is outside; otherwise the nearest point
+// of C is an exposed SAS-surface candidate: the radial projection of p on an SAS
+// sphere, the nearest point of an SAS circle (pair), or an SAS vertex (triple).
+//
+// Rays are the distinct vertex lines of the mesh (as for the sphere union). Along a
+// line the SES boundary lies on an atom sphere, a torus (tube rp around an SAS
+// circle) or a probe sphere (radius rp around an exposed SAS vertex); every such
+// intersection is a candidate, the candidates are sorted by (t, kind, id), and a
+// crossing is kept where the classification of the midpoints between consecutive
+// distinct candidates changes. Torus roots: the quartic in tau = t - c_axis is
+// sampled at 65 points on [-T, T] and each sign change bisected 64 times; candidates
+// are normalised by + 0.0 (no negative zero, so equal values have equal keys). Normals
+// point out of the molecule: (p - a)/r on atoms, (q - p)/rp towards the probe centre q
+// on tori and probe spheres. Only IEEE double + - * / sqrt are used, so the results
+// equal the CPU restatement's bit for bit.
+//
+// Work split: the SAS pairs (circles) and exposed SAS vertices are built on the host
+// from a cell grid of the atoms (ordered by atom indices); per axis, every line
+// gathers the atoms, pairs and vertices within reach into compact lists, counts and
+// writes its candidates, the candidates are sorted per line (segmented radix sorts:
+// by (kind, id), then stably by t), and one thread per line classifies the midpoints
+// and emits the crossings.
+#include <algorithm>
+#include <array>
+#include <cmath>
+#include <vector>
+
+#include <hipcub/hipcub.hpp>
+
+#include "ngpb_internal.h"
+
+namespace ngpb {
+
+namespace {
+
+struct SesPairD {
+  double c[3], n[3], rho;
+  int32_t i, j;
+};
+
+struct SesGrid2 {            // 2D bins of items by their transverse coordinates
+  const int32_t* ptr;        // gn0 * gn1 + 1
+  const int32_t* item;
+  double lo0, lo1, cell;
+  int gn0, gn1;
+};
+
+struct SesView {
+  const double* xyz;
+  const double* r;
+  int64_t na;
+  double rp;
+  const SesPairD* pairs;
+  const double* verts;       // 3 per exposed vertex
+  SesGrid2 ga, gp, gv;       // atoms, pairs (by circle centre), vertices
+};
+
+__device__ __forceinline__ uint64_t ses_bits(double x) {
+  const uint64_t b = (uint64_t)__double_as_longlong(x);
+  return (b >> 63) ? ~b : (b | 0x8000000000000000ULL);
+}
+__device__ __forceinline__ double ses_unbits(uint64_t k) {
+  const uint64_t b = (k >> 63) ? (k & 0x7fffffffffffffffULL) : ~k;
+  return __longlong_as_double((long long)b);
+}
+
+__device__ __forceinline__ double sq3(const double* a, const double* b) {
+  const double x = a[0] - b[0], y = a[1] - b[1], z = a[2] - b[2];
+  return x * x + y * y + z * z;
+}
+
+template <class F>
+__device__ __forceinline__ void ses_visit(const SesGrid2& G, double u, double v, F&& f) {
+  const int b0 = (int)floor((u - G.lo0) / G.cell), b1 = (int)floor((v - G.lo1) / G.cell);
+  for (int y = b1 - 1; y <= b1 + 1; ++y)
+    for (int x = b0 - 1; x <= b0 + 1; ++x) {
+      if (x < 0 || y < 0 || x >= G.gn0 || y >= G.gn1) continue;
+      const int64_t bin = (int64_t)y * G.gn0 + x;
+      for (int32_t k = G.ptr[bin]; k < G.ptr[bin + 1]; ++k) f(G.item[k]);
+    }
+}
+
+// reach tests of the line lists (a small margin keeps every item whose predicates
+// can hold somewhere on the line)
+__device__ __forceinline__ bool atom_in_reach(const SesView& S, int a, double u, double v, int d1, int d2) {
+  const double du = u - S.xyz[3 * a + d1], dv = v - S.xyz[3 * a + d2];
+  const double reach = S.r[a] + 2.0 * S.rp + 1e-6;
+  return du * du + dv * dv < reach * reach;
+}
+__device__ __forceinline__ bool pair_in_reach(const SesView& S, int k, double u, double v, int d1, int d2) {
+  const SesPairD& P = S.pairs[k];
+  const double du = u - P.c[d1], dv = v - P.c[d2];
+  const double reach = P.rho + S.rp + 1e-6;
+  return du * du + dv * dv < reach * reach;
+}
+__device__ __forceinline__ bool vert_in_reach(const SesView& S, int k, double u, double v, int d1, int d2) {
+  const double du = u - S.verts[3 * k + d1], dv = v - S.verts[3 * k + d2];
+  const double reach = S.rp + 1e-6;
+  return du * du + dv * dv < reach * reach;
+}
+
+__global__ void k_ses_list_count(int64_t nl, const uint64_t* __restrict__ lu, const uint64_t* __restrict__ lv,
+                                 SesView S, int d1, int d2, int64_t* __restrict__ ca, int64_t* __restrict__ cp,
+                                 int64_t* __restrict__ cv) {
+  const int64_t li = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (li >= nl) return;
+  const double u = ses_unbits(lu[li]), v = ses_unbits(lv[li]);
+  int64_t na = 0, np = 0, nv = 0;
+  ses_visit(S.ga, u, v, [&](int a) { na += atom_in_reach(S, a, u, v, d1, d2); });
+  ses_visit(S.gp, u, v, [&](int k) { np += pair_in_reach(S, k, u, v, d1, d2); });
+  ses_visit(S.gv, u, v, [&](int k) { nv += vert_in_reach(S, k, u, v, d1, d2); });
+  ca[li] = na;
+  cp[li] = np;
+  cv[li] = nv;
+}
+
+__global__ void k_ses_list_fill(int64_t nl, const uint64_t* __restrict__ lu, const uint64_t* __restrict__ lv,
+                                SesView S, int d1, int d2, const int64_t* __restrict__ oa,
+                                const int64_t* __restrict__ op, const int64_t* __restrict__ ov,
+                                int32_t* __restrict__ la, int32_t* __restrict__ lp, int32_t* __restrict__ lvv) {
+  const int64_t li = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (li >= nl) return;
+  const double u = ses_unbits(lu[li]), v = ses_unbits(lv[li]);
+  int64_t ia = oa[li], ip = op[li], iv = ov[li];
+  ses_visit(S.ga, u, v, [&](int a) {
+    if (atom_in_reach(S, a, u, v, d1, d2)) la[ia++] = a;
+  });
+  ses_visit(S.gp, u, v, [&](int k) {
+    if (pair_in_reach(S, k, u, v, d1, d2)) lp[ip++] = k;
+  });
+  ses_visit(S.gv, u, v, [&](int k) {
+    if (vert_in_reach(S, k, u, v, d1, d2)) lvv[iv++] = k;
+  });
+}
+
+struct LineLists {
+  const int32_t* la;
+  const int32_t* lp;
+  const int32_t* lv;
+  const int64_t* oa;  // nl + 1 each
+  const int64_t* op;
+  const int64_t* ov;
+};
+
+// the torus of pair P against the line (u, v): calls f(t) for every root, in increasing t
+template <class F>
+__device__ __forceinline__ void torus_roots(const SesPairD& P, double rp, int axis, int d1, int d2, double u, double v,
+                                            F&& f) {
+  const double wu = u - P.c[d1], wv = v - P.c[d2];
+  const double D2 = wu * wu + wv * wv;
+  const double T2 = (P.rho + rp) * (P.rho + rp) - D2;
+  if (!(T2 > 0.0)) return;
+  const double T = sqrt(T2);
+  const double s0 = P.n[d1] * wu + P.n[d2] * wv;
+  const double na = P.n[axis];
+  const double K = D2 + P.rho * P.rho - rp * rp;
+  const double fr = 4.0 * P.rho * P.rho;
+  auto g = [&](double tau) {
+    const double a = tau * tau + K;
+    const double b = na * tau + s0;
+    return a * a - fr * (tau * tau + D2 - b * b);
+  };
+  double t0 = -T, g0 = g(t0);
+  for (int m = 1; m <= 64; ++m) {
+    const double t1 = -T + (2.0 * T) * ((double)m / 64.0);
+    const double g1 = g(t1);
+    if (g0 == 0.0) {
+      f(P.c[axis] + t0 + 0.0);
+    } else if ((g0 < 0.0) != (g1 < 0.0) && g1 != 0.0) {
+      double lo = t0, hi = t1, glo = g0;
+      for (int it = 0; it < 64; ++it) {
+        const double mid = 0.5 * (lo + hi);
+        const double gm = g(mid);
+        if ((gm < 0.0) == (glo < 0.0)) {
+          lo = mid;
+          glo = gm;
+        } else {
+          hi = mid;
+        }
+      }
+      f(P.c[axis] + 0.5 * (lo + hi) + 0.0);
+    }
+    t0 = t1;
+    g0 = g1;
+  }
+  if (g0 == 0.0) f(P.c[axis] + t0 + 0.0);
+}
+
+// every candidate of line li: f(t, kind, id)
+template <class F>
+__device__ __forceinline__ void line_candidates(const SesView& S, const LineLists& L, int64_t li, int axis, int d1,
+                                                int d2, double u, double v, F&& f) {
+  for (int64_t k = L.oa[li]; k < L.oa[li + 1]; ++k) {
+    const int a = L.la[k];
+    const double du = u - S.xyz[3 * a + d1], dv = v - S.xyz[3 * a + d2];
+    const double q = S.r[a] * S.r[a] - (du * du + dv * dv);
+    if (!(q > 0.0)) continue;
+    const double s = sqrt(q);
+    f(S.xyz[3 * a + axis] - s + 0.0, 0, a);
+    f(S.xyz[3 * a + axis] + s + 0.0, 0, a);
+  }
+  for (int64_t k = L.ov[li]; k < L.ov[li + 1]; ++k) {
+    const int id = L.lv[k];
+    const double* c = S.verts + 3 * id;
+    const double du = u - c[d1], dv = v - c[d2];
+    const double q = S.rp * S.rp - (du * du + dv * dv);
+    if (!(q > 0.0)) continue;
+    const double s = sqrt(q);
+    f(c[axis] - s + 0.0, 2, id);
+    f(c[axis] + s + 0.0, 2, id);
+  }
+  for (int64_t k = L.op[li]; k < L.op[li + 1]; ++k) {
+    const int id = L.lp[k];
+    torus_roots(S.pairs[id], S.rp, axis, d1, d2, u, v, [&](double t) { f(t, 1, id); });
+  }
+}
+
+__global__ void k_ses_cand_count(int64_t nl, const uint64_t* __restrict__ lu, const uint64_t* __restrict__ lv,
+                                 SesView S, LineLists L, int axis, int d1, int d2, int64_t* __restrict__ cnt) {
+  const int64_t li = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (li >= nl) return;
+  int64_t c = 0;
+  line_candidates(S, L, li, axis, d1, d2, ses_unbits(lu[li]), ses_unbits(lv[li]), [&](double, int, int) { ++c; });
+  cnt[li] = c;
+}
+
+__global__ void k_ses_cand_fill(int64_t nl, const uint64_t* __restrict__ lu, const uint64_t* __restrict__ lv,
+                                SesView S, LineLists L, int axis, int d1, int d2, const int64_t* __restrict__ off,
+                                uint64_t* __restrict__ tkey, uint32_t* __restrict__ kind_id) {
+  const int64_t li = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (li >= nl) return;
+  int64_t o = off[li];
+  line_candidates(S, L, li, axis, d1, d2, ses_unbits(lu[li]), ses_unbits(lv[li]), [&](double t, int kind, int id) {
+    tkey[o] = ses_bits(t);
+    kind_id[o] = ((uint32_t)kind << 28) | (uint32_t)id;
+    ++o;
+  });
+}
+
+// nearest point of the SAS circle of P to p
+__device__ __forceinline__ void circle_point(const SesPairD& P, const double* p, double* q) {
+  double w[3] = {p[0] - P.c[0], p[1] - P.c[1], p[2] - P.c[2]};
+  const double s = w[0] * P.n[0] + w[1] * P.n[1] + w[2] * P.n[2];
+  double u[3] = {w[0] - s * P.n[0], w[1] - s * P.n[1], w[2] - s * P.n[2]};
+  double L = sqrt(u[0] * u[0] + u[1] * u[1] + u[2] * u[2]);
+  if (!(L > 0.0)) {
+    const int a = fabs(P.n[0]) < 0.9 ? 0 : 1;
+    double e[3] = {0, 0, 0};
+    e[a] = 1.0;
+    const double ds = e[0] * P.n[0] + e[1] * P.n[1] + e[2] * P.n[2];
+    for (int k = 0; k < 3; ++k) u[k] = e[k] - ds * P.n[k];
+    L = sqrt(u[0] * u[0] + u[1] * u[1] + u[2] * u[2]);
+  }
+  for (int k = 0; k < 3; ++k) q[k] = P.c[k] + P.rho * (u[k] / L);
+}
+
+__device__ __forceinline__ bool exposed_local(const SesView& S, const LineLists& L, int64_t li, const double* q, int e0,
+                                              int e1) {
+  for (int64_t k = L.oa[li]; k < L.oa[li + 1]; ++k) {
+    const int m = L.la[k];
+    if (m == e0 || m == e1) continue;
+    const double Rm = S.r[m] + S.rp;
+    if (sq3(q, S.xyz + 3 * m) < Rm * Rm) return false;
+  }
+  return true;
+}
+
+// 1 inside the SES, 0 outside, for a point p on line li
+__device__ int ses_classify(const SesView& S, const LineLists& L, int64_t li, const double* p) {
+  bool in_sas = false;
+  for (int64_t k = L.oa[li]; k < L.oa[li + 1]; ++k) {
+    const int i = L.la[k];
+    const double dd = sq3(p, S.xyz + 3 * i);
+    if (dd < S.r[i] * S.r[i]) return 1;
+    const double Ri = S.r[i] + S.rp;
+    if (dd < Ri * Ri) in_sas = true;
+  }
+  if (!in_sas) return 0;
+  const double rp2 = S.rp * S.rp;
+  for (int64_t k = L.oa[li]; k < L.oa[li + 1]; ++k) {
+    const int i = L.la[k];
+    const double* a = S.xyz + 3 * i;
+    const double dd = sqrt(sq3(p, a));
+    const double Ri = S.r[i] + S.rp;
+    const double gap = Ri - dd;
+    if (!(dd > 0.0) || fabs(gap) > S.rp) continue;
+    double q[3];
+    for (int c = 0; c < 3; ++c) q[c] = a[c] + Ri * ((p[c] - a[c]) / dd);
+    if (exposed_local(S, L, li, q, i, -1)) return 0;
+  }
+  for (int64_t k = L.op[li]; k < L.op[li + 1]; ++k) {
+    const SesPairD& P = S.pairs[L.lp[k]];
+    double q[3];
+    circle_point(P, p, q);
+    if (sq3(p, q) <= rp2 && exposed_local(S, L, li, q, P.i, P.j)) return 0;
+  }
+  for (int64_t k = L.ov[li]; k < L.ov[li + 1]; ++k)
+    if (sq3(p, S.verts + 3 * L.lv[k]) <= rp2) return 0;
+  return 1;
+}
+
+// Per line: walks its sorted candidates in groups of equal t and flags the first
+// candidate of every group across which the classification changes; cnt = crossings.
+__global__ void k_ses_classify(int64_t nl, const uint64_t* __restrict__ lu, const uint64_t* __restrict__ lv,
+                               SesView S, LineLists L, int axis, int d1, int d2, const int64_t* __restrict__ off,
+                               const uint64_t* __restrict__ tkey, uint8_t* __restrict__ flag,
+                               int64_t* __restrict__ cnt) {
+  const int64_t li = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (li >= nl) return;
+  const double u = ses_unbits(lu[li]), v = ses_unbits(lv[li]);
+  const int64_t b = off[li], e = off[li + 1];
+  int before = 0;
+  int64_t c = 0;
+  int64_t g = b;
+  while (g < e) {
+    int64_t h = g + 1;  // next group
+    while (h < e && tkey[h] == tkey[g]) flag[h++] = 0;
+    int after = 0;
+    if (h < e) {
+      double p[3];
+      p[axis] = 0.5 * (ses_unbits(tkey[g]) + ses_unbits(tkey[h]));
+      p[d1] = u;
+      p[d2] = v;
+      after = ses_classify(S, L, li, p);
+    }
+    const bool cross = after != before;
+    flag[g] = cross ? 1 : 0;
+    c += cross;
+    before = after;
+    g = h;
+  }
+  cnt[li] = c;
+}
+
+__global__ void k_ses_nonzero(int64_t n, const int64_t* __restrict__ c, uint8_t* __restrict__ f) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) f[i] = c[i] > 0 ? 1 : 0;
+}
+
+__global__ void k_ses_ray_keys(int64_t nr, const int32_t* __restrict__ kept, const uint64_t* __restrict__ lu,
+                               const uint64_t* __restrict__ lv, const int64_t* __restrict__ cnt,
+                               double* __restrict__ key, int64_t* __restrict__ ptr) {
+  const int64_t rr = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (rr >= nr) return;
+  const int64_t li = kept[rr];
+  key[2 * rr] = ses_unbits(lu[li]);
+  key[2 * rr + 1] = ses_unbits(lv[li]);
+  ptr[rr] = cnt[li];
+}
+
+__global__ void k_ses_emit(int64_t nr, const int32_t* __restrict__ kept, const int64_t* __restrict__ ptr,
+                           const int64_t* __restrict__ off, const uint64_t* __restrict__ tkey,
+                           const uint32_t* __restrict__ kind_id, const uint8_t* __restrict__ flag,
+                           const double* __restrict__ key, SesView S, int axis, int d1, int d2,
+                           double* __restrict__ inters, double* __restrict__ normals) {
+  const int64_t rr = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (rr >= nr) return;
+  const int64_t li = kept[rr];
+  int64_t o = ptr[rr];
+  for (int64_t k = off[li]; k < off[li + 1]; ++k) {
+    if (!flag[k]) continue;
+    const double t = ses_unbits(tkey[k]);
+    const int kind = (int)(kind_id[k] >> 28);
+    const int id = (int)(kind_id[k] & 0x0fffffffu);
+    double p[3], n[3];
+    p[axis] = t;
+    p[d1] = key[2 * rr];
+    p[d2] = key[2 * rr + 1];
+    if (kind == 0) {
+      const double* a = S.xyz + 3 * id;
+      for (int c = 0; c < 3; ++c) n[c] = (p[c] - a[c]) / S.r[id];
+    } else if (kind == 2) {
+      const double* a = S.verts + 3 * id;
+      for (int c = 0; c < 3; ++c) n[c] = (a[c] - p[c]) / S.rp;
+    } else {
+      double q[3];
+      circle_point(S.pairs[id], p, q);
+      for (int c = 0; c < 3; ++c) n[c] = (q[c] - p[c]) / S.rp;
+    }
+    inters[o] = t;
+    for (int c = 0; c < 3; ++c) normals[3 * o + c] = n[c];
+    ++o;
+  }
+}
+
+// ---------------------------------------------------------------- host side
+struct SesHost {
+  std::vector<SesPairD> pairs;
+  std::vector<double> verts;
+};
+
+double sqd(const double* a, const double* b) {
+  const double x = a[0] - b[0], y = a[1] - b[1], z = a[2] - b[2];
+  return x * x + y * y + z * z;
+}
+
+// SAS circles of overlapping pairs (i < j, in index order) and exposed SAS vertices
+// (per pair in that order, third atom k > j ascending, +z before -z)
+SesHost ses_build_host(int64_t na, const double* xyz, const double* r, double rp) {
+  SesHost H;
+  std::vector<double> R((size_t)na);
+  double Rmax = 0.0;
+  for (int64_t i = 0; i < na; ++i) {
+    R[(size_t)i] = r[i] + rp;
+    Rmax = std::max(Rmax, R[(size_t)i]);
+  }
+  if (na == 0) return H;
+  // cell grid of edge 2 Rmax: SAS balls that overlap lie in adjacent cells
+  const double cell = std::max(2.0 * Rmax, 1e-3);
+  double lo[3] = {1e300, 1e300, 1e300};
+  for (int64_t i = 0; i < na; ++i)
+    for (int d = 0; d < 3; ++d) lo[d] = std::min(lo[d], xyz[3 * i + d]);
+  std::vector<std::array<int, 3>> ci((size_t)na);
+  int gn[3] = {1, 1, 1};
+  for (int64_t i = 0; i < na; ++i)
+    for (int d = 0; d < 3; ++d) {
+      ci[(size_t)i][d] = (int)std::floor((xyz[3 * i + d] - lo[d]) / cell);
+      gn[d] = std::max(gn[d], ci[(size_t)i][d] + 1);
+    }
+  std::vector<std::vector<int>> grid((size_t)gn[0] * gn[1] * gn[2]);
+  for (int64_t i = 0; i < na; ++i)
+    grid[((size_t)ci[(size_t)i][2] * gn[1] + ci[(size_t)i][1]) * gn[0] + ci[(size_t)i][0]].push_back((int)i);
+  std::vector<std::vector<int>> over((size_t)na), nb((size_t)na);  // overlapping SAS balls; circle pairs
+  for (int64_t i = 0; i < na; ++i) {
+    const auto& c = ci[(size_t)i];
+    for (int z = c[2] - 1; z <= c[2] + 1; ++z)
+      for (int y = c[1] - 1; y <= c[1] + 1; ++y)
+        for (int x = c[0] - 1; x <= c[0] + 1; ++x) {
+          if (x < 0 || y < 0 || z < 0 || x >= gn[0] || y >= gn[1] || z >= gn[2]) continue;
+          for (int j : grid[((size_t)z * gn[1] + y) * gn[0] + x]) {
+            if (j == i) continue;
+            const double dd = std::sqrt(sqd(xyz + 3 * i, xyz + 3 * j));
+            if (dd < R[(size_t)i] + R[(size_t)j]) over[(size_t)i].push_back(j);
+          }
+        }
+    std::sort(over[(size_t)i].begin(), over[(size_t)i].end());
+  }
+  for (int64_t i = 0; i < na; ++i)
+    for (int j : over[(size_t)i]) {
+      if (j <= i) continue;
+      const double* ai = xyz + 3 * i;
+      const double* aj = xyz + 3 * j;
+      const double dd = std::sqrt(sqd(ai, aj));
+      if (!(dd < R[(size_t)i] + R[(size_t)j]) || !(dd > std::fabs(R[(size_t)i] - R[(size_t)j]))) continue;
+      const double t = (dd * dd + R[(size_t)i] * R[(size_t)i] - R[(size_t)j] * R[(size_t)j]) / (2.0 * dd);
+      const double rho2 = R[(size_t)i] * R[(size_t)i] - t * t;
+      if (!(rho2 > 0.0)) continue;
+      SesPairD P;
+      P.i = (int32_t)i;
+      P.j = (int32_t)j;
+      for (int k = 0; k < 3; ++k) {
+        P.n[k] = (aj[k] - ai[k]) / dd;
+        P.c[k] = ai[k] + t * P.n[k];
+      }
+      P.rho = std::sqrt(rho2);
+      H.pairs.push_back(P);
+      nb[(size_t)i].push_back(j);
+      nb[(size_t)j].push_back((int)i);
+    }
+  for (auto& v : nb) std::sort(v.begin(), v.end());
+  auto exposed = [&](const double* q, int e0, int e1, int e2) {
+    for (int m : over[(size_t)e0]) {  // any ball covering q overlaps ball e0
+      if (m == e1 || m == e2) continue;
+      if (sqd(q, xyz + 3 * m) < R[(size_t)m] * R[(size_t)m]) return false;
+    }
+    return true;
+  };
+  for (const SesPairD& P : H.pairs) {
+    for (int k : nb[(size_t)P.j]) {
+      if (k <= P.j || !std::binary_search(nb[(size_t)P.i].begin(), nb[(size_t)P.i].end(), k)) continue;
+      const double* ai = xyz + 3 * P.i;
+      const double* ak = xyz + 3 * k;
+      double ex[3] = {P.n[0], P.n[1], P.n[2]};
+      double w[3] = {ak[0] - ai[0], ak[1] - ai[1], ak[2] - ai[2]};
+      const double ii = ex[0] * w[0] + ex[1] * w[1] + ex[2] * w[2];
+      double ey[3] = {w[0] - ii * ex[0], w[1] - ii * ex[1], w[2] - ii * ex[2]};
+      const double ey_n = std::sqrt(ey[0] * ey[0] + ey[1] * ey[1] + ey[2] * ey[2]);
+      if (!(ey_n > 0.0)) continue;
+      for (int q = 0; q < 3; ++q) ey[q] = ey[q] / ey_n;
+      const double ez[3] = {ex[1] * ey[2] - ex[2] * ey[1], ex[2] * ey[0] - ex[0] * ey[2], ex[0] * ey[1] - ex[1] * ey[0]};
+      const double jj = ey[0] * w[0] + ey[1] * w[1] + ey[2] * w[2];
+      const double dd = std::sqrt(sqd(ai, xyz + 3 * P.j));
+      const double Ri = R[(size_t)P.i], Rj = R[(size_t)P.j], Rk = R[(size_t)k];
+      const double x = (Ri * Ri - Rj * Rj + dd * dd) / (2.0 * dd);
+      const double y = (Ri * Ri - Rk * Rk + ii * ii + jj * jj) / (2.0 * jj) - (ii / jj) * x;
+      const double z2 = Ri * Ri - x * x - y * y;
+      if (!(z2 > 0.0)) continue;
+      const double z = std::sqrt(z2);
+      for (int sgn = 0; sgn < 2; ++sgn) {
+        const double zz = sgn == 0 ? z : -z;
+        double v[3];
+        for (int q = 0; q < 3; ++q) v[q] = ai[q] + x * ex[q] + y * ey[q] + zz * ez[q];
+        if (exposed(v, P.i, P.j, k)) H.verts.insert(H.verts.end(), v, v + 3);
+      }
+    }
+  }
+  return H;
+}
+
+// 2D bins of n items with transverse coordinates (c1[k], c2[k]), items ascending
+void bin2(int64_t n, const std::vector<double>& c1, const std::vector<double>& c2, double cell,
+          std::vector<int32_t>& ptr, std::vector<int32_t>& item, double* lo, int* gn) {
+  lo[0] = lo[1] = 0.0;
+  double hi[2] = {0.0, 0.0};
+  if (n > 0) {
+    lo[0] = hi[0] = c1[0];
+    lo[1] = hi[1] = c2[0];
+  }
+  for (int64_t k = 0; k < n; ++k) {
+    lo[0] = std::min(lo[0], c1[(size_t)k]);
+    hi[0] = std::max(hi[0], c1[(size_t)k]);
+    lo[1] = std::min(lo[1], c2[(size_t)k]);
+    hi[1] = std::max(hi[1], c2[(size_t)k]);
+  }
+  gn[0] = (int)std::floor((hi[0] - lo[0]) / cell) + 1;
+  gn[1] = (int)std::floor((hi[1] - lo[1]) / cell) + 1;
+  ptr.assign((size_t)gn[0] * gn[1] + 1, 0);
+  std::vector<int32_t> bin((size_t)std::max<int64_t>(n, 1));
+  for (int64_t k = 0; k < n; ++k) {
+    const int b0 = (int)std::floor((c1[(size_t)k] - lo[0]) / cell);
+    const int b1 = (int)std::floor((c2[(size_t)k] - lo[1]) / cell);
+    bin[(size_t)k] = b1 * gn[0] + b0;
+    ++ptr[(size_t)bin[(size_t)k] + 1];
+  }
+  for (size_t i = 1; i < ptr.size(); ++i) ptr[i] += ptr[i - 1];
+  item.assign((size_t)std::max<int64_t>(n, 1), 0);
+  std::vector<int32_t> cur(ptr.begin(), ptr.end() - 1);
+  for (int64_t k = 0; k < n; ++k) item[(size_t)cur[(size_t)bin[(size_t)k]]++] = (int32_t)k;
+}
+
+}  // namespace
+
+// Device SES data of one molecule (built once per ngpb_cast_rays call).
+struct SesDevice {
+  DBuf<double> xyz, r, verts;
+  DBuf<SesPairD> pairs;
+  int64_t na = 0, npairs = 0, nverts = 0;
+  double rp = 0.0, reach_cell = 1.0;
+  explicit SesDevice(hipStream_t s) : xyz(s), r(s), verts(s), pairs(s) {}
+};
+
+SesDevice* ses_prepare(ngpb_ctx* c, int64_t na, const double* xyz, const double* r, double rp) {
+  SesHost H = ses_build_host(na, xyz, r, rp);
+  auto* D = new SesDevice(c->stream);
+  D->na = na;
+  D->rp = rp;
+  D->npairs = (int64_t)H.pairs.size();
+  D->nverts = (int64_t)H.verts.size() / 3;
+  double rmax = 0.0;
+  for (int64_t a = 0; a < na; ++a) rmax = std::max(rmax, r[a]);
+  D->reach_cell = std::max(rmax + 2.0 * rp + 1e-3, 1.0);  // every reach is below it
+  D->xyz.upload(na ? xyz : nullptr, (size_t)(3 * na), c->stream);
+  D->r.upload(na ? r : nullptr, (size_t)na, c->stream);
+  if (!na) {
+    D->xyz.resize(3);
+    D->r.resize(1);
+  }
+  if (D->npairs) D->pairs.upload(H.pairs.data(), H.pairs.size(), c->stream);
+  else D->pairs.resize(1);
+  if (D->nverts) D->verts.upload(H.verts.data(), H.verts.size(), c->stream);
+  else D->verts.resize(3);
+  NGPB_HIP(hipStreamSynchronize(c->stream));
+  return D;
+}
+
+void ses_release(SesDevice* D) { delete D; }
+
+void ses_counts(const SesDevice* D, int64_t* npairs, int64_t* nverts) {
+  *npairs = D->npairs;
+  *nverts = D->nverts;
+}
+
+// SES rays of the nl distinct lines (lu, lv) of one axis into c->rays[axis]
+void ses_cast_lines(ngpb_ctx* c, const SesDevice* D, int axis, int64_t nl, const uint64_t* lu, const uint64_t* lv,
+                    const double* h_xyz) {
+  const int d1 = axis == 0 ? 1 : 0;
+  const int d2 = axis == 2 ? 1 : 2;
+  const double cell = D->reach_cell;
+  // bins of atoms, pairs and vertices in the (d1, d2) plane
+  std::vector<double> a1((size_t)D->na), a2((size_t)D->na);
+  for (int64_t a = 0; a < D->na; ++a) {
+    a1[(size_t)a] = h_xyz[3 * a + d1];
+    a2[(size_t)a] = h_xyz[3 * a + d2];
+  }
+  std::vector<SesPairD> hp((size_t)D->npairs);
+  std::vector<double> hv((size_t)(3 * D->nverts));
+  if (D->npairs) D->pairs.download(hp.data(), hp.size(), c->stream);
+  if (D->nverts) D->verts.download(hv.data(), hv.size(), c->stream);
+  NGPB_HIP(hipStreamSynchronize(c->stream));
+  std::vector<double> p1((size_t)D->npairs), p2((size_t)D->npairs), v1((size_t)D->nverts), v2((size_t)D->nverts);
+  for (int64_t k = 0; k < D->npairs; ++k) {
+    p1[(size_t)k] = hp[(size_t)k].c[d1];
+    p2[(size_t)k] = hp[(size_t)k].c[d2];
+  }
+  for (int64_t k = 0; k < D->nverts; ++k) {
+    v1[(size_t)k] = hv[(size_t)(3 * k + d1)];
+    v2[(size_t)k] = hv[(size_t)(3 * k + d2)];
+  }
+  std::vector<int32_t> bpa, bia, bpp, bip, bpv, biv;
+  double loa[2], lop[2], lov[2];
+  int gna[2], gnp[2], gnv[2];
+  bin2(D->na, a1, a2, cell, bpa, bia, loa, gna);
+  bin2(D->npairs, p1, p2, cell, bpp, bip, lop, gnp);
+  bin2(D->nverts, v1, v2, cell, bpv, biv, lov, gnv);
+  DBuf<int32_t> dbpa(c->stream), dbia(c->stream), dbpp(c->stream), dbip(c->stream), dbpv(c->stream), dbiv(c->stream);
+  dbpa.upload(bpa.data(), bpa.size(), c->stream);
+  dbia.upload(bia.data(), bia.size(), c->stream);
+  dbpp.upload(bpp.data(), bpp.size(), c->stream);
+  dbip.upload(bip.data(), bip.size(), c->stream);
+  dbpv.upload(bpv.data(), bpv.size(), c->stream);
+  dbiv.upload(biv.data(), biv.size(), c->stream);
+  SesView S;
+  S.xyz = D->xyz.p;
+  S.r = D->r.p;
+  S.na = D->na;
+  S.rp = D->rp;
+  S.pairs = D->pairs.p;
+  S.verts = D->verts.p;
+  S.ga = SesGrid2{dbpa.p, dbia.p, loa[0], loa[1], cell, gna[0], gna[1]};
+  S.gp = SesGrid2{dbpp.p, dbip.p, lop[0], lop[1], cell, gnp[0], gnp[1]};
+  S.gv = SesGrid2{dbpv.p, dbiv.p, lov[0], lov[1], cell, gnv[0], gnv[1]};
+  const dim3 B(128), G((unsigned)blocks_for(nl, 128));
+  // per-line lists of the atoms, pairs and vertices within reach
+  DBuf<int64_t> oa(c->stream), op(c->stream), ov(c->stream);
+  oa.resize((size_t)nl + 1);
+  op.resize((size_t)nl + 1);
+  ov.resize((size_t)nl + 1);
+  hipLaunchKernelGGL(k_ses_list_count, G, B, 0, c->stream, nl, lu, lv, S, d1, d2, oa.p, op.p, ov.p);
+  NGPB_HIP(hipGetLastError());
+  const int64_t ta = exclusive_scan(c, oa.p, oa.p, nl);
+  const int64_t tp = exclusive_scan(c, op.p, op.p, nl);
+  const int64_t tv = exclusive_scan(c, ov.p, ov.p, nl);
+  NGPB_HIP(hipMemcpyAsync(oa.p + nl, &ta, sizeof(int64_t), hipMemcpyHostToDevice, c->stream));
+  NGPB_HIP(hipMemcpyAsync(op.p + nl, &tp, sizeof(int64_t), hipMemcpyHostToDevice, c->stream));
+  NGPB_HIP(hipMemcpyAsync(ov.p + nl, &tv, sizeof(int64_t), hipMemcpyHostToDevice, c->stream));
+  DBuf<int32_t> la(c->stream), lpp(c->stream), lvv(c->stream);
+  la.resize((size_t)std::max<int64_t>(ta, 1));
+  lpp.resize((size_t)std::max<int64_t>(tp, 1));
+  lvv.resize((size_t)std::max<int64_t>(tv, 1));
+  hipLaunchKernelGGL(k_ses_list_fill, G, B, 0, c->stream, nl, lu, lv, S, d1, d2, oa.p, op.p, ov.p, la.p, lpp.p, lvv.p);
+  NGPB_HIP(hipGetLastError());
+  const LineLists L{la.p, lpp.p, lvv.p, oa.p, op.p, ov.p};
+  // candidates, sorted per line by (t, kind, id)
+  DBuf<int64_t> coff(c->stream);
+  coff.resize((size_t)nl + 1);
+  hipLaunchKernelGGL(k_ses_cand_count, G, B, 0, c->stream, nl, lu, lv, S, L, axis, d1, d2, coff.p);
+  NGPB_HIP(hipGetLastError());
+  const int64_t nc = exclusive_scan(c, coff.p, coff.p, nl);
+  NGPB_HIP(hipMemcpyAsync(coff.p + nl, &nc, sizeof(int64_t), hipMemcpyHostToDevice, c->stream));
+  NGPB_REQUIRE(nc < INT32_MAX, NGPB_E_ARG, "too many SES candidates for one axis");
+  const size_t ncs = (size_t)std::max<int64_t>(nc, 1);
+  DBuf<uint64_t> tk(c->stream), tk2(c->stream);
+  DBuf<uint32_t> kid(c->stream), kid2(c->stream);
+  tk.resize(ncs);
+  tk2.resize(ncs);
+  kid.resize(ncs);
+  kid2.resize(ncs);
+  hipLaunchKernelGGL(k_ses_cand_fill, G, B, 0, c->stream, nl, lu, lv, S, L, axis, d1, d2, coff.p, tk.p, kid.p);
+  NGPB_HIP(hipGetLastError());
+  if (nc > 0) {
+    size_t tb1 = 0, tb2 = 0;
+    NGPB_HIP(hipcub::DeviceSegmentedRadixSort::SortPairs(nullptr, tb1, kid.p, kid2.p, tk.p, tk2.p, (int)nc, (int)nl,
+                                                         coff.p, coff.p + 1, 0, 32, c->stream));
+    NGPB_HIP(hipcub::DeviceSegmentedRadixSort::SortPairs(nullptr, tb2, tk2.p, tk.p, kid2.p, kid.p, (int)nc, (int)nl,
+                                                         coff.p, coff.p + 1, 0, 64, c->stream));
+    DBuf<uint8_t> tmp(c->stream);
+    tmp.resize(std::max(tb1, tb2));
+    // by (kind, id), then stably by t
+    NGPB_HIP(hipcub::DeviceSegmentedRadixSort::SortPairs(tmp.p, tb1, kid.p, kid2.p, tk.p, tk2.p, (int)nc, (int)nl,
+                                                         coff.p, coff.p + 1, 0, 32, c->stream));
+    NGPB_HIP(hipcub::DeviceSegmentedRadixSort::SortPairs(tmp.p, tb2, tk2.p, tk.p, kid2.p, kid.p, (int)nc, (int)nl,
+                                                         coff.p, coff.p + 1, 0, 64, c->stream));
+  }
+  DBuf<uint8_t> flag(c->stream);
+  flag.resize(ncs);
+  DBuf<int64_t> xcnt(c->stream);
+  xcnt.resize((size_t)nl);
+  hipLaunchKernelGGL(k_ses_classify, G, B, 0, c->stream, nl, lu, lv, S, L, axis, d1, d2, coff.p, tk.p, flag.p, xcnt.p);
+  NGPB_HIP(hipGetLastError());
+  // rays = lines with crossings
+  DBuf<uint8_t> keep(c->stream);
+  keep.resize((size_t)nl);
+  hipLaunchKernelGGL(k_ses_nonzero, G, B, 0, c->stream, nl, xcnt.p, keep.p);
+  DBuf<int32_t> kept(c->stream);
+  kept.resize((size_t)nl);
+  const int64_t nr = compact_flags(c, keep.p, nl, kept.p);
+  auto& R = c->rays[axis];
+  R.n = nr;
+  R.key.resize((size_t)2 * std::max<int64_t>(nr, 1));
+  R.ptr.resize((size_t)nr + 1);
+  if (nr > 0) {
+    hipLaunchKernelGGL(k_ses_ray_keys, dim3((unsigned)blocks_for(nr, 256)), dim3(256), 0, c->stream, nr, kept.p, lu,
+                       lv, xcnt.p, R.key.p, R.ptr.p);
+    R.k = exclusive_scan(c, R.ptr.p, R.ptr.p, nr);
+  } else {
+    R.k = 0;
+  }
+  NGPB_HIP(hipMemcpyAsync(R.ptr.p + nr, &R.k, sizeof(int64_t), hipMemcpyHostToDevice, c->stream));
+  R.inters.resize((size_t)std::max<int64_t>(R.k, 1));
+  R.normals.resize((size_t)3 * std::max<int64_t>(R.k, 1));
+  if (nr > 0)
+    hipLaunchKernelGGL(k_ses_emit, dim3((unsigned)blocks_for(nr, 128)), dim3(128), 0, c->stream, nr, kept.p, R.ptr.p,
+                       coff.p, tk.p, kid.p, flag.p, R.key.p, S, axis, d1, d2, R.inters.p, R.normals.p);
+  NGPB_HIP(hipGetLastError());
+  NGPB_HIP(hipStreamSynchronize(c->stream));
+}
+
+}  // namespace ngpb

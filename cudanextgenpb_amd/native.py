@@ -108,6 +108,7 @@ SIGNATURES = {
     "ngpb_energy": (_I, [_P, _P]),
     "ngpb_phi_on_atoms": (_I, [_P, _P]),
     "ngpb_cast_rays": (_I, [_P, _I64, _P, _P]),
+    "ngpb_cast_rays_ses": (_I, [_P, _I64, _P, _P, _D]),
     "ngpb_locate_atoms": (_I, [_P, _I64, _P, ctypes.c_double, _P]),
     "ngpb_build_octree": (_I, [_P, _P, ctypes.c_double, _P, _I, _P, _I64, _P, _P, _P]),
     "ngpb_get_mesh": (_I, [_P, _P, _P, _P, _P, _P, _P, _P]),
@@ -272,6 +273,13 @@ class Context:
         xyz = _c(xyz, np.float64)
         r = _c(r, np.float64)
         self._check(lib().ngpb_cast_rays(self._h, int(r.shape[0]), _p(xyz), _p(r)), "cast_rays")
+
+    def cast_rays_ses(self, xyz, r, probe: float = 1.4):
+        """Ray table of the solvent-excluded surface of the balls (probe radius
+        `probe`), cast on the device (replaces set_rays)."""
+        xyz = _c(xyz, np.float64)
+        r = _c(r, np.float64)
+        self._check(lib().ngpb_cast_rays_ses(self._h, int(r.shape[0]), _p(xyz), _p(r), float(probe)), "cast_rays_ses")
 
     def locate_atoms(self, xyz, tol: float):
         """Owner leaf of each atom (-1 outside), located on the device."""

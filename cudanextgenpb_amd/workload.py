@@ -263,17 +263,65 @@ def globular(seed: int = 20502, n_atoms: int = 2500, ball_radius: float = 20.0, 
                              domain_factor, bc_type, physics)
 
 
-def _molecule_problem(name, atoms, h_fine, levels, bands, domain_factor, bc_type, physics):
+def _molecule_box(atoms, h_fine, levels, domain_factor):
+    """Cubic box of domain_factor times the molecule's largest extent, centred on it,
+    rounded up to a whole number of coarsest leaves: (origin, n finest cells per axis)."""
     lo = (atoms.xyz - atoms.r[:, None]).min(axis=0)
     hi = (atoms.xyz + atoms.r[:, None]).max(axis=0)
     extent = float((hi - lo).max())
     center = 0.5 * (lo + hi)
     R = 1 << levels
     n = int(math.ceil(domain_factor * extent / (h_fine * R))) * R
-    edge = n * h_fine
-    origin = center - edge / 2
+    return center - n * h_fine / 2, n
+
+
+def _molecule_problem(name, atoms, h_fine, levels, bands, domain_factor, bc_type, physics):
+    origin, n = _molecule_box(atoms, h_fine, levels, domain_factor)
     mesh = make_mesh(origin, h_fine, [n, n, n], levels, bands[:levels], atoms)
     return _assemble_problem(name, physics or Physics(), mesh, atoms, bc_type)
+
+
+@dataclass
+class Spec:
+    """A molecule with the octree generator's parameters only, for requests whose mesh,
+    rays and owners are built on the device (ngpb_build_octree, ngpb_cast_rays,
+    ngpb_locate_atoms): nothing mesh-sized is made on the host."""
+    name: str
+    physics: Physics
+    atoms: Atoms
+    origin: np.ndarray
+    hf: float
+    n: np.ndarray
+    smax: int
+    bands: tuple
+    bc_type: int
+    atom_tol: float
+
+    def generator_params(self):
+        return self.origin, self.hf, self.n, self.smax, self.bands
+
+
+def molecule_spec(name, atoms, h_fine, levels, bands, domain_factor=3.0, bc_type=2,
+                  physics: Optional[Physics] = None) -> Spec:
+    origin, n = _molecule_box(atoms, h_fine, levels, domain_factor)
+    nn = np.array([n, n, n], np.int64)
+    return Spec(name, physics or Physics(), atoms, np.asarray(origin, np.float64), float(h_fine), nn, int(levels),
+                tuple(float(b) for b in bands[:levels]), bc_type, P4EST_TOL * n * h_fine)
+
+
+def globular_spec(seed: int = 20502, n_atoms: int = 2500, ball_radius: float = 20.0, min_sep: float = 2.0,
+                  h_fine: float = 0.5, levels: int = 2, fine_band: float = 39.0, mid_band: float = 45.0,
+                  qnet: float = -3.0, bc_type: int = 2) -> Spec:
+    """The globular configuration (same atoms and box as globular()) without a host mesh."""
+    atoms = make_atoms(seed, n_atoms, [0.0, 0.0, 0.0], [ball_radius] * 3, min_sep, 1.2, 2.0, 0.35, qnet)
+    return molecule_spec(f"globular_s{seed}", atoms, h_fine, levels, [fine_band, mid_band], 3.0, bc_type)
+
+
+def large_spec(seed: int = 40003, n_atoms: int = 40000, semi_axes=(30.0, 22.5, 22.5), h_fine: float = 0.35,
+               levels: int = 2, fine_band: float = 58.0, mid_band: float = 64.0, bc_type: int = 2) -> Spec:
+    """The large-complex configuration (same atoms and box as large()) without a host mesh."""
+    atoms = make_atoms(seed, n_atoms, [0.0, 0.0, 0.0], list(semi_axes), 0.0, 1.2, 2.0, 0.35, -20.0)
+    return molecule_spec(f"large_s{seed}", atoms, h_fine, levels, [fine_band, mid_band], 3.0, bc_type)
 
 
 def large(seed: int = 40003, n_atoms: int = 40000, semi_axes=(30.0, 22.5, 22.5), min_sep: float = 0.0,
@@ -310,6 +358,23 @@ def ligand(index: int, base_seed: int = 512, h: float = 0.4, physics: Optional[P
     origin = 0.5 * (lo + hi) - n * h / 2
     mesh = make_mesh(origin, h, [n, n, n], 0, [], atoms)
     return _assemble_problem(f"ligand_{index}", physics or Physics(), mesh, atoms, bc_type)
+
+
+def ligand_spec(index: int, base_seed: int = 512, h: float = 0.4, bc_type: int = 2) -> Spec:
+    """Ligand `index` of the batched configuration (same atoms and box as ligand())
+    without a host mesh: uniform leaves, no refinement levels."""
+    seed = base_seed * 1000 + index
+    rng = np.random.Generator(np.random.PCG64(seed))
+    n_atoms = int(rng.integers(20, 81))
+    ball = float(rng.uniform(6.0, 10.0))
+    atoms = make_atoms(seed, n_atoms, [0.0, 0.0, 0.0], [ball] * 3, 1.2, 1.2, 1.9, 0.3, float("nan"))
+    lo = (atoms.xyz - atoms.r[:, None]).min(axis=0)
+    hi = (atoms.xyz + atoms.r[:, None]).max(axis=0)
+    extent = float((hi - lo).max())
+    n = int(min(96, max(48, 4 * math.ceil(1.6 * extent / (4 * h)))))
+    origin = 0.5 * (lo + hi) - n * h / 2
+    return Spec(f"ligand_{index}", Physics(), atoms, np.asarray(origin, np.float64), float(h),
+                np.array([n, n, n], np.int64), 0, (), bc_type, P4EST_TOL * n * h)
 
 
 def cluster(atoms_xyz, radii, charges, h_fine: float = 0.5, levels: int = 2, bands=(1.5, 4.0),

@@ -181,6 +181,16 @@ int ngpb_set_rays(ngpb_ctx* ctx, int axis, int64_t n_rays, const double* key, co
    omitted. Replaces the casting behind ray_cache_t (src/raytracer.cpp:22-63, the
    castAxisOrientedRay calls at :337-467) for the analytic union of atom spheres. */
 int ngpb_cast_rays(ngpb_ctx* ctx, int64_t n_balls, const double* xyz, const double* r);
+/* The same lines cast against the solvent-excluded surface (SES) of the balls with
+   probe radius probe_radius (the reference's default surface: surface_type 0,
+   probe 1.4 A, src/pb_class.cpp:956-963, cast by NanoShaper at src/raytracer.cpp:337-376).
+   A point is inside the SES iff no probe ball centred outside every solvent-accessible
+   ball (radius r + probe; interior cavities count as solvent) reaches it. Crossings
+   lie on atom spheres, re-entrant tori and probe spheres at SAS vertices; normals
+   point out of the molecule. Torus intersections are found by sampling the quartic
+   at 65 points and bisecting each sign change, so two torus crossings closer than
+   1/64 of the torus extent along the line are not resolved. */
+int ngpb_cast_rays_ses(ngpb_ctx* ctx, int64_t n_balls, const double* xyz, const double* r, double probe_radius);
 /* Owner leaf of each atom, located on the device as search_points / cerca_atomo /
    controlla_coordinate build lookup_table (src/pb_class.cpp:3633-3729): the first
    leaf, in leaf order, with corner0 - tol < x <= corner7 - tol on every axis; -1 when

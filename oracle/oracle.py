@@ -61,6 +61,9 @@ def lib():
         L.orc_pcg_time_iters.argtypes = [_I64, _P, _P, _P, _P, ctypes.c_int, ctypes.c_int]
         L.orc_pcg_time_iters.restype = _D
         L.orc_energy.argtypes = [_P, _P, _P, _P, _I64, _P, _I64, _P, _P, _D, _D, _D, _D, ctypes.c_int, _P]
+        L.orc_ses_classify.argtypes = [_I64, _P, _P, _D, _I64, _P, _P]
+        L.orc_ses_rays.argtypes = [_I64, _P, _P, _D, ctypes.c_int, _I64, _P, ctypes.c_int, _P, _P, _P, _P]
+        L.orc_ses_rays.restype = _I64
         L.orc_coul_energy.argtypes = [_I64, _P, _P, _D]
         L.orc_coul_energy.restype = _D
         L.orc_analytic_solution.argtypes = [_D] * 7
@@ -336,3 +339,28 @@ def full_pipeline(problem, tol=1e-10, maxiter=100000, nthreads=1):
     E = energy(problem, eps, phi, border)
     return dict(eps=eps, reac=reac, leaf_marker=lm, border=border, frac=frac, rhs_charge=rc, system=S,
                 phi=phi, iters=iters, relres=relres, status=st, energy=E)
+
+
+def ses_classify(xyz, r, rp, pts):
+    """1 where a point lies inside the solvent-excluded surface of the balls (probe rp)."""
+    xyz, r, pts = _c(xyz, np.float64), _c(r, np.float64), _c(pts, np.float64)
+    out = np.zeros(len(pts), np.int32)
+    lib().orc_ses_classify(len(r), _p(xyz), _p(r), rp, len(pts), _p(pts), _p(out))
+    return out
+
+
+def ses_rays(xyz, r, rp, axis, uv, cap=64):
+    """SES crossings of axis lines at transverse coordinates uv (L x 2): a list of
+    (inters, normals) per line, plus the numbers of SAS pairs and exposed vertices."""
+    xyz, r, uv = _c(xyz, np.float64), _c(r, np.float64), _c(uv, np.float64)
+    L = len(uv)
+    cnt = np.zeros(L, np.int32)
+    xs = np.zeros(L * cap)
+    ns = np.zeros(L * cap * 3)
+    nv = np.zeros(1, np.int64)
+    npairs = lib().orc_ses_rays(len(r), _p(xyz), _p(r), rp, axis, L, _p(uv), cap, _p(cnt), _p(xs), _p(ns), _p(nv))
+    if cnt.max(initial=0) > cap:
+        raise ValueError("more crossings on a line than the capacity")
+    out = [(xs[l * cap:l * cap + cnt[l]].copy(), ns[3 * l * cap:3 * (l * cap + cnt[l])].reshape(-1, 3).copy())
+           for l in range(L)]
+    return out, int(npairs), int(nv[0])

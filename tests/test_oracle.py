@@ -592,3 +592,116 @@ def test_mc_table_structure():
         assert used == crossing, ci
         # complementary classification crosses the same edges
         assert {int(ch, 16) for ch in strings[255 - ci]} == crossing
+
+
+# ---------------------------------------------------------------- SES rays (surface row)
+RP = 1.4
+
+
+def _ses_molecule(seed=3, n=9):
+    rng = np.random.default_rng(seed)
+    xyz = rng.uniform(-3.0, 3.0, (n, 3))
+    r = rng.uniform(1.2, 1.9, n)
+    return xyz, r
+
+
+def test_ses_single_atom_is_its_sphere():
+    xyz, r = np.zeros((1, 3)), np.array([2.0])
+    uv = np.array([[0.5, 0.3], [1.9, 0.2], [2.5, 0.0]])
+    rays, npairs, nv = O.ses_rays(xyz, r, RP, 2, uv)
+    assert npairs == 0 and nv == 0
+    for (xs, ns), (u, v) in zip(rays, uv):
+        q = 4.0 - u * u - v * v
+        if q <= 0:
+            assert xs.size == 0
+            continue
+        np.testing.assert_allclose(xs, [-np.sqrt(q), np.sqrt(q)], rtol=0, atol=1e-15)
+        np.testing.assert_allclose(ns, [[u / 2, v / 2, -np.sqrt(q) / 2], [u / 2, v / 2, np.sqrt(q) / 2]], atol=1e-15)
+
+
+@pytest.mark.parametrize("d", [2.5, 3.0, 4.5, 6.0])
+def test_ses_two_atom_neck_closed_form(d):
+    """Two atoms of radius 2 at distance d: in the mid-plane the re-entrant torus
+    (major radius rho = sqrt((2 + rp)^2 - d^2/4), tube rp) bounds the SES at distance
+    rho - rp from the axis; beyond 2 (2 + rp) apart the SAS balls do not meet and the
+    mid-plane line misses the molecule."""
+    xyz = np.array([[0.0, 0.0, -d / 2], [0.0, 0.0, d / 2]])
+    r = np.array([2.0, 2.0])
+    rays, npairs, nv = O.ses_rays(xyz, r, RP, 0, np.array([[0.0, 0.0]]))  # x line through the mid-point
+    xs, ns = rays[0]
+    rho2 = (2.0 + RP) ** 2 - d * d / 4
+    want = np.sqrt(rho2) - RP
+    if rho2 <= 0 or want <= 0:
+        assert xs.size == 0
+        return
+    assert npairs == 1
+    np.testing.assert_allclose(xs, [-want, want], rtol=0, atol=1e-12)
+    np.testing.assert_allclose(ns, [[-1, 0, 0], [1, 0, 0]], atol=1e-12)
+    # the neck is wider than the atoms' own cross-section in the mid-plane
+    assert want >= np.sqrt(max(4.0 - d * d / 4, 0.0))
+
+
+def _probe_free_distance(xyz, r, p, h=0.07, reach=RP + 0.25):
+    """Brute force: distance from p to the nearest sampled probe centre outside every
+    SAS ball, over a cubic lattice of spacing h within `reach` of p (inf if none)."""
+    g = np.arange(-reach, reach + 1e-12, h)
+    q = p + np.stack(np.meshgrid(g, g, g, indexing="ij"), -1).reshape(-1, 3)
+    q = q[((q - p) ** 2).sum(1) <= reach * reach]
+    R = r + RP
+    free = (((q[:, None, :] - xyz[None, :, :]) ** 2).sum(-1) >= R[None, :] ** 2).all(1)
+    if not free.any():
+        return np.inf
+    return float(np.sqrt(((q[free] - p) ** 2).sum(1)).min())
+
+
+def test_ses_classification_matches_probe_brute_force():
+    """Inside iff no probe ball centred outside the SAS reaches the point: where the
+    oracle says inside, no sampled free probe centre lies within rp; where it says
+    outside, one lies within rp plus the lattice's half diagonal."""
+    xyz, r = _ses_molecule()
+    rng = np.random.default_rng(11)
+    pts = rng.uniform(-5.5, 5.5, (400, 3))
+    cls = O.ses_classify(xyz, r, RP, pts)
+    h = 0.07
+    slack = h * np.sqrt(3) / 2
+    tested = {0: 0, 1: 0}
+    for p, c in zip(pts, cls):
+        d = _probe_free_distance(xyz, r, p, h)
+        if c == 1:
+            assert d > RP, (p, d)
+        else:
+            inside_sas = (((p - xyz) ** 2).sum(1) < (r + RP) ** 2).any()
+            if inside_sas:
+                assert d <= RP + slack, (p, d)
+        tested[int(c)] += 1
+    assert tested[0] > 50 and tested[1] > 50
+    # the SES encloses the atoms and lies inside the SAS
+    assert (O.ses_classify(xyz, r, RP, xyz) == 1).all()
+
+
+def test_ses_rays_agree_with_classification():
+    """Along every line, a point is inside iff an odd number of crossings lies below it
+    (checked away from the crossings); normals are unit and point out of the molecule
+    (the classification just outside along the normal is 0, just inside 1)."""
+    xyz, r = _ses_molecule()
+    g = np.arange(-5.0, 5.01, 0.45)
+    uv = np.array([(u, v) for u in g for v in g])
+    for axis in range(3):
+        rays, npairs, nv = O.ses_rays(xyz, r, RP, axis, uv)
+        assert npairs > 0 and nv > 0
+        d1, d2 = (1, 2) if axis == 0 else (0, 2) if axis == 1 else (0, 1)
+        for (xs, ns), (u, v) in zip(rays, uv):
+            assert xs.size % 2 == 0 and np.all(np.diff(xs) > 0)
+            t = np.linspace(-7, 7, 57)
+            keep = np.min(np.abs(t[:, None] - xs[None, :]), axis=1, initial=np.inf) > 1e-6
+            t = t[keep]
+            p = np.zeros((t.size, 3))
+            p[:, axis], p[:, d1], p[:, d2] = t, u, v
+            parity = (np.sum(xs[None, :] <= t[:, None], axis=1) % 2).astype(np.int32)
+            assert np.array_equal(parity, O.ses_classify(xyz, r, RP, p)), (axis, u, v)
+            if xs.size:
+                np.testing.assert_allclose(np.linalg.norm(ns, axis=1), 1.0, atol=1e-9)
+                pc = np.zeros((xs.size, 3))
+                pc[:, axis], pc[:, d1], pc[:, d2] = xs, u, v
+                assert (O.ses_classify(xyz, r, RP, pc + 1e-5 * ns) == 0).all()
+                assert (O.ses_classify(xyz, r, RP, pc - 1e-5 * ns) == 1).all()
