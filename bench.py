@@ -201,7 +201,7 @@ def bench_batch(args):
     }
 
 
-def serving_leg(n_requests, rate, tol, device_mesh):
+def serving_leg(n_requests, rate, tol, device_mesh, probe=0.0):
     """Serving stream: Poisson arrivals, continuous batching with HBM-footprint
     admission, against the same stream served one at a time in FP64. Returns the two
     summaries, the peak admitted bytes, the largest relative energy difference and
@@ -215,18 +215,18 @@ def serving_leg(n_requests, rate, tol, device_mesh):
     log(f"{len(reqs)} requests generated in {gen_s:.1f} s: N {min(r.n for r in reqs)}..{max(r.n for r in reqs)}, "
         f"total {tot_n / 1e6:.1f}M unknowns, span {reqs[-1].arrival:.1f} s")
     srv = serve.Server(budget_bytes=250e9, small_max_unknowns=1_500_000, max_batch=256, tol=tol,
-                       device_mesh=device_mesh)
+                       device_mesh=device_mesh, probe=probe)
     # warm-up: one small request through both paths
     warm = [r for r in reqs if r.n <= 200_000][:1]
     if warm:
         srv.run(warm, realtime=False)
-        serve.run_sequential_fp64(warm, realtime=False, tol=tol, device_mesh=device_mesh)
+        serve.run_sequential_fp64(warm, realtime=False, tol=tol, device_mesh=device_mesh, probe=probe)
     res = srv.run(reqs, realtime=True)
     peak = srv.peak_bytes
     srv.close()
     summ = serve.summarize(res)
     log(f"continuous batching: {summ}")
-    seq = serve.run_sequential_fp64(reqs, realtime=True, tol=tol, device_mesh=device_mesh)
+    seq = serve.run_sequential_fp64(reqs, realtime=True, tol=tol, device_mesh=device_mesh, probe=probe)
     seq_summ = serve.summarize(seq)
     log(f"sequential FP64: {seq_summ}")
     ok = [k for k in res if res[k].energy is not None and seq[k].energy is not None]
@@ -237,7 +237,7 @@ def serving_leg(n_requests, rate, tol, device_mesh):
 
 def bench_serve(args):
     dm = args.device_mesh
-    L = serving_leg(args.requests, args.rate, args.tol, dm)
+    L = serving_leg(args.requests, args.rate, args.tol, dm, args.probe)
     summ, seq_summ, peak = L["summary"], L["fp64"], L["peak"]
     reqs_n, tot_n, warm = L["requests"], L["unknowns"], L["warm"]
     return {
@@ -288,7 +288,7 @@ def bench_mesh(args):
     log(f"host: problem {host_problem_s:.2f} s, mesh alone {host_mesh_s:.3f} s; N={m.n_nodes} L={m.n_leaves}")
     ctx = native.Context(0)
     ctx.set_physics(P.physics)
-    tb, tr, tl = [], [], []
+    tb, tr, tl, ts = [], [], [], []
     for it in range(args.warmup + args.steps):
         t0 = time.perf_counter()
         sizes = ctx.build_octree(m.origin, m.hf, m.n, m.smax, m.bands, a.xyz, a.r)
@@ -297,10 +297,18 @@ def bench_mesh(args):
         t2 = time.perf_counter()
         owner = ctx.locate_atoms(a.xyz, P.atom_tol)
         t3 = time.perf_counter()
+        if args.probe > 0.0:
+            ctx.cast_rays_ses(a.xyz, a.r, args.probe)
+        t4 = time.perf_counter()
+        log(f"  build {it + 1}/{args.warmup + args.steps}: octree {(t1 - t0) * 1e3:.1f} ms, rays {(t2 - t1) * 1e3:.1f} ms, "
+            f"owners {(t3 - t2) * 1e3:.1f} ms" + (f", SES rays {(t4 - t3) * 1e3:.1f} ms" if args.probe > 0 else ""))
         if it >= args.warmup:
             tb.append(t1 - t0)
             tr.append(t2 - t1)
             tl.append(t3 - t2)
+            ts.append(t4 - t3)
+    ses_rays = [int(ctx.get_rays(axis)[1].size - 1) for axis in range(3)] if args.probe > 0.0 else None
+    ctx.cast_rays(a.xyz, a.r)  # the sphere-union table is what the host caster is compared with
     g = ctx.get_mesh()
     same = all(np.array_equal(g[k], getattr(m, k))
                for k in ("node_xyz", "hang_xyz", "hang_np", "hang_par", "leaf_corner", "bnd_nodes"))
@@ -335,6 +343,8 @@ def bench_mesh(args):
         "octree_ms": med(tb),
         "rays_ms": med(tr),
         "locate_ms": med(tl),
+        "ses_rays_ms": med(ts) if args.probe > 0.0 else None,
+        "ses_rays_per_axis": ses_rays,
         "identical_to_host": True,
         "cpu_baseline": {"value": host_mesh_s * 1e3, "unit": "ms", "kind": "port",
                          "cores": int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or (os.cpu_count() or 1),
@@ -346,7 +356,7 @@ def bench_mesh(args):
 REQUEST_STAGES = ("octree", "rays", "markers", "locate", "charges", "dirichlet", "assemble", "solve", "energy")
 
 
-def request_leg(P, warmup, steps, tol, check_every):
+def request_leg(P, warmup, steps, tol, check_every, probe=0.0):
     """Requests from atoms to energy on fresh contexts (octree, rays, markers, owners,
     Dirichlet data, assembly, mixed solve, energy); per-request stage times in ms and
     the energy (identical across requests)."""
@@ -361,7 +371,10 @@ def request_leg(P, warmup, steps, tol, check_every):
         tt = [time.perf_counter()]
         c.build_octree(m.origin, m.hf, m.n, m.smax, m.bands, a.xyz, a.r)
         tt.append(time.perf_counter())
-        c.cast_rays(a.xyz, a.r)
+        if probe > 0.0:
+            c.cast_rays_ses(a.xyz, a.r, probe)
+        else:
+            c.cast_rays(a.xyz, a.r)
         tt.append(time.perf_counter())
         c.create_markers()
         tt.append(time.perf_counter())
@@ -396,7 +409,7 @@ def bench_request(args):
     P = W.large() if big else W.globular()
     m = P.mesh
     stages = REQUEST_STAGES
-    r, energy = request_leg(P, args.warmup, args.steps, args.tol, args.check_every)
+    r, energy = request_leg(P, args.warmup, args.steps, args.tol, args.check_every, args.probe)
     tot = r.sum(axis=1)
     return {
         "metric": "request latency from atoms to energy (mesh, assembly, mixed solve, energy on the GPU)",
@@ -487,6 +500,9 @@ def main():
     ap.add_argument("--device-mesh", action="store_true",
                     help="serve workload: requests carry only atoms and box; mesh, rays and owners built on the GPU")
     ap.add_argument("--tol", type=float, default=1e-10)
+    ap.add_argument("--probe", type=float, default=1.4,
+                    help="probe radius of the solvent-excluded surface the rays are cast against (0: the union of "
+                         "the atom spheres)")
     ap.add_argument("--check-every", type=int, default=32)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
@@ -541,8 +557,12 @@ def main():
     log(f"workload {P.name}: N={m.n_nodes} leaves={m.n_leaves} hanging={m.n_hang} atoms={P.atoms.n} "
         f"(host generation {time.perf_counter() - t:.1f} s)")
 
+    # the Born configuration's surface is the sphere itself; the molecules' is the SES
+    probe = 0.0 if args.workload == "born" else args.probe
+    surface = (f"solvent-excluded surface, probe {probe:g} A, rays cast on the device" if probe > 0
+               else "union of the atom spheres")
     t = time.perf_counter()
-    pb = PoissonBoltzmann(P)
+    pb = PoissonBoltzmann(P, probe_radius=probe)
     pb.create_markers()
     pb.create_density_map()
     pb.assemple_system_matrix()
@@ -584,6 +604,7 @@ def main():
         "data": "synthetic",
         "config": {
             "workload": wname,
+            "surface": surface,
             "unknowns": n,
             "nnz": ctx.nnz,
             "leaves": m.n_leaves,
@@ -641,7 +662,7 @@ def main():
     if not args.no_serve and args.workload == "globular":
         # the metric's latency: a request stream (config 5) and one request from atoms
         log("request from atoms to energy (globular, fresh context per request)")
-        r, _ = request_leg(P, 1, 5, args.tol, args.check_every)
+        r, _ = request_leg(P, 1, 5, args.tol, args.check_every, probe)
         tot = r.sum(axis=1)
         out["request_from_atoms"] = {
             "median_ms": float(np.median(tot)),
@@ -650,7 +671,7 @@ def main():
             "stages_ms": {k: float(np.median(r[:, i])) for i, k in enumerate(REQUEST_STAGES)},
         }
         log(f"serving stream: {args.requests} requests at {args.rate:g}/s, atoms in [50, 20000], 0.5 A leaves")
-        L = serving_leg(args.requests, args.rate, args.tol, True)
+        L = serving_leg(args.requests, args.rate, args.tol, True, probe)
         summ = L["summary"]
         out["p50_latency_ms"] = summ["p50_latency_ms"]
         out["p99_latency_ms"] = summ["p99_latency_ms"]

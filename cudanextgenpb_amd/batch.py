@@ -34,11 +34,12 @@ class Prepared:
 
 
 def prepare(problem: Problem, device: int = 0, ctx: Optional[native.Context] = None,
-            device_mesh: bool = False) -> Prepared:
+            device_mesh: bool = False, probe: float = 0.0) -> Prepared:
     """Uploads the problem and assembles it on the device (no solve). device_mesh: only
     the atoms and the box parameters cross to the device; the octree with its boundary
     list, the ray table and the owner leaves are built there (identical to the
-    problem's host-built ones)."""
+    problem's host-built ones). probe > 0: the rays are cast on the device against the
+    solvent-excluded surface with that probe radius."""
     t0 = time.perf_counter()
     c = ctx or native.Context(device)
     from .workload import Spec
@@ -49,12 +50,18 @@ def prepare(problem: Problem, device: int = 0, ctx: Optional[native.Context] = N
         if device_mesh:
             gen = problem if isinstance(problem, Spec) else problem.mesh
             c.build_octree(*gen.generator_params(), a.xyz, a.r)
-            c.cast_rays(a.xyz, a.r)
+            if probe > 0.0:
+                c.cast_rays_ses(a.xyz, a.r, probe)
+            else:
+                c.cast_rays(a.xyz, a.r)
             c.create_markers()
             c.set_charges(a.xyz, a.q, c.locate_atoms(a.xyz, problem.atom_tol))
         else:
             c.set_mesh(problem.mesh)
-            c.set_rays(problem.rays)
+            if probe > 0.0:
+                c.cast_rays_ses(a.xyz, a.r, probe)
+            else:
+                c.set_rays(problem.rays)
             c.create_markers()
             c.set_charges(a.xyz, a.q, problem.owner)
             c.set_dirichlet(problem.mesh.bnd_nodes, None)

@@ -967,7 +967,7 @@ int ngpb_set_compact_options(ngpb_ctx* c, int constant_slices) {
 int ngpb_set_spmv_schedule(ngpb_ctx* c, int schedule) {
   return guarded(c, [&] {
     not_in_batch(c);
-    if (schedule < 0 || schedule > 4) throw Error(NGPB_E_ARG, "schedule must be 0, 1, 2, 3 or 4");
+    if (schedule < 0 || schedule > 5) throw Error(NGPB_E_ARG, "schedule must be 0 .. 5");
     c->spmv_sched = schedule;
   });
 }

@@ -511,7 +511,7 @@ __global__ void __launch_bounds__(kBS) k_sell_packed(Sell A, const int32_t* __re
 // interleave within it; every XCD's L2 then sees the input vector entries of its own
 // row range instead of entries from all over the vector. Needs a grid that is a
 // multiple of 8.
-template <bool DOT, bool XCD>
+template <bool DOT, bool XCD, bool NT = false>
 __global__ void __launch_bounds__(kBS) k_sell_both(Sell A, const int32_t* __restrict__ dlist, int64_t dcnt,
                                                    const int32_t* __restrict__ plist, int64_t pcnt,
                                                    const float* __restrict__ x, float* __restrict__ y,
@@ -545,31 +545,31 @@ __global__ void __launch_bounds__(kBS) k_sell_both(Sell A, const int32_t* __rest
       bool done = false;
       if (s1 == s0 + 1 && w0 == w1) {
         switch (w0) {
-          case -4: delta_pair<4>(A, s0, b0, b1, x, y, &part); done = true; break;
-          case -5: delta_pair<5>(A, s0, b0, b1, x, y, &part); done = true; break;
-          case -6: delta_pair<6>(A, s0, b0, b1, x, y, &part); done = true; break;
-          case -7: delta_pair<7>(A, s0, b0, b1, x, y, &part); done = true; break;
-          case -8: delta_pair<8>(A, s0, b0, b1, x, y, &part); done = true; break;
-          case -36: delta_pair<4, true>(A, s0, b0, b1, x, y, &part); done = true; break;
-          case -37: delta_pair<5, true>(A, s0, b0, b1, x, y, &part); done = true; break;
-          case -38: delta_pair<6, true>(A, s0, b0, b1, x, y, &part); done = true; break;
-          case -39: delta_pair<7, true>(A, s0, b0, b1, x, y, &part); done = true; break;
-          case -40: delta_pair<8, true>(A, s0, b0, b1, x, y, &part); done = true; break;
+          case -4: delta_pair<4, false, NT>(A, s0, b0, b1, x, y, &part); done = true; break;
+          case -5: delta_pair<5, false, NT>(A, s0, b0, b1, x, y, &part); done = true; break;
+          case -6: delta_pair<6, false, NT>(A, s0, b0, b1, x, y, &part); done = true; break;
+          case -7: delta_pair<7, false, NT>(A, s0, b0, b1, x, y, &part); done = true; break;
+          case -8: delta_pair<8, false, NT>(A, s0, b0, b1, x, y, &part); done = true; break;
+          case -36: delta_pair<4, true, NT>(A, s0, b0, b1, x, y, &part); done = true; break;
+          case -37: delta_pair<5, true, NT>(A, s0, b0, b1, x, y, &part); done = true; break;
+          case -38: delta_pair<6, true, NT>(A, s0, b0, b1, x, y, &part); done = true; break;
+          case -39: delta_pair<7, true, NT>(A, s0, b0, b1, x, y, &part); done = true; break;
+          case -40: delta_pair<8, true, NT>(A, s0, b0, b1, x, y, &part); done = true; break;
           default: break;
         }
       }
       if (!done) {
-        delta_slice(A, s0, w0, b0, x, y, &part);
-        if (s1 >= 0) delta_slice(A, s1, w1, b1, x, y, &part);
+        delta_slice<NT>(A, s0, w0, b0, x, y, &part);
+        if (s1 >= 0) delta_slice<NT>(A, s1, w1, b1, x, y, &part);
       }
       td += W;
     }
     if (tp < phi) {
       const int64_t s = plist[tp];
       const PkHdr h = A.pk_hdr[s];
-      if (h.nd <= kW) packed_slice_h<1>(A, s, h, x, y, &part);
-      else if (h.nd <= 2 * kW) packed_slice_h<2>(A, s, h, x, y, &part);
-      else packed_slice_h<4>(A, s, h, x, y, &part);
+      if (h.nd <= kW) packed_slice_h<1, NT>(A, s, h, x, y, &part);
+      else if (h.nd <= 2 * kW) packed_slice_h<2, NT>(A, s, h, x, y, &part);
+      else packed_slice_h<4, NT>(A, s, h, x, y, &part);
       tp += W;
     }
   }
@@ -824,6 +824,10 @@ void launch_sell_t(ngpb_ctx* c, const float* x, float* y, double* part) {
     if (c->spmv_sched == 4) {
       NGPB_REQUIRE(g % 8 == 0, NGPB_E_STATE, "XCD-segmented SpMV needs a grid that is a multiple of 8");
       hipLaunchKernelGGL((k_sell_both<DOT, true>), dim3(g), dim3(kBS), 0, c->stream, A, c->sell_list_delta.p,
+                         c->n_list_delta, c->sell_list_packed.p, c->n_list_packed, x, y, ks, part);
+    } else if (c->spmv_sched == 5) {
+      NGPB_REQUIRE(g % 8 == 0, NGPB_E_STATE, "XCD-segmented SpMV needs a grid that is a multiple of 8");
+      hipLaunchKernelGGL((k_sell_both<DOT, true, true>), dim3(g), dim3(kBS), 0, c->stream, A, c->sell_list_delta.p,
                          c->n_list_delta, c->sell_list_packed.p, c->n_list_packed, x, y, ks, part);
     } else {
       hipLaunchKernelGGL((k_sell_both<DOT, false>), dim3(g), dim3(kBS), 0, c->stream, A, c->sell_list_delta.p,

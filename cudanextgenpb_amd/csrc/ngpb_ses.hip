@@ -22,14 +22,17 @@
 // equal the CPU restatement's bit for bit.
 //
 // Work split: the SAS pairs (circles) and exposed SAS vertices are built on the host
-// from a cell grid of the atoms (ordered by atom indices); per axis, every line
-// gathers the atoms, pairs and vertices within reach into compact lists, counts and
+// from a cell grid of the atoms (ordered by atom indices; buried circles dropped);
+// per axis, every line gathers the atoms, pairs and vertices within reach into
+// compact lists sorted by axial coordinate, one thread per list item counts and
 // writes its candidates, the candidates are sorted per line (segmented radix sorts:
-// by (kind, id), then stably by t), and one thread per line classifies the midpoints
-// and emits the crossings.
+// by (kind, id), then stably by t), one thread per candidate classifies the interval
+// after its group of equal t (within an axial window of the line's lists), and the
+// crossings are the group starts whose state differs from the previous group's.
 #include <algorithm>
 #include <array>
 #include <cmath>
+#include <thread>
 #include <vector>
 
 #include <hipcub/hipcub.hpp>
@@ -60,6 +63,7 @@ struct SesView {
   const SesPairD* pairs;
   const double* verts;       // 3 per exposed vertex
   SesGrid2 ga, gp, gv;       // atoms, pairs (by circle centre), vertices
+  double wa, wp, wv;         // axial half-widths of the classification windows
 };
 
 __device__ __forceinline__ uint64_t ses_bits(double x) {
@@ -121,22 +125,30 @@ __global__ void k_ses_list_count(int64_t nl, const uint64_t* __restrict__ lu, co
   cv[li] = nv;
 }
 
+// fills the lists with the items' axial coordinates as keys (sorted per line next)
 __global__ void k_ses_list_fill(int64_t nl, const uint64_t* __restrict__ lu, const uint64_t* __restrict__ lv,
-                                SesView S, int d1, int d2, const int64_t* __restrict__ oa,
+                                SesView S, int axis, int d1, int d2, const int64_t* __restrict__ oa,
                                 const int64_t* __restrict__ op, const int64_t* __restrict__ ov,
-                                int32_t* __restrict__ la, int32_t* __restrict__ lp, int32_t* __restrict__ lvv) {
+                                int32_t* __restrict__ la, int32_t* __restrict__ lp, int32_t* __restrict__ lvv,
+                                uint64_t* __restrict__ ka, uint64_t* __restrict__ kp, uint64_t* __restrict__ kv) {
   const int64_t li = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (li >= nl) return;
   const double u = ses_unbits(lu[li]), v = ses_unbits(lv[li]);
   int64_t ia = oa[li], ip = op[li], iv = ov[li];
   ses_visit(S.ga, u, v, [&](int a) {
-    if (atom_in_reach(S, a, u, v, d1, d2)) la[ia++] = a;
+    if (!atom_in_reach(S, a, u, v, d1, d2)) return;
+    ka[ia] = ses_bits(S.xyz[3 * a + axis]);
+    la[ia++] = a;
   });
   ses_visit(S.gp, u, v, [&](int k) {
-    if (pair_in_reach(S, k, u, v, d1, d2)) lp[ip++] = k;
+    if (!pair_in_reach(S, k, u, v, d1, d2)) return;
+    kp[ip] = ses_bits(S.pairs[k].c[axis]);
+    lp[ip++] = k;
   });
   ses_visit(S.gv, u, v, [&](int k) {
-    if (vert_in_reach(S, k, u, v, d1, d2)) lvv[iv++] = k;
+    if (!vert_in_reach(S, k, u, v, d1, d2)) return;
+    kv[iv] = ses_bits(S.verts[3 * k + axis]);
+    lvv[iv++] = k;
   });
 }
 
@@ -147,7 +159,30 @@ struct LineLists {
   const int64_t* oa;  // nl + 1 each
   const int64_t* op;
   const int64_t* ov;
+  const uint64_t* ka;  // axial coordinate keys, ascending within each line
+  const uint64_t* kp;
+  const uint64_t* kv;
 };
+
+// positions [lo, hi) of the keys within [b, e) that lie in [t - w, t + w]
+__device__ __forceinline__ void key_window(const uint64_t* k, int64_t b, int64_t e, double t, double w, int64_t* lo,
+                                           int64_t* hi) {
+  const uint64_t x0 = ses_bits(t - w), x1 = ses_bits(t + w);
+  int64_t l = b, h = e;
+  while (l < h) {
+    const int64_t m = (l + h) >> 1;
+    if (k[m] < x0) l = m + 1;
+    else h = m;
+  }
+  *lo = l;
+  h = e;
+  while (l < h) {
+    const int64_t m = (l + h) >> 1;
+    if (k[m] <= x1) l = m + 1;
+    else h = m;
+  }
+  *hi = l;
+}
 
 // the torus of pair P against the line (u, v): calls f(t) for every root, in increasing t
 template <class F>
@@ -193,57 +228,6 @@ __device__ __forceinline__ void torus_roots(const SesPairD& P, double rp, int ax
   if (g0 == 0.0) f(P.c[axis] + t0 + 0.0);
 }
 
-// every candidate of line li: f(t, kind, id)
-template <class F>
-__device__ __forceinline__ void line_candidates(const SesView& S, const LineLists& L, int64_t li, int axis, int d1,
-                                                int d2, double u, double v, F&& f) {
-  for (int64_t k = L.oa[li]; k < L.oa[li + 1]; ++k) {
-    const int a = L.la[k];
-    const double du = u - S.xyz[3 * a + d1], dv = v - S.xyz[3 * a + d2];
-    const double q = S.r[a] * S.r[a] - (du * du + dv * dv);
-    if (!(q > 0.0)) continue;
-    const double s = sqrt(q);
-    f(S.xyz[3 * a + axis] - s + 0.0, 0, a);
-    f(S.xyz[3 * a + axis] + s + 0.0, 0, a);
-  }
-  for (int64_t k = L.ov[li]; k < L.ov[li + 1]; ++k) {
-    const int id = L.lv[k];
-    const double* c = S.verts + 3 * id;
-    const double du = u - c[d1], dv = v - c[d2];
-    const double q = S.rp * S.rp - (du * du + dv * dv);
-    if (!(q > 0.0)) continue;
-    const double s = sqrt(q);
-    f(c[axis] - s + 0.0, 2, id);
-    f(c[axis] + s + 0.0, 2, id);
-  }
-  for (int64_t k = L.op[li]; k < L.op[li + 1]; ++k) {
-    const int id = L.lp[k];
-    torus_roots(S.pairs[id], S.rp, axis, d1, d2, u, v, [&](double t) { f(t, 1, id); });
-  }
-}
-
-__global__ void k_ses_cand_count(int64_t nl, const uint64_t* __restrict__ lu, const uint64_t* __restrict__ lv,
-                                 SesView S, LineLists L, int axis, int d1, int d2, int64_t* __restrict__ cnt) {
-  const int64_t li = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (li >= nl) return;
-  int64_t c = 0;
-  line_candidates(S, L, li, axis, d1, d2, ses_unbits(lu[li]), ses_unbits(lv[li]), [&](double, int, int) { ++c; });
-  cnt[li] = c;
-}
-
-__global__ void k_ses_cand_fill(int64_t nl, const uint64_t* __restrict__ lu, const uint64_t* __restrict__ lv,
-                                SesView S, LineLists L, int axis, int d1, int d2, const int64_t* __restrict__ off,
-                                uint64_t* __restrict__ tkey, uint32_t* __restrict__ kind_id) {
-  const int64_t li = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (li >= nl) return;
-  int64_t o = off[li];
-  line_candidates(S, L, li, axis, d1, d2, ses_unbits(lu[li]), ses_unbits(lv[li]), [&](double t, int kind, int id) {
-    tkey[o] = ses_bits(t);
-    kind_id[o] = ((uint32_t)kind << 28) | (uint32_t)id;
-    ++o;
-  });
-}
-
 // nearest point of the SAS circle of P to p
 __device__ __forceinline__ void circle_point(const SesPairD& P, const double* p, double* q) {
   double w[3] = {p[0] - P.c[0], p[1] - P.c[1], p[2] - P.c[2]};
@@ -261,9 +245,9 @@ __device__ __forceinline__ void circle_point(const SesPairD& P, const double* p,
   for (int k = 0; k < 3; ++k) q[k] = P.c[k] + P.rho * (u[k] / L);
 }
 
-__device__ __forceinline__ bool exposed_local(const SesView& S, const LineLists& L, int64_t li, const double* q, int e0,
-                                              int e1) {
-  for (int64_t k = L.oa[li]; k < L.oa[li + 1]; ++k) {
+__device__ __forceinline__ bool exposed_local(const SesView& S, const LineLists& L, int64_t a0, int64_t a1,
+                                              const double* q, int e0, int e1) {
+  for (int64_t k = a0; k < a1; ++k) {
     const int m = L.la[k];
     if (m == e0 || m == e1) continue;
     const double Rm = S.r[m] + S.rp;
@@ -272,10 +256,14 @@ __device__ __forceinline__ bool exposed_local(const SesView& S, const LineLists&
   return true;
 }
 
-// 1 inside the SES, 0 outside, for a point p on line li
-__device__ int ses_classify(const SesView& S, const LineLists& L, int64_t li, const double* p) {
+// 1 inside the SES, 0 outside, for the point p at t on line li. Only the items whose
+// axial coordinate lies within a window around t can satisfy any test here (an atom
+// containing p, a covered candidate within rp of p, a circle or vertex within rp).
+__device__ int ses_classify(const SesView& S, const LineLists& L, int64_t li, const double* p, double t) {
+  int64_t a0, a1, p0, p1, v0, v1;
+  key_window(L.ka, L.oa[li], L.oa[li + 1], t, S.wa, &a0, &a1);
   bool in_sas = false;
-  for (int64_t k = L.oa[li]; k < L.oa[li + 1]; ++k) {
+  for (int64_t k = a0; k < a1; ++k) {
     const int i = L.la[k];
     const double dd = sq3(p, S.xyz + 3 * i);
     if (dd < S.r[i] * S.r[i]) return 1;
@@ -284,7 +272,7 @@ __device__ int ses_classify(const SesView& S, const LineLists& L, int64_t li, co
   }
   if (!in_sas) return 0;
   const double rp2 = S.rp * S.rp;
-  for (int64_t k = L.oa[li]; k < L.oa[li + 1]; ++k) {
+  for (int64_t k = a0; k < a1; ++k) {
     const int i = L.la[k];
     const double* a = S.xyz + 3 * i;
     const double dd = sqrt(sq3(p, a));
@@ -293,49 +281,163 @@ __device__ int ses_classify(const SesView& S, const LineLists& L, int64_t li, co
     if (!(dd > 0.0) || fabs(gap) > S.rp) continue;
     double q[3];
     for (int c = 0; c < 3; ++c) q[c] = a[c] + Ri * ((p[c] - a[c]) / dd);
-    if (exposed_local(S, L, li, q, i, -1)) return 0;
+    if (exposed_local(S, L, a0, a1, q, i, -1)) return 0;
   }
-  for (int64_t k = L.op[li]; k < L.op[li + 1]; ++k) {
+  key_window(L.kp, L.op[li], L.op[li + 1], t, S.wp, &p0, &p1);
+  for (int64_t k = p0; k < p1; ++k) {
     const SesPairD& P = S.pairs[L.lp[k]];
     double q[3];
     circle_point(P, p, q);
-    if (sq3(p, q) <= rp2 && exposed_local(S, L, li, q, P.i, P.j)) return 0;
+    if (sq3(p, q) <= rp2 && exposed_local(S, L, a0, a1, q, P.i, P.j)) return 0;
   }
-  for (int64_t k = L.ov[li]; k < L.ov[li + 1]; ++k)
+  key_window(L.kv, L.ov[li], L.ov[li + 1], t, S.wv, &v0, &v1);
+  for (int64_t k = v0; k < v1; ++k)
     if (sq3(p, S.verts + 3 * L.lv[k]) <= rp2) return 0;
   return 1;
 }
 
-// Per line: walks its sorted candidates in groups of equal t and flags the first
-// candidate of every group across which the classification changes; cnt = crossings.
-__global__ void k_ses_classify(int64_t nl, const uint64_t* __restrict__ lu, const uint64_t* __restrict__ lv,
-                               SesView S, LineLists L, int axis, int d1, int d2, const int64_t* __restrict__ off,
-                               const uint64_t* __restrict__ tkey, uint8_t* __restrict__ flag,
-                               int64_t* __restrict__ cnt) {
+// ---- item-parallel candidates and classification (one thread per list item or per
+// candidate, which balances lines of very different lengths)
+
+// line owning position k of a list whose lines start at off[0 .. nl] (off[nl] = total)
+__device__ __forceinline__ int64_t line_of(const int64_t* off, int64_t nl, int64_t k) {
+  int64_t l = 0, h = nl;  // largest l with off[l] <= k
+  while (h - l > 1) {
+    const int64_t m = (l + h) >> 1;
+    if (off[m] <= k) l = m;
+    else h = m;
+  }
+  return l;
+}
+
+// kind 0 atoms, 1 pairs (tori), 2 vertices: candidates of list item k (count, or
+// written at out[..] when out is not null)
+template <int KIND>
+__global__ void k_ses_item_cands(int64_t n_items, int64_t nl, const uint64_t* __restrict__ lu,
+                                 const uint64_t* __restrict__ lv, SesView S, LineLists L, int axis, int d1, int d2,
+                                 int64_t* __restrict__ cnt, const int64_t* __restrict__ pos,
+                                 uint64_t* __restrict__ tkey, uint32_t* __restrict__ kind_id) {
+  const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= n_items) return;
+  const int64_t* offs = KIND == 0 ? L.oa : KIND == 1 ? L.op : L.ov;
+  const int64_t li = line_of(offs, nl, k);
+  const double u = ses_unbits(lu[li]), v = ses_unbits(lv[li]);
+  int64_t o = pos ? pos[k] : 0, c = 0;
+  auto emit = [&](double t, int id) {
+    if (pos) {
+      tkey[o] = ses_bits(t);
+      kind_id[o] = ((uint32_t)KIND << 28) | (uint32_t)id;
+      ++o;
+    }
+    ++c;
+  };
+  if (KIND == 0) {
+    const int a = L.la[k];
+    const double du = u - S.xyz[3 * a + d1], dv = v - S.xyz[3 * a + d2];
+    const double q = S.r[a] * S.r[a] - (du * du + dv * dv);
+    if (q > 0.0) {
+      const double sq = sqrt(q);
+      emit(S.xyz[3 * a + axis] - sq + 0.0, a);
+      emit(S.xyz[3 * a + axis] + sq + 0.0, a);
+    }
+  } else if (KIND == 2) {
+    const int id = L.lv[k];
+    const double* cc = S.verts + 3 * id;
+    const double du = u - cc[d1], dv = v - cc[d2];
+    const double q = S.rp * S.rp - (du * du + dv * dv);
+    if (q > 0.0) {
+      const double sq = sqrt(q);
+      emit(cc[axis] - sq + 0.0, id);
+      emit(cc[axis] + sq + 0.0, id);
+    }
+  } else {
+    const int id = L.lp[k];
+    if (pos) torus_roots(S.pairs[id], S.rp, axis, d1, d2, u, v, [&](double t) { emit(t, id); });
+    else torus_roots(S.pairs[id], S.rp, axis, d1, d2, u, v, [&](double) { ++c; });
+  }
+  if (!pos) cnt[k] = c;
+}
+
+// per line: its candidate total from the items' scanned counts (atoms | vertices | tori)
+__global__ void k_ses_line_total(int64_t nl, LineLists L, const int64_t* __restrict__ sa,
+                                 const int64_t* __restrict__ sp, const int64_t* __restrict__ sv,
+                                 int64_t* __restrict__ tot) {
   const int64_t li = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (li >= nl) return;
-  const double u = ses_unbits(lu[li]), v = ses_unbits(lv[li]);
-  const int64_t b = off[li], e = off[li + 1];
-  int before = 0;
-  int64_t c = 0;
-  int64_t g = b;
-  while (g < e) {
-    int64_t h = g + 1;  // next group
-    while (h < e && tkey[h] == tkey[g]) flag[h++] = 0;
-    int after = 0;
-    if (h < e) {
-      double p[3];
-      p[axis] = 0.5 * (ses_unbits(tkey[g]) + ses_unbits(tkey[h]));
-      p[d1] = u;
-      p[d2] = v;
-      after = ses_classify(S, L, li, p);
-    }
-    const bool cross = after != before;
-    flag[g] = cross ? 1 : 0;
-    c += cross;
-    before = after;
-    g = h;
+  tot[li] = (sa[L.oa[li + 1]] - sa[L.oa[li]]) + (sv[L.ov[li + 1]] - sv[L.ov[li]]) + (sp[L.op[li + 1]] - sp[L.op[li]]);
+}
+
+// output position of every item's first candidate: the line's block holds the atoms'
+// candidates, then the vertices', then the tori's
+template <int KIND>
+__global__ void k_ses_item_pos(int64_t n_items, int64_t nl, LineLists L, const int64_t* __restrict__ coff,
+                               const int64_t* __restrict__ sa, const int64_t* __restrict__ sp,
+                               const int64_t* __restrict__ sv, int64_t* __restrict__ pos) {
+  const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= n_items) return;
+  const int64_t* offs = KIND == 0 ? L.oa : KIND == 1 ? L.op : L.ov;
+  const int64_t li = line_of(offs, nl, k);
+  const int64_t A = sa[L.oa[li + 1]] - sa[L.oa[li]];
+  const int64_t V = sv[L.ov[li + 1]] - sv[L.ov[li]];
+  int64_t p = coff[li];
+  if (KIND == 0) p += sa[k] - sa[L.oa[li]];
+  else if (KIND == 2) p += A + (sv[k] - sv[L.ov[li]]);
+  else p += A + V + (sp[k] - sp[L.op[li]]);
+  pos[k] = p;
+}
+
+// state of the open interval after each group of equal t (2 marks positions that do
+// not start a group): 1 inside the SES, 0 outside
+__global__ void k_ses_state(int64_t nc, int64_t nl, const uint64_t* __restrict__ lu, const uint64_t* __restrict__ lv,
+                            SesView S, LineLists L, int axis, int d1, int d2, const int64_t* __restrict__ coff,
+                            const uint64_t* __restrict__ tkey, uint8_t* __restrict__ state) {
+  const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= nc) return;
+  const int64_t li = line_of(coff, nl, k);
+  const int64_t b = coff[li], e = coff[li + 1];
+  if (k > b && tkey[k] == tkey[k - 1]) {
+    state[k] = 2;
+    return;
   }
+  int64_t h = k + 1;
+  while (h < e && tkey[h] == tkey[k]) ++h;
+  if (h >= e) {
+    state[k] = 0;
+    return;
+  }
+  double p[3];
+  p[axis] = 0.5 * (ses_unbits(tkey[k]) + ses_unbits(tkey[h]));
+  p[d1] = ses_unbits(lu[li]);
+  p[d2] = ses_unbits(lv[li]);
+  state[k] = (uint8_t)ses_classify(S, L, li, p, p[axis]);
+}
+
+// a crossing at every group start whose state differs from the previous group's
+__global__ void k_ses_flag(int64_t nc, int64_t nl, const int64_t* __restrict__ coff, const uint8_t* __restrict__ state,
+                           uint8_t* __restrict__ flag) {
+  const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= nc) return;
+  if (state[k] == 2) {
+    flag[k] = 0;
+    return;
+  }
+  const int64_t li = line_of(coff, nl, k);
+  const int64_t b = coff[li];
+  int before = 0;
+  if (k > b) {
+    int64_t j = k - 1;
+    while (state[j] == 2) --j;  // the previous group's start (b starts a group)
+    before = state[j];
+  }
+  flag[k] = state[k] != before ? 1 : 0;
+}
+
+__global__ void k_ses_line_cross(int64_t nl, const int64_t* __restrict__ coff, const uint8_t* __restrict__ flag,
+                                 int64_t* __restrict__ cnt) {
+  const int64_t li = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (li >= nl) return;
+  int64_t c = 0;
+  for (int64_t k = coff[li]; k < coff[li + 1]; ++k) c += flag[k];
   cnt[li] = c;
 }
 
@@ -401,8 +503,74 @@ double sqd(const double* a, const double* b) {
   return x * x + y * y + z * z;
 }
 
-// SAS circles of overlapping pairs (i < j, in index order) and exposed SAS vertices
-// (per pair in that order, third atom k > j ascending, +z before -z)
+// Whether some arc of the SAS circle (c, n, rho) of pair (i, j) lies outside the other
+// SAS balls overlapping ball i: each covers the arc where P cos th + Q sin th < S in the
+// circle's (u, w) frame; the covered open arcs are swept in angle for a gap. Buried
+// circles carry no SES patch and are dropped (an exposed SAS vertex lies on three
+// exposed circles). Host arithmetic, with the C library's atan2 and acos.
+bool circle_exposed(const double* c, const double* n, double rho, int i, int j, const std::vector<int>& over_i,
+                    const double* xyz, const std::vector<double>& R) {
+  const int ax = std::fabs(n[0]) < 0.9 ? 0 : 1;
+  double u[3] = {0, 0, 0};
+  u[ax] = 1.0;
+  const double un = u[0] * n[0] + u[1] * n[1] + u[2] * n[2];
+  for (int k = 0; k < 3; ++k) u[k] = u[k] - un * n[k];
+  const double ul = std::sqrt(u[0] * u[0] + u[1] * u[1] + u[2] * u[2]);
+  for (int k = 0; k < 3; ++k) u[k] = u[k] / ul;
+  const double w[3] = {n[1] * u[2] - n[2] * u[1], n[2] * u[0] - n[0] * u[2], n[0] * u[1] - n[1] * u[0]};
+  const double two_pi = 2.0 * 3.14159265358979323846;
+  std::vector<std::pair<double, double>> iv;
+  for (int m : over_i) {  // ascending, every ball overlapping ball i
+    if (m == j) continue;
+    const double* am = xyz + 3 * m;
+    const double d[3] = {c[0] - am[0], c[1] - am[1], c[2] - am[2]};
+    const double P = 2.0 * rho * (d[0] * u[0] + d[1] * u[1] + d[2] * u[2]);
+    const double Q = 2.0 * rho * (d[0] * w[0] + d[1] * w[1] + d[2] * w[2]);
+    const double S = R[(size_t)m] * R[(size_t)m] - (d[0] * d[0] + d[1] * d[1] + d[2] * d[2]) - rho * rho;
+    const double A = std::sqrt(P * P + Q * Q);
+    if (S >= A) return false;
+    if (S <= -A) continue;
+    const double beta = std::acos(S / A);
+    double s0 = std::atan2(Q, P) + beta;
+    if (s0 < 0.0) s0 += two_pi;
+    if (s0 >= two_pi) s0 -= two_pi;
+    const double e0 = s0 + (two_pi - 2.0 * beta);
+    if (e0 > two_pi) {
+      iv.push_back({s0, two_pi});
+      iv.push_back({0.0, e0 - two_pi});
+    } else {
+      iv.push_back({s0, e0});
+    }
+  }
+  if (iv.empty()) return true;
+  std::sort(iv.begin(), iv.end());
+  double cur = 0.0;
+  for (const auto& a : iv) {
+    if (a.first > cur) return true;
+    cur = std::max(cur, a.second);
+  }
+  return cur < two_pi;
+}
+
+// SAS circles of overlapping pairs (i < j, in index order) that are not buried, and
+// exposed SAS vertices (per kept pair in that order, third atom k > j ascending, +z
+// before -z)
+// f(k) for k in [0, n) on the host's hardware threads (each k handled once)
+template <class F>
+void host_parallel(int64_t n, F&& f) {
+  const int64_t T = std::max<int64_t>(1, std::min<int64_t>((int64_t)std::thread::hardware_concurrency(), 64));
+  if (n < 256 || T == 1) {
+    for (int64_t k = 0; k < n; ++k) f(k);
+    return;
+  }
+  std::vector<std::thread> th;
+  for (int64_t t = 0; t < T; ++t)
+    th.emplace_back([&, t] {
+      for (int64_t k = t; k < n; k += T) f(k);
+    });
+  for (auto& x : th) x.join();
+}
+
 SesHost ses_build_host(int64_t na, const double* xyz, const double* r, double rp) {
   SesHost H;
   std::vector<double> R((size_t)na);
@@ -428,7 +596,7 @@ SesHost ses_build_host(int64_t na, const double* xyz, const double* r, double rp
   for (int64_t i = 0; i < na; ++i)
     grid[((size_t)ci[(size_t)i][2] * gn[1] + ci[(size_t)i][1]) * gn[0] + ci[(size_t)i][0]].push_back((int)i);
   std::vector<std::vector<int>> over((size_t)na), nb((size_t)na);  // overlapping SAS balls; circle pairs
-  for (int64_t i = 0; i < na; ++i) {
+  host_parallel(na, [&](int64_t i) {
     const auto& c = ci[(size_t)i];
     for (int z = c[2] - 1; z <= c[2] + 1; ++z)
       for (int y = c[1] - 1; y <= c[1] + 1; ++y)
@@ -441,8 +609,9 @@ SesHost ses_build_host(int64_t na, const double* xyz, const double* r, double rp
           }
         }
     std::sort(over[(size_t)i].begin(), over[(size_t)i].end());
-  }
-  for (int64_t i = 0; i < na; ++i)
+  });
+  std::vector<std::vector<SesPairD>> pairs_of((size_t)na);
+  host_parallel(na, [&](int64_t i) {
     for (int j : over[(size_t)i]) {
       if (j <= i) continue;
       const double* ai = xyz + 3 * i;
@@ -460,9 +629,15 @@ SesHost ses_build_host(int64_t na, const double* xyz, const double* r, double rp
         P.c[k] = ai[k] + t * P.n[k];
       }
       P.rho = std::sqrt(rho2);
+      if (!circle_exposed(P.c, P.n, P.rho, (int)i, j, over[(size_t)i], xyz, R)) continue;
+      pairs_of[(size_t)i].push_back(P);
+    }
+  });
+  for (int64_t i = 0; i < na; ++i)
+    for (const SesPairD& P : pairs_of[(size_t)i]) {
       H.pairs.push_back(P);
-      nb[(size_t)i].push_back(j);
-      nb[(size_t)j].push_back((int)i);
+      nb[(size_t)P.i].push_back(P.j);
+      nb[(size_t)P.j].push_back(P.i);
     }
   for (auto& v : nb) std::sort(v.begin(), v.end());
   auto exposed = [&](const double* q, int e0, int e1, int e2) {
@@ -472,7 +647,9 @@ SesHost ses_build_host(int64_t na, const double* xyz, const double* r, double rp
     }
     return true;
   };
-  for (const SesPairD& P : H.pairs) {
+  std::vector<std::vector<double>> verts_of(H.pairs.size());
+  host_parallel((int64_t)H.pairs.size(), [&](int64_t pk) {
+    const SesPairD& P = H.pairs[(size_t)pk];
     for (int k : nb[(size_t)P.j]) {
       if (k <= P.j || !std::binary_search(nb[(size_t)P.i].begin(), nb[(size_t)P.i].end(), k)) continue;
       const double* ai = xyz + 3 * P.i;
@@ -497,10 +674,11 @@ SesHost ses_build_host(int64_t na, const double* xyz, const double* r, double rp
         const double zz = sgn == 0 ? z : -z;
         double v[3];
         for (int q = 0; q < 3; ++q) v[q] = ai[q] + x * ex[q] + y * ey[q] + zz * ez[q];
-        if (exposed(v, P.i, P.j, k)) H.verts.insert(H.verts.end(), v, v + 3);
+        if (exposed(v, P.i, P.j, k)) verts_of[(size_t)pk].insert(verts_of[(size_t)pk].end(), v, v + 3);
       }
     }
-  }
+  });
+  for (const auto& v : verts_of) H.verts.insert(H.verts.end(), v.begin(), v.end());
   return H;
 }
 
@@ -539,6 +717,7 @@ void bin2(int64_t n, const std::vector<double>& c1, const std::vector<double>& c
 
 // Device SES data of one molecule (built once per ngpb_cast_rays call).
 struct SesDevice {
+  std::vector<double> rmax_of;  // atom radii (host)
   DBuf<double> xyz, r, verts;
   DBuf<SesPairD> pairs;
   int64_t na = 0, npairs = 0, nverts = 0;
@@ -549,6 +728,7 @@ struct SesDevice {
 SesDevice* ses_prepare(ngpb_ctx* c, int64_t na, const double* xyz, const double* r, double rp) {
   SesHost H = ses_build_host(na, xyz, r, rp);
   auto* D = new SesDevice(c->stream);
+  D->rmax_of.assign(r, r + na);
   D->na = na;
   D->rp = rp;
   D->npairs = (int64_t)H.pairs.size();
@@ -617,6 +797,12 @@ void ses_cast_lines(ngpb_ctx* c, const SesDevice* D, int axis, int64_t nl, const
   dbpv.upload(bpv.data(), bpv.size(), c->stream);
   dbiv.upload(biv.data(), biv.size(), c->stream);
   SesView S;
+  double rmax = 0.0, rhomax = 0.0;
+  for (int64_t a = 0; a < D->na; ++a) rmax = std::max(rmax, D->rmax_of[(size_t)a]);
+  for (int64_t k = 0; k < D->npairs; ++k) rhomax = std::max(rhomax, hp[(size_t)k].rho);
+  S.wa = rmax + 2.0 * D->rp + 1e-6;  // R + rp bounds every atom test
+  S.wp = rhomax + D->rp + 1e-6;
+  S.wv = D->rp + 1e-6;
   S.xyz = D->xyz.p;
   S.r = D->r.p;
   S.na = D->na;
@@ -640,17 +826,57 @@ void ses_cast_lines(ngpb_ctx* c, const SesDevice* D, int axis, int64_t nl, const
   NGPB_HIP(hipMemcpyAsync(oa.p + nl, &ta, sizeof(int64_t), hipMemcpyHostToDevice, c->stream));
   NGPB_HIP(hipMemcpyAsync(op.p + nl, &tp, sizeof(int64_t), hipMemcpyHostToDevice, c->stream));
   NGPB_HIP(hipMemcpyAsync(ov.p + nl, &tv, sizeof(int64_t), hipMemcpyHostToDevice, c->stream));
-  DBuf<int32_t> la(c->stream), lpp(c->stream), lvv(c->stream);
-  la.resize((size_t)std::max<int64_t>(ta, 1));
-  lpp.resize((size_t)std::max<int64_t>(tp, 1));
-  lvv.resize((size_t)std::max<int64_t>(tv, 1));
-  hipLaunchKernelGGL(k_ses_list_fill, G, B, 0, c->stream, nl, lu, lv, S, d1, d2, oa.p, op.p, ov.p, la.p, lpp.p, lvv.p);
+  DBuf<int32_t> la(c->stream), lpp(c->stream), lvv(c->stream), la2(c->stream), lpp2(c->stream), lvv2(c->stream);
+  DBuf<uint64_t> ka(c->stream), kp(c->stream), kv(c->stream), ka2(c->stream), kp2(c->stream), kv2(c->stream);
+  for (auto* b : {&la, &la2}) b->resize((size_t)std::max<int64_t>(ta, 1));
+  for (auto* b : {&lpp, &lpp2}) b->resize((size_t)std::max<int64_t>(tp, 1));
+  for (auto* b : {&lvv, &lvv2}) b->resize((size_t)std::max<int64_t>(tv, 1));
+  for (auto* b : {&ka, &ka2}) b->resize((size_t)std::max<int64_t>(ta, 1));
+  for (auto* b : {&kp, &kp2}) b->resize((size_t)std::max<int64_t>(tp, 1));
+  for (auto* b : {&kv, &kv2}) b->resize((size_t)std::max<int64_t>(tv, 1));
+  hipLaunchKernelGGL(k_ses_list_fill, G, B, 0, c->stream, nl, lu, lv, S, axis, d1, d2, oa.p, op.p, ov.p, la.p, lpp.p,
+                     lvv.p, ka.p, kp.p, kv.p);
   NGPB_HIP(hipGetLastError());
-  const LineLists L{la.p, lpp.p, lvv.p, oa.p, op.p, ov.p};
-  // candidates, sorted per line by (t, kind, id)
+  // each line's lists by axial coordinate
+  auto seg_sort = [&](int64_t n, DBuf<uint64_t>& k_in, DBuf<uint64_t>& k_out, DBuf<int32_t>& v_in,
+                      DBuf<int32_t>& v_out, const int64_t* offs) {
+    if (n <= 0) return;
+    size_t tb = 0;
+    NGPB_HIP(hipcub::DeviceSegmentedRadixSort::SortPairs(nullptr, tb, k_in.p, k_out.p, v_in.p, v_out.p, (int)n,
+                                                         (int)nl, offs, offs + 1, 0, 64, c->stream));
+    DBuf<uint8_t> tmp(c->stream);
+    tmp.resize(std::max<size_t>(tb, 1));
+    NGPB_HIP(hipcub::DeviceSegmentedRadixSort::SortPairs(tmp.p, tb, k_in.p, k_out.p, v_in.p, v_out.p, (int)n, (int)nl,
+                                                         offs, offs + 1, 0, 64, c->stream));
+  };
+  NGPB_REQUIRE(ta < INT32_MAX && tp < INT32_MAX && tv < INT32_MAX, NGPB_E_ARG, "SES line lists too long");
+  seg_sort(ta, ka, ka2, la, la2, oa.p);
+  seg_sort(tp, kp, kp2, lpp, lpp2, op.p);
+  seg_sort(tv, kv, kv2, lvv, lvv2, ov.p);
+  const LineLists L{ta > 0 ? la2.p : la.p, tp > 0 ? lpp2.p : lpp.p, tv > 0 ? lvv2.p : lvv.p, oa.p, op.p, ov.p,
+                    ta > 0 ? ka2.p : ka.p, tp > 0 ? kp2.p : kp.p, tv > 0 ? kv2.p : kv.p};
+  // candidates: counted and written per list item, then sorted per line by (t, kind, id)
+  auto items = [&](int64_t n) { return dim3((unsigned)blocks_for(std::max<int64_t>(n, 1), 128)); };
+  DBuf<int64_t> sa(c->stream), sp(c->stream), sv(c->stream);
+  sa.resize((size_t)ta + 1);
+  sp.resize((size_t)tp + 1);
+  sv.resize((size_t)tv + 1);
+  if (ta) hipLaunchKernelGGL(k_ses_item_cands<0>, items(ta), B, 0, c->stream, ta, nl, lu, lv, S, L, axis, d1, d2, sa.p,
+                             nullptr, nullptr, nullptr);
+  if (tp) hipLaunchKernelGGL(k_ses_item_cands<1>, items(tp), B, 0, c->stream, tp, nl, lu, lv, S, L, axis, d1, d2, sp.p,
+                             nullptr, nullptr, nullptr);
+  if (tv) hipLaunchKernelGGL(k_ses_item_cands<2>, items(tv), B, 0, c->stream, tv, nl, lu, lv, S, L, axis, d1, d2, sv.p,
+                             nullptr, nullptr, nullptr);
+  NGPB_HIP(hipGetLastError());
+  const int64_t na_c = ta ? exclusive_scan(c, sa.p, sa.p, ta) : 0;
+  const int64_t np_c = tp ? exclusive_scan(c, sp.p, sp.p, tp) : 0;
+  const int64_t nv_c = tv ? exclusive_scan(c, sv.p, sv.p, tv) : 0;
+  NGPB_HIP(hipMemcpyAsync(sa.p + ta, &na_c, sizeof(int64_t), hipMemcpyHostToDevice, c->stream));
+  NGPB_HIP(hipMemcpyAsync(sp.p + tp, &np_c, sizeof(int64_t), hipMemcpyHostToDevice, c->stream));
+  NGPB_HIP(hipMemcpyAsync(sv.p + tv, &nv_c, sizeof(int64_t), hipMemcpyHostToDevice, c->stream));
   DBuf<int64_t> coff(c->stream);
   coff.resize((size_t)nl + 1);
-  hipLaunchKernelGGL(k_ses_cand_count, G, B, 0, c->stream, nl, lu, lv, S, L, axis, d1, d2, coff.p);
+  hipLaunchKernelGGL(k_ses_line_total, G, B, 0, c->stream, nl, L, sa.p, sp.p, sv.p, coff.p);
   NGPB_HIP(hipGetLastError());
   const int64_t nc = exclusive_scan(c, coff.p, coff.p, nl);
   NGPB_HIP(hipMemcpyAsync(coff.p + nl, &nc, sizeof(int64_t), hipMemcpyHostToDevice, c->stream));
@@ -662,8 +888,26 @@ void ses_cast_lines(ngpb_ctx* c, const SesDevice* D, int axis, int64_t nl, const
   tk2.resize(ncs);
   kid.resize(ncs);
   kid2.resize(ncs);
-  hipLaunchKernelGGL(k_ses_cand_fill, G, B, 0, c->stream, nl, lu, lv, S, L, axis, d1, d2, coff.p, tk.p, kid.p);
-  NGPB_HIP(hipGetLastError());
+  {
+    DBuf<int64_t> pos(c->stream);
+    pos.resize((size_t)std::max<int64_t>(std::max(ta, std::max(tp, tv)), 1));
+    if (ta) {
+      hipLaunchKernelGGL(k_ses_item_pos<0>, items(ta), B, 0, c->stream, ta, nl, L, coff.p, sa.p, sp.p, sv.p, pos.p);
+      hipLaunchKernelGGL(k_ses_item_cands<0>, items(ta), B, 0, c->stream, ta, nl, lu, lv, S, L, axis, d1, d2, nullptr,
+                         pos.p, tk.p, kid.p);
+    }
+    if (tv) {
+      hipLaunchKernelGGL(k_ses_item_pos<2>, items(tv), B, 0, c->stream, tv, nl, L, coff.p, sa.p, sp.p, sv.p, pos.p);
+      hipLaunchKernelGGL(k_ses_item_cands<2>, items(tv), B, 0, c->stream, tv, nl, lu, lv, S, L, axis, d1, d2, nullptr,
+                         pos.p, tk.p, kid.p);
+    }
+    if (tp) {
+      hipLaunchKernelGGL(k_ses_item_pos<1>, items(tp), B, 0, c->stream, tp, nl, L, coff.p, sa.p, sp.p, sv.p, pos.p);
+      hipLaunchKernelGGL(k_ses_item_cands<1>, items(tp), B, 0, c->stream, tp, nl, lu, lv, S, L, axis, d1, d2, nullptr,
+                         pos.p, tk.p, kid.p);
+    }
+    NGPB_HIP(hipGetLastError());
+  }
   if (nc > 0) {
     size_t tb1 = 0, tb2 = 0;
     NGPB_HIP(hipcub::DeviceSegmentedRadixSort::SortPairs(nullptr, tb1, kid.p, kid2.p, tk.p, tk2.p, (int)nc, (int)nl,
@@ -682,8 +926,17 @@ void ses_cast_lines(ngpb_ctx* c, const SesDevice* D, int axis, int64_t nl, const
   flag.resize(ncs);
   DBuf<int64_t> xcnt(c->stream);
   xcnt.resize((size_t)nl);
-  hipLaunchKernelGGL(k_ses_classify, G, B, 0, c->stream, nl, lu, lv, S, L, axis, d1, d2, coff.p, tk.p, flag.p, xcnt.p);
-  NGPB_HIP(hipGetLastError());
+  {
+    DBuf<uint8_t> state(c->stream);
+    state.resize(ncs);
+    if (nc > 0) {
+      hipLaunchKernelGGL(k_ses_state, items(nc), B, 0, c->stream, nc, nl, lu, lv, S, L, axis, d1, d2, coff.p, tk.p,
+                         state.p);
+      hipLaunchKernelGGL(k_ses_flag, items(nc), B, 0, c->stream, nc, nl, coff.p, state.p, flag.p);
+    }
+    hipLaunchKernelGGL(k_ses_line_cross, G, B, 0, c->stream, nl, coff.p, flag.p, xcnt.p);
+    NGPB_HIP(hipGetLastError());
+  }
   // rays = lines with crossings
   DBuf<uint8_t> keep(c->stream);
   keep.resize((size_t)nl);

@@ -27,7 +27,8 @@ DEFAULT_SOLVER_OPTIONS = "-i cg -p jacobi -tol 1.e-10 -maxiter 100000"
 
 class PoissonBoltzmann:
     def __init__(self, problem: Problem, device: int = 0, ctx: Optional[native.Context] = None,
-                 device_rays: bool = False, device_locate: bool = False, device_mesh: bool = False):
+                 device_rays: bool = False, device_locate: bool = False, device_mesh: bool = False,
+                 probe_radius: float = 0.0):
         """device_rays: cast the axis rays of the atom spheres on the GPU instead of
         uploading the problem's host-cast ray table (the two are bit-identical).
         device_locate: find the atoms' owner leaves on the GPU (search_points) instead
@@ -35,7 +36,10 @@ class PoissonBoltzmann:
         device_mesh: build the octree on the GPU from the atoms and the mesh's box,
         levels and bands (ngpb_build_octree) instead of uploading the problem's
         host-built mesh (the two are identical); implies device_rays and device_locate,
-        so that only the atoms cross to the device."""
+        so that only the atoms cross to the device.
+        probe_radius > 0: the rays are cast on the device against the solvent-excluded
+        surface with that probe radius (the reference's default surface, probe 1.4 A)
+        instead of the union of the atom spheres."""
         self.problem = problem
         self.device_mesh = device_mesh
         self.device_locate = device_locate or device_mesh
@@ -46,7 +50,9 @@ class PoissonBoltzmann:
             self.ctx.build_octree(*problem.mesh.generator_params(), problem.atoms.xyz, problem.atoms.r)
         else:
             self.ctx.set_mesh(problem.mesh)
-        if device_rays or device_mesh:
+        if probe_radius > 0.0:
+            self.ctx.cast_rays_ses(problem.atoms.xyz, problem.atoms.r, probe_radius)
+        elif device_rays or device_mesh:
             self.ctx.cast_rays(problem.atoms.xyz, problem.atoms.r)
         else:
             self.ctx.set_rays(problem.rays)

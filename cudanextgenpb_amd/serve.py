@@ -113,9 +113,10 @@ class Result:
 class Server:
     def __init__(self, device: int = 0, budget_bytes: float = 250e9, small_max_unknowns: int = 1_500_000,
                  max_batch: int = 256, tol: float = 1e-10, inner_per_replay: int = 8, stream_workers: int = 2,
-                 prep_workers: int = 2, keep_phi: bool = False, device_mesh: bool = False):
+                 prep_workers: int = 2, keep_phi: bool = False, device_mesh: bool = False, probe: float = 0.0):
         self.device = device
         self.device_mesh = device_mesh
+        self.probe = probe  # > 0: solvent-excluded surface rays with this probe radius
         self.budget = float(budget_bytes)
         self.small_max = small_max_unknowns
         self.tol = tol
@@ -135,7 +136,7 @@ class Server:
         """Large request: prepared and solved on its own context (worker thread). A
         solve that stops at maxiter or stagnates is reported through its status, as a
         batched one is; the context is closed whatever happens."""
-        p = prepare(req.problem, self.device, device_mesh=self.device_mesh)
+        p = prepare(req.problem, self.device, device_mesh=self.device_mesh, probe=self.probe)
         try:
             _, st = p.ctx.solve(native.PREC_MIXED, tol=self.tol, want_phi=False, raise_on_noconv=False)
             e = p.ctx.energy()
@@ -182,7 +183,7 @@ class Server:
                 used += req.predicted_bytes
                 self.peak_bytes = max(self.peak_bytes, used)
                 if small:
-                    fut = self.prep_pool.submit(prepare, req.problem, self.device, None, self.device_mesh)
+                    fut = self.prep_pool.submit(prepare, req.problem, self.device, None, self.device_mesh, self.probe)
                     preparing.append((req, clock(), fut))
                 else:
                     streaming.append((req, clock(), self.pool.submit(self._stream_solve, req)))
@@ -261,7 +262,7 @@ class Server:
 
 
 def run_sequential_fp64(requests: List[Request], device: int = 0, tol: float = 1e-10,
-                        realtime: bool = True, device_mesh: bool = False) -> Dict[int, Result]:
+                        realtime: bool = True, device_mesh: bool = False, probe: float = 0.0) -> Dict[int, Result]:
     """The same arrival stream served one request at a time with the FP64 solver."""
     reqs = sorted(requests, key=lambda r: r.arrival)
     t0 = time.perf_counter()
@@ -272,7 +273,7 @@ def run_sequential_fp64(requests: List[Request], device: int = 0, tol: float = 1
             if wait > 0:
                 time.sleep(wait)
         adm = time.perf_counter() - t0
-        p = prepare(req.problem, device, device_mesh=device_mesh)
+        p = prepare(req.problem, device, device_mesh=device_mesh, probe=probe)
         _, st = p.ctx.solve(native.PREC_FP64, tol=tol, want_phi=False)
         e = p.ctx.energy()
         t = time.perf_counter() - t0

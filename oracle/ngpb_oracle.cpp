@@ -1176,6 +1176,8 @@ double orc_analytic_solution(double x, double y, double z, double e_in, double e
 //   the radial projection of p onto SAS sphere i          (convex atom patches),
 //   the point of the SAS circle of a pair (i, j) nearest p  (re-entrant tori),
 //   an SAS vertex of a triple (i, j, k)                     (re-entrant probe spheres).
+// Circles buried in the union of the other SAS balls are dropped beforehand
+// (circle_exposed), and SAS vertices are taken from triples of kept circles.
 // A point inside an atom's own ball (|p - a_i| < r_i) is inside; a point outside
 // every SAS ball is outside. Along an axis ray the SES boundary can only lie on an
 // atom sphere (radius r_i), a torus (tube rp around an SAS circle) or a probe sphere
@@ -1216,6 +1218,59 @@ bool ses_exposed(const SesModel& M, const double* q, int e0, int e1, int e2) {
   return true;
 }
 
+// Whether some arc of the SAS circle (c, n, rho) of pair (i, j) lies outside every
+// other SAS ball m that overlaps ball i (only those can cover its points): ball m covers
+// the circle points q(th) = c + rho (cos th u + sin th w) with P cos th + Q sin th < S,
+// P = 2 rho (c - a_m).u, Q = 2 rho (c - a_m).w, S = R_m^2 - |c - a_m|^2 - rho^2, an arc
+// of half-width pi - acos(S / A) (A = |(P, Q)|) about the direction opposite (P, Q).
+// The union of the covered open arcs is swept in angle; a circle whose arcs leave no
+// gap is buried and contributes no SES patch (an exposed SAS vertex lies on three
+// exposed circles). Host-only arithmetic (atan2, acos of the C library).
+bool circle_exposed(const double* c, const double* n, double rho, int i, int j, int64_t na, const double* xyz,
+                    const std::vector<double>& R) {
+  const int ax = std::fabs(n[0]) < 0.9 ? 0 : 1;
+  double u[3] = {0, 0, 0};
+  u[ax] = 1.0;
+  const double un = u[0] * n[0] + u[1] * n[1] + u[2] * n[2];
+  for (int k = 0; k < 3; ++k) u[k] = u[k] - un * n[k];
+  const double ul = std::sqrt(u[0] * u[0] + u[1] * u[1] + u[2] * u[2]);
+  for (int k = 0; k < 3; ++k) u[k] = u[k] / ul;
+  const double w[3] = {n[1] * u[2] - n[2] * u[1], n[2] * u[0] - n[0] * u[2], n[0] * u[1] - n[1] * u[0]};
+  const double two_pi = 2.0 * kPi;
+  std::vector<std::pair<double, double>> iv;
+  for (int64_t m = 0; m < na; ++m) {
+    if (m == i || m == j) continue;
+    const double* am = xyz + 3 * m;
+    if (!(std::sqrt(d2(xyz + 3 * i, am)) < R[i] + R[m])) continue;
+    const double d[3] = {c[0] - am[0], c[1] - am[1], c[2] - am[2]};
+    const double P = 2.0 * rho * (d[0] * u[0] + d[1] * u[1] + d[2] * u[2]);
+    const double Q = 2.0 * rho * (d[0] * w[0] + d[1] * w[1] + d[2] * w[2]);
+    const double S = R[m] * R[m] - (d[0] * d[0] + d[1] * d[1] + d[2] * d[2]) - rho * rho;
+    const double A = std::sqrt(P * P + Q * Q);
+    if (S >= A) return false;  // the whole circle
+    if (S <= -A) continue;     // none of it
+    const double beta = std::acos(S / A);
+    double s0 = std::atan2(Q, P) + beta;  // covered: angles in (s0, s0 + 2 pi - 2 beta)
+    if (s0 < 0.0) s0 += two_pi;
+    if (s0 >= two_pi) s0 -= two_pi;
+    const double e0 = s0 + (two_pi - 2.0 * beta);
+    if (e0 > two_pi) {
+      iv.push_back({s0, two_pi});
+      iv.push_back({0.0, e0 - two_pi});
+    } else {
+      iv.push_back({s0, e0});
+    }
+  }
+  if (iv.empty()) return true;
+  std::sort(iv.begin(), iv.end());
+  double cur = 0.0;
+  for (const auto& a : iv) {
+    if (a.first > cur) return true;  // an uncovered gap
+    cur = std::max(cur, a.second);
+  }
+  return cur < two_pi;
+}
+
 void ses_build(SesModel& M) {
   M.R.resize(M.na);
   for (int64_t i = 0; i < M.na; ++i) M.R[i] = M.r[i] + M.rp;
@@ -1236,7 +1291,7 @@ void ses_build(SesModel& M) {
         P.c[k] = ai[k] + t * P.n[k];
       }
       P.rho = std::sqrt(rho2);
-      M.pairs.push_back(P);
+      if (circle_exposed(P.c, P.n, P.rho, i, j, M.na, M.xyz, M.R)) M.pairs.push_back(P);
     }
   // SAS vertices: the two points at distance R_i, R_j, R_k of a triple whose SAS
   // spheres meet pairwise, kept where no other SAS ball covers them
