@@ -68,11 +68,12 @@ def test_reference_default_options_string():
     parse to CGS with SSOR at w = 0.51 and tol 1e-6; the alternative it lists,
     "-i cg -p ilu" (data/options.prm:177), is refused with NGPB_E_ARG so that the
     caller falls back to LIS."""
-    o = native.parse_lis_options("-p ssor -ssor_omega 0.51 -i cgs -tol 1.e-6")
+    o = native.parse_lis_options("-p ssor -ssor_omega 0.51 -i cgs -tol 1.e-6 -print 2 -conv_cond 2 -tol_w 0")
     assert (o.solver, o.precon, o.omega, o.tol) == (native.SOLVER_CGS, native.PRECON_SSOR, 0.51, 1e-6)
     assert o.precision == native.PREC_FP64 and o.maxiter == 1000
     o2 = native.SolveOpts()
-    rc = native.lib().ngpb_parse_lis_options(b"-i cg -p ilu -tol 1e-6", ctypes.byref(o2))
+    rc = native.lib().ngpb_parse_lis_options(
+        b"-i cg -p ilu -ilu_fill [0] -tol 1.e-6 -print 2 -conv_cond 2 -tol_w 0", ctypes.byref(o2))
     assert rc == 1  # NGPB_E_ARG
 
 
