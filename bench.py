@@ -490,7 +490,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--workload", choices=["globular", "born", "large", "batch", "serve", "mesh", "request", "spmv"],
                     default="globular")
-    ap.add_argument("--spmv-schedule", type=int, default=4, help="spmv workload: compact SpMV schedule (0-4)")
+    ap.add_argument("--spmv-schedule", type=int, default=6, help="spmv workload: compact SpMV schedule (0-6)")
     ap.add_argument("--spmv-reps", type=int, default=50, help="spmv workload: timed launches per kernel")
     ap.add_argument("--mesh-size", choices=["globular", "large"], default="globular",
                     help="mesh and request workloads: which system")
@@ -623,7 +623,7 @@ def main():
             "energy_polar_kT": mixed["energy"]["polar"],
         },
         "roofline": {
-            "kernel": "compact FP32 SpMV of the mixed inner CG (k_sell_both<true,true>: SELL-64 delta, constant and packed slices, XCD-segmented)",
+            "kernel": "compact FP32 SpMV of the mixed inner CG (k_sell_lean<true>: SELL-64 delta, constant and packed slices, XCD-segmented, guard-banded input)",
             "bound": "hbm",
             "achieved": achieved,
             "peak": HBM_PEAK_GBS,

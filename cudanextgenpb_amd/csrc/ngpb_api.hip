@@ -80,7 +80,7 @@ int64_t ngpb_ctx::device_bytes() const {
        sell_val.bytes() + sell_off.bytes() + sell_base.bytes() + sell_col32.bytes() + sell_ov_cnt.bytes() +
        sell_ov_val.bytes() + sell_ov_col.bytes() + dscale.bytes() + sell_pk_val.bytes() + sell_pk_idx.bytes() +
        sell_pk_lp.bytes() + sell_pk_dict.bytes() + sell_pk_hdr.bytes() + sell_cval.bytes() + sell_list_delta.bytes() +
-       sell_list_packed.bytes() + sell_list_offset.bytes();
+       sell_list_packed.bytes() + sell_list_offset.bytes() + sell_dpair.bytes() + sell_pk_hdr_list.bytes();
   b += x.bytes() + r.bytes() + z.bytes() + p.bytes() + q.bytes() + dinv.bytes();
   b += lis_rt.bytes() + lis_u.bytes() + lis_v.bytes() + lis_ph.bytes() + lis_uh.bytes() + lis_t.bytes() +
        diag_pos.bytes() + lev_rows_f.bytes() + lev_rows_b.bytes();
@@ -967,7 +967,8 @@ int ngpb_set_compact_options(ngpb_ctx* c, int constant_slices) {
 int ngpb_set_spmv_schedule(ngpb_ctx* c, int schedule) {
   return guarded(c, [&] {
     not_in_batch(c);
-    if (schedule < 0 || schedule > 5) throw Error(NGPB_E_ARG, "schedule must be 0 .. 5");
+    if (schedule < 0 || schedule > 6) throw Error(NGPB_E_ARG, "schedule must be 0 .. 6");
+    if (schedule != c->spmv_sched) c->sell_resident = 0;  // grid follows the schedule's kernel
     c->spmv_sched = schedule;
   });
 }

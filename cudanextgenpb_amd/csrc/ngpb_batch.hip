@@ -595,7 +595,7 @@ int ngpb_batch_add(ngpb_batch* b, ngpb_ctx* c, int* slot) {
     c->q.reserve((size_t)n);
     c->xs.reserve((size_t)n);
     c->rs.reserve((size_t)n);
-    c->ps.reserve((size_t)n);
+    float* const pin = ngpb::spmv_input(c);  // zero guard entries either side
     c->qs.reserve((size_t)n);
     c->red.partials.reserve((size_t)std::max<int64_t>(2 * c->n_slices, 1));
     NGPB_HIP(hipStreamSynchronize(c->stream));
@@ -615,6 +615,7 @@ int ngpb_batch_add(ngpb_batch* b, ngpb_ctx* c, int* slot) {
     d.A.ov_val = c->sell_ov_val.p;
     d.A.ov_col = c->sell_ov_col.p;
     d.A.pk_val = c->sell_pk_val.p;
+    d.A.pk_zero = (uint32_t)c->sell_pk_entries;
     d.A.pk_idx = c->sell_pk_idx.p;
     d.A.pk_lp = c->sell_pk_lp.p;
     d.A.pk_dict = c->sell_pk_dict.p;
@@ -629,7 +630,7 @@ int ngpb_batch_add(ngpb_batch* b, ngpb_ctx* c, int* slot) {
     d.t = c->q.p;
     d.xs = c->xs.p;
     d.rs = c->rs.p;
-    d.ps = c->ps.p;
+    d.ps = pin;
     d.qs = c->qs.p;
     d.part = c->red.partials.p;
     d.n = n;

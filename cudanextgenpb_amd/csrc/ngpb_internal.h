@@ -260,8 +260,12 @@ struct ngpb_ctx {
   // slices of each form, for the form-specialised SpMV kernels
   int64_t n_list_delta = 0, n_list_packed = 0, n_list_offset = 0;
   ngpb::DBuf<int32_t> sell_list_delta, sell_list_packed, sell_list_offset;
+  // pipelined SpMV: per pair of delta-list entries its slice ids, slot pointers, widths
+  // and pairing (8 x int32), and the packed slices' headers in list order
+  ngpb::DBuf<int32_t> sell_dpair;
+  ngpb::DBuf<int64_t> sell_pk_hdr_list;
   ngpb::DBuf<double> dscale;         // 1/sqrt(diag)
-  int spmv_sched = 4;                // SpMV work schedule (0 .. 4)
+  int spmv_sched = 6;                // SpMV work schedule (0 .. 6)
   int sell_resident = 0;             // resident blocks of the compact SpMV (0: not yet queried)
   int csr_resident = 0;              // resident blocks of the FP64 CSR SpMV (0: not yet queried)
 
@@ -355,6 +359,7 @@ double time_spmv_fp64(ngpb_ctx* c, int reps);
 
 // mixed
 void build_compact(ngpb_ctx* c);
+float* spmv_input(ngpb_ctx* c);  // FP32 SpMV input vector (zero guard entries either side)
 void spmv_compact(ngpb_ctx* c, const double* x, double* y);
 void solve_mixed(ngpb_ctx* c, const ngpb_solve_opts& o, ngpb_solve_stats& st);
 double spmv_compact_bytes(const ngpb_ctx* c);

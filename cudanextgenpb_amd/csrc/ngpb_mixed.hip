@@ -579,6 +579,303 @@ __global__ void __launch_bounds__(kBS) k_sell_both(Sell A, const int32_t* __rest
   }
 }
 
+// ---- pipelined SpMV (schedule 6)
+// The work of k_sell_both<DOT, true> (same lists, same XCD segmentation, same items per
+// wave in the same order, same per-row arithmetic, so the same results bit for bit),
+// with the metadata of a wave's items requested ahead so that an item's operator values
+// and input gathers go out together in one memory round trip:
+//   item i + 2  the delta pair's record (slice ids, slot pointers, widths, pairing) and
+//               the packed slice's id and 32-byte header (scalar loads);
+//   item i + 1  the slot deltas (and constant-slice values) of both delta slices, one
+//               per lane (lanes 0-15 the first slice, 16-31 the second), and the packed
+//               slice's dictionary, row map and dictionary-index words (vector loads);
+//   item i      values and gathers; deltas come from the prefetched lanes by readlane,
+//               packed columns from the prefetched index words and dictionary.
+struct DPair {
+  int32_t s0, s1;  // slice ids (s1 = -1: none)
+  int32_t b0, b1;  // slot pointers (base_ptr)
+  int32_t w0, w1;  // widths
+  int32_t paired;  // 1: s1 = s0 + 1 with the same width of 4 .. 8 slots (plain or constant)
+  int32_t pad;
+};
+static_assert(sizeof(DPair) == 32, "DPair is 32 bytes");
+
+__device__ __forceinline__ int delta_slots(int w) { return w >= -16 ? -w : (w >= -48 ? -w - kConstBase : 0); }
+__device__ __forceinline__ bool delta_const(int w) { return w <= -(kConstBase + 1) && w >= -(kConstBase + 16); }
+
+__global__ void k_dpair_fill(int64_t npairs, const int32_t* __restrict__ list, int64_t cnt,
+                             const int32_t* __restrict__ width, const int64_t* __restrict__ base_ptr,
+                             DPair* __restrict__ out) {
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= npairs) return;
+  DPair m;
+  m.s0 = list[2 * t];
+  m.s1 = 2 * t + 1 < cnt ? list[2 * t + 1] : -1;
+  m.w0 = width[m.s0];
+  m.b0 = (int32_t)base_ptr[m.s0];
+  m.w1 = m.s1 >= 0 ? width[m.s1] : 0;
+  m.b1 = m.s1 >= 0 ? (int32_t)base_ptr[m.s1] : 0;
+  const int sw = m.w0 <= -(kConstBase + 1) ? -m.w0 - kConstBase : -m.w0;
+  m.paired = (m.s1 == m.s0 + 1 && m.w0 == m.w1 && sw >= 4 && sw <= 8) ? 1 : 0;
+  m.pad = 0;
+  out[t] = m;
+}
+
+__global__ void k_pk_hdr_list(int64_t cnt, const int32_t* __restrict__ list, const PkHdr* __restrict__ hdr,
+                              PkHdr* __restrict__ out) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < cnt) out[i] = hdr[list[i]];
+}
+
+// ---- lean SpMV (schedule 6)
+// The work of k_sell_both<DOT, true> (same lists and XCD segmentation, the same items
+// per wave in the same order, the same per-row arithmetic) with fewer vector
+// instructions per entry:
+//   * the input vector has kGuard zero entries on either side (spmv_input), so gathers
+//     need no clamping: every gathered index is a column of some row of the same
+//     slice plus a row offset below 64 (absent slots and rows past n included);
+//   * delta slots address x through a scalar base (x + slice row 0 + delta) and the
+//     lane index, values through a scalar base and an immediate slot offset;
+//   * packed entries of rows that do not take part in a slot read the operator's one
+//     +0 entry (the value the select of k_sell_both gives them), so no select follows;
+//   * a pair's slice ids, slot pointers and widths come from one 32-byte record, a
+//     packed slice's id and header from two independent loads.
+// Results equal those of k_sell_both except possibly for the sign of an exact zero.
+constexpr int kGuard = 64;
+
+template <int ND, bool CONST>
+__device__ __forceinline__ double lean_rows(const Sell& A, int64_t s, int64_t bp, const float* __restrict__ x,
+                                            double acc) {
+  const int lane = lane_id();
+  const float* vb = A.val + bp * kW;
+  float v[ND], xv[ND];
+  int d[ND];
+#pragma unroll
+  for (int j = 0; j < ND; ++j) {
+    v[j] = CONST ? A.cval[bp + j] : vb[j * kW + lane];
+    d[j] = A.base[bp + j];
+  }
+#pragma unroll
+  for (int j = 0; j < ND; ++j) {
+    const float* xb = x + (s * kW + d[j]);
+    xv[j] = xb[lane];
+  }
+#pragma unroll
+  for (int j = 0; j < ND; ++j) acc = fma((double)v[j], (double)xv[j], acc);
+  return acc;
+}
+
+__device__ __forceinline__ void lean_delta(const Sell& A, int64_t s, int w, int64_t bp, const float* __restrict__ x,
+                                           float* __restrict__ y, double* part) {
+  const int lane = lane_id();
+  const float* xr = x + s * kW;
+  const float xi = xr[lane];
+  double acc = 0.0;
+  switch (w) {
+#define NGPB_LR(W, ND, C) \
+  case W: acc = lean_rows<ND, C>(A, s, bp, x, acc); break;
+    NGPB_LR(-1, 1, false) NGPB_LR(-2, 2, false) NGPB_LR(-3, 3, false) NGPB_LR(-4, 4, false)
+    NGPB_LR(-5, 5, false) NGPB_LR(-6, 6, false) NGPB_LR(-7, 7, false) NGPB_LR(-8, 8, false)
+    NGPB_LR(-9, 9, false) NGPB_LR(-10, 10, false) NGPB_LR(-11, 11, false) NGPB_LR(-12, 12, false)
+    NGPB_LR(-13, 13, false) NGPB_LR(-14, 14, false) NGPB_LR(-15, 15, false) NGPB_LR(-16, 16, false)
+    NGPB_LR(-33, 1, true) NGPB_LR(-34, 2, true) NGPB_LR(-35, 3, true) NGPB_LR(-36, 4, true)
+    NGPB_LR(-37, 5, true) NGPB_LR(-38, 6, true) NGPB_LR(-39, 7, true) NGPB_LR(-40, 8, true)
+    NGPB_LR(-41, 9, true) NGPB_LR(-42, 10, true) NGPB_LR(-43, 11, true) NGPB_LR(-44, 12, true)
+    NGPB_LR(-45, 13, true) NGPB_LR(-46, 14, true) NGPB_LR(-47, 15, true) NGPB_LR(-48, 16, true)
+#undef NGPB_LR
+    default: break;
+  }
+  acc += (double)xi;
+  if (s * kW + lane < A.n) {
+    const float yi = (float)acc;
+    y[s * kW + lane] = yi;
+    *part += (double)xi * (double)yi;
+  }
+}
+
+template <int ND, bool CONST>
+__device__ __forceinline__ void lean_pair(const Sell& A, int64_t s0, int64_t b0, int64_t b1,
+                                          const float* __restrict__ x, float* __restrict__ y, double* part) {
+  const int lane = lane_id();
+  const int64_t s1 = s0 + 1;
+  const float* vb0 = A.val + b0 * kW;
+  const float* vb1 = A.val + b1 * kW;
+  float v0[ND], v1[ND], x0[ND], x1[ND];
+  int d0[ND], d1[ND];
+#pragma unroll
+  for (int j = 0; j < ND; ++j) {
+    v0[j] = CONST ? A.cval[b0 + j] : vb0[j * kW + lane];
+    v1[j] = CONST ? A.cval[b1 + j] : vb1[j * kW + lane];
+    d0[j] = A.base[b0 + j];
+    d1[j] = A.base[b1 + j];
+  }
+  const float xi0 = x[s0 * kW + lane], xi1 = x[s1 * kW + lane];
+#pragma unroll
+  for (int j = 0; j < ND; ++j) {
+    const float* xb0 = x + (s0 * kW + d0[j]);
+    const float* xb1 = x + (s1 * kW + d1[j]);
+    x0[j] = xb0[lane];
+    x1[j] = xb1[lane];
+  }
+  double a0 = 0.0, a1 = 0.0;
+#pragma unroll
+  for (int j = 0; j < ND; ++j) {
+    a0 = fma((double)v0[j], (double)x0[j], a0);
+    a1 = fma((double)v1[j], (double)x1[j], a1);
+  }
+  a0 += (double)xi0;
+  a1 += (double)xi1;
+  if (s0 * kW + lane < A.n) {
+    const float yi = (float)a0;
+    y[s0 * kW + lane] = yi;
+    *part += (double)xi0 * (double)yi;
+  }
+  if (s1 * kW + lane < A.n) {
+    const float yi = (float)a1;
+    y[s1 * kW + lane] = yi;
+    *part += (double)xi1 * (double)yi;
+  }
+}
+
+__device__ __forceinline__ void lean_dpair(const Sell& A, const DPair& m, const float* __restrict__ x,
+                                           float* __restrict__ y, double* part) {
+  if (m.paired) {
+    switch (m.w0) {
+      case -4: lean_pair<4, false>(A, m.s0, m.b0, m.b1, x, y, part); return;
+      case -5: lean_pair<5, false>(A, m.s0, m.b0, m.b1, x, y, part); return;
+      case -6: lean_pair<6, false>(A, m.s0, m.b0, m.b1, x, y, part); return;
+      case -7: lean_pair<7, false>(A, m.s0, m.b0, m.b1, x, y, part); return;
+      case -8: lean_pair<8, false>(A, m.s0, m.b0, m.b1, x, y, part); return;
+      case -36: lean_pair<4, true>(A, m.s0, m.b0, m.b1, x, y, part); return;
+      case -37: lean_pair<5, true>(A, m.s0, m.b0, m.b1, x, y, part); return;
+      case -38: lean_pair<6, true>(A, m.s0, m.b0, m.b1, x, y, part); return;
+      case -39: lean_pair<7, true>(A, m.s0, m.b0, m.b1, x, y, part); return;
+      case -40: lean_pair<8, true>(A, m.s0, m.b0, m.b1, x, y, part); return;
+      default: break;
+    }
+  }
+  lean_delta(A, m.s0, m.w0, m.b0, x, y, part);
+  if (m.s1 >= 0) lean_delta(A, m.s1, m.w1, m.b1, x, y, part);
+}
+
+// hdr_rows without clamps; an entry of a row that does not take part in slot j reads
+// the +0 entry
+template <int K, int NV>
+__device__ __forceinline__ double lean_hdr_rows(const Sell& A, const PkHdr& h, int row, const int* dv,
+                                                const float* __restrict__ x, double acc) {
+  const int lane = lane_id();
+  float v[K], xv[K];
+  uint32_t wd[K / 4];
+  uint32_t ss = (uint32_t)h.E, gs = (uint32_t)h.G;
+#pragma unroll
+  for (int g = 0; g < K / 4; ++g) {
+    const int mg = 4 * g < (int)h.maxlen ? (int)h.m[4 * g] : 0;
+    wd[g] = A.pk_idx[lane < mg ? gs + (uint32_t)lane : (uint32_t)h.G];
+    gs += (uint32_t)mg;
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      const int j = 4 * g + q;
+      const int mk = j < (int)h.maxlen ? (int)h.m[j] : 0;
+      v[j] = A.pk_val[lane < mk ? ss + (uint32_t)lane : A.pk_zero];
+      ss += (uint32_t)mk;
+    }
+  }
+#pragma unroll
+  for (int j = 0; j < K; ++j) {
+    const int d = dict_lookup<NV>(dv, (int)((wd[j / 4] >> (8 * (j % 4))) & 255u));
+    xv[j] = x[row + d];
+  }
+#pragma unroll
+  for (int j = 0; j < K; ++j) acc = fma((double)v[j], (double)xv[j], acc);
+  return acc;
+}
+
+template <int NV>
+__device__ __forceinline__ void lean_packed(const Sell& A, int64_t s, const PkHdr& h, const float* __restrict__ x,
+                                            float* __restrict__ y, double* part) {
+  const int lane = lane_id();
+  const int lp = A.pk_lp[s * kW + lane];
+  int dv[NV];
+#pragma unroll
+  for (int q = 0; q < NV; ++q) dv[q] = q * kW + lane < (int)h.nd ? A.pk_dict[h.D + q * kW + lane] : 0;
+  const int row = (int)(s * kW) + (lp & 255);
+  const float xi = x[row];
+  const int maxlen = h.maxlen;
+  double acc = 0.0;
+  if (maxlen > 0) {
+    if (maxlen <= 8) acc = lean_hdr_rows<8, NV>(A, h, row, dv, x, acc);
+    else acc = lean_hdr_rows<16, NV>(A, h, row, dv, x, acc);
+  }
+  if (maxlen > 16) {
+    const int len = lp >> 8;
+    int64_t ss = 0, gs = 0;
+#pragma unroll
+    for (int j = 0; j < 16; ++j) ss += h.m[j];
+#pragma unroll
+    for (int g = 0; g < 4; ++g) gs += h.m[4 * g];
+    for (int k0 = 16; k0 < maxlen; k0 += 4) {
+      const int mg = __popcll(__ballot(len > k0));
+      const uint32_t wd = A.pk_idx[lane < mg ? h.G + gs + lane : (int64_t)h.G];
+      gs += mg;
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        const bool act = len > k0 + q;
+        const int mk = __popcll(__ballot(act));
+        const float v = A.pk_val[act ? h.E + ss + lane : (int64_t)A.pk_zero];
+        ss += mk;
+        const int d = dict_lookup<NV>(dv, (int)((wd >> (8 * q)) & 255u));
+        acc = fma((double)v, (double)x[row + d], acc);
+      }
+    }
+  }
+  acc += (double)xi;
+  if (row < A.n) {
+    const float yi = (float)acc;
+    y[row] = yi;
+    *part += (double)xi * (double)yi;
+  }
+}
+
+template <bool DOT>
+__global__ void __launch_bounds__(kBS) k_sell_lean(Sell A, const DPair* __restrict__ dp, int64_t npairs,
+                                                   const int32_t* __restrict__ plist, const PkHdr* __restrict__ hl,
+                                                   int64_t pcnt, const float* __restrict__ x, float* __restrict__ y,
+                                                   const KrylovScalars* ks, double* __restrict__ partials) {
+  __shared__ double s_red[kBS / 64];
+  if (DOT && ks->done) return;
+  const int64_t xcd = blockIdx.x % 8u, local = blockIdx.x / 8u;
+  const int64_t W = (int64_t)(gridDim.x / 8u) * (kBS / kW);
+  const int64_t wave = __builtin_amdgcn_readfirstlane((int)(local * (kBS / kW) + (threadIdx.x >> 6)));
+  const int64_t dhi = (xcd + 1) * npairs / 8, phi = (xcd + 1) * pcnt / 8;
+  int64_t td = xcd * npairs / 8 + wave, tp = xcd * pcnt / 8 + wave;
+  double part = 0.0;
+  while (td < dhi || tp < phi) {
+    if (td < dhi) {
+      lean_dpair(A, dp[td], x, y, &part);
+      td += W;
+    }
+    if (tp < phi) {
+      const int64_t s = plist[tp];
+      const PkHdr h = hl[tp];
+      if (h.nd <= kW) lean_packed<1>(A, s, h, x, y, &part);
+      else if (h.nd <= 2 * kW) lean_packed<2>(A, s, h, x, y, &part);
+      else lean_packed<4>(A, s, h, x, y, &part);
+      tp += W;
+    }
+  }
+  if (DOT) {
+    part = dev::block_sum<kBS>(part, s_red);
+    if (threadIdx.x == 0) partials[blockIdx.x] = part;
+  }
+}
+
+// zero the guard entries of the SpMV input vector (p points kGuard entries before x_0)
+__global__ void k_zero_guard(float* __restrict__ p, int64_t n) {
+  const int t = threadIdx.x;
+  if (t < kGuard) p[t] = 0.f;
+  else if (t < 2 * kGuard) p[kGuard + n + (t - kGuard)] = 0.f;
+}
+
 // offset slices (rows too irregular for the other forms; rare)
 template <bool DOT>
 __global__ void __launch_bounds__(kBS) k_sell_offset(Sell A, const int32_t* __restrict__ list, int64_t cnt,
@@ -772,6 +1069,7 @@ Sell sell_view(const ngpb_ctx* c) {
   A.ov_val = c->sell_ov_val.p;
   A.ov_col = c->sell_ov_col.p;
   A.pk_val = c->sell_pk_val.p;
+  A.pk_zero = (uint32_t)c->sell_pk_entries;
   A.pk_idx = c->sell_pk_idx.p;
   A.pk_lp = c->sell_pk_lp.p;
   A.pk_hdr = reinterpret_cast<const PkHdr*>(c->sell_pk_hdr.p);
@@ -785,7 +1083,10 @@ unsigned sell_resident(ngpb_ctx* c) {
   if (c->sell_resident > 0) return (unsigned)c->sell_resident;
   int cus = 0, per_cu = 0;
   NGPB_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, c->device));
-  NGPB_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_sell_both<true, false>, kBS, 0));
+  if (c->spmv_sched == 6)
+    NGPB_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_sell_lean<true>, kBS, 0));
+  else
+    NGPB_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_sell_both<true, false>, kBS, 0));
   const int r = std::max(8, (std::max(cus, 1) * std::max(per_cu, 1)) / 8 * 8);
   c->sell_resident = r;
   return (unsigned)r;
@@ -825,6 +1126,12 @@ void launch_sell_t(ngpb_ctx* c, const float* x, float* y, double* part) {
       NGPB_REQUIRE(g % 8 == 0, NGPB_E_STATE, "XCD-segmented SpMV needs a grid that is a multiple of 8");
       hipLaunchKernelGGL((k_sell_both<DOT, true>), dim3(g), dim3(kBS), 0, c->stream, A, c->sell_list_delta.p,
                          c->n_list_delta, c->sell_list_packed.p, c->n_list_packed, x, y, ks, part);
+    } else if (c->spmv_sched == 6) {
+      NGPB_REQUIRE(g % 8 == 0, NGPB_E_STATE, "XCD-segmented SpMV needs a grid that is a multiple of 8");
+      hipLaunchKernelGGL(k_sell_lean<DOT>, dim3(g), dim3(kBS), 0, c->stream, A,
+                         reinterpret_cast<const DPair*>(c->sell_dpair.p), (c->n_list_delta + 1) / 2,
+                         c->sell_list_packed.p, reinterpret_cast<const PkHdr*>(c->sell_pk_hdr_list.p),
+                         c->n_list_packed, x, y, ks, part);
     } else if (c->spmv_sched == 5) {
       NGPB_REQUIRE(g % 8 == 0, NGPB_E_STATE, "XCD-segmented SpMV needs a grid that is a multiple of 8");
       hipLaunchKernelGGL((k_sell_both<DOT, true, true>), dim3(g), dim3(kBS), 0, c->stream, A, c->sell_list_delta.p,
@@ -917,7 +1224,9 @@ void build_compact(ngpb_ctx* c) {
   }
   c->sell_pk_entries = npk;
   c->sell_pk_dict_entries = npkd;
-  c->sell_pk_val.resize((size_t)std::max<int64_t>(npk, 1));
+  // one more value, +0, read by the lean SpMV for rows that do not take part in a slot
+  c->sell_pk_val.resize((size_t)npk + 1);
+  NGPB_HIP(hipMemsetAsync(c->sell_pk_val.p + npk, 0, sizeof(float), c->stream));
   c->sell_pk_words = npkg;
   c->sell_pk_idx.resize((size_t)std::max<int64_t>(npkg, 1));
   // bytes of rows that end inside a group stay 0, a valid dictionary index
@@ -964,9 +1273,32 @@ void build_compact(ngpb_ctx* c) {
     c->n_list_delta = compact_flags(c, fd.p, ns, c->sell_list_delta.p);
     c->n_list_packed = compact_flags(c, fp.p, ns, c->sell_list_packed.p);
     c->n_list_offset = compact_flags(c, fo.p, ns, c->sell_list_offset.p);
+    const int64_t npairs = (c->n_list_delta + 1) / 2;
+    NGPB_REQUIRE(nbases < INT32_MAX, NGPB_E_ARG, "operator too large for 32-bit slot pointers");
+    c->sell_dpair.resize((size_t)std::max<int64_t>(npairs, 1) * (sizeof(DPair) / sizeof(int32_t)));
+    if (npairs > 0)
+      hipLaunchKernelGGL(k_dpair_fill, dim3(blocks_for(npairs, 256)), dim3(256), 0, c->stream, npairs,
+                         c->sell_list_delta.p, c->n_list_delta, c->slice_width.p, c->slice_base_ptr.p,
+                         reinterpret_cast<DPair*>(c->sell_dpair.p));
+    c->sell_pk_hdr_list.resize((size_t)std::max<int64_t>(c->n_list_packed, 1) * (sizeof(PkHdr) / sizeof(int64_t)));
+    if (c->n_list_packed > 0)
+      hipLaunchKernelGGL(k_pk_hdr_list, dim3(blocks_for(c->n_list_packed, 256)), dim3(256), 0, c->stream,
+                         c->n_list_packed, c->sell_list_packed.p, reinterpret_cast<const PkHdr*>(c->sell_pk_hdr.p),
+                         reinterpret_cast<PkHdr*>(c->sell_pk_hdr_list.p));
+    NGPB_HIP(hipGetLastError());
   }
   NGPB_HIP(hipStreamSynchronize(c->stream));
   c->have_compact = true;
+}
+
+// The FP32 SpMV input vector of the context (the scaled direction p of the mixed
+// solve): x_0 .. x_n-1 with kGuard zero entries on either side, which the lean SpMV
+// gathers from in place of clamped indices. Every kernel writes only x_0 .. x_n-1.
+float* spmv_input(ngpb_ctx* c) {
+  c->ps.reserve((size_t)(c->n + 2 * kGuard));
+  hipLaunchKernelGGL(k_zero_guard, dim3(1), dim3(2 * kGuard), 0, c->stream, c->ps.p, c->n);
+  NGPB_HIP(hipGetLastError());
+  return c->ps.p + kGuard;
 }
 
 double spmv_compact_bytes(const ngpb_ctx* c) {
@@ -987,25 +1319,25 @@ double spmv_compact_bytes(const ngpb_ctx* c) {
 void spmv_compact(ngpb_ctx* c, const double* x, double* y) {
   if (!c->have_compact) build_compact(c);
   const int64_t n = c->n;
-  c->ps.reserve((size_t)n);
+  float* pin = spmv_input(c);
   c->qs.reserve((size_t)n);
-  hipLaunchKernelGGL(k_to_scaled, dim3(blocks_for(n, 256)), dim3(256), 0, c->stream, n, x, c->dscale.p, c->ps.p);
-  launch_sell(c, false, c->ps.p, c->qs.p, nullptr);
+  hipLaunchKernelGGL(k_to_scaled, dim3(blocks_for(n, 256)), dim3(256), 0, c->stream, n, x, c->dscale.p, pin);
+  launch_sell(c, false, pin, c->qs.p, nullptr);
   hipLaunchKernelGGL(k_from_scaled, dim3(blocks_for(n, 256)), dim3(256), 0, c->stream, n, c->qs.p, c->dscale.p, y);
   NGPB_HIP(hipGetLastError());
 }
 
 double time_spmv_compact(ngpb_ctx* c, int reps) {
   if (!c->have_compact) build_compact(c);
-  c->ps.reserve((size_t)c->n);
+  float* pin = spmv_input(c);
   c->qs.reserve((size_t)c->n);
-  NGPB_HIP(hipMemsetAsync(c->ps.p, 0, c->n * sizeof(float), c->stream));
+  NGPB_HIP(hipMemsetAsync(pin, 0, c->n * sizeof(float), c->stream));
   hipEvent_t e0, e1;
   NGPB_HIP(hipEventCreate(&e0));
   NGPB_HIP(hipEventCreate(&e1));
-  launch_sell(c, false, c->ps.p, c->qs.p, nullptr);
+  launch_sell(c, false, pin, c->qs.p, nullptr);
   NGPB_HIP(hipEventRecord(e0, c->stream));
-  for (int i = 0; i < reps; ++i) launch_sell(c, false, c->ps.p, c->qs.p, nullptr);
+  for (int i = 0; i < reps; ++i) launch_sell(c, false, pin, c->qs.p, nullptr);
   NGPB_HIP(hipEventRecord(e1, c->stream));
   NGPB_HIP(hipEventSynchronize(e1));
   float ms = 0.f;
@@ -1024,7 +1356,7 @@ void solve_mixed(ngpb_ctx* c, const ngpb_solve_opts& o, ngpb_solve_stats& st) {
   c->q.reserve((size_t)n);
   c->xs.reserve((size_t)n);
   c->rs.reserve((size_t)n);
-  c->ps.reserve((size_t)n);
+  float* const pv = spmv_input(c);
   c->qs.reserve((size_t)n);
   const unsigned gv = vec_grid(n);
   const unsigned gs = (unsigned)sell_partials(c);
@@ -1043,7 +1375,7 @@ void solve_mixed(ngpb_ctx* c, const ngpb_solve_opts& o, ngpb_solve_stats& st) {
   NGPB_HIP(hipEventCreate(&e1));
   NGPB_HIP(hipEventRecord(e0, c->stream));
   hipLaunchKernelGGL(k_mx_resid, dim3(gv), dim3(kBS), 0, c->stream, n, c->rhs.p, (const double*)nullptr, c->dscale.p,
-                     c->rs.p, c->x.p, c->xs.p, c->ps.p, 1, part);
+                     c->rs.p, c->x.p, c->xs.p, pv, 1, part);
   hipLaunchKernelGGL(k_mx_fin_resid, dim3(1), dim3(kFin), 0, c->stream, ks, part, (int)gv, 1);
   NGPB_HIP(hipGetLastError());
   NGPB_HIP(hipMemcpyAsync(h, ks, sizeof(KrylovScalars), hipMemcpyDeviceToHost, c->stream));
@@ -1069,16 +1401,16 @@ void solve_mixed(ngpb_ctx* c, const ngpb_solve_opts& o, ngpb_solve_stats& st) {
       h->iter2[launch & 1] = iters;
       NGPB_HIP(hipMemcpyAsync(ks, h, sizeof(KrylovScalars), hipMemcpyHostToDevice, c->stream));
       const int64_t cycle_start = launch;
-      hipLaunchKernelGGL(k_mx_dir, dim3(gv), dim3(kBS), 0, c->stream, n, c->rs.p, c->ps.p, ks);
+      hipLaunchKernelGGL(k_mx_dir, dim3(gv), dim3(kBS), 0, c->stream, n, c->rs.p, pv, ks);
       while (true) {
         for (int k = 0; k < chunk; ++k, ++launch) {
           const int par = (int)(launch & 1);
           prof_begin(c, 1, launch);
-          launch_sell(c, true, c->ps.p, c->qs.p, ppq);
+          launch_sell(c, true, pv, c->qs.p, ppq);
           prof_end(c, 1);
           hipLaunchKernelGGL(k_mx_update, dim3(gv), dim3(kBS), 0, c->stream, n, c->rs.p, c->qs.p, ks, ppq, (int)gs, pup,
                              par);
-          hipLaunchKernelGGL(k_mx_dir_x, dim3(gv), dim3(kBS), 0, c->stream, n, c->rs.p, c->ps.p, c->xs.p, ks, pup,
+          hipLaunchKernelGGL(k_mx_dir_x, dim3(gv), dim3(kBS), 0, c->stream, n, c->rs.p, pv, c->xs.p, ks, pup,
                              (int)gv, par);
         }
         NGPB_HIP(hipGetLastError());
@@ -1089,12 +1421,12 @@ void solve_mixed(ngpb_ctx* c, const ngpb_solve_opts& o, ngpb_solve_stats& st) {
       }
       iters = h->iter;
       // FP64 residual replacement
-      hipLaunchKernelGGL(k_mx_accum, dim3(gv), dim3(kBS), 0, c->stream, n, c->dscale.p, c->x.p, c->xs.p, c->ps.p, ks);
+      hipLaunchKernelGGL(k_mx_accum, dim3(gv), dim3(kBS), 0, c->stream, n, c->dscale.p, c->x.p, c->xs.p, pv, ks);
       prof_begin(c, 0, -1);
       spmv_fp64(c, c->x.p, c->q.p);
       prof_end(c, 0);
       hipLaunchKernelGGL(k_mx_resid, dim3(gv), dim3(kBS), 0, c->stream, n, c->rhs.p, c->q.p, c->dscale.p, c->rs.p, c->x.p,
-                         c->xs.p, c->ps.p, 0, part);
+                         c->xs.p, pv, 0, part);
       hipLaunchKernelGGL(k_mx_fin_resid, dim3(1), dim3(kFin), 0, c->stream, ks, part, (int)gv, 0);
       NGPB_HIP(hipGetLastError());
       NGPB_HIP(hipMemcpyAsync(h, ks, sizeof(KrylovScalars), hipMemcpyDeviceToHost, c->stream));

@@ -340,9 +340,14 @@ int ngpb_set_compact_options(ngpb_ctx* ctx, int constant_slices);
    1 a single kernel for all forms, 2 as 0 on the resident block count, 3 the delta
    and packed slices in one launch on the resident grid, every wave alternating
    between its share of the two lists so that the two forms run side by side (then
-   the offset slices, if any), 4 (default) as 3 with the lists cut into 8 contiguous
-   parts, one per XCD (block b runs on XCD b % 8), the blocks of an XCD interleaving
-   within their part, so that each XCD's L2 holds the input entries of one row range.
+   the offset slices, if any), 4 as 3 with the lists cut into 8 contiguous parts, one
+   per XCD (block b runs on XCD b % 8), the blocks of an XCD interleaving within their
+   part, so that each XCD's L2 holds the input entries of one row range, 5 as 4 with
+   non-temporal operator loads (measured slower), 6 (default) as 4 with fewer vector
+   instructions per entry: the input vector carries 64 zero entries on either side so
+   gathers are not clamped, delta gathers use scalar bases, and packed entries of rows
+   that skip a slot read a stored +0 instead of being selected away (bit-identical to
+   4 up to the sign of exact zeros).
    FP64 CSR: 0 grid-stride over row chunks on up to 2048 blocks, 1 one contiguous
    chunk range per block with XCD-aware block numbering on the resident grid
    (measured 1.36x slower), 2, 3 and 4 grid-stride on the resident grid (an

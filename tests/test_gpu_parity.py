@@ -316,14 +316,17 @@ def test_repeat_runs_bit_identical():
 def test_spmv_schedules_agree(name):
     """Every work schedule evaluates each row with the same instructions in the same
     order: the compact and FP64 SpMV results are bit-identical across schedules, and
-    mixed solves (whose dot-product partials are grouped differently) agree to 1e-12."""
+    mixed solves (whose dot-product partials are grouped differently) agree to 1e-12.
+    Schedules 4, 5 and 6 also group the partials alike, so their mixed solves are
+    bit-identical (6 gathers from zero guard entries instead of clamped indices)."""
     P, pb = gpu_pipeline(name)
     info = pb.ctx.operator_info()
     assert info["packed_slices"] > 0 and info["delta_slices"] > 0
     x = np.random.default_rng(5).standard_normal(P.mesh.n_nodes)
     ref32 = ref64 = None
     energies = []
-    for sched in (0, 1, 2, 3, 4):
+    phis = {}
+    for sched in (0, 1, 2, 3, 4, 5, 6):
         pb.ctx.set_spmv_schedule(sched)
         y32 = pb.ctx.spmv(x, native.PREC_MIXED)
         y64 = pb.ctx.spmv(x, native.PREC_FP64)
@@ -334,7 +337,9 @@ def test_spmv_schedules_agree(name):
         pb.hip_compute_electric_potential(precision=native.PREC_MIXED)
         assert pb.stats["status"] == 0
         energies.append(pb.energy()["polar"])
+        phis[sched] = pb.phi.copy()
     assert max(abs(e - energies[0]) for e in energies) <= 1e-12 * abs(energies[0])
+    assert np.array_equal(phis[4], phis[5]) and np.array_equal(phis[4], phis[6])
     pb.close()
 
 
