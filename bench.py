@@ -446,6 +446,7 @@ def bench_spmv(args):
     ctx = p.ctx
     info = ctx.operator_info()
     x = np.random.default_rng(1).standard_normal(p.n)
+    ctx.set_row_order(args.row_band)
     ctx.set_spmv_schedule(4)
     y_ref = ctx.spmv(x, native.PREC_MIXED)
     ctx.set_spmv_schedule(args.spmv_schedule)
@@ -474,7 +475,8 @@ def bench_spmv(args):
         "dtype": "fp32",
         "data": "synthetic",
         "config": {"workload": f"{spec.name}: operator of {p.n} rows, {p.nnz} nnz assembled from atoms on the device; "
-                               f"SpMV schedule {args.spmv_schedule}", "parallelism": "single GPU"},
+                               f"SpMV schedule {args.spmv_schedule}, row order {args.row_band}",
+                   "parallelism": "single GPU"},
         "roofline": {"bound": "hbm", "achieved": gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": gbs / HBM_PEAK_GBS,
                      "traffic": None, "bytes_per_launch": b32, "avg_launch_ms": ms32},
         "fp64_spmv": {"ms": ms64, "bytes": b64, "gbs": b64 / (ms64 * 1e-3) / 1e9},
@@ -492,6 +494,8 @@ def main():
                     default="globular")
     ap.add_argument("--spmv-schedule", type=int, default=6, help="spmv workload: compact SpMV schedule (0-6)")
     ap.add_argument("--spmv-reps", type=int, default=50, help="spmv workload: timed launches per kernel")
+    ap.add_argument("--row-band", type=int, default=0,
+                    help="spmv workload: row order of the compact operator (0 node order, >0 band height)")
     ap.add_argument("--mesh-size", choices=["globular", "large"], default="globular",
                     help="mesh and request workloads: which system")
     ap.add_argument("--requests", type=int, default=48, help="serve workload: number of requests")

@@ -964,6 +964,27 @@ int ngpb_set_compact_options(ngpb_ctx* c, int constant_slices) {
   });
 }
 
+int ngpb_set_row_order(ngpb_ctx* c, int band) {
+  return guarded(c, [&] {
+    not_in_batch(c);
+    if (band < 0 || band > 4096) throw Error(NGPB_E_ARG, "band must be 0 (node order) or 1 .. 4096");
+    if (band != c->row_band) {
+      c->row_band = band;
+      c->have_compact = false;  // rebuilt in the new order on next use
+    }
+  });
+}
+
+int ngpb_row_order(ngpb_ctx* c, int* band) {
+  return guarded(c, [&] {
+    NGPB_REQUIRE(band, NGPB_E_ARG, "NULL output");
+    if (c->in_batch) need(c->have_compact, "context is in a batch");
+    need(c->have_system, "no assembled system");
+    if (!c->have_compact) ngpb::build_compact(c);
+    *band = c->sell_banded ? c->sell_band : 0;
+  });
+}
+
 int ngpb_set_spmv_schedule(ngpb_ctx* c, int schedule) {
   return guarded(c, [&] {
     not_in_batch(c);

@@ -32,7 +32,7 @@
 
 namespace ngpb {
 
-void build_compact(ngpb_ctx* c);
+void build_compact(ngpb_ctx* c, bool allow_band);
 void check_operator(ngpb_ctx* c);
 
 namespace {
@@ -589,7 +589,7 @@ int ngpb_batch_add(ngpb_batch* b, ngpb_ctx* c, int* slot) {
     NGPB_REQUIRE(s >= 0, NGPB_E_STATE, "batch is full");
     NGPB_HIP(hipSetDevice(c->device));
     ngpb::check_operator(c);
-    if (!c->have_compact) ngpb::build_compact(c);
+    if (!c->have_compact || c->sell_banded) ngpb::build_compact(c, false);  // batch kernels use node order
     const int64_t n = c->n;
     c->x.resize((size_t)n);
     c->q.reserve((size_t)n);

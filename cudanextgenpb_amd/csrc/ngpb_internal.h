@@ -263,6 +263,13 @@ struct ngpb_ctx {
   // pipelined SpMV: per pair of delta-list entries its slice ids, slot pointers, widths
   // and pairing (8 x int32), and the packed slices' headers in list order
   ngpb::DBuf<int32_t> sell_dpair;
+  // row order of the compact operator: row_band 0 node order, > 0 bands of that many
+  // finest-grid spacings in y; sell_banded: the built operator's rows are nodes
+  // row_perm[0 .. n-1]
+  int row_band = 0;
+  bool sell_banded = false;
+  int sell_band = 0;  // band of the built operator when sell_banded
+  ngpb::DBuf<int32_t> row_perm;
   ngpb::DBuf<int64_t> sell_pk_hdr_list;
   ngpb::DBuf<double> dscale;         // 1/sqrt(diag)
   int spmv_sched = 6;                // SpMV work schedule (0 .. 6)
@@ -358,7 +365,7 @@ double spmv_fp64_bytes(const ngpb_ctx* c);
 double time_spmv_fp64(ngpb_ctx* c, int reps);
 
 // mixed
-void build_compact(ngpb_ctx* c);
+void build_compact(ngpb_ctx* c, bool allow_band = true);  // allow_band: banded row order may apply
 float* spmv_input(ngpb_ctx* c);  // FP32 SpMV input vector (zero guard entries either side)
 void spmv_compact(ngpb_ctx* c, const double* x, double* y);
 void solve_mixed(ngpb_ctx* c, const ngpb_solve_opts& o, ngpb_solve_stats& st);

@@ -31,6 +31,8 @@
 #include <cmath>
 #include <vector>
 
+#include <hipcub/hipcub.hpp>
+
 #include "ngpb_internal.h"
 
 namespace ngpb {
@@ -981,33 +983,38 @@ __global__ void __launch_bounds__(kBS) k_mx_dir_x(int64_t n, const float* __rest
 }
 
 // x += S (xhat + alpha p) for the iteration that raised the stop flag; xhat = 0
+// perm (NULL: identity): FP32 row i is FP64 row perm[i] (banded row order)
 __global__ void __launch_bounds__(kBS) k_mx_accum(int64_t n, const double* __restrict__ dscale, double* __restrict__ x,
                                                   float* __restrict__ xs, const float* __restrict__ ps,
-                                                  const KrylovScalars* ks) {
+                                                  const KrylovScalars* ks, const int32_t* __restrict__ perm) {
   const float alpha = (float)ks->alpha2[ks->stop_par];
   for (int64_t i = (int64_t)blockIdx.x * kBS + threadIdx.x; i < n; i += (int64_t)gridDim.x * kBS) {
-    x[i] += dscale[i] * (double)fmaf(alpha, ps[i], xs[i]);
+    const int64_t o = perm ? (int64_t)perm[i] : i;
+    x[o] += dscale[i] * (double)fmaf(alpha, ps[i], xs[i]);
     xs[i] = 0.f;
   }
 }
 
 // r = b - t (t = A x in FP64); rhat = S r; per block (r, r), (rhat, rhat).
 // first: r = b and x = 0, xhat = 0, p = 0.
+// (with perm: FP32 row i takes FP64 row perm[i]; the sums run in FP32 row order)
 __global__ void __launch_bounds__(kBS) k_mx_resid(int64_t n, const double* __restrict__ b, const double* __restrict__ t,
                                                   const double* __restrict__ dscale, float* __restrict__ rs,
                                                   double* __restrict__ x, float* __restrict__ xs, float* __restrict__ ps,
-                                                  int first, double* __restrict__ partials) {
+                                                  int first, double* __restrict__ partials,
+                                                  const int32_t* __restrict__ perm) {
   __shared__ double s_red[kBS / 64];
   double rr = 0.0, ss = 0.0;
   for (int64_t i = (int64_t)blockIdx.x * kBS + threadIdx.x; i < n; i += (int64_t)gridDim.x * kBS) {
+    const int64_t o = perm ? (int64_t)perm[i] : i;
     double ri;
     if (first) {
-      ri = b[i];
-      x[i] = 0.0;
+      ri = b[o];
+      x[o] = 0.0;
       xs[i] = 0.f;
       ps[i] = 0.f;
     } else {
-      ri = b[i] - t[i];
+      ri = b[o] - t[o];
     }
     const float si = (float)(dscale[i] * ri);
     rs[i] = si;
@@ -1043,14 +1050,104 @@ __global__ void __launch_bounds__(kFin) k_mx_fin_resid(KrylovScalars* ks, const 
 }
 
 __global__ void k_to_scaled(int64_t n, const double* __restrict__ x, const double* __restrict__ dscale,
-                            float* __restrict__ xs) {
+                            float* __restrict__ xs, const int32_t* __restrict__ perm) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < n) xs[i] = (float)(x[i] / dscale[i]);
+  if (i < n) xs[i] = (float)(x[perm ? (int64_t)perm[i] : i] / dscale[i]);
 }
 __global__ void k_from_scaled(int64_t n, const float* __restrict__ ys, const double* __restrict__ dscale,
-                              double* __restrict__ y) {
+                              double* __restrict__ y, const int32_t* __restrict__ perm) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < n) y[i] = (double)ys[i] / dscale[i];
+  if (i < n) y[perm ? (int64_t)perm[i] : i] = (double)ys[i] / dscale[i];
+}
+
+// ---- banded row order of the compact operator
+// Mesh nodes come in lexicographic order (x fastest, then y, then z), so a row's z
+// neighbours are a whole z-plane of rows away; where planes hold hundreds of
+// thousands of nodes, the input entries a plane's rows gather are evicted from L2
+// before the next plane but one gathers them again. The banded order cuts y into bands
+// of `band` finest-grid spacings and takes the bands one after another, each in the
+// node order, so that a row and its z neighbour are one band-plane apart. It is a
+// stable partition of the node order by band; the FP32 solve runs in this order and
+// maps rows to nodes where it meets FP64 vectors (residual, accumulation).
+__global__ void k_leaf_edge(int64_t L, const int32_t* __restrict__ corner, const double* __restrict__ node_xyz,
+                            const double* __restrict__ hang_xyz, double* __restrict__ out) {
+  const int64_t l = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (l >= L) return;
+  const int32_t a = corner[8 * l], b = corner[8 * l + 7];  // opposite corners (z-order 0 and 7)
+  const double xa = a >= 0 ? node_xyz[3 * (int64_t)a] : hang_xyz[3 * (int64_t)(-1 - a)];
+  const double xb = b >= 0 ? node_xyz[3 * (int64_t)b] : hang_xyz[3 * (int64_t)(-1 - b)];
+  out[l] = fabs(xb - xa);
+}
+
+// per-block minimum and maximum of a[off + stride * i], i < n
+__global__ void __launch_bounds__(256) k_minmax(int64_t n, const double* __restrict__ a, int64_t stride, int64_t off,
+                                                double* __restrict__ part) {
+  __shared__ double smin[256], smax[256];
+  double mn = INFINITY, mx = -INFINITY;
+  for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (int64_t)gridDim.x * 256) {
+    const double v = a[off + stride * i];
+    mn = fmin(mn, v);
+    mx = fmax(mx, v);
+  }
+  smin[threadIdx.x] = mn;
+  smax[threadIdx.x] = mx;
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w) {
+      smin[threadIdx.x] = fmin(smin[threadIdx.x], smin[threadIdx.x + w]);
+      smax[threadIdx.x] = fmax(smax[threadIdx.x], smax[threadIdx.x + w]);
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    part[2 * blockIdx.x] = smin[0];
+    part[2 * blockIdx.x + 1] = smax[0];
+  }
+}
+
+__global__ void k_band_key(int64_t n, const double* __restrict__ node_xyz, double y0, double inv_w,
+                           uint32_t* __restrict__ key, int32_t* __restrict__ id) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  key[i] = (uint32_t)floor((node_xyz[3 * i + 1] - y0) * inv_w);
+  id[i] = (int32_t)i;
+}
+
+__global__ void k_inv_perm(int64_t n, const int32_t* __restrict__ perm, int32_t* __restrict__ inv) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) inv[perm[i]] = (int32_t)i;
+}
+
+__global__ void k_perm_len(int64_t n, const int32_t* __restrict__ perm, const int32_t* __restrict__ rowptr,
+                           int64_t* __restrict__ len) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) len[i] = rowptr[perm[i] + 1] - rowptr[perm[i]];
+}
+
+// row i of P A P^T: row perm[i] of A with columns renumbered by inv, sorted ascending
+__global__ void k_perm_fill(int64_t n, const int32_t* __restrict__ perm, const int32_t* __restrict__ inv,
+                            const int32_t* __restrict__ rowptr, const int32_t* __restrict__ col,
+                            const double* __restrict__ val, const int64_t* __restrict__ off,
+                            int32_t* __restrict__ rp2, int32_t* __restrict__ col2, double* __restrict__ val2) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const int32_t o = perm[i];
+  const int64_t b = off[i];
+  const int32_t k0 = rowptr[o], k1 = rowptr[o + 1];
+  for (int32_t k = k0; k < k1; ++k) {
+    const int32_t cn = inv[col[k]];
+    const double v = val[k];
+    int64_t j = b + (k - k0);
+    while (j > b && col2[j - 1] > cn) {
+      col2[j] = col2[j - 1];
+      val2[j] = val2[j - 1];
+      --j;
+    }
+    col2[j] = cn;
+    val2[j] = v;
+  }
+  rp2[i] = (int32_t)b;
+  if (i == n - 1) rp2[n] = (int32_t)(b + (k1 - k0));
 }
 
 Sell sell_view(const ngpb_ctx* c) {
@@ -1167,12 +1264,99 @@ void launch_sell(ngpb_ctx* c, bool dot, const float* x, float* y, double* part) 
 
 }  // namespace
 
-void build_compact(ngpb_ctx* c) {
+namespace {
+// Decides the row order of the compact operator and, for the banded order, computes
+// c->row_perm (FP32 row -> node). Banded needs the context's own mesh and assembly.
+bool band_rows(ngpb_ctx* c, bool allow) {
+  c->sell_banded = false;
+  if (!allow || !c->have_mesh || c->external_csr || c->n != c->N || c->L <= 0 || c->n >= INT32_MAX) return false;
+  const int band = c->row_band;
+  if (band <= 0) return false;
+  const int64_t n = c->n, L = c->L;
+  DBuf<double> edge;
+  edge.resize((size_t)L);
+  hipLaunchKernelGGL(k_leaf_edge, dim3(blocks_for(L, 256)), dim3(256), 0, c->stream, L, c->leaf_corner.p,
+                     c->node_xyz.p, c->hang_xyz.p, edge.p);
+  const unsigned gb = (unsigned)std::min<int64_t>(1024, blocks_for(std::max(L, n), 256));
+  DBuf<double> part;
+  part.resize((size_t)4 * gb);
+  hipLaunchKernelGGL(k_minmax, dim3(gb), dim3(256), 0, c->stream, L, edge.p, (int64_t)1, (int64_t)0, part.p);
+  hipLaunchKernelGGL(k_minmax, dim3(gb), dim3(256), 0, c->stream, n, c->node_xyz.p, (int64_t)3, (int64_t)1,
+                     part.p + 2 * gb);
+  NGPB_HIP(hipGetLastError());
+  std::vector<double> hp((size_t)4 * gb);
+  NGPB_HIP(hipMemcpyAsync(hp.data(), part.p, hp.size() * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  NGPB_HIP(hipStreamSynchronize(c->stream));
+  double hmin = INFINITY, y0 = INFINITY, y1 = -INFINITY;
+  for (unsigned k = 0; k < gb; ++k) {
+    hmin = std::min(hmin, hp[2 * k]);
+    y0 = std::min(y0, hp[2 * gb + 2 * k]);
+    y1 = std::max(y1, hp[2 * gb + 2 * k + 1]);
+  }
+  if (!(hmin > 0.0) || !std::isfinite(y0) || !std::isfinite(y1)) return false;
+  const double w = band * hmin;
+  const double nb = std::floor((y1 - y0) / w) + 1.0;
+  if (nb < 2.0 || nb > 1e6) return false;
+  int bits = 1;
+  while ((double)(1u << bits) < nb + 1.0) ++bits;
+  DBuf<uint32_t> key, key2;
+  DBuf<int32_t> id;
+  key.resize((size_t)n);
+  key2.resize((size_t)n);
+  id.resize((size_t)n);
+  c->row_perm.resize((size_t)n);
+  hipLaunchKernelGGL(k_band_key, dim3(blocks_for(n, 256)), dim3(256), 0, c->stream, n, c->node_xyz.p, y0, 1.0 / w,
+                     key.p, id.p);
+  NGPB_HIP(hipGetLastError());
+  size_t tb = 0;
+  NGPB_HIP(hipcub::DeviceRadixSort::SortPairs(nullptr, tb, key.p, key2.p, id.p, c->row_perm.p, (int)n, 0, bits,
+                                              c->stream));
+  DBuf<uint8_t> tmp;
+  tmp.resize(std::max<size_t>(tb, 1));
+  NGPB_HIP(hipcub::DeviceRadixSort::SortPairs(tmp.p, tb, key.p, key2.p, id.p, c->row_perm.p, (int)n, 0, bits,
+                                              c->stream));
+  NGPB_HIP(hipStreamSynchronize(c->stream));
+  c->sell_banded = true;
+  c->sell_band = band;
+  return true;
+}
+}  // namespace
+
+void build_compact_from(ngpb_ctx* c, const int32_t* rowptr, const int32_t* colidx, const double* vals);
+
+void build_compact(ngpb_ctx* c, bool allow_band) {
+  if (!band_rows(c, allow_band)) {
+    build_compact_from(c, c->rowptr.p, c->col.p, c->val.p);
+    return;
+  }
+  const int64_t n = c->n, nnz = c->nnz;
+  DBuf<int32_t> inv, rp2, col2;
+  DBuf<int64_t> len, off;
+  DBuf<double> val2;
+  inv.resize((size_t)n);
+  len.resize((size_t)n);
+  off.resize((size_t)n);
+  rp2.resize((size_t)n + 1);
+  col2.resize((size_t)std::max<int64_t>(nnz, 1));
+  val2.resize((size_t)std::max<int64_t>(nnz, 1));
+  const unsigned g = (unsigned)blocks_for(n, 256);
+  hipLaunchKernelGGL(k_inv_perm, dim3(g), dim3(256), 0, c->stream, n, c->row_perm.p, inv.p);
+  hipLaunchKernelGGL(k_perm_len, dim3(g), dim3(256), 0, c->stream, n, c->row_perm.p, c->rowptr.p, len.p);
+  NGPB_HIP(hipGetLastError());
+  const int64_t tot = exclusive_scan(c, len.p, off.p, n);
+  NGPB_REQUIRE(tot == nnz, NGPB_E_STATE, "banded row order: entry count mismatch");
+  hipLaunchKernelGGL(k_perm_fill, dim3(g), dim3(256), 0, c->stream, n, c->row_perm.p, inv.p, c->rowptr.p, c->col.p,
+                     c->val.p, off.p, rp2.p, col2.p, val2.p);
+  NGPB_HIP(hipGetLastError());
+  build_compact_from(c, rp2.p, col2.p, val2.p);
+}
+
+void build_compact_from(ngpb_ctx* c, const int32_t* rowptr, const int32_t* colidx, const double* vals) {
   const int64_t n = c->n;
   const int64_t ns = (n + kW - 1) / kW;
   c->n_slices = ns;
   c->dscale.resize((size_t)n);
-  hipLaunchKernelGGL(k_scale, dim3(blocks_for(n, 256)), dim3(256), 0, c->stream, n, c->rowptr.p, c->col.p, c->val.p,
+  hipLaunchKernelGGL(k_scale, dim3(blocks_for(n, 256)), dim3(256), 0, c->stream, n, rowptr, colidx, vals,
                      c->dscale.p);
   c->slice_width.resize((size_t)ns);
   // per-slice counts and offsets in the context's scratch slots
@@ -1189,7 +1373,7 @@ void build_compact(ngpb_ctx* c) {
   IArr bad{tmp_buf<int>(c, slot++, 1)};
   NGPB_HIP(hipMemsetAsync(bad.p, 0, sizeof(int), c->stream));
   const unsigned g = (unsigned)((ns + (kBS / kW) - 1) / (kBS / kW));
-  hipLaunchKernelGGL(k_sell_shape, dim3(g), dim3(kBS), 0, c->stream, n, ns, c->rowptr.p, c->col.p, c->val.p,
+  hipLaunchKernelGGL(k_sell_shape, dim3(g), dim3(kBS), 0, c->stream, n, ns, rowptr, colidx, vals,
                      c->dscale.p, c->sell_allow_const ? 1 : 0, cslots.p, c->slice_width.p, wcount.p, widec.p, ovc.p,
                      dslots.p, pkc.p, pkd.p, pkg.p, bad.p);
   NGPB_HIP(hipGetLastError());
@@ -1252,7 +1436,7 @@ void build_compact(ngpb_ctx* c) {
   c->sell_ov_cnt.resize((size_t)n);
   c->sell_ov_val.resize((size_t)std::max<int64_t>(nov, 1));
   c->sell_ov_col.resize((size_t)std::max<int64_t>(nov, 1));
-  hipLaunchKernelGGL(k_sell_fill, dim3(g), dim3(kBS), 0, c->stream, n, ns, c->rowptr.p, c->col.p, c->val.p, c->dscale.p,
+  hipLaunchKernelGGL(k_sell_fill, dim3(g), dim3(kBS), 0, c->stream, n, ns, rowptr, colidx, vals, c->dscale.p,
                      c->slice_width.p, c->slice_base_ptr.p, wide_off.p, widec.p, ov_off.p, ovc.p, c->sell_val.p,
                      c->sell_off.p, c->sell_base.p, c->sell_col32.p, c->slice_wide_ptr.p, c->slice_ov_ptr.p,
                      c->sell_ov_cnt.p, c->sell_ov_val.p, c->sell_ov_col.p, pk_off.p, pk_doff.p, pk_goff.p,
@@ -1321,9 +1505,11 @@ void spmv_compact(ngpb_ctx* c, const double* x, double* y) {
   const int64_t n = c->n;
   float* pin = spmv_input(c);
   c->qs.reserve((size_t)n);
-  hipLaunchKernelGGL(k_to_scaled, dim3(blocks_for(n, 256)), dim3(256), 0, c->stream, n, x, c->dscale.p, pin);
+  const int32_t* perm = c->sell_banded ? c->row_perm.p : nullptr;
+  hipLaunchKernelGGL(k_to_scaled, dim3(blocks_for(n, 256)), dim3(256), 0, c->stream, n, x, c->dscale.p, pin, perm);
   launch_sell(c, false, pin, c->qs.p, nullptr);
-  hipLaunchKernelGGL(k_from_scaled, dim3(blocks_for(n, 256)), dim3(256), 0, c->stream, n, c->qs.p, c->dscale.p, y);
+  hipLaunchKernelGGL(k_from_scaled, dim3(blocks_for(n, 256)), dim3(256), 0, c->stream, n, c->qs.p, c->dscale.p, y,
+                     perm);
   NGPB_HIP(hipGetLastError());
 }
 
@@ -1357,6 +1543,7 @@ void solve_mixed(ngpb_ctx* c, const ngpb_solve_opts& o, ngpb_solve_stats& st) {
   c->xs.reserve((size_t)n);
   c->rs.reserve((size_t)n);
   float* const pv = spmv_input(c);
+  const int32_t* const perm = c->sell_banded ? c->row_perm.p : nullptr;  // FP32 row -> node
   c->qs.reserve((size_t)n);
   const unsigned gv = vec_grid(n);
   const unsigned gs = (unsigned)sell_partials(c);
@@ -1375,7 +1562,7 @@ void solve_mixed(ngpb_ctx* c, const ngpb_solve_opts& o, ngpb_solve_stats& st) {
   NGPB_HIP(hipEventCreate(&e1));
   NGPB_HIP(hipEventRecord(e0, c->stream));
   hipLaunchKernelGGL(k_mx_resid, dim3(gv), dim3(kBS), 0, c->stream, n, c->rhs.p, (const double*)nullptr, c->dscale.p,
-                     c->rs.p, c->x.p, c->xs.p, pv, 1, part);
+                     c->rs.p, c->x.p, c->xs.p, pv, 1, part, perm);
   hipLaunchKernelGGL(k_mx_fin_resid, dim3(1), dim3(kFin), 0, c->stream, ks, part, (int)gv, 1);
   NGPB_HIP(hipGetLastError());
   NGPB_HIP(hipMemcpyAsync(h, ks, sizeof(KrylovScalars), hipMemcpyDeviceToHost, c->stream));
@@ -1421,12 +1608,12 @@ void solve_mixed(ngpb_ctx* c, const ngpb_solve_opts& o, ngpb_solve_stats& st) {
       }
       iters = h->iter;
       // FP64 residual replacement
-      hipLaunchKernelGGL(k_mx_accum, dim3(gv), dim3(kBS), 0, c->stream, n, c->dscale.p, c->x.p, c->xs.p, pv, ks);
+      hipLaunchKernelGGL(k_mx_accum, dim3(gv), dim3(kBS), 0, c->stream, n, c->dscale.p, c->x.p, c->xs.p, pv, ks, perm);
       prof_begin(c, 0, -1);
       spmv_fp64(c, c->x.p, c->q.p);
       prof_end(c, 0);
       hipLaunchKernelGGL(k_mx_resid, dim3(gv), dim3(kBS), 0, c->stream, n, c->rhs.p, c->q.p, c->dscale.p, c->rs.p, c->x.p,
-                         c->xs.p, pv, 0, part);
+                         c->xs.p, pv, 0, part, perm);
       hipLaunchKernelGGL(k_mx_fin_resid, dim3(1), dim3(kFin), 0, c->stream, ks, part, (int)gv, 0);
       NGPB_HIP(hipGetLastError());
       NGPB_HIP(hipMemcpyAsync(h, ks, sizeof(KrylovScalars), hipMemcpyDeviceToHost, c->stream));

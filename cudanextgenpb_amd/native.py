@@ -118,6 +118,8 @@ SIGNATURES = {
     "ngpb_set_profiling": (_I, [_P, _I]),
     "ngpb_kernel_stats": (_I, [_P, _I, _P, _P, _P]),
     "ngpb_time_copy": (_I, [_P, _I64, _I, _P]),
+    "ngpb_set_row_order": (_I, [_P, _I]),
+    "ngpb_row_order": (_I, [_P, _P]),
     "ngpb_operator_info": (_I, [_P, _P]),
     "ngpb_set_compact_options": (_I, [_P, _I]),
     "ngpb_set_spmv_schedule": (_I, [_P, _I]),
@@ -430,6 +432,17 @@ class Context:
 
     def set_compact_options(self, constant_slices: bool = True):
         self._check(lib().ngpb_set_compact_options(self._h, 1 if constant_slices else 0), "set_compact_options")
+
+    def set_row_order(self, band: int):
+        """Row order of the compact FP32 operator: 0 node order, > 0 y-bands of that
+        many finest-grid spacings."""
+        self._check(lib().ngpb_set_row_order(self._h, int(band)), "set_row_order")
+
+    def row_order(self) -> int:
+        """Band of the compact operator's row order (0: node order)."""
+        b = ctypes.c_int(0)
+        self._check(lib().ngpb_row_order(self._h, ctypes.byref(b)), "row_order")
+        return int(b.value)
 
     def set_spmv_schedule(self, schedule: int):
         self._check(lib().ngpb_set_spmv_schedule(self._h, int(schedule)), "set_spmv_schedule")
