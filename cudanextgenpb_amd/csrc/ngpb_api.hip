@@ -303,6 +303,45 @@ int ngpb_build_octree(ngpb_ctx* c, const double* origin, double hf, const int64_
   });
 }
 
+int ngpb_build_octree_surface(ngpb_ctx* c, const double* origin, double hf, const int64_t* n, int smax, int s_uni,
+                              int cycles, int64_t n_balls, const double* xyz, const double* r, double probe,
+                              int64_t* sizes) {
+  return guarded(c, [&] {
+    not_in_batch(c);
+    NGPB_REQUIRE(origin && n, NGPB_E_ARG, "NULL origin or n");
+    NGPB_REQUIRE(smax >= 0 && smax <= 10, NGPB_E_ARG, "smax must be in 0..10");
+    NGPB_REQUIRE(s_uni >= 0 && s_uni <= smax, NGPB_E_ARG, "s_uni must be in 0..smax");
+    NGPB_REQUIRE(cycles <= 64, NGPB_E_ARG, "at most 64 cycles");
+    NGPB_REQUIRE(std::isfinite(hf) && hf > 0, NGPB_E_ARG, "hf must be finite and > 0");
+    NGPB_REQUIRE(std::isfinite(probe) && probe >= 0, NGPB_E_ARG, "probe must be finite and >= 0");
+    for (int d = 0; d < 3; ++d) {
+      NGPB_REQUIRE(std::isfinite(origin[d]), NGPB_E_ARG, "non-finite origin");
+      NGPB_REQUIRE(n[d] > 0 && n[d] <= ((int64_t)1 << 21) && n[d] % ((int64_t)1 << smax) == 0, NGPB_E_ARG,
+                   "n must be positive multiples of 2^smax, at most 2^21");
+    }
+    NGPB_REQUIRE((double)(n[0] + 1) * (double)(n[1] + 1) * (double)(n[2] + 1) < 4294967296.0, NGPB_E_ARG,
+                 "at most 2^32 - 1 grid vertices");
+    NGPB_REQUIRE((double)(n[0] + 1) * (double)(n[1] + 1) < 2147483647.0, NGPB_E_ARG, "too many grid z-lines");
+    NGPB_REQUIRE(n_balls >= 0 && n_balls < INT32_MAX, NGPB_E_ARG, "bad ball count");
+    NGPB_REQUIRE(n_balls == 0 || (xyz && r), NGPB_E_ARG, "NULL ball array");
+    for (int64_t a = 0; a < n_balls; ++a)
+      NGPB_REQUIRE(std::isfinite(r[a]) && r[a] >= 0 && std::isfinite(xyz[3 * a]) && std::isfinite(xyz[3 * a + 1]) &&
+                       std::isfinite(xyz[3 * a + 2]),
+                   NGPB_E_ARG, "ball centres and radii must be finite, radii >= 0");
+    c->have_mesh = c->have_dirichlet = c->have_bnd_values = c->have_charges = false;
+    invalidate_after_mesh(c);
+    const int cyc = cycles < 0 ? std::max(s_uni, smax - s_uni) : cycles;
+    const ngpb::OctreeSizes z =
+        ngpb::run_build_octree_surface(c, origin, hf, n, smax, s_uni, cyc, n_balls, xyz, r, probe);
+    if (sizes) {
+      sizes[0] = z.N;
+      sizes[1] = z.H;
+      sizes[2] = z.L;
+      sizes[3] = z.NB;
+    }
+  });
+}
+
 int ngpb_get_mesh(ngpb_ctx* c, int64_t* sizes, double* node_xyz, double* hang_xyz, int32_t* hang_np,
                   int32_t* hang_par, int32_t* leaf_corner, int32_t* bnd_nodes) {
   return guarded(c, [&] {

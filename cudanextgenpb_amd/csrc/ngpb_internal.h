@@ -334,12 +334,19 @@ void run_numeric(ngpb_ctx* c);
 void run_dirichlet_values(ngpb_ctx* c, int bc_type);
 void run_phi_on_atoms(ngpb_ctx* c, double* d_out);
 void run_cast_rays(ngpb_ctx* c, int64_t na, const double* xyz, const double* r, double probe);
+// rays of every grid line along z through (origin0 + i hf, origin1 + j hf), i <= n0,
+// j <= n1, into c->rays[2] (sphere union, or SES for probe > 0)
+void cast_grid_zlines(ngpb_ctx* c, const double* origin, double hf, int64_t n0, int64_t n1, int64_t na,
+                      const double* xyz, const double* r, double probe);
 void run_locate_atoms(ngpb_ctx* c, int64_t A, const double* h_xyz, double tol, int32_t* h_owner);
 struct OctreeSizes {
   int64_t N, H, L, NB;
 };
 OctreeSizes run_build_octree(ngpb_ctx* c, const double* origin, double hf, const int64_t* n, int smax,
                              const double* band, int64_t na, const double* h_xyz, const double* h_r);
+OctreeSizes run_build_octree_surface(ngpb_ctx* c, const double* origin, double hf, const int64_t* n, int smax,
+                                     int s_uni, int cycles, int64_t na, const double* h_xyz, const double* h_r,
+                                     double probe);
 
 // Krylov
 unsigned vec_grid(int64_t n);

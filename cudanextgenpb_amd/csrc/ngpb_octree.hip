@@ -218,6 +218,152 @@ __global__ void k_balance_flag(const OLeaf* __restrict__ lv, int64_t L, Geom g, 
   cnt[i] = split ? kSplit : kKeep;
 }
 
+// ---- marker-driven refinement (refine_surface, src/pb_class.cpp:1467-1745)
+// In/out flag of a grid vertex as is_in_ns_surf gives it for the z direction
+// (src/pb_class.cpp:769-806): the parity of the crossings at or below its z on the
+// z-line through it. Z holds the rays of every grid z-line; lmap maps grid line
+// i (n1 + 1) + j to its ray (-1: no crossings).
+struct ZRays {
+  const int32_t* lmap;
+  const int64_t* ptr;
+  const double* inters;
+};
+
+__device__ int zflag(const Geom& g, const ZRays& Z, int64_t x, int64_t y, int64_t z) {
+  const int32_t r = Z.lmap[x * g.vn[1] + y];
+  if (r < 0) return 0;
+  const double zc = gcoord(g, 2, z);
+  int64_t lo = Z.ptr[r], hi = Z.ptr[r + 1];
+  const int64_t a = lo;
+  while (lo < hi) {  // first crossing above zc
+    const int64_t mid = lo + (hi - lo) / 2;
+    if (Z.inters[mid] <= zc) lo = mid + 1;
+    else hi = mid;
+  }
+  return (int)((lo - a) & 1);
+}
+
+// a vertex is regular (not hanging) when it is a corner of every leaf around it
+__device__ bool vertex_regular(const Geom& g, const int8_t* __restrict__ lev, int64_t px, int64_t py, int64_t pz) {
+  for (int64_t z = pz - 1; z <= pz; ++z) {
+    if (z < 0 || z >= g.n[2]) continue;
+    for (int64_t y = py - 1; y <= py; ++y) {
+      if (y < 0 || y >= g.n[1]) continue;
+      for (int64_t x = px - 1; x <= px; ++x) {
+        if (x < 0 || x >= g.n[0]) continue;
+        const int64_t S = (int64_t)1 << lev[cell_at(g, x, y, z)];
+        if (px % S || py % S || pz % S) return false;
+      }
+    }
+  }
+  return true;
+}
+
+// flags of the regular corners of the cube (x, y, z) of edge S: bit 0 some 0, bit 1 some 1
+__device__ int corner_flags(const Geom& g, const int8_t* __restrict__ lev, const ZRays& Z, int64_t x, int64_t y,
+                            int64_t z, int64_t S) {
+  int seen = 0;
+  for (int k = 0; k < 8; ++k) {
+    const int64_t px = x + ((k & 1) ? S : 0), py = y + ((k & 2) ? S : 0), pz = z + ((k & 4) ? S : 0);
+    if (!vertex_regular(g, lev, px, py, pz)) continue;
+    seen |= zflag(g, Z, px, py, pz) ? 2 : 1;
+  }
+  return seen;
+}
+
+__global__ void k_zline_map(int64_t nr, const double* __restrict__ key, Geom g, int32_t* __restrict__ lmap) {
+  const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= nr) return;
+  const int64_t i = llround((key[2 * r] - g.origin[0]) / g.hf), j = llround((key[2 * r + 1] - g.origin[1]) / g.hf);
+  if (i >= 0 && i < g.vn[0] && j >= 0 && j < g.vn[1]) lmap[i * g.vn[1] + j] = (int32_t)r;
+}
+
+__global__ void k_fill_i32(int64_t n, int32_t v, int32_t* __restrict__ a) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) a[i] = v;
+}
+
+// uniform leaves of level s
+__global__ void k_uniform(Geom g, int s, int64_t m0, int64_t m1, int64_t cnt, OLeaf* __restrict__ out) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= cnt) return;
+  const int32_t S = 1 << s;
+  out[i] = OLeaf{(int32_t)(i % m0) * S, (int32_t)((i / m0) % m1) * S, (int32_t)(i / (m0 * m1)) * S, s};
+}
+
+// refinement marker: a leaf above the finest level whose regular corners are on both
+// sides of the surface is split once (src/pb_class.cpp:1569-1607; the atom-box test
+// there never fires, as the atom list is empty at that point)
+__global__ void k_surf_split(const OLeaf* __restrict__ lv, int64_t L, Geom g, const int8_t* __restrict__ lev,
+                             ZRays Z, uint8_t* __restrict__ flag, int64_t* __restrict__ cnt) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= L) return;
+  const OLeaf l = lv[i];
+  const bool split = l.s > 0 && corner_flags(g, lev, Z, l.x, l.y, l.z, (int64_t)1 << l.s) == 3;
+  flag[i] = split ? 1 : 0;
+  cnt[i] = split ? kSplit : kKeep;
+}
+
+// coarsening marker (src/pb_class.cpp:1700-1741): a leaf below the coarsest level
+// whose regular corners are all on one side; a family of 8 sibling leaves is merged
+// when every member is marked. The first child decides for its family and writes the
+// decision at its anchor cell.
+__global__ void k_surf_coarsen_decide(const OLeaf* __restrict__ lv, int64_t L, Geom g, int smax,
+                                      const int8_t* __restrict__ lev, ZRays Z, uint8_t* __restrict__ dmap) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= L) return;
+  const OLeaf l = lv[i];
+  if (l.s >= smax) return;
+  const int64_t P = (int64_t)1 << (l.s + 1), S = (int64_t)1 << l.s;
+  if (l.x % P || l.y % P || l.z % P) return;  // not a first child
+  bool merge = true;
+  for (int k = 0; k < 8 && merge; ++k) {
+    const int64_t x = l.x + ((k & 1) ? S : 0), y = l.y + ((k & 2) ? S : 0), z = l.z + ((k & 4) ? S : 0);
+    if (lev[cell_at(g, x, y, z)] != l.s) merge = false;  // sibling is not a leaf of this level
+    else if (corner_flags(g, lev, Z, x, y, z, S) == 3) merge = false;
+  }
+  dmap[cell_at(g, l.x, l.y, l.z)] = merge ? 1 : 0;
+}
+
+// emission after the coarsening decision: a merged family becomes its parent (written
+// by the first child), every other leaf is kept
+__global__ void k_surf_coarsen_count(const OLeaf* __restrict__ lv, int64_t L, Geom g, int smax,
+                                     const int8_t* __restrict__ lev, const uint8_t* __restrict__ dmap,
+                                     int64_t* __restrict__ cnt) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= L) return;
+  const OLeaf l = lv[i];
+  int64_t c = 1;
+  if (l.s < smax) {
+    const int64_t P = (int64_t)1 << (l.s + 1), S = (int64_t)1 << l.s;
+    const int64_t px = l.x - l.x % P, py = l.y - l.y % P, pz = l.z - l.z % P;
+    bool family = true;
+    for (int k = 0; k < 8 && family; ++k)
+      family = lev[cell_at(g, px + ((k & 1) ? S : 0), py + ((k & 2) ? S : 0), pz + ((k & 4) ? S : 0))] == l.s;
+    if (family && dmap[cell_at(g, px, py, pz)]) c = (l.x == px && l.y == py && l.z == pz) ? 1 : 0;
+  }
+  cnt[i] = c;
+}
+
+__global__ void k_surf_coarsen_emit(const OLeaf* __restrict__ lv, int64_t L, Geom g, int smax,
+                                    const int8_t* __restrict__ lev, const uint8_t* __restrict__ dmap,
+                                    const int64_t* __restrict__ off, const int64_t* __restrict__ cnt,
+                                    OLeaf* __restrict__ out) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= L) return;
+  if (cnt[i] == 0) return;
+  OLeaf l = lv[i];
+  if (l.s < smax) {
+    const int64_t P = (int64_t)1 << (l.s + 1), S = (int64_t)1 << l.s;
+    const int64_t px = l.x - l.x % P, py = l.y - l.y % P, pz = l.z - l.z % P;
+    bool family = true;
+    for (int k = 0; k < 8 && family; ++k)
+      family = lev[cell_at(g, px + ((k & 1) ? S : 0), py + ((k & 2) ? S : 0), pz + ((k & 4) ? S : 0))] == l.s;
+    if (family && dmap[cell_at(g, px, py, pz)]) l.s += 1;  // the first child becomes the parent
+  }
+  out[off[i]] = l;
+}
+
 // bit interleave of a 21-bit coordinate (x to bits 0, 3, 6, ...)
 __device__ __forceinline__ uint64_t spread3(uint64_t v) {
   uint64_t r = 0;
@@ -376,6 +522,41 @@ std::pair<int64_t, int64_t> split_pass(ngpb_ctx* c, const OLeaf* in, int64_t n, 
   return {nk, ns};
 }
 
+// 2:1 balance on the level map of the finest cells: Jacobi sweeps of k_balance_flag
+// until no leaf is split (at most kBalanceSweeps)
+void balance_leaves(ngpb_ctx* c, const Geom& g, DBuf<OLeaf>& leaves, int64_t& L) {
+  hipStream_t st = c->stream;
+  const int64_t ncell = g.n[0] * g.n[1] * g.n[2];
+  DBuf<OLeaf> next(st);
+  DBuf<uint8_t> flag(st);
+  DBuf<int64_t> cnt(st);
+  DBuf<int8_t> lev(st);
+  lev.resize((size_t)ncell);
+  hipLaunchKernelGGL(k_paint, dim3(blocks_for(L, 64)), dim3(64), 0, st, leaves.p, L, nullptr, g, lev.p);
+  NGPB_HIP(hipGetLastError());
+  for (int sweep = 0; sweep < kBalanceSweeps; ++sweep) {
+    flag.resize((size_t)L);
+    cnt.resize((size_t)L);
+    hipLaunchKernelGGL(k_balance_flag, dim3(blocks_for(L, 128)), dim3(128), 0, st, leaves.p, L, g, lev.p, flag.p,
+                       cnt.p);
+    NGPB_HIP(hipGetLastError());
+    const int64_t tot = exclusive_scan(c, cnt.p, cnt.p, L);
+    const int64_t nk = tot & 0xffffffffLL, ns = tot >> 32;
+    if (ns == 0) break;
+    NGPB_REQUIRE(nk + 8 * ns < INT32_MAX, NGPB_E_ARG, "leaf count out of range");
+    hipLaunchKernelGGL(k_paint, dim3(blocks_for(L, 64)), dim3(64), 0, st, leaves.p, L, flag.p, g, lev.p);
+    next.resize((size_t)(nk + 8 * ns));  // kept leaves first, children after them
+    hipLaunchKernelGGL(k_split_emit, dim3(blocks_for(L, 256)), dim3(256), 0, st, leaves.p, L, flag.p, cnt.p, next.p,
+                       next.p + nk);
+    NGPB_HIP(hipGetLastError());
+    std::swap(leaves.p, next.p);
+    std::swap(leaves.n, next.n);
+    L = nk + 8 * ns;
+  }
+}
+
+OctreeSizes finish_octree(ngpb_ctx* c, const Geom& g, const int64_t* n, DBuf<OLeaf>& leaves, int64_t L);
+
 }  // namespace
 
 OctreeSizes run_build_octree(ngpb_ctx* c, const double* origin, double hf, const int64_t* n, int smax,
@@ -389,8 +570,6 @@ OctreeSizes run_build_octree(ngpb_ctx* c, const double* origin, double hf, const
   g.hf = hf;
   for (int s = 0; s < smax; ++s) g.band[s] = band[s];
   const int64_t R = (int64_t)1 << smax;
-  const int64_t ncell = n[0] * n[1] * n[2];
-  const int64_t V = g.vn[0] * g.vn[1] * g.vn[2];
   hipStream_t st = c->stream;
 
   // ---- atoms binned on the host (a few arrays of the atom count), uploaded once
@@ -517,34 +696,124 @@ OctreeSizes run_build_octree(ngpb_ctx* c, const double* origin, double hf, const
   }
   work.release();
 
-  // ---- 2:1 balance on the level map of the finest cells
-  {
-    DBuf<int8_t> lev(st);
-    lev.resize((size_t)ncell);
+  balance_leaves(c, g, leaves, L);
+  return finish_octree(c, g, n, leaves, L);
+}
+
+OctreeSizes run_build_octree_surface(ngpb_ctx* c, const double* origin, double hf, const int64_t* n, int smax,
+                                     int s_uni, int cycles, int64_t na, const double* h_xyz, const double* h_r,
+                                     double probe) {
+  Geom g{};
+  for (int d = 0; d < 3; ++d) {
+    g.origin[d] = origin[d];
+    g.n[d] = n[d];
+    g.vn[d] = n[d] + 1;
+  }
+  g.hf = hf;
+  hipStream_t st = c->stream;
+  const int64_t ncell = n[0] * n[1] * n[2];
+  // rays of every grid z-line, taken out of the context's ray table
+  cast_grid_zlines(c, origin, hf, n[0], n[1], na, h_xyz, h_r, probe);
+  auto& R = c->rays[2];
+  DBuf<int64_t> zptr;  // plain allocations, like the table they are taken from
+  DBuf<double> zint;
+  std::swap(zptr.p, R.ptr.p);
+  std::swap(zptr.n, R.ptr.n);
+  std::swap(zint.p, R.inters.p);
+  std::swap(zint.n, R.inters.n);
+  DBuf<int32_t> lmap(st);
+  const int64_t nlines = g.vn[0] * g.vn[1];
+  lmap.resize((size_t)nlines);
+  hipLaunchKernelGGL(k_fill_i32, dim3(blocks_for(nlines, 256)), dim3(256), 0, st, nlines, (int32_t)-1, lmap.p);
+  if (R.n > 0)
+    hipLaunchKernelGGL(k_zline_map, dim3(blocks_for(R.n, 256)), dim3(256), 0, st, R.n, R.key.p, g, lmap.p);
+  NGPB_HIP(hipGetLastError());
+  NGPB_HIP(hipStreamSynchronize(st));  // R.key is released below
+  // the context's table stays empty: the caller casts the mesh's rays afterwards
+  R.n = R.k = 0;
+  R.key.resize(2);
+  R.ptr.resize(1);
+  R.inters.resize(1);
+  R.normals.resize(3);
+  NGPB_HIP(hipMemsetAsync(R.ptr.p, 0, sizeof(int64_t), st));
+  const ZRays Z{lmap.p, zptr.p, zint.p};
+
+  // uniform start at level s_uni (init_tmesh, src/pb_class.cpp:1289-1296)
+  const int64_t U = (int64_t)1 << s_uni;
+  int64_t L = (n[0] / U) * (n[1] / U) * (n[2] / U);
+  NGPB_REQUIRE(L > 0 && L < INT32_MAX, NGPB_E_ARG, "leaf count out of range");
+  DBuf<OLeaf> leaves(st), next(st);
+  leaves.resize((size_t)L);
+  hipLaunchKernelGGL(k_uniform, dim3(blocks_for(L, 256)), dim3(256), 0, st, g, s_uni, n[0] / U, n[1] / U, L,
+                     leaves.p);
+  NGPB_HIP(hipGetLastError());
+  DBuf<int8_t> lev(st);
+  DBuf<uint8_t> flag(st), dmap(st);
+  DBuf<int64_t> cnt(st);
+  lev.resize((size_t)ncell);
+  dmap.resize((size_t)ncell);
+  auto paint = [&]() {
     hipLaunchKernelGGL(k_paint, dim3(blocks_for(L, 64)), dim3(64), 0, st, leaves.p, L, nullptr, g, lev.p);
     NGPB_HIP(hipGetLastError());
-    for (int sweep = 0; sweep < kBalanceSweeps; ++sweep) {
-      flag.resize((size_t)L);
-      cnt.resize((size_t)L);
-      hipLaunchKernelGGL(k_balance_flag, dim3(blocks_for(L, 128)), dim3(128), 0, st, leaves.p, L, g, lev.p, flag.p,
-                         cnt.p);
-      NGPB_HIP(hipGetLastError());
+  };
+  for (int cyc = 0; cyc < cycles; ++cyc) {
+    // refinement, then 2:1 balance (tmsh.refine (0, 1))
+    paint();
+    flag.resize((size_t)L);
+    cnt.resize((size_t)L);
+    hipLaunchKernelGGL(k_surf_split, dim3(blocks_for(L, 128)), dim3(128), 0, st, leaves.p, L, g, lev.p, Z, flag.p,
+                       cnt.p);
+    NGPB_HIP(hipGetLastError());
+    {
       const int64_t tot = exclusive_scan(c, cnt.p, cnt.p, L);
       const int64_t nk = tot & 0xffffffffLL, ns = tot >> 32;
-      if (ns == 0) break;
       NGPB_REQUIRE(nk + 8 * ns < INT32_MAX, NGPB_E_ARG, "leaf count out of range");
-      hipLaunchKernelGGL(k_paint, dim3(blocks_for(L, 64)), dim3(64), 0, st, leaves.p, L, flag.p, g, lev.p);
-      next.resize((size_t)(nk + 8 * ns));  // kept leaves first, children after them
-      hipLaunchKernelGGL(k_split_emit, dim3(blocks_for(L, 256)), dim3(256), 0, st, leaves.p, L, flag.p, cnt.p, next.p,
-                         next.p + nk);
-      NGPB_HIP(hipGetLastError());
-      std::swap(leaves.p, next.p);
-      std::swap(leaves.n, next.n);
-      L = nk + 8 * ns;
+      if (ns > 0) {
+        next.resize((size_t)(nk + 8 * ns));
+        hipLaunchKernelGGL(k_split_emit, dim3(blocks_for(L, 256)), dim3(256), 0, st, leaves.p, L, flag.p, cnt.p,
+                           next.p, next.p + nk);
+        NGPB_HIP(hipGetLastError());
+        std::swap(leaves.p, next.p);
+        std::swap(leaves.n, next.n);
+        L = nk + 8 * ns;
+      }
     }
+    balance_leaves(c, g, leaves, L);
+    // coarsening, then 2:1 balance (tmsh.coarsen (0, 1))
+    paint();
+    hipLaunchKernelGGL(k_surf_coarsen_decide, dim3(blocks_for(L, 128)), dim3(128), 0, st, leaves.p, L, g, smax, lev.p,
+                       Z, dmap.p);
+    cnt.resize((size_t)L + 1);
+    hipLaunchKernelGGL(k_surf_coarsen_count, dim3(blocks_for(L, 256)), dim3(256), 0, st, leaves.p, L, g, smax, lev.p,
+                       dmap.p, cnt.p);
+    NGPB_HIP(hipGetLastError());
+    {
+      DBuf<int64_t> off(st);
+      off.resize((size_t)L);
+      const int64_t nl = exclusive_scan(c, cnt.p, off.p, L);
+      if (nl != L) {
+        next.resize((size_t)nl);
+        hipLaunchKernelGGL(k_surf_coarsen_emit, dim3(blocks_for(L, 256)), dim3(256), 0, st, leaves.p, L, g, smax,
+                           lev.p, dmap.p, off.p, cnt.p, next.p);
+        NGPB_HIP(hipGetLastError());
+        std::swap(leaves.p, next.p);
+        std::swap(leaves.n, next.n);
+        L = nl;
+      }
+    }
+    balance_leaves(c, g, leaves, L);
   }
-  next.release();
+  lev.release();
+  dmap.release();
+  return finish_octree(c, g, n, leaves, L);
+}
 
+namespace {
+
+OctreeSizes finish_octree(ngpb_ctx* c, const Geom& g, const int64_t* n, DBuf<OLeaf>& leaves, int64_t L) {
+  hipStream_t st = c->stream;
+  const int64_t V = g.vn[0] * g.vn[1] * g.vn[2];
+  DBuf<OLeaf> next(st);
   // ---- Morton order
   {
     DBuf<uint64_t> key(st), key2(st);
@@ -611,5 +880,8 @@ OctreeSizes run_build_octree(ngpb_ctx* c, const double* origin, double hf, const
   install_device_mesh(c, N, H, L, NB);
   return OctreeSizes{N, H, L, NB};
 }
+
+}  // namespace
+
 
 }  // namespace ngpb

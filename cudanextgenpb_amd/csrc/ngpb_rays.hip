@@ -289,6 +289,17 @@ __global__ void k_gather_u64(int64_t n, const int32_t* __restrict__ idx, const u
   if (i < n) dst[i] = src[idx[i]];
 }
 
+// keys of every line of the grid along axis 2 through (origin0 + i hf, origin1 + j hf),
+// i <= n0, j <= n1, i major (the order of the distinct lines of a mesh)
+__global__ void k_grid_zline_keys(int64_t n0, int64_t n1, double o0, double o1, double hf, uint64_t* __restrict__ lu,
+                                  uint64_t* __restrict__ lv) {
+  const int64_t l = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (l >= (n0 + 1) * (n1 + 1)) return;
+  const int64_t i = l / (n1 + 1), j = l % (n1 + 1);
+  lu[l] = ordered_bits(o0 + (double)i * hf);
+  lv[l] = ordered_bits(o1 + (double)j * hf);
+}
+
 }  // namespace
 
 struct SesDevice;
@@ -297,56 +308,122 @@ void ses_release(SesDevice* D);
 void ses_cast_lines(ngpb_ctx* c, const SesDevice* D, int axis, int64_t nl, const uint64_t* lu, const uint64_t* lv,
                     const double* h_xyz);
 
-void run_cast_rays(ngpb_ctx* c, int64_t na, const double* h_xyz, const double* h_r, double probe) {
-  const int64_t N = c->N, NV = c->N + c->H;
-  // solvent-excluded surface: SAS circles and vertices once, then per axis below
-  struct SesHolder {
-    SesDevice* d = nullptr;
-    ~SesHolder() { if (d) ses_release(d); }
-  } ses;
-  if (probe > 0.0 && na > 0) ses.d = ses_prepare(c, na, h_xyz, h_r, probe);
-  // balls and their bins (built on the host as the host caster does, uploaded once)
+namespace {
+
+// Rays of the nl distinct lines (lu, lv: order-preserving keys, sorted) of one axis
+// into c->rays[axis]: SES rays when ses is given, sphere-union rays otherwise (balls
+// bxyz / br on the device, h_xyz / h_r on the host).
+void cast_axis_lines(ngpb_ctx* c, int64_t na, const double* h_xyz, const double* h_r, const SesDevice* ses,
+                     const double* bxyz, const double* br, int axis, int64_t nl, const uint64_t* lu,
+                     const uint64_t* lv) {
+  const int d1 = axis == 0 ? 1 : 0;
+  const int d2 = axis == 2 ? 1 : 2;
+  auto& R = c->rays[axis];
+  if (nl == 0 || na == 0) {
+    R.n = R.k = 0;
+    R.key.resize(2);
+    R.ptr.resize(1);
+    R.inters.resize(1);
+    R.normals.resize(3);
+    NGPB_HIP(hipMemsetAsync(R.ptr.p, 0, sizeof(int64_t), c->stream));
+    NGPB_HIP(hipStreamSynchronize(c->stream));
+    return;
+  }
+  if (ses) {
+    ses_cast_lines(c, ses, axis, nl, lu, lv, h_xyz);
+    return;
+  }
+  // balls and their bins (built on the host as the host caster does)
   double rmax = 0;
   for (int64_t a = 0; a < na; ++a) rmax = std::max(rmax, h_r[a]);
   const double cell = std::max(2.0 * rmax, 1.0);
+  double lo[2] = {1e300, 1e300}, hi[2] = {-1e300, -1e300};
+  for (int64_t a = 0; a < na; ++a) {
+    lo[0] = std::min(lo[0], h_xyz[3 * a + d1]);
+    hi[0] = std::max(hi[0], h_xyz[3 * a + d1]);
+    lo[1] = std::min(lo[1], h_xyz[3 * a + d2]);
+    hi[1] = std::max(hi[1], h_xyz[3 * a + d2]);
+  }
+  const int gn0 = (int)std::floor((hi[0] - lo[0]) / cell) + 1, gn1 = (int)std::floor((hi[1] - lo[1]) / cell) + 1;
+  std::vector<int32_t> bptr((size_t)gn0 * gn1 + 1, 0), bball((size_t)std::max<int64_t>(na, 1));
+  std::vector<int32_t> bin_of((size_t)std::max<int64_t>(na, 1));
+  for (int64_t a = 0; a < na; ++a) {
+    const int b0 = (int)std::floor((h_xyz[3 * a + d1] - lo[0]) / cell);
+    const int b1 = (int)std::floor((h_xyz[3 * a + d2] - lo[1]) / cell);
+    bin_of[a] = b1 * gn0 + b0;
+    ++bptr[(size_t)bin_of[a] + 1];
+  }
+  for (size_t i = 1; i < bptr.size(); ++i) bptr[i] += bptr[i - 1];
+  {
+    std::vector<int32_t> cur(bptr.begin(), bptr.end() - 1);
+    for (int64_t a = 0; a < na; ++a) bball[(size_t)cur[(size_t)bin_of[a]]++] = (int32_t)a;
+  }
+  DBuf<int32_t> d_bptr(c->stream), d_bball(c->stream);
+  d_bptr.upload(bptr.data(), bptr.size(), c->stream);
+  d_bball.upload(bball.data(), bball.size(), c->stream);
+  BallGrid G{bxyz, br, d_bptr.p, d_bball.p, lo[0], lo[1], cell, gn0, gn1};
+  // pierced balls per line, their intervals sorted and merged
+  DBuf<int64_t> off(c->stream), mcnt(c->stream);
+  off.resize((size_t)nl);
+  mcnt.resize((size_t)nl);
+  hipLaunchKernelGGL(k_count_pierced, dim3(blocks_for(nl, 128)), dim3(128), 0, c->stream, nl, lu, lv, G, d1, d2,
+                     off.p);
+  NGPB_HIP(hipGetLastError());
+  const int64_t niv = exclusive_scan(c, off.p, off.p, nl);
+  DBuf<Iv> iv(c->stream);
+  iv.resize((size_t)std::max<int64_t>(niv, 1));
+  hipLaunchKernelGGL(k_merge_pierced_wave, dim3((unsigned)nl), dim3(64), 0, c->stream, nl, lu, lv, G, axis, d1, d2,
+                     off.p, niv, iv.p, mcnt.p);
+  NGPB_HIP(hipGetLastError());
+  // rays = lines with crossings
+  DBuf<uint8_t> keep(c->stream);
+  keep.resize((size_t)nl);
+  hipLaunchKernelGGL(k_nonzero, dim3(blocks_for(nl, 256)), dim3(256), 0, c->stream, nl, mcnt.p, keep.p);
+  DBuf<int32_t> kept(c->stream);
+  kept.resize((size_t)nl);
+  const int64_t nr = compact_flags(c, keep.p, nl, kept.p);
+  R.n = nr;
+  R.key.resize((size_t)2 * std::max<int64_t>(nr, 1));
+  R.ptr.resize((size_t)nr + 1);
+  if (nr > 0) {
+    hipLaunchKernelGGL(k_ray_keys, dim3(blocks_for(nr, 256)), dim3(256), 0, c->stream, nr, kept.p, lu, lv, mcnt.p,
+                       R.key.p, R.ptr.p);
+    R.k = exclusive_scan(c, R.ptr.p, R.ptr.p, nr);
+  } else {
+    R.k = 0;
+  }
+  NGPB_HIP(hipMemcpyAsync(R.ptr.p + nr, &R.k, sizeof(int64_t), hipMemcpyHostToDevice, c->stream));
+  R.inters.resize((size_t)std::max<int64_t>(R.k, 1));
+  R.normals.resize((size_t)3 * std::max<int64_t>(R.k, 1));
+  if (nr > 0)
+    hipLaunchKernelGGL(k_ray_fill, dim3(blocks_for(nr, 128)), dim3(128), 0, c->stream, nr, kept.p, R.ptr.p, off.p,
+                       iv.p, R.key.p, G, axis, d1, d2, R.inters.p, R.normals.p);
+  NGPB_HIP(hipGetLastError());
+  NGPB_HIP(hipStreamSynchronize(c->stream));
+}
+
+// solvent-excluded surface data for probe > 0 (released with the holder)
+struct SesHolder {
+  SesDevice* d = nullptr;
+  ~SesHolder() {
+    if (d) ses_release(d);
+  }
+};
+
+}  // namespace
+
+void run_cast_rays(ngpb_ctx* c, int64_t na, const double* h_xyz, const double* h_r, double probe) {
+  const int64_t N = c->N, NV = c->N + c->H;
+  SesHolder ses;
+  if (probe > 0.0 && na > 0) ses.d = ses_prepare(c, na, h_xyz, h_r, probe);
   DBuf<double> bxyz(c->stream), br(c->stream);
   if (na > 0) {
     bxyz.upload(h_xyz, (size_t)(3 * na), c->stream);
     br.upload(h_r, (size_t)na, c->stream);
-  } else {
-    bxyz.resize(3);
-    br.resize(1);
   }
   for (int axis = 0; axis < 3; ++axis) {
     const int d1 = axis == 0 ? 1 : 0;
     const int d2 = axis == 2 ? 1 : 2;
-    double lo[2] = {1e300, 1e300}, hi[2] = {-1e300, -1e300};
-    for (int64_t a = 0; a < na; ++a) {
-      lo[0] = std::min(lo[0], h_xyz[3 * a + d1]);
-      hi[0] = std::max(hi[0], h_xyz[3 * a + d1]);
-      lo[1] = std::min(lo[1], h_xyz[3 * a + d2]);
-      hi[1] = std::max(hi[1], h_xyz[3 * a + d2]);
-    }
-    if (na == 0) lo[0] = lo[1] = hi[0] = hi[1] = 0;
-    const int gn0 = (int)std::floor((hi[0] - lo[0]) / cell) + 1, gn1 = (int)std::floor((hi[1] - lo[1]) / cell) + 1;
-    std::vector<int32_t> bptr((size_t)gn0 * gn1 + 1, 0), bball((size_t)std::max<int64_t>(na, 1));
-    std::vector<int32_t> bin_of((size_t)std::max<int64_t>(na, 1));
-    for (int64_t a = 0; a < na; ++a) {
-      const int b0 = (int)std::floor((h_xyz[3 * a + d1] - lo[0]) / cell);
-      const int b1 = (int)std::floor((h_xyz[3 * a + d2] - lo[1]) / cell);
-      bin_of[a] = b1 * gn0 + b0;
-      ++bptr[(size_t)bin_of[a] + 1];
-    }
-    for (size_t i = 1; i < bptr.size(); ++i) bptr[i] += bptr[i - 1];
-    {
-      std::vector<int32_t> cur(bptr.begin(), bptr.end() - 1);
-      for (int64_t a = 0; a < na; ++a) bball[(size_t)cur[(size_t)bin_of[a]]++] = (int32_t)a;
-    }
-    DBuf<int32_t> d_bptr(c->stream), d_bball(c->stream);
-    d_bptr.upload(bptr.data(), bptr.size(), c->stream);
-    d_bball.upload(bball.data(), bball.size(), c->stream);
-    BallGrid G{bxyz.p, br.p, d_bptr.p, d_bball.p, lo[0], lo[1], cell, gn0, gn1};
-
     // distinct lines in lexicographic order
     DBuf<uint64_t> ku(c->stream), kv(c->stream), su(c->stream), sv(c->stream);
     ku.resize((size_t)NV);
@@ -374,62 +451,32 @@ void run_cast_rays(ngpb_ctx* c, int64_t na, const double* h_xyz, const double* h
     DBuf<uint64_t> lu(c->stream), lv(c->stream);
     lu.resize((size_t)std::max<int64_t>(nl, 1));
     lv.resize((size_t)std::max<int64_t>(nl, 1));
-    auto& R = c->rays[axis];
-    if (nl == 0 || na == 0) {
-      R.n = R.k = 0;
-      R.key.resize(2);
-      R.ptr.resize(1);
-      R.inters.resize(1);
-      R.normals.resize(3);
-      NGPB_HIP(hipMemsetAsync(R.ptr.p, 0, sizeof(int64_t), c->stream));
-      NGPB_HIP(hipStreamSynchronize(c->stream));
-      continue;
+    if (nl > 0) {
+      hipLaunchKernelGGL(k_gather_u64, dim3(blocks_for(nl, 256)), dim3(256), 0, c->stream, nl, uniq.p, ku.p, lu.p);
+      hipLaunchKernelGGL(k_gather_u64, dim3(blocks_for(nl, 256)), dim3(256), 0, c->stream, nl, uniq.p, kv.p, lv.p);
+      NGPB_HIP(hipGetLastError());
     }
-    hipLaunchKernelGGL(k_gather_u64, dim3(blocks_for(nl, 256)), dim3(256), 0, c->stream, nl, uniq.p, ku.p, lu.p);
-    hipLaunchKernelGGL(k_gather_u64, dim3(blocks_for(nl, 256)), dim3(256), 0, c->stream, nl, uniq.p, kv.p, lv.p);
-    if (ses.d) {
-      ses_cast_lines(c, ses.d, axis, nl, lu.p, lv.p, h_xyz);
-      continue;
-    }
-    // pierced balls per line, their intervals sorted and merged
-    DBuf<int64_t> off(c->stream), mcnt(c->stream);
-    off.resize((size_t)nl);
-    mcnt.resize((size_t)nl);
-    hipLaunchKernelGGL(k_count_pierced, dim3(blocks_for(nl, 128)), dim3(128), 0, c->stream, nl, lu.p, lv.p, G, d1, d2,
-                       off.p);
-    NGPB_HIP(hipGetLastError());
-    const int64_t niv = exclusive_scan(c, off.p, off.p, nl);
-    DBuf<Iv> iv(c->stream);
-    iv.resize((size_t)std::max<int64_t>(niv, 1));
-    hipLaunchKernelGGL(k_merge_pierced_wave, dim3((unsigned)nl), dim3(64), 0, c->stream, nl, lu.p, lv.p, G, axis, d1,
-                       d2, off.p, niv, iv.p, mcnt.p);
-    NGPB_HIP(hipGetLastError());
-    // rays = lines with crossings
-    DBuf<uint8_t> keep(c->stream);
-    keep.resize((size_t)nl);
-    hipLaunchKernelGGL(k_nonzero, dim3(blocks_for(nl, 256)), dim3(256), 0, c->stream, nl, mcnt.p, keep.p);
-    DBuf<int32_t> kept(c->stream);
-    kept.resize((size_t)nl);
-    const int64_t nr = compact_flags(c, keep.p, nl, kept.p);
-    R.n = nr;
-    R.key.resize((size_t)2 * std::max<int64_t>(nr, 1));
-    R.ptr.resize((size_t)nr + 1);
-    if (nr > 0) {
-      hipLaunchKernelGGL(k_ray_keys, dim3(blocks_for(nr, 256)), dim3(256), 0, c->stream, nr, kept.p, lu.p, lv.p,
-                         mcnt.p, R.key.p, R.ptr.p);
-      R.k = exclusive_scan(c, R.ptr.p, R.ptr.p, nr);
-    } else {
-      R.k = 0;
-    }
-    NGPB_HIP(hipMemcpyAsync(R.ptr.p + nr, &R.k, sizeof(int64_t), hipMemcpyHostToDevice, c->stream));
-    R.inters.resize((size_t)std::max<int64_t>(R.k, 1));
-    R.normals.resize((size_t)3 * std::max<int64_t>(R.k, 1));
-    if (nr > 0)
-      hipLaunchKernelGGL(k_ray_fill, dim3(blocks_for(nr, 128)), dim3(128), 0, c->stream, nr, kept.p, R.ptr.p, off.p,
-                         iv.p, R.key.p, G, axis, d1, d2, R.inters.p, R.normals.p);
-    NGPB_HIP(hipGetLastError());
-    NGPB_HIP(hipStreamSynchronize(c->stream));
+    cast_axis_lines(c, na, h_xyz, h_r, ses.d, bxyz.p, br.p, axis, nl, lu.p, lv.p);
   }
+}
+
+void cast_grid_zlines(ngpb_ctx* c, const double* origin, double hf, int64_t n0, int64_t n1, int64_t na,
+                      const double* h_xyz, const double* h_r, double probe) {
+  SesHolder ses;
+  if (probe > 0.0 && na > 0) ses.d = ses_prepare(c, na, h_xyz, h_r, probe);
+  DBuf<double> bxyz(c->stream), br(c->stream);
+  if (na > 0) {
+    bxyz.upload(h_xyz, (size_t)(3 * na), c->stream);
+    br.upload(h_r, (size_t)na, c->stream);
+  }
+  const int64_t nl = (n0 + 1) * (n1 + 1);
+  DBuf<uint64_t> lu(c->stream), lv(c->stream);
+  lu.resize((size_t)nl);
+  lv.resize((size_t)nl);
+  hipLaunchKernelGGL(k_grid_zline_keys, dim3(blocks_for(nl, 256)), dim3(256), 0, c->stream, n0, n1, origin[0],
+                     origin[1], hf, lu.p, lv.p);
+  NGPB_HIP(hipGetLastError());
+  cast_axis_lines(c, na, h_xyz, h_r, ses.d, bxyz.p, br.p, 2, nl, lu.p, lv.p);
 }
 
 }  // namespace ngpb

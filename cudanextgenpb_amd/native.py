@@ -111,6 +111,7 @@ SIGNATURES = {
     "ngpb_cast_rays_ses": (_I, [_P, _I64, _P, _P, _D]),
     "ngpb_locate_atoms": (_I, [_P, _I64, _P, ctypes.c_double, _P]),
     "ngpb_build_octree": (_I, [_P, _P, ctypes.c_double, _P, _I, _P, _I64, _P, _P, _P]),
+    "ngpb_build_octree_surface": (_I, [_P, _P, ctypes.c_double, _P, _I, _I, _I, _I64, _P, _P, ctypes.c_double, _P]),
     "ngpb_get_mesh": (_I, [_P, _P, _P, _P, _P, _P, _P, _P]),
     "ngpb_get_rays": (_I, [_P, _I, _P, _P, _P, _P, _P]),
     "ngpb_pot_field": (_I, [_P, _I, _I, _P, _P]),
@@ -248,6 +249,22 @@ class Context:
         sz = np.zeros(4, np.int64)
         self._check(lib().ngpb_build_octree(self._h, _p(o), float(hf), _p(nn), int(smax), _p(b), int(r.shape[0]),
                                             _p(xyz), _p(r), _p(sz)), "build_octree")
+        return tuple(int(v) for v in sz)
+
+    def build_octree_surface(self, origin, hf: float, n, smax: int, s_uni: int, xyz, r, probe: float = 0.0,
+                             cycles: int = -1):
+        """Builds the octree with the reference's marker-driven rule (refine_surface):
+        uniform leaves of level s_uni, then `cycles` refine/coarsen passes driven by
+        the in/out flags of the leaves' corners (cycles < 0: max(s_uni, smax - s_uni)).
+        Returns (n_nodes, n_hang, n_leaves, n_boundary)."""
+        o = _c(origin, np.float64)
+        nn = _c(n, np.int64)
+        xyz = _c(xyz, np.float64)
+        r = _c(r, np.float64)
+        sz = np.zeros(4, np.int64)
+        self._check(lib().ngpb_build_octree_surface(self._h, _p(o), float(hf), _p(nn), int(smax), int(s_uni),
+                                                    int(cycles), int(r.shape[0]), _p(xyz), _p(r), float(probe),
+                                                    _p(sz)), "build_octree_surface")
         return tuple(int(v) for v in sz)
 
     def get_mesh(self):

@@ -160,6 +160,27 @@ int ngpb_set_mesh(ngpb_ctx* ctx, int64_t n_nodes, const double* node_xyz, int64_
 int ngpb_build_octree(ngpb_ctx* ctx, const double* origin, double hf, const int64_t* n, int smax,
                       const double* band, int64_t n_balls, const double* xyz, const double* r,
                       int64_t* sizes);
+/* As ngpb_build_octree with the reference's marker-driven rule (refine_surface,
+   src/pb_class.cpp:1467-1745, after init_tmesh, :1289-1296): the box is tiled by
+   uniform leaves of level s_uni (maxlevel - unilevel in the reference's terms; smax =
+   maxlevel - minlevel, 0 <= s_uni <= smax), then `cycles` times (cycles < 0: the
+   reference's max(s_uni, smax - s_uni)):
+     refine   every leaf of level s >= 1 whose regular (non-hanging) corners are on
+              both sides of the surface is split once, then 2:1 balance;
+     coarsen  a family of 8 sibling leaves of level s < smax is merged when each of
+              them has all its regular corners on one side, then 2:1 balance.
+   A vertex is inside when an odd number of the crossings of the z-line through it lie
+   at or below it (is_in_ns_surf, src/pb_class.cpp:769-806); the crossings are those
+   of ngpb_cast_rays (probe = 0: union of the balls) or ngpb_cast_rays_ses (probe > 0)
+   on every grid z-line. The atom-box test of the reference's markers is left out: it
+   never fires there (the atom list is still empty). Merging a family only when all 8
+   members are marked is this library's reading of tmesh_3d's coarsening callback,
+   which is not part of the reference. Leaves, nodes, numbering and the boundary list
+   as in ngpb_build_octree; the context's ray table is left empty (cast the mesh's
+   rays next). */
+int ngpb_build_octree_surface(ngpb_ctx* ctx, const double* origin, double hf, const int64_t* n, int smax,
+                              int s_uni, int cycles, int64_t n_balls, const double* xyz, const double* r,
+                              double probe, int64_t* sizes);
 /* Reads the context's mesh back in ngpb_set_mesh's layout: sizes receives {n_nodes,
    n_hang, n_leaves, n_boundary}; the arrays (each may be NULL) receive
    node_xyz[3 N], hang_xyz[3 H], hang_np[H], hang_par[4 H], leaf_corner[8 L] and the

@@ -19,6 +19,7 @@
 //   orc_energy         src/pb_class.cpp:2534-2788 (helpers), 2790-3063 (energy)
 //   orc_phi_on_atoms   src/pb_class.cpp:2333-2368 (write_potential_on_atoms_fast)
 //   orc_pot_field      src/pb_class.cpp:3731-5060 (pot_field_fast / pot_field)
+//   orc_refine_surface src/pb_class.cpp:1289-1296 (init_tmesh), 1467-1745 (refine_surface)
 //
 // Pinning (how each is checked without the reference's libraries, which are absent):
 //   constants  closed forms; markers: brute-force distance test on sphere unions +
@@ -1523,6 +1524,164 @@ int64_t orc_ses_rays(int64_t na, const double* xyz, const double* r, double rp, 
     }
   }
   return (int64_t)M.pairs.size();
+}
+
+}  // extern "C"
+
+// ---------------------------------------------------------------------------
+// refine_surface (src/pb_class.cpp:1467-1745) after init_tmesh (:1289-1296), on a
+// dense map of the finest cells holding the level of their leaf (a leaf of level s is
+// an aligned cube of 2^s cells; its anchor is the cell whose coordinates are multiples
+// of 2^s). Flags are is_in_ns_surf along z (:769-806): the number of crossings at or
+// below the vertex on its z-line, modulo 2. Steps of a cycle, each decided on the
+// state before it:
+//   refine   leaves of level s >= 1 with regular corners on both sides split once;
+//   balance  leaves of level s >= 2 with a cell of level < s - 1 in the layer around
+//            them split, repeated until none does;
+//   coarsen  complete families of 8 sibling leaves of level s < smax whose members all
+//            have their regular corners on one side merge;
+//   balance.
+// A corner is regular when it is a corner of every leaf around it. The all-members
+// rule for families is this restatement's reading of the tmesh_3d coarsening callback
+// (bimpp, absent from the reference): parity unpinned against bimpp; pinned by the
+// closed forms and the independent Python restatement in the tests.
+namespace {
+
+struct SurfGrid {
+  int64_t n[3];
+  double origin[3], hf;
+  std::vector<int8_t> lev;
+  std::map<std::pair<int64_t, int64_t>, std::vector<double>> zlines;  // (i, j) -> crossings
+
+  int8_t& at(int64_t x, int64_t y, int64_t z) { return lev[(size_t)((z * n[1] + y) * n[0] + x)]; }
+
+  int flag(int64_t x, int64_t y, int64_t z) const {
+    const auto it = zlines.find({x, y});
+    if (it == zlines.end()) return 0;
+    const double zc = origin[2] + (double)z * hf;
+    int cnt = 0;
+    for (double t : it->second)
+      if (zc >= t) ++cnt;
+    return cnt % 2;
+  }
+
+  bool regular(int64_t px, int64_t py, int64_t pz) {
+    for (int64_t z = pz - 1; z <= pz; ++z)
+      for (int64_t y = py - 1; y <= py; ++y)
+        for (int64_t x = px - 1; x <= px; ++x) {
+          if (x < 0 || y < 0 || z < 0 || x >= n[0] || y >= n[1] || z >= n[2]) continue;
+          const int64_t S = (int64_t)1 << at(x, y, z);
+          if (px % S != 0 || py % S != 0 || pz % S != 0) return false;
+        }
+    return true;
+  }
+
+  // true when the regular corners of cube (x, y, z, edge S) are all on one side
+  bool uniform(int64_t x, int64_t y, int64_t z, int64_t S) {
+    int seen0 = 0, seen1 = 0;
+    for (int k = 0; k < 8; ++k) {
+      const int64_t px = x + ((k & 1) ? S : 0), py = y + ((k & 2) ? S : 0), pz = z + ((k & 4) ? S : 0);
+      if (!regular(px, py, pz)) continue;
+      if (flag(px, py, pz)) seen1 = 1;
+      else seen0 = 1;
+    }
+    return !(seen0 && seen1);
+  }
+
+  void set_cube(int64_t x, int64_t y, int64_t z, int64_t S, int8_t v) {
+    for (int64_t c = z; c < z + S; ++c)
+      for (int64_t b = y; b < y + S; ++b)
+        for (int64_t a = x; a < x + S; ++a) at(a, b, c) = v;
+  }
+
+  template <class F>
+  void each_leaf(F&& f) {  // anchor and level of every leaf
+    for (int64_t z = 0; z < n[2]; ++z)
+      for (int64_t y = 0; y < n[1]; ++y)
+        for (int64_t x = 0; x < n[0]; ++x) {
+          const int s = at(x, y, z);
+          const int64_t S = (int64_t)1 << s;
+          if (x % S == 0 && y % S == 0 && z % S == 0) f(x, y, z, s);
+        }
+  }
+
+  void balance() {
+    for (;;) {
+      std::vector<std::array<int64_t, 4>> split;
+      each_leaf([&](int64_t x, int64_t y, int64_t z, int s) {
+        if (s < 2) return;
+        const int64_t S = (int64_t)1 << s;
+        bool bad = false;
+        for (int64_t c = z - 1; c <= z + S && !bad; ++c)
+          for (int64_t b = y - 1; b <= y + S && !bad; ++b)
+            for (int64_t a = x - 1; a <= x + S && !bad; ++a) {
+              if (a < 0 || b < 0 || c < 0 || a >= n[0] || b >= n[1] || c >= n[2]) continue;
+              if (a >= x && a < x + S && b >= y && b < y + S && c >= z && c < z + S) continue;
+              if (at(a, b, c) < s - 1) bad = true;
+            }
+        if (bad) split.push_back({x, y, z, s});
+      });
+      if (split.empty()) return;
+      for (const auto& l : split) set_cube(l[0], l[1], l[2], (int64_t)1 << l[3], (int8_t)(l[3] - 1));
+    }
+  }
+};
+
+}  // namespace
+
+extern "C" {
+
+// Leaves of the refine_surface mesh of the box origin + [0, n] hf: z-line rays (nr
+// lines, key[2r] = x, key[2r+1] = y of line r, crossings inters[ptr[r] .. ptr[r+1]));
+// out[4 l ..] = anchor x, y, z (finest cells) and level of leaf l, in anchor order
+// (x fastest); returns the leaf count (out may be NULL to count), -1 for bad input.
+int64_t orc_refine_surface(const int64_t* n, const double* origin, double hf, int smax, int s_uni, int cycles,
+                           int64_t nr, const double* key, const int64_t* ptr, const double* inters, int32_t* out) {
+  if (smax < 0 || s_uni < 0 || s_uni > smax || hf <= 0) return -1;
+  SurfGrid G;
+  for (int d = 0; d < 3; ++d) {
+    G.n[d] = n[d];
+    G.origin[d] = origin[d];
+    if (n[d] <= 0 || n[d] % ((int64_t)1 << smax) != 0) return -1;
+  }
+  G.hf = hf;
+  for (int64_t r = 0; r < nr; ++r) {
+    const int64_t i = std::llround((key[2 * r] - origin[0]) / hf), j = std::llround((key[2 * r + 1] - origin[1]) / hf);
+    G.zlines[{i, j}] = std::vector<double>(inters + ptr[r], inters + ptr[r + 1]);
+  }
+  G.lev.assign((size_t)(n[0] * n[1] * n[2]), (int8_t)s_uni);
+  for (int cyc = 0; cyc < cycles; ++cyc) {
+    std::vector<std::array<int64_t, 4>> split;
+    G.each_leaf([&](int64_t x, int64_t y, int64_t z, int s) {
+      if (s > 0 && !G.uniform(x, y, z, (int64_t)1 << s)) split.push_back({x, y, z, s});
+    });
+    for (const auto& l : split) G.set_cube(l[0], l[1], l[2], (int64_t)1 << l[3], (int8_t)(l[3] - 1));
+    G.balance();
+    std::vector<std::array<int64_t, 4>> merge;
+    G.each_leaf([&](int64_t x, int64_t y, int64_t z, int s) {
+      if (s >= smax) return;
+      const int64_t S = (int64_t)1 << s, P = 2 * S;
+      if (x % P || y % P || z % P) return;
+      for (int k = 0; k < 8; ++k) {
+        const int64_t a = x + ((k & 1) ? S : 0), b = y + ((k & 2) ? S : 0), c = z + ((k & 4) ? S : 0);
+        if (G.at(a, b, c) != s || !G.uniform(a, b, c, S)) return;
+      }
+      merge.push_back({x, y, z, s});
+    });
+    for (const auto& l : merge) G.set_cube(l[0], l[1], l[2], (int64_t)2 << l[3], (int8_t)(l[3] + 1));
+    G.balance();
+  }
+  int64_t L = 0;
+  G.each_leaf([&](int64_t x, int64_t y, int64_t z, int s) {
+    if (out) {
+      out[4 * L] = (int32_t)x;
+      out[4 * L + 1] = (int32_t)y;
+      out[4 * L + 2] = (int32_t)z;
+      out[4 * L + 3] = s;
+    }
+    ++L;
+  });
+  return L;
 }
 
 }  // extern "C"

@@ -64,6 +64,8 @@ def lib():
         L.orc_ses_classify.argtypes = [_I64, _P, _P, _D, _I64, _P, _P]
         L.orc_ses_rays.argtypes = [_I64, _P, _P, _D, ctypes.c_int, _I64, _P, ctypes.c_int, _P, _P, _P, _P]
         L.orc_ses_rays.restype = _I64
+        L.orc_refine_surface.argtypes = [_P, _P, _D, ctypes.c_int, ctypes.c_int, ctypes.c_int, _I64, _P, _P, _P, _P]
+        L.orc_refine_surface.restype = _I64
         L.orc_coul_energy.argtypes = [_I64, _P, _P, _D]
         L.orc_coul_energy.restype = _D
         L.orc_analytic_solution.argtypes = [_D] * 7
@@ -364,3 +366,26 @@ def ses_rays(xyz, r, rp, axis, uv, cap=64):
     out = [(xs[l * cap:l * cap + cnt[l]].copy(), ns[3 * l * cap:3 * (l * cap + cnt[l])].reshape(-1, 3).copy())
            for l in range(L)]
     return out, int(npairs), int(nv[0])
+
+
+def refine_surface(origin, hf, n, smax, s_uni, cycles, zkey, zptr, zinters):
+    """Leaves (L x 4: anchor x, y, z in finest cells, level) of the refine_surface mesh
+    of the box origin + [0, n] hf, from the rays of its grid z-lines (zkey: (x, y) per
+    line with crossings, zptr, zinters), in anchor order (x fastest).
+    cycles < 0: max(s_uni, smax - s_uni), the reference's count."""
+    if cycles < 0:
+        cycles = max(s_uni, smax - s_uni)
+    nn = _c(n, np.int64)
+    o = _c(origin, np.float64)
+    key = _c(np.asarray(zkey, np.float64).reshape(-1), np.float64)
+    ptr = _c(zptr, np.int64)
+    it = _c(zinters if len(zinters) else np.zeros(1), np.float64)
+    nr = len(ptr) - 1
+    L = lib().orc_refine_surface(_p(nn), _p(o), float(hf), int(smax), int(s_uni), int(cycles), nr, _p(key),
+                                 _p(ptr), _p(it), None)
+    if L < 0:
+        raise ValueError("refine_surface: bad input")
+    out = np.zeros((L, 4), np.int32)
+    lib().orc_refine_surface(_p(nn), _p(o), float(hf), int(smax), int(s_uni), int(cycles), nr, _p(key), _p(ptr),
+                             _p(it), _p(out))
+    return out
