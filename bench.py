@@ -196,6 +196,18 @@ def bench_batch(args):
             "solve_s": fp64_s,
             "iterations_total": it64,
         },
+        "roofline": {
+            "kernel": "batched compact FP32 SpMV (kb_spmv: tiles of the batch's systems, autotuned tile and waves)",
+            "bound": "hbm",
+            "achieved": spmv_bytes / (tune["ms_per_spmv"] * 1e-3) / 1e9,
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": spmv_bytes / (tune["ms_per_spmv"] * 1e-3) / 1e9 / HBM_PEAK_GBS,
+            "traffic": None,
+            "bytes_per_launch": spmv_bytes,
+            "avg_launch_ms": tune["ms_per_spmv"],
+            "timing": "autotuner: HIP events around 5 launches of the chosen tiling over every system",
+        },
         "batch_over_fp64_one_at_a_time": fp64_s / batch_s,
         "setup_s": setup_s,
     }

@@ -371,8 +371,10 @@ struct ngpb_batch {
   int64_t n_tiles = 0;
   bool tiles_dirty = true;
   int tiled_systems = 0;  // systems in the current tile list
-  hipGraph_t graph = nullptr;
+  hipGraph_t graph = nullptr;        // replacement step + inner iterations
   hipGraphExec_t exec = nullptr;
+  hipGraph_t graph_inner = nullptr;  // inner iterations only (no slot joining or replacing)
+  hipGraphExec_t exec_inner = nullptr;
   int64_t replays = 0;
   double ms_total = 0.0;
 };
@@ -403,8 +405,12 @@ int bguard(ngpb_batch* b, F&& f) {
 void drop_graph(ngpb_batch* b) {
   if (b->exec) (void)hipGraphExecDestroy(b->exec);
   if (b->graph) (void)hipGraphDestroy(b->graph);
+  if (b->exec_inner) (void)hipGraphExecDestroy(b->exec_inner);
+  if (b->graph_inner) (void)hipGraphDestroy(b->graph_inner);
   b->exec = nullptr;
   b->graph = nullptr;
+  b->exec_inner = nullptr;
+  b->graph_inner = nullptr;
 }
 
 // Tiles of the systems not yet done (with all_systems, of every system: the autotuner
@@ -428,19 +434,25 @@ void build_tiles(ngpb_batch* b, bool all_systems = false) {
   drop_graph(b);
 }
 
+// One replay: the FP64 replacement step of the systems joining or ending an inner
+// cycle (repl; every other system skips those kernels), then inner_per_replay FP32
+// iterations. A replay without repl is the same work when no system is joining or
+// replacing: the replacement kernels would all return at once.
 template <int WPB>
-void enqueue_replay(ngpb_batch* b) {
+void enqueue_replay(ngpb_batch* b, bool repl) {
   const dim3 g((unsigned)b->n_tiles), t(WPB * 64), gf((unsigned)b->cap), tf(kFinBS);
   const BatchDesc* D = b->desc.p;
   BatchScal* S = b->scal.p;
   const Tile* T = b->tiles.p;
   const int ts = b->tile_slices;
   hipStream_t st = b->stream;
-  hipLaunchKernelGGL(kb_accum<WPB>, g, t, 0, st, D, S, T, ts);
-  hipLaunchKernelGGL(kb_spmv64<WPB>, g, t, 0, st, D, S, T, ts);
-  hipLaunchKernelGGL(kb_resid<WPB>, g, t, 0, st, D, S, T, ts);
-  hipLaunchKernelGGL(kb_fin_resid, gf, tf, 0, st, D, S);
-  hipLaunchKernelGGL(kb_dir<WPB>, g, t, 0, st, D, S, T, ts);
+  if (repl) {
+    hipLaunchKernelGGL(kb_accum<WPB>, g, t, 0, st, D, S, T, ts);
+    hipLaunchKernelGGL(kb_spmv64<WPB>, g, t, 0, st, D, S, T, ts);
+    hipLaunchKernelGGL(kb_resid<WPB>, g, t, 0, st, D, S, T, ts);
+    hipLaunchKernelGGL(kb_fin_resid, gf, tf, 0, st, D, S);
+    hipLaunchKernelGGL(kb_dir<WPB>, g, t, 0, st, D, S, T, ts);
+  }
   for (int k = 0; k < b->inner_per_replay; ++k) {
     hipLaunchKernelGGL(kb_spmv<WPB>, g, t, 0, st, D, S, T, ts, 0);
     hipLaunchKernelGGL(kb_fin_alpha, gf, tf, 0, st, D, S);
@@ -450,30 +462,34 @@ void enqueue_replay(ngpb_batch* b) {
   }
 }
 
-void enqueue_replay_any(ngpb_batch* b) {
+void enqueue_replay_any(ngpb_batch* b, bool repl) {
   switch (b->wpb) {
-    case 1: enqueue_replay<1>(b); break;
-    case 2: enqueue_replay<2>(b); break;
-    case 4: enqueue_replay<4>(b); break;
-    case 8: enqueue_replay<8>(b); break;
+    case 1: enqueue_replay<1>(b, repl); break;
+    case 2: enqueue_replay<2>(b, repl); break;
+    case 4: enqueue_replay<4>(b, repl); break;
+    case 8: enqueue_replay<8>(b, repl); break;
     default: throw Error(NGPB_E_ARG, "waves per block must be 1, 2, 4 or 8");
   }
 }
 
-void ensure_graph(ngpb_batch* b) {
+// the replay graph with (repl) or without the replacement step, captured on first use
+hipGraphExec_t ensure_graph(ngpb_batch* b, bool repl) {
   if (b->tiles_dirty) build_tiles(b);
-  if (b->exec || b->n_tiles == 0) return;
+  hipGraph_t& graph = repl ? b->graph : b->graph_inner;
+  hipGraphExec_t& exec = repl ? b->exec : b->exec_inner;
+  if (exec || b->n_tiles == 0) return exec;
   NGPB_HIP(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
   try {
-    enqueue_replay_any(b);
+    enqueue_replay_any(b, repl);
   } catch (...) {
     hipGraph_t g = nullptr;
     (void)hipStreamEndCapture(b->stream, &g);
     if (g) (void)hipGraphDestroy(g);
     throw;
   }
-  NGPB_HIP(hipStreamEndCapture(b->stream, &b->graph));
-  NGPB_HIP(hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0));
+  NGPB_HIP(hipStreamEndCapture(b->stream, &graph));
+  NGPB_HIP(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0));
+  return exec;
 }
 
 template <int WPB>
@@ -749,9 +765,15 @@ int ngpb_batch_run(ngpb_batch* b, int max_replays, int* running) {
     NGPB_HIP(hipEventCreate(&e0));
     NGPB_HIP(hipEventCreate(&e1));
     for (int r = 0; r < max_replays && left > 0; ++r) {
-      ngpb::ensure_graph(b);
+      // the host mirror holds every slot's phase after the last replay (or join)
+      bool repl = false;
+      for (int s = 0; s < b->cap; ++s)
+        if (b->sys[s] && (b->scal_host[s].phase == ngpb::kJoin || b->scal_host[s].phase == ngpb::kReplace))
+          repl = true;
+      hipGraphExec_t ex = ngpb::ensure_graph(b, repl);
+      if (!ex) break;
       NGPB_HIP(hipEventRecord(e0, b->stream));
-      NGPB_HIP(hipGraphLaunch(b->exec, b->stream));
+      NGPB_HIP(hipGraphLaunch(ex, b->stream));
       NGPB_HIP(hipEventRecord(e1, b->stream));
       NGPB_HIP(hipMemcpyAsync(b->scal_host, b->scal.p, sizeof(ngpb::BatchScal) * (size_t)b->cap,
                               hipMemcpyDeviceToHost, b->stream));

@@ -645,100 +645,6 @@ __global__ void k_pk_hdr_list(int64_t cnt, const int32_t* __restrict__ list, con
 // Results equal those of k_sell_both except possibly for the sign of an exact zero.
 constexpr int kGuard = 64;
 
-template <int ND, bool CONST>
-__device__ __forceinline__ double lean_rows(const Sell& A, int64_t s, int64_t bp, const float* __restrict__ x,
-                                            double acc) {
-  const int lane = lane_id();
-  const float* vb = A.val + bp * kW;
-  float v[ND], xv[ND];
-  int d[ND];
-#pragma unroll
-  for (int j = 0; j < ND; ++j) {
-    v[j] = CONST ? A.cval[bp + j] : vb[j * kW + lane];
-    d[j] = A.base[bp + j];
-  }
-#pragma unroll
-  for (int j = 0; j < ND; ++j) {
-    const float* xb = x + (s * kW + d[j]);
-    xv[j] = xb[lane];
-  }
-#pragma unroll
-  for (int j = 0; j < ND; ++j) acc = fma((double)v[j], (double)xv[j], acc);
-  return acc;
-}
-
-__device__ __forceinline__ void lean_delta(const Sell& A, int64_t s, int w, int64_t bp, const float* __restrict__ x,
-                                           float* __restrict__ y, double* part) {
-  const int lane = lane_id();
-  const float* xr = x + s * kW;
-  const float xi = xr[lane];
-  double acc = 0.0;
-  switch (w) {
-#define NGPB_LR(W, ND, C) \
-  case W: acc = lean_rows<ND, C>(A, s, bp, x, acc); break;
-    NGPB_LR(-1, 1, false) NGPB_LR(-2, 2, false) NGPB_LR(-3, 3, false) NGPB_LR(-4, 4, false)
-    NGPB_LR(-5, 5, false) NGPB_LR(-6, 6, false) NGPB_LR(-7, 7, false) NGPB_LR(-8, 8, false)
-    NGPB_LR(-9, 9, false) NGPB_LR(-10, 10, false) NGPB_LR(-11, 11, false) NGPB_LR(-12, 12, false)
-    NGPB_LR(-13, 13, false) NGPB_LR(-14, 14, false) NGPB_LR(-15, 15, false) NGPB_LR(-16, 16, false)
-    NGPB_LR(-33, 1, true) NGPB_LR(-34, 2, true) NGPB_LR(-35, 3, true) NGPB_LR(-36, 4, true)
-    NGPB_LR(-37, 5, true) NGPB_LR(-38, 6, true) NGPB_LR(-39, 7, true) NGPB_LR(-40, 8, true)
-    NGPB_LR(-41, 9, true) NGPB_LR(-42, 10, true) NGPB_LR(-43, 11, true) NGPB_LR(-44, 12, true)
-    NGPB_LR(-45, 13, true) NGPB_LR(-46, 14, true) NGPB_LR(-47, 15, true) NGPB_LR(-48, 16, true)
-#undef NGPB_LR
-    default: break;
-  }
-  acc += (double)xi;
-  if (s * kW + lane < A.n) {
-    const float yi = (float)acc;
-    y[s * kW + lane] = yi;
-    *part += (double)xi * (double)yi;
-  }
-}
-
-template <int ND, bool CONST>
-__device__ __forceinline__ void lean_pair(const Sell& A, int64_t s0, int64_t b0, int64_t b1,
-                                          const float* __restrict__ x, float* __restrict__ y, double* part) {
-  const int lane = lane_id();
-  const int64_t s1 = s0 + 1;
-  const float* vb0 = A.val + b0 * kW;
-  const float* vb1 = A.val + b1 * kW;
-  float v0[ND], v1[ND], x0[ND], x1[ND];
-  int d0[ND], d1[ND];
-#pragma unroll
-  for (int j = 0; j < ND; ++j) {
-    v0[j] = CONST ? A.cval[b0 + j] : vb0[j * kW + lane];
-    v1[j] = CONST ? A.cval[b1 + j] : vb1[j * kW + lane];
-    d0[j] = A.base[b0 + j];
-    d1[j] = A.base[b1 + j];
-  }
-  const float xi0 = x[s0 * kW + lane], xi1 = x[s1 * kW + lane];
-#pragma unroll
-  for (int j = 0; j < ND; ++j) {
-    const float* xb0 = x + (s0 * kW + d0[j]);
-    const float* xb1 = x + (s1 * kW + d1[j]);
-    x0[j] = xb0[lane];
-    x1[j] = xb1[lane];
-  }
-  double a0 = 0.0, a1 = 0.0;
-#pragma unroll
-  for (int j = 0; j < ND; ++j) {
-    a0 = fma((double)v0[j], (double)x0[j], a0);
-    a1 = fma((double)v1[j], (double)x1[j], a1);
-  }
-  a0 += (double)xi0;
-  a1 += (double)xi1;
-  if (s0 * kW + lane < A.n) {
-    const float yi = (float)a0;
-    y[s0 * kW + lane] = yi;
-    *part += (double)xi0 * (double)yi;
-  }
-  if (s1 * kW + lane < A.n) {
-    const float yi = (float)a1;
-    y[s1 * kW + lane] = yi;
-    *part += (double)xi1 * (double)yi;
-  }
-}
-
 __device__ __forceinline__ void lean_dpair(const Sell& A, const DPair& m, const float* __restrict__ x,
                                            float* __restrict__ y, double* part) {
   if (m.paired) {
@@ -758,84 +664,6 @@ __device__ __forceinline__ void lean_dpair(const Sell& A, const DPair& m, const 
   }
   lean_delta(A, m.s0, m.w0, m.b0, x, y, part);
   if (m.s1 >= 0) lean_delta(A, m.s1, m.w1, m.b1, x, y, part);
-}
-
-// hdr_rows without clamps; an entry of a row that does not take part in slot j reads
-// the +0 entry
-template <int K, int NV>
-__device__ __forceinline__ double lean_hdr_rows(const Sell& A, const PkHdr& h, int row, const int* dv,
-                                                const float* __restrict__ x, double acc) {
-  const int lane = lane_id();
-  float v[K], xv[K];
-  uint32_t wd[K / 4];
-  uint32_t ss = (uint32_t)h.E, gs = (uint32_t)h.G;
-#pragma unroll
-  for (int g = 0; g < K / 4; ++g) {
-    const int mg = 4 * g < (int)h.maxlen ? (int)h.m[4 * g] : 0;
-    wd[g] = A.pk_idx[lane < mg ? gs + (uint32_t)lane : (uint32_t)h.G];
-    gs += (uint32_t)mg;
-#pragma unroll
-    for (int q = 0; q < 4; ++q) {
-      const int j = 4 * g + q;
-      const int mk = j < (int)h.maxlen ? (int)h.m[j] : 0;
-      v[j] = A.pk_val[lane < mk ? ss + (uint32_t)lane : A.pk_zero];
-      ss += (uint32_t)mk;
-    }
-  }
-#pragma unroll
-  for (int j = 0; j < K; ++j) {
-    const int d = dict_lookup<NV>(dv, (int)((wd[j / 4] >> (8 * (j % 4))) & 255u));
-    xv[j] = x[row + d];
-  }
-#pragma unroll
-  for (int j = 0; j < K; ++j) acc = fma((double)v[j], (double)xv[j], acc);
-  return acc;
-}
-
-template <int NV>
-__device__ __forceinline__ void lean_packed(const Sell& A, int64_t s, const PkHdr& h, const float* __restrict__ x,
-                                            float* __restrict__ y, double* part) {
-  const int lane = lane_id();
-  const int lp = A.pk_lp[s * kW + lane];
-  int dv[NV];
-#pragma unroll
-  for (int q = 0; q < NV; ++q) dv[q] = q * kW + lane < (int)h.nd ? A.pk_dict[h.D + q * kW + lane] : 0;
-  const int row = (int)(s * kW) + (lp & 255);
-  const float xi = x[row];
-  const int maxlen = h.maxlen;
-  double acc = 0.0;
-  if (maxlen > 0) {
-    if (maxlen <= 8) acc = lean_hdr_rows<8, NV>(A, h, row, dv, x, acc);
-    else acc = lean_hdr_rows<16, NV>(A, h, row, dv, x, acc);
-  }
-  if (maxlen > 16) {
-    const int len = lp >> 8;
-    int64_t ss = 0, gs = 0;
-#pragma unroll
-    for (int j = 0; j < 16; ++j) ss += h.m[j];
-#pragma unroll
-    for (int g = 0; g < 4; ++g) gs += h.m[4 * g];
-    for (int k0 = 16; k0 < maxlen; k0 += 4) {
-      const int mg = __popcll(__ballot(len > k0));
-      const uint32_t wd = A.pk_idx[lane < mg ? h.G + gs + lane : (int64_t)h.G];
-      gs += mg;
-#pragma unroll
-      for (int q = 0; q < 4; ++q) {
-        const bool act = len > k0 + q;
-        const int mk = __popcll(__ballot(act));
-        const float v = A.pk_val[act ? h.E + ss + lane : (int64_t)A.pk_zero];
-        ss += mk;
-        const int d = dict_lookup<NV>(dv, (int)((wd >> (8 * q)) & 255u));
-        acc = fma((double)v, (double)x[row + d], acc);
-      }
-    }
-  }
-  acc += (double)xi;
-  if (row < A.n) {
-    const float yi = (float)acc;
-    y[row] = yi;
-    *part += (double)xi * (double)yi;
-  }
 }
 
 template <bool DOT>
