@@ -5,8 +5,10 @@
 // values are read once, coalesced, and each (value, x[column]) pair is staged in
 // LDS; each thread then forms its row sum in column order as a chain of fused
 // multiply-adds starting from zero (the same arithmetic as the oracle's SpMV, so the
-// results are bit-identical). Blocks whose rows hold more than kCap entries read
-// global memory directly.
+// results are bit-identical). Blocks whose rows hold more than kCap entries stage
+// them in pieces of kCap, each row carrying its chain from piece to piece (reading
+// such blocks' entries row by row from global memory instead made the 8.04M-row
+// SpMV 10% slower: 217 against 195 us).
 //
 // CG iteration, three kernels: SpMV + per-block (p, q) -> x/r update + per-block
 // (r, D^-1 r), (r, r) -> stop test and new direction. Each kernel first combines the
@@ -59,38 +61,36 @@ __global__ void __launch_bounds__(kBS) k_spmv64(int64_t n, const int32_t* __rest
     const int64_t r1 = min(r0 + kBS, n);
     const int64_t row = r0 + threadIdx.x;
     const int64_t e0 = rowptr[r0], e1 = rowptr[r1];
+    const int64_t rb = row < n ? (int64_t)rowptr[row] : e1, re = row < n ? (int64_t)rowptr[row + 1] : e1;
     double sum = 0.0;
-    if (e1 > e0 && e1 - e0 <= kCap) {
-      __syncthreads();  // the previous chunk's rows are done with the staging buffers
-      // all loads of the chunk are issued before any is used: indices past the
-      // chunk's last entry are clamped to it (every row holds its diagonal, so the
-      // chunk is not empty) and their LDS slots are never read
+    // the chunk's entries staged kCap at a time; each row continues its fma chain
+    // across pieces, so the order of its terms is the column order as before
+    for (int64_t p0 = e0; p0 < e1; p0 += kCap) {
+      const int64_t p1 = min(p0 + (int64_t)kCap, e1);
+      __syncthreads();  // the previous piece's rows are done with the staging buffers
+      // all loads of the piece are issued before any is used: indices past the piece's
+      // last entry are clamped to it and their LDS slots are never read
       constexpr int kU = kCap / kBS;
       double v[kU];
       int c[kU];
-      const int64_t kl = e1 - 1;
+      const int64_t kl = p1 - 1;
 #pragma unroll
       for (int u = 0; u < kU; ++u) {
-        const int64_t k = min(e0 + threadIdx.x + (int64_t)u * kBS, kl);
+        const int64_t k = min(p0 + threadIdx.x + (int64_t)u * kBS, kl);
         v[u] = val[k];
         c[u] = col[k];
       }
 #pragma unroll
       for (int u = 0; u < kU; ++u) {
-        const int64_t k = e0 + threadIdx.x + (int64_t)u * kBS;
-        s_val[k - e0] = v[u];
-        s_x[k - e0] = x[c[u]];
+        const int64_t k = p0 + threadIdx.x + (int64_t)u * kBS;
+        s_val[k - p0] = v[u];
+        s_x[k - p0] = x[c[u]];
       }
       __syncthreads();
-      if (row < n) {
-        const int64_t b = rowptr[row] - e0, e = rowptr[row + 1] - e0;
-        for (int64_t k = b; k < e; ++k) sum = fma(s_val[k], s_x[k], sum);
-        y[row] = sum;
-      }
-    } else if (row < n) {
-      for (int64_t k = rowptr[row]; k < rowptr[row + 1]; ++k) sum = fma(val[k], x[col[k]], sum);
-      y[row] = sum;
+      const int64_t kb = max(rb, p0), ke = min(re, p1);
+      for (int64_t k = kb; k < ke; ++k) sum = fma(s_val[k - p0], s_x[k - p0], sum);
     }
+    if (row < n) y[row] = sum;
     if (DOT && row < n) part += x[row] * sum;
   }
   if (DOT) {
