@@ -107,21 +107,22 @@ def cpu_baseline(pb, fp64_iters, seconds):
     }
 
 
-def bench_batch(args):
-    """Batched ligand systems: one batch (mixed precision, hipGraph replays, autotuned
-    tiles) against one-at-a-time FP64 solves of the same systems."""
+def batch_leg(n_systems, tol):
+    """The batched ligand systems (config 4's generator): one batch (mixed precision,
+    hipGraph replays, autotuned tiles) against one-at-a-time FP64 solves of the same
+    systems; the batched SpMV's roofline from the autotuner's timing."""
     from cudanextgenpb_amd import native, workload as W
     from cudanextgenpb_amd.batch import BatchedSolver, prepare
 
     t = time.perf_counter()
     preps, tot_n = [], 0
-    for i in range(args.systems):
+    for i in range(n_systems):
         p = prepare(W.ligand(i))
         p.ctx._keep_mesh = None
         preps.append(p)
         tot_n += p.n
         if (i + 1) % 64 == 0:
-            log(f"prepared {i + 1}/{args.systems} systems ({tot_n / 1e6:.1f}M unknowns, "
+            log(f"prepared {i + 1}/{n_systems} systems ({tot_n / 1e6:.1f}M unknowns, "
                 f"{time.perf_counter() - t:.0f} s)")
     setup_s = time.perf_counter() - t
 
@@ -129,14 +130,14 @@ def bench_batch(args):
     t = time.perf_counter()
     e64, it64 = [], 0
     for p in preps:
-        _, st = p.ctx.solve(native.PREC_FP64, tol=args.tol, want_phi=False)
+        _, st = p.ctx.solve(native.PREC_FP64, tol=tol, want_phi=False)
         e64.append(p.ctx.energy()["polar"])
         it64 += st["iterations"]
     fp64_s = time.perf_counter() - t
-    log(f"one-at-a-time FP64: {args.systems} solves in {fp64_s:.2f} s")
+    log(f"one-at-a-time FP64: {n_systems} solves in {fp64_s:.2f} s")
 
     spmv_bytes = sum(p.ctx.operator_info()["compact_spmv_bytes"] for p in preps)
-    bs = BatchedSolver(0, max_systems=max(args.systems, 1), tol=args.tol)
+    bs = BatchedSolver(0, max_systems=max(n_systems, 1), tol=tol)
     for p in preps:
         bs.add(p)
     tune = bs.autotune()
@@ -145,7 +146,7 @@ def bench_batch(args):
     res = bs.solve(energies=True)
     batch_s = time.perf_counter() - t
     timing = bs.batch.timing()
-    log(f"batched mixed: {args.systems} solves in {batch_s:.2f} s ({timing['replays']} replays)")
+    log(f"batched mixed: {n_systems} solves in {batch_s:.2f} s ({timing['replays']} replays)")
     errs, iters, dof_it = [], 0, 0.0
     for p, e in zip(preps, e64):
         r = res[p.name]
@@ -153,18 +154,61 @@ def bench_batch(args):
         iters += r["stats"]["iterations"]
         dof_it += p.n * r["stats"]["iterations"]
         assert r["stats"]["status"] == 0, (p.name, r["stats"])
+    last = bs.last_timing
     bs.close()
     for p in preps:
         p.ctx.close()
-    value = args.systems / batch_s
+    gbs = spmv_bytes / (tune["ms_per_spmv"] * 1e-3) / 1e9
     return {
+        "systems": n_systems,
+        "unknowns_total": tot_n,
+        "solves_per_s": n_systems / batch_s,
+        "batch": {
+            "solve_s": batch_s,
+            "replays": timing["replays"],
+            "device_ms": timing["ms_total"],
+            "solve_all_s": last["solve_all_s"],
+            "energies_s": last["energies_s"],
+            "inner_iterations_total": iters,
+            "dof_iter_per_s": dof_it / batch_s,
+            "autotune": tune,
+            "spmv_bytes_all_systems": spmv_bytes,
+            "max_rel_energy_err_vs_fp64": max(errs),
+        },
+        "roofline": {
+            "kernel": "batched compact FP32 SpMV (kb_spmv: tiles of the batch's systems, autotuned tile and waves)",
+            "bound": "hbm",
+            "achieved": gbs,
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": gbs / HBM_PEAK_GBS,
+            "traffic": None,
+            "bytes_per_launch": spmv_bytes,
+            "avg_launch_ms": tune["ms_per_spmv"],
+            "timing": "autotuner: HIP events around 5 launches of the chosen tiling over every system",
+        },
+        "fp64_one_at_a_time": {
+            "solves_per_s": n_systems / fp64_s,
+            "solve_s": fp64_s,
+            "iterations_total": it64,
+        },
+        "batch_over_fp64_one_at_a_time": fp64_s / batch_s,
+        "setup_s": setup_s,
+    }
+
+
+def bench_batch(args):
+    """Batched ligand systems: one batch (mixed precision, hipGraph replays, autotuned
+    tiles) against one-at-a-time FP64 solves of the same systems."""
+    L = batch_leg(args.systems, args.tol)
+    out = {
         "metric": "solves/s & DOF-iter/s at 1e-6 rel energy err, mixed vs FP64 GPU; p99 latency",
-        "value": value,
+        "value": L["solves_per_s"],
         "unit": "solves/s",
         "n_gpus": 1,
         "steps": 1,
         "warmup": 0,
-        "ms_per_step": batch_s * 1e3,
+        "ms_per_step": L["batch"]["solve_s"] * 1e3,
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
@@ -175,42 +219,13 @@ def bench_batch(args):
                          f"U[1.2,1.9], q N(0,0.3); uniform 0.4 A leaves, 48^3-96^3 per system; Coulomb Dirichlet; "
                          f"tol {args.tol:g}; a step = solve + energy of every system"),
             "systems": args.systems,
-            "unknowns_total": tot_n,
+            "unknowns_total": L["unknowns_total"],
             "parallelism": "single GPU",
         },
-        "batch": {
-            "solve_s": batch_s,
-            "replays": timing["replays"],
-            "device_ms": timing["ms_total"],
-            "solve_all_s": bs.last_timing["solve_all_s"],
-            "energies_s": bs.last_timing["energies_s"],
-            "inner_iterations_total": iters,
-            "dof_iter_per_s": dof_it / batch_s,
-            "autotune": tune,
-            "spmv_all_systems_gbs": spmv_bytes / (tune["ms_per_spmv"] * 1e-3) / 1e9,
-            "spmv_bytes_all_systems": spmv_bytes,
-            "max_rel_energy_err_vs_fp64": max(errs),
-        },
-        "fp64_one_at_a_time": {
-            "solves_per_s": args.systems / fp64_s,
-            "solve_s": fp64_s,
-            "iterations_total": it64,
-        },
-        "roofline": {
-            "kernel": "batched compact FP32 SpMV (kb_spmv: tiles of the batch's systems, autotuned tile and waves)",
-            "bound": "hbm",
-            "achieved": spmv_bytes / (tune["ms_per_spmv"] * 1e-3) / 1e9,
-            "peak": HBM_PEAK_GBS,
-            "unit": "GB/s",
-            "frac": spmv_bytes / (tune["ms_per_spmv"] * 1e-3) / 1e9 / HBM_PEAK_GBS,
-            "traffic": None,
-            "bytes_per_launch": spmv_bytes,
-            "avg_launch_ms": tune["ms_per_spmv"],
-            "timing": "autotuner: HIP events around 5 launches of the chosen tiling over every system",
-        },
-        "batch_over_fp64_one_at_a_time": fp64_s / batch_s,
-        "setup_s": setup_s,
     }
+    for k in ("batch", "roofline", "fp64_one_at_a_time", "batch_over_fp64_one_at_a_time", "setup_s"):
+        out[k] = L[k]
+    return out
 
 
 def serving_leg(n_requests, rate, tol, device_mesh, probe=0.0):
@@ -513,6 +528,8 @@ def main():
     ap.add_argument("--requests", type=int, default=48, help="serve workload: number of requests")
     ap.add_argument("--rate", type=float, default=8.0, help="serve workload: Poisson arrival rate [1/s]")
     ap.add_argument("--systems", type=int, default=512, help="batch workload: number of ligand systems")
+    ap.add_argument("--batch-systems", type=int, default=128,
+                    help="default run: ligand systems of the batched leg (0: skip it)")
     ap.add_argument("--device-mesh", action="store_true",
                     help="serve workload: requests carry only atoms and box; mesh, rays and owners built on the GPU")
     ap.add_argument("--tol", type=float, default=1e-10)
@@ -695,6 +712,10 @@ def main():
                                  "(mixed precision) with HBM-footprint admission, requests arriving as atoms")
         out["serving"] = dict(summ, rate_hz=args.rate, unknowns_total=L["unknowns"], peak_admitted_bytes=L["peak"],
                               max_rel_energy_err_vs_fp64=L["max_err"], fp64_sequential=L["fp64"])
+        if args.batch_systems > 0:
+            # config 4's batched ligands (the first batch_systems of its 512)
+            log(f"batched ligands: {args.batch_systems} systems, one batch against one-at-a-time FP64")
+            out["batched_ligands"] = batch_leg(args.batch_systems, args.tol)
     else:
         out["p50_latency_ms"] = pct(mixed["lat_ms"], 50)
         out["p99_latency_ms"] = pct(mixed["lat_ms"], 99)
