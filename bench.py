@@ -469,7 +469,7 @@ def bench_spmv(args):
     from cudanextgenpb_amd.batch import prepare
 
     spec = W.large_spec() if args.mesh_size == "large" else W.globular_spec()
-    p = prepare(spec)
+    p = prepare(spec, probe=args.probe)  # the molecules' surface: SES with the default probe
     ctx = p.ctx
     info = ctx.operator_info()
     x = np.random.default_rng(1).standard_normal(p.n)
@@ -501,8 +501,8 @@ def bench_spmv(args):
         "vs_baseline": None,
         "dtype": "fp32",
         "data": "synthetic",
-        "config": {"workload": f"{spec.name}: operator of {p.n} rows, {p.nnz} nnz assembled from atoms on the device; "
-                               f"SpMV schedule {args.spmv_schedule}, row order {args.row_band}",
+        "config": {"workload": f"{spec.name}: operator of {p.n} rows, {p.nnz} nnz assembled from atoms on the device "
+                               f"(SES, probe {args.probe:g} A); SpMV schedule {args.spmv_schedule}, row order {args.row_band}",
                    "parallelism": "single GPU"},
         "roofline": {"bound": "hbm", "achieved": gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": gbs / HBM_PEAK_GBS,
                      "traffic": None, "bytes_per_launch": b32, "avg_launch_ms": ms32},
