@@ -5,7 +5,8 @@
   energies (per-system reductions in a fixed order, independent of the others).
 * Tiling: every tile shape the autotuner may pick gives bit-identical results.
 * Accuracy: each batched solve reaches ||b - A x|| <= 1e-10 ||b|| in FP64 and its
-  polar energy agrees with the single-system FP64 solve to 1e-6 relative.
+  polar energy agrees with the single-system FP64 solve to 1e-6 relative, and with the
+  oracle's FP64 solution of the same system (energy 1e-6, potential 1e-7).
 """
 import functools
 
@@ -158,3 +159,21 @@ def test_remove_running_system_mid_batch():
     for c in ctxs:
         c.close()
     b.close()
+
+
+def test_batch_against_oracle():
+    """The batched mixed solves against the oracle's FP64 solution of the same systems
+    (markers, fractions, rhs, assembly, PCG to 1e-12 and energy, all on the CPU): polar
+    energy within 1e-6 and potential within 1e-7 relative."""
+    import oracle as O
+
+    probs = systems()
+    out = solve_in_batch(probs)
+    for P, (phi, E, st) in zip(probs, out):
+        R = O.full_pipeline(P, tol=1e-12, nthreads=8)
+        assert R["status"] == 0
+        er = R["energy"]["polar"]
+        dphi = np.linalg.norm(phi - R["phi"]) / np.linalg.norm(R["phi"])
+        print(f"[batch vs oracle] {P.name}: E {E['polar']:.10f} vs {er:.10f}, phi rel {dphi:.1e}", flush=True)
+        assert abs(E["polar"] - er) <= ENERGY_RTOL * abs(er)
+        assert dphi <= 1e-7
