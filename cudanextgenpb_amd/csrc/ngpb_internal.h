@@ -2,6 +2,7 @@
 #pragma once
 
 #include <hip/hip_runtime.h>
+#include <hip/hip_ext.h>
 
 #include <algorithm>
 #include <cstdint>
@@ -305,12 +306,16 @@ struct ngpb_ctx {
   // optional HIP-event timing of SpMV launches (kind 0: FP64 CSR, 1: compact FP32)
   bool profile = false;
   struct KProf {
-    std::vector<hipEvent_t> ev;  // begin/end pairs
-    std::vector<int64_t> tag;    // launch index of each pair
-    size_t used = 0;             // events recorded since the last collection
+    std::vector<hipEvent_t> ev;  // start/stop pairs, one per timed kernel
+    std::vector<int64_t> tag;    // launch index of the SpMV each pair belongs to
+    std::vector<char> opens;     // 1 for the first kernel of that SpMV
+    size_t used = 0;             // events attached since the last collection
     int64_t launches = 0;
     double ms = 0.0;
   } kprof[2];
+  int prof_kind = -1;   // SpMV being timed: its kind, or -1
+  int64_t prof_tag = 0;
+  bool prof_first = false;
 
   int64_t device_bytes() const;
 };
@@ -384,11 +389,23 @@ void run_energy(ngpb_ctx* c, ngpb_energy_out* out);
 int pot_field_branch(int P, int F, double k);
 void run_pot_field(ngpb_ctx* c, int P, int F, double* table_host, double out4[4], int* branch);
 
-// SpMV launch timing: every kProfEvery-th launch is bracketed by events
+// SpMV launch timing: every kProfEvery-th SpMV of a solve is timed. Between prof_begin
+// and prof_end, the kernels launched through spmv_launch carry a start/stop event pair
+// of their own (hipExtLaunchKernelGGL), which times the kernel itself: the gap between
+// kernels and the end-of-kernel cache release are not counted.
 constexpr int64_t kProfEvery = 8;
 void prof_begin(ngpb_ctx* c, int kind, int64_t tag);
 void prof_end(ngpb_ctx* c, int kind);
 void prof_collect(ngpb_ctx* c, int kind, int64_t limit);
+hipEvent_t* prof_pair(ngpb_ctx* c);  // nullptr unless an SpMV is being timed
+
+template <typename... K, typename... A>
+void spmv_launch(ngpb_ctx* c, void (*k)(K...), dim3 grid, dim3 block, A... args) {
+  if (hipEvent_t* e = prof_pair(c))
+    hipExtLaunchKernelGGL(k, grid, block, 0, c->stream, e[0], e[1], 0, args...);
+  else
+    hipLaunchKernelGGL(k, grid, block, 0, c->stream, args...);
+}
 double time_copy(ngpb_ctx* c, int64_t bytes, int reps);
 
 // launch helper

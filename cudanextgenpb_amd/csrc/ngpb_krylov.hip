@@ -118,11 +118,9 @@ template <bool DOT>
 void launch_spmv64(ngpb_ctx* c, unsigned grid, const double* x, double* y, double* part) {
   const KrylovScalars* ks = DOT ? c->ks.p : nullptr;
   if (c->spmv_sched == 1)
-    hipLaunchKernelGGL((k_spmv64<DOT, 1>), dim3(grid), dim3(kBS), 0, c->stream, c->n, c->rowptr.p, c->col.p, c->val.p,
-                       x, y, ks, part);
+    spmv_launch(c, k_spmv64<DOT, 1>, dim3(grid), dim3(kBS), c->n, c->rowptr.p, c->col.p, c->val.p, x, y, ks, part);
   else
-    hipLaunchKernelGGL((k_spmv64<DOT, 0>), dim3(grid), dim3(kBS), 0, c->stream, c->n, c->rowptr.p, c->col.p, c->val.p,
-                       x, y, ks, part);
+    spmv_launch(c, k_spmv64<DOT, 0>, dim3(grid), dim3(kBS), c->n, c->rowptr.p, c->col.p, c->val.p, x, y, ks, part);
 }
 
 // x = 0, r = b, p = D^-1 b; per block (r, D^-1 r), (b, b)
@@ -255,37 +253,49 @@ unsigned vec_grid(int64_t n) {
   return (unsigned)std::max<int64_t>(1, std::min<int64_t>((n + 4 * kBS - 1) / (4 * kBS), 2048));
 }
 
-namespace {
-void prof_record(ngpb_ctx* c, int kind) {
-  auto& p = c->kprof[kind];
-  if (p.used == p.ev.size()) {
+// opens the timing of one SpMV when profiling is on and tag is kProfEvery / 2 modulo
+// kProfEvery (tag < 0: always). The solvers poll the host every check_every launches
+// (32 by default, a multiple of kProfEvery), so the sample leaves out the first launch
+// after each poll, which starts on an idle GPU.
+void prof_begin(ngpb_ctx* c, int kind, int64_t tag) {
+  c->prof_kind = -1;
+  if (!c->profile || (tag >= 0 && tag % kProfEvery != kProfEvery / 2)) return;
+  c->prof_kind = kind;
+  c->prof_tag = tag;
+  c->prof_first = true;
+}
+
+void prof_end(ngpb_ctx* c, int kind) {
+  (void)kind;
+  c->prof_kind = -1;
+}
+
+// the event pair for the next kernel of the SpMV being timed
+hipEvent_t* prof_pair(ngpb_ctx* c) {
+  if (c->prof_kind < 0) return nullptr;
+  auto& p = c->kprof[c->prof_kind];
+  while (p.ev.size() < p.used + 2) {
     hipEvent_t e;
     NGPB_HIP(hipEventCreate(&e));
     p.ev.push_back(e);
   }
-  NGPB_HIP(hipEventRecord(p.ev[p.used++], c->stream));
-}
-}  // namespace
-
-// brackets the launch with events when profiling is on and tag is a multiple of
-// kProfEvery (tag < 0: always)
-void prof_begin(ngpb_ctx* c, int kind, int64_t tag) {
-  if (!c->profile || (tag >= 0 && tag % kProfEvery != 0)) return;
-  auto& p = c->kprof[kind];
-  if (p.tag.size() < p.used / 2 + 1) p.tag.resize(p.used / 2 + 1);
-  p.tag[p.used / 2] = tag;
-  prof_record(c, kind);
-}
-
-void prof_end(ngpb_ctx* c, int kind) {
-  if (!c->profile) return;
-  auto& p = c->kprof[kind];
-  if (p.used % 2 == 1) prof_record(c, kind);
+  const size_t k = p.used / 2;
+  if (p.tag.size() <= k) {
+    p.tag.resize(k + 1);
+    p.opens.resize(k + 1);
+  }
+  p.tag[k] = c->prof_tag;
+  p.opens[k] = c->prof_first ? 1 : 0;
+  c->prof_first = false;
+  hipEvent_t* e = &p.ev[p.used];
+  p.used += 2;
+  return e;
 }
 
-// Call after the stream has been synchronized: adds the recorded launches whose tag
-// is below `limit` (launches queued after the stop flag exit at once and would bias
-// the average; limit < 0 keeps all) and clears the record.
+// Call after the stream has been synchronized: adds the timed SpMVs whose tag is below
+// `limit` (launches queued after the stop flag exit at once and would bias the
+// average; limit < 0 keeps all) and clears the record. An SpMV's time is the sum of
+// its kernels' own times.
 void prof_collect(ngpb_ctx* c, int kind, int64_t limit) {
   auto& p = c->kprof[kind];
   for (size_t i = 0; i + 1 < p.used; i += 2) {
@@ -293,7 +303,7 @@ void prof_collect(ngpb_ctx* c, int kind, int64_t limit) {
     float ms = 0.f;
     NGPB_HIP(hipEventElapsedTime(&ms, p.ev[i], p.ev[i + 1]));
     p.ms += ms;
-    ++p.launches;
+    if (p.opens[i / 2]) ++p.launches;
   }
   p.used = 0;
 }

@@ -1045,47 +1045,47 @@ void launch_sell_t(ngpb_ctx* c, const float* x, float* y, double* part) {
   const Sell A = sell_view(c);
   if (c->spmv_sched == 1) {
     const unsigned g = (unsigned)sell_partials(c);
-    hipLaunchKernelGGL(k_sell_spmv<DOT>, dim3(g), dim3(kBS), 0, c->stream, A, x, y, ks, part);
+    spmv_launch(c, k_sell_spmv<DOT>, dim3(g), dim3(kBS), A, x, y, ks, part);
     return;
   }
   if (c->spmv_sched >= 3) {
     const unsigned g = (unsigned)std::max<int64_t>(3, sell_resident(c));
     if (c->spmv_sched == 4) {
       NGPB_REQUIRE(g % 8 == 0, NGPB_E_STATE, "XCD-segmented SpMV needs a grid that is a multiple of 8");
-      hipLaunchKernelGGL((k_sell_both<DOT, true>), dim3(g), dim3(kBS), 0, c->stream, A, c->sell_list_delta.p,
-                         c->n_list_delta, c->sell_list_packed.p, c->n_list_packed, x, y, ks, part);
+      spmv_launch(c, k_sell_both<DOT, true>, dim3(g), dim3(kBS), A, c->sell_list_delta.p,
+                  c->n_list_delta, c->sell_list_packed.p, c->n_list_packed, x, y, ks, part);
     } else if (c->spmv_sched == 6) {
       NGPB_REQUIRE(g % 8 == 0, NGPB_E_STATE, "XCD-segmented SpMV needs a grid that is a multiple of 8");
-      hipLaunchKernelGGL(k_sell_lean<DOT>, dim3(g), dim3(kBS), 0, c->stream, A,
-                         reinterpret_cast<const DPair*>(c->sell_dpair.p), (c->n_list_delta + 1) / 2,
-                         c->sell_list_packed.p, reinterpret_cast<const PkHdr*>(c->sell_pk_hdr_list.p),
-                         c->n_list_packed, x, y, ks, part);
+      spmv_launch(c, k_sell_lean<DOT>, dim3(g), dim3(kBS), A,
+                  reinterpret_cast<const DPair*>(c->sell_dpair.p), (c->n_list_delta + 1) / 2,
+                  c->sell_list_packed.p, reinterpret_cast<const PkHdr*>(c->sell_pk_hdr_list.p),
+                  c->n_list_packed, x, y, ks, part);
     } else if (c->spmv_sched == 5) {
       NGPB_REQUIRE(g % 8 == 0, NGPB_E_STATE, "XCD-segmented SpMV needs a grid that is a multiple of 8");
-      hipLaunchKernelGGL((k_sell_both<DOT, true, true>), dim3(g), dim3(kBS), 0, c->stream, A, c->sell_list_delta.p,
-                         c->n_list_delta, c->sell_list_packed.p, c->n_list_packed, x, y, ks, part);
+      spmv_launch(c, k_sell_both<DOT, true, true>, dim3(g), dim3(kBS), A, c->sell_list_delta.p,
+                  c->n_list_delta, c->sell_list_packed.p, c->n_list_packed, x, y, ks, part);
     } else {
-      hipLaunchKernelGGL((k_sell_both<DOT, false>), dim3(g), dim3(kBS), 0, c->stream, A, c->sell_list_delta.p,
-                         c->n_list_delta, c->sell_list_packed.p, c->n_list_packed, x, y, ks, part);
+      spmv_launch(c, k_sell_both<DOT, false>, dim3(g), dim3(kBS), A, c->sell_list_delta.p,
+                  c->n_list_delta, c->sell_list_packed.p, c->n_list_packed, x, y, ks, part);
     }
     const unsigned go = form_grid(c, c->n_list_offset);
     if (go)
-      hipLaunchKernelGGL(k_sell_offset<DOT>, dim3(go), dim3(kBS), 0, c->stream, A, c->sell_list_offset.p,
-                         c->n_list_offset, x, y, ks, DOT ? part + g : nullptr);
+      spmv_launch(c, k_sell_offset<DOT>, dim3(go), dim3(kBS), A, c->sell_list_offset.p,
+                  c->n_list_offset, x, y, ks, DOT ? part + g : nullptr);
     return;
   }
   const unsigned gd = form_grid(c, (c->n_list_delta + 1) / 2);
   const unsigned gp = form_grid(c, c->n_list_packed);
   const unsigned go = form_grid(c, c->n_list_offset);
   if (gd)
-    hipLaunchKernelGGL(k_sell_delta<DOT>, dim3(gd), dim3(kBS), 0, c->stream, A, c->sell_list_delta.p, c->n_list_delta,
-                       x, y, ks, part);
+    spmv_launch(c, k_sell_delta<DOT>, dim3(gd), dim3(kBS), A, c->sell_list_delta.p, c->n_list_delta,
+                x, y, ks, part);
   if (gp)
-    hipLaunchKernelGGL(k_sell_packed<DOT>, dim3(gp), dim3(kBS), 0, c->stream, A, c->sell_list_packed.p,
-                       c->n_list_packed, x, y, ks, DOT ? part + gd : nullptr);
+    spmv_launch(c, k_sell_packed<DOT>, dim3(gp), dim3(kBS), A, c->sell_list_packed.p,
+                c->n_list_packed, x, y, ks, DOT ? part + gd : nullptr);
   if (go)
-    hipLaunchKernelGGL(k_sell_offset<DOT>, dim3(go), dim3(kBS), 0, c->stream, A, c->sell_list_offset.p,
-                       c->n_list_offset, x, y, ks, DOT ? part + gd + gp : nullptr);
+    spmv_launch(c, k_sell_offset<DOT>, dim3(go), dim3(kBS), A, c->sell_list_offset.p,
+                c->n_list_offset, x, y, ks, DOT ? part + gd + gp : nullptr);
 }
 
 void launch_sell(ngpb_ctx* c, bool dot, const float* x, float* y, double* part) {
