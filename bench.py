@@ -112,21 +112,33 @@ def batch_leg(n_systems, tol):
     hipGraph replays, autotuned tiles) against one-at-a-time FP64 solves of the same
     systems; the batched SpMV's roofline from the autotuner's timing."""
     from cudanextgenpb_amd import native, workload as W
-    from cudanextgenpb_amd.batch import BatchedSolver, prepare
+    from cudanextgenpb_amd.batch import BatchedSolver, prepare, prepare_many
 
+    # every system from its atoms on the device (octree, rays, owners, boundary values),
+    # first one system at a time, then with one grouped assembly over all of them
+    specs = [W.ligand_spec(i) for i in range(n_systems)]
     t = time.perf_counter()
     preps, tot_n = [], 0
-    for i in range(n_systems):
-        p = prepare(W.ligand(i))
-        p.ctx._keep_mesh = None
-        preps.append(p)
-        tot_n += p.n
+    for i, spec in enumerate(specs):
+        preps.append(prepare(spec))
+        tot_n += preps[-1].n
         if (i + 1) % 64 == 0:
-            log(f"prepared {i + 1}/{n_systems} systems ({tot_n / 1e6:.1f}M unknowns, "
-                f"{time.perf_counter() - t:.0f} s)")
+            log(f"prepared {i + 1}/{n_systems} systems one at a time ({tot_n / 1e6:.1f}M unknowns, "
+                f"{time.perf_counter() - t:.1f} s)")
     setup_s = time.perf_counter() - t
+    asm_one_s = sum(p.meta["assemble_ms"] for p in preps) * 1e-3
 
-    # one-at-a-time FP64 (single-system path), solve + energy per system
+    for p in preps:
+        p.ctx.close()
+    t = time.perf_counter()
+    preps = prepare_many(specs)
+    grouped_s = time.perf_counter() - t
+    asm_many_s = sum(p.meta["assemble_ms"] for p in preps) * 1e-3
+    log(f"prepared {n_systems} systems with one grouped assembly in {grouped_s:.2f} s (assembly {asm_many_s:.3f} s; "
+        f"one at a time {setup_s:.2f} s, assembly {asm_one_s:.3f} s)")
+
+    # one-at-a-time FP64 (single-system path), solve + energy per system, on the same
+    # contexts the batch then solves
     t = time.perf_counter()
     e64, it64 = [], 0
     for p in preps:
@@ -194,6 +206,14 @@ def batch_leg(n_systems, tol):
         },
         "batch_over_fp64_one_at_a_time": fp64_s / batch_s,
         "setup_s": setup_s,
+        # preparation from atoms on the device (not in the timed solve): one system at a
+        # time, and with one grouped assembly (ngpb_assemble_many) over all systems
+        "prepare": {
+            "one_at_a_time_s": setup_s,
+            "one_at_a_time_assembly_s": asm_one_s,
+            "grouped_s": grouped_s,
+            "grouped_assembly_s": asm_many_s,
+        },
     }
 
 
@@ -223,7 +243,7 @@ def bench_batch(args):
             "parallelism": "single GPU",
         },
     }
-    for k in ("batch", "roofline", "fp64_one_at_a_time", "batch_over_fp64_one_at_a_time", "setup_s"):
+    for k in ("batch", "roofline", "fp64_one_at_a_time", "batch_over_fp64_one_at_a_time", "setup_s", "prepare"):
         out[k] = L[k]
     return out
 

@@ -1,8 +1,9 @@
 """Batched solves of many small, uneven PB systems (ligand poses, titration states).
 
 Every system is prepared on the device with its own context, exactly as a single
-solve is (markers -> charges -> Dirichlet data -> assembly), and then handed to one
-``native.Batch``. The batch iterates all systems together in a captured hipGraph with
+solve is (markers -> charges -> Dirichlet data -> assembly; prepare_many assembles all
+of them together, one launch per assembly stage over a table of the systems), and then
+handed to one ``native.Batch``. The batch iterates all systems together in a captured hipGraph with
 per-system CG state in HBM; the tile shape of its kernels comes from the autotuner.
 Energies are evaluated per context once a system is done.
 
@@ -33,6 +34,34 @@ class Prepared:
     meta: Dict = field(default_factory=dict)
 
 
+def _inputs(problem: Problem, c: native.Context, device_mesh: bool, probe: float) -> bool:
+    """Everything of a preparation before the assembly; returns device_mesh as used."""
+    from .workload import Spec
+    device_mesh = device_mesh or isinstance(problem, Spec)  # a Spec carries no host mesh
+    c.set_physics(problem.physics)
+    a = problem.atoms
+    if device_mesh:
+        gen = problem if isinstance(problem, Spec) else problem.mesh
+        c.build_octree(*gen.generator_params(), a.xyz, a.r)
+        if probe > 0.0:
+            c.cast_rays_ses(a.xyz, a.r, probe)
+        else:
+            c.cast_rays(a.xyz, a.r)
+        c.create_markers()
+        c.set_charges(a.xyz, a.q, c.locate_atoms(a.xyz, problem.atom_tol))
+    else:
+        c.set_mesh(problem.mesh)
+        if probe > 0.0:
+            c.cast_rays_ses(a.xyz, a.r, probe)
+        else:
+            c.set_rays(problem.rays)
+        c.create_markers()
+        c.set_charges(a.xyz, a.q, problem.owner)
+        c.set_dirichlet(problem.mesh.bnd_nodes, None)
+    c.dirichlet_values(problem.bc_type)
+    return device_mesh
+
+
 def prepare(problem: Problem, device: int = 0, ctx: Optional[native.Context] = None,
             device_mesh: bool = False, probe: float = 0.0) -> Prepared:
     """Uploads the problem and assembles it on the device (no solve). device_mesh: only
@@ -42,36 +71,41 @@ def prepare(problem: Problem, device: int = 0, ctx: Optional[native.Context] = N
     solvent-excluded surface with that probe radius."""
     t0 = time.perf_counter()
     c = ctx or native.Context(device)
-    from .workload import Spec
-    device_mesh = device_mesh or isinstance(problem, Spec)  # a Spec carries no host mesh
     try:
-        c.set_physics(problem.physics)
-        a = problem.atoms
-        if device_mesh:
-            gen = problem if isinstance(problem, Spec) else problem.mesh
-            c.build_octree(*gen.generator_params(), a.xyz, a.r)
-            if probe > 0.0:
-                c.cast_rays_ses(a.xyz, a.r, probe)
-            else:
-                c.cast_rays(a.xyz, a.r)
-            c.create_markers()
-            c.set_charges(a.xyz, a.q, c.locate_atoms(a.xyz, problem.atom_tol))
-        else:
-            c.set_mesh(problem.mesh)
-            if probe > 0.0:
-                c.cast_rays_ses(a.xyz, a.r, probe)
-            else:
-                c.set_rays(problem.rays)
-            c.create_markers()
-            c.set_charges(a.xyz, a.q, problem.owner)
-            c.set_dirichlet(problem.mesh.bnd_nodes, None)
-        c.dirichlet_values(problem.bc_type)
+        _inputs(problem, c, device_mesh, probe)
+        t1 = time.perf_counter()
         c.assemble()
     except BaseException:
         if ctx is None:  # a context made here does not outlive a failed preparation
             c.close()
         raise
-    return Prepared(problem.name, c, c.n, c.nnz, a.n, (time.perf_counter() - t0) * 1e3)
+    t2 = time.perf_counter()
+    return Prepared(problem.name, c, c.n, c.nnz, problem.atoms.n, (t2 - t0) * 1e3, {"assemble_ms": (t2 - t1) * 1e3})
+
+
+def prepare_many(problems, device: int = 0, device_mesh: bool = False, probe: float = 0.0) -> List[Prepared]:
+    """prepare() of many problems, with the assembly of all of them in one launch per
+    stage over a table of the systems (ngpb_assemble_many): each system is
+    bit-identical to its own prepare(). The per-problem inputs (mesh, rays, markers,
+    charges, boundary values) are still set up one problem at a time."""
+    t0 = time.perf_counter()
+    problems = list(problems)
+    ctxs: List[native.Context] = []
+    try:
+        for P in problems:
+            ctxs.append(native.Context(device))
+            _inputs(P, ctxs[-1], device_mesh, probe)
+        t1 = time.perf_counter()
+        native.assemble_many(ctxs)
+    except BaseException:
+        for c in ctxs:
+            c.close()
+        raise
+    t2 = time.perf_counter()
+    k = max(len(problems), 1)
+    # per system: its share of the whole preparation and of the one grouped assembly
+    return [Prepared(P.name, c, c.n, c.nnz, P.atoms.n, (t2 - t0) * 1e3 / k, {"assemble_ms": (t2 - t1) * 1e3 / k})
+            for P, c in zip(problems, ctxs)]
 
 
 class BatchedSolver:

@@ -638,6 +638,73 @@ int ngpb_assemble(ngpb_ctx* c) {
   });
 }
 
+int ngpb_assemble_many(ngpb_ctx* const* cs, int n) {
+  if (n < 0 || (n > 0 && !cs)) return NGPB_E_ARG;
+  if (n == 0) return NGPB_OK;
+  std::vector<ngpb_ctx*> v(cs, cs + n);
+  for (ngpb_ctx* c : v)
+    if (!c) return NGPB_E_ARG;
+  {
+    std::vector<ngpb_ctx*> u(v);
+    std::sort(u.begin(), u.end());
+    if (std::adjacent_find(u.begin(), u.end()) != u.end()) {
+      for (ngpb_ctx* c : v) c->err = "a context appears twice in ngpb_assemble_many";
+      return NGPB_E_ARG;
+    }
+  }
+  // each context's own preparation, as in ngpb_assemble
+  std::vector<char> sym((size_t)n);
+  for (int i = 0; i < n; ++i) {
+    ngpb_ctx* c = v[i];
+    const int rc = guarded(c, [&] {
+      NGPB_REQUIRE(c->device == v[0]->device, NGPB_E_ARG, "ngpb_assemble_many needs contexts on one device");
+      not_in_batch(c);
+      need(c->have_mesh, "no mesh");
+      need(c->have_markers, "markers must be created or set before assembly");
+      if (!c->have_dirichlet) {
+        c->NB = 0;
+        std::vector<uint8_t> isb((size_t)c->N, 0);
+        std::vector<double> g((size_t)c->N, 0.0);
+        c->is_bnd.upload(isb.data(), isb.size(), c->stream);
+        c->gval.upload(g.data(), g.size(), c->stream);
+        c->bnd_node.resize(1);
+        c->have_dirichlet = c->have_bnd_values = true;
+        c->bnd_bc_type = 0;
+      }
+      if (!c->have_bnd_values && c->bnd_bc_type) ngpb::run_dirichlet_values(c, c->bnd_bc_type);
+      need(c->have_bnd_values, "Dirichlet values missing (ngpb_dirichlet_values)");
+      if (!c->have_frac) ngpb::run_fractions(c);
+      sym[i] = !c->have_symbolic || c->external_csr;
+      // a system that fails below is left without one
+      c->have_system = c->have_compact = c->have_phi = false;
+      if (sym[i]) c->have_symbolic = false;
+    });
+    if (rc != NGPB_OK) return rc;
+  }
+  auto fail_all = [&](int code, const std::string& msg) {
+    for (ngpb_ctx* c : v) c->err = msg;
+    return code;
+  };
+  try {
+    NGPB_HIP(hipSetDevice(v[0]->device));
+    ngpb::run_assemble_many(v);
+  } catch (const ngpb::Error& e) {
+    return fail_all(e.code, e.what());
+  } catch (const std::bad_alloc&) {
+    return fail_all(NGPB_E_NOMEM, "host allocation failed");
+  } catch (const std::exception& e) {
+    return fail_all(NGPB_E_HIP, e.what());
+  }
+  for (int i = 0; i < n; ++i) {
+    ngpb_ctx* c = v[i];
+    if (sym[i]) c->have_levels = false;
+    c->external_csr = false;
+    c->have_phi = false;
+    c->err.clear();
+  }
+  return NGPB_OK;
+}
+
 int ngpb_system_size(ngpb_ctx* c, int64_t* n, int64_t* nnz) {
   return guarded(c, [&] {
     NGPB_REQUIRE(n && nnz, NGPB_E_ARG, "NULL output");

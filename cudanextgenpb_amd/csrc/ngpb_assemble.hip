@@ -14,6 +14,7 @@
 // Emission ids: edge e of leaf l -> 12 l + e; corner i of leaf l -> 12 L + 8 l + i.
 // Weight code: bit 7 = negative, bits 0..6 = e with |w| = 2^-e (products of 1/np).
 #include <algorithm>
+#include <list>
 
 #include "ngpb_internal.h"
 
@@ -58,9 +59,9 @@ __device__ __forceinline__ void corner_parents(const MeshView& m, int64_t l, int
 // edges through i and the mass term of i -- into one chunk reserved with a single
 // atomic per (leaf, corner, parent); pass 0 only counts.
 template <int PASS>
-__global__ void k_emit_rows(MeshView m, unsigned* __restrict__ cnt, const int64_t* __restrict__ em_rowptr,
+__device__ __forceinline__ void k_emit_rows_body(int64_t bid, MeshView m, unsigned* __restrict__ cnt, const int64_t* __restrict__ em_rowptr,
                             uint64_t* __restrict__ em_key, uint16_t* __restrict__ em_kw) {
-  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int64_t t = bid * blockDim.x + threadIdx.x;
   if (t >= 8 * m.L) return;
   const int64_t l = t >> 3;
   const int i = (int)(t & 7);
@@ -113,9 +114,19 @@ __global__ void k_emit_rows(MeshView m, unsigned* __restrict__ cnt, const int64_
   }
 }
 
-__global__ void k_u32_to_i64(int64_t n, const unsigned* __restrict__ a, int64_t* __restrict__ b) {
-  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+template <int PASS>
+__global__ void k_emit_rows(MeshView m, unsigned* __restrict__ cnt, const int64_t* __restrict__ em_rowptr,
+                            uint64_t* __restrict__ em_key, uint16_t* __restrict__ em_kw) {
+  k_emit_rows_body<PASS>(blockIdx.x, m, cnt, em_rowptr, em_key, em_kw);
+}
+
+__device__ __forceinline__ void k_u32_to_i64_body(int64_t bid, int64_t n, const unsigned* __restrict__ a, int64_t* __restrict__ b) {
+  const int64_t i = bid * blockDim.x + threadIdx.x;
   if (i < n) b[i] = a[i];
+}
+
+__global__ void k_u32_to_i64(int64_t n, const unsigned* __restrict__ a, int64_t* __restrict__ b) {
+  k_u32_to_i64_body(blockIdx.x, n, a, b);
 }
 
 __device__ __forceinline__ int64_t wave_sum_i64(int64_t v) {
@@ -176,12 +187,12 @@ __device__ __forceinline__ uint64_t shfl_u64(uint64_t v, int src) {
 // Rows of at most 64 emissions: one wave per row, one emission per lane, sorted by a
 // bitonic network over the lanes (unused lanes hold keys that sort last). Longer rows
 // are listed for k_row_sort_block.
-__global__ void __launch_bounds__(kSortBS) k_row_sort_wave(int64_t N, const int64_t* __restrict__ em_rowptr,
+__device__ __forceinline__ void k_row_sort_wave_body(int64_t bid, int64_t N, const int64_t* __restrict__ em_rowptr,
                                                            uint64_t* __restrict__ key, uint16_t* __restrict__ kw,
                                                            const uint8_t* __restrict__ is_bnd, int64_t* __restrict__ cnt4,
                                                            int32_t* __restrict__ big_rows, int* __restrict__ n_big) {
   const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
-  const int64_t row = (int64_t)blockIdx.x * (kSortBS / 64) + wv;
+  const int64_t row = bid * (kSortBS / 64) + wv;
   if (row >= N) return;  // the whole wave
   const int64_t b = em_rowptr[row], e = em_rowptr[row + 1];
   const int E = (int)(e - b);
@@ -218,15 +229,22 @@ __global__ void __launch_bounds__(kSortBS) k_row_sort_wave(int64_t N, const int6
   if (lane < 4) cnt4[(int64_t)lane * N + row] = v[lane];
 }
 
+__global__ void __launch_bounds__(kSortBS) k_row_sort_wave(int64_t N, const int64_t* __restrict__ em_rowptr,
+                                                           uint64_t* __restrict__ key, uint16_t* __restrict__ kw,
+                                                           const uint8_t* __restrict__ is_bnd, int64_t* __restrict__ cnt4,
+                                                           int32_t* __restrict__ big_rows, int* __restrict__ n_big) {
+  k_row_sort_wave_body(blockIdx.x, N, em_rowptr, key, kw, is_bnd, cnt4, big_rows, n_big);
+}
+
 // one block per listed long row
-__global__ void __launch_bounds__(kSortBS) k_row_sort_block(const int32_t* __restrict__ rows,
+__device__ __forceinline__ void k_row_sort_block_body(int64_t bid, const int32_t* __restrict__ rows,
                                                             const int64_t* __restrict__ em_rowptr,
                                                             uint64_t* __restrict__ key, uint16_t* __restrict__ kw,
                                                             const uint8_t* __restrict__ is_bnd, int64_t N,
                                                             int64_t* __restrict__ cnt4) {
   extern __shared__ uint64_t s_dyn[];  // key[cap], then kw[cap]
   __shared__ int64_t s_red[4][kSortBS / 64];
-  const int64_t row = rows[blockIdx.x];
+  const int64_t row = rows[bid];
   const int64_t b = em_rowptr[row], e = em_rowptr[row + 1];
   const int E = (int)(e - b);
   const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
@@ -298,6 +316,14 @@ __global__ void __launch_bounds__(kSortBS) k_row_sort_block(const int32_t* __res
   }
 }
 
+__global__ void __launch_bounds__(kSortBS) k_row_sort_block(const int32_t* __restrict__ rows,
+                                                            const int64_t* __restrict__ em_rowptr,
+                                                            uint64_t* __restrict__ key, uint16_t* __restrict__ kw,
+                                                            const uint8_t* __restrict__ is_bnd, int64_t N,
+                                                            int64_t* __restrict__ cnt4) {
+  k_row_sort_block_body(blockIdx.x, rows, em_rowptr, key, kw, is_bnd, N, cnt4);
+}
+
 // One wave per row over its sorted emissions, 64 at a time: the matrix slots (col,
 // diag_slot, sptr = first emission of the slot in the repacked list) and the couplings
 // to Dirichlet columns (bc_col, bc_ptr), and the emissions of each, repacked to 5 bytes
@@ -305,7 +331,7 @@ __global__ void __launch_bounds__(kSortBS) k_row_sort_block(const int32_t* __res
 // the row's offsets (prefix sums of the per-row counts).
 constexpr int kSlotBS = 256;
 
-__global__ void __launch_bounds__(kSlotBS) k_row_slots(
+__device__ __forceinline__ void k_row_slots_body(int64_t bid, 
     int64_t N, const int64_t* __restrict__ em_rowptr, const uint64_t* __restrict__ key,
     const uint16_t* __restrict__ kw, const uint8_t* __restrict__ is_bnd, const int64_t* __restrict__ off4,
     int32_t* __restrict__ rowptr, int32_t* __restrict__ col, int32_t* __restrict__ diag_slot,
@@ -314,7 +340,7 @@ __global__ void __launch_bounds__(kSlotBS) k_row_slots(
     uint32_t* __restrict__ bc_id, uint8_t* __restrict__ bc_code, int64_t n_slots, int64_t n_coup, int64_t n_slot_em,
     int64_t n_coup_em) {
   const int lane = threadIdx.x & 63;
-  const int64_t row = (int64_t)blockIdx.x * (kSlotBS / 64) + (threadIdx.x >> 6);
+  const int64_t row = bid * (kSlotBS / 64) + (threadIdx.x >> 6);
   if (row >= N) return;  // the whole wave
   const int64_t b = em_rowptr[row], e = em_rowptr[row + 1];
   const bool rb = is_bnd[row] != 0;
@@ -370,19 +396,40 @@ __global__ void __launch_bounds__(kSlotBS) k_row_slots(
   }
 }
 
+__global__ void __launch_bounds__(kSlotBS) k_row_slots(
+    int64_t N, const int64_t* __restrict__ em_rowptr, const uint64_t* __restrict__ key,
+    const uint16_t* __restrict__ kw, const uint8_t* __restrict__ is_bnd, const int64_t* __restrict__ off4,
+    int32_t* __restrict__ rowptr, int32_t* __restrict__ col, int32_t* __restrict__ diag_slot,
+    int32_t* __restrict__ bc_rowptr, int32_t* __restrict__ bc_col, int64_t* __restrict__ sptr,
+    uint32_t* __restrict__ em_id, uint8_t* __restrict__ em_code, int64_t* __restrict__ bc_ptr,
+    uint32_t* __restrict__ bc_id, uint8_t* __restrict__ bc_code, int64_t n_slots, int64_t n_coup, int64_t n_slot_em,
+    int64_t n_coup_em) {
+  k_row_slots_body(blockIdx.x, N, em_rowptr, key, kw, is_bnd, off4, rowptr, col, diag_slot, bc_rowptr, bc_col, sptr, em_id, em_code, bc_ptr, bc_id, bc_code, n_slots, n_coup, n_slot_em, n_coup_em);
+}
+
 // blk_em[b] = first emission of the first slot of rows kSvRows b ..; blk_em[nblk] = total
+__device__ __forceinline__ void k_blk_em_body(int64_t bid, int64_t nblk, int64_t N, const int32_t* __restrict__ rowptr, const int64_t* __restrict__ sptr,
+                         int64_t* __restrict__ blk_em) {
+  const int64_t b = bid * blockDim.x + threadIdx.x;
+  if (b <= nblk) blk_em[b] = sptr[rowptr[min(N, b * kSvRows)]];
+}
+
 __global__ void k_blk_em(int64_t nblk, int64_t N, const int32_t* __restrict__ rowptr, const int64_t* __restrict__ sptr,
                          int64_t* __restrict__ blk_em) {
-  const int64_t b = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (b <= nblk) blk_em[b] = sptr[rowptr[min(N, b * kSvRows)]];
+  k_blk_em_body(blockIdx.x, nblk, N, rowptr, sptr, blk_em);
+}
+
+__device__ __forceinline__ void k_slot_rel_body(int64_t bid, int64_t N, const int32_t* __restrict__ rowptr, const int64_t* __restrict__ sptr,
+                           const int64_t* __restrict__ blk_em, uint32_t* __restrict__ rel) {
+  const int64_t row = bid * blockDim.x + threadIdx.x;
+  if (row >= N) return;
+  const int64_t base = blk_em[row / kSvRows];
+  for (int64_t sl = rowptr[row]; sl < rowptr[row + 1]; ++sl) rel[sl] = (uint32_t)(sptr[sl] - base);
 }
 
 __global__ void k_slot_rel(int64_t N, const int32_t* __restrict__ rowptr, const int64_t* __restrict__ sptr,
                            const int64_t* __restrict__ blk_em, uint32_t* __restrict__ rel) {
-  const int64_t row = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (row >= N) return;
-  const int64_t base = blk_em[row / kSvRows];
-  for (int64_t sl = rowptr[row]; sl < rowptr[row + 1]; ++sl) rel[sl] = (uint32_t)(sptr[sl] - base);
+  k_slot_rel_body(blockIdx.x, N, rowptr, sptr, blk_em, rel);
 }
 
 // ------------------------------------------------------------------------ numeric
@@ -390,10 +437,10 @@ __device__ __forceinline__ double wha(double eps1, double eps2, double frac) {
   return 1.0 / (frac / eps1 + (1 - frac) / eps2);
 }
 
-__global__ void k_leaf_coeffs(MeshView m, const double* __restrict__ eps, const int32_t* __restrict__ border_of,
+__device__ __forceinline__ void k_leaf_coeffs_body(int64_t bid, MeshView m, const double* __restrict__ eps, const int32_t* __restrict__ border_of,
                               const double* __restrict__ frac, double* __restrict__ c_edge,
                               double* __restrict__ v8) {
-  const int64_t l = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int64_t l = bid * blockDim.x + threadIdx.x;
   if (l >= m.L) return;
   const int32_t c0 = m.leaf_corner[8 * l], c7 = m.leaf_corner[8 * l + 7];
   double h[3];
@@ -421,6 +468,12 @@ __global__ void k_leaf_coeffs(MeshView m, const double* __restrict__ eps, const 
     c_edge[12 * l + e] = wha(ec[kEN[2 * e]], ec[kEN[2 * e + 1]], f) * (h[b] * h[cc] / (4.0 * h[a]));
   }
   v8[l] = h[0] * h[1] * h[2] * 0.125;
+}
+
+__global__ void k_leaf_coeffs(MeshView m, const double* __restrict__ eps, const int32_t* __restrict__ border_of,
+                              const double* __restrict__ frac, double* __restrict__ c_edge,
+                              double* __restrict__ v8) {
+  k_leaf_coeffs_body(blockIdx.x, m, eps, border_of, frac, c_edge, v8);
 }
 
 __device__ __forceinline__ double decode_code(uint8_t code) {
@@ -459,7 +512,7 @@ __device__ __forceinline__ int last_le(const uint32_t* a, int nr, uint32_t v) {
   return lo;
 }
 
-__global__ void __launch_bounds__(kSvBS) k_slot_values(int64_t N, int64_t L, const int32_t* __restrict__ rowptr,
+__device__ __forceinline__ void k_slot_values_body(int64_t bid, int64_t N, int64_t L, const int32_t* __restrict__ rowptr,
                                                        const int64_t* __restrict__ blk_em,
                                                        const uint32_t* __restrict__ slot_rel,
                                                        const uint32_t* __restrict__ em_id,
@@ -470,7 +523,7 @@ __global__ void __launch_bounds__(kSvBS) k_slot_values(int64_t N, int64_t L, con
   __shared__ double s_v[kSvCap];
   __shared__ uint32_t s_re[kSvRows + 1];  // per row: first emission (block-relative)
   __shared__ uint32_t s_rs[kSvRows + 1];  // per row: first slot (block-relative)
-  const int64_t blk = blockIdx.x;
+  const int64_t blk = bid;
   const int64_t r0 = blk * kSvRows;
   const int64_t r1 = min(r0 + kSvRows, N);
   const int nr = (int)(r1 - r0);
@@ -516,12 +569,23 @@ __global__ void __launch_bounds__(kSvBS) k_slot_values(int64_t N, int64_t L, con
   }
 }
 
+__global__ void __launch_bounds__(kSvBS) k_slot_values(int64_t N, int64_t L, const int32_t* __restrict__ rowptr,
+                                                       const int64_t* __restrict__ blk_em,
+                                                       const uint32_t* __restrict__ slot_rel,
+                                                       const uint32_t* __restrict__ em_id,
+                                                       const uint8_t* __restrict__ em_code,
+                                                       const double* __restrict__ c_edge,
+                                                       const double* __restrict__ reac,
+                                                       const double* __restrict__ v8, double* __restrict__ val) {
+  k_slot_values_body(blockIdx.x, N, L, rowptr, blk_em, slot_rel, em_id, em_code, c_edge, reac, v8, val);
+}
+
 // Charge projection (src/pb_class.cpp:2002-2039). Contribution t = 32 a + 4 ii + jj
 // (atom a, leaf corner ii, parent jj); each valid one is pushed onto the list of its
 // node.
-__global__ void k_rho_push(MeshView m, int64_t A, const int32_t* __restrict__ owner, int32_t* __restrict__ cnode,
+__device__ __forceinline__ void k_rho_push_body(int64_t bid, MeshView m, int64_t A, const int32_t* __restrict__ owner, int32_t* __restrict__ cnode,
                            int32_t* __restrict__ cnext, int32_t* __restrict__ head) {
-  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int64_t t = bid * blockDim.x + threadIdx.x;
   if (t >= 32 * A) return;
   const int64_t a = t >> 5;
   const int ii = (int)((t >> 2) & 7), jj = (int)(t & 3);
@@ -542,12 +606,17 @@ __global__ void k_rho_push(MeshView m, int64_t A, const int32_t* __restrict__ ow
   cnext[t] = atomicExch(&head[node], (int32_t)t);
 }
 
+__global__ void k_rho_push(MeshView m, int64_t A, const int32_t* __restrict__ owner, int32_t* __restrict__ cnode,
+                           int32_t* __restrict__ cnext, int32_t* __restrict__ head) {
+  k_rho_push_body(blockIdx.x, m, A, owner, cnode, cnext, head);
+}
+
 // One thread per touched node (the head of its list): the patch volume (the node's
 // lumped-mass emissions with unit coefficient, in order), the density
 // rho = sum over its contributions in increasing t of q 4 pi w / vol_patch
 // (/ (np vol_patch) through a hanging corner), and the right-hand side
 // sum over the same emissions of (V/8 w) rho. Untouched nodes keep rhs 0.
-__global__ void k_rho_node(MeshView m, int64_t C, int64_t L, const double* __restrict__ axyz,
+__device__ __forceinline__ void k_rho_node_body(int64_t bid, MeshView m, int64_t C, int64_t L, const double* __restrict__ axyz,
                            const double* __restrict__ aq, const int32_t* __restrict__ owner,
                            const int32_t* __restrict__ cnode, const int32_t* __restrict__ cnext,
                            const int32_t* __restrict__ head, const int32_t* __restrict__ diag_slot,
@@ -555,7 +624,7 @@ __global__ void k_rho_node(MeshView m, int64_t C, int64_t L, const double* __res
                            const uint32_t* __restrict__ slot_rel, const uint32_t* __restrict__ em_id,
                            const uint8_t* __restrict__ em_code, const double* __restrict__ v8,
                            double* __restrict__ rho, double* __restrict__ rhs_charge) {
-  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int64_t t = bid * blockDim.x + threadIdx.x;
   if (t >= C) return;
   const int32_t node = cnode[t];
   if (node < 0 || head[node] != (int32_t)t) return;
@@ -608,7 +677,18 @@ __global__ void k_rho_node(MeshView m, int64_t C, int64_t L, const double* __res
   rhs_charge[node] = r;
 }
 
-__global__ void k_dirichlet_rhs(int64_t N, int64_t L, const uint8_t* __restrict__ is_bnd,
+__global__ void k_rho_node(MeshView m, int64_t C, int64_t L, const double* __restrict__ axyz,
+                           const double* __restrict__ aq, const int32_t* __restrict__ owner,
+                           const int32_t* __restrict__ cnode, const int32_t* __restrict__ cnext,
+                           const int32_t* __restrict__ head, const int32_t* __restrict__ diag_slot,
+                           int64_t N, const int32_t* __restrict__ rowptr, const int64_t* __restrict__ blk_em,
+                           const uint32_t* __restrict__ slot_rel, const uint32_t* __restrict__ em_id,
+                           const uint8_t* __restrict__ em_code, const double* __restrict__ v8,
+                           double* __restrict__ rho, double* __restrict__ rhs_charge) {
+  k_rho_node_body(blockIdx.x, m, C, L, axyz, aq, owner, cnode, cnext, head, diag_slot, N, rowptr, blk_em, slot_rel, em_id, em_code, v8, rho, rhs_charge);
+}
+
+__device__ __forceinline__ void k_dirichlet_rhs_body(int64_t bid, int64_t N, int64_t L, const uint8_t* __restrict__ is_bnd,
                                 const double* __restrict__ gval, const int32_t* __restrict__ diag_slot,
                                 const double* __restrict__ val, const int32_t* __restrict__ bc_rowptr,
                                 const int32_t* __restrict__ bc_col, const int64_t* __restrict__ bc_ptr,
@@ -616,7 +696,7 @@ __global__ void k_dirichlet_rhs(int64_t N, int64_t L, const uint8_t* __restrict_
                                 const double* __restrict__ c_edge, const double* __restrict__ reac,
                                 const double* __restrict__ v8, const double* __restrict__ rhs_charge,
                                 double* __restrict__ rhs) {
-  const int64_t row = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int64_t row = bid * blockDim.x + threadIdx.x;
   if (row >= N) return;
   if (is_bnd[row]) {
     rhs[row] = val[diag_slot[row]] * gval[row];
@@ -629,6 +709,17 @@ __global__ void k_dirichlet_rhs(int64_t N, int64_t L, const uint8_t* __restrict_
     r -= a * gval[bc_col[s]];
   }
   rhs[row] = r;
+}
+
+__global__ void k_dirichlet_rhs(int64_t N, int64_t L, const uint8_t* __restrict__ is_bnd,
+                                const double* __restrict__ gval, const int32_t* __restrict__ diag_slot,
+                                const double* __restrict__ val, const int32_t* __restrict__ bc_rowptr,
+                                const int32_t* __restrict__ bc_col, const int64_t* __restrict__ bc_ptr,
+                                const uint32_t* __restrict__ bc_id, const uint8_t* __restrict__ bc_code,
+                                const double* __restrict__ c_edge, const double* __restrict__ reac,
+                                const double* __restrict__ v8, const double* __restrict__ rhs_charge,
+                                double* __restrict__ rhs) {
+  k_dirichlet_rhs_body(blockIdx.x, N, L, is_bnd, gval, diag_slot, val, bc_rowptr, bc_col, bc_ptr, bc_id, bc_code, c_edge, reac, v8, rhs_charge, rhs);
 }
 
 // ------------------------------------------------------------------ Dirichlet data
@@ -657,6 +748,127 @@ __global__ void k_bc_values(int64_t nb, const int32_t* __restrict__ nodes, const
   bval[t] = v;
   gval[nd] = v;
 }
+
+// ------------------------------------------------------------- system tables
+// One launch per assembly stage over many systems: system s owns the blocks
+// [blk0[s], blk0[s + 1]) and runs the single-system kernel's body on them as its
+// blocks 0, 1, ..., so its results are those of its own launches, bit for bit.
+struct AsmSys {
+  MeshView m;
+  int64_t N = 0, L = 0, A = 0;
+  // symbolic temporaries and counts
+  unsigned* cnt = nullptr;
+  int64_t* em_rowptr = nullptr;  // N + 1
+  uint64_t* em_key = nullptr;
+  uint16_t* em_kw = nullptr;
+  int64_t* cnt4 = nullptr;  // 4 N
+  int32_t* big = nullptr;   // N
+  int* nbig = nullptr;
+  int64_t* sptr = nullptr;  // nnz + 1
+  int64_t n_em = 0, n_slots = 0, n_coup = 0, n_slot_em = 0, n_coup_em = 0;
+  // the system's own arrays
+  const uint8_t* is_bnd = nullptr;
+  int32_t *rowptr = nullptr, *col = nullptr, *diag_slot = nullptr, *bc_rowptr = nullptr, *bc_col = nullptr;
+  int64_t *bc_ptr = nullptr, *blk_em = nullptr;
+  uint32_t *em_id = nullptr, *bc_id = nullptr, *slot_rel = nullptr;
+  uint8_t *em_code = nullptr, *bc_code = nullptr;
+  // numeric
+  const double *eps = nullptr, *frac = nullptr, *reac = nullptr, *gval = nullptr, *axyz = nullptr, *aq = nullptr;
+  const int32_t *border_of = nullptr, *owner = nullptr;
+  double *c_edge = nullptr, *v8 = nullptr, *val = nullptr, *rho = nullptr, *rhs_charge = nullptr, *rhs = nullptr;
+  int32_t *cnode = nullptr, *cnext = nullptr, *head = nullptr;
+};
+
+#define NGPB_SYS                                      \
+  const int s_ = group_of(blk0, ns, blockIdx.x);     \
+  const AsmSys& S = T[s_];                           \
+  const int64_t bid = (int64_t)blockIdx.x - blk0[s_]
+
+template <int PASS>
+__global__ void kg_emit_rows(const AsmSys* __restrict__ T, const int64_t* __restrict__ blk0, int ns) {
+  NGPB_SYS;
+  k_emit_rows_body<PASS>(bid, S.m, S.cnt, S.em_rowptr, S.em_key, S.em_kw);
+}
+
+__global__ void kg_u32_to_i64(const AsmSys* __restrict__ T, const int64_t* __restrict__ blk0, int ns) {
+  NGPB_SYS;
+  k_u32_to_i64_body(bid, S.N, S.cnt, S.em_rowptr);
+}
+
+// one thread per system: the end of its emission row pointers
+__global__ void kg_em_end(const AsmSys* __restrict__ T, int ns) {
+  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s < ns) T[s].em_rowptr[T[s].N] = T[s].n_em;
+}
+
+__global__ void __launch_bounds__(kSortBS) kg_row_sort_wave(const AsmSys* __restrict__ T,
+                                                            const int64_t* __restrict__ blk0, int ns) {
+  NGPB_SYS;
+  k_row_sort_wave_body(bid, S.N, S.em_rowptr, S.em_key, S.em_kw, S.is_bnd, S.cnt4, S.big, S.nbig);
+}
+
+__global__ void __launch_bounds__(kSortBS) kg_row_sort_block(const AsmSys* __restrict__ T,
+                                                             const int64_t* __restrict__ blk0, int ns) {
+  NGPB_SYS;
+  k_row_sort_block_body(bid, S.big, S.em_rowptr, S.em_key, S.em_kw, S.is_bnd, S.N, S.cnt4);
+}
+
+__global__ void __launch_bounds__(kSlotBS) kg_row_slots(const AsmSys* __restrict__ T,
+                                                        const int64_t* __restrict__ blk0, int ns) {
+  NGPB_SYS;
+  const int64_t N = S.N;
+  k_row_slots_body(bid, N, S.em_rowptr, S.em_key, S.em_kw, S.is_bnd, S.cnt4, S.rowptr, S.col, S.diag_slot,
+                   S.bc_rowptr, S.bc_col, S.sptr, S.em_id, S.em_code, S.bc_ptr, S.bc_id, S.bc_code, S.n_slots,
+                   S.n_coup, S.n_slot_em, S.n_coup_em);
+}
+
+__global__ void kg_blk_em(const AsmSys* __restrict__ T, const int64_t* __restrict__ blk0, int ns) {
+  NGPB_SYS;
+  k_blk_em_body(bid, (S.N + kSvRows - 1) / kSvRows, S.N, S.rowptr, S.sptr, S.blk_em);
+}
+
+__global__ void kg_slot_rel(const AsmSys* __restrict__ T, const int64_t* __restrict__ blk0, int ns) {
+  NGPB_SYS;
+  k_slot_rel_body(bid, S.N, S.rowptr, S.sptr, S.blk_em, S.slot_rel);
+}
+
+__global__ void kg_leaf_coeffs(const AsmSys* __restrict__ T, const int64_t* __restrict__ blk0, int ns) {
+  NGPB_SYS;
+  k_leaf_coeffs_body(bid, S.m, S.eps, S.border_of, S.frac, S.c_edge, S.v8);
+}
+
+__global__ void __launch_bounds__(kSvBS) kg_slot_values(const AsmSys* __restrict__ T,
+                                                        const int64_t* __restrict__ blk0, int ns) {
+  NGPB_SYS;
+  k_slot_values_body(bid, S.N, S.L, S.rowptr, S.blk_em, S.slot_rel, S.em_id, S.em_code, S.c_edge, S.reac, S.v8,
+                     S.val);
+}
+
+// rhs_charge = 0 (the nodes no atom touches keep it)
+__global__ void kg_zero_charge(const AsmSys* __restrict__ T, const int64_t* __restrict__ blk0, int ns) {
+  NGPB_SYS;
+  const int64_t i = bid * blockDim.x + threadIdx.x;
+  if (i < S.N) S.rhs_charge[i] = 0.0;
+}
+
+__global__ void kg_rho_push(const AsmSys* __restrict__ T, const int64_t* __restrict__ blk0, int ns) {
+  NGPB_SYS;
+  k_rho_push_body(bid, S.m, S.A, S.owner, S.cnode, S.cnext, S.head);
+}
+
+__global__ void kg_rho_node(const AsmSys* __restrict__ T, const int64_t* __restrict__ blk0, int ns) {
+  NGPB_SYS;
+  k_rho_node_body(bid, S.m, 32 * S.A, S.L, S.axyz, S.aq, S.owner, S.cnode, S.cnext, S.head, S.diag_slot, S.N,
+                  S.rowptr, S.blk_em, S.slot_rel, S.em_id, S.em_code, S.v8, S.rho, S.rhs_charge);
+}
+
+__global__ void kg_dirichlet_rhs(const AsmSys* __restrict__ T, const int64_t* __restrict__ blk0, int ns) {
+  NGPB_SYS;
+  k_dirichlet_rhs_body(bid, S.N, S.L, S.is_bnd, S.gval, S.diag_slot, S.val, S.bc_rowptr, S.bc_col, S.bc_ptr,
+                       S.bc_id, S.bc_code, S.c_edge, S.reac, S.v8, S.rhs_charge, S.rhs);
+}
+
+#undef NGPB_SYS
 
 }  // namespace
 
@@ -838,6 +1050,283 @@ void run_dirichlet_values(ngpb_ctx* c, int bc_type) {
   NGPB_HIP(hipGetLastError());
   NGPB_HIP(hipStreamSynchronize(c->stream));
   c->have_bnd_values = true;
+}
+
+// ------------------------------------------------------- assembly of many systems
+namespace {
+
+// The launches of one grouped stage: block ranges and system tables uploaded on the
+// stream stay allocated, with their host copies, until the run ends.
+struct GroupRun {
+  hipStream_t st;
+  std::list<std::vector<int64_t>> host_ranges;
+  std::list<DBuf<int64_t>> ranges;
+  std::list<std::vector<AsmSys>> host_tables;
+  std::list<DBuf<AsmSys>> tables;
+  explicit GroupRun(hipStream_t s) : st(s) {}
+
+  const AsmSys* table(const std::vector<AsmSys>& t) {
+    host_tables.push_back(t);
+    tables.emplace_back(st);
+    tables.back().upload(host_tables.back().data(), t.size(), st);
+    return tables.back().p;
+  }
+
+  // one launch of kernel over every system; nb[s] = blocks of system s
+  template <typename K>
+  void launch(K kernel, const std::vector<int64_t>& nb, int bs, size_t lds, const AsmSys* T) {
+    host_ranges.emplace_back(nb.size() + 1, 0);
+    std::vector<int64_t>& h = host_ranges.back();
+    for (size_t i = 0; i < nb.size(); ++i) h[i + 1] = h[i] + nb[i];
+    const int64_t total = h.back();
+    if (total == 0) return;
+    NGPB_REQUIRE(total * bs < ((int64_t)1 << 32), NGPB_E_ARG, "grouped launch exceeds 2^32 work-items");
+    ranges.emplace_back(st);
+    ranges.back().upload(h.data(), h.size(), st);
+    hipLaunchKernelGGL(kernel, dim3((unsigned)total), dim3(bs), lds, st, T, ranges.back().p, (int)nb.size());
+    NGPB_HIP(hipGetLastError());
+  }
+};
+
+template <typename F>
+std::vector<int64_t> per_system(const std::vector<AsmSys>& T, F f) {
+  std::vector<int64_t> v(T.size());
+  for (size_t i = 0; i < T.size(); ++i) v[i] = f(T[i]);
+  return v;
+}
+
+// run_symbolic of every context, stage by stage in one launch each
+void symbolic_many(hipStream_t st, const std::vector<ngpb_ctx*>& cs) {
+  GroupRun g(st);
+  const int ns = (int)cs.size();
+  std::vector<AsmSys> T((size_t)ns);
+  int64_t sumN = 0;
+  for (int i = 0; i < ns; ++i) {
+    const ngpb_ctx* c = cs[i];
+    T[i].m = mesh_view_of(c);
+    NGPB_REQUIRE(20 * T[i].m.L < (int64_t)UINT32_MAX, NGPB_E_ARG, "too many leaves for 32-bit emission ids");
+    T[i].N = c->N;
+    T[i].L = c->L;
+    T[i].is_bnd = c->is_bnd.p;
+    sumN += c->N;
+  }
+  DBuf<unsigned> cnt(st);
+  DBuf<int64_t> emrp(st), cnt4(st);
+  DBuf<int32_t> big(st);
+  DBuf<int> nbig(st);
+  cnt.resize((size_t)sumN);
+  emrp.resize((size_t)(sumN + ns));
+  cnt4.resize((size_t)(4 * sumN));
+  big.resize((size_t)sumN);
+  nbig.resize((size_t)ns);
+  for (int64_t i = 0, o = 0; i < ns; o += T[i].N, ++i) {
+    T[i].cnt = cnt.p + o;
+    T[i].em_rowptr = emrp.p + o + i;
+    T[i].cnt4 = cnt4.p + 4 * o;
+    T[i].big = big.p + o;
+    T[i].nbig = nbig.p + i;
+  }
+  NGPB_HIP(hipMemsetAsync(cnt.p, 0, (size_t)sumN * sizeof(unsigned), st));
+  NGPB_HIP(hipMemsetAsync(nbig.p, 0, (size_t)ns * sizeof(int), st));
+  const AsmSys* d = g.table(T);
+  const auto nb_emit = per_system(T, [](const AsmSys& a) { return (int64_t)blocks_for(8 * a.L, 128); });
+  g.launch(kg_emit_rows<0>, nb_emit, 128, 0, d);
+  g.launch(kg_u32_to_i64, per_system(T, [](const AsmSys& a) { return (int64_t)blocks_for(a.N, 256); }), 256, 0, d);
+  std::vector<ScanSeg> segs((size_t)ns);
+  for (int i = 0; i < ns; ++i) segs[i] = {T[i].em_rowptr, T[i].em_rowptr, T[i].N};
+  std::vector<int64_t> n_em((size_t)ns);
+  exclusive_scan_many(st, segs, n_em.data());
+  int64_t sumE = 0;
+  for (int i = 0; i < ns; ++i) sumE += n_em[i];
+  DBuf<uint64_t> key(st);
+  DBuf<uint16_t> kw(st);
+  key.resize((size_t)std::max<int64_t>(sumE, 1));
+  kw.resize((size_t)std::max<int64_t>(sumE, 1));
+  for (int64_t i = 0, o = 0; i < ns; o += n_em[i], ++i) {
+    T[i].em_key = key.p + o;
+    T[i].em_kw = kw.p + o;
+    T[i].n_em = n_em[i];
+  }
+  d = g.table(T);
+  hipLaunchKernelGGL(kg_em_end, dim3(blocks_for(ns, 64)), dim3(64), 0, st, d, ns);
+  NGPB_HIP(hipMemsetAsync(cnt.p, 0, (size_t)sumN * sizeof(unsigned), st));
+  g.launch(kg_emit_rows<1>, nb_emit, 128, 0, d);
+  g.launch(kg_row_sort_wave, per_system(T, [](const AsmSys& a) { return (int64_t)blocks_for(a.N, kSortBS / 64); }),
+           kSortBS, 0, d);
+  std::vector<int> hbig((size_t)ns);
+  NGPB_HIP(hipMemcpyAsync(hbig.data(), nbig.p, (size_t)ns * sizeof(int), hipMemcpyDeviceToHost, st));
+  NGPB_HIP(hipStreamSynchronize(st));
+  std::vector<int64_t> nb_big((size_t)ns);
+  for (int i = 0; i < ns; ++i) nb_big[i] = hbig[i];
+  g.launch(kg_row_sort_block, nb_big, kSortBS, (size_t)kSortBigCap * (8 + 2), d);
+  std::vector<ScanSeg> segs4((size_t)4 * ns);
+  for (int i = 0; i < ns; ++i)
+    for (int q = 0; q < 4; ++q) segs4[4 * i + q] = {T[i].cnt4 + q * T[i].N, T[i].cnt4 + q * T[i].N, T[i].N};
+  std::vector<int64_t> tot((size_t)4 * ns);
+  exclusive_scan_many(st, segs4, tot.data());
+  int64_t sumS = 0;
+  for (int i = 0; i < ns; ++i) {
+    ngpb_ctx* c = cs[i];
+    const int64_t N = c->N, nnz = tot[4 * i], nbc = tot[4 * i + 1], nse = tot[4 * i + 2], nce = tot[4 * i + 3];
+    NGPB_REQUIRE(nnz < (int64_t)INT32_MAX && nbc < (int64_t)INT32_MAX, NGPB_E_ARG, "nnz exceeds 32-bit CSR");
+    c->n = N;
+    c->nnz = nnz;
+    c->rowptr.resize((size_t)N + 1);
+    c->col.resize((size_t)std::max<int64_t>(nnz, 1));
+    c->diag_slot.resize((size_t)N);
+    c->bc_rowptr.resize((size_t)N + 1);
+    c->bc_col.resize((size_t)std::max<int64_t>(nbc, 1));
+    c->bc_ptr.resize((size_t)nbc + 1);
+    c->em_id.resize((size_t)std::max<int64_t>(nse, 1));
+    c->em_code.resize((size_t)std::max<int64_t>(nse, 1));
+    c->bc_id.resize((size_t)std::max<int64_t>(nce, 1));
+    c->bc_code.resize((size_t)std::max<int64_t>(nce, 1));
+    c->blk_em.resize((size_t)(N + kSvRows - 1) / kSvRows + 1);
+    c->slot_rel.resize((size_t)std::max<int64_t>(nnz, 1));
+    AsmSys& a = T[i];
+    a.n_slots = nnz;
+    a.n_coup = nbc;
+    a.n_slot_em = nse;
+    a.n_coup_em = nce;
+    a.rowptr = c->rowptr.p;
+    a.col = c->col.p;
+    a.diag_slot = c->diag_slot.p;
+    a.bc_rowptr = c->bc_rowptr.p;
+    a.bc_col = c->bc_col.p;
+    a.bc_ptr = c->bc_ptr.p;
+    a.em_id = c->em_id.p;
+    a.em_code = c->em_code.p;
+    a.bc_id = c->bc_id.p;
+    a.bc_code = c->bc_code.p;
+    a.blk_em = c->blk_em.p;
+    a.slot_rel = c->slot_rel.p;
+    sumS += nnz + 1;
+  }
+  DBuf<int64_t> sptr(st);
+  sptr.resize((size_t)sumS);
+  for (int64_t i = 0, o = 0; i < ns; o += T[i].n_slots + 1, ++i) T[i].sptr = sptr.p + o;
+  d = g.table(T);
+  g.launch(kg_row_slots, per_system(T, [](const AsmSys& a) { return (int64_t)blocks_for(a.N, kSlotBS / 64); }),
+           kSlotBS, 0, d);
+  g.launch(kg_blk_em,
+           per_system(T, [](const AsmSys& a) { return (int64_t)blocks_for((a.N + kSvRows - 1) / kSvRows + 1, 256); }),
+           256, 0, d);
+  g.launch(kg_slot_rel, per_system(T, [](const AsmSys& a) { return (int64_t)blocks_for(a.N, 256); }), 256, 0, d);
+  NGPB_HIP(hipStreamSynchronize(st));
+  for (ngpb_ctx* c : cs) c->have_symbolic = true;
+}
+
+// run_numeric of every context, stage by stage in one launch each
+void numeric_many(hipStream_t st, const std::vector<ngpb_ctx*>& cs) {
+  GroupRun g(st);
+  const int ns = (int)cs.size();
+  std::vector<AsmSys> T((size_t)ns);
+  int64_t sumC = 0, sumH = 0;
+  for (int i = 0; i < ns; ++i) {
+    ngpb_ctx* c = cs[i];
+    const int64_t N = c->N, L = c->L;
+    c->c_edge.resize((size_t)12 * L);
+    c->v8.resize((size_t)L);
+    c->val.resize((size_t)std::max<int64_t>(c->nnz, 1));
+    c->rho.resize((size_t)N);
+    c->rhs_charge.resize((size_t)N);
+    c->rhs.resize((size_t)N);
+    AsmSys& a = T[i];
+    a.m = mesh_view_of(c);
+    a.N = N;
+    a.L = L;
+    a.A = c->have_charges ? c->A : 0;
+    a.is_bnd = c->is_bnd.p;
+    a.rowptr = c->rowptr.p;
+    a.diag_slot = c->diag_slot.p;
+    a.bc_rowptr = c->bc_rowptr.p;
+    a.bc_col = c->bc_col.p;
+    a.bc_ptr = c->bc_ptr.p;
+    a.blk_em = c->blk_em.p;
+    a.slot_rel = c->slot_rel.p;
+    a.em_id = c->em_id.p;
+    a.em_code = c->em_code.p;
+    a.bc_id = c->bc_id.p;
+    a.bc_code = c->bc_code.p;
+    a.eps = c->eps.p;
+    a.border_of = c->border_of.p;
+    a.frac = c->frac.p;
+    a.reac = c->reac.p;
+    a.gval = c->gval.p;
+    a.axyz = c->atom_xyz.p;
+    a.aq = c->atom_q.p;
+    a.owner = c->atom_owner.p;
+    a.c_edge = c->c_edge.p;
+    a.v8 = c->v8.p;
+    a.val = c->val.p;
+    a.rho = c->rho.p;
+    a.rhs_charge = c->rhs_charge.p;
+    a.rhs = c->rhs.p;
+    if (a.A > 0) {
+      sumC += 32 * a.A;
+      sumH += N;
+    }
+  }
+  // charge lists of the systems with atoms: contributions 32 A, list heads N
+  DBuf<int32_t> cnode(st), cnext(st), head(st);
+  if (sumC > 0) {
+    cnode.resize((size_t)sumC);
+    cnext.resize((size_t)sumC);
+    head.resize((size_t)sumH);
+    NGPB_HIP(hipMemsetAsync(head.p, 0xff, (size_t)sumH * sizeof(int32_t), st));
+    for (int64_t i = 0, oc = 0, oh = 0; i < ns; ++i) {
+      if (T[i].A == 0) continue;
+      T[i].cnode = cnode.p + oc;
+      T[i].cnext = cnext.p + oc;
+      T[i].head = head.p + oh;
+      oc += 32 * T[i].A;
+      oh += T[i].N;
+    }
+  }
+  const AsmSys* d = g.table(T);
+  g.launch(kg_leaf_coeffs, per_system(T, [](const AsmSys& a) { return (int64_t)blocks_for(a.L, 128); }), 128, 0, d);
+  g.launch(kg_slot_values, per_system(T, [](const AsmSys& a) { return (int64_t)blocks_for(a.N, kSvRows); }), kSvBS,
+           0, d);
+  g.launch(kg_zero_charge, per_system(T, [](const AsmSys& a) { return (int64_t)blocks_for(a.N, 256); }), 256, 0, d);
+  const auto nb_charge = per_system(T, [](const AsmSys& a) { return (int64_t)(a.A > 0 ? blocks_for(32 * a.A, 256) : 0); });
+  g.launch(kg_rho_push, nb_charge, 256, 0, d);
+  g.launch(kg_rho_node, nb_charge, 256, 0, d);
+  g.launch(kg_dirichlet_rhs, per_system(T, [](const AsmSys& a) { return (int64_t)blocks_for(a.N, 128); }), 128, 0,
+           d);
+  NGPB_HIP(hipStreamSynchronize(st));
+  for (ngpb_ctx* c : cs) {
+    c->have_system = true;
+    c->have_compact = false;
+  }
+}
+
+// systems per grouped launch: at most this many leaves (the symbolic temporaries take
+// about 0.6 KB per leaf) so that the temporaries and the grids stay bounded
+constexpr int64_t kGroupLeaves = (int64_t)16 << 20;
+
+template <typename F>
+void in_chunks(const std::vector<ngpb_ctx*>& cs, F f) {
+  size_t i = 0;
+  while (i < cs.size()) {
+    size_t j = i;
+    int64_t L = 0;
+    while (j < cs.size() && (j == i || L + cs[j]->L <= kGroupLeaves)) L += cs[j++]->L;
+    f(std::vector<ngpb_ctx*>(cs.begin() + i, cs.begin() + j));
+    i = j;
+  }
+}
+
+}  // namespace
+
+void run_assemble_many(const std::vector<ngpb_ctx*>& cs) {
+  if (cs.empty()) return;
+  hipStream_t st = cs[0]->stream;
+  for (ngpb_ctx* c : cs) NGPB_HIP(hipStreamSynchronize(c->stream));
+  std::vector<ngpb_ctx*> sym;
+  for (ngpb_ctx* c : cs)
+    if (!c->have_symbolic || c->external_csr) sym.push_back(c);
+  in_chunks(sym, [&](const std::vector<ngpb_ctx*>& part) { symbolic_many(st, part); });
+  in_chunks(cs, [&](const std::vector<ngpb_ctx*>& part) { numeric_many(st, part); });
 }
 
 }  // namespace ngpb

@@ -325,6 +325,27 @@ namespace ngpb {
 // ----------------------------------------------------------------- kernels (host wrappers)
 // scan
 int64_t exclusive_scan(ngpb_ctx* c, const int64_t* d_in, int64_t* d_out, int64_t n);
+// Grouped launches run one kernel body over many independent ranges (systems,
+// segments): range i owns the blocks [blk0[i], blk0[i + 1]) of the launch and sees
+// them as its blocks 0, 1, ... . group_of returns the range of block b (empty ranges
+// are skipped: the last i with blk0[i] <= b).
+__device__ __forceinline__ int group_of(const int64_t* __restrict__ blk0, int n, int64_t b) {
+  int lo = 0, hi = n - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (blk0[mid] <= b) lo = mid;
+    else hi = mid - 1;
+  }
+  return lo;
+}
+struct ScanSeg {
+  const int64_t* in;
+  int64_t* out;  // may alias in
+  int64_t n;
+};
+// exclusive scans of many segments in three launches; totals (host, one per segment)
+// receive the segment sums. Synchronizes the stream.
+void exclusive_scan_many(hipStream_t st, const std::vector<ScanSeg>& segs, int64_t* totals);
 // compaction: indices i in [0, n) with flag[i] != 0, in order; returns count
 int64_t compact_flags(ngpb_ctx* c, const uint8_t* d_flag, int64_t n, int32_t* d_out);
 
@@ -336,6 +357,10 @@ void run_fractions(ngpb_ctx* c);
 // assembly
 void run_symbolic(ngpb_ctx* c);
 void run_numeric(ngpb_ctx* c);
+// run_symbolic (where a context needs it) and run_numeric of many contexts on one
+// device, each stage in one launch over all of them, on the first context's stream;
+// every context's results equal its own run_symbolic / run_numeric bit for bit
+void run_assemble_many(const std::vector<ngpb_ctx*>& cs);
 void run_dirichlet_values(ngpb_ctx* c, int bc_type);
 void run_phi_on_atoms(ngpb_ctx* c, double* d_out);
 void run_cast_rays(ngpb_ctx* c, int64_t na, const double* xyz, const double* r, double probe);

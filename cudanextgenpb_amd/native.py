@@ -95,6 +95,7 @@ SIGNATURES = {
     "ngpb_set_dirichlet": (_I, [_P, _I64, _P, _P]),
     "ngpb_dirichlet_values": (_I, [_P, _I]),
     "ngpb_assemble": (_I, [_P]),
+    "ngpb_assemble_many": (_I, [_P, _I]),
     "ngpb_system_size": (_I, [_P, _P, _P]),
     "ngpb_export_csr": (_I, [_P, _P, _P, _P, _P]),
     "ngpb_set_csr": (_I, [_P, _I64, _I64, _P, _P, _P, _P]),
@@ -183,6 +184,22 @@ def parse_lis_options(s: str) -> SolveOpts:
     if rc != NGPB_OK:
         raise NGPBError(rc, f"cannot parse solver options {s!r}")
     return o
+
+
+def assemble_many(ctxs) -> None:
+    """ngpb_assemble of several contexts (one device) with each assembly stage run as
+    one launch over all of them; each system is bit-identical to its own assemble()."""
+    ctxs = list(ctxs)
+    if not ctxs:
+        return
+    arr = (ctypes.c_void_p * len(ctxs))(*[c._h.value for c in ctxs])
+    rc = lib().ngpb_assemble_many(ctypes.cast(arr, _P), len(ctxs))
+    if rc != NGPB_OK:
+        bad = [c for c in ctxs if lib().ngpb_last_error(c._h)]
+        msg = lib().ngpb_last_error(bad[0]._h).decode() if bad else ""
+        raise NGPBError(rc, f"assemble_many: {msg}")
+    for c in ctxs:
+        c._size()
 
 
 class Context:

@@ -251,6 +251,15 @@ int ngpb_dirichlet_values(ngpb_ctx* ctx, int bc_type);
 /* Edge fractions, Laplacian with harmonic-mean face coefficients, kappa^2 mass,
    hanging-node constraints, trilinear charge projection and Dirichlet elimination. */
 int ngpb_assemble(ngpb_ctx* ctx);
+/* ngpb_assemble of n contexts on one device (a batch's systems): each stage of the
+   symbolic and numeric assembly runs as one launch over a table of the systems, on
+   ctx[0]'s stream, instead of one launch per system. Every context's system is
+   bit-identical to its own ngpb_assemble. No counterpart in the reference, which
+   assembles one system per run (create_density_map + assemple_system_matrix,
+   src/pb_class.cpp:1943-2159). On an error in a context's own preparation that
+   context holds the message; on an error in the shared stages every context does, and
+   none holds an assembled system. */
+int ngpb_assemble_many(ngpb_ctx* const* ctx, int n);
 int ngpb_system_size(ngpb_ctx* ctx, int64_t* n, int64_t* nnz);
 int ngpb_export_csr(ngpb_ctx* ctx, int32_t* rowptr, int32_t* col, double* val, double* rhs);
 /* Load an external CSR system (0-based, columns ascending, n+1 row pointers).

@@ -101,4 +101,10 @@ def test_null_arguments_rejected():
     assert L.ngpb_get_mesh(None, None, None, None, None, None, None, None) == 1
     assert L.ngpb_cast_rays(None, 0, None, None) == 1
     assert L.ngpb_locate_atoms(None, 0, None, 0.0, None) == 1
+    # many contexts: a null list or a null entry is rejected, an empty list is a no-op
+    assert L.ngpb_assemble_many(None, 1) == 1
+    assert L.ngpb_assemble_many(None, -1) == 1
+    assert L.ngpb_assemble_many(None, 0) == 0
+    nulls = (ctypes.c_void_p * 2)(None, None)
+    assert L.ngpb_assemble_many(ctypes.cast(nulls, ctypes.c_void_p), 2) == 1
     del out
