@@ -83,18 +83,25 @@ def prepare(problem: Problem, device: int = 0, ctx: Optional[native.Context] = N
     return Prepared(problem.name, c, c.n, c.nnz, problem.atoms.n, (t2 - t0) * 1e3, {"assemble_ms": (t2 - t1) * 1e3})
 
 
-def prepare_many(problems, device: int = 0, device_mesh: bool = False, probe: float = 0.0) -> List[Prepared]:
+def prepare_many(problems, device: int = 0, device_mesh: bool = False, probe: float = 0.0,
+                 threads: int = 8) -> List[Prepared]:
     """prepare() of many problems, with the assembly of all of them in one launch per
     stage over a table of the systems (ngpb_assemble_many): each system is
     bit-identical to its own prepare(). The per-problem inputs (mesh, rays, markers,
-    charges, boundary values) are still set up one problem at a time."""
+    charges, boundary values) are set up per problem, `threads` problems at a time,
+    each on its own context stream."""
     t0 = time.perf_counter()
     problems = list(problems)
     ctxs: List[native.Context] = []
     try:
-        for P in problems:
+        for _ in problems:
             ctxs.append(native.Context(device))
-            _inputs(P, ctxs[-1], device_mesh, probe)
+        if threads > 1 and len(problems) > 1:
+            with ThreadPoolExecutor(max_workers=threads) as ex:
+                list(ex.map(lambda pc: _inputs(pc[0], pc[1], device_mesh, probe), zip(problems, ctxs)))
+        else:
+            for P, c in zip(problems, ctxs):
+                _inputs(P, c, device_mesh, probe)
         t1 = time.perf_counter()
         native.assemble_many(ctxs)
     except BaseException:
