@@ -508,7 +508,14 @@ double sqd(const double* a, const double* b) {
 // circle's (u, w) frame; the covered open arcs are swept in angle for a gap. Buried
 // circles carry no SES patch and are dropped (an exposed SAS vertex lies on three
 // exposed circles). Host arithmetic, with the C library's atan2 and acos.
-bool circle_exposed(const double* c, const double* n, double rho, int i, int j, const std::vector<int>& over_i,
+//
+// A circle is buried iff one ball covers all of it (S >= A) or the union of the arcs
+// does, and that union only grows with more balls. So the balls are taken in the order
+// given (deepest over ball i's centre first), and the arcs gathered so far are swept
+// after 16, 64, 256 ... balls: in dense packings most circles are covered early, and
+// only circles that stay uncovered are swept with every arc. The decision is the one a
+// single sweep over every ball makes.
+bool circle_exposed(const double* c, const double* n, double rho, int j, const std::vector<int>& balls,
                     const double* xyz, const std::vector<double>& R) {
   const int ax = std::fabs(n[0]) < 0.9 ? 0 : 1;
   double u[3] = {0, 0, 0};
@@ -519,8 +526,21 @@ bool circle_exposed(const double* c, const double* n, double rho, int i, int j, 
   for (int k = 0; k < 3; ++k) u[k] = u[k] / ul;
   const double w[3] = {n[1] * u[2] - n[2] * u[1], n[2] * u[0] - n[0] * u[2], n[0] * u[1] - n[1] * u[0]};
   const double two_pi = 2.0 * 3.14159265358979323846;
-  std::vector<std::pair<double, double>> iv;
-  for (int m : over_i) {  // ascending, every ball overlapping ball i
+  thread_local std::vector<std::pair<double, double>> iv;
+  iv.clear();
+  // whether the arcs in iv leave no gap
+  auto covered = [&]() {
+    if (iv.empty()) return false;
+    std::sort(iv.begin(), iv.end());
+    double cur = 0.0;
+    for (const auto& a : iv) {
+      if (a.first > cur) return false;
+      cur = std::max(cur, a.second);
+    }
+    return !(cur < two_pi);
+  };
+  size_t seen = 0, next = 16;
+  for (int m : balls) {  // every ball overlapping ball i
     if (m == j) continue;
     const double* am = xyz + 3 * m;
     const double d[3] = {c[0] - am[0], c[1] - am[1], c[2] - am[2]};
@@ -529,6 +549,10 @@ bool circle_exposed(const double* c, const double* n, double rho, int i, int j, 
     const double S = R[(size_t)m] * R[(size_t)m] - (d[0] * d[0] + d[1] * d[1] + d[2] * d[2]) - rho * rho;
     const double A = std::sqrt(P * P + Q * Q);
     if (S >= A) return false;
+    if (++seen == next) {
+      if (covered()) return false;
+      next *= 4;
+    }
     if (S <= -A) continue;
     const double beta = std::acos(S / A);
     double s0 = std::atan2(Q, P) + beta;
@@ -542,14 +566,7 @@ bool circle_exposed(const double* c, const double* n, double rho, int i, int j, 
       iv.push_back({s0, e0});
     }
   }
-  if (iv.empty()) return true;
-  std::sort(iv.begin(), iv.end());
-  double cur = 0.0;
-  for (const auto& a : iv) {
-    if (a.first > cur) return true;
-    cur = std::max(cur, a.second);
-  }
-  return cur < two_pi;
+  return !covered();
 }
 
 // SAS circles of overlapping pairs (i < j, in index order) that are not buried, and
@@ -610,6 +627,18 @@ SesHost ses_build_host(int64_t na, const double* xyz, const double* r, double rp
         }
     std::sort(over[(size_t)i].begin(), over[(size_t)i].end());
   });
+  // the same balls per atom, deepest over its centre first (|a_m - a_i| - R_m
+  // ascending): the order in which circle_exposed tries them
+  std::vector<std::vector<int>> deep((size_t)na);
+  host_parallel(na, [&](int64_t i) {
+    std::vector<std::pair<double, int>> k;
+    k.reserve(over[(size_t)i].size());
+    for (int m : over[(size_t)i]) k.push_back({std::sqrt(sqd(xyz + 3 * i, xyz + 3 * m)) - R[(size_t)m], m});
+    std::sort(k.begin(), k.end());
+    auto& v = deep[(size_t)i];
+    v.reserve(k.size());
+    for (const auto& e : k) v.push_back(e.second);
+  });
   std::vector<std::vector<SesPairD>> pairs_of((size_t)na);
   host_parallel(na, [&](int64_t i) {
     for (int j : over[(size_t)i]) {
@@ -629,7 +658,7 @@ SesHost ses_build_host(int64_t na, const double* xyz, const double* r, double rp
         P.c[k] = ai[k] + t * P.n[k];
       }
       P.rho = std::sqrt(rho2);
-      if (!circle_exposed(P.c, P.n, P.rho, (int)i, j, over[(size_t)i], xyz, R)) continue;
+      if (!circle_exposed(P.c, P.n, P.rho, j, deep[(size_t)i], xyz, R)) continue;
       pairs_of[(size_t)i].push_back(P);
     }
   });
