@@ -38,16 +38,16 @@ def pct(a, q):
     return float(np.percentile(np.asarray(a), q))
 
 
-def measured_traffic(bytes_per_launch):
-    """HBM bytes per compact-SpMV launch from the committed rocprofv3 FETCH_SIZE /
-    WRITE_SIZE summary (copy-kernel calibrated), if it was measured on an operator of
-    exactly this many algorithmic bytes; otherwise None."""
-    path = os.path.join(REPO, "profiles", "compact_spmv_globular_8M_traffic.txt")
+def measured_traffic(bytes_per_launch, name="compact_spmv_globular_8M_traffic.txt", key="compact_spmv_bytes_per_launch"):
+    """HBM bytes per SpMV launch from a committed rocprofv3 FETCH_SIZE / WRITE_SIZE
+    summary (copy-kernel calibrated), if it was measured on an operator of exactly this
+    many algorithmic bytes; otherwise None."""
+    path = os.path.join(REPO, "profiles", name)
     try:
         for line in open(path):
-            if line.startswith("compact_spmv_bytes_per_launch="):
+            if line.startswith(key + "="):
                 kv = dict(t.split("=") for t in line.split())
-                if int(kv["compact_spmv_bytes_per_launch"]) == int(bytes_per_launch):
+                if int(kv[key]) == int(bytes_per_launch):
                     return float(kv["traffic_bytes_per_launch"])
     except OSError:
         pass
@@ -194,7 +194,8 @@ def batch_leg(n_systems, tol):
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": gbs / HBM_PEAK_GBS,
-            "traffic": None,
+            "traffic": measured_traffic(spmv_bytes, "batch_spmv_128_traffic.txt", "batch_spmv_bytes_per_launch"),
+            "traffic_source": "profiles/batch_spmv_128_traffic.txt (rocprofv3 FETCH_SIZE x2.000 + WRITE_SIZE x1.000)",
             "bytes_per_launch": spmv_bytes,
             "avg_launch_ms": tune["ms_per_spmv"],
             "timing": "autotuner: HIP events around 5 launches of the chosen tiling over every system",
