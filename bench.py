@@ -107,10 +107,12 @@ def cpu_baseline(pb, fp64_iters, seconds):
     }
 
 
-def batch_leg(n_systems, tol):
+def batch_leg(n_systems, tol, prep_threads=8, tiling=None):
     """The batched ligand systems (config 4's generator): one batch (mixed precision,
     hipGraph replays, autotuned tiles) against one-at-a-time FP64 solves of the same
-    systems; the batched SpMV's roofline from the autotuner's timing."""
+    systems; the batched SpMV's roofline from the autotuner's timing. tiling = (tile
+    slices, waves per block) skips the autotuner (no roofline then): the form the
+    counter profiles use, with prep_threads = 1."""
     from cudanextgenpb_amd import native, workload as W
     from cudanextgenpb_amd.batch import BatchedSolver, prepare, prepare_many
 
@@ -131,7 +133,7 @@ def batch_leg(n_systems, tol):
     for p in preps:
         p.ctx.close()
     t = time.perf_counter()
-    preps = prepare_many(specs)
+    preps = prepare_many(specs, threads=prep_threads)
     grouped_s = time.perf_counter() - t
     asm_many_s = sum(p.meta["assemble_ms"] for p in preps) * 1e-3
     log(f"prepared {n_systems} systems with one grouped assembly in {grouped_s:.2f} s (assembly {asm_many_s:.3f} s; "
@@ -152,7 +154,11 @@ def batch_leg(n_systems, tol):
     bs = BatchedSolver(0, max_systems=max(n_systems, 1), tol=tol)
     for p in preps:
         bs.add(p)
-    tune = bs.autotune()
+    if tiling:
+        bs.batch.set_tiling(*tiling)
+        tune = {"tile_slices": tiling[0], "waves_per_block": tiling[1], "ms_per_spmv": None}
+    else:
+        tune = bs.autotune()
     log(f"autotune: {tune}")
     t = time.perf_counter()
     res = bs.solve(energies=True)
@@ -170,7 +176,7 @@ def batch_leg(n_systems, tol):
     bs.close()
     for p in preps:
         p.ctx.close()
-    gbs = spmv_bytes / (tune["ms_per_spmv"] * 1e-3) / 1e9
+    gbs = spmv_bytes / (tune["ms_per_spmv"] * 1e-3) / 1e9 if tune["ms_per_spmv"] else None
     return {
         "systems": n_systems,
         "unknowns_total": tot_n,
@@ -193,7 +199,7 @@ def batch_leg(n_systems, tol):
             "achieved": gbs,
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
-            "frac": gbs / HBM_PEAK_GBS,
+            "frac": gbs / HBM_PEAK_GBS if gbs else None,
             "traffic": measured_traffic(spmv_bytes, "batch_spmv_128_traffic.txt", "batch_spmv_bytes_per_launch"),
             "traffic_source": "profiles/batch_spmv_128_traffic.txt (rocprofv3 FETCH_SIZE x2.000 + WRITE_SIZE x1.000)",
             "bytes_per_launch": spmv_bytes,
@@ -221,7 +227,8 @@ def batch_leg(n_systems, tol):
 def bench_batch(args):
     """Batched ligand systems: one batch (mixed precision, hipGraph replays, autotuned
     tiles) against one-at-a-time FP64 solves of the same systems."""
-    L = batch_leg(args.systems, args.tol)
+    tiling = tuple(int(v) for v in args.tiling.split("x")) if args.tiling else None
+    L = batch_leg(args.systems, args.tol, args.prep_threads, tiling)
     out = {
         "metric": "solves/s & DOF-iter/s at 1e-6 rel energy err, mixed vs FP64 GPU; p99 latency",
         "value": L["solves_per_s"],
@@ -549,6 +556,8 @@ def main():
     ap.add_argument("--requests", type=int, default=48, help="serve workload: number of requests")
     ap.add_argument("--rate", type=float, default=8.0, help="serve workload: Poisson arrival rate [1/s]")
     ap.add_argument("--systems", type=int, default=512, help="batch workload: number of ligand systems")
+    ap.add_argument("--prep-threads", type=int, default=8, help="batch workload: threads preparing the systems")
+    ap.add_argument("--tiling", default=None, help="batch workload: fixed tiling SLICESxWAVES (skips the autotuner)")
     ap.add_argument("--batch-systems", type=int, default=128,
                     help="default run: ligand systems of the batched leg (0: skip it)")
     ap.add_argument("--device-mesh", action="store_true",
