@@ -77,8 +77,12 @@ struct Tile {
 // ---------------------------------------------------------------- row kernels
 // One block per tile; wave w of the block takes slices first + w, first + w + WPB, ...
 
+// At most 72 VGPRs (7 waves per SIMD; the compiler's own choice was 101 and 4 waves):
+// 128 ligands 0.565 -> 0.421 ms per SpMV over every system (5 waves: 0.492, 6: 0.435,
+// 8: 0.448 ms with 24 spilled VGPRs). The few values spilled at 7 waves belong to the
+// widest slice forms.
 template <int WPB>
-__global__ void __launch_bounds__(WPB * 64) kb_spmv(const BatchDesc* __restrict__ D, const BatchScal* __restrict__ S,
+__global__ void __launch_bounds__(WPB * 64) __attribute__((amdgpu_waves_per_eu(7))) kb_spmv(const BatchDesc* __restrict__ D, const BatchScal* __restrict__ S,
                                                     const Tile* __restrict__ tiles, int tile_slices, int force) {
   const Tile tl = tiles[blockIdx.x];
   const int slot = tl.slot;
