@@ -125,11 +125,16 @@ class Server:
         self.max_batch = max_batch
         self.pool = ThreadPoolExecutor(max_workers=stream_workers)
         self.prep_pool = ThreadPoolExecutor(max_workers=prep_workers)
+        # finished batched contexts are released off the serving loop: freeing device
+        # memory waits for the work queued on every stream of the device, which held the
+        # loop (replays, admissions) for up to ~165 ms per context
+        self.close_pool = ThreadPoolExecutor(max_workers=1)
         self.peak_bytes = 0.0
 
     def close(self):
         self.pool.shutdown(wait=True)
         self.prep_pool.shutdown(wait=True)
+        self.close_pool.shutdown(wait=True)
         self.batch.close()
 
     def _stream_solve(self, req: Request):
@@ -233,7 +238,7 @@ class Server:
                         results[req.rid] = self._failed(req, "batch", adm, clock(), exc)
                         continue
                     finally:
-                        ctx.close()
+                        self.close_pool.submit(ctx.close)
                     results[req.rid] = Result(req.rid, "batch", req.arrival, adm, t, (t - req.arrival) * 1e3,
                                               st["iterations"], st["status"], e, nbytes, phi)
             for item in list(streaming):
