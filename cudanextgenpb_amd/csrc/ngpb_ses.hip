@@ -32,6 +32,7 @@
 #include <algorithm>
 #include <array>
 #include <cmath>
+#include <cstdlib>
 #include <thread>
 #include <vector>
 
@@ -544,6 +545,11 @@ bool circle_exposed(const double* c, const double* n, double rho, int j, const s
     if (m == j) continue;
     const double* am = xyz + 3 * m;
     const double d[3] = {c[0] - am[0], c[1] - am[1], c[2] - am[2]};
+    // a ball farther than R_m (1 + 1e-9) from every point of the circle covers no arc:
+    // S + A (= R_m^2 - its squared distance to the circle) is then below -2e-9 R_m^2,
+    // far outside rounding, so the sweep would skip it too
+    const double far = R[(size_t)m] * (1.0 + 1e-9) + rho;
+    if (d[0] * d[0] + d[1] * d[1] + d[2] * d[2] > far * far) continue;
     const double P = 2.0 * rho * (d[0] * u[0] + d[1] * u[1] + d[2] * u[2]);
     const double Q = 2.0 * rho * (d[0] * w[0] + d[1] * w[1] + d[2] * w[2]);
     const double S = R[(size_t)m] * R[(size_t)m] - (d[0] * d[0] + d[1] * d[1] + d[2] * d[2]) - rho * rho;
@@ -575,7 +581,14 @@ bool circle_exposed(const double* c, const double* n, double rho, int j, const s
 // f(k) for k in [0, n) on the host's hardware threads (each k handled once)
 template <class F>
 void host_parallel(int64_t n, F&& f) {
-  const int64_t T = std::max<int64_t>(1, std::min<int64_t>((int64_t)std::thread::hardware_concurrency(), 64));
+  // the hardware threads, at most OMP_NUM_THREADS when that is set (the CPU share a
+  // process is given) and at most 64
+  int64_t T = std::min<int64_t>((int64_t)std::thread::hardware_concurrency(), 64);
+  if (const char* e = std::getenv("OMP_NUM_THREADS")) {
+    const long v = std::strtol(e, nullptr, 10);
+    if (v > 0) T = std::min<int64_t>(T, v);
+  }
+  T = std::max<int64_t>(T, 1);
   if (n < 256 || T == 1) {
     for (int64_t k = 0; k < n; ++k) f(k);
     return;
