@@ -125,9 +125,10 @@ class Server:
         self.max_batch = max_batch
         self.pool = ThreadPoolExecutor(max_workers=stream_workers)
         self.prep_pool = ThreadPoolExecutor(max_workers=prep_workers)
-        # finished batched contexts are released off the serving loop: freeing device
-        # memory waits for the work queued on every stream of the device, which held the
-        # loop (replays, admissions) for up to ~165 ms per context
+        # finished contexts are released on their own thread: freeing device memory
+        # waits for the work queued on every stream of the device, which held the
+        # serving loop (replays, admissions) for up to ~165 ms per context and delayed
+        # streamed results by ~16 ms
         self.close_pool = ThreadPoolExecutor(max_workers=1)
         self.peak_bytes = 0.0
 
@@ -140,7 +141,8 @@ class Server:
     def _stream_solve(self, req: Request):
         """Large request: prepared and solved on its own context (worker thread). A
         solve that stops at maxiter or stagnates is reported through its status, as a
-        batched one is; the context is closed whatever happens."""
+        batched one is; the context is released whatever happens, after the result is
+        in hand (on the releasing thread, so the result does not wait for it)."""
         p = prepare(req.problem, self.device, device_mesh=self.device_mesh, probe=self.probe)
         try:
             _, st = p.ctx.solve(native.PREC_MIXED, tol=self.tol, want_phi=False, raise_on_noconv=False)
@@ -148,7 +150,7 @@ class Server:
             phi = p.ctx.get_phi() if self.keep_phi else None
             nbytes = p.ctx.device_bytes()
         finally:
-            p.ctx.close()
+            self.close_pool.submit(p.ctx.close)
         return st, e, phi, nbytes
 
     @staticmethod
