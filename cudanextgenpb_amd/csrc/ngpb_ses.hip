@@ -19,10 +19,12 @@
 // are normalised by + 0.0 (no negative zero, so equal values have equal keys). Normals
 // point out of the molecule: (p - a)/r on atoms, (q - p)/rp towards the probe centre q
 // on tori and probe spheres. Only IEEE double + - * / sqrt are used, so the results
-// equal the CPU restatement's bit for bit.
+// equal the CPU restatement's bit for bit (the circles' exposure aside, see below).
 //
 // Work split: the SAS pairs (circles) and exposed SAS vertices are built on the host
-// from a cell grid of the atoms (ordered by atom indices; buried circles dropped);
+// from a cell grid of the atoms (ordered by atom indices; buried circles dropped, the
+// exposure of each circle tested by one wave on the device, whose arc angles come from
+// the device's acos and atan2);
 // per axis, every line gathers the atoms, pairs and vertices within reach into
 // compact lists sorted by axial coordinate, one thread per list item counts and
 // writes its candidates, the candidates are sorted per line (segmented radix sorts:
@@ -493,6 +495,152 @@ __global__ void k_ses_emit(int64_t nr, const int32_t* __restrict__ kept, const i
   }
 }
 
+// ------------------------------------------------- SAS circle exposure on the device
+// One wave per candidate circle (pair i < j, in the host's order) runs the test of
+// circle_exposed below on the balls of atom i, deepest first, 64 at a time: a ball
+// covering the whole circle (S >= A) buries it; every other ball that reaches the
+// circle adds the arc it covers, as the same [start, end] angle intervals, to the
+// wave's part of LDS; after each group of balls the arcs so far are tested for a gap,
+// and a circle they cover is buried (more arcs only cover more). Every other step is
+// the host's arithmetic (IEEE + - * / sqrt in the same order); the arc angles use the
+// device's acos and atan2, so an arc end can differ from the host's in the last bit,
+// which changes a decision only where two arc ends meet to within rounding.
+// out: 0 buried, 1 exposed, 2 more arcs than the wave holds (decided on the host).
+constexpr int kCxWaves = 2;
+constexpr int kCxCap = 1024;  // arc intervals per wave (16 KB of LDS)
+
+__device__ __forceinline__ void wave_lds_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// Whether the n intervals [s_k, e_k] leave part of [0, 2 pi) open. The sorted sweep of
+// circle_exposed finds a gap exactly when some v in {0} and the ends below 2 pi lies in
+// no interval as s <= v < e, so each lane tests its share of those v against all.
+__device__ bool arcs_leave_gap(const double* s, const double* e, int n, double two_pi) {
+  bool gap = false;
+  for (int q = (int)(threadIdx.x & 63); q <= n; q += 64) {
+    const double v = q == n ? 0.0 : e[q];
+    if (!(v < two_pi)) continue;
+    bool held = false;
+    for (int k = 0; k < n && !held; ++k) held = s[k] <= v && v < e[k];
+    gap = gap || !held;
+  }
+  return __any(gap) != 0;
+}
+
+__global__ void __launch_bounds__(64 * kCxWaves) k_circle_exposed(int64_t ncand, const int32_t* __restrict__ ci,
+                                                                  const int32_t* __restrict__ cj,
+                                                                  const double* __restrict__ xyz,
+                                                                  const double* __restrict__ R,
+                                                                  const int64_t* __restrict__ dptr,
+                                                                  const int32_t* __restrict__ didx,
+                                                                  uint8_t* __restrict__ out) {
+  __shared__ double s_lo[kCxWaves][kCxCap], s_hi[kCxWaves][kCxCap];
+  const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int64_t pid = (int64_t)blockIdx.x * kCxWaves + wv;
+  if (pid >= ncand) return;  // the whole wave; no block-wide barrier follows
+  const int i = ci[pid], j = cj[pid];
+  const double* ai = xyz + 3 * i;
+  const double* aj = xyz + 3 * j;
+  // the circle of the pair, as the host computes it
+  const double dx = ai[0] - aj[0], dy = ai[1] - aj[1], dz = ai[2] - aj[2];
+  const double dd = sqrt(dx * dx + dy * dy + dz * dz);
+  const double Ri = R[i], Rj = R[j];
+  const double t = (dd * dd + Ri * Ri - Rj * Rj) / (2.0 * dd);
+  const double rho = sqrt(Ri * Ri - t * t);
+  double n[3], c[3];
+#pragma unroll
+  for (int k = 0; k < 3; ++k) {
+    n[k] = (aj[k] - ai[k]) / dd;
+    c[k] = ai[k] + t * n[k];
+  }
+  // the circle's (u, w) frame, as circle_exposed builds it
+  const int ax = fabs(n[0]) < 0.9 ? 0 : 1;
+  double u[3] = {0.0, 0.0, 0.0};
+  u[ax] = 1.0;
+  const double un = u[0] * n[0] + u[1] * n[1] + u[2] * n[2];
+#pragma unroll
+  for (int k = 0; k < 3; ++k) u[k] = u[k] - un * n[k];
+  const double ul = sqrt(u[0] * u[0] + u[1] * u[1] + u[2] * u[2]);
+#pragma unroll
+  for (int k = 0; k < 3; ++k) u[k] = u[k] / ul;
+  const double w[3] = {n[1] * u[2] - n[2] * u[1], n[2] * u[0] - n[0] * u[2], n[0] * u[1] - n[1] * u[0]};
+  const double two_pi = 2.0 * 3.14159265358979323846;
+  double* lo = s_lo[wv];
+  double* hi = s_hi[wv];
+  int nv = 0;  // intervals so far (the same in every lane)
+  const int64_t b1 = dptr[i + 1];
+  for (int64_t b = dptr[i]; b < b1; b += 64) {
+    const int64_t k = b + lane;
+    const int m = k < b1 ? didx[k] : j;
+    bool cov = false, arc = false, two = false;
+    double s0 = 0.0, e0 = 0.0;
+    if (m != j) {
+      const double* am = xyz + 3 * m;
+      const double d[3] = {c[0] - am[0], c[1] - am[1], c[2] - am[2]};
+      const double far = R[m] * (1.0 + 1e-9) + rho;
+      if (!(d[0] * d[0] + d[1] * d[1] + d[2] * d[2] > far * far)) {
+        const double P = 2.0 * rho * (d[0] * u[0] + d[1] * u[1] + d[2] * u[2]);
+        const double Q = 2.0 * rho * (d[0] * w[0] + d[1] * w[1] + d[2] * w[2]);
+        const double S = R[m] * R[m] - (d[0] * d[0] + d[1] * d[1] + d[2] * d[2]) - rho * rho;
+        const double A = sqrt(P * P + Q * Q);
+        cov = S >= A;
+        if (!cov && !(S <= -A)) {
+          arc = true;
+          const double beta = acos(S / A);
+          s0 = atan2(Q, P) + beta;
+          if (s0 < 0.0) s0 += two_pi;
+          if (s0 >= two_pi) s0 -= two_pi;
+          e0 = s0 + (two_pi - 2.0 * beta);
+          two = e0 > two_pi;
+        }
+      }
+    }
+    if (__any(cov)) {
+      if (lane == 0) out[pid] = 0;
+      return;
+    }
+    const int cnt = arc ? (two ? 2 : 1) : 0;
+    int pre = cnt;  // inclusive prefix over the lanes
+#pragma unroll
+    for (int off = 1; off < 64; off <<= 1) {
+      const int v = __shfl_up(pre, off, 64);
+      if (lane >= off) pre += v;
+    }
+    const int total = __shfl(pre, 63, 64);
+    if (nv + total > kCxCap) {
+      if (lane == 0) out[pid] = 2;
+      return;
+    }
+    if (arc) {
+      const int q = nv + pre - cnt;
+      if (two) {
+        lo[q] = s0;
+        hi[q] = two_pi;
+        lo[q + 1] = 0.0;
+        hi[q + 1] = e0 - two_pi;
+      } else {
+        lo[q] = s0;
+        hi[q] = e0;
+      }
+    }
+    nv += total;
+    if (total > 0) {
+      wave_lds_sync();
+      const bool gap = arcs_leave_gap(lo, hi, nv, two_pi);
+      wave_lds_sync();
+      if (!gap) {
+        if (lane == 0) out[pid] = 0;
+        return;
+      }
+    }
+  }
+  // every ball taken: a gap remains, or no ball reaches the circle
+  if (lane == 0) out[pid] = 1;
+}
+
 // ---------------------------------------------------------------- host side
 struct SesHost {
   std::vector<SesPairD> pairs;
@@ -601,7 +749,7 @@ void host_parallel(int64_t n, F&& f) {
   for (auto& x : th) x.join();
 }
 
-SesHost ses_build_host(int64_t na, const double* xyz, const double* r, double rp) {
+SesHost ses_build(ngpb_ctx* c, int64_t na, const double* xyz, const double* r, double rp) {
   SesHost H;
   std::vector<double> R((size_t)na);
   double Rmax = 0.0;
@@ -652,26 +800,71 @@ SesHost ses_build_host(int64_t na, const double* xyz, const double* r, double rp
     v.reserve(k.size());
     for (const auto& e : k) v.push_back(e.second);
   });
-  std::vector<std::vector<SesPairD>> pairs_of((size_t)na);
+  // the circle of a pair (i < j) of overlapping balls, if it has one
+  auto circle_of = [&](int64_t i, int j, SesPairD& P) {
+    const double* ai = xyz + 3 * i;
+    const double* aj = xyz + 3 * j;
+    const double dd = std::sqrt(sqd(ai, aj));
+    if (!(dd < R[(size_t)i] + R[(size_t)j]) || !(dd > std::fabs(R[(size_t)i] - R[(size_t)j]))) return false;
+    const double t = (dd * dd + R[(size_t)i] * R[(size_t)i] - R[(size_t)j] * R[(size_t)j]) / (2.0 * dd);
+    const double rho2 = R[(size_t)i] * R[(size_t)i] - t * t;
+    if (!(rho2 > 0.0)) return false;
+    P.i = (int32_t)i;
+    P.j = (int32_t)j;
+    for (int k = 0; k < 3; ++k) {
+      P.n[k] = (aj[k] - ai[k]) / dd;
+      P.c[k] = ai[k] + t * P.n[k];
+    }
+    P.rho = std::sqrt(rho2);
+    return true;
+  };
+  // candidate circles in (i, j) order; their exposure is tested on the device
+  std::vector<std::vector<int32_t>> cand_j((size_t)na);
   host_parallel(na, [&](int64_t i) {
-    for (int j : over[(size_t)i]) {
-      if (j <= i) continue;
-      const double* ai = xyz + 3 * i;
-      const double* aj = xyz + 3 * j;
-      const double dd = std::sqrt(sqd(ai, aj));
-      if (!(dd < R[(size_t)i] + R[(size_t)j]) || !(dd > std::fabs(R[(size_t)i] - R[(size_t)j]))) continue;
-      const double t = (dd * dd + R[(size_t)i] * R[(size_t)i] - R[(size_t)j] * R[(size_t)j]) / (2.0 * dd);
-      const double rho2 = R[(size_t)i] * R[(size_t)i] - t * t;
-      if (!(rho2 > 0.0)) continue;
+    SesPairD P;
+    for (int j : over[(size_t)i])
+      if (j > i && circle_of(i, j, P)) cand_j[(size_t)i].push_back(j);
+  });
+  std::vector<int32_t> pair_i, pair_j, didx;
+  std::vector<int64_t> dptr((size_t)na + 1, 0);
+  for (int64_t i = 0; i < na; ++i) {
+    pair_i.insert(pair_i.end(), cand_j[(size_t)i].size(), (int32_t)i);
+    pair_j.insert(pair_j.end(), cand_j[(size_t)i].begin(), cand_j[(size_t)i].end());
+    dptr[(size_t)i + 1] = dptr[(size_t)i] + (int64_t)deep[(size_t)i].size();
+  }
+  didx.reserve((size_t)dptr[(size_t)na]);
+  for (const auto& v : deep) didx.insert(didx.end(), v.begin(), v.end());
+  const int64_t ncand = (int64_t)pair_i.size();
+  std::vector<uint8_t> flag((size_t)ncand, 0);
+  if (ncand > 0) {
+    hipStream_t st = c->stream;
+    DBuf<double> d_xyz(st), d_R(st);
+    DBuf<int64_t> d_ptr(st);
+    DBuf<int32_t> d_idx(st), d_ci(st), d_cj(st);
+    DBuf<uint8_t> d_out(st);
+    d_xyz.upload(xyz, (size_t)(3 * na), st);
+    d_R.upload(R.data(), R.size(), st);
+    d_ptr.upload(dptr.data(), dptr.size(), st);
+    d_idx.upload(didx.empty() ? nullptr : didx.data(), std::max<size_t>(didx.size(), 1), st);
+    d_ci.upload(pair_i.data(), pair_i.size(), st);
+    d_cj.upload(pair_j.data(), pair_j.size(), st);
+    d_out.resize((size_t)ncand);
+    hipLaunchKernelGGL(k_circle_exposed, dim3((unsigned)((ncand + kCxWaves - 1) / kCxWaves)), dim3(64 * kCxWaves), 0,
+                       st, ncand, d_ci.p, d_cj.p, d_xyz.p, d_R.p, d_ptr.p, d_idx.p, d_out.p);
+    NGPB_HIP(hipGetLastError());
+    NGPB_HIP(hipMemcpyAsync(flag.data(), d_out.p, (size_t)ncand, hipMemcpyDeviceToHost, st));
+    NGPB_HIP(hipStreamSynchronize(st));
+  }
+  std::vector<std::vector<SesPairD>> pairs_of((size_t)na);
+  std::vector<int64_t> first((size_t)na + 1, 0);
+  for (int64_t i = 0; i < na; ++i) first[(size_t)i + 1] = first[(size_t)i] + (int64_t)cand_j[(size_t)i].size();
+  host_parallel(na, [&](int64_t i) {
+    for (int64_t q = first[(size_t)i]; q < first[(size_t)i + 1]; ++q) {
+      if (flag[(size_t)q] == 0) continue;
       SesPairD P;
-      P.i = (int32_t)i;
-      P.j = (int32_t)j;
-      for (int k = 0; k < 3; ++k) {
-        P.n[k] = (aj[k] - ai[k]) / dd;
-        P.c[k] = ai[k] + t * P.n[k];
-      }
-      P.rho = std::sqrt(rho2);
-      if (!circle_exposed(P.c, P.n, P.rho, j, deep[(size_t)i], xyz, R)) continue;
+      circle_of(i, pair_j[(size_t)q], P);
+      // more arcs than a wave holds: the host sweep decides
+      if (flag[(size_t)q] == 2 && !circle_exposed(P.c, P.n, P.rho, P.j, deep[(size_t)i], xyz, R)) continue;
       pairs_of[(size_t)i].push_back(P);
     }
   });
@@ -768,7 +961,7 @@ struct SesDevice {
 };
 
 SesDevice* ses_prepare(ngpb_ctx* c, int64_t na, const double* xyz, const double* r, double rp) {
-  SesHost H = ses_build_host(na, xyz, r, rp);
+  SesHost H = ses_build(c, na, xyz, r, rp);
   auto* D = new SesDevice(c->stream);
   D->rmax_of.assign(r, r + na);
   D->na = na;
