@@ -97,6 +97,7 @@ using ngpb::Error;
 template <class F>
 int guarded(ngpb_ctx* c, F&& f) {
   if (!c) return NGPB_E_ARG;
+  c->prof_kind = -1;  // no SpMV timing left open by a call that failed inside one
   try {
     NGPB_HIP(hipSetDevice(c->device));
     f();
