@@ -14,6 +14,7 @@
 // Emission ids: edge e of leaf l -> 12 l + e; corner i of leaf l -> 12 L + 8 l + i.
 // Weight code: bit 7 = negative, bits 0..6 = e with |w| = 2^-e (products of 1/np).
 #include <algorithm>
+#include <cstdlib>
 #include <list>
 
 #include "ngpb_internal.h"
@@ -1301,16 +1302,26 @@ void numeric_many(hipStream_t st, const std::vector<ngpb_ctx*>& cs) {
 }
 
 // systems per grouped launch: at most this many leaves (the symbolic temporaries take
-// about 0.6 KB per leaf) so that the temporaries and the grids stay bounded
+// about 0.6 KB per leaf) so that the temporaries and the grids stay bounded; the
+// environment variable NGPB_GROUP_LEAVES sets another bound
 constexpr int64_t kGroupLeaves = (int64_t)16 << 20;
+
+int64_t group_leaves() {
+  if (const char* e = std::getenv("NGPB_GROUP_LEAVES")) {
+    const long long v = std::strtoll(e, nullptr, 10);
+    if (v > 0) return (int64_t)v;
+  }
+  return kGroupLeaves;
+}
 
 template <typename F>
 void in_chunks(const std::vector<ngpb_ctx*>& cs, F f) {
+  const int64_t cap = group_leaves();
   size_t i = 0;
   while (i < cs.size()) {
     size_t j = i;
     int64_t L = 0;
-    while (j < cs.size() && (j == i || L + cs[j]->L <= kGroupLeaves)) L += cs[j++]->L;
+    while (j < cs.size() && (j == i || L + cs[j]->L <= cap)) L += cs[j++]->L;
     f(std::vector<ngpb_ctx*>(cs.begin() + i, cs.begin() + j));
     i = j;
   }

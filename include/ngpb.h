@@ -258,7 +258,8 @@ int ngpb_assemble(ngpb_ctx* ctx);
    assembles one system per run (create_density_map + assemple_system_matrix,
    src/pb_class.cpp:1943-2159). On an error in a context's own preparation that
    context holds the message; on an error in the shared stages every context does, and
-   none holds an assembled system. */
+   none holds an assembled system. The systems are taken in groups of at most 16M
+   leaves (the environment variable NGPB_GROUP_LEAVES sets another bound). */
 int ngpb_assemble_many(ngpb_ctx* const* ctx, int n);
 int ngpb_system_size(ngpb_ctx* ctx, int64_t* n, int64_t* nnz);
 int ngpb_export_csr(ngpb_ctx* ctx, int32_t* rowptr, int32_t* col, double* val, double* rhs);

@@ -82,6 +82,20 @@ def test_grouped_assembly_matches_oracle():
         c.close()
 
 
+def test_grouped_assembly_in_several_groups(monkeypatch):
+    """A leaf bound that splits the systems into groups of one or two: the same CSR and
+    rhs as the oracle's for every system."""
+    Ps = problems()
+    monkeypatch.setenv("NGPB_GROUP_LEAVES", str(max(P.mesh.n_leaves for P in Ps) + 1))
+    ctxs = [_inputs(P) for P in Ps]
+    native.assemble_many(ctxs)
+    for i, (P, c) in enumerate(zip(Ps, ctxs)):
+        S = oracle_system(i)
+        assert _same(_csr(c), (S.rowptr, S.col, S.val, S.rhs)), P.name
+    for c in ctxs:
+        c.close()
+
+
 def test_grouped_equals_single_on_device_built_systems():
     """Ligand systems built from atoms on the device (octree, rays, owners, boundary
     values) and a host mesh: prepare_many against prepare, per system."""
