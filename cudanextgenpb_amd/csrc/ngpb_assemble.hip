@@ -175,6 +175,27 @@ __device__ __forceinline__ bool em_before(uint64_t ka, int wa, uint64_t kb, int 
   return ka < kb || (ka == kb && wa < wb);
 }
 
+// v of lane (lane ^ j) for j in {1, 2, 4, 8, 16, 32}: DPP lane moves inside each row of
+// 16 lanes (quad permutations for 1 and 2, a half-row mirror then a quad reversal for
+// 4, a rotation by 8 for 8), the LDS permute across rows (16, 32). With the sort loops
+// unrolled, j is a constant and the switch folds away.
+__device__ __forceinline__ unsigned lane_xor(unsigned v, int j) {
+  switch (j) {
+    case 1: return (unsigned)__builtin_amdgcn_update_dpp(0, (int)v, 0xB1, 0xF, 0xF, true);   // quad_perm [1,0,3,2]
+    case 2: return (unsigned)__builtin_amdgcn_update_dpp(0, (int)v, 0x4E, 0xF, 0xF, true);   // quad_perm [2,3,0,1]
+    case 4: {
+      const int m = __builtin_amdgcn_update_dpp(0, (int)v, 0x141, 0xF, 0xF, true);         // row_half_mirror
+      return (unsigned)__builtin_amdgcn_update_dpp(0, m, 0x1B, 0xF, 0xF, true);            // quad_perm [3,2,1,0]
+    }
+    case 8: return (unsigned)__builtin_amdgcn_update_dpp(0, (int)v, 0x128, 0xF, 0xF, true);  // row_ror:8
+    default: return __shfl_xor(v, j, 64);
+  }
+}
+
+__device__ __forceinline__ uint64_t lane_xor_u64(uint64_t v, int j) {
+  return ((uint64_t)lane_xor((unsigned)(v >> 32), j) << 32) | lane_xor((unsigned)v, j);
+}
+
 __device__ __forceinline__ uint64_t shfl_xor_u64(uint64_t v, int mask) {
   const unsigned lo = __shfl_xor((unsigned)v, mask, 64), hi = __shfl_xor((unsigned)(v >> 32), mask, 64);
   return ((uint64_t)hi << 32) | lo;
@@ -207,8 +228,8 @@ __device__ __forceinline__ void k_row_sort_wave_body(int64_t bid, int64_t N, con
   for (int w = 2; w <= 64; w <<= 1)
 #pragma unroll
     for (int j = w >> 1; j > 0; j >>= 1) {
-      const uint64_t ok = shfl_xor_u64(kk, j);
-      const unsigned ow = __shfl_xor(ww, j, 64);
+      const uint64_t ok = lane_xor_u64(kk, j);
+      const unsigned ow = lane_xor(ww, j);
       const bool keep_min = ((lane & j) == 0) == ((lane & w) == 0);
       const bool take = keep_min ? em_before(ok, (int)(ow >> 8), kk, (int)(ww >> 8))
                                  : em_before(kk, (int)(ww >> 8), ok, (int)(ow >> 8));
@@ -225,9 +246,13 @@ __device__ __forceinline__ void k_row_sort_wave_body(int64_t bid, int64_t N, con
     const int64_t col = (int64_t)(kk >> 32);
     rc.add(em_class(row, is_bnd[row] != 0, col, is_bnd), lane == 0 || (int64_t)(prev >> 32) != col);
   }
-  const int64_t v[4] = {wave_sum_i64(rc.slots), wave_sum_i64(rc.couplings), wave_sum_i64(rc.slot_em),
-                        wave_sum_i64(rc.coupling_em)};
-  if (lane < 4) cnt4[(int64_t)lane * N + row] = v[lane];
+  // the four counts of a lane are 0 or 1 and a row has at most 64 emissions, so they
+  // are summed over the wave as four 16-bit fields of one word
+  uint64_t packed = (uint64_t)rc.slots | ((uint64_t)rc.couplings << 16) | ((uint64_t)rc.slot_em << 32) |
+                    ((uint64_t)rc.coupling_em << 48);
+#pragma unroll
+  for (int j = 32; j > 0; j >>= 1) packed += lane_xor_u64(packed, j);
+  if (lane < 4) cnt4[(int64_t)lane * N + row] = (int64_t)((packed >> (16 * lane)) & 0xffffu);
 }
 
 __global__ void __launch_bounds__(kSortBS) k_row_sort_wave(int64_t N, const int64_t* __restrict__ em_rowptr,
