@@ -192,18 +192,23 @@ __device__ __forceinline__ unsigned lane_xor(unsigned v, int j) {
   }
 }
 
+// v of lane i - 1 (DPP wave shift); lane 0 keeps its own
+__device__ __forceinline__ uint64_t lane_prev_u64(uint64_t v) {
+  const unsigned lo = (unsigned)v, hi = (unsigned)(v >> 32);
+  const unsigned plo = (unsigned)__builtin_amdgcn_update_dpp((int)lo, (int)lo, 0x138, 0xF, 0xF, false);
+  const unsigned phi = (unsigned)__builtin_amdgcn_update_dpp((int)hi, (int)hi, 0x138, 0xF, 0xF, false);
+  return ((uint64_t)phi << 32) | plo;
+}
+
+// v of lane 63
+__device__ __forceinline__ uint64_t lane_last_u64(uint64_t v) {
+  const unsigned lo = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)v, 63);
+  const unsigned hi = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)(v >> 32), 63);
+  return ((uint64_t)hi << 32) | lo;
+}
+
 __device__ __forceinline__ uint64_t lane_xor_u64(uint64_t v, int j) {
   return ((uint64_t)lane_xor((unsigned)(v >> 32), j) << 32) | lane_xor((unsigned)v, j);
-}
-
-__device__ __forceinline__ uint64_t shfl_xor_u64(uint64_t v, int mask) {
-  const unsigned lo = __shfl_xor((unsigned)v, mask, 64), hi = __shfl_xor((unsigned)(v >> 32), mask, 64);
-  return ((uint64_t)hi << 32) | lo;
-}
-
-__device__ __forceinline__ uint64_t shfl_u64(uint64_t v, int src) {
-  const unsigned lo = __shfl((unsigned)v, src, 64), hi = __shfl((unsigned)(v >> 32), src, 64);
-  return ((uint64_t)hi << 32) | lo;
 }
 
 // Rows of at most 64 emissions: one wave per row, one emission per lane, sorted by a
@@ -238,7 +243,7 @@ __device__ __forceinline__ void k_row_sort_wave_body(int64_t bid, int64_t N, con
         ww = ow;
       }
     }
-  const uint64_t prev = shfl_u64(kk, lane > 0 ? lane - 1 : 0);
+  const uint64_t prev = lane_prev_u64(kk);
   RowCounts rc;
   if (lane < E) {
     key[b + lane] = kk;
@@ -388,7 +393,7 @@ __device__ __forceinline__ void k_row_slots_body(int64_t bid,
     const bool valid = i < e;
     const uint64_t kk = valid ? key[i] : 0;
     const int64_t cl = valid ? (int64_t)(kk >> 32) : -1;
-    const int64_t up = (int64_t)shfl_u64((uint64_t)cl, lane > 0 ? lane - 1 : 0);
+    const int64_t up = (int64_t)lane_prev_u64((uint64_t)cl);
     const int64_t prev = lane == 0 ? lastcol : up;
     const bool start = valid && cl != prev;
     const int cls = valid ? em_class(row, rb, cl, is_bnd) : 3;
@@ -418,7 +423,7 @@ __device__ __forceinline__ void k_row_slots_body(int64_t bid,
     be += __popcll(mb);
     fs += __popcll(sf);
     bs += __popcll(sb);
-    lastcol = (int64_t)shfl_u64((uint64_t)cl, 63);
+    lastcol = (int64_t)lane_last_u64((uint64_t)cl);
   }
 }
 
