@@ -467,10 +467,33 @@ __device__ __forceinline__ int64_t ray_find(const RayView& r, double a, double b
   return -1;
 }
 
-// wave64 sum (fixed butterfly order)
+// 32-bit value of lane (lane ^ j), j a power of two below 64 (a constant once the
+// caller's loop is unrolled): DPP lane moves inside 16-lane rows for j <= 8 (quad
+// permutations for 1 and 2, a half-row mirror then a quad reversal for 4, a rotation
+// by 8 for 8), the LDS permute across rows for 16 and 32.
+__device__ __forceinline__ int lane_xor_i32(int v, int j) {
+  switch (j) {
+    case 1: return __builtin_amdgcn_update_dpp(0, v, 0xB1, 0xF, 0xF, true);
+    case 2: return __builtin_amdgcn_update_dpp(0, v, 0x4E, 0xF, 0xF, true);
+    case 4: return __builtin_amdgcn_update_dpp(0, __builtin_amdgcn_update_dpp(0, v, 0x141, 0xF, 0xF, true), 0x1B, 0xF,
+                                               0xF, true);
+    case 8: return __builtin_amdgcn_update_dpp(0, v, 0x128, 0xF, 0xF, true);
+    default: return __shfl_xor(v, j, 64);
+  }
+}
+
+__device__ __forceinline__ double lane_xor_f64(double v, int j) {
+  const long long b = __double_as_longlong(v);
+  const int lo = lane_xor_i32((int)(unsigned)(b & 0xffffffffLL), j);
+  const int hi = lane_xor_i32((int)(b >> 32), j);
+  return __longlong_as_double(((long long)hi << 32) | (long long)(unsigned)lo);
+}
+
+// wave64 sum (fixed butterfly order: partners at distance 32, 16, ..., 1); every lane
+// gets the sum. The lane moves do not change the arithmetic, only how values travel.
 __device__ __forceinline__ double wave_sum(double v) {
 #pragma unroll
-  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+  for (int off = 32; off > 0; off >>= 1) v += lane_xor_f64(v, off);
   return v;
 }
 
