@@ -6,6 +6,9 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <map>
+#include <mutex>
+#include <unordered_map>
 #include <sstream>
 #include <string>
 #include <vector>
@@ -94,10 +97,19 @@ namespace {
 
 using ngpb::Error;
 
+// at the end of an API call the context's stream keeps a bounded set of cached blocks
+struct CallEnd {
+  hipStream_t s;
+  ~CallEnd() {
+    if (s) ngpb::stream_cache_end_call(s);
+  }
+};
+
 template <class F>
 int guarded(ngpb_ctx* c, F&& f) {
   if (!c) return NGPB_E_ARG;
   c->prof_kind = -1;  // no SpMV timing left open by a call that failed inside one
+  CallEnd end{c->stream};
   try {
     NGPB_HIP(hipSetDevice(c->device));
     f();
@@ -129,6 +141,81 @@ void not_in_batch(const ngpb_ctx* c) {
 }
 
 }  // namespace
+
+namespace ngpb {
+
+// ---- per-stream cache of released stream-ordered blocks
+// Bounds: a stream caches at most kCacheMax bytes during a call; at the end of an API
+// call it keeps at most kCacheKeep bytes, and none once the streams of all contexts
+// together keep more than kGlobalKeep (512 contexts of a batch each keeping blocks
+// between calls would hold the device's memory).
+namespace {
+constexpr size_t kCacheMax = size_t(1) << 30;
+constexpr size_t kCacheKeep = size_t(256) << 20;
+constexpr size_t kGlobalKeep = size_t(1) << 30;
+struct StreamBlocks {
+  std::multimap<size_t, void*> free;  // by size
+  size_t bytes = 0;
+};
+std::mutex g_cache_mu;
+std::unordered_map<hipStream_t, StreamBlocks> g_cache;
+size_t g_cached = 0;  // bytes cached over all streams
+}  // namespace
+
+void* stream_cache_take(hipStream_t s, size_t bytes, size_t* cap) {
+  std::lock_guard<std::mutex> g(g_cache_mu);
+  auto it = g_cache.find(s);
+  if (it == g_cache.end()) return nullptr;
+  StreamBlocks& b = it->second;
+  auto f = b.free.lower_bound(bytes);
+  // a block at most twice (plus 1 MiB) the request: no large block for a small buffer
+  if (f == b.free.end() || f->first > 2 * bytes + (size_t(1) << 20)) return nullptr;
+  void* p = f->second;
+  *cap = f->first;
+  b.bytes -= f->first;
+  g_cached -= f->first;
+  b.free.erase(f);
+  return p;
+}
+
+bool stream_cache_put(hipStream_t s, void* p, size_t cap) {
+  std::lock_guard<std::mutex> g(g_cache_mu);
+  StreamBlocks& b = g_cache[s];
+  if (b.bytes + cap > kCacheMax) return false;
+  b.free.emplace(cap, p);
+  b.bytes += cap;
+  g_cached += cap;
+  return true;
+}
+
+void stream_cache_trim(hipStream_t s, size_t keep) {
+  std::vector<void*> out;
+  {
+    std::lock_guard<std::mutex> g(g_cache_mu);
+    auto it = g_cache.find(s);
+    if (it == g_cache.end()) return;
+    StreamBlocks& b = it->second;
+    if (keep > 0 && g_cached > kGlobalKeep) keep = 0;
+    while (b.bytes > keep && !b.free.empty()) {  // largest first
+      auto f = std::prev(b.free.end());
+      out.push_back(f->second);
+      b.bytes -= f->first;
+      g_cached -= f->first;
+      b.free.erase(f);
+    }
+  }
+  for (void* p : out) (void)hipFreeAsync(p, s);
+}
+
+void stream_cache_end_call(hipStream_t s) { stream_cache_trim(s, kCacheKeep); }
+
+void stream_cache_drop(hipStream_t s) {
+  stream_cache_trim(s, 0);
+  std::lock_guard<std::mutex> g(g_cache_mu);
+  g_cache.erase(s);
+}
+
+}  // namespace ngpb
 
 extern "C" {
 
@@ -183,7 +270,11 @@ int ngpb_destroy(ngpb_ctx* c) {
   for (auto& p : c->kprof)
     for (hipEvent_t e : p.ev) (void)hipEventDestroy(e);
   c->ks_host = nullptr;
-  if (c->stream) (void)hipStreamDestroy(c->stream);
+  if (c->stream) {
+    ngpb::stream_cache_drop(c->stream);
+    (void)hipStreamSynchronize(c->stream);
+    (void)hipStreamDestroy(c->stream);
+  }
   delete c;  // device buffers free themselves
   return NGPB_OK;
 }

@@ -34,15 +34,34 @@ struct Error : std::runtime_error {
   } while (0)
 
 // --------------------------------------------------------------- device buffers
+// Blocks of stream-ordered buffers released on a stream are kept for later buffers on
+// the same stream (stream order makes the reuse safe: every use of the old buffer was
+// enqueued before every use of the new one), so a call's repeated temporaries (the
+// per-axis arrays of a ray cast, the per-level arrays of an octree build) are not
+// handed back to the driver and fetched again: a hipFreeAsync of a 64 MB block has
+// held the host for ~5 ms. No events are involved, so no thread ever waits on work
+// of another thread's stream. stream_cache_take returns a block of at least `bytes`
+// (and not much more) with its size in *cap, or nullptr; stream_cache_put returns
+// false when the stream's cache is full (the caller frees the block);
+// stream_cache_trim frees blocks until at most `keep` bytes stay cached;
+// stream_cache_end_call trims to the bound kept between API calls; and
+// stream_cache_drop frees them all (before the stream is destroyed).
+void* stream_cache_take(hipStream_t s, size_t bytes, size_t* cap);
+bool stream_cache_put(hipStream_t s, void* p, size_t cap);
+void stream_cache_trim(hipStream_t s, size_t keep);
+void stream_cache_end_call(hipStream_t s);
+void stream_cache_drop(hipStream_t s);
+
 // A buffer constructed with a stream (or given one by order_on) is allocated and
-// freed in that stream's order (hipMallocAsync / hipFreeAsync): freeing it waits
-// neither for the device nor for other streams, which matters for the temporaries of
-// one call while other contexts' work runs on the same GPU. The others use
-// hipMalloc / hipFree (the resident arrays of a context).
+// freed in that stream's order (hipMallocAsync / hipFreeAsync, through the stream's
+// block cache): freeing it waits neither for the device nor for other streams, which
+// matters for the temporaries of one call while other contexts' work runs on the
+// same GPU. The others use hipMalloc / hipFree (the resident arrays of a context).
 template <class T>
 struct DBuf {
   T* p = nullptr;
   size_t n = 0;
+  size_t cap = 0;  // bytes of the block (stream-ordered buffers)
   hipStream_t os = nullptr;
   bool ordered = false;
   DBuf() = default;
@@ -56,22 +75,40 @@ struct DBuf {
     ordered = true;
   }
   void release() {
-    if (p) (void)(ordered ? hipFreeAsync(p, os) : hipFree(p));
+    if (p) {
+      if (!ordered) (void)hipFree(p);
+      else if (!stream_cache_put(os, p, cap)) (void)hipFreeAsync(p, os);
+    }
     p = nullptr;
     n = 0;
+    cap = 0;
   }
   // (re)allocates when the size differs; contents are undefined afterwards
   void resize(size_t count) {
     if (count == n && p) return;
     release();
     if (count == 0) return;
-    hipError_t e = ordered ? hipMallocAsync((void**)&p, count * sizeof(T), os) : hipMalloc((void**)&p, count * sizeof(T));
+    const size_t bytes = count * sizeof(T);
+    if (ordered && (p = static_cast<T*>(stream_cache_take(os, bytes, &cap)))) {
+      n = count;
+      return;
+    }
+    cap = bytes;
+    hipError_t e = ordered ? hipMallocAsync((void**)&p, bytes, os) : hipMalloc((void**)&p, bytes);
     if (e != hipSuccess) {
       p = nullptr;
       (void)hipGetLastError();
       throw Error(NGPB_E_NOMEM, "hipMalloc of " + std::to_string(count * sizeof(T)) + " bytes failed");
     }
     n = count;
+  }
+  // exchanges the blocks of two buffers of the same allocation kind and stream (the
+  // block sizes go with them: a stream-ordered block returns to the cache with its own)
+  void swap(DBuf& o) {
+    if (ordered != o.ordered || os != o.os) throw Error(NGPB_E_STATE, "buffer swap across allocation kinds");
+    std::swap(p, o.p);
+    std::swap(n, o.n);
+    std::swap(cap, o.cap);
   }
   // grows only
   void reserve(size_t count) {

@@ -549,8 +549,7 @@ void balance_leaves(ngpb_ctx* c, const Geom& g, DBuf<OLeaf>& leaves, int64_t& L)
     hipLaunchKernelGGL(k_split_emit, dim3(blocks_for(L, 256)), dim3(256), 0, st, leaves.p, L, flag.p, cnt.p, next.p,
                        next.p + nk);
     NGPB_HIP(hipGetLastError());
-    std::swap(leaves.p, next.p);
-    std::swap(leaves.n, next.n);
+    leaves.swap(next);
     L = nk + 8 * ns;
   }
 }
@@ -678,8 +677,7 @@ OctreeSizes run_build_octree(ngpb_ctx* c, const double* origin, double hf, const
     const auto ks = split_pass(c, work.p, nw, flag.p, cnt.p, kept[pass].p, next);
     nkept[pass] = ks.first;
     L += ks.first;
-    std::swap(work.p, next.p);
-    std::swap(work.n, next.n);
+    work.swap(next);
     nw = 8 * ks.second;
   }
   NGPB_REQUIRE(L > 0 && L < INT32_MAX, NGPB_E_ARG, "leaf count out of range");
@@ -717,10 +715,8 @@ OctreeSizes run_build_octree_surface(ngpb_ctx* c, const double* origin, double h
   auto& R = c->rays[2];
   DBuf<int64_t> zptr;  // plain allocations, like the table they are taken from
   DBuf<double> zint;
-  std::swap(zptr.p, R.ptr.p);
-  std::swap(zptr.n, R.ptr.n);
-  std::swap(zint.p, R.inters.p);
-  std::swap(zint.n, R.inters.n);
+  zptr.swap(R.ptr);
+  zint.swap(R.inters);
   DBuf<int32_t> lmap(st);
   const int64_t nlines = g.vn[0] * g.vn[1];
   lmap.resize((size_t)nlines);
@@ -773,8 +769,7 @@ OctreeSizes run_build_octree_surface(ngpb_ctx* c, const double* origin, double h
         hipLaunchKernelGGL(k_split_emit, dim3(blocks_for(L, 256)), dim3(256), 0, st, leaves.p, L, flag.p, cnt.p,
                            next.p, next.p + nk);
         NGPB_HIP(hipGetLastError());
-        std::swap(leaves.p, next.p);
-        std::swap(leaves.n, next.n);
+        leaves.swap(next);
         L = nk + 8 * ns;
       }
     }
@@ -796,8 +791,7 @@ OctreeSizes run_build_octree_surface(ngpb_ctx* c, const double* origin, double h
         hipLaunchKernelGGL(k_surf_coarsen_emit, dim3(blocks_for(L, 256)), dim3(256), 0, st, leaves.p, L, g, smax,
                            lev.p, dmap.p, off.p, cnt.p, next.p);
         NGPB_HIP(hipGetLastError());
-        std::swap(leaves.p, next.p);
-        std::swap(leaves.n, next.n);
+        leaves.swap(next);
         L = nl;
       }
     }
@@ -834,8 +828,7 @@ OctreeSizes finish_octree(ngpb_ctx* c, const Geom& g, const int64_t* n, DBuf<OLe
     next.resize((size_t)L);
     hipLaunchKernelGGL(k_gather_leaves, dim3(blocks_for(L, 256)), dim3(256), 0, st, leaves.p, idx2.p, L, next.p);
     NGPB_HIP(hipGetLastError());
-    std::swap(leaves.p, next.p);
-    std::swap(leaves.n, next.n);
+    leaves.swap(next);
     next.release();
   }
 
