@@ -277,7 +277,8 @@ int ngpb_set_csr(ngpb_ctx* ctx, int64_t n, int64_t nnz, const int32_t* rowptr, c
    -tol_w take a value and have no effect. Anything else (other LIS solvers or
    preconditioners such as -p ilu, unknown options) returns NGPB_E_ARG and leaves *out
    holding the LIS defaults for the options read so far: a caller should then fall
-   back to LIS (see INTEGRATION.md for the reference-side branch). */
+   back to LIS (lis_compute_electric_potential) and say so, rather than solve with
+   those defaults. */
 int ngpb_parse_lis_options(const char* options, ngpb_solve_opts* out);
 /* Solves the resident system. FP64: CG with Jacobi runs the fused CG kernels; every
    other solver/preconditioner pair runs the general Krylov kernels with
